@@ -1,0 +1,267 @@
+// Headless driver for the original SPH2O sources (test infrastructure only).
+//
+// This file is compiled together with the reference's own sources, read in
+// place from /root/reference by oracle/Makefile, into oracle/_ref/. It never
+// ships as product code and nothing under sph2o_amd/ links it. It drives the
+// reference through its public API, the way its CLI does
+// (src/main.cpp:79-80: importScene -> init -> step), but:
+//   * loops on a step count instead of SPHSimulation::run(), which never
+//     terminates (SPHSimulation.cpp:193-199 never refreshes `time`);
+//   * calls SPHSolver::step() directly so the start/end time window and the
+//     frame writer of SPHSimulation::step() (SPHSimulation.cpp:157-191) do
+//     not gate the run;
+//   * zero-initialises the two solver fields the reference leaves
+//     uninitialised before they are first read: DFSPHSolver::iterationsV
+//     (DFSPHSolver.h:31, read at DFSPHSolver.cpp:1146 in step 1) and
+//     PCISPHSolver::avgDensityError (PCISPHSolver.h:18, read at
+//     PCISPHSolver.cpp:374);
+//   * optionally replaces the lattice positions/velocities built by
+//     FluidObject::build with caller-supplied ones (importFluidState is dead
+//     code: it calls exit(0), SPHSimulation.cpp:548).
+//
+// Usage:
+//   sph2o_ref <scene.json> <nsteps> <outdir> [--init state.bin] [--nbr]
+// state.bin: int32 N, then N*3 float32 positions, then N*3 float32 velocities.
+// Outputs are .npy files in <outdir> (see dump_* below).
+
+#include "SPH2O/SPHSimulation.h"
+#include "SPH2O/SPHSolver/DFSPHSolver.h"
+#include "SPH2O/SPHSolver/PCISPHSolver.h"
+#include "SPH2O/SPHSolver/WCSPHSolver.h"
+#include "SPH2O/BoundaryObject/AkinciBoundaryObject.h"
+#include "SPH2O/HashTable.h"
+
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace {
+
+// Accessors for protected solver counters (read-only peeks, plus the two
+// zero-initialisations described above).
+struct DFPeek : public DFSPHSolver {
+    uint its() const { return iterations; }
+    uint itsV() const { return iterationsV; }
+    Real errD() const { return avgDensError; }
+    Real errV() const { return avgDivError; }
+    void zeroUninit() { iterationsV = 0; avgDensError = 0; avgDivError = 0; }
+};
+struct PCIPeek : public PCISPHSolver {
+    uint its() const { return iterations; }
+    Real errD() const { return avgDensityError; }
+    Real delta() const { return scalingFactor.empty() ? 0 : scalingFactor[0]; }
+    void zeroUninit() { avgDensityError = 0; maxDensityError = 0; }
+};
+
+void write_npy(const std::string& path, const char* descr, const std::vector<size_t>& shape,
+               const void* data, size_t bytes) {
+    std::string dict = "{'descr': '";
+    dict += descr;
+    dict += "', 'fortran_order': False, 'shape': (";
+    for (size_t i = 0; i < shape.size(); ++i) {
+        dict += std::to_string(shape[i]);
+        dict += (shape.size() == 1 || i + 1 < shape.size()) ? ", " : "";
+    }
+    dict += "), }";
+    size_t total = 10 + dict.size() + 1;
+    size_t pad = (64 - total % 64) % 64;
+    dict.append(pad, ' ');
+    dict += '\n';
+    std::ofstream f(path, std::ios::binary);
+    const char magic[] = "\x93NUMPY\x01\x00";
+    f.write(magic, 8);
+    uint16_t hl = static_cast<uint16_t>(dict.size());
+    f.write(reinterpret_cast<const char*>(&hl), 2);
+    f.write(dict.data(), dict.size());
+    f.write(static_cast<const char*>(data), bytes);
+}
+
+template <typename T>
+void dump(const std::string& dir, const std::string& name, const char* descr,
+          const std::vector<T>& v, size_t cols) {
+    std::vector<size_t> shape;
+    if (cols == 1) shape = {v.size()};
+    else shape = {v.size() / cols, cols};
+    write_npy(dir + "/" + name + ".npy", descr, shape, v.data(), v.size() * sizeof(T));
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 4) {
+        std::fprintf(stderr, "usage: %s scene.json nsteps outdir [--init state.bin] [--nbr]\n", argv[0]);
+        return 2;
+    }
+    const std::string scene = argv[1];
+    const int nsteps = std::atoi(argv[2]);
+    const std::string outdir = argv[3];
+    std::string initPath;
+    bool wantNbr = false;
+    for (int a = 4; a < argc; ++a) {
+        if (!std::strcmp(argv[a], "--init") && a + 1 < argc) initPath = argv[++a];
+        else if (!std::strcmp(argv[a], "--nbr")) wantNbr = true;
+    }
+
+    SPHSimulation* sim = SPHSimulation::getCurrent();
+    if (!sim->importScene(scene)) return 3;
+    if (sim->numberFluidModels() != 1) {
+        std::fprintf(stderr, "driver supports single-fluid scenes only\n");
+        return 4;
+    }
+    FluidObject* f = sim->getFluidModel(0);
+    const uint N = f->getSize();
+
+    if (!initPath.empty()) {
+        std::ifstream in(initPath, std::ios::binary);
+        int32_t n = 0;
+        in.read(reinterpret_cast<char*>(&n), 4);
+        if (static_cast<uint>(n) != N) {
+            std::fprintf(stderr, "init state has %d particles, scene has %u\n", n, N);
+            return 5;
+        }
+        std::vector<float> buf(6 * static_cast<size_t>(N));
+        in.read(reinterpret_cast<char*>(buf.data()), buf.size() * 4);
+        for (uint i = 0; i < N; ++i) {
+            f->getPosition(i) = Vector3r(buf[3 * i], buf[3 * i + 1], buf[3 * i + 2]);
+            f->getVelocity(i) = Vector3r(buf[3 * N + 3 * i], buf[3 * N + 3 * i + 1], buf[3 * N + 3 * i + 2]);
+        }
+    }
+
+    const int method = sim->getSimulationMethod();
+    if (method == SPHSimulation::DFSPH_METHOD)
+        static_cast<DFPeek*>(static_cast<DFSPHSolver*>(sim->getSolver()))->zeroUninit();
+    if (method == SPHSimulation::PCISPH_METHOD)
+        static_cast<PCIPeek*>(static_cast<PCISPHSolver*>(sim->getSolver()))->zeroUninit();
+
+    sim->init();
+
+    // Boundary particles (all Akinci objects, concatenated in object order).
+    std::vector<float> bx, bvol;
+    if (sim->getBoundaryHandlingMethod() == SPHSimulation::AKINCI_BOUNDARY_METHOD) {
+        for (uint o = 0; o < sim->numberBoundaryModels(); ++o) {
+            auto* b = static_cast<AkinciBoundaryObject*>(sim->getBoundaryModel(o));
+            for (uint i = 0; i < b->getSize(); ++i) {
+                const Vector3r& p = b->getPosition(i);
+                bx.push_back(p.x); bx.push_back(p.y); bx.push_back(p.z);
+                bvol.push_back(b->getVolume(i));
+            }
+        }
+    }
+    std::vector<float> x0;
+    for (uint i = 0; i < N; ++i) {
+        const Vector3r& p = f->getPosition(i);
+        x0.push_back(p.x); x0.push_back(p.y); x0.push_back(p.z);
+    }
+    std::vector<float> rho0(N), alpha0(N, 0.f);
+    for (uint i = 0; i < N; ++i) rho0[i] = f->getDensity(i);
+    if (method == SPHSimulation::DFSPH_METHOD)
+        for (uint i = 0; i < N; ++i)
+            alpha0[i] = static_cast<DFSPHSolver*>(sim->getSolver())->getAlpha(0, i);
+
+    // Per-step stats: t, dt, iterations, iterationsV, avg density err, avg div err
+    std::vector<double> stats;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int s = 0; s < nsteps; ++s) {
+        sim->getSolver()->step();
+        double its = 0, itsV = 0, eD = 0, eV = 0;
+        if (method == SPHSimulation::DFSPH_METHOD) {
+            auto* p = static_cast<DFPeek*>(static_cast<DFSPHSolver*>(sim->getSolver()));
+            its = p->its(); itsV = p->itsV(); eD = p->errD(); eV = p->errV();
+        } else if (method == SPHSimulation::PCISPH_METHOD) {
+            auto* p = static_cast<PCIPeek*>(static_cast<PCISPHSolver*>(sim->getSolver()));
+            its = p->its(); eD = p->errD();
+        }
+        stats.insert(stats.end(), {static_cast<double>(sim->getTime()),
+                                   static_cast<double>(sim->getTimeStep()), its, itsV, eD, eV});
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    double secs = std::chrono::duration<double>(t1 - t0).count();
+
+    std::vector<float> x, v, a, rho, p, alpha;
+    for (uint i = 0; i < N; ++i) {
+        const Vector3r& pi = f->getPosition(i);
+        const Vector3r& vi = f->getVelocity(i);
+        const Vector3r& ai = f->getAcceleration(i);
+        x.insert(x.end(), {pi.x, pi.y, pi.z});
+        v.insert(v.end(), {vi.x, vi.y, vi.z});
+        a.insert(a.end(), {ai.x, ai.y, ai.z});
+        rho.push_back(f->getDensity(i));
+        p.push_back(f->getPressure(i));
+        alpha.push_back(method == SPHSimulation::DFSPH_METHOD
+                            ? static_cast<DFSPHSolver*>(sim->getSolver())->getAlpha(0, i) : 0.f);
+    }
+    std::vector<double> meta = {static_cast<double>(N), static_cast<double>(bvol.size()),
+                                static_cast<double>(sim->getSupportRadius()), secs,
+                                method == SPHSimulation::PCISPH_METHOD
+                                    ? static_cast<double>(static_cast<PCIPeek*>(static_cast<PCISPHSolver*>(sim->getSolver()))->delta())
+                                    : 0.0};
+
+    dump(outdir, "x0", "<f4", x0, 3);
+    dump(outdir, "rho0", "<f4", rho0, 1);
+    dump(outdir, "alpha0", "<f4", alpha0, 1);
+    dump(outdir, "bx", "<f4", bx, 3);
+    dump(outdir, "bvol", "<f4", bvol, 1);
+    dump(outdir, "x", "<f4", x, 3);
+    dump(outdir, "v", "<f4", v, 3);
+    dump(outdir, "a", "<f4", a, 3);
+    dump(outdir, "rho", "<f4", rho, 1);
+    dump(outdir, "p", "<f4", p, 1);
+    dump(outdir, "alpha", "<f4", alpha, 1);
+    dump(outdir, "stats", "<f8", stats, 6);
+    dump(outdir, "meta", "<f8", meta, 1);
+
+    if (wantNbr) {
+        // Rebuild the grid at the final positions with the reference's own
+        // insert + search, then read cell keys and the within-support sets
+        // {j in bucket.neighbors : length(xi - xj) <= h} (DFSPHSolver.cpp:575).
+        sim->getSolver()->insertFluidParticles();
+        sim->getSolver()->neighborhoodSearch();
+        HashTable* grid = sim->getGrid();
+        const Real h = sim->getSupportRadius();
+        std::vector<int32_t> keys(3 * static_cast<size_t>(N), 0);
+        std::vector<const HashTable::Bucket*> cellOf(N, nullptr);
+        for (auto& kv : grid->table) {
+            for (auto& pi : kv.second.points) {
+                keys[3 * pi.id] = kv.first.x;
+                keys[3 * pi.id + 1] = kv.first.y;
+                keys[3 * pi.id + 2] = kv.first.z;
+                cellOf[pi.id] = &kv.second;
+            }
+        }
+        // boundary object offsets for global boundary ids
+        std::vector<uint> bOff(sim->numberBoundaryModels() + 1, 0);
+        for (uint o = 0; o < sim->numberBoundaryModels(); ++o)
+            bOff[o + 1] = bOff[o] + sim->getBoundaryModel(o)->getSize();
+        std::vector<int64_t> off(N + 1, 0), boff(N + 1, 0);
+        std::vector<int32_t> ids, bids;
+        for (uint i = 0; i < N; ++i) {
+            const Vector3r& ri = f->getPosition(i);
+            std::vector<int32_t> s, sb;
+            for (auto& pj : cellOf[i]->neighbors) {
+                if (glm::length(ri - f->getPosition(pj.id)) <= h) s.push_back(static_cast<int32_t>(pj.id));
+            }
+            for (auto& pb : cellOf[i]->boundary_neighbors) {
+                const Vector3r& rb = sim->getBoundaryModel(pb.pointSetId)->getPosition(pb.id);
+                if (glm::length(ri - rb) <= h) sb.push_back(static_cast<int32_t>(bOff[pb.pointSetId] + pb.id));
+            }
+            std::sort(s.begin(), s.end());
+            std::sort(sb.begin(), sb.end());
+            ids.insert(ids.end(), s.begin(), s.end());
+            bids.insert(bids.end(), sb.begin(), sb.end());
+            off[i + 1] = static_cast<int64_t>(ids.size());
+            boff[i + 1] = static_cast<int64_t>(bids.size());
+        }
+        dump(outdir, "keys", "<i4", keys, 3);
+        dump(outdir, "nbr_off", "<i8", off, 1);
+        dump(outdir, "nbr_ids", "<i4", ids, 1);
+        dump(outdir, "bnbr_off", "<i8", boff, 1);
+        dump(outdir, "bnbr_ids", "<i4", bids, 1);
+    }
+    std::printf("done N=%u Nb=%zu steps=%d secs=%.6f\n", N, bvol.size(), nsteps, secs);
+    return 0;
+}
