@@ -1,0 +1,177 @@
+"""Benchmark: DFSPH particle-steps/s on the 1M-particle dam break (one MI355X).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Workload (the largest configuration that fits one GPU and the one the metric
+is quoted on): DFSPH dam break, 100^3 = 1,000,000 FP32 fluid particles
+(r = 0.005, h = 0.02, lattice in [0,1]^3) plus 400,002 Akinci boundary
+particles sampled on a 4 x 2 x 2 m box; CFL-adaptive dt (max 1e-3); density
+tolerance 0.01%, divergence tolerance 0.1%, 2..100 iterations; Akinci
+surface tension gamma = 0.1. A step is one DFSPH step of every fluid
+particle; inputs are resident in HBM when the timed region starts.
+
+Prints ONE JSON line with value = fluid particle-steps per second over K
+timed steps (after W untimed warmup steps), the roofline of the dominant
+kernel (its algorithmic bytes per launch over its average launch time,
+measured with HIP events on the context's stream inside the timed region)
+and a CPU baseline: the original SPH2O code compiled with OpenMP
+(oracle/_ref/sph2o_ref) timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DFSPH particle-steps/s at 1M (ρ err≤0.1%); batched gain; adjoint/fwd cost"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per fluid particle per launch (streaming model: each
+# pass reads/writes each particle's own fields once, x counted as 12 B, FP32;
+# neighbour re-reads hit LDS/L2; boundary records (x 12 B + volume 4 B) read
+# once per boundary-reading pass, i.e. + 16 Nb/Nf bytes per particle).
+# name -> (own-field bytes, reads boundary)
+ALGO_BYTES = {
+    "rho_star": (48, True),     # x, v, rho, alpha, k_ws in; rho*, k, k_ws out
+    "velocity": (40, True),     # x, k, v in; v out
+    "divergence": (48, True),   # x, v, rho, alpha, kv_ws in; d, kv, kv_ws out
+    "density": (20, True),      # x in; rho, alpha out
+    "forces": (28, False),      # x, rho in; normal out (surface-tension pass 1)
+    "surface_tension": (52, False),  # x, normal, rho, a in; a out
+    "build_neighbors": (28, True),   # x, cell key in; ranges (model cost of the grid build)
+    "predict_velocity": (36, False),
+    "keys": (36, False),
+    "reorder": (56, False),
+    "vmax": (24, False),
+    "scale": (8, False),
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scene_path, nf, steps, threads):
+    """Time the compiled reference on the first `steps` steps of the workload."""
+    from oracle import sphref as S
+
+    if not S.reference_available():
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+        r = subprocess.run([S.REF_BIN, scene_path, str(steps), td], env=env, capture_output=True, text=True,
+                           timeout=900)
+        if r.returncode != 0:
+            log("cpu baseline failed: " + r.stderr[-500:])
+            return None
+        meta = np.load(os.path.join(td, "meta.npy"))
+    secs = float(meta[3])
+    return {"value": nf * steps / secs, "unit": "particle-steps/s", "cores": threads, "kind": "reference",
+            "sample": "original SPH2O DFSPH step (OpenMP, -O3 -ffast-math) on the same 1M-particle scene, "
+                      "steps 1-%d, step() calls only (%.1f s)" % (steps, secs)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=100, help="fluid lattice edge (100 -> 1M particles)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from sph2o_amd import scenes as SC
+    from sph2o_amd.simulation import Context
+
+    sc, _, _ = SC.d1m(args.n)
+    td = tempfile.mkdtemp()
+    scene_path = os.path.join(td, "d1m.json")
+    SC.write_json(sc, scene_path)
+
+    ctx = Context(0)
+    sid = ctx.load_scene_json(scene_path)
+    nf, nb = ctx.num_fluid(sid), ctx.num_boundary(sid)
+    log("workload: %d fluid + %d boundary particles" % (nf, nb))
+    t0 = time.time()
+    ctx.init()
+    log("init %.2f s" % (time.time() - t0))
+
+    # warmup (untimed); the last few warmup steps are profiled to find the dominant kernel
+    prof_steps = min(5, args.warmup)
+    if args.warmup - prof_steps:
+        ctx.step(args.warmup - prof_steps)
+    ctx.set_profiling(True)
+    ctx.step(prof_steps)
+    ks = ctx.kernel_stats()
+    ctx.set_profiling(False)
+    ctx.reset_kernel_stats()
+    dominant = max(ks.items(), key=lambda kv: kv[1][0])[0] if ks else "velocity"
+    names = [ctx.L.sph_kernel_name(k).decode() for k in range(ctx.L.sph_kernel_count())]
+    kid = names.index(dominant)
+    log("dominant kernel: %s (%s)" % (dominant, ", ".join("%s %.2f ms" % (k, v[0]) for k, v in
+                                                          sorted(ks.items(), key=lambda kv: -kv[1][0])[:6])))
+
+    # timed region: only the dominant kernel carries an event pair
+    ctx.set_profiling(1 << kid if kid > 0 else 1)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    stats = ctx.step(args.steps)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    kms, klaunch = ctx.kernel_stats().get(dominant, (0.0, 0))
+    ctx.set_profiling(False)
+
+    st = ctx.fluid_state(sid)
+    rho_err = float(np.mean(np.maximum(st["rho"] / 1000.0 - 1.0, 0.0)))
+    its = np.array([[s[sid].iterations, s[sid].iterations_v] for s in stats])
+    value = nf * args.steps / elapsed
+    own, reads_b = ALGO_BYTES.get(dominant, (0, False))
+    bytes_per_launch = nf * (own + (16.0 * nb / nf if reads_b else 0.0))
+    avg_ms = kms / klaunch if klaunch else float("nan")
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if klaunch else None
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_%s_d1m.json" % dominant)
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+        log("cpu baseline: %d steps of the reference on %d threads" % (args.cpu_steps, threads))
+        cpu = cpu_baseline(scene_path, nf, args.cpu_steps, threads)
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1000.0 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "D1M: DFSPH dam break, %d fluid + %d Akinci boundary particles, r=0.005 m, "
+                               "h=0.02 m, 4x2x2 m box, CFL dt<=1e-3, eta=0.01%%, etaV=0.1%%, surface tension 0.1"
+                               % (nf, nb),
+                   "fluid_particles": nf, "boundary_particles": nb},
+        "mean_density_error": rho_err,
+        "density_error_ok": rho_err <= 1e-3,
+        "iterations_per_step": {"density": float(its[:, 0].mean()), "divergence": float(its[:, 1].mean())},
+        "sim_time_s": float(stats[-1][sid].t),
+        "roofline": {"kernel": dominant, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
+                     "avg_launch_ms": avg_ms, "launches": klaunch, "algorithmic_bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

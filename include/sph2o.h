@@ -1,0 +1,170 @@
+/*
+ * sph2o.h — C ABI of the MI355X particle step (libsph2o.so).
+ *
+ * One context owns one or more independent scenes on one GPU and one HIP
+ * stream. Every scene is a single-fluid SPH2O scene (the reference's
+ * neighbour macros ignore pointSetId, SPHSimulation.h:27-32, so only one
+ * fluid per scene is well defined) with Akinci boundary particles. The three
+ * solvers of the reference are supported: WCSPH, PCISPH and DFSPH.
+ *
+ * Conventions (replacing the reference's bool returns, uncaught exceptions
+ * and exit(0); SPHSimulation.cpp:299-555, SceneLoader.cpp:11-38):
+ *   - every function returns an int status: SPH_OK (0) or a negative code;
+ *   - sph_last_error(ctx) gives a message for the last failure;
+ *   - host arrays are caller-owned and use the scene's original particle
+ *     order (xyz interleaved, float32);
+ *   - no function throws across the ABI.
+ *
+ * Reference interfaces replaced (file:line in josemanuel28v/SPH2O):
+ *   sph_load_scene_json   SPHSimulation::importScene, SPHSimulation.cpp:299-481
+ *                         + SceneLoader::readConfiguration, SceneLoader.cpp:11-209
+ *   sph_init              SPHSimulation::init, SPHSimulation.cpp:117-126
+ *                         + {WCSPH,PCISPH,DFSPH}Solver::init
+ *   sph_step              SPHSolver::step (WCSPHSolver.cpp:14-78,
+ *                         PCISPHSolver.cpp:20-80, DFSPHSolver.cpp:33-107)
+ *                         + the time advance of SPHSimulation::step, :157-191
+ *   sph_get_fluid_state   FluidObject getters (FluidObject.h:54-63),
+ *                         SimulationObject::getPositions (SimulationObject.h:16)
+ *   sph_set_fluid_state   FluidObject::getPosition/getVelocity writes
+ *                         (SimulationObject.h:17, FluidObject.h:59)
+ *   sph_dump_cells        HashTable key floor(pos/h), HashTable.cpp:5
+ *   sph_dump_neighbors    HashTable::neighborhoodSearch, HashTable.cpp:23-86,
+ *                         filtered by the support test of DFSPHSolver.cpp:575
+ */
+#ifndef SPH2O_H_
+#define SPH2O_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SPH_OK 0
+#define SPH_E_ARG (-1)          /* bad argument (NULL, size mismatch) */
+#define SPH_E_SCENE (-2)        /* scene JSON missing/ill-typed key or unreadable */
+#define SPH_E_UNSUPPORTED (-3)  /* method/option outside the hot path (e.g. adhesion) */
+#define SPH_E_HIP (-4)          /* HIP runtime error */
+#define SPH_E_NONFINITE (-5)    /* NaN/Inf found in particle state */
+#define SPH_E_OUT_OF_GRID (-6)  /* particles left every representable grid */
+#define SPH_E_STATE (-7)        /* call order (e.g. step before init) */
+#define SPH_E_CAPACITY (-8)     /* caller buffer too small */
+
+/* method codes of SPHSimulation.cpp:21-33 */
+#define SPH_WCSPH 0
+#define SPH_PCISPH 1
+#define SPH_DFSPH 2
+#define SPH_BOUNDARY_NONE (-1)
+#define SPH_BOUNDARY_AKINCI 2
+#define SPH_VISCOSITY_STANDARD 0
+#define SPH_VISCOSITY_ARTIFICIAL 1
+#define SPH_VISCOSITY_XSPH 2
+#define SPH_SURFACE_TENSION_AKINCI 3
+
+/* Scene parameters: the fields of SceneInfo / FluidInfo / Fluid
+ * (SPHStructs.h:11-64) as the reference stores them (FP32). */
+typedef struct sph_scene_params {
+    int32_t simulation_method;      /* SPH_WCSPH / SPH_PCISPH / SPH_DFSPH */
+    int32_t boundary_method;        /* SPH_BOUNDARY_AKINCI or SPH_BOUNDARY_NONE */
+    int32_t viscosity_method;       /* -1 or SPH_VISCOSITY_* */
+    int32_t surface_tension_method; /* -1 or SPH_SURFACE_TENSION_AKINCI */
+    int32_t adhesion_method;        /* must be -1 */
+    uint32_t min_iterations, max_iterations;
+    uint32_t min_iterations_v, max_iterations_v;
+    int32_t adaptive_time_step;     /* DFSPH: 1 = CFL + iteration rule of the reference (default) */
+    float particle_radius, density0;
+    float gravity[3];
+    float time_step, min_time_step, max_time_step, fps, cfl_factor;
+    float start_time, end_time;
+    float stiffness, gamma;         /* WCSPH */
+    float eta, eta_v;               /* PCISPH / DFSPH tolerances */
+    float viscosity, boundary_viscosity, surface_tension, adhesion;
+    int32_t reserved[8];
+} sph_scene_params;
+
+/* Per-step statistics (one per scene per step). */
+typedef struct sph_step_stats {
+    double t, dt;
+    int32_t iterations;              /* density / pressure solve iterations */
+    int32_t iterations_v;            /* DFSPH divergence iterations */
+    double avg_density_error;        /* FP64 reduction the solver tested last */
+    double avg_divergence_error;     /* FP64 (DFSPH) */
+    double max_velocity;             /* FP64 CFL reduction (DFSPH) */
+    int32_t capped_density;          /* 1 if the density loop stopped at max_iterations */
+    int32_t capped_divergence;       /* 1 if the divergence loop stopped at the bound */
+    int32_t scene;
+    int32_t reserved;
+} sph_step_stats;
+
+typedef struct sph_ctx sph_ctx;
+
+/* Context: device ordinal and a hipStream_t (NULL = a stream the context creates). */
+int sph_create(sph_ctx** ctx, int device, void* stream);
+int sph_destroy(sph_ctx* ctx);
+const char* sph_last_error(const sph_ctx* ctx);
+const char* sph_version(void);
+
+/* Scenes. sph_add_scene returns the scene id (>= 0) or a negative status.
+ * fluid/boundary positions are N*3 float32 host arrays; vel may be NULL (zero). */
+int sph_add_scene(sph_ctx* ctx, const sph_scene_params* params, int64_t nf, const float* pos, const float* vel,
+                  int64_t nb, const float* boundary_pos);
+/* Reads a scene JSON of the reference schema, samples fluid blocks and
+ * boundary boxes/spheres like FluidObject::build / AkinciBoundaryObject, and
+ * adds it. Returns the scene id or a negative status. */
+int sph_load_scene_json(sph_ctx* ctx, const char* path);
+/* The same reader and samplers without a context or a GPU: fills `params`,
+ * writes the fluid and boundary particle counts to *nf / *nb and, when the
+ * capacities (in particles) allow, their positions (N*3 float32). Returns
+ * SPH_E_CAPACITY if pos_cap or bpos_cap is too small (counts still written). */
+int sph_scene_from_json(const char* path, sph_scene_params* params, int64_t* nf, float* pos, int64_t pos_cap,
+                        int64_t* nb, float* bpos, int64_t bpos_cap);
+int sph_num_scenes(const sph_ctx* ctx);
+int sph_get_params(const sph_ctx* ctx, int scene, sph_scene_params* out);
+int64_t sph_num_fluid(const sph_ctx* ctx, int scene);
+int64_t sph_num_boundary(const sph_ctx* ctx, int scene);
+
+/* Build device state, boundary volumes, and the solver's init pass. */
+int sph_init(sph_ctx* ctx);
+/* Advance every scene nsteps. stats: NULL or nsteps*num_scenes entries (step-major). */
+int sph_step(sph_ctx* ctx, int nsteps, sph_step_stats* stats);
+int sph_synchronize(sph_ctx* ctx);
+
+/* State I/O in original order; any pointer may be NULL. */
+int sph_set_fluid_state(sph_ctx* ctx, int scene, int64_t n, const float* pos, const float* vel);
+int sph_get_fluid_state(sph_ctx* ctx, int scene, float* pos, float* vel, float* density, float* pressure);
+int sph_get_boundary(sph_ctx* ctx, int scene, float* pos, float* volume);
+/* field: 0 alpha (DFSPH), 1 acceleration xyz (N*3), 2 predicted density / error scratch,
+ * 3 k_ws, 4 kv_ws, 5 surface normals xyz (N*3) */
+int sph_get_field(sph_ctx* ctx, int scene, int field, float* out);
+int sph_get_time(const sph_ctx* ctx, int scene, double* t, double* dt);
+
+/* Neighbour search results at the current positions (original ids).
+ * cells: N*3 int32 cell keys. neighbours: CSR offsets[N+1] into ids; each
+ * row lists the fluid j with |x_i - x_j| <= h among the 27-cell candidates
+ * (self included), unsorted; likewise the boundary rows. Returns
+ * SPH_E_CAPACITY when cap/bcap is too small (offsets are still written). */
+int sph_dump_cells(sph_ctx* ctx, int scene, int32_t* cells);
+int sph_dump_neighbors(sph_ctx* ctx, int scene, int64_t* offsets, int32_t* ids, int64_t cap, int64_t* boffsets,
+                       int32_t* bids, int64_t bcap);
+
+/* Wall time of the last sph_step call on the GPU timeline (HIP events on the
+ * context's stream) and the number of kernel launches it made. */
+int sph_last_step_timing(const sph_ctx* ctx, double* total_ms, int64_t* launches);
+
+/* Per-kernel timing with HIP events on the context's stream (no reference
+ * counterpart; the reference's only timer is TimeHelper, TimeHelper.h:70-98).
+ * on: 0 off; 1 or -1 every kernel; otherwise a bit mask of kernel ids
+ * ((on >> id) & 1, ids 1..30). Profiled launches are bracketed by an event
+ * pair and the elapsed times are summed per kernel id at the end of each
+ * sph_step. */
+int sph_set_profiling(sph_ctx* ctx, int on);
+int sph_kernel_count(void);
+const char* sph_kernel_name(int kernel_id);
+int sph_kernel_stats(const sph_ctx* ctx, int kernel_id, double* total_ms, int64_t* launches);
+int sph_reset_kernel_stats(sph_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPH2O_H_ */

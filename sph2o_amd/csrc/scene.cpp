@@ -1,0 +1,341 @@
+// Scene-JSON reader and particle samplers (host side of the drop-in boundary).
+#include "scene.hpp"
+
+#include <cctype>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+
+namespace sph {
+namespace {
+
+// ---- a small JSON DOM (objects, arrays, numbers, strings, bools, null) ----
+struct JVal {
+    enum Kind { Null, Bool, Num, Str, Arr, Obj } kind = Null;
+    bool b = false;
+    double num = 0;
+    std::string str;
+    std::vector<JVal> arr;
+    std::vector<std::pair<std::string, JVal>> obj;
+    const JVal* find(const char* k) const {
+        if (kind != Obj) return nullptr;
+        for (auto& kv : obj)
+            if (kv.first == k) return &kv.second;
+        return nullptr;
+    }
+};
+
+struct Parser {
+    const char* p;
+    const char* end;
+    std::string err;
+    void ws() {
+        while (p < end && std::isspace(static_cast<unsigned char>(*p))) ++p;
+    }
+    bool fail(const char* m) {
+        if (err.empty()) err = m;
+        return false;
+    }
+    bool parse(JVal& v, int depth = 0) {
+        if (depth > 64) return fail("JSON nested too deeply");
+        ws();
+        if (p >= end) return fail("unexpected end of JSON");
+        const char c = *p;
+        if (c == '{') {
+            ++p;
+            v.kind = JVal::Obj;
+            ws();
+            if (p < end && *p == '}') { ++p; return true; }
+            while (true) {
+                ws();
+                std::string key;
+                if (!parseString(key)) return false;
+                ws();
+                if (p >= end || *p != ':') return fail("expected ':' in JSON object");
+                ++p;
+                JVal child;
+                if (!parse(child, depth + 1)) return false;
+                v.obj.emplace_back(std::move(key), std::move(child));
+                ws();
+                if (p < end && *p == ',') { ++p; continue; }
+                if (p < end && *p == '}') { ++p; return true; }
+                return fail("expected ',' or '}' in JSON object");
+            }
+        }
+        if (c == '[') {
+            ++p;
+            v.kind = JVal::Arr;
+            ws();
+            if (p < end && *p == ']') { ++p; return true; }
+            while (true) {
+                JVal child;
+                if (!parse(child, depth + 1)) return false;
+                v.arr.push_back(std::move(child));
+                ws();
+                if (p < end && *p == ',') { ++p; continue; }
+                if (p < end && *p == ']') { ++p; return true; }
+                return fail("expected ',' or ']' in JSON array");
+            }
+        }
+        if (c == '"') {
+            v.kind = JVal::Str;
+            return parseString(v.str);
+        }
+        if (!std::strncmp(p, "true", std::min<size_t>(4, end - p)) && end - p >= 4) { p += 4; v.kind = JVal::Bool; v.b = true; return true; }
+        if (!std::strncmp(p, "false", std::min<size_t>(5, end - p)) && end - p >= 5) { p += 5; v.kind = JVal::Bool; return true; }
+        if (!std::strncmp(p, "null", std::min<size_t>(4, end - p)) && end - p >= 4) { p += 4; v.kind = JVal::Null; return true; }
+        if (c == '-' || (c >= '0' && c <= '9')) {
+            std::string tok;
+            while (p < end && (std::isdigit(static_cast<unsigned char>(*p)) || *p == '-' || *p == '+' || *p == '.' ||
+                               *p == 'e' || *p == 'E'))
+                tok.push_back(*p++);
+            char* e = nullptr;
+            v.num = std::strtod(tok.c_str(), &e);
+            if (!e || *e) return fail("bad JSON number");
+            v.kind = JVal::Num;
+            return true;
+        }
+        return fail("unexpected character in JSON");
+    }
+    bool parseString(std::string& s) {
+        if (p >= end || *p != '"') return fail("expected JSON string");
+        ++p;
+        while (p < end && *p != '"') {
+            if (*p == '\\') {
+                ++p;
+                if (p >= end) return fail("bad escape");
+                const char e = *p++;
+                switch (e) {
+                    case 'n': s.push_back('\n'); break;
+                    case 't': s.push_back('\t'); break;
+                    case 'r': s.push_back('\r'); break;
+                    case 'b': s.push_back('\b'); break;
+                    case 'f': s.push_back('\f'); break;
+                    case 'u': {  // keep ASCII subset only
+                        if (end - p < 4) return fail("bad \\u escape");
+                        const unsigned cp = static_cast<unsigned>(std::strtoul(std::string(p, 4).c_str(), nullptr, 16));
+                        s.push_back(cp < 128 ? static_cast<char>(cp) : '?');
+                        p += 4;
+                        break;
+                    }
+                    default: s.push_back(e);
+                }
+            } else {
+                s.push_back(*p++);
+            }
+        }
+        if (p >= end) return fail("unterminated JSON string");
+        ++p;
+        return true;
+    }
+};
+
+struct Reader {
+    std::string err;
+    bool num(const JVal* obj, const char* key, double& out, const std::string& where) {
+        const JVal* v = obj ? obj->find(key) : nullptr;
+        if (!v) { err = "missing key " + where + "." + key; return false; }
+        if (v->kind == JVal::Num) { out = v->num; return true; }
+        if (v->kind == JVal::Bool) { out = v->b ? 1 : 0; return true; }
+        err = "key " + where + "." + key + " is not a number";
+        return false;
+    }
+    bool f32(const JVal* obj, const char* key, float& out, const std::string& where) {
+        double d;
+        if (!num(obj, key, d, where)) return false;
+        out = static_cast<float>(d);
+        return true;
+    }
+    bool i32(const JVal* obj, const char* key, int32_t& out, const std::string& where) {
+        double d;
+        if (!num(obj, key, d, where)) return false;
+        out = static_cast<int32_t>(d);
+        return true;
+    }
+    bool u32(const JVal* obj, const char* key, uint32_t& out, const std::string& where) {
+        double d;
+        if (!num(obj, key, d, where)) return false;
+        out = static_cast<uint32_t>(d < 0 ? 0 : d);
+        return true;
+    }
+    bool vec3(const JVal* obj, const char* key, float out[3], const std::string& where) {
+        const JVal* v = obj ? obj->find(key) : nullptr;
+        if (!v || v->kind != JVal::Arr || v->arr.size() < 3) { err = "key " + where + "." + key + " must be [x,y,z]"; return false; }
+        for (int a = 0; a < 3; ++a) {
+            if (v->arr[a].kind != JVal::Num) { err = "key " + where + "." + key + " must hold numbers"; return false; }
+            out[a] = static_cast<float>(v->arr[a].num);
+        }
+        return true;
+    }
+};
+
+}  // namespace
+
+int parseSceneJson(const std::string& text, SceneDesc& out, std::string& err) {
+    JVal root;
+    Parser ps{text.data(), text.data() + text.size(), {}};
+    if (!ps.parse(root)) { err = "scene JSON: " + ps.err; return SPH_E_SCENE; }
+    Reader rd;
+    sph_scene_params& P = out.params;
+    std::memset(&P, 0, sizeof(P));
+    P.adaptive_time_step = 1;
+    P.min_iterations = 3; P.max_iterations = 100;   // SPHSolver.cpp:15-16
+    P.min_iterations_v = 1; P.max_iterations_v = 100;
+    P.eta = 0.01f;                                  // SPHSolver.cpp:18
+    P.density0 = 1000.0f;                           // FluidObject.cpp:12
+    P.viscosity_method = P.surface_tension_method = P.adhesion_method = -1;
+
+    // Configuration (SceneLoader.cpp:40-78)
+    const JVal* C = root.find("Configuration");
+    const std::string cw = "Configuration";
+    if (!C) { err = "missing key Configuration"; return SPH_E_SCENE; }
+    if (!rd.f32(C, "startTime", P.start_time, cw) || !rd.f32(C, "endTime", P.end_time, cw) ||
+        !rd.f32(C, "timeStep", P.time_step, cw) || !rd.f32(C, "fps", P.fps, cw) ||
+        !rd.f32(C, "minTimeStep", P.min_time_step, cw) || !rd.f32(C, "maxTimeStep", P.max_time_step, cw) ||
+        !rd.f32(C, "cflFactor", P.cfl_factor, cw) || !rd.f32(C, "particleRadius", P.particle_radius, cw) ||
+        !rd.i32(C, "boundaryMethod", P.boundary_method, cw) || !rd.i32(C, "simulationMethod", P.simulation_method, cw) ||
+        !rd.vec3(C, "gravity", P.gravity, cw)) {
+        err = rd.err;
+        return SPH_E_SCENE;
+    }
+    if (P.simulation_method == SPH_WCSPH) {
+        if (!rd.f32(C, "stiffness", P.stiffness, cw) || !rd.f32(C, "gamma", P.gamma, cw)) { err = rd.err; return SPH_E_SCENE; }
+    } else if (P.simulation_method == SPH_PCISPH) {
+        if (!rd.f32(C, "eta", P.eta, cw) || !rd.u32(C, "minIterations", P.min_iterations, cw) ||
+            !rd.u32(C, "maxIterations", P.max_iterations, cw)) { err = rd.err; return SPH_E_SCENE; }
+    } else if (P.simulation_method == SPH_DFSPH) {
+        if (!rd.f32(C, "eta", P.eta, cw) || !rd.f32(C, "etaV", P.eta_v, cw) ||
+            !rd.u32(C, "minIterations", P.min_iterations, cw) || !rd.u32(C, "maxIterations", P.max_iterations, cw) ||
+            !rd.u32(C, "minIterationsV", P.min_iterations_v, cw) ||
+            !rd.u32(C, "maxIterationsV", P.max_iterations_v, cw)) { err = rd.err; return SPH_E_SCENE; }
+    } else {
+        err = "Configuration.simulationMethod must be 0 (WCSPH), 1 (PCISPH) or 2 (DFSPH)";
+        return SPH_E_SCENE;
+    }
+
+    // Fluid (SceneLoader.cpp:80-146)
+    const JVal* F = root.find("Fluid");
+    const std::string fw = "Fluid";
+    if (!F) { err = "missing key Fluid"; return SPH_E_SCENE; }
+    if (!rd.i32(F, "viscosityMethod", P.viscosity_method, fw) ||
+        !rd.i32(F, "surfaceTensionMethod", P.surface_tension_method, fw) ||
+        !rd.i32(F, "adhesionMethod", P.adhesion_method, fw)) { err = rd.err; return SPH_E_SCENE; }
+    const JVal* FM = F->find("FluidModel");
+    if (!FM || FM->kind != JVal::Arr) { err = "missing array Fluid.FluidModel"; return SPH_E_SCENE; }
+    out.numFluidModels = static_cast<int>(FM->arr.size());
+    if (out.numFluidModels != 1) {
+        err = "exactly one FluidModel is supported (the reference's neighbour loops ignore pointSetId)";
+        return SPH_E_UNSUPPORTED;
+    }
+    const JVal& m = FM->arr[0];
+    const std::string mw = "Fluid.FluidModel[0]";
+    if (!rd.f32(&m, "density0", P.density0, mw)) { err = rd.err; return SPH_E_SCENE; }
+    if (P.viscosity_method > -1 &&
+        (!rd.f32(&m, "viscosity", P.viscosity, mw) || !rd.f32(&m, "boundaryViscosity", P.boundary_viscosity, mw))) {
+        err = rd.err;
+        return SPH_E_SCENE;
+    }
+    if (P.surface_tension_method > -1 && !rd.f32(&m, "surfaceTension", P.surface_tension, mw)) { err = rd.err; return SPH_E_SCENE; }
+    if (P.adhesion_method > -1 && !rd.f32(&m, "adhesion", P.adhesion, mw)) { err = rd.err; return SPH_E_SCENE; }
+    const JVal* blocks = m.find("fluidBlocks");
+    if (!blocks || blocks->kind != JVal::Arr) { err = "missing array " + mw + ".fluidBlocks"; return SPH_E_SCENE; }
+    for (size_t j = 0; j < blocks->arr.size(); ++j) {
+        BlockDesc b;
+        const std::string bw = mw + ".fluidBlocks[" + std::to_string(j) + "]";
+        if (!rd.vec3(&blocks->arr[j], "min", b.min, bw) || !rd.vec3(&blocks->arr[j], "max", b.max, bw)) { err = rd.err; return SPH_E_SCENE; }
+        out.fluidBlocks.push_back(b);
+    }
+    if (const JVal* em = m.find("emitters")) out.numEmitters = em->kind == JVal::Arr ? static_cast<int>(em->arr.size()) : 0;
+
+    // Boundary (SceneLoader.cpp:148-209); absent sphere/geometry arrays are empty.
+    if (const JVal* B = root.find("Boundary")) {
+        if (B->kind != JVal::Arr) { err = "Boundary must be an array"; return SPH_E_SCENE; }
+        for (size_t i = 0; i < B->arr.size(); ++i) {
+            const JVal& bo = B->arr[i];
+            const std::string bw = "Boundary[" + std::to_string(i) + "]";
+            if (const JVal* bx = bo.find("box")) {
+                if (bx->kind != JVal::Arr) { err = bw + ".box must be an array"; return SPH_E_SCENE; }
+                for (size_t j = 0; j < bx->arr.size(); ++j) {
+                    BlockDesc b;
+                    const std::string w = bw + ".box[" + std::to_string(j) + "]";
+                    if (!rd.vec3(&bx->arr[j], "min", b.min, w) || !rd.vec3(&bx->arr[j], "max", b.max, w)) { err = rd.err; return SPH_E_SCENE; }
+                    out.boxes.push_back(b);
+                }
+            }
+            if (const JVal* sp = bo.find("sphere")) {
+                if (sp->kind != JVal::Arr) { err = bw + ".sphere must be an array"; return SPH_E_SCENE; }
+                for (size_t j = 0; j < sp->arr.size(); ++j) {
+                    SphereDesc s;
+                    const std::string w = bw + ".sphere[" + std::to_string(j) + "]";
+                    if (!rd.vec3(&sp->arr[j], "pos", s.pos, w) || !rd.f32(&sp->arr[j], "radius", s.radius, w)) { err = rd.err; return SPH_E_SCENE; }
+                    out.spheres.push_back(s);
+                }
+            }
+            if (const JVal* ge = bo.find("geometry"))
+                out.numGeometries += ge->kind == JVal::Arr ? static_cast<int>(ge->arr.size()) : 0;
+        }
+    }
+    return SPH_OK;
+}
+
+void sampleFluidBlock(const BlockDesc& b, float radius, std::vector<float>& xyz) {
+    const float dist = 2.0f * radius;
+    unsigned pp[3];
+    for (int a = 0; a < 3; ++a) pp[a] = static_cast<unsigned>(std::floor((b.max[a] - b.min[a]) / dist + 1e-5f));
+    float base[3];
+    for (int a = 0; a < 3; ++a) base[a] = b.min[a] + radius;
+    for (unsigned i = 0; i < pp[0]; ++i)
+        for (unsigned j = 0; j < pp[1]; ++j)
+            for (unsigned k = 0; k < pp[2]; ++k) {
+                xyz.push_back(static_cast<float>(i) * dist + base[0]);
+                xyz.push_back(static_cast<float>(j) * dist + base[1]);
+                xyz.push_back(static_cast<float>(k) * dist + base[2]);
+            }
+}
+
+void sampleBoxSurface(const BlockDesc& b, float radius, std::vector<float>& xyz) {
+    float l[3], n[3], d[3];
+    for (int a = 0; a < 3; ++a) {
+        l[a] = std::fabs(b.min[a] - b.max[a]);
+        n[a] = std::ceil(l[a] / (2.0f * radius));
+        d[a] = l[a] / n[a];
+    }
+    for (unsigned i = 0; i <= n[0]; i++)
+        for (unsigned j = 0; j <= n[1]; j++)
+            for (unsigned k = 0; k <= n[2]; k++)
+                if (i == 0 || i == n[0] || j == 0 || j == n[1] || k == 0 || k == n[2]) {
+                    xyz.push_back(static_cast<float>(i) * d[0] + b.min[0]);
+                    xyz.push_back(static_cast<float>(j) * d[1] + b.min[1]);
+                    xyz.push_back(static_cast<float>(k) * d[2] + b.min[2]);
+                }
+}
+
+void sampleSphereSurface(const SphereDesc& s, float radius, std::vector<float>& xyz) {
+    const double sep = 1.5;
+    const unsigned numParts = static_cast<unsigned>(
+        std::floor(4.0 * M_PI * s.radius * s.radius / (M_PI * radius * radius) * sep));
+    if (numParts == 0) return;
+    const float a = static_cast<float>(4.0 * M_PI) / numParts;
+    const float d = std::sqrt(a);
+    const unsigned mTheta = static_cast<unsigned>(std::round(M_PI / d));
+    if (mTheta == 0) return;
+    const float dTheta = static_cast<float>(M_PI) / mTheta;
+    const float dPhi = a / dTheta;
+    for (unsigned m = 0; m < mTheta; ++m) {
+        const float theta = static_cast<float>(M_PI * (m + 0.5)) / mTheta;
+        const unsigned mPhi = static_cast<unsigned>(std::round(2.0 * M_PI * std::sin(theta) / dPhi));
+        for (unsigned n = 0; n < mPhi; ++n) {
+            const float phi = static_cast<float>(2.0 * M_PI) * n / mPhi;
+            const float x = std::sin(theta) * std::cos(phi);
+            const float y = std::sin(theta) * std::sin(phi);
+            const float z = std::cos(theta);
+            xyz.push_back(s.radius * x + s.pos[0]);
+            xyz.push_back(s.radius * y + s.pos[1]);
+            xyz.push_back(s.radius * z + s.pos[2]);
+        }
+    }
+}
+
+}  // namespace sph

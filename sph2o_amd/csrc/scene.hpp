@@ -1,0 +1,36 @@
+// Scene description and the scene-JSON reader of the reference schema
+// (SceneLoader.cpp:40-209, SPHStructs.h:11-106), plus the particle samplers
+// (FluidObject::build, AkinciBoundaryObject::sampleCube / sampleSphere).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/sph2o.h"
+
+namespace sph {
+
+struct BlockDesc { float min[3], max[3]; };
+struct SphereDesc { float pos[3]; float radius; };
+
+struct SceneDesc {
+    sph_scene_params params{};
+    std::vector<BlockDesc> fluidBlocks;   // FluidModel[0].fluidBlocks
+    std::vector<BlockDesc> boxes;         // Boundary[*].box (all boundary objects)
+    std::vector<SphereDesc> spheres;      // Boundary[*].sphere
+    int numFluidModels = 0;
+    int numEmitters = 0;
+    int numGeometries = 0;
+};
+
+// Parses `text` (a scene JSON). Returns SPH_OK or SPH_E_SCENE/SPH_E_UNSUPPORTED with `err` set.
+int parseSceneJson(const std::string& text, SceneDesc& out, std::string& err);
+
+// FluidObject::build (FluidObject.cpp:30-61): lattice of spacing 2r starting at min + r.
+void sampleFluidBlock(const BlockDesc& b, float radius, std::vector<float>& xyz);
+// AkinciBoundaryObject::sampleCube (AkinciBoundaryObject.cpp:97-123).
+void sampleBoxSurface(const BlockDesc& b, float radius, std::vector<float>& xyz);
+// AkinciBoundaryObject::sampleSphere (AkinciBoundaryObject.cpp:125-153).
+void sampleSphereSurface(const SphereDesc& s, float radius, std::vector<float>& xyz);
+
+}  // namespace sph

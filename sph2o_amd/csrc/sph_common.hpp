@@ -1,0 +1,54 @@
+// Shared host/device definitions of the MI355X particle step.
+#pragma once
+
+#include <cstdint>
+
+namespace sph {
+
+constexpr int kBlock = 256;  // particles per workgroup (4 wave64)
+
+// Per-scene constants, uploaded at init / regrid. One scene = one SPH2O
+// single-fluid scene with its own grid, particle ranges and coefficients.
+struct SceneConst {
+    // grid: cell key = floor(x * invh), stored relative to the origin cell
+    float h, invh;
+    int32_t g0x, g0y, g0z;     // origin cell
+    int32_t gnx, gny, gnz;     // cells per axis
+    uint32_t cellBase;         // first entry of this scene in the cell-range arrays
+    uint32_t keyBase;          // first sort key of this scene
+    uint32_t deadKey;          // sort key of padding slots (last key of the scene's range)
+    int32_t morton;            // 1: Morton-ordered keys, 0: x-fastest linear keys
+    // particle slots
+    uint32_t fBegin, fEnd;     // fluid slots [fBegin, fEnd)
+    uint32_t fPadEnd;          // padded end (multiple of kBlock)
+    uint32_t blockBegin, blockEnd;
+    uint32_t bBegin, bEnd;     // boundary particles [bBegin, bEnd)
+    // physics (reference FP32 values)
+    float m, rho0, gx, gy, gz;
+    float kW, lW, laplC, cohC1, cohC2;  // CubicSpline k,l; ViscoK laplCoef; Cohesion valueCoef1/2
+    float visc, bvisc, st;
+    int32_t viscMethod;        // -1 off, 0 standard, 1 artificial, 2 XSPH
+    int32_t stOn, bviscOn, akinci;
+    float wcB, gamma;          // WCSPH: stiffness*rho0/gamma, gamma
+    float pciDelta;            // PCISPH scaling factor
+    float eps;                 // DFSPH gate 1e-5
+    float h2c;                 // 0.01*h*h (artificial viscosity)
+    int32_t pad[3];
+};
+
+// Per-scene scalars the host rewrites during a step.
+struct SceneDyn {
+    float dt;        // current time step
+    float dt2;       // dt*dt
+    float scale;     // multiplier used by the rescale kernels
+    int32_t active;  // scene takes part in the current launch
+    int32_t first;   // first solver iteration: accumulate k_ws = k instead of +=
+    int32_t pad[3];
+};
+
+// Device-side flag slots per scene (int32 each).
+enum FlagSlot { kFlagOutOfGrid = 0, kFlagMaxFluidNbr = 1, kFlagMaxBoundaryNbr = 2, kFlagNonFinite = 3, kFlagSlots = 4 };
+// Reduction output slots per scene (double each).
+enum RedSlot { kRedSum = 0, kRedMax = 1, kRedSlots = 2 };
+
+}  // namespace sph

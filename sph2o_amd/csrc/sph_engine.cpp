@@ -1,0 +1,1274 @@
+// Host engine of the MI355X particle step and the C ABI of include/sph2o.h.
+//
+// The engine owns every device buffer of a context, lays scenes out in
+// 256-slot blocks, runs the neighbour search (cell keys -> radix sort ->
+// reorder + cell ranges -> within-support lists) and drives the per-method
+// step exactly in the reference's call order:
+//   WCSPH  WCSPHSolver::step   (WCSPHSolver.cpp:14-78)
+//   PCISPH PCISPHSolver::step  (PCISPHSolver.cpp:20-80)
+//   DFSPH  DFSPHSolver::step   (DFSPHSolver.cpp:33-107)
+// The solver loops stay on the host: each iteration launches its two
+// neighbour passes, reads back the FP64 per-scene error while the second
+// pass runs, and decides per scene whether to iterate again. Scenes of one
+// context advance together; a converged scene sits out later iterations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <new>
+#include <numeric>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/sph2o.h"
+#include "scene.hpp"
+#include "sph_kernels.hpp"
+
+using namespace sph;
+
+namespace {
+
+struct HipError {
+    hipError_t e;
+    const char* where;
+};
+#define HIPCHK(call)                                        \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) throw HipError{e_, #call};    \
+    } while (0)
+
+struct SphError {
+    int code;
+    std::string msg;
+};
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        HIPCHK(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+enum KernelId {
+    K_KEYS, K_SORT, K_REORDER, K_BUILD, K_DENSITY, K_FORCES, K_SURFACE_TENSION, K_VMAX, K_PREDICT, K_RHO_STAR,
+    K_VELOCITY, K_DIVERGENCE, K_SCALE, K_INTEGRATE, K_PCI_INIT, K_PCI_DENSITY, K_PCI_PACC, K_PCI_INTEGRATE,
+    K_FILL, K_COUNT
+};
+const char* kKernelNames[K_COUNT] = {"keys", "sort", "reorder", "build_neighbors", "density", "forces",
+                                     "surface_tension", "vmax", "predict_velocity", "rho_star", "velocity",
+                                     "divergence", "scale", "integrate", "pci_init", "pci_density",
+                                     "pci_pressure_acc", "pci_integrate", "fill_ranges"};
+
+// Host mirror of the reference's FP32 cubic-spline gradient (CubicSpline.h:52-74)
+// with the compiled reference's length; used for the PCISPH scaling factor.
+float hostLen(float x, float y, float z) { return std::sqrt(std::fma(x, x, std::fma(z, z, y * y))); }
+void hostGradW(float h, float lW, const float r[3], float out[3]) {
+    const float rl = hostLen(r[0], r[1], r[2]);
+    const float q = rl / h;
+    out[0] = out[1] = out[2] = 0.0f;
+    if (rl > 1.0e-9f && q <= 1.0f) {
+        float gq[3] = {r[0] / rl, r[1] / rl, r[2] / rl};
+        for (int a = 0; a < 3; ++a) gq[a] /= h;
+        float f;
+        if (q <= 0.5f) f = lW * q * (3.0f * q - 2.0f);
+        else {
+            const float t = 1.0f - q;
+            f = lW * (-t * t);
+        }
+        for (int a = 0; a < 3; ++a) out[a] = f * gq[a];
+    }
+}
+
+struct SceneH {
+    sph_scene_params P{};
+    std::vector<float> xf, vf, xb;  // original order
+    uint32_t nf = 0, nb = 0;
+    // FP32 constants as the reference derives them
+    float r = 0, h = 0, invh = 0, m = 0, rho0 = 0, minTs = 0, maxTs = 0;
+    float dt = 0, t = 0;
+    float kW = 0, lW = 0, laplC = 0, cohC1 = 0, cohC2 = 0, pciDelta = 0;
+    bool viscOn = false, stOn = false, akinci = false;
+    int its = 0, itsV = 0;
+    double pciAvg = 0;
+    // layout
+    uint32_t fBegin = 0, fEnd = 0, fPadEnd = 0, blockBegin = 0, blockEnd = 0, bBegin = 0, bEnd = 0;
+    int g0[3] = {0, 0, 0}, gn[3] = {1, 1, 1};
+    uint32_t cellBase = 0, nCells = 0, keyBase = 0, keyRange = 0;
+    int morton = 0;
+    std::vector<uint32_t> bOrder;  // sorted boundary position -> original index
+};
+
+class Engine {
+   public:
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool ownStream = false;
+    std::string lastError;
+    std::vector<SceneH> scenes;
+    bool initialized = false;
+    int method = -1;
+
+    size_t NP = 0, NB = 0, totalCells = 0;
+    uint32_t numBlocks = 0;
+    int keyBits = 1;
+    uint32_t capF = 64, capB = 48;
+
+    DevBuf<SceneConst> dScenes;
+    DevBuf<SceneDyn> dDyn;
+    DevBuf<int32_t> dBlockScene;
+    DevBuf<float4> x, xAlt, v, vAlt, a, n, xs, ap, xb;
+    DevBuf<float> rho, alpha, p, k, kws, kwsAlt, kvws, kvwsAlt, aux, vb;
+    DevBuf<uint32_t> id, idAlt, keys, keysAlt, vals, valsAlt;
+    DevBuf<uint2> fRange, bRange;
+    DevBuf<uint32_t> nbrF, cntF, nbrB, cntB;
+    DevBuf<double> partials, red;
+    DevBuf<uint32_t> counter;
+    DevBuf<int32_t> flags;
+    DevBuf<unsigned char> sortTemp;
+    size_t sortTempBytes = 0;
+
+    // pinned host staging
+    SceneDyn* hDynRing = nullptr;
+    int dynSlot = 0;
+    static constexpr int kDynRing = 64;
+    double* hRed = nullptr;
+    int32_t* hFlags = nullptr;
+    hipEvent_t evRed = nullptr, evStart = nullptr, evStop = nullptr;
+    std::vector<SceneDyn> dynHost;
+
+    // profiling
+    bool profiling = false;
+    uint64_t profileMask = ~0ull;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> evPool;
+    std::vector<int> evKernel;
+    size_t evUsed = 0;
+    double kernelMs[K_COUNT] = {0};
+    int64_t kernelLaunches[K_COUNT] = {0};
+    double lastStepMs = 0;
+    int64_t launchesInStep = 0;
+
+    ~Engine() {
+        releasePinned();
+        for (auto& e : evPool) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        if (evRed) (void)hipEventDestroy(evRed);
+        if (evStart) (void)hipEventDestroy(evStart);
+        if (evStop) (void)hipEventDestroy(evStop);
+        if (ownStream && stream) (void)hipStreamDestroy(stream);
+    }
+    void releasePinned() {
+        if (hDynRing) (void)hipHostFree(hDynRing);
+        if (hRed) (void)hipHostFree(hRed);
+        if (hFlags) (void)hipHostFree(hFlags);
+        hDynRing = nullptr;
+        hRed = nullptr;
+        hFlags = nullptr;
+    }
+
+    // ------------------------------------------------------------ scenes
+    int addScene(const sph_scene_params& P, int64_t nf, const float* pos, const float* vel, int64_t nb,
+                 const float* bpos) {
+        if (initialized) throw SphError{SPH_E_STATE, "scenes must be added before sph_init"};
+        if (nf < 0 || nb < 0 || (nf > 0 && !pos) || (nb > 0 && !bpos))
+            throw SphError{SPH_E_ARG, "bad particle arrays"};
+        if (nf > (1ll << 30)) throw SphError{SPH_E_ARG, "too many fluid particles"};
+        if (P.simulation_method < 0 || P.simulation_method > 2)
+            throw SphError{SPH_E_ARG, "simulation_method must be 0, 1 or 2"};
+        if (P.boundary_method != SPH_BOUNDARY_AKINCI && !(P.boundary_method == SPH_BOUNDARY_NONE) &&
+            !(nb == 0 && (P.boundary_method == 0)))
+            throw SphError{SPH_E_UNSUPPORTED, "only Akinci boundaries are supported"};
+        if (P.adhesion_method > -1 && P.adhesion != 0.0f)
+            throw SphError{SPH_E_UNSUPPORTED, "adhesion force is not supported"};
+        if (P.viscosity_method > 2 || P.viscosity_method < -1)
+            throw SphError{SPH_E_ARG, "viscosity_method must be -1, 0, 1 or 2"};
+        if (!(P.particle_radius > 0.0f)) throw SphError{SPH_E_ARG, "particle_radius must be > 0"};
+        if (!scenes.empty() && scenes[0].P.simulation_method != P.simulation_method)
+            throw SphError{SPH_E_UNSUPPORTED, "all scenes of a context must use the same simulation method"};
+        SceneH s;
+        s.P = P;
+        s.nf = static_cast<uint32_t>(nf);
+        s.nb = static_cast<uint32_t>(nb);
+        s.xf.assign(pos, pos + 3 * nf);
+        s.vf.assign(3 * nf, 0.0f);
+        if (vel) std::copy(vel, vel + 3 * nf, s.vf.begin());
+        if (nb) s.xb.assign(bpos, bpos + 3 * nb);
+        for (float f : s.xf)
+            if (!std::isfinite(f)) throw SphError{SPH_E_NONFINITE, "non-finite fluid position"};
+        for (float f : s.xb)
+            if (!std::isfinite(f)) throw SphError{SPH_E_NONFINITE, "non-finite boundary position"};
+        derive(s);
+        scenes.push_back(std::move(s));
+        method = P.simulation_method;
+        return static_cast<int>(scenes.size()) - 1;
+    }
+
+    // FP32 constants in the reference's evaluation order
+    static void derive(SceneH& s) {
+        const sph_scene_params& P = s.P;
+        s.r = P.particle_radius;
+        s.h = 4.0f * s.r;  // SPHSimulation.cpp:206
+        s.invh = 1.0f / s.h;
+        s.rho0 = P.density0;
+        const float diam = 2.0f * s.r;
+        s.m = s.rho0 * (diam * diam * diam);  // SPHSimulation.cpp:145-153
+        const float pi = static_cast<float>(M_PI);
+        const float h = s.h;
+        const float h3 = h * h * h;
+        s.kW = 8.0f / (pi * h3);
+        s.lW = 48.0f / (pi * h3);
+        const float h6 = h * h * h * h * h * h;
+        s.laplC = 45.0f / (pi * h6);
+        const float h9 = h * h * h * h * h * h * h * h * h;
+        s.cohC1 = static_cast<float>(32.0 / M_PI) / h9;
+        s.cohC2 = h6 / 64.0f;
+        const float fpsTs = 1.0f / P.fps;  // TimeHelper.h:57
+        s.minTs = fpsTs < P.min_time_step ? fpsTs : P.min_time_step;
+        s.maxTs = P.max_time_step;
+        s.dt = P.time_step;
+        s.t = 0.0f;
+        // SPHSimulation.cpp:390-394
+        s.viscOn = P.viscosity_method > -1 && P.viscosity != 0.0f;
+        s.stOn = P.surface_tension_method == SPH_SURFACE_TENSION_AKINCI && P.surface_tension != 0.0f;
+        s.akinci = P.boundary_method == SPH_BOUNDARY_AKINCI && s.nb > 0;
+        if (P.simulation_method == SPH_PCISPH) {  // PCISPHSolver.cpp:403-436 with the initial dt
+            const float dist = 2.0f * s.r;
+            float sg[3] = {0, 0, 0}, ssq = 0;
+            for (int i = -1; i <= 1; i++)
+                for (int j = -1; j <= 1; j++)
+                    for (int k = -1; k <= 1; k++) {
+                        float rij[3] = {static_cast<float>(-i) * dist, static_cast<float>(-j) * dist,
+                                        static_cast<float>(-k) * dist};
+                        float g[3];
+                        hostGradW(s.h, s.lW, rij, g);
+                        for (int a = 0; a < 3; ++a) sg[a] += g[a];
+                        ssq += (g[0] * g[0] + g[1] * g[1]) + g[2] * g[2];
+                    }
+            const float mdt = s.m * s.dt / s.rho0;
+            const float beta = 2.0f * (mdt * mdt);
+            const float dsg = (sg[0] * sg[0] + sg[1] * sg[1]) + sg[2] * sg[2];
+            s.pciDelta = -1.0f / (beta * (-dsg - ssq));
+        }
+    }
+
+    // ------------------------------------------------------------ layout
+    static void aabb(const std::vector<float>& p, float lo[3], float hi[3]) {
+        for (size_t i = 0; i + 2 < p.size(); i += 3)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], p[i + a]);
+                hi[a] = std::max(hi[a], p[i + a]);
+            }
+    }
+    void setGrid(SceneH& s, const float lo[3], const float hi[3]) {
+        for (int a = 0; a < 3; ++a) {
+            const double clo = std::floor(static_cast<double>(lo[a] * s.invh));
+            const double chi = std::floor(static_cast<double>(hi[a] * s.invh));
+            const double ext = chi - clo + 1;
+            const double margin = 4 + std::ceil(0.1 * ext);
+            const double g0 = clo - margin, gn = ext + 2 * margin;
+            if (g0 < -1.0e9 || g0 + gn > 1.0e9 || gn > (1 << 20))
+                throw SphError{SPH_E_OUT_OF_GRID, "particles spread over too many grid cells"};
+            s.g0[a] = static_cast<int>(g0);
+            s.gn[a] = static_cast<int>(gn);
+        }
+        const double cells = static_cast<double>(s.gn[0]) * s.gn[1] * s.gn[2];
+        if (cells > static_cast<double>(1u << 28)) throw SphError{SPH_E_OUT_OF_GRID, "grid too large (> 2^28 cells)"};
+        s.nCells = static_cast<uint32_t>(cells);
+        const int maxDim = std::max(s.gn[0], std::max(s.gn[1], s.gn[2]));
+        int bits = 0;
+        while ((1 << bits) < maxDim) ++bits;
+        if (bits <= 10 && 3 * bits <= 30 && (1u << (3 * bits)) <= 4 * s.nCells + 1024) {
+            s.morton = 1;
+            s.keyRange = (1u << (3 * bits)) + 1;
+        } else {
+            s.morton = 0;
+            s.keyRange = s.nCells + 1;
+        }
+    }
+    void cellOfHost(const SceneH& s, const float* p, int c[3]) const {
+        for (int a = 0; a < 3; ++a) c[a] = static_cast<int>(std::floor(p[a] * s.invh)) - s.g0[a];
+    }
+    static uint32_t spread3h(uint32_t v) {
+        v &= 0x3ffu;
+        v = (v | (v << 16)) & 0x030000ffu;
+        v = (v | (v << 8)) & 0x0300f00fu;
+        v = (v | (v << 4)) & 0x030c30c3u;
+        v = (v | (v << 2)) & 0x09249249u;
+        return v;
+    }
+
+    // Assign cell/key bases, upload scene constants and the boundary grid.
+    void uploadLayout() {
+        uint64_t cells = 0, keysTot = 0;
+        for (auto& s : scenes) {
+            s.cellBase = static_cast<uint32_t>(cells);
+            s.keyBase = static_cast<uint32_t>(keysTot);
+            cells += s.nCells;
+            keysTot += s.keyRange;
+            if (cells > (1ull << 30) || keysTot > (1ull << 32) - 1)
+                throw SphError{SPH_E_OUT_OF_GRID, "combined grids too large"};
+        }
+        totalCells = cells;
+        keyBits = 1;
+        while (keyBits < 32 && (1ull << keyBits) < keysTot) ++keyBits;
+        std::vector<SceneConst> sc(scenes.size());
+        for (size_t i = 0; i < scenes.size(); ++i) {
+            const SceneH& s = scenes[i];
+            SceneConst& c = sc[i];
+            std::memset(&c, 0, sizeof(c));
+            c.h = s.h;
+            c.invh = s.invh;
+            c.g0x = s.g0[0]; c.g0y = s.g0[1]; c.g0z = s.g0[2];
+            c.gnx = s.gn[0]; c.gny = s.gn[1]; c.gnz = s.gn[2];
+            c.cellBase = s.cellBase;
+            c.keyBase = s.keyBase;
+            c.deadKey = s.keyBase + s.keyRange - 1;
+            c.morton = s.morton;
+            c.fBegin = s.fBegin; c.fEnd = s.fEnd; c.fPadEnd = s.fPadEnd;
+            c.blockBegin = s.blockBegin; c.blockEnd = s.blockEnd;
+            c.bBegin = s.bBegin; c.bEnd = s.bEnd;
+            c.m = s.m; c.rho0 = s.rho0;
+            c.gx = s.P.gravity[0]; c.gy = s.P.gravity[1]; c.gz = s.P.gravity[2];
+            c.kW = s.kW; c.lW = s.lW; c.laplC = s.laplC; c.cohC1 = s.cohC1; c.cohC2 = s.cohC2;
+            c.visc = s.P.viscosity; c.bvisc = s.P.boundary_viscosity; c.st = s.P.surface_tension;
+            c.viscMethod = s.viscOn ? s.P.viscosity_method : -1;
+            c.stOn = s.stOn ? 1 : 0;
+            c.akinci = s.akinci ? 1 : 0;
+            c.bviscOn = (s.viscOn && s.P.boundary_viscosity != 0.0f && s.akinci) ? 1 : 0;
+            c.gamma = s.P.gamma;
+            c.wcB = s.P.gamma != 0.0f ? s.P.stiffness * s.rho0 / s.P.gamma : 0.0f;
+            c.pciDelta = s.pciDelta;
+            c.eps = 1.0e-5f;
+            c.h2c = 0.01f * (s.h * s.h);
+        }
+        dScenes.alloc(scenes.size());
+        HIPCHK(hipMemcpy(dScenes.p, sc.data(), sc.size() * sizeof(SceneConst), hipMemcpyHostToDevice));
+        // boundary grid: sort boundary particles of each scene by cell key on the host
+        std::vector<uint2> br(totalCells ? totalCells : 1, make_uint2(0u, 0u));
+        std::vector<float4> xbs(NB ? NB : 1);
+        for (auto& s : scenes) {
+            s.bOrder.resize(s.nb);
+            std::vector<uint32_t> key(s.nb), lin(s.nb);
+            for (uint32_t b = 0; b < s.nb; ++b) {
+                int c[3];
+                cellOfHost(s, &s.xb[3 * b], c);
+                for (int a = 0; a < 3; ++a)
+                    if (c[a] < 0 || c[a] >= s.gn[a]) throw SphError{SPH_E_OUT_OF_GRID, "boundary particle outside grid"};
+                lin[b] = static_cast<uint32_t>((c[2] * s.gn[1] + c[1]) * s.gn[0] + c[0]);
+                key[b] = s.morton ? (spread3h(c[0]) | (spread3h(c[1]) << 1) | (spread3h(c[2]) << 2)) : lin[b];
+            }
+            std::iota(s.bOrder.begin(), s.bOrder.end(), 0u);
+            std::stable_sort(s.bOrder.begin(), s.bOrder.end(), [&](uint32_t A, uint32_t B) { return key[A] < key[B]; });
+            for (uint32_t q = 0; q < s.nb; ++q) {
+                const uint32_t o = s.bOrder[q];
+                const uint32_t g = s.bBegin + q;
+                xbs[g] = make_float4(s.xb[3 * o], s.xb[3 * o + 1], s.xb[3 * o + 2], 0.0f);
+                uint2& r = br[s.cellBase + lin[o]];
+                if (r.y == 0) r.x = g;
+                r.y = g + 1;
+            }
+        }
+        bRange.alloc(br.size());
+        HIPCHK(hipMemcpy(bRange.p, br.data(), br.size() * sizeof(uint2), hipMemcpyHostToDevice));
+        xb.alloc(xbs.size());
+        HIPCHK(hipMemcpy(xb.p, xbs.data(), xbs.size() * sizeof(float4), hipMemcpyHostToDevice));
+        vb.alloc(xbs.size());
+        fRange.alloc(totalCells ? totalCells : 1);
+        // Akinci volumes (AkinciBoundaryObject::computeVolume), once per layout
+        DevView d = view();
+        for (size_t i = 0; i < scenes.size(); ++i)
+            if (scenes[i].akinci) HIPCHK(launchBoundaryVolume(d, static_cast<int>(i), scenes[i].bBegin, scenes[i].bEnd, stream));
+    }
+
+    void allocNeighborLists() {
+        nbrF.alloc(static_cast<size_t>(capF) * NP);
+        nbrB.alloc(static_cast<size_t>(capB) * NP);
+    }
+
+    DevView view() {
+        DevView d{};
+        d.scenes = dScenes.p;
+        d.dyn = dDyn.p;
+        d.blockScene = dBlockScene.p;
+        d.numScenes = static_cast<int32_t>(scenes.size());
+        d.numBlocks = numBlocks;
+        d.NP = static_cast<uint32_t>(NP);
+        d.x = x.p; d.v = v.p; d.a = a.p; d.n = n.p; d.xs = xs.p; d.ap = ap.p;
+        d.rho = rho.p; d.alpha = alpha.p; d.p = p.p; d.k = k.p; d.kws = kws.p; d.kvws = kvws.p; d.aux = aux.p;
+        d.id = id.p;
+        d.xb = xb.p; d.vb = vb.p;
+        d.fRange = fRange.p; d.bRange = bRange.p;
+        d.nbrF = nbrF.p; d.cntF = cntF.p; d.nbrB = nbrB.p; d.cntB = cntB.p;
+        d.capF = capF; d.capB = capB;
+        d.partials = partials.p; d.counter = counter.p; d.red = red.p; d.flags = flags.p;
+        return d;
+    }
+
+    // ------------------------------------------------------------ init
+    void init() {
+        if (initialized) throw SphError{SPH_E_STATE, "sph_init called twice"};
+        if (scenes.empty()) throw SphError{SPH_E_STATE, "no scene added"};
+        HIPCHK(hipSetDevice(device));
+        if (!stream) {
+            HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+            ownStream = true;
+        }
+        HIPCHK(hipEventCreateWithFlags(&evRed, hipEventDisableTiming));
+        HIPCHK(hipEventCreate(&evStart));
+        HIPCHK(hipEventCreate(&evStop));
+        // slots
+        uint32_t off = 0, boff = 0;
+        for (auto& s : scenes) {
+            s.fBegin = off;
+            s.fEnd = off + s.nf;
+            s.fPadEnd = off + ((s.nf + kBlock - 1) / kBlock) * kBlock;
+            s.blockBegin = s.fBegin / kBlock;
+            s.blockEnd = s.fPadEnd / kBlock;
+            off = s.fPadEnd;
+            s.bBegin = boff;
+            s.bEnd = boff + s.nb;
+            boff += s.nb;
+            float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+            aabb(s.xf, lo, hi);
+            aabb(s.xb, lo, hi);
+            if (s.nf == 0 && s.nb == 0) { lo[0] = lo[1] = lo[2] = 0; hi[0] = hi[1] = hi[2] = 0; }
+            setGrid(s, lo, hi);
+        }
+        NP = off;
+        NB = boff;
+        numBlocks = static_cast<uint32_t>(NP / kBlock);
+        const size_t np = NP ? NP : 1;
+        dDyn.alloc(scenes.size());
+        std::vector<int32_t> bs(numBlocks ? numBlocks : 1, 0);
+        for (size_t i = 0; i < scenes.size(); ++i)
+            for (uint32_t b = scenes[i].blockBegin; b < scenes[i].blockEnd; ++b) bs[b] = static_cast<int32_t>(i);
+        dBlockScene.alloc(bs.size());
+        HIPCHK(hipMemcpy(dBlockScene.p, bs.data(), bs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        for (auto* b : {&x, &xAlt, &v, &vAlt, &a, &n, &xs, &ap}) b->alloc(np);
+        for (auto* b : {&rho, &alpha, &p, &k, &kws, &kwsAlt, &kvws, &kvwsAlt, &aux}) b->alloc(np);
+        for (auto* b : {&id, &idAlt, &keys, &keysAlt, &vals, &valsAlt, &cntF, &cntB}) b->alloc(np);
+        allocNeighborLists();
+        partials.alloc(2 * static_cast<size_t>(numBlocks ? numBlocks : 1));
+        red.alloc(scenes.size() * kRedSlots);
+        counter.alloc(1);
+        flags.alloc(scenes.size() * kFlagSlots);
+        HIPCHK(hipMemset(counter.p, 0, sizeof(uint32_t)));
+        HIPCHK(hipMemset(a.p, 0, np * sizeof(float4)));
+        HIPCHK(hipMemset(n.p, 0, np * sizeof(float4)));
+        for (auto* b : {&rho, &alpha, &p, &k, &kws, &kvws, &aux}) HIPCHK(hipMemset(b->p, 0, np * sizeof(float)));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&hDynRing), kDynRing * scenes.size() * sizeof(SceneDyn)));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&hRed), scenes.size() * kRedSlots * sizeof(double)));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&hFlags), scenes.size() * kFlagSlots * sizeof(int32_t)));
+        dynHost.assign(scenes.size(), SceneDyn{});
+        for (size_t i = 0; i < scenes.size(); ++i) {
+            dynHost[i].dt = scenes[i].dt;
+            dynHost[i].dt2 = scenes[i].dt * scenes[i].dt;
+            dynHost[i].active = 1;
+        }
+        uploadState();
+        // sort keys temp storage
+        sortTempBytes = 0;
+        HIPCHK(sortPairs(nullptr, sortTempBytes, keys.p, keysAlt.p, vals.p, valsAlt.p, static_cast<int>(NP), 32, stream,
+                         nullptr));
+        sortTemp.alloc(sortTempBytes + 256);
+        uploadLayout();
+        initialized = true;
+        uploadDyn();
+        // solver init passes
+        if (method == SPH_DFSPH && NP) {  // DFSPHSolver::init (DFSPHSolver.cpp:22-31)
+            neighborSearch(false);
+            timed(K_DENSITY, [&] { return launchDensityAlpha(view(), 1, stream); });
+        }
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    void uploadState() {
+        const size_t np = NP ? NP : 1;
+        std::vector<float4> hx(np, make_float4(0, 0, 0, 0)), hv(np, make_float4(0, 0, 0, 0));
+        std::vector<uint32_t> hid(np, 0);
+        for (auto& s : scenes)
+            for (uint32_t i = 0; i < s.nf; ++i) {
+                const uint32_t slot = s.fBegin + i;
+                hx[slot] = make_float4(s.xf[3 * i], s.xf[3 * i + 1], s.xf[3 * i + 2], 0.0f);
+                hv[slot] = make_float4(s.vf[3 * i], s.vf[3 * i + 1], s.vf[3 * i + 2], 0.0f);
+                hid[slot] = i;
+            }
+        // padding slots: park far away from nothing (never read as neighbours)
+        HIPCHK(hipMemcpy(x.p, hx.data(), np * sizeof(float4), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(v.p, hv.data(), np * sizeof(float4), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(id.p, hid.data(), np * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+
+    void uploadDyn() {
+        SceneDyn* slot = hDynRing + static_cast<size_t>(dynSlot) * scenes.size();
+        if (dynSlot == kDynRing - 1) HIPCHK(hipStreamSynchronize(stream));  // every ring slot is free after this
+        std::memcpy(slot, dynHost.data(), scenes.size() * sizeof(SceneDyn));
+        HIPCHK(hipMemcpyAsync(dDyn.p, slot, scenes.size() * sizeof(SceneDyn), hipMemcpyHostToDevice, stream));
+        dynSlot = (dynSlot + 1) % kDynRing;
+    }
+    void setActiveAll(bool on) {
+        for (auto& d : dynHost) d.active = on ? 1 : 0;
+    }
+
+    template <class F>
+    void timed(KernelId kid, F&& launch) {
+        ++launchesInStep;
+        if (!profiling || !((profileMask >> kid) & 1ull)) {
+            HIPCHK(launch());
+            return;
+        }
+        if (evUsed == evPool.size()) {
+            hipEvent_t e0, e1;
+            HIPCHK(hipEventCreate(&e0));
+            HIPCHK(hipEventCreate(&e1));
+            evPool.emplace_back(e0, e1);
+            evKernel.push_back(0);
+        }
+        auto& pr = evPool[evUsed];
+        evKernel[evUsed] = kid;
+        ++evUsed;
+        HIPCHK(hipEventRecord(pr.first, stream));
+        HIPCHK(launch());
+        HIPCHK(hipEventRecord(pr.second, stream));
+    }
+    void collectProfile() {
+        if (!evUsed) return;
+        HIPCHK(hipStreamSynchronize(stream));
+        for (size_t i = 0; i < evUsed; ++i) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, evPool[i].first, evPool[i].second));
+            kernelMs[evKernel[i]] += ms;
+            kernelLaunches[evKernel[i]] += 1;
+        }
+        evUsed = 0;
+    }
+
+    void readReductions() {
+        HIPCHK(hipMemcpyAsync(hRed, red.p, scenes.size() * kRedSlots * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipEventRecord(evRed, stream));
+    }
+    void waitReductions() { HIPCHK(hipEventSynchronize(evRed)); }
+
+    // ------------------------------------------------------------ neighbour search
+    // keys -> sort -> reorder (+ cell ranges) -> lists. dfsphAdvance fuses the
+    // DFSPH position update. Handles grid growth and list capacity.
+    void neighborSearch(bool dfsphAdvance) {
+        if (!NP) return;
+        for (int attempt = 0; attempt < 4; ++attempt) {
+            HIPCHK(hipMemsetAsync(flags.p, 0, scenes.size() * kFlagSlots * sizeof(int32_t), stream));
+            DevView d = view();
+            timed(K_KEYS, [&] { return launchKeys(d, (dfsphAdvance && attempt == 0) ? 1 : 0, keys.p, vals.p, stream); });
+            timed(K_FILL, [&] { return launchFillRanges(fRange.p, totalCells, stream); });
+            bool swapped = false;
+            timed(K_SORT, [&] {
+                return sortPairs(sortTemp.p, sortTempBytes, keys.p, keysAlt.p, vals.p, valsAlt.p, static_cast<int>(NP),
+                                 keyBits, stream, &swapped);
+            });
+            const uint32_t* ks = swapped ? keysAlt.p : keys.p;
+            const uint32_t* vs = swapped ? valsAlt.p : vals.p;
+            timed(K_REORDER, [&] { return launchReorder(d, ks, vs, xAlt.p, vAlt.p, kwsAlt.p, kvwsAlt.p, idAlt.p, stream); });
+            std::swap(x.p, xAlt.p);
+            std::swap(v.p, vAlt.p);
+            std::swap(kws.p, kwsAlt.p);
+            std::swap(kvws.p, kvwsAlt.p);
+            std::swap(id.p, idAlt.p);
+            d = view();
+            timed(K_BUILD, [&] { return launchBuildNeighbors(d, stream); });
+            HIPCHK(hipMemcpyAsync(hFlags, flags.p, scenes.size() * kFlagSlots * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                  stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            bool outOfGrid = false;
+            int maxF = 0, maxB = 0;
+            for (size_t s = 0; s < scenes.size(); ++s) {
+                outOfGrid |= hFlags[s * kFlagSlots + kFlagOutOfGrid] != 0;
+                maxF = std::max(maxF, hFlags[s * kFlagSlots + kFlagMaxFluidNbr]);
+                maxB = std::max(maxB, hFlags[s * kFlagSlots + kFlagMaxBoundaryNbr]);
+            }
+            if (outOfGrid) {
+                regrid();
+                continue;
+            }
+            if (static_cast<uint32_t>(maxF) > capF || static_cast<uint32_t>(maxB) > capB) {
+                if (static_cast<uint32_t>(maxF) > capF) capF = ((static_cast<uint32_t>(maxF) * 5 / 4) + 7) & ~7u;
+                if (static_cast<uint32_t>(maxB) > capB) capB = ((static_cast<uint32_t>(maxB) * 5 / 4) + 7) & ~7u;
+                allocNeighborLists();
+                timed(K_BUILD, [&] { return launchBuildNeighbors(view(), stream); });
+                HIPCHK(hipStreamSynchronize(stream));
+            }
+            return;
+        }
+        throw SphError{SPH_E_OUT_OF_GRID, "could not place particles into a grid"};
+    }
+
+    // Grow the grids to the current fluid extents (rare path).
+    void regrid() {
+        std::vector<float4> hx(NP);
+        HIPCHK(hipMemcpy(hx.data(), x.p, NP * sizeof(float4), hipMemcpyDeviceToHost));
+        for (auto& s : scenes) {
+            float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+            for (uint32_t i = s.fBegin; i < s.fEnd; ++i) {
+                const float4 q = hx[i];
+                if (!std::isfinite(q.x) || !std::isfinite(q.y) || !std::isfinite(q.z))
+                    throw SphError{SPH_E_NONFINITE, "non-finite particle position"};
+                lo[0] = std::min(lo[0], q.x); lo[1] = std::min(lo[1], q.y); lo[2] = std::min(lo[2], q.z);
+                hi[0] = std::max(hi[0], q.x); hi[1] = std::max(hi[1], q.y); hi[2] = std::max(hi[2], q.z);
+            }
+            aabb(s.xb, lo, hi);
+            if (s.nf == 0 && s.nb == 0) continue;
+            setGrid(s, lo, hi);
+        }
+        uploadLayout();
+    }
+
+    // ------------------------------------------------------------ steps
+    void stepAll(int nsteps, sph_step_stats* stats) {
+        if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
+        HIPCHK(hipSetDevice(device));
+        launchesInStep = 0;
+        HIPCHK(hipEventRecord(evStart, stream));
+        for (int st = 0; st < nsteps; ++st) {
+            if (method == SPH_DFSPH) stepDFSPH();
+            else if (method == SPH_PCISPH) stepPCISPH();
+            else stepWCSPH();
+            if (stats)
+                for (size_t s = 0; s < scenes.size(); ++s) stats[st * scenes.size() + s] = lastStats[s];
+        }
+        HIPCHK(hipEventRecord(evStop, stream));
+        HIPCHK(hipEventSynchronize(evStop));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, evStart, evStop));
+        lastStepMs = ms;
+        collectProfile();
+    }
+    std::vector<sph_step_stats> lastStats;
+
+    bool anyVisc() const {
+        for (auto& s : scenes) if (s.viscOn) return true;
+        return false;
+    }
+    bool anyST() const {
+        for (auto& s : scenes) if (s.stOn) return true;
+        return false;
+    }
+
+    void beginStats() {
+        lastStats.assign(scenes.size(), sph_step_stats{});
+        for (size_t s = 0; s < scenes.size(); ++s) lastStats[s].scene = static_cast<int32_t>(s);
+    }
+    void endStats() {
+        for (size_t s = 0; s < scenes.size(); ++s) {
+            SceneH& sc = scenes[s];
+            sc.t = sc.t + sc.dt;  // SPHSimulation::setTime(getTime() + getTimeStep())
+            lastStats[s].t = sc.t;
+            lastStats[s].dt = sc.dt;
+        }
+    }
+    void syncDt() {
+        for (size_t s = 0; s < scenes.size(); ++s) {
+            dynHost[s].dt = scenes[s].dt;
+            dynHost[s].dt2 = scenes[s].dt * scenes[s].dt;
+        }
+    }
+
+    // Jacobi loop of one DFSPH solve, per scene (DFSPHSolver.cpp:429-514 / :744-827).
+    // density: pass A = kRhoStar, else kDivergence.
+    void dfsphLoop(bool density) {
+        const size_t S = scenes.size();
+        std::vector<int> its(S, 0);
+        std::vector<double> err(S, 0.0);
+        DevView d;
+        while (true) {
+            bool any = false;
+            for (size_t s = 0; s < S; ++s) {
+                const SceneH& sc = scenes[s];
+                bool act;
+                if (density)
+                    act = (err[s] > static_cast<double>(sc.P.eta) || its[s] < static_cast<int>(sc.P.min_iterations)) &&
+                          its[s] < static_cast<int>(sc.P.max_iterations);
+                else  // bound is maxIterations, not maxIterationsV (:746)
+                    act = (err[s] > static_cast<double>(sc.P.eta_v / sc.dt) ||
+                           its[s] < static_cast<int>(sc.P.min_iterations_v)) &&
+                          its[s] < static_cast<int>(sc.P.max_iterations);
+                dynHost[s].active = act ? 1 : 0;
+                dynHost[s].first = its[s] == 0 ? 1 : 0;
+                any |= act;
+            }
+            if (!any || !NP) break;
+            uploadDyn();
+            d = view();
+            if (density) timed(K_RHO_STAR, [&] { return launchRhoStar(d, 0, stream); });
+            else timed(K_DIVERGENCE, [&] { return launchDivergence(d, 0, stream); });
+            readReductions();
+            timed(K_VELOCITY, [&] { return launchVelocity(d, k.p, stream); });
+            waitReductions();
+            for (size_t s = 0; s < S; ++s)
+                if (dynHost[s].active) {
+                    err[s] = scenes[s].nf ? hRed[s * kRedSlots + kRedSum] / scenes[s].nf : 0.0;
+                    its[s]++;
+                }
+        }
+        for (size_t s = 0; s < S; ++s) {
+            const SceneH& sc = scenes[s];
+            if (density) {
+                scenes[s].its = its[s];
+                lastStats[s].iterations = its[s];
+                lastStats[s].avg_density_error = err[s];
+                lastStats[s].capped_density = its[s] >= static_cast<int>(sc.P.max_iterations) &&
+                                              err[s] > static_cast<double>(sc.P.eta);
+            } else {
+                scenes[s].itsV = its[s];
+                lastStats[s].iterations_v = its[s];
+                lastStats[s].avg_divergence_error = err[s];
+                lastStats[s].capped_divergence = its[s] >= static_cast<int>(sc.P.max_iterations) &&
+                                                 err[s] > static_cast<double>(sc.P.eta_v / sc.dt);
+            }
+        }
+    }
+
+    void stepDFSPH() {
+        beginStats();
+        const size_t S = scenes.size();
+        // 1. non-pressure forces (SPHSimulation::computeNonPressureForces) with the current dt
+        syncDt();
+        setActiveAll(true);
+        uploadDyn();
+        DevView d = view();
+        if (NP) {
+            if (anyVisc() || anyST()) timed(K_FORCES, [&] { return launchForces(d, 0, stream); });
+            if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 1, stream); });
+            else timed(K_VMAX, [&] { return launchVmax(d, stream); });
+            readReductions();
+            waitReductions();
+        }
+        // 2. updateTimeStep (DFSPHSolver.cpp:1090-1162) in FP32 on the FP64 max
+        for (size_t s = 0; s < S; ++s) {
+            SceneH& sc = scenes[s];
+            const double vmaxD = sc.nf ? hRed[s * kRedSlots + kRedMax] : 0.0;
+            lastStats[s].max_velocity = vmaxD;
+            if (!sc.P.adaptive_time_step) continue;
+            float maxVel = static_cast<float>(vmaxD);
+            if (maxVel < 1.0e-9f) maxVel = 1.0e-9f;
+            float hh = sc.dt;
+            const float cflTs = sc.P.cfl_factor * 0.4f * (2.0f * sc.r) / maxVel;
+            const int itsMax = std::max(sc.its, sc.itsV);
+            if (itsMax > 6) hh *= 0.9f;
+            else if (itsMax < 4) hh *= 1.05f;
+            hh = std::min(hh, cflTs);
+            hh = std::max(hh, sc.minTs);
+            hh = std::min(hh, sc.maxTs);
+            sc.dt = hh;
+        }
+        syncDt();
+        uploadDyn();
+        d = view();
+        if (NP) {
+            // 3. predictVelocities
+            timed(K_PREDICT, [&] { return launchPredictVelocity(d, stream); });
+            // 4. correctDensityError with densityWarmStart
+            timed(K_RHO_STAR, [&] { return launchRhoStar(d, 1, stream); });
+            timed(K_VELOCITY, [&] { return launchVelocity(d, kws.p, stream); });
+        }
+        dfsphLoop(true);
+        // 5. updatePositions fused with the keys; k_ws *= ts^2 (or cleared if the loop never ran)
+        for (size_t s = 0; s < S; ++s) {
+            dynHost[s].active = 1;
+            dynHost[s].scale = scenes[s].its > 0 ? dynHost[s].dt2 : 0.0f;
+        }
+        uploadDyn();
+        // 6-7. neighbour search
+        neighborSearch(true);
+        d = view();
+        if (NP) {
+            // 8-9. computeDensities + computeAlpha
+            timed(K_DENSITY, [&] { return launchDensityAlpha(d, 1, stream); });
+            // 10. correctDivergenceError with divergenceWarmStart
+            timed(K_DIVERGENCE, [&] { return launchDivergence(d, 1, stream); });
+            timed(K_VELOCITY, [&] { return launchVelocity(d, kvws.p, stream); });
+        }
+        dfsphLoop(false);
+        for (size_t s = 0; s < S; ++s) {
+            dynHost[s].active = 1;
+            dynHost[s].scale = scenes[s].itsV > 0 ? dynHost[s].dt : 0.0f;
+        }
+        uploadDyn();
+        if (NP) timed(K_SCALE, [&] { return launchScale(view(), kvws.p, stream); });
+        endStats();
+    }
+
+    void stepWCSPH() {
+        beginStats();
+        syncDt();
+        setActiveAll(true);
+        uploadDyn();
+        neighborSearch(false);
+        DevView d = view();
+        if (NP) {
+            timed(K_DENSITY, [&] { return launchDensityPressure(d, stream); });
+            timed(K_FORCES, [&] { return launchForces(d, 1, stream); });
+            if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
+            timed(K_INTEGRATE, [&] { return launchIntegrate(d, stream); });
+        }
+        endStats();
+    }
+
+    void stepPCISPH() {
+        beginStats();
+        const size_t S = scenes.size();
+        syncDt();
+        setActiveAll(true);
+        uploadDyn();
+        neighborSearch(false);
+        DevView d = view();
+        if (NP) {
+            timed(K_DENSITY, [&] { return launchDensityAlpha(d, 0, stream); });
+            if (anyVisc() || anyST()) timed(K_FORCES, [&] { return launchForces(d, 0, stream); });
+            if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
+            timed(K_PCI_INIT, [&] { return launchPciInit(d, stream); });
+        }
+        std::vector<int> its(S, 0);
+        while (true) {  // PCISPHSolver::pressureSolver (:369-401); the error is carried across steps
+            bool any = false;
+            for (size_t s = 0; s < S; ++s) {
+                const SceneH& sc = scenes[s];
+                const bool act = (sc.pciAvg > static_cast<double>(sc.P.eta) ||
+                                  its[s] < static_cast<int>(sc.P.min_iterations)) &&
+                                 its[s] < static_cast<int>(sc.P.max_iterations);
+                dynHost[s].active = act ? 1 : 0;
+                any |= act;
+            }
+            if (!any || !NP) break;
+            uploadDyn();
+            d = view();
+            timed(K_PCI_DENSITY, [&] { return launchPciDensity(d, stream); });
+            readReductions();
+            timed(K_PCI_PACC, [&] { return launchPciPressureAcc(d, stream); });
+            waitReductions();
+            for (size_t s = 0; s < S; ++s)
+                if (dynHost[s].active) {
+                    scenes[s].pciAvg = scenes[s].nf ? hRed[s * kRedSlots + kRedSum] / scenes[s].nf : 0.0;
+                    lastStats[s].max_velocity = 0;
+                    its[s]++;
+                }
+        }
+        for (size_t s = 0; s < S; ++s) {
+            scenes[s].its = its[s];
+            lastStats[s].iterations = its[s];
+            lastStats[s].avg_density_error = scenes[s].pciAvg;
+            lastStats[s].capped_density = its[s] >= static_cast<int>(scenes[s].P.max_iterations) &&
+                                          scenes[s].pciAvg > static_cast<double>(scenes[s].P.eta);
+        }
+        setActiveAll(true);
+        uploadDyn();
+        if (NP) timed(K_PCI_INTEGRATE, [&] { return launchPciIntegrate(view(), stream); });
+        endStats();
+    }
+
+    // ------------------------------------------------------------ I/O
+    template <class T>
+    std::vector<T> download(const T* src, size_t count) {
+        std::vector<T> out(count);
+        HIPCHK(hipStreamSynchronize(stream));
+        if (count) HIPCHK(hipMemcpy(out.data(), src, count * sizeof(T), hipMemcpyDeviceToHost));
+        return out;
+    }
+    SceneH& scene(int s) {
+        if (s < 0 || static_cast<size_t>(s) >= scenes.size()) throw SphError{SPH_E_ARG, "bad scene index"};
+        return scenes[s];
+    }
+
+    void getState(int si, float* pos, float* vel, float* dens, float* pres) {
+        SceneH& s = scene(si);
+        if (!initialized) {
+            if (pos) std::copy(s.xf.begin(), s.xf.end(), pos);
+            if (vel) std::copy(s.vf.begin(), s.vf.end(), vel);
+            if (dens) std::fill(dens, dens + s.nf, 0.0f);
+            if (pres) std::fill(pres, pres + s.nf, 0.0f);
+            return;
+        }
+        auto ids = download(id.p + s.fBegin, s.nf);
+        if (pos || vel) {
+            auto hx = pos ? download(x.p + s.fBegin, s.nf) : std::vector<float4>();
+            auto hv = vel ? download(v.p + s.fBegin, s.nf) : std::vector<float4>();
+            for (uint32_t i = 0; i < s.nf; ++i) {
+                const uint32_t o = ids[i];
+                if (o >= s.nf) throw SphError{SPH_E_STATE, "corrupt particle ids"};
+                if (pos) { pos[3 * o] = hx[i].x; pos[3 * o + 1] = hx[i].y; pos[3 * o + 2] = hx[i].z; }
+                if (vel) { vel[3 * o] = hv[i].x; vel[3 * o + 1] = hv[i].y; vel[3 * o + 2] = hv[i].z; }
+            }
+        }
+        if (dens) {
+            auto hr = download(rho.p + s.fBegin, s.nf);
+            for (uint32_t i = 0; i < s.nf; ++i) dens[ids[i]] = hr[i];
+        }
+        if (pres) {
+            auto hp = download(p.p + s.fBegin, s.nf);
+            for (uint32_t i = 0; i < s.nf; ++i) pres[ids[i]] = hp[i];
+        }
+    }
+
+    void getField(int si, int field, float* out) {
+        SceneH& s = scene(si);
+        if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
+        auto ids = download(id.p + s.fBegin, s.nf);
+        if (field == 1 || field == 5) {
+            auto hv = download((field == 1 ? a.p : n.p) + s.fBegin, s.nf);
+            for (uint32_t i = 0; i < s.nf; ++i) {
+                out[3 * ids[i]] = hv[i].x; out[3 * ids[i] + 1] = hv[i].y; out[3 * ids[i] + 2] = hv[i].z;
+            }
+            return;
+        }
+        const float* src = field == 0 ? alpha.p : field == 2 ? aux.p : field == 3 ? kws.p : field == 4 ? kvws.p : nullptr;
+        if (!src) throw SphError{SPH_E_ARG, "unknown field"};
+        auto hv = download(src + s.fBegin, s.nf);
+        for (uint32_t i = 0; i < s.nf; ++i) out[ids[i]] = hv[i];
+    }
+
+    void setState(int si, int64_t nn, const float* pos, const float* vel) {
+        SceneH& s = scene(si);
+        if (nn != static_cast<int64_t>(s.nf)) throw SphError{SPH_E_ARG, "particle count mismatch"};
+        if (pos)
+            for (int64_t i = 0; i < 3 * nn; ++i)
+                if (!std::isfinite(pos[i])) throw SphError{SPH_E_NONFINITE, "non-finite position"};
+        if (!initialized) {
+            if (pos) std::copy(pos, pos + 3 * nn, s.xf.begin());
+            if (vel) std::copy(vel, vel + 3 * nn, s.vf.begin());
+            return;
+        }
+        auto ids = download(id.p + s.fBegin, s.nf);
+        auto hx = download(x.p + s.fBegin, s.nf);
+        auto hv = download(v.p + s.fBegin, s.nf);
+        for (uint32_t i = 0; i < s.nf; ++i) {
+            const uint32_t o = ids[i];
+            if (pos) hx[i] = make_float4(pos[3 * o], pos[3 * o + 1], pos[3 * o + 2], 0.0f);
+            if (vel) hv[i] = make_float4(vel[3 * o], vel[3 * o + 1], vel[3 * o + 2], 0.0f);
+        }
+        HIPCHK(hipMemcpy(x.p + s.fBegin, hx.data(), s.nf * sizeof(float4), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(v.p + s.fBegin, hv.data(), s.nf * sizeof(float4), hipMemcpyHostToDevice));
+        if (method == SPH_DFSPH) {  // density and alpha belong to the positions
+            setActiveAll(true);
+            uploadDyn();
+            neighborSearch(false);
+            HIPCHK(launchDensityAlpha(view(), 1, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+        }
+    }
+
+    void dumpCells(int si, int32_t* cells) {
+        SceneH& s = scene(si);
+        if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
+        DevBuf<int32_t> dc;
+        dc.alloc(3 * (NP ? NP : 1));
+        HIPCHK(launchCellsOut(view(), dc.p, stream));
+        auto hc = download(dc.p + 3 * static_cast<size_t>(s.fBegin), 3 * static_cast<size_t>(s.nf));
+        auto ids = download(id.p + s.fBegin, s.nf);
+        for (uint32_t i = 0; i < s.nf; ++i)
+            for (int a3 = 0; a3 < 3; ++a3) cells[3 * ids[i] + a3] = hc[3 * i + a3];
+    }
+
+    int dumpNeighbors(int si, int64_t* off, int32_t* ids_, int64_t cap, int64_t* boff, int32_t* bids, int64_t bcap) {
+        SceneH& s = scene(si);
+        if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
+        setActiveAll(true);
+        uploadDyn();
+        neighborSearch(false);
+        auto ids = download(id.p, NP);
+        auto cf = download(cntF.p + s.fBegin, s.nf);
+        auto cb = download(cntB.p + s.fBegin, s.nf);
+        auto lf = download(nbrF.p, static_cast<size_t>(capF) * NP);
+        auto lb = download(nbrB.p, static_cast<size_t>(capB) * NP);
+        std::vector<uint32_t> slotOf(s.nf);
+        for (uint32_t i = 0; i < s.nf; ++i) slotOf[ids[s.fBegin + i]] = s.fBegin + i;
+        off[0] = 0;
+        boff[0] = 0;
+        for (uint32_t o = 0; o < s.nf; ++o) {
+            const uint32_t sl = slotOf[o] - s.fBegin;
+            off[o + 1] = off[o] + cf[sl];
+            boff[o + 1] = boff[o] + cb[sl];
+        }
+        if (off[s.nf] > cap || boff[s.nf] > bcap) return SPH_E_CAPACITY;
+        for (uint32_t o = 0; o < s.nf; ++o) {
+            const uint32_t slot = slotOf[o];
+            const uint32_t sl = slot - s.fBegin;
+            for (uint32_t q = 0; q < cf[sl]; ++q) ids_[off[o] + q] = static_cast<int32_t>(ids[lf[static_cast<size_t>(q) * NP + slot]]);
+            for (uint32_t q = 0; q < cb[sl]; ++q)
+                bids[boff[o] + q] = static_cast<int32_t>(s.bOrder[lb[static_cast<size_t>(q) * NP + slot] - s.bBegin]);
+        }
+        return SPH_OK;
+    }
+
+    void getBoundary(int si, float* pos, float* vol) {
+        SceneH& s = scene(si);
+        if (pos) std::copy(s.xb.begin(), s.xb.end(), pos);
+        if (vol) {
+            if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
+            auto hv = download(vb.p + s.bBegin, s.nb);
+            for (uint32_t q = 0; q < s.nb; ++q) vol[s.bOrder[q]] = hv[q];
+        }
+    }
+};
+
+}  // namespace
+
+struct sph_ctx {
+    Engine eng;
+};
+
+namespace {
+template <class F>
+int guarded(sph_ctx* ctx, F&& f) {
+    if (!ctx) return SPH_E_ARG;
+    try {
+        return f();
+    } catch (const SphError& e) {
+        ctx->eng.lastError = e.msg;
+        return e.code;
+    } catch (const HipError& e) {
+        ctx->eng.lastError = std::string("HIP error ") + hipGetErrorString(e.e) + " at " + e.where;
+        return SPH_E_HIP;
+    } catch (const std::bad_alloc&) {
+        ctx->eng.lastError = "host out of memory";
+        return SPH_E_HIP;
+    } catch (...) {
+        ctx->eng.lastError = "unknown error";
+        return SPH_E_STATE;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+const char* sph_version(void) { return "sph2o-mi355x 0.1"; }
+
+int sph_create(sph_ctx** ctx, int device, void* stream) {
+    if (!ctx) return SPH_E_ARG;
+    *ctx = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return SPH_E_HIP;
+    if (device < 0 || device >= count) return SPH_E_ARG;
+    auto* c = new (std::nothrow) sph_ctx();
+    if (!c) return SPH_E_HIP;
+    c->eng.device = device;
+    c->eng.stream = static_cast<hipStream_t>(stream);
+    if (hipSetDevice(device) != hipSuccess) {
+        delete c;
+        return SPH_E_HIP;
+    }
+    *ctx = c;
+    return SPH_OK;
+}
+
+int sph_destroy(sph_ctx* ctx) {
+    if (!ctx) return SPH_E_ARG;
+    (void)hipSetDevice(ctx->eng.device);
+    if (ctx->eng.stream) (void)hipStreamSynchronize(ctx->eng.stream);
+    delete ctx;
+    return SPH_OK;
+}
+
+const char* sph_last_error(const sph_ctx* ctx) { return ctx ? ctx->eng.lastError.c_str() : "null context"; }
+
+int sph_add_scene(sph_ctx* ctx, const sph_scene_params* params, int64_t nf, const float* pos, const float* vel,
+                  int64_t nb, const float* bpos) {
+    return guarded(ctx, [&] {
+        if (!params) throw SphError{SPH_E_ARG, "params is NULL"};
+        return ctx->eng.addScene(*params, nf, pos, vel, nb, bpos);
+    });
+}
+
+namespace {
+// SPHSimulation::importScene (SPHSimulation.cpp:299-481) minus the object
+// factory: JSON -> parameters + sampled fluid and boundary particles.
+void readScene(const char* path, sph_scene_params& params, std::vector<float>& xf, std::vector<float>& xbv) {
+    if (!path) throw SphError{SPH_E_ARG, "path is NULL"};
+    std::ifstream f(path);
+    if (!f) throw SphError{SPH_E_SCENE, std::string("cannot read ") + path};
+    std::stringstream ss;
+    ss << f.rdbuf();
+    SceneDesc desc;
+    std::string err;
+    const int rc = parseSceneJson(ss.str(), desc, err);
+    if (rc != SPH_OK) throw SphError{rc, err};
+    if (desc.numEmitters > 0) throw SphError{SPH_E_UNSUPPORTED, "emitters are not supported"};
+    if (desc.numGeometries > 0) throw SphError{SPH_E_UNSUPPORTED, "mesh (geometry) boundaries are not supported"};
+    if (desc.params.boundary_method != SPH_BOUNDARY_AKINCI && (!desc.boxes.empty() || !desc.spheres.empty()))
+        throw SphError{SPH_E_UNSUPPORTED, "only Akinci boundaries are supported"};
+    for (auto& b : desc.fluidBlocks) sampleFluidBlock(b, desc.params.particle_radius, xf);
+    if (desc.params.boundary_method == SPH_BOUNDARY_AKINCI) {
+        for (auto& b : desc.boxes) sampleBoxSurface(b, desc.params.particle_radius, xbv);
+        for (auto& s : desc.spheres) sampleSphereSurface(s, desc.params.particle_radius, xbv);
+    }
+    params = desc.params;
+}
+}  // namespace
+
+int sph_load_scene_json(sph_ctx* ctx, const char* path) {
+    return guarded(ctx, [&] {
+        sph_scene_params params;
+        std::vector<float> xf, xbv;
+        readScene(path, params, xf, xbv);
+        return ctx->eng.addScene(params, static_cast<int64_t>(xf.size() / 3), xf.data(), nullptr,
+                                 static_cast<int64_t>(xbv.size() / 3), xbv.data());
+    });
+}
+
+int sph_scene_from_json(const char* path, sph_scene_params* params, int64_t* nf, float* pos, int64_t pos_cap,
+                        int64_t* nb, float* bpos, int64_t bpos_cap) {
+    sph_ctx dummy;  // error text sink
+    return guarded(&dummy, [&] {
+        if (!params || !nf || !nb) throw SphError{SPH_E_ARG, "NULL output"};
+        std::vector<float> xf, xbv;
+        readScene(path, *params, xf, xbv);
+        *nf = static_cast<int64_t>(xf.size() / 3);
+        *nb = static_cast<int64_t>(xbv.size() / 3);
+        if (*nf > pos_cap || *nb > bpos_cap) return SPH_E_CAPACITY;
+        if (pos && !xf.empty()) std::memcpy(pos, xf.data(), xf.size() * sizeof(float));
+        if (bpos && !xbv.empty()) std::memcpy(bpos, xbv.data(), xbv.size() * sizeof(float));
+        return SPH_OK;
+    });
+}
+
+int sph_num_scenes(const sph_ctx* ctx) { return ctx ? static_cast<int>(ctx->eng.scenes.size()) : SPH_E_ARG; }
+
+int sph_get_params(const sph_ctx* ctx, int scene, sph_scene_params* out) {
+    if (!ctx || !out || scene < 0 || static_cast<size_t>(scene) >= ctx->eng.scenes.size()) return SPH_E_ARG;
+    *out = ctx->eng.scenes[scene].P;
+    return SPH_OK;
+}
+int64_t sph_num_fluid(const sph_ctx* ctx, int scene) {
+    if (!ctx || scene < 0 || static_cast<size_t>(scene) >= ctx->eng.scenes.size()) return SPH_E_ARG;
+    return ctx->eng.scenes[scene].nf;
+}
+int64_t sph_num_boundary(const sph_ctx* ctx, int scene) {
+    if (!ctx || scene < 0 || static_cast<size_t>(scene) >= ctx->eng.scenes.size()) return SPH_E_ARG;
+    return ctx->eng.scenes[scene].nb;
+}
+
+int sph_init(sph_ctx* ctx) {
+    return guarded(ctx, [&] {
+        ctx->eng.init();
+        return SPH_OK;
+    });
+}
+
+int sph_step(sph_ctx* ctx, int nsteps, sph_step_stats* stats) {
+    return guarded(ctx, [&] {
+        if (nsteps < 0) throw SphError{SPH_E_ARG, "nsteps < 0"};
+        ctx->eng.stepAll(nsteps, stats);
+        return SPH_OK;
+    });
+}
+
+int sph_synchronize(sph_ctx* ctx) {
+    return guarded(ctx, [&] {
+        if (ctx->eng.stream) HIPCHK(hipStreamSynchronize(ctx->eng.stream));
+        return SPH_OK;
+    });
+}
+
+int sph_set_fluid_state(sph_ctx* ctx, int scene, int64_t nn, const float* pos, const float* vel) {
+    return guarded(ctx, [&] {
+        ctx->eng.setState(scene, nn, pos, vel);
+        return SPH_OK;
+    });
+}
+
+int sph_get_fluid_state(sph_ctx* ctx, int scene, float* pos, float* vel, float* density, float* pressure) {
+    return guarded(ctx, [&] {
+        ctx->eng.getState(scene, pos, vel, density, pressure);
+        return SPH_OK;
+    });
+}
+
+int sph_get_boundary(sph_ctx* ctx, int scene, float* pos, float* volume) {
+    return guarded(ctx, [&] {
+        ctx->eng.getBoundary(scene, pos, volume);
+        return SPH_OK;
+    });
+}
+
+int sph_get_field(sph_ctx* ctx, int scene, int field, float* out) {
+    return guarded(ctx, [&] {
+        if (!out) throw SphError{SPH_E_ARG, "out is NULL"};
+        ctx->eng.getField(scene, field, out);
+        return SPH_OK;
+    });
+}
+
+int sph_get_time(const sph_ctx* ctx, int scene, double* t, double* dt) {
+    if (!ctx || scene < 0 || static_cast<size_t>(scene) >= ctx->eng.scenes.size()) return SPH_E_ARG;
+    if (t) *t = ctx->eng.scenes[scene].t;
+    if (dt) *dt = ctx->eng.scenes[scene].dt;
+    return SPH_OK;
+}
+
+int sph_dump_cells(sph_ctx* ctx, int scene, int32_t* cells) {
+    return guarded(ctx, [&] {
+        if (!cells) throw SphError{SPH_E_ARG, "cells is NULL"};
+        ctx->eng.dumpCells(scene, cells);
+        return SPH_OK;
+    });
+}
+
+int sph_dump_neighbors(sph_ctx* ctx, int scene, int64_t* offsets, int32_t* ids, int64_t cap, int64_t* boffsets,
+                       int32_t* bids, int64_t bcap) {
+    return guarded(ctx, [&] {
+        if (!offsets || !boffsets || (cap > 0 && !ids) || (bcap > 0 && !bids)) throw SphError{SPH_E_ARG, "NULL output"};
+        return ctx->eng.dumpNeighbors(scene, offsets, ids, cap, boffsets, bids, bcap);
+    });
+}
+
+int sph_last_step_timing(const sph_ctx* ctx, double* total_ms, int64_t* launches) {
+    if (!ctx) return SPH_E_ARG;
+    if (total_ms) *total_ms = ctx->eng.lastStepMs;
+    if (launches) *launches = ctx->eng.launchesInStep;
+    return SPH_OK;
+}
+
+int sph_set_profiling(sph_ctx* ctx, int on) {
+    return guarded(ctx, [&] {
+        ctx->eng.profiling = on != 0;
+        ctx->eng.profileMask = on == 1 || on == -1 ? ~0ull : static_cast<uint64_t>(static_cast<uint32_t>(on));
+        if (!on) {
+            std::fill(std::begin(ctx->eng.kernelMs), std::end(ctx->eng.kernelMs), 0.0);
+            std::fill(std::begin(ctx->eng.kernelLaunches), std::end(ctx->eng.kernelLaunches), 0);
+        }
+        return SPH_OK;
+    });
+}
+
+int sph_kernel_count(void) { return K_COUNT; }
+const char* sph_kernel_name(int kid) { return (kid >= 0 && kid < K_COUNT) ? kKernelNames[kid] : ""; }
+
+int sph_kernel_stats(const sph_ctx* ctx, int kid, double* total_ms, int64_t* launches) {
+    if (!ctx || kid < 0 || kid >= K_COUNT) return SPH_E_ARG;
+    if (total_ms) *total_ms = ctx->eng.kernelMs[kid];
+    if (launches) *launches = ctx->eng.kernelLaunches[kid];
+    return SPH_OK;
+}
+
+int sph_reset_kernel_stats(sph_ctx* ctx) {
+    if (!ctx) return SPH_E_ARG;
+    std::fill(std::begin(ctx->eng.kernelMs), std::end(ctx->eng.kernelMs), 0.0);
+    std::fill(std::begin(ctx->eng.kernelLaunches), std::end(ctx->eng.kernelLaunches), 0);
+    return SPH_OK;
+}
+
+}  // extern "C"

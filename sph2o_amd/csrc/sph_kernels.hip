@@ -1,0 +1,886 @@
+// gfx950 kernels of the SPH particle step (WCSPH / PCISPH / DFSPH with
+// Akinci boundaries, viscosity and Akinci surface tension).
+//
+// Layout: fluid particles of every scene live in one padded slot array,
+// sorted by cell key (Morton order inside each scene's grid). Each scene
+// owns whole 256-slot blocks, so a workgroup always belongs to one scene and
+// reads that scene's constants through the scalar cache. Neighbour lists are
+// column-major (list[slot * NP + i]) so the 64 lanes of a wave read one
+// contiguous 256-byte row per neighbour slot.
+//
+// Numerics follow the reference's FP32 formulas (CubicSpline.h:27-74,
+// ViscoK.h:29-37, Cohesion.h:20-40 and the per-pass sums cited below). The
+// cell key and the support test reproduce what the reference computes under
+// its shipped -O3 -ffast-math build: key = floor(x * fl(1/h)) and
+// |r| = sqrt(fma(x,x, fma(z,z, y*y))), both with explicitly rounded
+// operations so the neighbour sets and the cohesion branch at r = h/2 match
+// bit for bit. Inside the smooth kernel sums the hardware sqrt/rcp are used.
+// Convergence and CFL reductions are FP64 and deterministic (fixed-order
+// block partials, finished by the last block of the launch).
+#include <hip/hip_runtime.h>
+
+#include "sph_kernels.hpp"
+
+namespace sph {
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ float lenRef(float x, float y, float z) {
+    return __fsqrt_rn(__fmaf_rn(x, x, __fmaf_rn(z, z, __fmul_rn(y, y))));
+}
+__device__ __forceinline__ float fastSqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
+__device__ __forceinline__ float fastRcp(float v) { return __builtin_amdgcn_rcpf(v); }
+
+// CubicSpline::W (CubicSpline.h:27-50)
+__device__ __forceinline__ float cubicW(const SceneConst& S, float rl) {
+    const float q = rl * S.invh;
+    if (q > 1.0f) return 0.0f;
+    if (q <= 0.5f) {
+        const float q2 = q * q;
+        return S.kW * (6.0f * (q2 * q) - 6.0f * q2 + 1.0f);
+    }
+    const float f = 1.0f - q;
+    return S.kW * (2.0f * (f * f * f));
+}
+// CubicSpline::gradW (CubicSpline.h:52-74) as gradW = r * factor
+__device__ __forceinline__ float gradFactor(const SceneConst& S, float rl) {
+    const float q = rl * S.invh;
+    if (!(rl > 1.0e-9f) || q > 1.0f) return 0.0f;
+    const float inv = fastRcp(rl * S.h);
+    if (q <= 0.5f) return S.lW * q * (3.0f * q - 2.0f) * inv;
+    const float f = 1.0f - q;
+    return S.lW * (-f * f) * inv;
+}
+// ViscoK::laplW (ViscoK.h:29-37)
+__device__ __forceinline__ float laplW(const SceneConst& S, float rl) {
+    return rl <= S.h ? S.laplC * (S.h - rl) : 0.0f;
+}
+// Cohesion::W (Cohesion.h:20-40), with the reference's exact length
+__device__ __forceinline__ float cohesionW(const SceneConst& S, float rl) {
+    if (2.0f * rl > S.h && rl <= S.h) {
+        const float a = S.h - rl;
+        return S.cohC1 * (a * a * a) * (rl * rl * rl);
+    }
+    if (rl > 0.0f && 2.0f * rl <= S.h) {
+        const float a = S.h - rl;
+        return S.cohC1 * 2.0f * (a * a * a) * (rl * rl * rl) - S.cohC2;
+    }
+    return 0.0f;
+}
+
+__device__ __forceinline__ int floorCell(float x, float invh) { return static_cast<int>(floorf(__fmul_rn(x, invh))); }
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {  // 10 bits -> every third bit
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// local cell coordinates (clamped into the grid); returns false if clamped
+__device__ __forceinline__ bool localCell(const SceneConst& S, float4 p, int& cx, int& cy, int& cz) {
+    bool ok = isfinite(p.x) && isfinite(p.y) && isfinite(p.z);
+    int x = ok ? floorCell(p.x, S.invh) - S.g0x : 0;
+    int y = ok ? floorCell(p.y, S.invh) - S.g0y : 0;
+    int z = ok ? floorCell(p.z, S.invh) - S.g0z : 0;
+    // guard against int overflow from huge coordinates: floorf of |x*invh| > 2^31 is UB on cast
+    if (ok && (fabsf(p.x * S.invh) > 1.0e9f || fabsf(p.y * S.invh) > 1.0e9f || fabsf(p.z * S.invh) > 1.0e9f)) {
+        ok = false;
+        x = y = z = 0;
+    }
+    const bool in = ok && x >= 0 && y >= 0 && z >= 0 && x < S.gnx && y < S.gny && z < S.gnz;
+    cx = min(max(x, 0), S.gnx - 1);
+    cy = min(max(y, 0), S.gny - 1);
+    cz = min(max(z, 0), S.gnz - 1);
+    return in;
+}
+__device__ __forceinline__ uint32_t cellLinear(const SceneConst& S, int cx, int cy, int cz) {
+    return static_cast<uint32_t>((cz * S.gny + cy) * S.gnx + cx);
+}
+__device__ __forceinline__ uint32_t cellKey(const SceneConst& S, int cx, int cy, int cz) {
+    if (S.morton) return S.keyBase + (spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2));
+    return S.keyBase + cellLinear(S, cx, cy, cz);
+}
+
+#define SCENE_PROLOGUE                                        \
+    const int sIdx = d.blockScene[blockIdx.x];                \
+    const SceneConst& S = d.scenes[sIdx];                     \
+    const SceneDyn D = d.dyn[sIdx];                           \
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;     \
+    const bool valid = D.active && i < S.fEnd;
+
+// ---- FP64 deterministic reductions: block partial + last-block finish ----
+template <bool MAX>
+__device__ __forceinline__ double combine(double a, double b) {
+    return MAX ? fmax(a, b) : a + b;
+}
+template <bool MAX>
+__device__ double blockReduce(double v) {
+    __shared__ double sm[kBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) v = combine<MAX>(v, __shfl_down(v, off, 64));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sm[wid] = v;
+    __syncthreads();
+    double t = sm[0];
+    for (int w = 1; w < kBlock / 64; ++w) t = combine<MAX>(t, sm[w]);
+    return t;  // valid in every thread
+}
+
+// Every block of the launch must call this exactly once (inactive blocks pass 0).
+// slot: partial slot (0 or 1). Writes red[scene * kRedSlots + redSlot] for active scenes.
+template <bool MAX>
+__device__ void finishReduction(const DevView& d, double v, int slot, int redSlot) {
+    const double t = blockReduce<MAX>(v);
+    __shared__ bool amLast;
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&d.partials[static_cast<size_t>(slot) * d.numBlocks + blockIdx.x], t, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const uint32_t ticket = atomicAdd(d.counter, 1u);
+        amLast = (ticket == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!amLast) return;
+    __threadfence();
+    for (int s = 0; s < d.numScenes; ++s) {
+        if (!d.dyn[s].active) continue;
+        const SceneConst& S = d.scenes[s];
+        double acc = MAX ? 0.0 : 0.0;
+        for (uint32_t b = S.blockBegin + threadIdx.x; b < S.blockEnd; b += kBlock)
+            acc = combine<MAX>(acc, __hip_atomic_load(&d.partials[static_cast<size_t>(slot) * d.numBlocks + b],
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        const double tot = blockReduce<MAX>(acc);
+        if (threadIdx.x == 0) d.red[s * kRedSlots + redSlot] = tot;
+    }
+    if (threadIdx.x == 0) __hip_atomic_store(d.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------- grid
+// Cell keys of every slot. dfsphAdvance: first x += v*dt and k_ws *= scale
+// (DFSPHSolver::updatePositions :204-224 and the k_ws *= ts^2 of :516-530).
+__global__ void __launch_bounds__(kBlock) kKeys(DevView d, int dfsphAdvance, uint32_t* keys, uint32_t* vals) {
+    const int sIdx = d.blockScene[blockIdx.x];
+    const SceneConst& S = d.scenes[sIdx];
+    const SceneDyn D = d.dyn[sIdx];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    vals[i] = i;
+    if (i >= S.fEnd) {
+        keys[i] = S.deadKey;
+        return;
+    }
+    float4 p = d.x[i];
+    if (dfsphAdvance) {
+        const float4 vi = d.v[i];
+        p.x += vi.x * D.dt;
+        p.y += vi.y * D.dt;
+        p.z += vi.z * D.dt;
+        d.x[i] = p;
+        d.kws[i] *= D.scale;
+    }
+    int cx, cy, cz;
+    if (!localCell(S, p, cx, cy, cz)) atomicOr(&d.flags[sIdx * kFlagSlots + kFlagOutOfGrid], 1);
+    keys[i] = cellKey(S, cx, cy, cz);
+}
+
+__global__ void kFillRanges(uint2* r, size_t n) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = make_uint2(0u, 0u);
+}
+
+// Gather the persistent state into sorted order and record cell ranges.
+__global__ void __launch_bounds__(kBlock) kReorder(DevView d, const uint32_t* keysSorted, const uint32_t* valsSorted,
+                                                   float4* xAlt, float4* vAlt, float* kwsAlt, float* kvwsAlt,
+                                                   uint32_t* idAlt) {
+    const int sIdx = d.blockScene[blockIdx.x];
+    const SceneConst& S = d.scenes[sIdx];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t o = valsSorted[i];
+    const float4 p = d.x[o];
+    xAlt[i] = p;
+    vAlt[i] = d.v[o];
+    kwsAlt[i] = d.kws[o];
+    kvwsAlt[i] = d.kvws[o];
+    idAlt[i] = d.id[o];
+    if (i >= S.fEnd) return;
+    const uint32_t key = keysSorted[i];
+    int cx, cy, cz;
+    localCell(S, p, cx, cy, cz);
+    const uint32_t c = S.cellBase + cellLinear(S, cx, cy, cz);
+    if (i == S.fBegin || keysSorted[i - 1] != key) d.fRange[c].x = i;
+    if (i + 1 == S.fEnd || keysSorted[i + 1] != key) d.fRange[c].y = i + 1;
+}
+
+// Within-support neighbour lists from the 27-cell candidates (HashTable.cpp:23-86
+// walk order: dx, dy, dz with dz fastest), filtered by |r| <= h with the
+// reference's rounding (DFSPHSolver.cpp:575). Self is included.
+__global__ void __launch_bounds__(kBlock) kBuildNeighbors(DevView d) {
+    SCENE_PROLOGUE
+    uint32_t nf = 0, nb = 0;
+    if (valid) {
+        const float4 xi = d.x[i];
+        int cx, cy, cz;
+        localCell(S, xi, cx, cy, cz);
+        const size_t NP = d.NP;
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int ax = cx + dx;
+            if (ax < 0 || ax >= S.gnx) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int ay = cy + dy;
+                if (ay < 0 || ay >= S.gny) continue;
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const int az = cz + dz;
+                    if (az < 0 || az >= S.gnz) continue;
+                    const uint32_t c = S.cellBase + cellLinear(S, ax, ay, az);
+                    const uint2 fr = d.fRange[c];
+                    for (uint32_t j = fr.x; j < fr.y; ++j) {
+                        const float4 xj = d.x[j];
+                        if (lenRef(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) <= S.h) {
+                            if (nf < d.capF) d.nbrF[nf * NP + i] = j;
+                            ++nf;
+                        }
+                    }
+                    if (S.akinci) {
+                        const uint2 br = d.bRange[c];
+                        for (uint32_t b = br.x; b < br.y; ++b) {
+                            const float4 xb = d.xb[b];
+                            if (lenRef(xi.x - xb.x, xi.y - xb.y, xi.z - xb.z) <= S.h) {
+                                if (nb < d.capB) d.nbrB[nb * NP + i] = b;
+                                ++nb;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        d.cntF[i] = nf;
+        d.cntB[i] = nb;
+    }
+    // block-wide max, one atomic per block
+    int mf = static_cast<int>(nf), mb = static_cast<int>(nb);
+    for (int off = 32; off > 0; off >>= 1) {
+        mf = max(mf, __shfl_down(mf, off, 64));
+        mb = max(mb, __shfl_down(mb, off, 64));
+    }
+    __shared__ int smf[kBlock / 64], smb[kBlock / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        smf[wid] = mf;
+        smb[wid] = mb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; ++w) {
+            mf = max(mf, smf[w]);
+            mb = max(mb, smb[w]);
+        }
+        atomicMax(&d.flags[sIdx * kFlagSlots + kFlagMaxFluidNbr], mf);
+        atomicMax(&d.flags[sIdx * kFlagSlots + kFlagMaxBoundaryNbr], mb);
+    }
+}
+
+// AkinciBoundaryObject::computeVolume (AkinciBoundaryObject.cpp:10-34):
+// V_b = 1 / sum of W over the boundary candidates of the 27 cells.
+__global__ void kBoundaryVolume(DevView d, int scene, uint32_t bBegin, uint32_t bEnd) {
+    const SceneConst& S = d.scenes[scene];
+    const uint32_t b = bBegin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= bEnd) return;
+    const float4 xb = d.xb[b];
+    int cx, cy, cz;
+    localCell(S, xb, cx, cy, cz);
+    float delta = 0.0f;
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz) {
+                const int ax = cx + dx, ay = cy + dy, az = cz + dz;
+                if (ax < 0 || ay < 0 || az < 0 || ax >= S.gnx || ay >= S.gny || az >= S.gnz) continue;
+                const uint2 br = d.bRange[S.cellBase + cellLinear(S, ax, ay, az)];
+                for (uint32_t c = br.x; c < br.y; ++c) {
+                    const float4 xc = d.xb[c];
+                    delta += cubicW(S, lenRef(xb.x - xc.x, xb.y - xc.y, xb.z - xc.z));
+                }
+            }
+    const float vol = 1.0f / delta;
+    d.vb[b] = vol;
+    d.xb[b].w = S.rho0 * vol;
+}
+
+// ---------------------------------------------------------------- density
+// SPHSolver::computeFluidDensities (SPHSolver.cpp:42-89) fused with
+// DFSPHSolver::computeAlpha (DFSPHSolver.cpp:109-178).
+__global__ void __launch_bounds__(kBlock) kDensityAlpha(DevView d, int withAlpha) {
+    SCENE_PROLOGUE
+    if (!valid) return;
+    const size_t NP = d.NP;
+    const float4 xi = d.x[i];
+    float rho = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, ssq = 0.0f;
+    const uint32_t nf = min(d.cntF[i], d.capF);
+    for (uint32_t s = 0; s < nf; ++s) {
+        const uint32_t j = d.nbrF[s * NP + i];
+        const float4 xj = d.x[j];
+        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        const float rl = fastSqrt(rx * rx + ry * ry + rz * rz);
+        rho += S.m * cubicW(S, rl);
+        const float g = S.m * gradFactor(S, rl);
+        const float gx = g * rx, gy = g * ry, gz = g * rz;
+        sgx += gx; sgy += gy; sgz += gz;
+        ssq += gx * gx + gy * gy + gz * gz;
+    }
+    const uint32_t nb = S.akinci ? min(d.cntB[i], d.capB) : 0u;
+    for (uint32_t s = 0; s < nb; ++s) {
+        const float4 xb = d.xb[d.nbrB[s * NP + i]];
+        const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
+        const float rl = fastSqrt(rx * rx + ry * ry + rz * rz);
+        rho += xb.w * cubicW(S, rl);
+        const float g = xb.w * gradFactor(S, rl);
+        sgx += g * rx; sgy += g * ry; sgz += g * rz;
+    }
+    d.rho[i] = rho;
+    if (withAlpha) {
+        const float den = sgx * sgx + sgy * sgy + sgz * sgz + ssq;
+        d.alpha[i] = den > S.eps ? rho / den : 0.0f;
+    }
+}
+
+// WCSPH density + Tait pressure (WCSPHSolver.cpp:80-95).
+__global__ void __launch_bounds__(kBlock) kDensityPressure(DevView d) {
+    SCENE_PROLOGUE
+    if (!valid) return;
+    const size_t NP = d.NP;
+    const float4 xi = d.x[i];
+    float rho = 0.0f;
+    const uint32_t nf = min(d.cntF[i], d.capF);
+    for (uint32_t s = 0; s < nf; ++s) {
+        const float4 xj = d.x[d.nbrF[s * NP + i]];
+        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        rho += S.m * cubicW(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+    }
+    const uint32_t nb = S.akinci ? min(d.cntB[i], d.capB) : 0u;
+    for (uint32_t s = 0; s < nb; ++s) {
+        const float4 xb = d.xb[d.nbrB[s * NP + i]];
+        const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
+        rho += xb.w * cubicW(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+    }
+    d.rho[i] = rho;
+    d.p[i] = fmaxf(0.0f, S.wcB * (powf(rho / S.rho0, S.gamma) - 1.0f));
+}
+
+// ---------------------------------------------------------------- forces
+// One neighbour pass computing, in the reference's accumulation order:
+//   WCSPH pressure force (WCSPHSolver.cpp:132-188) when `pressure`,
+//   viscosity: standard (StandardViscosity.cpp:5-49), artificial
+//     (ArtificialViscosity.cpp:6-53) or XSPH (XSPHViscosity.cpp:6-42),
+//   surface-tension normals (SurfaceTensionAkinci.cpp:13-33).
+// The acceleration is written fresh (it is zero at this point of every step).
+__global__ void __launch_bounds__(kBlock) kForces(DevView d, int pressure) {
+    SCENE_PROLOGUE
+    if (!valid) return;
+    const size_t NP = d.NP;
+    const float4 xi = d.x[i];
+    const float4 vi = d.v[i];
+    const float rhoi = d.rho[i];
+    const uint32_t nf = min(d.cntF[i], d.capF);
+    const uint32_t nb = S.akinci ? min(d.cntB[i], d.capB) : 0u;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    if (pressure) {
+        const float pi_ = d.p[i];
+        const float dpi = pi_ / (rhoi * rhoi);
+        const float dpb = pi_ / (S.rho0 * S.rho0);
+        for (uint32_t s = 0; s < nf; ++s) {
+            const uint32_t j = d.nbrF[s * NP + i];
+            const float4 xj = d.x[j];
+            const float rhoj = d.rho[j];
+            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            const float g = S.m * (dpi + d.p[j] / (rhoj * rhoj)) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+            ax -= g * rx; ay -= g * ry; az -= g * rz;
+        }
+        for (uint32_t s = 0; s < nb; ++s) {
+            const float4 xb = d.xb[d.nbrB[s * NP + i]];
+            const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
+            const float g = xb.w * (dpi + dpb) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+            ax -= g * rx; ay -= g * ry; az -= g * rz;
+        }
+    }
+    const int vm = S.viscMethod;
+    const bool normals = S.stOn;
+    if (vm >= 0 || normals) {
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;   // viscosity sum
+        float nx = 0.0f, ny = 0.0f, nz = 0.0f;   // normal sum
+        const float invdt = 1.0f / D.dt;
+        for (uint32_t s = 0; s < nf; ++s) {
+            const uint32_t j = d.nbrF[s * NP + i];
+            const float4 xj = d.x[j];
+            const float rhoj = d.rho[j];
+            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            const float r2 = rx * rx + ry * ry + rz * rz;
+            const float rl = fastSqrt(r2);
+            if (vm >= 0) {
+                const float4 vj = d.v[j];
+                if (vm == 2) {  // XSPH: nu (m/rho_j) (v_j - v_i) W / dt
+                    const float c = S.visc * (S.m / rhoj) * cubicW(S, rl) * invdt;
+                    fx += c * (vj.x - vi.x); fy += c * (vj.y - vi.y); fz += c * (vj.z - vi.z);
+                } else if (vm == 1) {  // artificial: nu (m/rho_j) (v_ij . r_ij)/(r^2 + 0.01 h^2) gradW
+                    const float dp = (vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz;
+                    const float c = S.visc * (S.m / rhoj) * dp / (r2 + S.h2c) * gradFactor(S, rl);
+                    fx += c * rx; fy += c * ry; fz += c * rz;
+                } else {  // standard: m (v_j - v_i)/rho_j laplW, scaled by nu/rho_i below
+                    const float c = S.m / rhoj * laplW(S, rl);
+                    fx += c * (vj.x - vi.x); fy += c * (vj.y - vi.y); fz += c * (vj.z - vi.z);
+                }
+            }
+            if (normals) {
+                const float c = (S.m / rhoj) * gradFactor(S, rl);
+                nx += c * rx; ny += c * ry; nz += c * rz;
+            }
+        }
+        if (vm == 0) {
+            const float c = S.visc / rhoi;
+            fx *= c; fy *= c; fz *= c;
+        }
+        ax += fx; ay += fy; az += fz;
+        if (vm >= 0 && S.bviscOn) {
+            float bx = 0.0f, by = 0.0f, bz = 0.0f;
+            for (uint32_t s = 0; s < nb; ++s) {
+                const float4 xb = d.xb[d.nbrB[s * NP + i]];
+                const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                const float rl = fastSqrt(r2);
+                if (vm == 2) {
+                    const float c = S.bvisc * (xb.w / rhoi) * cubicW(S, rl) * invdt;
+                    bx -= c * vi.x; by -= c * vi.y; bz -= c * vi.z;
+                } else if (vm == 1) {
+                    const float dp = vi.x * rx + vi.y * ry + vi.z * rz;
+                    const float c = S.bvisc * (xb.w / rhoi) * dp / (r2 + S.h2c) * gradFactor(S, rl);
+                    bx += c * rx; by += c * ry; bz += c * rz;
+                } else {
+                    const float c = xb.w / rhoi * laplW(S, rl);
+                    bx -= c * vi.x; by -= c * vi.y; bz -= c * vi.z;
+                }
+            }
+            if (vm == 0) {
+                const float c = S.bvisc / rhoi;
+                bx *= c; by *= c; bz *= c;
+            }
+            ax += bx; ay += by; az += bz;
+        }
+        if (normals) d.n[i] = make_float4(nx * S.h, ny * S.h, nz * S.h, 0.0f);
+    }
+    d.a[i] = make_float4(ax, ay, az, 0.0f);
+}
+
+// SurfaceTensionAkinci::step pass 2 (SurfaceTensionAkinci.cpp:35-70).
+// With vmax: the FP64 max of |v + (a + g) dt| (DFSPHSolver.cpp:1096-1113)
+// over the final accelerations of this step.
+__global__ void __launch_bounds__(kBlock) kSurfaceTension(DevView d, int vmax) {
+    SCENE_PROLOGUE
+    double vm = 0.0;
+    if (valid) {
+        const size_t NP = d.NP;
+        float4 ai = d.a[i];
+        if (S.stOn) {
+            const float4 xi = d.x[i];
+            const float4 ni = d.n[i];
+            const float rhoi = d.rho[i];
+            const uint32_t nf = min(d.cntF[i], d.capF);
+            for (uint32_t s = 0; s < nf; ++s) {
+                const uint32_t j = d.nbrF[s * NP + i];
+                const float4 xj = d.x[j];
+                const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+                const float rl = lenRef(rx, ry, rz);
+                if (rl > 0.0f && rl <= S.h) {
+                    const float4 nj = d.n[j];
+                    const float kij = 2.0f * S.rho0 / (rhoi + d.rho[j]);
+                    const float c = S.st * S.m * cohesionW(S, rl) / rl;
+                    ai.x += kij * (-c * rx - S.st * (ni.x - nj.x));
+                    ai.y += kij * (-c * ry - S.st * (ni.y - nj.y));
+                    ai.z += kij * (-c * rz - S.st * (ni.z - nj.z));
+                }
+            }
+            d.a[i] = ai;
+        }
+        if (vmax) {
+            const float4 vi = d.v[i];
+            const float cx = vi.x + (ai.x + S.gx) * D.dt;
+            const float cy = vi.y + (ai.y + S.gy) * D.dt;
+            const float cz = vi.z + (ai.z + S.gz) * D.dt;
+            vm = sqrt(static_cast<double>(cx) * cx + static_cast<double>(cy) * cy + static_cast<double>(cz) * cz);
+        }
+    }
+    if (vmax) finishReduction<true>(d, vm, 0, kRedMax);
+}
+
+__global__ void __launch_bounds__(kBlock) kVmax(DevView d) {
+    SCENE_PROLOGUE
+    double vm = 0.0;
+    if (valid) {
+        const float4 vi = d.v[i], ai = d.a[i];
+        const float cx = vi.x + (ai.x + S.gx) * D.dt;
+        const float cy = vi.y + (ai.y + S.gy) * D.dt;
+        const float cz = vi.z + (ai.z + S.gz) * D.dt;
+        vm = sqrt(static_cast<double>(cx) * cx + static_cast<double>(cy) * cy + static_cast<double>(cz) * cz);
+    }
+    finishReduction<true>(d, vm, 0, kRedMax);
+}
+
+// DFSPHSolver::predictVelocities (DFSPHSolver.cpp:180-202)
+__global__ void __launch_bounds__(kBlock) kPredictVelocity(DevView d) {
+    SCENE_PROLOGUE
+    if (!valid) return;
+    float4 vi = d.v[i];
+    const float4 ai = d.a[i];
+    vi.x += (ai.x + S.gx) * D.dt;
+    vi.y += (ai.y + S.gy) * D.dt;
+    vi.z += (ai.z + S.gz) * D.dt;
+    d.v[i] = vi;
+    d.a[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// ---------------------------------------------------------------- DFSPH solves
+// DFSPHSolver::predictDensities (DFSPHSolver.cpp:226-299). warm: the
+// warm-start rescale of densityWarmStart (:863-886); otherwise the k pass of
+// correctDensityError (:435-453) with the FP64 average error.
+__global__ void __launch_bounds__(kBlock) kRhoStar(DevView d, int warm) {
+    SCENE_PROLOGUE
+    double err = 0.0;
+    if (valid) {
+        const size_t NP = d.NP;
+        const float4 xi = d.x[i];
+        const float4 vi = d.v[i];
+        float dc = 0.0f;
+        const uint32_t nf = min(d.cntF[i], d.capF);
+        for (uint32_t s = 0; s < nf; ++s) {
+            const uint32_t j = d.nbrF[s * NP + i];
+            const float4 xj = d.x[j];
+            const float4 vj = d.v[j];
+            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            const float g = gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+            dc += S.m * (g * ((vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz));
+        }
+        const uint32_t nb = S.akinci ? min(d.cntB[i], d.capB) : 0u;
+        for (uint32_t s = 0; s < nb; ++s) {
+            const float4 xb = d.xb[d.nbrB[s * NP + i]];
+            const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
+            const float g = gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+            dc += xb.w * (g * (vi.x * rx + vi.y * ry + vi.z * rz));
+        }
+        const float rhoi = d.rho[i];
+        const float pd = fmaxf(rhoi + dc * D.dt, S.rho0);
+        d.aux[i] = pd;
+        if (warm) {
+            d.kws[i] = pd > S.rho0 ? d.kws[i] / D.dt2 : 0.0f;
+        } else {
+            const float ki = (pd - S.rho0) * d.alpha[i] / (D.dt2 * rhoi);
+            d.k[i] = ki;
+            d.kws[i] = D.first ? ki : d.kws[i] + ki;
+            err = static_cast<double>((pd - S.rho0) / S.rho0);
+        }
+    }
+    if (!warm) finishReduction<false>(d, err, 0, kRedSum);
+}
+
+// The velocity correction shared by both solves and both warm starts
+// (DFSPHSolver.cpp:455-511, :788-823, :894-941, :998-1045).
+__global__ void __launch_bounds__(kBlock) kVelocity(DevView d, const float* kb) {
+    SCENE_PROLOGUE
+    if (!valid) return;
+    const size_t NP = d.NP;
+    const float4 xi = d.x[i];
+    const float ki = kb[i];
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    const uint32_t nf = min(d.cntF[i], d.capF);
+    for (uint32_t s = 0; s < nf; ++s) {
+        const uint32_t j = d.nbrF[s * NP + i];
+        const float ks = ki + kb[j];
+        if (fabsf(ks) > S.eps) {
+            const float4 xj = d.x[j];
+            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            const float c = (S.m * ks) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+            sx += c * rx; sy += c * ry; sz += c * rz;
+        }
+    }
+    if (S.akinci && fabsf(ki) > S.eps) {
+        const uint32_t nb = min(d.cntB[i], d.capB);
+        for (uint32_t s = 0; s < nb; ++s) {
+            const float4 xb = d.xb[d.nbrB[s * NP + i]];
+            const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
+            const float c = (xb.w * ki) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+            sx += c * rx; sy += c * ry; sz += c * rz;
+        }
+    }
+    float4 vi = d.v[i];
+    vi.x -= D.dt * sx;
+    vi.y -= D.dt * sy;
+    vi.z -= D.dt * sz;
+    d.v[i] = vi;
+}
+
+// DFSPHSolver::computeDivergenceError (DFSPHSolver.cpp:538-621); the
+// neighbour count n_i (|r| <= h, fluid + boundary, self included) is the
+// length of the within-support lists. warm: divergenceWarmStart rescale
+// (:968-990); otherwise the kv pass of correctDivergenceError (:750-768).
+__global__ void __launch_bounds__(kBlock) kDivergence(DevView d, int warm) {
+    SCENE_PROLOGUE
+    double err = 0.0;
+    if (valid) {
+        const size_t NP = d.NP;
+        const float4 xi = d.x[i];
+        const float4 vi = d.v[i];
+        float de = 0.0f;
+        const uint32_t nfTrue = d.cntF[i];
+        const uint32_t nf = min(nfTrue, d.capF);
+        for (uint32_t s = 0; s < nf; ++s) {
+            const uint32_t j = d.nbrF[s * NP + i];
+            const float4 xj = d.x[j];
+            const float4 vj = d.v[j];
+            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            const float g = gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+            de += S.m * (g * ((vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz));
+        }
+        uint32_t cnt = nfTrue;
+        if (S.akinci) {
+            const uint32_t nbTrue = d.cntB[i];
+            const uint32_t nb = min(nbTrue, d.capB);
+            cnt += nbTrue;
+            for (uint32_t s = 0; s < nb; ++s) {
+                const float4 xb = d.xb[d.nbrB[s * NP + i]];
+                const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
+                const float g = gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+                de += xb.w * (g * (vi.x * rx + vi.y * ry + vi.z * rz));
+            }
+        }
+        de = cnt < 20u ? 0.0f : fmaxf(de, 0.0f);
+        d.aux[i] = de;
+        if (warm) {
+            d.kvws[i] = de > 0.0f ? d.kvws[i] / D.dt : 0.0f;
+        } else {
+            const float kvi = de * d.alpha[i] / (D.dt * d.rho[i]);
+            d.k[i] = kvi;
+            d.kvws[i] = D.first ? kvi : d.kvws[i] + kvi;
+            err = static_cast<double>(de / S.rho0);
+        }
+    }
+    if (!warm) finishReduction<false>(d, err, 0, kRedSum);
+}
+
+// buf *= dyn.scale (k_ws *= ts^2, kv_ws *= ts; DFSPHSolver.cpp:516-530, :829-843)
+__global__ void __launch_bounds__(kBlock) kScale(DevView d, float* buf) {
+    SCENE_PROLOGUE
+    if (!valid) return;
+    buf[i] *= D.scale;
+}
+
+// SPHSolver::integrate (SPHSolver.cpp:139-168)
+__global__ void __launch_bounds__(kBlock) kIntegrate(DevView d) {
+    SCENE_PROLOGUE
+    if (!valid) return;
+    float4 ai = d.a[i], vi = d.v[i], xi = d.x[i];
+    ai.x += S.gx; ai.y += S.gy; ai.z += S.gz;
+    vi.x += ai.x * D.dt; vi.y += ai.y * D.dt; vi.z += ai.z * D.dt;
+    xi.x += vi.x * D.dt; xi.y += vi.y * D.dt; xi.z += vi.z * D.dt;
+    d.v[i] = vi;
+    d.x[i] = xi;
+    d.a[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// ---------------------------------------------------------------- PCISPH
+// initPressure (PCISPHSolver.cpp:82-118) + first predictVelocityAndPosition (:120-147)
+__global__ void __launch_bounds__(kBlock) kPciInit(DevView d) {
+    SCENE_PROLOGUE
+    if (!valid) return;
+    d.p[i] = 0.0f;
+    d.ap[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 xi = d.x[i], vi = d.v[i], ai = d.a[i];
+    const float vx = vi.x + (ai.x + S.gx) * D.dt;
+    const float vy = vi.y + (ai.y + S.gy) * D.dt;
+    const float vz = vi.z + (ai.z + S.gz) * D.dt;
+    d.xs[i] = make_float4(xi.x + vx * D.dt, xi.y + vy * D.dt, xi.z + vz * D.dt, 0.0f);
+}
+
+// predictFluidDensities (PCISPHSolver.cpp:162-209): predicted positions,
+// candidates of the *current* position's 27 cells (:178); then
+// updatePressure (:246-298) with FP64 average and max error.
+__global__ void __launch_bounds__(kBlock) kPciDensity(DevView d) {
+    SCENE_PROLOGUE
+    double err = 0.0;
+    if (valid) {
+        const float4 xsi = d.xs[i];
+        int cx, cy, cz;
+        localCell(S, d.x[i], cx, cy, cz);
+        float rho = 0.0f;
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int ax = cx + dx;
+            if (ax < 0 || ax >= S.gnx) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int ay = cy + dy;
+                if (ay < 0 || ay >= S.gny) continue;
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const int az = cz + dz;
+                    if (az < 0 || az >= S.gnz) continue;
+                    const uint32_t c = S.cellBase + cellLinear(S, ax, ay, az);
+                    const uint2 fr = d.fRange[c];
+                    for (uint32_t j = fr.x; j < fr.y; ++j) {
+                        const float4 xj = d.xs[j];
+                        const float rx = xsi.x - xj.x, ry = xsi.y - xj.y, rz = xsi.z - xj.z;
+                        rho += S.m * cubicW(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+                    }
+                    if (S.akinci) {
+                        const uint2 br = d.bRange[c];
+                        for (uint32_t b = br.x; b < br.y; ++b) {
+                            const float4 xb = d.xb[b];
+                            const float rx = xsi.x - xb.x, ry = xsi.y - xb.y, rz = xsi.z - xb.z;
+                            rho += xb.w * cubicW(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+                        }
+                    }
+                }
+            }
+        }
+        d.rho[i] = rho;
+        const float e = fmaxf(0.0f, rho - S.rho0);
+        d.p[i] += S.pciDelta * e;
+        err = static_cast<double>(e / S.rho0);
+    }
+    // slot 0: sum, slot 1: max
+    const double t = err;
+    finishReduction<false>(d, t, 0, kRedSum);
+}
+__global__ void __launch_bounds__(kBlock) kPciMaxErr(DevView d) {
+    SCENE_PROLOGUE
+    double e = 0.0;
+    if (valid) e = static_cast<double>(fmaxf(0.0f, d.rho[i] - S.rho0) / S.rho0);
+    finishReduction<true>(d, e, 1, kRedMax);
+}
+
+// computePressureAcc (PCISPHSolver.cpp:300-367) at current positions with
+// the predicted densities, then the next predictVelocityAndPosition.
+__global__ void __launch_bounds__(kBlock) kPciPressureAcc(DevView d) {
+    SCENE_PROLOGUE
+    if (!valid) return;
+    const size_t NP = d.NP;
+    const float4 xi = d.x[i];
+    const float rhoi = d.rho[i];
+    const float dpi = d.p[i] / (rhoi * rhoi);
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    const uint32_t nf = min(d.cntF[i], d.capF);
+    for (uint32_t s = 0; s < nf; ++s) {
+        const uint32_t j = d.nbrF[s * NP + i];
+        const float4 xj = d.x[j];
+        const float rhoj = d.rho[j];
+        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        const float g = S.m * (dpi + d.p[j] / (rhoj * rhoj)) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+        ax -= g * rx; ay -= g * ry; az -= g * rz;
+    }
+    const uint32_t nb = S.akinci ? min(d.cntB[i], d.capB) : 0u;
+    for (uint32_t s = 0; s < nb; ++s) {
+        const float4 xb = d.xb[d.nbrB[s * NP + i]];
+        const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
+        const float g = xb.w * dpi * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+        ax -= g * rx; ay -= g * ry; az -= g * rz;
+    }
+    d.ap[i] = make_float4(ax, ay, az, 0.0f);
+    const float4 vi = d.v[i], ai = d.a[i];
+    const float vx = vi.x + ((ai.x + ax) + S.gx) * D.dt;
+    const float vy = vi.y + ((ai.y + ay) + S.gy) * D.dt;
+    const float vz = vi.z + ((ai.z + az) + S.gz) * D.dt;
+    d.xs[i] = make_float4(xi.x + vx * D.dt, xi.y + vy * D.dt, xi.z + vz * D.dt, 0.0f);
+}
+
+// PCISPHSolver::integrate (PCISPHSolver.cpp:459-488)
+__global__ void __launch_bounds__(kBlock) kPciIntegrate(DevView d) {
+    SCENE_PROLOGUE
+    if (!valid) return;
+    float4 ai = d.a[i], vi = d.v[i], xi = d.x[i];
+    const float4 pa = d.ap[i];
+    ai.x += S.gx + pa.x; ai.y += S.gy + pa.y; ai.z += S.gz + pa.z;
+    vi.x += ai.x * D.dt; vi.y += ai.y * D.dt; vi.z += ai.z * D.dt;
+    xi.x += vi.x * D.dt; xi.y += vi.y * D.dt; xi.z += vi.z * D.dt;
+    d.v[i] = vi;
+    d.x[i] = xi;
+    d.a[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// ---------------------------------------------------------------- utilities
+__global__ void __launch_bounds__(kBlock) kCellsOut(DevView d, int32_t* cells) {
+    const int sIdx = d.blockScene[blockIdx.x];
+    const SceneConst& S = d.scenes[sIdx];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= S.fEnd) return;
+    const float4 p = d.x[i];
+    cells[3 * i] = floorCell(p.x, S.invh);
+    cells[3 * i + 1] = floorCell(p.y, S.invh);
+    cells[3 * i + 2] = floorCell(p.z, S.invh);
+}
+
+__global__ void __launch_bounds__(kBlock) kCheckFinite(DevView d) {
+    const int sIdx = d.blockScene[blockIdx.x];
+    const SceneConst& S = d.scenes[sIdx];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= S.fEnd) return;
+    const float4 p = d.x[i], v = d.v[i];
+    if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(v.x) && isfinite(v.y) && isfinite(v.z)))
+        atomicOr(&d.flags[sIdx * kFlagSlots + kFlagNonFinite], 1);
+}
+
+// ---------------------------------------------------------------- launchers
+#define LAUNCH(kernel, grid, ...)                                                     \
+    do {                                                                              \
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, st, __VA_ARGS__);     \
+        return hipGetLastError();                                                     \
+    } while (0)
+
+hipError_t launchKeys(const DevView& d, int adv, uint32_t* keys, uint32_t* vals, hipStream_t st) {
+    LAUNCH(kKeys, d.numBlocks, d, adv, keys, vals);
+}
+hipError_t launchFillRanges(uint2* r, size_t n, hipStream_t st) {
+    const size_t g = (n + kBlock - 1) / kBlock;
+    if (g == 0) return hipSuccess;
+    LAUNCH(kFillRanges, g, r, n);
+}
+hipError_t launchReorder(const DevView& d, const uint32_t* ks, const uint32_t* vs, float4* xa, float4* va, float* kwa,
+                         float* kva, uint32_t* ida, hipStream_t st) {
+    LAUNCH(kReorder, d.numBlocks, d, ks, vs, xa, va, kwa, kva, ida);
+}
+hipError_t launchBuildNeighbors(const DevView& d, hipStream_t st) { LAUNCH(kBuildNeighbors, d.numBlocks, d); }
+hipError_t launchBoundaryVolume(const DevView& d, int scene, uint32_t b0, uint32_t b1, hipStream_t st) {
+    const uint32_t g = (b1 - b0 + kBlock - 1) / kBlock;
+    if (g == 0) return hipSuccess;
+    LAUNCH(kBoundaryVolume, g, d, scene, b0, b1);
+}
+hipError_t launchDensityAlpha(const DevView& d, int wa, hipStream_t st) { LAUNCH(kDensityAlpha, d.numBlocks, d, wa); }
+hipError_t launchDensityPressure(const DevView& d, hipStream_t st) { LAUNCH(kDensityPressure, d.numBlocks, d); }
+hipError_t launchForces(const DevView& d, int pressure, hipStream_t st) { LAUNCH(kForces, d.numBlocks, d, pressure); }
+hipError_t launchSurfaceTension(const DevView& d, int vmax, hipStream_t st) {
+    LAUNCH(kSurfaceTension, d.numBlocks, d, vmax);
+}
+hipError_t launchVmax(const DevView& d, hipStream_t st) { LAUNCH(kVmax, d.numBlocks, d); }
+hipError_t launchPredictVelocity(const DevView& d, hipStream_t st) { LAUNCH(kPredictVelocity, d.numBlocks, d); }
+hipError_t launchRhoStar(const DevView& d, int warm, hipStream_t st) { LAUNCH(kRhoStar, d.numBlocks, d, warm); }
+hipError_t launchVelocity(const DevView& d, const float* kb, hipStream_t st) { LAUNCH(kVelocity, d.numBlocks, d, kb); }
+hipError_t launchDivergence(const DevView& d, int warm, hipStream_t st) { LAUNCH(kDivergence, d.numBlocks, d, warm); }
+hipError_t launchScale(const DevView& d, float* buf, hipStream_t st) { LAUNCH(kScale, d.numBlocks, d, buf); }
+hipError_t launchIntegrate(const DevView& d, hipStream_t st) { LAUNCH(kIntegrate, d.numBlocks, d); }
+hipError_t launchPciInit(const DevView& d, hipStream_t st) { LAUNCH(kPciInit, d.numBlocks, d); }
+hipError_t launchPciDensity(const DevView& d, hipStream_t st) {
+    hipLaunchKernelGGL(kPciDensity, dim3(d.numBlocks), dim3(kBlock), 0, st, d);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    LAUNCH(kPciMaxErr, d.numBlocks, d);
+}
+hipError_t launchPciPressureAcc(const DevView& d, hipStream_t st) { LAUNCH(kPciPressureAcc, d.numBlocks, d); }
+hipError_t launchPciIntegrate(const DevView& d, hipStream_t st) { LAUNCH(kPciIntegrate, d.numBlocks, d); }
+hipError_t launchCellsOut(const DevView& d, int32_t* cells, hipStream_t st) { LAUNCH(kCellsOut, d.numBlocks, d, cells); }
+hipError_t launchCheckFinite(const DevView& d, hipStream_t st) { LAUNCH(kCheckFinite, d.numBlocks, d); }
+
+}  // namespace sph
+
+#include <hipcub/hipcub.hpp>
+
+namespace sph {
+hipError_t sortPairs(void* temp, size_t& tempBytes, uint32_t* k0, uint32_t* k1, uint32_t* v0, uint32_t* v1, int n,
+                     int bits, hipStream_t st, bool* swapped) {
+    hipcub::DoubleBuffer<uint32_t> kb(k0, k1), vb(v0, v1);
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, tempBytes, kb, vb, n, 0, bits, st);
+    if (swapped) *swapped = (kb.Current() == k1);
+    return e;
+}
+}  // namespace sph

@@ -1,0 +1,88 @@
+// Launch wrappers of the gfx950 kernels (sph_kernels.hip). Every wrapper
+// enqueues on `st` and returns hipSuccess or the launch error.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sph_common.hpp"
+
+namespace sph {
+
+// Device views shared by the launches of one context.
+struct DevView {
+    const SceneConst* scenes;
+    const SceneDyn* dyn;
+    const int32_t* blockScene;  // scene of each fluid block
+    int32_t numScenes;
+    uint32_t numBlocks;         // fluid blocks (NP / kBlock)
+    uint32_t NP;                // padded fluid slots
+    // fluid state (sorted slots)
+    float4* x;
+    float4* v;
+    float4* a;
+    float4* n;     // surface-tension normals
+    float4* xs;    // PCISPH predicted positions
+    float4* ap;    // PCISPH pressure acceleration
+    float* rho;
+    float* alpha;
+    float* p;
+    float* k;      // k / kv of the current iteration
+    float* kws;    // density warm-start accumulator
+    float* kvws;   // divergence warm-start accumulator
+    float* aux;    // predicted density / divergence error
+    uint32_t* id;  // original particle index of each slot
+    // boundary (sorted), w = density0 * volume
+    float4* xb;
+    float* vb;
+    // grid
+    uint2* fRange;
+    const uint2* bRange;
+    // neighbour lists, column-major: list[slot * NP + i]
+    uint32_t* nbrF;
+    uint32_t* cntF;
+    uint32_t* nbrB;
+    uint32_t* cntB;
+    uint32_t capF, capB;
+    // reductions and flags
+    double* partials;   // numBlocks * 2
+    uint32_t* counter;
+    double* red;        // numScenes * kRedSlots
+    int32_t* flags;     // numScenes * kFlagSlots
+};
+
+hipError_t launchKeys(const DevView& d, int dfsphAdvance, uint32_t* keys, uint32_t* vals, hipStream_t st);
+hipError_t launchFillRanges(uint2* r, size_t n, hipStream_t st);
+hipError_t launchReorder(const DevView& d, const uint32_t* keysSorted, const uint32_t* valsSorted, float4* xAlt,
+                         float4* vAlt, float* kwsAlt, float* kvwsAlt, uint32_t* idAlt, hipStream_t st);
+hipError_t launchBuildNeighbors(const DevView& d, hipStream_t st);
+hipError_t launchBoundaryVolume(const DevView& d, int scene, uint32_t bBegin, uint32_t bEnd, hipStream_t st);
+
+hipError_t launchDensityAlpha(const DevView& d, int withAlpha, hipStream_t st);
+hipError_t launchDensityPressure(const DevView& d, hipStream_t st);
+hipError_t launchForces(const DevView& d, int pressure, hipStream_t st);
+hipError_t launchSurfaceTension(const DevView& d, int vmax, hipStream_t st);
+hipError_t launchVmax(const DevView& d, hipStream_t st);
+hipError_t launchPredictVelocity(const DevView& d, hipStream_t st);
+hipError_t launchRhoStar(const DevView& d, int warm, hipStream_t st);
+hipError_t launchVelocity(const DevView& d, const float* kbuf, hipStream_t st);
+hipError_t launchDivergence(const DevView& d, int warm, hipStream_t st);
+hipError_t launchScale(const DevView& d, float* buf, hipStream_t st);
+hipError_t launchIntegrate(const DevView& d, hipStream_t st);
+
+hipError_t launchPciInit(const DevView& d, hipStream_t st);
+hipError_t launchPciDensity(const DevView& d, hipStream_t st);
+hipError_t launchPciPressureAcc(const DevView& d, hipStream_t st);
+hipError_t launchPciIntegrate(const DevView& d, hipStream_t st);
+
+hipError_t launchCellsOut(const DevView& d, int32_t* cells, hipStream_t st);
+hipError_t launchCheckFinite(const DevView& d, hipStream_t st);
+
+}  // namespace sph
+
+namespace sph {
+// Stable LSD radix sort of (key, value) pairs over the low `bits` key bits
+// (rocPRIM's onesweep sort through hipCUB). On return *swapped tells whether
+// the sorted data is in the alternate buffers (k1, v1).
+hipError_t sortPairs(void* temp, size_t& tempBytes, uint32_t* k0, uint32_t* k1, uint32_t* v0, uint32_t* v1, int n,
+                     int bits, hipStream_t st, bool* swapped);
+}  // namespace sph

@@ -1,0 +1,254 @@
+"""Host-side mirror of the reference's simulation interface over the C ABI.
+
+:class:`SPHSimulation` keeps the method names and meaning of the reference's
+``SPHSimulation`` (inc/SPH2O/SPHSimulation.h:208-365, SPHSimulation.cpp):
+``importScene`` / ``init`` / ``step`` / ``run`` and the getters, with the
+same time-window rule in ``step()`` (SPHSimulation.cpp:157-191). Each call
+goes through ``libsph2o.so``; there is no CPU fallback. Unlike the reference
+(a process-wide singleton with static kernel coefficients) several instances
+may coexist, and one instance may hold many scenes (batched stepping).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import SPHError, SceneParams, StepStats
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if a is not None else None
+
+
+class Context:
+    """One sph_ctx: scenes on one GPU and one HIP stream."""
+
+    def __init__(self, device=0, stream=None):
+        self.L = _lib.load()
+        h = ctypes.c_void_p()
+        rc = self.L.sph_create(ctypes.byref(h), int(device), ctypes.c_void_p(stream) if stream else None)
+        if rc != 0:
+            raise SPHError(rc, "sph_create failed (no HIP device?)")
+        self.h = h
+        self.initialized = False
+
+    def _check(self, rc):
+        if rc < 0:
+            raise SPHError(rc, self.L.sph_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.sph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- scenes
+    def load_scene_json(self, path):
+        return self._check(self.L.sph_load_scene_json(self.h, str(path).encode()))
+
+    def add_scene(self, params, x, v=None, xb=None):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+        v = None if v is None else np.ascontiguousarray(v, np.float32).reshape(-1, 3)
+        xb = np.zeros((0, 3), np.float32) if xb is None else np.ascontiguousarray(xb, np.float32).reshape(-1, 3)
+        return self._check(self.L.sph_add_scene(self.h, ctypes.byref(params), len(x), _fp(x), _fp(v), len(xb),
+                                                _fp(xb) if len(xb) else None))
+
+    def num_scenes(self):
+        return self.L.sph_num_scenes(self.h)
+
+    def params(self, scene=0):
+        p = SceneParams()
+        self._check(self.L.sph_get_params(self.h, scene, ctypes.byref(p)))
+        return p
+
+    def num_fluid(self, scene=0):
+        return self._check(self.L.sph_num_fluid(self.h, scene))
+
+    def num_boundary(self, scene=0):
+        return self._check(self.L.sph_num_boundary(self.h, scene))
+
+    # -- lifecycle
+    def init(self):
+        self._check(self.L.sph_init(self.h))
+        self.initialized = True
+
+    def step(self, nsteps=1):
+        ns = self.num_scenes()
+        st = (StepStats * (max(1, nsteps) * ns))()
+        self._check(self.L.sph_step(self.h, int(nsteps), st))
+        return [[st[k * ns + s] for s in range(ns)] for k in range(nsteps)]
+
+    def synchronize(self):
+        self._check(self.L.sph_synchronize(self.h))
+
+    # -- state
+    def set_fluid_state(self, scene, x=None, v=None):
+        n = self.num_fluid(scene)
+        x = None if x is None else np.ascontiguousarray(x, np.float32).reshape(n, 3)
+        v = None if v is None else np.ascontiguousarray(v, np.float32).reshape(n, 3)
+        self._check(self.L.sph_set_fluid_state(self.h, scene, n, _fp(x), _fp(v)))
+
+    def fluid_state(self, scene=0):
+        n = self.num_fluid(scene)
+        x = np.zeros((n, 3), np.float32)
+        v = np.zeros((n, 3), np.float32)
+        rho = np.zeros(n, np.float32)
+        p = np.zeros(n, np.float32)
+        self._check(self.L.sph_get_fluid_state(self.h, scene, _fp(x), _fp(v), _fp(rho), _fp(p)))
+        return dict(x=x, v=v, rho=rho, p=p)
+
+    def boundary(self, scene=0, volumes=True):
+        nb = self.num_boundary(scene)
+        xb = np.zeros((nb, 3), np.float32)
+        vol = np.zeros(nb, np.float32) if volumes else None
+        self._check(self.L.sph_get_boundary(self.h, scene, _fp(xb) if nb else None, _fp(vol) if (volumes and nb) else None))
+        return xb, vol
+
+    FIELDS = {"alpha": (0, 1), "a": (1, 3), "aux": (2, 1), "k_ws": (3, 1), "kv_ws": (4, 1), "normal": (5, 3)}
+
+    def field(self, name, scene=0):
+        fid, comp = self.FIELDS[name]
+        n = self.num_fluid(scene)
+        out = np.zeros((n, comp) if comp > 1 else n, np.float32)
+        self._check(self.L.sph_get_field(self.h, scene, fid, _fp(out)))
+        return out
+
+    def time(self, scene=0):
+        t = ctypes.c_double()
+        dt = ctypes.c_double()
+        self._check(self.L.sph_get_time(self.h, scene, ctypes.byref(t), ctypes.byref(dt)))
+        return t.value, dt.value
+
+    def cells(self, scene=0):
+        n = self.num_fluid(scene)
+        out = np.zeros((n, 3), np.int32)
+        self._check(self.L.sph_dump_cells(self.h, scene, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return out
+
+    def neighbors(self, scene=0):
+        """Within-support sets (CSR, original ids, unsorted rows)."""
+        n = self.num_fluid(scene)
+        off = np.zeros(n + 1, np.int64)
+        boff = np.zeros(n + 1, np.int64)
+        cap, bcap = 48 * n + 1, 32 * n + 1
+        P64 = ctypes.POINTER(ctypes.c_int64)
+        P32 = ctypes.POINTER(ctypes.c_int32)
+        while True:
+            ids = np.zeros(cap, np.int32)
+            bids = np.zeros(bcap, np.int32)
+            rc = self.L.sph_dump_neighbors(self.h, scene, off.ctypes.data_as(P64), ids.ctypes.data_as(P32), cap,
+                                           boff.ctypes.data_as(P64), bids.ctypes.data_as(P32), bcap)
+            if rc == _lib.SPH_E_CAPACITY:
+                cap, bcap = int(off[-1]) + 1, int(boff[-1]) + 1
+                continue
+            self._check(rc)
+            return off, ids[: off[-1]], boff, bids[: boff[-1]]
+
+    # -- timing
+    def last_step_ms(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self._check(self.L.sph_last_step_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def set_profiling(self, on):
+        self._check(self.L.sph_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_stats(self):
+        out = {}
+        for k in range(self.L.sph_kernel_count()):
+            ms = ctypes.c_double()
+            n = ctypes.c_int64()
+            self._check(self.L.sph_kernel_stats(self.h, k, ctypes.byref(ms), ctypes.byref(n)))
+            if n.value:
+                out[self.L.sph_kernel_name(k).decode()] = (ms.value, n.value)
+        return out
+
+    def reset_kernel_stats(self):
+        self._check(self.L.sph_reset_kernel_stats(self.h))
+
+
+class SPHSimulation:
+    """Drop-in for the reference's ``SPHSimulation`` on one scene
+    (SPHSimulation.cpp:117-199): ``importScene(path)``, ``init()``,
+    ``step()`` (advances only while startTime <= t <= endTime and returns
+    whether a frame is due, TimeHelper.h:100-110), ``run()`` (which, unlike
+    SPHSimulation.cpp:193-199, re-reads the time each iteration so it ends)."""
+
+    def __init__(self, device=0):
+        self.ctx = Context(device)
+        self.scene = None
+        self.frame = 1
+        self.last_save = 0.0
+
+    def importScene(self, path):
+        try:
+            self.scene = self.ctx.load_scene_json(path)
+        except SPHError:
+            return False
+        return True
+
+    def init(self):
+        self.ctx.init()
+        self.frame = 1
+        self.last_save = 0.0
+
+    def getTime(self):
+        return self.ctx.time(self.scene)[0]
+
+    def getTimeStep(self):
+        return self.ctx.time(self.scene)[1]
+
+    def getStartTime(self):
+        return self.ctx.params(self.scene).start_time
+
+    def getEndTime(self):
+        return self.ctx.params(self.scene).end_time
+
+    def getParticleRadius(self):
+        return self.ctx.params(self.scene).particle_radius
+
+    def getSupportRadius(self):
+        return np.float32(4.0) * np.float32(self.getParticleRadius())
+
+    def getFrame(self):
+        return self.frame
+
+    def step(self):
+        t = np.float32(self.getTime())
+        p = self.ctx.params(self.scene)
+        if np.float32(p.start_time) <= t <= np.float32(p.end_time):
+            self.last_stats = self.ctx.step(1)[0][self.scene]
+            t = np.float32(self.getTime())
+            if t - np.float32(self.last_save) >= 1.0 / np.float32(p.fps):
+                self.last_save = np.float32(self.frame) / np.float32(p.fps)
+                self.frame += 1
+                return True
+        return False
+
+    def run(self, max_steps=None):
+        n = 0
+        p = self.ctx.params(self.scene)
+        while np.float32(p.start_time) <= np.float32(self.getTime()) <= np.float32(p.end_time):
+            self.step()
+            n += 1
+            if max_steps is not None and n >= max_steps:
+                break
+        return n
+
+    def getPositions(self):
+        """Positions as vec4 with w = 1 (SimulationObject::getPositions, GUI instancing)."""
+        x = self.ctx.fluid_state(self.scene)["x"]
+        return np.concatenate([x, np.ones((len(x), 1), np.float32)], axis=1)
+
+    def getVelocities(self):
+        return self.ctx.fluid_state(self.scene)["v"]
+
+    def getDensities(self):
+        return self.ctx.fluid_state(self.scene)["rho"]

@@ -1,0 +1,166 @@
+"""Record golden vectors from the compiled reference (oracle/_ref/sph2o_ref).
+
+Runs only where the reference tree and its headless build exist (this
+container): ``make -C oracle ref && python tests/golden/make_golden.py``.
+Writes small .npz fixtures next to this script; the tests read only those
+fixtures, never the reference itself.
+
+Fixtures:
+  ref_cell_keys.npz      probe positions (incl. values within a few ulps of a
+                         cell face) and the reference's int3 cell keys
+  ref_support_pairs.npz  particle pairs at |r| ~ h whose membership depends on
+                         how |r| is rounded, and the reference's answer
+  ref_neighbors_*.npz    within-support neighbour sets (fluid and boundary) of
+                         a perfect and a jittered lattice
+  ref_traj_*.npz         initial state, boundary, Akinci volumes, final state
+                         and per-step stats of short runs of each solver
+  ref_example_scene.npz  the reference's own example scene
+                         (data/example_scene/example_scene.json): sampled
+                         fluid/boundary positions, volumes, one DFSPH step
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.abspath(os.path.join(HERE, "..", "..")))
+
+from oracle import sphref as S  # noqa: E402
+from sph2o_amd import scenes as SC  # noqa: E402
+
+f32 = np.float32
+
+
+def save(name, **arrays):
+    path = os.path.join(HERE, name)
+    np.savez_compressed(path, **arrays)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def scene_json(sc):
+    return np.array(json.dumps(sc))
+
+
+def cell_keys():
+    sc = SC.small(2, n=10)
+    base = S.run_reference(sc, 0)
+    n = len(base["x0"])
+    h = f32(4) * f32(0.01)
+    rng = np.random.default_rng(11)
+    pts = []
+    # coordinates within a few ulps of cell faces k*h, on every axis
+    for k in range(1, 10):
+        face = f32(k) * h
+        for d in range(-6, 7):
+            v = face
+            for _ in range(abs(d)):
+                v = np.nextafter(v, f32(np.inf) if d > 0 else f32(-np.inf))
+            pts.append(v)
+    pts = np.array(pts, f32)
+    x = np.zeros((n, 3), f32)
+    m = len(pts)
+    x[:m, 0] = pts
+    x[:m, 1] = rng.uniform(0.05, 0.25, m)
+    x[:m, 2] = rng.uniform(0.05, 0.35, m)
+    x[m:2 * m, 1] = pts
+    x[m:2 * m, 0] = rng.uniform(0.05, 0.55, m)
+    x[m:2 * m, 2] = rng.uniform(0.05, 0.35, m)
+    x[2 * m:3 * m, 2] = pts
+    x[2 * m:3 * m, 0] = rng.uniform(0.05, 0.55, m)
+    x[2 * m:3 * m, 1] = rng.uniform(0.05, 0.25, m)
+    rest = n - 3 * m
+    x[3 * m:] = rng.uniform([0.0, 0.0, 0.0], [0.6, 0.3, 0.4], (rest, 3)).astype(f32)
+    out = S.run_reference(sc, 0, x=x, neighbors=True)
+    save("ref_cell_keys.npz", scene=scene_json(sc), x=x, keys=out["keys"])
+
+
+def support_pairs():
+    """Pairs whose |r| <= h outcome differs between plausible roundings."""
+    from fractions import Fraction
+
+    def fmaf(a, b, c):
+        return f32(float(Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c))))
+
+    h = f32(4) * f32(0.01)
+    sc = SC.small(2, n=10)
+    base = S.run_reference(sc, 0)
+    n = len(base["x0"])
+    rng = np.random.default_rng(2)
+    pairs = []
+    while len(pairs) < n // 2:
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        k = len(pairs)
+        c = np.array([0.03 + 0.054 * (k % 10), 0.03 + 0.024 * ((k // 10) % 10), 0.03 + 0.068 * (k // 100)], f32)
+        b = (c + d * float(h)).astype(f32)
+        for ax in range(3):
+            b[ax] = np.nextafter(b[ax], f32(np.inf) if rng.random() < 0.5 else f32(-np.inf))
+        r = (c - b).astype(f32)
+        plain = np.sqrt(f32(f32(r[0] * r[0] + r[1] * r[1]) + r[2] * r[2])) <= h
+        fz = np.sqrt(fmaf(r[2], r[2], fmaf(r[1], r[1], f32(r[0] * r[0])))) <= h
+        fx = np.sqrt(fmaf(r[0], r[0], fmaf(r[2], r[2], f32(r[1] * r[1])))) <= h
+        if len({bool(plain), bool(fz), bool(fx)}) > 1:
+            pairs.append((c, b))
+    x = base["x0"].copy()
+    for k, (a, b) in enumerate(pairs):
+        x[2 * k] = a
+        x[2 * k + 1] = b
+    out = S.run_reference(sc, 0, x=x, neighbors=True)
+    off, ids = out["nbr_off"], out["nbr_ids"]
+    member = np.array([(2 * k + 1) in set(ids[off[2 * k]:off[2 * k + 1]].tolist()) for k in range(len(pairs))])
+    a = np.array([p[0] for p in pairs], f32)
+    b = np.array([p[1] for p in pairs], f32)
+    save("ref_support_pairs.npz", a=a, b=b, h=np.array(h), member=member)
+
+
+def neighbors(tag, jit):
+    sc = SC.small(2, n=10, surface_tension_method=3, surface_tension=0.1)
+    base = S.run_reference(sc, 0)
+    x = base["x0"] if jit == 0 else SC.jitter(base["x0"], jit * 0.02, 7)
+    out = S.run_reference(sc, 0, x=x, neighbors=True)
+    save("ref_neighbors_%s.npz" % tag, scene=scene_json(sc), x=x, bx=out["bx"], keys=out["keys"],
+         off=out["nbr_off"], ids=out["nbr_ids"], boff=out["bnbr_off"], bids=out["bnbr_ids"])
+
+
+def trajectory(tag, sc, nsteps, jit=0.01, seed=1):
+    base = S.run_reference(sc, 0)
+    x0 = SC.jitter(base["x0"], jit * 2 * sc["Configuration"]["particleRadius"], seed)
+    out = S.run_reference(sc, nsteps, x=x0)
+    save("ref_traj_%s.npz" % tag, scene=scene_json(sc), nsteps=np.array(nsteps), x0=x0, bx=out["bx"],
+         bvol=out["bvol"], rho0=out["rho0"], alpha0=out["alpha0"], x=out["x"], v=out["v"], rho=out["rho"],
+         alpha=out["alpha"], stats=out["stats"], meta=out["meta"])
+
+
+def example_scene():
+    path = "/root/reference/data/example_scene/example_scene.json"
+    with open(path) as f:
+        sc = json.load(f)
+    out = S.run_reference(sc, 1)
+    save("ref_example_scene.npz", scene=scene_json(sc), x0=out["x0"], bx=out["bx"], bvol=out["bvol"],
+         rho0=out["rho0"], alpha0=out["alpha0"], x=out["x"], v=out["v"], rho=out["rho"], alpha=out["alpha"],
+         stats=out["stats"])
+
+
+def main():
+    if not S.reference_available():
+        raise SystemExit("compiled reference missing: make -C oracle ref")
+    cell_keys()
+    support_pairs()
+    neighbors("lattice", 0.0)
+    neighbors("jitter", 0.01)
+    visc = dict(viscosity_method=2, viscosity=0.01, surface_tension_method=3, surface_tension=0.1)
+    trajectory("wcsph20", SC.small(0, **visc), 20)
+    trajectory("pcisph20", SC.small(1, viscosity_method=2, viscosity=0.01), 20)
+    trajectory("dfsph1", SC.small(2, **visc), 1)
+    trajectory("dfsph5", SC.small(2, **visc), 5)
+    trajectory("dfsph40", SC.small(2, **visc), 40)
+    trajectory("dfsph20_art", SC.small(2, viscosity_method=1, viscosity=0.03, boundary_viscosity=0.002,
+                                       surface_tension_method=3, surface_tension=0.15), 20)
+    trajectory("wcsph20_std", SC.small(0, viscosity_method=0, viscosity=0.5, boundary_viscosity=0.1), 20)
+    example_scene()
+
+
+if __name__ == "__main__":
+    main()

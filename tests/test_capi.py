@@ -1,0 +1,117 @@
+"""The C-ABI library loads, exports every symbol include/sph2o.h declares,
+and its GPU-free scene reader matches the reference schema (CPU only)."""
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, golden
+from sph2o_amd import _lib, scenes as SC
+
+
+def _declared():
+    with open(os.path.join(ROOT, "include", "sph2o.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sph_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    declared = _declared()
+    assert len(declared) >= 25
+    for name in declared:
+        assert hasattr(L, name), name
+    assert set(declared) == set(_lib.EXPORTS)
+
+
+def test_version_and_kernel_names_need_no_gpu():
+    L = _lib.load()
+    assert L.sph_version().startswith(b"sph2o")
+    names = [L.sph_kernel_name(k).decode() for k in range(L.sph_kernel_count())]
+    assert "build_neighbors" in names and "velocity" in names
+
+
+def test_scene_reader_example_scene_matches_reference_sampling():
+    g = golden("ref_example_scene.npz")
+    sc = json.loads(str(g["scene"]))
+    path = os.path.join(ROOT, "tests", "golden", "_example_scene.json")
+    try:
+        SC.write_json(sc, path)
+        p, x, xb = _lib.scene_from_json(path)
+    finally:
+        if os.path.exists(path):
+            os.remove(path)
+    assert p.simulation_method == 2 and p.boundary_method == 2
+    assert p.viscosity_method == 1 and abs(p.viscosity - 0.03) < 1e-7
+    assert p.surface_tension_method == 3 and abs(p.surface_tension - 0.15) < 1e-7
+    assert p.min_iterations == 2 and p.max_iterations == 100 and p.min_iterations_v == 1
+    assert abs(p.eta - np.float32(0.0005000000237487257)) == 0
+    assert x.shape == g["x0"].shape and xb.shape == g["bx"].shape
+    assert np.abs(x - g["x0"]).max() <= 1e-7
+    assert np.abs(xb - g["bx"]).max() <= 1e-7
+
+
+def _write(tmp, sc):
+    p = os.path.join(tmp, "scene.json")
+    with open(p, "w") as f:
+        json.dump(sc, f)
+    return p
+
+
+def test_scene_reader_errors(tmp_path):
+    tmp = str(tmp_path)
+    base = SC.small(2)
+    # missing key -> SPH_E_SCENE, not an abort (SceneLoader.cpp:44 would assert)
+    sc = json.loads(json.dumps(base))
+    del sc["Configuration"]["etaV"]
+    with pytest.raises(_lib.SPHError) as e:
+        _lib.scene_from_json(_write(tmp, sc))
+    assert e.value.code == _lib.SPH_E_SCENE
+    # ill-typed key
+    sc = json.loads(json.dumps(base))
+    sc["Configuration"]["particleRadius"] = "big"
+    with pytest.raises(_lib.SPHError) as e:
+        _lib.scene_from_json(_write(tmp, sc))
+    assert e.value.code == _lib.SPH_E_SCENE
+    # adhesion/emitters/multi-fluid are outside the hot path
+    sc = json.loads(json.dumps(base))
+    sc["Fluid"]["FluidModel"].append(sc["Fluid"]["FluidModel"][0])
+    with pytest.raises(_lib.SPHError) as e:
+        _lib.scene_from_json(_write(tmp, sc))
+    assert e.value.code == _lib.SPH_E_UNSUPPORTED
+    # unreadable file
+    with pytest.raises(_lib.SPHError) as e:
+        _lib.scene_from_json(os.path.join(tmp, "missing.json"))
+    assert e.value.code == _lib.SPH_E_SCENE
+    # truncated JSON
+    with open(os.path.join(tmp, "bad.json"), "w") as f:
+        f.write('{"Configuration": {')
+    with pytest.raises(_lib.SPHError) as e:
+        _lib.scene_from_json(os.path.join(tmp, "bad.json"))
+    assert e.value.code == _lib.SPH_E_SCENE
+
+
+def test_scene_reader_conditional_keys(tmp_path):
+    # WCSPH needs stiffness/gamma but not eta (SceneLoader.cpp:58-77); absent sphere/geometry arrays are empty
+    sc = SC.small(0)
+    del sc["Boundary"][0]["sphere"]
+    del sc["Boundary"][0]["geometry"]
+    p, x, xb = _lib.scene_from_json(_write(str(tmp_path), sc))
+    assert p.simulation_method == 0 and p.stiffness == 1000.0 and p.gamma == 7.0
+    assert len(x) == 1000 and len(xb) > 0
+
+
+def test_scene_workload_sizes():
+    sc, x, v = SC.w8k()
+    assert x.shape == (8000, 3)
+    sc, _, _ = SC.d1m(100)
+    blk = sc["Fluid"]["FluidModel"][0]["fluidBlocks"][0]
+    assert blk["max"][0] == pytest.approx(1.0)
+    batch = SC.b64()
+    assert len(batch) == 64
+    # SplitMix64 is a fixed function of the seed
+    assert np.array_equal(SC.splitmix64(6, 10), SC.splitmix64(6, 10))
+    assert abs(SC.splitmix64(1, 100000).mean() - 0.5) < 0.01

@@ -1,0 +1,269 @@
+"""Parity of the HIP path (through the C ABI) with the compiled reference's
+golden vectors and with the FP64 oracle.
+
+Tolerances (FP32 GPU vs FP64 oracle / FP32 reference), stated per check:
+  cell keys, neighbour sets (fluid and boundary): exact
+  Akinci volumes, density, alpha from identical input: 2e-6 relative
+  WCSPH / PCISPH trajectories (20 steps): |dx| <= 1e-6 m, rms(dv)/rms(v) <= 2e-3,
+      |drho|/rho0 <= 1e-5, PCISPH iterations per step equal
+  DFSPH one step: iterations equal, rms(dv)/rms(v) <= 1e-3
+  DFSPH 40 steps: iteration totals within 5%, mean density error within 20%,
+      kinetic energy within 5%, centre of mass within 1e-4 m (per-particle
+      states decorrelate through the reference's |k| > 1e-5 gates)
+  batched scenes: bit-identical to the same scenes stepped alone
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import golden
+from oracle import sphref as S
+from sph2o_amd import _lib, scenes as SC
+from sph2o_amd.simulation import Context
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(sc, tmp):
+    path = os.path.join(tmp, "scene.json")
+    SC.write_json(sc, path)
+    p, x, xb = _lib.scene_from_json(path)
+    return p, x, xb
+
+
+def _ctx(sc, tmp, x=None, v=None, xb=None):
+    p, x0, xb0 = _params(sc, tmp)
+    ctx = Context(0)
+    sid = ctx.add_scene(p, x0 if x is None else x, v, xb0 if xb is None else xb)
+    ctx.init()
+    return ctx, sid
+
+
+def _sorted_rows(off, ids):
+    return [np.sort(ids[off[i]:off[i + 1]]) for i in range(len(off) - 1)]
+
+
+def _rms_rel(a, b):
+    d = a - b
+    return float(np.sqrt((d ** 2).mean() / max(1e-30, (b ** 2).mean())))
+
+
+# ---------------------------------------------------------------- neighbour search
+def test_gpu_cell_keys_match_reference(tmp_path):
+    g = golden("ref_cell_keys.npz")
+    sc = json.loads(str(g["scene"]))
+    ctx, sid = _ctx(sc, str(tmp_path), x=g["x"])
+    assert np.array_equal(ctx.cells(sid), g["keys"])
+
+
+@pytest.mark.parametrize("tag", ["lattice", "jitter"])
+def test_gpu_neighbor_sets_match_reference(tmp_path, tag):
+    g = golden("ref_neighbors_%s.npz" % tag)
+    sc = json.loads(str(g["scene"]))
+    ctx, sid = _ctx(sc, str(tmp_path), x=g["x"], xb=g["bx"])
+    off, ids, boff, bids = ctx.neighbors(sid)
+    assert np.array_equal(off, g["off"]) and np.array_equal(boff, g["boff"])
+    for a, b in zip(_sorted_rows(off, ids), _sorted_rows(g["off"], g["ids"])):
+        assert np.array_equal(a, b)
+    for a, b in zip(_sorted_rows(boff, bids), _sorted_rows(g["boff"], g["bids"])):
+        assert np.array_equal(a, b)
+    assert np.array_equal(ctx.cells(sid), g["keys"])
+
+
+def test_gpu_support_test_matches_reference_pairs(tmp_path):
+    g = golden("ref_support_pairs.npz")
+    a, b = g["a"], g["b"]
+    x = np.empty((2 * len(a), 3), np.float32)
+    x[0::2] = a
+    x[1::2] = b
+    sc = SC.small(2)
+    ctx, sid = _ctx(sc, str(tmp_path), x=x)
+    off, ids, _, _ = ctx.neighbors(sid)
+    member = np.array([(2 * k + 1) in set(ids[off[2 * k]:off[2 * k + 1]].tolist()) for k in range(len(a))])
+    assert np.array_equal(member, g["member"])
+
+
+def test_gpu_neighbor_sets_match_oracle_dense_and_far(tmp_path):
+    # a 1.5x-dense jittered cluster (lists grow past their first capacity) plus a
+    # particle far outside the box moving away (the grid must grow); sets stay exact
+    r = 0.01
+    s = 2 * r
+    x = SC.jitter(SC.lattice((0.05, 0.05, 0.05), (14, 8, 14), s / 1.5 ** (1 / 3) / 1.3, 0.0), 0.2 * s, 9)
+    x = np.concatenate([x, np.array([[2.5, 0.15, 0.2]], np.float32)])
+    v = np.zeros_like(x)
+    v[-1] = (40.0, 0.0, 0.0)
+    sc = SC.small(0)
+    p, _, xb = _params(sc, str(tmp_path))
+    ctx = Context(0)
+    sid = ctx.add_scene(p, x, v, xb)
+    ctx.init()
+    ctx.step(3)
+    st = ctx.fluid_state(sid)
+    o = S.Oracle(S.params_from_scene(sc), st["x"], None, xb, bits=32)
+    off, ids, boff, bids = ctx.neighbors(sid)
+    ooff, oids, oboff, obids = o.neighbors()
+    assert np.array_equal(off, ooff) and np.array_equal(boff, oboff)
+    for a_, b_ in zip(_sorted_rows(off, ids), _sorted_rows(ooff, oids)):
+        assert np.array_equal(a_, b_)
+    assert np.array_equal(ctx.cells(sid), o.cell_keys())
+    assert (np.diff(off)).max() > 64
+
+
+# ---------------------------------------------------------------- single passes
+def test_gpu_boundary_volume_density_alpha(tmp_path):
+    g = golden("ref_traj_dfsph1.npz")
+    sc = json.loads(str(g["scene"]))
+    ctx, sid = _ctx(sc, str(tmp_path), x=g["x0"], xb=g["bx"])
+    _, vol = ctx.boundary(sid)
+    assert np.abs(vol - g["bvol"]).max() / np.abs(g["bvol"]).max() < 2e-6
+    st = ctx.fluid_state(sid)
+    alpha = ctx.field("alpha", sid)
+    assert np.abs(st["rho"] - g["rho0"]).max() / 1000.0 < 2e-6
+    assert np.abs(alpha - g["alpha0"]).max() / np.abs(g["alpha0"]).max() < 2e-6
+    o = S.Oracle(S.params_from_scene(sc), g["x0"], None, g["bx"], bits=64)
+    o.init()
+    so = o.state()
+    assert np.abs(st["rho"] - so["rho"]).max() / 1000.0 < 2e-6
+    assert np.abs(alpha - so["alpha"]).max() / np.abs(so["alpha"]).max() < 2e-6
+    assert np.abs(vol - o.boundary_volumes()).max() / np.abs(vol).max() < 2e-6
+
+
+# ---------------------------------------------------------------- trajectories
+@pytest.mark.parametrize("tag", ["wcsph20", "wcsph20_std", "pcisph20"])
+def test_gpu_explicit_solvers_match(tmp_path, tag):
+    g = golden("ref_traj_%s.npz" % tag)
+    sc = json.loads(str(g["scene"]))
+    n = int(g["nsteps"])
+    ctx, sid = _ctx(sc, str(tmp_path), x=g["x0"], xb=g["bx"])
+    stats = ctx.step(n)
+    st = ctx.fluid_state(sid)
+    o = S.Oracle(S.params_from_scene(sc), g["x0"], None, g["bx"], bits=64)
+    o.init()
+    ost = o.step(n)
+    so = o.state()
+    for ref_x, ref_v, ref_rho in ((g["x"], g["v"], g["rho"]), (so["x"], so["v"], so["rho"])):
+        assert np.abs(st["x"] - ref_x).max() <= 1e-6
+        assert _rms_rel(st["v"], ref_v) <= 2e-3
+        assert np.abs(st["rho"] - ref_rho).max() / 1000.0 <= 1e-5
+    its = np.array([s[sid].iterations for s in stats])
+    assert np.array_equal(its, g["stats"][:, 2].astype(int))
+    assert np.array_equal(its, ost[:, 2].astype(int))
+
+
+def test_gpu_dfsph_one_step(tmp_path):
+    g = golden("ref_traj_dfsph1.npz")
+    sc = json.loads(str(g["scene"]))
+    ctx, sid = _ctx(sc, str(tmp_path), x=g["x0"], xb=g["bx"])
+    s = ctx.step(1)[0][sid]
+    st = ctx.fluid_state(sid)
+    assert (s.iterations, s.iterations_v) == tuple(g["stats"][0, 2:4].astype(int))
+    assert abs(s.dt - g["stats"][0, 1]) < 1e-12
+    assert _rms_rel(st["v"], g["v"]) <= 1e-3
+    o = S.Oracle(S.params_from_scene(sc), g["x0"], None, g["bx"], bits=64)
+    o.init()
+    o.step(1)
+    assert _rms_rel(st["v"], o.state()["v"]) <= 1e-3
+
+
+@pytest.mark.parametrize("tag", ["dfsph40", "dfsph20_art"])
+def test_gpu_dfsph_aggregates(tmp_path, tag):
+    g = golden("ref_traj_%s.npz" % tag)
+    sc = json.loads(str(g["scene"]))
+    n = int(g["nsteps"])
+    ctx, sid = _ctx(sc, str(tmp_path), x=g["x0"], xb=g["bx"])
+    stats = ctx.step(n)
+    st = ctx.fluid_state(sid)
+    ref = g["stats"]
+    its = np.array([[s[sid].iterations, s[sid].iterations_v] for s in stats])
+    assert abs(its[:, 0].sum() - ref[:, 2].sum()) <= 0.05 * ref[:, 2].sum() + 2
+    assert abs(its[:, 1].sum() - ref[:, 3].sum()) <= 0.05 * ref[:, 3].sum() + 2
+    assert abs(stats[-1][sid].t - ref[-1, 0]) / ref[-1, 0] < 0.05
+    e_ref = np.mean(np.maximum(g["rho"] / 1000 - 1, 0))
+    e_me = np.mean(np.maximum(st["rho"] / 1000 - 1, 0))
+    assert abs(e_me - e_ref) < 0.2 * e_ref + 1e-5
+    assert abs((st["v"] ** 2).sum() - (g["v"] ** 2).sum()) / (g["v"] ** 2).sum() < 0.05
+    assert np.abs(st["x"].mean(0) - g["x"].mean(0)).max() < 1e-4
+
+
+def test_gpu_reference_example_scene(tmp_path):
+    # the reference's own scene: artificial viscosity with boundary viscosity, surface tension, DFSPH
+    g = golden("ref_example_scene.npz")
+    sc = json.loads(str(g["scene"]))
+    ctx, sid = _ctx(sc, str(tmp_path), x=g["x0"], xb=g["bx"])
+    st0 = ctx.fluid_state(sid)
+    assert np.abs(st0["rho"] - g["rho0"]).max() / 1000.0 < 2e-6
+    s = ctx.step(1)[0][sid]
+    st = ctx.fluid_state(sid)
+    assert (s.iterations, s.iterations_v) == tuple(g["stats"][0, 2:4].astype(int))
+    assert _rms_rel(st["v"], g["v"]) <= 1e-3
+    assert np.abs(st["x"] - g["x"]).max() <= 1e-6
+
+
+# ---------------------------------------------------------------- batching, scale, edge cases
+def test_gpu_batched_scenes_bit_identical(tmp_path):
+    tmp = str(tmp_path)
+    defs = [SC.small(2, n=6, viscosity_method=2, viscosity=0.01),
+            SC.small(2, n=9, surface_tension_method=3, surface_tension=0.1, box=(0.5, 0.4, 0.5)),
+            SC.small(2, n=4, viscosity_method=2, viscosity=0.05, surface_tension_method=3, surface_tension=0.05)]
+    singles = []
+    for k, sc in enumerate(defs):
+        p, x, xb = _params(sc, tmp)
+        x = SC.jitter(x, 0.0002, 20 + k)
+        c = Context(0)
+        sid = c.add_scene(p, x, None, xb)
+        c.init()
+        c.step(6)
+        singles.append((c.fluid_state(sid), (p, x, xb)))
+    batch = Context(0)
+    for _, (p, x, xb) in singles:
+        batch.add_scene(p, x, None, xb)
+    batch.init()
+    batch.step(6)
+    for k, (ref, _) in enumerate(singles):
+        st = batch.fluid_state(k)
+        assert np.array_equal(st["x"], ref["x"]) and np.array_equal(st["v"], ref["v"])
+
+
+def test_gpu_scale_125k_properties(tmp_path):
+    sc, _, _ = SC.d1m(50)
+    ctx, sid = _ctx(sc, str(tmp_path))
+    off, ids, _, _ = ctx.neighbors(sid)
+    n = len(off) - 1
+    assert n == 125000
+    rows = off[1:] - off[:-1]
+    pairs = np.stack([np.repeat(np.arange(n), rows), ids], 1)
+    # symmetric sets that contain self
+    fwd = set(map(tuple, pairs[::97].tolist()))
+    allp = set(map(tuple, pairs.tolist()))
+    assert all((j, i) in allp for i, j in fwd)
+    assert all(i in set(ids[off[i]:off[i + 1]].tolist()) for i in range(0, n, 1013))
+    x = ctx.fluid_state(sid)["x"]
+    o = S.Oracle(S.params_from_scene(sc), x, None, np.zeros((0, 3), np.float32), bits=32)
+    assert np.array_equal(ctx.cells(sid), o.cell_keys())
+    stats = ctx.step(5)
+    st = ctx.fluid_state(sid)
+    assert np.isfinite(st["x"]).all() and np.isfinite(st["v"]).all()
+    assert np.mean(np.maximum(st["rho"] / 1000 - 1, 0)) < 1e-3
+    assert all(s[sid].iterations >= 2 for s in stats)
+
+
+def test_gpu_empty_fluid_scene_and_nonfinite(tmp_path):
+    tmp = str(tmp_path)
+    p, x, xb = _params(SC.small(0), tmp)
+    ctx = Context(0)
+    sid = ctx.add_scene(p, np.zeros((0, 3), np.float32), None, xb)
+    ctx.init()
+    ctx.step(2)
+    t, dt = ctx.time(sid)
+    assert abs(t - 2 * np.float32(1e-4)) < 1e-9
+    # a NaN velocity makes the positions non-finite: the next neighbour search reports it
+    ctx2 = Context(0)
+    v = np.zeros_like(x)
+    v[5] = np.nan
+    sid2 = ctx2.add_scene(p, x, v, xb)
+    ctx2.init()
+    with pytest.raises(_lib.SPHError) as e:
+        ctx2.step(3)
+    assert e.value.code == _lib.SPH_E_NONFINITE

@@ -1,0 +1,241 @@
+"""The oracle (oracle/sphref.hpp) pinned against closed forms, brute force and
+the golden vectors recorded from the compiled reference (CPU only)."""
+import json
+
+import numpy as np
+import pytest
+
+from conftest import golden
+from oracle import sphref as S
+from sph2o_amd import scenes as SC
+
+H = 0.04  # h = 4r at r = 0.01
+
+
+# ---------------------------------------------------------------- kernels
+@pytest.mark.parametrize("bits", [32, 64])
+def test_cubic_spline_normalisation_and_peak(bits):
+    # W(0) h^3 = 8/pi and the integral of W over its support is 1
+    w0 = S.kernel_eval(bits, H, 0, np.array([0.0]))[0]
+    assert abs(w0 * H ** 3 - 8 / np.pi) < (1e-6 if bits == 32 else 1e-12)
+    r = np.linspace(0.0, H, 40001)
+    w = S.kernel_eval(bits, H, 0, r)
+    integral = np.trapezoid(4 * np.pi * r ** 2 * w, r)
+    assert abs(integral - 1.0) < (2e-6 if bits == 32 else 1e-8)
+    # continuity at q = 1/2 and compact support at q = 1
+    eps = 1e-9 * H
+    a, b = S.kernel_eval(64, H, 0, np.array([H / 2 - eps, H / 2 + eps]))
+    assert abs(a - b) < 1e-6 * a
+    assert S.kernel_eval(64, H, 0, np.array([H]))[0] == 0.0
+    assert S.kernel_eval(64, H, 0, np.array([H * 1.0001]))[0] == 0.0
+
+
+def test_gradient_is_derivative_of_kernel():
+    # gradW(r x_hat).x = dW/dr: central differences of W, both branches
+    r = np.array([0.1, 0.3, 0.45, 0.55, 0.7, 0.95]) * H
+    d = 1e-7 * H
+    fd = (S.kernel_eval(64, H, 0, r + d) - S.kernel_eval(64, H, 0, r - d)) / (2 * d)
+    g = S.kernel_eval(64, H, 1, r)
+    assert np.allclose(g, fd, rtol=1e-6)
+    assert S.kernel_eval(64, H, 1, np.array([0.0]))[0] == 0.0  # self pair, CubicSpline.h:58
+
+
+def test_viscosity_laplacian_and_cohesion_closed_forms():
+    r = np.array([0.0, 0.25, 0.5, 0.75, 1.0, 1.2]) * H
+    lap = S.kernel_eval(64, H, 2, r)
+    assert np.allclose(lap, np.where(r <= H, 45 / (np.pi * H ** 6) * (H - r), 0.0), rtol=1e-12)
+    c = S.kernel_eval(64, H, 3, r)
+    c1 = 32 / np.pi / H ** 9
+    # r > h/2: c1 (h-r)^3 r^3; r <= h/2: c1 2 (h-r)^3 r^3 - h^6/64 (h^6/64 is not scaled, Cohesion.h:37)
+    assert abs(c[3] - c1 * (H / 4) ** 3 * (3 * H / 4) ** 3) < 1e-9 * c[3]
+    assert abs(c[1] - (c1 * 2 * (3 * H / 4) ** 3 * (H / 4) ** 3 - H ** 6 / 64)) < 1e-9 * abs(c[1])
+    assert c[0] == 0.0 and c[5] == 0.0
+    # the jump at r = h/2: the inner branch doubles the outer one
+    lo, hi = S.kernel_eval(64, H, 3, np.array([H / 2, H / 2 * (1 + 1e-12)]))
+    assert abs(lo / hi - 2.0) < 1e-6
+
+
+# ---------------------------------------------------------------- lattice closed forms
+def _interior_lattice(method, n=12):
+    # r = 2^-7 so that every lattice coordinate and h are exact in FP32
+    r = 0.0078125
+    s = 2 * r
+    x = SC.lattice((0, 0, 0), (n, n, n), s, r)
+    sc = SC.small(method, n=n, radius=r)
+    p = S.params_from_scene(sc)
+    p.boundary_method = -1
+    o = S.Oracle(p, x, None, np.zeros((0, 3), np.float32), bits=64)
+    return o, x, s
+
+
+def test_lattice_density_alpha_closed_form():
+    o, x, s = _interior_lattice(2)
+    o.init()
+    st = o.state()
+    c = np.argmin(np.linalg.norm(x - x.mean(0), axis=1))
+    # interior of a perfect lattice, spacing s = 2r, h = 2s: rho/rho0 = 0.9999724660910 (33 neighbours)
+    assert abs(st["rho"][c] / 1000.0 - 0.9999724660910) < 1e-11
+    # alpha = rho s^2 / (0.4248399544056 rho0^2): alpha rho0 / s^2 = 2.3537627658
+    assert abs(st["alpha"][c] * 1000.0 / s ** 2 - 2.3537627658) < 1e-8
+
+
+def test_pcisph_scaling_factor_closed_form():
+    o, x, s = _interior_lattice(1)
+    o.init()
+    sc = o.scalars()
+    dt = S.f32(5e-4)
+    # delta dt^2 / s^2 = 1 / (2 * 0.4248399544056) with the initial dt (PCISPHSolver.cpp:431)
+    assert abs(sc["pcisph_delta"] * dt ** 2 / s ** 2 - 1.1769137879) < 1e-9
+
+
+def test_boundary_volume_of_flat_wall():
+    # interior particle of a flat single-layer wall at spacing s: V = 1 / sum of W over the 2D lattice
+    r = 0.01
+    s = 2 * r
+    n = 15
+    g = (np.stack(np.meshgrid(np.arange(n), np.arange(n), indexing="ij"), -1).reshape(-1, 2) * s).astype(np.float64)
+    xb = np.concatenate([g, np.full((len(g), 1), 0.5)], 1).astype(np.float32)
+    p = S.params_from_scene(SC.small(2))
+    o = S.Oracle(p, np.zeros((0, 3), np.float32), None, xb, bits=64)
+    o.init()
+    vol = o.boundary_volumes()
+    c = np.argmin(np.linalg.norm(xb[:, :2] - xb[:, :2].mean(0), axis=1))
+    h = 4 * r
+    d = np.sqrt(((g - g[c]) ** 2).sum(1))
+    q = d / h
+    w = np.where(q <= 0.5, 6 * q ** 3 - 6 * q ** 2 + 1, np.where(q <= 1, 2 * (1 - q) ** 3, 0)) * 8 / (np.pi * h ** 3)
+    assert abs(vol[c] * w.sum() - 1.0) < 1e-6
+
+
+# ---------------------------------------------------------------- neighbour search
+def test_oracle_cell_keys_match_reference():
+    g = golden("ref_cell_keys.npz")
+    sc = json.loads(str(g["scene"]))
+    o = S.Oracle(S.params_from_scene(sc), g["x"], None, np.zeros((0, 3), np.float32), bits=32)
+    assert np.array_equal(o.cell_keys(), g["keys"])
+    # the IEEE key floor(x/h) differs on some of these probes: the fixture discriminates
+    h = np.float32(0.04)
+    ieee = np.floor(g["x"] / h).astype(np.int32)
+    assert not np.array_equal(ieee, g["keys"])
+
+
+def test_oracle_support_test_matches_reference_pairs():
+    g = golden("ref_support_pairs.npz")
+    h = np.float32(g["h"])
+    a, b = g["a"], g["b"]
+    # the FP32 oracle's test, evaluated through a 2-particle oracle per pair, must agree on every pair
+    p = S.params_from_scene(SC.small(2))
+    p.boundary_method = -1
+    agree = 0
+    for k in range(len(a)):
+        o = S.Oracle(p, np.stack([a[k], b[k]]), None, np.zeros((0, 3), np.float32), bits=32)
+        off, ids, _, _ = o.neighbors()
+        agree += (1 in ids[off[0]:off[1]]) == bool(g["member"][k])
+    assert agree == len(a)
+    # and a plain (x*x + y*y) + z*z rounding disagrees with the reference somewhere
+    r = (a - b).astype(np.float32)
+    plain = np.sqrt(((r[:, 0] * r[:, 0] + r[:, 1] * r[:, 1]).astype(np.float32) + r[:, 2] * r[:, 2]).astype(np.float32)) <= h
+    assert not np.array_equal(plain, g["member"])
+
+
+@pytest.mark.parametrize("tag", ["lattice", "jitter"])
+def test_oracle_neighbor_sets_match_reference(tag):
+    g = golden("ref_neighbors_%s.npz" % tag)
+    sc = json.loads(str(g["scene"]))
+    o = S.Oracle(S.params_from_scene(sc), g["x"], None, g["bx"], bits=32)
+    off, ids, boff, bids = o.neighbors()
+    assert np.array_equal(off, g["off"]) and np.array_equal(ids, g["ids"])
+    assert np.array_equal(boff, g["boff"]) and np.array_equal(bids, g["bids"])
+    assert np.array_equal(o.cell_keys(), g["keys"])
+
+
+def test_neighbor_sets_brute_force():
+    # 27-cell candidates never miss a pair within h (cell side = h): compare with O(N^2)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0, 0.3, (1500, 3)).astype(np.float32)
+    p = S.params_from_scene(SC.small(2))
+    p.boundary_method = -1
+    o = S.Oracle(p, x, None, np.zeros((0, 3), np.float32), bits=64)
+    off, ids, _, _ = o.neighbors()
+    d = np.sqrt(((x[:, None, :].astype(np.float64) - x[None, :, :]) ** 2).sum(-1))
+    for i in range(0, len(x), 37):
+        assert set(ids[off[i]:off[i + 1]].tolist()) == set(np.nonzero(d[i] <= 0.04)[0].tolist())
+
+
+# ---------------------------------------------------------------- sampling
+def test_sampling_matches_reference_example_scene():
+    g = golden("ref_example_scene.npz")
+    sc = json.loads(str(g["scene"]))
+    blk = sc["Fluid"]["FluidModel"][0]["fluidBlocks"][0]
+    box = sc["Boundary"][0]["box"][0]
+    r = sc["Configuration"]["particleRadius"]
+    xf = S.sample_block(blk["min"], blk["max"], r)
+    xb = S.sample_box(box["min"], box["max"], r)
+    assert xf.shape == g["x0"].shape and xb.shape == g["bx"].shape
+    # same points to within one FP32 rounding of the reference's contracted multiply-adds
+    assert np.abs(xf - g["x0"]).max() <= 1e-7
+    assert np.abs(xb - g["bx"]).max() <= 1e-7
+
+
+# ---------------------------------------------------------------- solver steps
+def _traj(tag):
+    g = golden("ref_traj_%s.npz" % tag)
+    sc = json.loads(str(g["scene"]))
+    return g, sc
+
+
+def _run(g, sc, bits):
+    o = S.Oracle(S.params_from_scene(sc), g["x0"], None, g["bx"], bits=bits)
+    o.init()
+    vol = o.boundary_volumes()
+    st = o.step(int(g["nsteps"]))
+    return o, vol, st, o.state()
+
+
+@pytest.mark.parametrize("tag", ["wcsph20", "wcsph20_std", "pcisph20"])
+def test_oracle_explicit_solvers_track_reference(tag):
+    # WCSPH / PCISPH have no data-dependent gates: the restatement tracks the compiled reference closely
+    g, sc = _traj(tag)
+    for bits, tol_v in ((32, 2e-4), (64, 2e-3)):
+        o, vol, st, s = _run(g, sc, bits)
+        assert np.abs(vol - g["bvol"]).max() / np.abs(g["bvol"]).max() < 1e-6
+        assert np.abs(s["x"] - g["x"]).max() < 1e-6
+        dv = s["v"] - g["v"]
+        assert np.sqrt((dv ** 2).mean() / (g["v"] ** 2).mean()) < tol_v
+        assert np.abs(s["rho"] - g["rho"]).max() / 1000.0 < 1e-5
+        assert np.array_equal(st[:, 2], g["stats"][:, 2])  # PCISPH iterations per step
+
+
+def test_oracle_dfsph_one_step_tracks_reference():
+    g, sc = _traj("dfsph1")
+    assert abs(g["rho0"] - _run(g, sc, 64)[0].state()["rho"]).max() >= 0  # smoke
+    for bits in (32, 64):
+        o = S.Oracle(S.params_from_scene(sc), g["x0"], None, g["bx"], bits=bits)
+        o.init()
+        s0 = o.state()
+        assert np.abs(s0["rho"] - g["rho0"]).max() / 1000.0 < 2e-6
+        assert np.abs(s0["alpha"] - g["alpha0"]).max() / np.abs(g["alpha0"]).max() < 2e-6
+        st = o.step(1)
+        s = o.state()
+        assert np.array_equal(st[:, 2:4], g["stats"][:, 2:4])
+        assert abs(st[0, 1] - g["stats"][0, 1]) < 1e-9  # dt
+        dv = s["v"] - g["v"]
+        assert np.sqrt((dv ** 2).mean() / (g["v"] ** 2).mean()) < 1e-4
+
+
+def test_oracle_dfsph_aggregates_over_40_steps():
+    # past a few steps per-particle DFSPH states diverge through the |k| > 1e-5 gates
+    # (DFSPHSolver.cpp:479,483); the aggregates the solver controls must still agree
+    g, sc = _traj("dfsph40")
+    o, vol, st, s = _run(g, sc, 64)
+    ref = g["stats"]
+    assert abs(st[:, 2].sum() - ref[:, 2].sum()) <= 0.05 * ref[:, 2].sum() + 2
+    assert abs(st[:, 3].sum() - ref[:, 3].sum()) <= 0.05 * ref[:, 3].sum() + 2
+    assert abs(st[-1, 0] - ref[-1, 0]) / ref[-1, 0] < 0.05  # simulated time
+    e_ref = np.mean(np.maximum(g["rho"] / 1000 - 1, 0))
+    e_me = np.mean(np.maximum(s["rho"] / 1000 - 1, 0))
+    assert abs(e_me - e_ref) < 0.2 * e_ref + 1e-5
+    ke_ref = (g["v"] ** 2).sum()
+    ke_me = (s["v"] ** 2).sum()
+    assert abs(ke_me - ke_ref) / ke_ref < 0.05
+    assert np.abs(s["x"].mean(0) - g["x"].mean(0)).max() < 1e-4
