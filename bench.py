@@ -58,6 +58,21 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/d1m_hbm_traffic.md,
+    lines `traffic <kernel> <bytes_per_launch>`), or None."""
+    path = os.path.join(ROOT, "profiles", "d1m_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.strip().strip("`").split()
+            if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel:
+                return float(parts[2])
+    return None
+
+
 def cpu_baseline(scene_path, nf, steps, threads):
     """Time the compiled reference on the first `steps` steps of the workload."""
     from oracle import sphref as S
@@ -78,6 +93,52 @@ def cpu_baseline(scene_path, nf, steps, threads):
                       "steps 1-%d, step() calls only (%.1f s)" % (steps, secs)}
 
 
+def batched_gain(steps, warmup=3):
+    """B64: 64 DFSPH scenes (2k..200k particles) advanced together in one
+    context vs the same scenes as 64 sequential single-scene contexts."""
+    from sph2o_amd import scenes as SC
+    from sph2o_amd.simulation import Context
+
+    td = tempfile.mkdtemp()
+    paths = []
+    for k, sc in enumerate(SC.b64()):
+        p = os.path.join(td, "b%02d.json" % k)
+        SC.write_json(sc, p)
+        paths.append(p)
+    batch = Context(0)
+    for p in paths:
+        batch.load_scene_json(p)
+    batch.init()
+    total = sum(batch.num_fluid(s) for s in range(len(paths)))
+    batch.step(warmup)
+    batch.synchronize()
+    t0 = time.perf_counter()
+    st = batch.step(steps)
+    batch.synchronize()
+    tb = time.perf_counter() - t0
+    its_b = float(np.mean([[x.iterations + x.iterations_v for x in row] for row in st]))
+    batch.close()
+    ts = 0.0
+    its_s = []
+    for p in paths:
+        c = Context(0)
+        c.load_scene_json(p)
+        c.init()
+        c.step(warmup)
+        c.synchronize()
+        t0 = time.perf_counter()
+        st = c.step(steps)
+        c.synchronize()
+        ts += time.perf_counter() - t0
+        its_s.append(np.mean([row[0].iterations + row[0].iterations_v for row in st]))
+        c.close()
+    b_ps = total * steps / tb
+    s_ps = total * steps / ts
+    return {"scenes": len(paths), "fluid_particles": total, "steps": steps, "warmup": warmup,
+            "batched_particle_steps_per_s": b_ps, "sequential_particle_steps_per_s": s_ps, "gain": b_ps / s_ps,
+            "mean_iterations_per_scene_step": {"batched": its_b, "sequential": float(np.mean(its_s))}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +147,9 @@ def main():
     ap.add_argument("--n", type=int, default=100, help="fluid lattice edge (100 -> 1M particles)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--batch-steps", type=int, default=20, help="B64 batched-gain steps (0 = skip)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
+                    help="finish with a device-to-device copy of this size (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -141,11 +205,23 @@ def main():
     bytes_per_launch = nf * (own + (16.0 * nb / nf if reads_b else 0.0))
     avg_ms = kms / klaunch if klaunch else float("nan")
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if klaunch else None
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_%s_d1m.json" % dominant)
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+    traffic = measured_traffic(dominant)
+
+    ctx.close()
+    if args.calibrate_copy > 0:
+        import torch
+
+        n = args.calibrate_copy << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: %d bytes read, %d bytes written" % (n, n))
+    batched = None
+    if args.batch_steps > 0:
+        log("batched gain: B64, %d steps" % args.batch_steps)
+        batched = batched_gain(args.batch_steps)
 
     cpu = None
     if not args.no_cpu:
@@ -168,6 +244,9 @@ def main():
         "roofline": {"kernel": dominant, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
                      "avg_launch_ms": avg_ms, "launches": klaunch, "algorithmic_bytes_per_launch": bytes_per_launch},
+        "batched_gain": batched["gain"] if batched else None,
+        "batched": batched,
+        "adjoint_fwd_cost": None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

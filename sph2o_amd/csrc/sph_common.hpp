@@ -33,7 +33,10 @@ struct SceneConst {
     float pciDelta;            // PCISPH scaling factor
     float eps;                 // DFSPH gate 1e-5
     float h2c;                 // 0.01*h*h (artificial viscosity)
-    int32_t pad[3];
+    // Exact forms of sqrt_rn(d) <= h and sqrt_rn(d) <= h/2 for d = fma(x,x, fma(z,z, y*y)):
+    // the largest floats whose correctly rounded square root does not exceed h (h/2).
+    float dmax, dhalf;
+    int32_t pad[1];
 };
 
 // Per-scene scalars the host rewrites during a step.

@@ -96,6 +96,16 @@ void hostGradW(float h, float lW, const float r[3], float out[3]) {
     }
 }
 
+// Largest float d with sqrt_rn(d) <= t (sqrt_rn is monotonic, so
+// sqrt_rn(d) <= t  <=>  d <= sqrtThreshold(t)). std::sqrt on float is the
+// correctly rounded IEEE square root.
+float sqrtThreshold(float t) {
+    float d = t * t;
+    while (std::sqrt(d) > t) d = std::nextafter(d, 0.0f);
+    while (std::sqrt(std::nextafter(d, INFINITY)) <= t) d = std::nextafter(d, INFINITY);
+    return d;
+}
+
 struct SceneH {
     sph_scene_params P{};
     std::vector<float> xf, vf, xb;  // original order
@@ -358,6 +368,8 @@ class Engine {
             c.pciDelta = s.pciDelta;
             c.eps = 1.0e-5f;
             c.h2c = 0.01f * (s.h * s.h);
+            c.dmax = sqrtThreshold(s.h);
+            c.dhalf = sqrtThreshold(0.5f * s.h);
         }
         dScenes.alloc(scenes.size());
         HIPCHK(hipMemcpy(dScenes.p, sc.data(), sc.size() * sizeof(SceneConst), hipMemcpyHostToDevice));

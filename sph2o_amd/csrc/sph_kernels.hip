@@ -24,9 +24,11 @@
 namespace sph {
 
 // ---------------------------------------------------------------- helpers
-__device__ __forceinline__ float lenRef(float x, float y, float z) {
-    return __fsqrt_rn(__fmaf_rn(x, x, __fmaf_rn(z, z, __fmul_rn(y, y))));
+// |r|^2 as the reference's compiled glm::length forms it: fma(x,x, fma(z,z, y*y)).
+__device__ __forceinline__ float dRef(float x, float y, float z) {
+    return __fmaf_rn(x, x, __fmaf_rn(z, z, __fmul_rn(y, y)));
 }
+__device__ __forceinline__ float lenRef(float x, float y, float z) { return __fsqrt_rn(dRef(x, y, z)); }
 __device__ __forceinline__ float fastSqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
 __device__ __forceinline__ float fastRcp(float v) { return __builtin_amdgcn_rcpf(v); }
 
@@ -54,13 +56,15 @@ __device__ __forceinline__ float gradFactor(const SceneConst& S, float rl) {
 __device__ __forceinline__ float laplW(const SceneConst& S, float rl) {
     return rl <= S.h ? S.laplC * (S.h - rl) : 0.0f;
 }
-// Cohesion::W (Cohesion.h:20-40), with the reference's exact length
-__device__ __forceinline__ float cohesionW(const SceneConst& S, float rl) {
-    if (2.0f * rl > S.h && rl <= S.h) {
+// Cohesion::W (Cohesion.h:20-40). The branch at r = h/2, where the
+// reference's form jumps by 2x, is decided exactly on d = |r|^2 in the
+// reference's rounding (d > dhalf <=> sqrt_rn(d) > h/2); rl is the length.
+__device__ __forceinline__ float cohesionW(const SceneConst& S, float d, float rl) {
+    if (d > S.dhalf && d <= S.dmax) {
         const float a = S.h - rl;
         return S.cohC1 * (a * a * a) * (rl * rl * rl);
     }
-    if (rl > 0.0f && 2.0f * rl <= S.h) {
+    if (d > 0.0f && d <= S.dhalf) {
         const float a = S.h - rl;
         return S.cohC1 * 2.0f * (a * a * a) * (rl * rl * rl) - S.cohC2;
     }
@@ -236,7 +240,7 @@ __global__ void __launch_bounds__(kBlock) kBuildNeighbors(DevView d) {
                     const uint2 fr = d.fRange[c];
                     for (uint32_t j = fr.x; j < fr.y; ++j) {
                         const float4 xj = d.x[j];
-                        if (lenRef(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) <= S.h) {
+                        if (dRef(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) <= S.dmax) {
                             if (nf < d.capF) d.nbrF[nf * NP + i] = j;
                             ++nf;
                         }
@@ -245,7 +249,7 @@ __global__ void __launch_bounds__(kBlock) kBuildNeighbors(DevView d) {
                         const uint2 br = d.bRange[c];
                         for (uint32_t b = br.x; b < br.y; ++b) {
                             const float4 xb = d.xb[b];
-                            if (lenRef(xi.x - xb.x, xi.y - xb.y, xi.z - xb.z) <= S.h) {
+                            if (dRef(xi.x - xb.x, xi.y - xb.y, xi.z - xb.z) <= S.dmax) {
                                 if (nb < d.capB) d.nbrB[nb * NP + i] = b;
                                 ++nb;
                             }
@@ -487,11 +491,12 @@ __global__ void __launch_bounds__(kBlock) kSurfaceTension(DevView d, int vmax) {
                 const uint32_t j = d.nbrF[s * NP + i];
                 const float4 xj = d.x[j];
                 const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                const float rl = lenRef(rx, ry, rz);
-                if (rl > 0.0f && rl <= S.h) {
+                const float dd = dRef(rx, ry, rz);
+                if (dd > 0.0f && dd <= S.dmax) {
+                    const float rl = fastSqrt(dd);
                     const float4 nj = d.n[j];
                     const float kij = 2.0f * S.rho0 / (rhoi + d.rho[j]);
-                    const float c = S.st * S.m * cohesionW(S, rl) / rl;
+                    const float c = S.st * S.m * cohesionW(S, dd, rl) / rl;
                     ai.x += kij * (-c * rx - S.st * (ni.x - nj.x));
                     ai.y += kij * (-c * ry - S.st * (ni.y - nj.y));
                     ai.z += kij * (-c * rz - S.st * (ni.z - nj.z));

@@ -17,6 +17,8 @@ Fixtures:
   ref_example_scene.npz  the reference's own example scene
                          (data/example_scene/example_scene.json): sampled
                          fluid/boundary positions, volumes, one DFSPH step
+  ref_example_wcsph.npz  the same scene stepped twice by WCSPH (no DFSPH
+                         gates): a tight check of the force passes
 """
 import json
 import os
@@ -143,6 +145,20 @@ def example_scene():
          stats=out["stats"])
 
 
+def example_scene_wcsph():
+    """The example scene stepped by WCSPH: no DFSPH gates, so the force
+    passes (artificial viscosity with boundary term, surface tension) can be
+    compared tightly on the reference's own lattice."""
+    path = "/root/reference/data/example_scene/example_scene.json"
+    with open(path) as f:
+        sc = json.load(f)
+    c = sc["Configuration"]
+    c.update(simulationMethod=0, stiffness=1000.0, gamma=7.0, timeStep=1e-4)
+    out = S.run_reference(sc, 2)
+    save("ref_example_wcsph.npz", scene=scene_json(sc), nsteps=np.array(2), x0=out["x0"], bx=out["bx"],
+         bvol=out["bvol"], x=out["x"], v=out["v"], rho=out["rho"], stats=out["stats"])
+
+
 def main():
     if not S.reference_available():
         raise SystemExit("compiled reference missing: make -C oracle ref")
@@ -160,6 +176,7 @@ def main():
                                        surface_tension_method=3, surface_tension=0.15), 20)
     trajectory("wcsph20_std", SC.small(0, viscosity_method=0, viscosity=0.5, boundary_viscosity=0.1), 20)
     example_scene()
+    example_scene_wcsph()
 
 
 if __name__ == "__main__":
