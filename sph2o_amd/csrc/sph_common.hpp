@@ -36,17 +36,28 @@ struct SceneConst {
     // Exact forms of sqrt_rn(d) <= h and sqrt_rn(d) <= h/2 for d = fma(x,x, fma(z,z, y*y)):
     // the largest floats whose correctly rounded square root does not exceed h (h/2).
     float dmax, dhalf;
-    int32_t pad[1];
+    // solver control (reference FP32 values)
+    float eta, etaV, cfl, radius, minTs, maxTs;
+    int32_t minIt, maxIt, minItV, adaptive;
+    int32_t nf;
+    int32_t pad[3];
 };
 
-// Per-scene scalars the host rewrites during a step.
-struct SceneDyn {
-    float dt;        // current time step
-    float dt2;       // dt*dt
-    float scale;     // multiplier used by the rescale kernels
-    int32_t active;  // scene takes part in the current launch
-    int32_t first;   // first solver iteration: accumulate k_ws = k instead of +=
-    int32_t pad[3];
+// Per-scene solver state, owned by the device while a context steps: the
+// time step, the Jacobi-loop control and the statistics of the last step.
+// The iteration flags are double-buffered by parity: iteration n's passes read
+// active[n & 1] while the last block of its error reduction writes
+// active[(n + 1) & 1]; active[2] is always 1 (passes outside a loop).
+struct SceneState {
+    double err;              // error of the solve in progress (PCISPH: carried across steps)
+    double errD, errV;       // final density / divergence errors of the last step
+    double vmax;             // FP64 CFL maximum of the last step
+    float dt, dt2, t;
+    int32_t its, itsV;       // iterations of the last density / divergence solve
+    int32_t it;              // iterations done in the solve in progress
+    int32_t active[3];
+    int32_t capped;          // bit 0: density loop capped, bit 1: divergence loop capped
+    int32_t pad[2];
 };
 
 // Device-side flag slots per scene (int32 each).

@@ -141,27 +141,26 @@ class Engine {
     uint32_t capF = 64, capB = 48;
 
     DevBuf<SceneConst> dScenes;
-    DevBuf<SceneDyn> dDyn;
+    DevBuf<SceneState> dState;
     DevBuf<int32_t> dBlockScene;
-    DevBuf<float4> x, xAlt, v, vAlt, a, n, xs, ap, xb;
+    DevBuf<float4> x, xAlt, v, vAlt, a, n, xs, ap, xk, xb;
     DevBuf<float> rho, alpha, p, k, kws, kwsAlt, kvws, kvwsAlt, aux, vb;
     DevBuf<uint32_t> id, idAlt, keys, keysAlt, vals, valsAlt;
     DevBuf<uint2> fRange, bRange;
     DevBuf<uint32_t> nbrF, cntF, nbrB, cntB;
-    DevBuf<double> partials, red;
+    DevBuf<double> partials;
+    DevBuf<float4> snapV;   // DFSPH: v and kv_ws before the passes that follow a neighbour search
+    DevBuf<float> snapKvws;
     DevBuf<uint32_t> counter;
     DevBuf<int32_t> flags;
     DevBuf<unsigned char> sortTemp;
     size_t sortTempBytes = 0;
 
-    // pinned host staging
-    SceneDyn* hDynRing = nullptr;
-    int dynSlot = 0;
-    static constexpr int kDynRing = 64;
-    double* hRed = nullptr;
+    // pinned host mirrors of the device state and flags
+    SceneState* hState = nullptr;
     int32_t* hFlags = nullptr;
-    hipEvent_t evRed = nullptr, evStart = nullptr, evStop = nullptr;
-    std::vector<SceneDyn> dynHost;
+    hipEvent_t evStart = nullptr, evStop = nullptr;
+    int64_t syncsInStep = 0;
 
     // profiling
     bool profiling = false;
@@ -180,17 +179,14 @@ class Engine {
             (void)hipEventDestroy(e.first);
             (void)hipEventDestroy(e.second);
         }
-        if (evRed) (void)hipEventDestroy(evRed);
         if (evStart) (void)hipEventDestroy(evStart);
         if (evStop) (void)hipEventDestroy(evStop);
         if (ownStream && stream) (void)hipStreamDestroy(stream);
     }
     void releasePinned() {
-        if (hDynRing) (void)hipHostFree(hDynRing);
-        if (hRed) (void)hipHostFree(hRed);
+        if (hState) (void)hipHostFree(hState);
         if (hFlags) (void)hipHostFree(hFlags);
-        hDynRing = nullptr;
-        hRed = nullptr;
+        hState = nullptr;
         hFlags = nullptr;
     }
 
@@ -370,6 +366,17 @@ class Engine {
             c.h2c = 0.01f * (s.h * s.h);
             c.dmax = sqrtThreshold(s.h);
             c.dhalf = sqrtThreshold(0.5f * s.h);
+            c.eta = s.P.eta;
+            c.etaV = s.P.eta_v;
+            c.cfl = s.P.cfl_factor;
+            c.radius = s.r;
+            c.minTs = s.minTs;
+            c.maxTs = s.maxTs;
+            c.minIt = static_cast<int32_t>(std::min<uint32_t>(s.P.min_iterations, 1u << 30));
+            c.maxIt = static_cast<int32_t>(std::min<uint32_t>(s.P.max_iterations, 1u << 30));
+            c.minItV = static_cast<int32_t>(std::min<uint32_t>(s.P.min_iterations_v, 1u << 30));
+            c.adaptive = (s.P.simulation_method == SPH_DFSPH && s.P.adaptive_time_step) ? 1 : 0;
+            c.nf = static_cast<int32_t>(s.nf);
         }
         dScenes.alloc(scenes.size());
         HIPCHK(hipMemcpy(dScenes.p, sc.data(), sc.size() * sizeof(SceneConst), hipMemcpyHostToDevice));
@@ -418,19 +425,19 @@ class Engine {
     DevView view() {
         DevView d{};
         d.scenes = dScenes.p;
-        d.dyn = dDyn.p;
+        d.state = dState.p;
         d.blockScene = dBlockScene.p;
         d.numScenes = static_cast<int32_t>(scenes.size());
         d.numBlocks = numBlocks;
         d.NP = static_cast<uint32_t>(NP);
-        d.x = x.p; d.v = v.p; d.a = a.p; d.n = n.p; d.xs = xs.p; d.ap = ap.p;
+        d.x = x.p; d.v = v.p; d.a = a.p; d.n = n.p; d.xs = xs.p; d.ap = ap.p; d.xk = xk.p;
         d.rho = rho.p; d.alpha = alpha.p; d.p = p.p; d.k = k.p; d.kws = kws.p; d.kvws = kvws.p; d.aux = aux.p;
         d.id = id.p;
         d.xb = xb.p; d.vb = vb.p;
         d.fRange = fRange.p; d.bRange = bRange.p;
         d.nbrF = nbrF.p; d.cntF = cntF.p; d.nbrB = nbrB.p; d.cntB = cntB.p;
         d.capF = capF; d.capB = capB;
-        d.partials = partials.p; d.counter = counter.p; d.red = red.p; d.flags = flags.p;
+        d.partials = partials.p; d.counter = counter.p; d.flags = flags.p;
         return d;
     }
 
@@ -443,7 +450,6 @@ class Engine {
             HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
             ownStream = true;
         }
-        HIPCHK(hipEventCreateWithFlags(&evRed, hipEventDisableTiming));
         HIPCHK(hipEventCreate(&evStart));
         HIPCHK(hipEventCreate(&evStop));
         // slots
@@ -468,33 +474,38 @@ class Engine {
         NB = boff;
         numBlocks = static_cast<uint32_t>(NP / kBlock);
         const size_t np = NP ? NP : 1;
-        dDyn.alloc(scenes.size());
+        dState.alloc(scenes.size());
         std::vector<int32_t> bs(numBlocks ? numBlocks : 1, 0);
         for (size_t i = 0; i < scenes.size(); ++i)
             for (uint32_t b = scenes[i].blockBegin; b < scenes[i].blockEnd; ++b) bs[b] = static_cast<int32_t>(i);
         dBlockScene.alloc(bs.size());
         HIPCHK(hipMemcpy(dBlockScene.p, bs.data(), bs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-        for (auto* b : {&x, &xAlt, &v, &vAlt, &a, &n, &xs, &ap}) b->alloc(np);
+        for (auto* b : {&x, &xAlt, &v, &vAlt, &a, &n, &xs, &ap, &xk}) b->alloc(np);
         for (auto* b : {&rho, &alpha, &p, &k, &kws, &kwsAlt, &kvws, &kvwsAlt, &aux}) b->alloc(np);
         for (auto* b : {&id, &idAlt, &keys, &keysAlt, &vals, &valsAlt, &cntF, &cntB}) b->alloc(np);
         allocNeighborLists();
-        partials.alloc(2 * static_cast<size_t>(numBlocks ? numBlocks : 1));
-        red.alloc(scenes.size() * kRedSlots);
+        partials.alloc(static_cast<size_t>(numBlocks ? numBlocks : 1));
+        if (method == SPH_DFSPH) {
+            snapV.alloc(np);
+            snapKvws.alloc(np);
+        }
         counter.alloc(1);
         flags.alloc(scenes.size() * kFlagSlots);
         HIPCHK(hipMemset(counter.p, 0, sizeof(uint32_t)));
         HIPCHK(hipMemset(a.p, 0, np * sizeof(float4)));
         HIPCHK(hipMemset(n.p, 0, np * sizeof(float4)));
         for (auto* b : {&rho, &alpha, &p, &k, &kws, &kvws, &aux}) HIPCHK(hipMemset(b->p, 0, np * sizeof(float)));
-        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&hDynRing), kDynRing * scenes.size() * sizeof(SceneDyn)));
-        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&hRed), scenes.size() * kRedSlots * sizeof(double)));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&hState), scenes.size() * sizeof(SceneState)));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&hFlags), scenes.size() * kFlagSlots * sizeof(int32_t)));
-        dynHost.assign(scenes.size(), SceneDyn{});
         for (size_t i = 0; i < scenes.size(); ++i) {
-            dynHost[i].dt = scenes[i].dt;
-            dynHost[i].dt2 = scenes[i].dt * scenes[i].dt;
-            dynHost[i].active = 1;
+            SceneState& T = hState[i];
+            std::memset(&T, 0, sizeof(T));
+            T.dt = scenes[i].dt;
+            T.dt2 = scenes[i].dt * scenes[i].dt;
+            T.t = 0.0f;
+            T.active[2] = 1;
         }
+        HIPCHK(hipMemcpy(dState.p, hState, scenes.size() * sizeof(SceneState), hipMemcpyHostToDevice));
         uploadState();
         // sort keys temp storage
         sortTempBytes = 0;
@@ -503,7 +514,6 @@ class Engine {
         sortTemp.alloc(sortTempBytes + 256);
         uploadLayout();
         initialized = true;
-        uploadDyn();
         // solver init passes
         if (method == SPH_DFSPH && NP) {  // DFSPHSolver::init (DFSPHSolver.cpp:22-31)
             neighborSearch(false);
@@ -529,15 +539,13 @@ class Engine {
         HIPCHK(hipMemcpy(id.p, hid.data(), np * sizeof(uint32_t), hipMemcpyHostToDevice));
     }
 
-    void uploadDyn() {
-        SceneDyn* slot = hDynRing + static_cast<size_t>(dynSlot) * scenes.size();
-        if (dynSlot == kDynRing - 1) HIPCHK(hipStreamSynchronize(stream));  // every ring slot is free after this
-        std::memcpy(slot, dynHost.data(), scenes.size() * sizeof(SceneDyn));
-        HIPCHK(hipMemcpyAsync(dDyn.p, slot, scenes.size() * sizeof(SceneDyn), hipMemcpyHostToDevice, stream));
-        dynSlot = (dynSlot + 1) % kDynRing;
-    }
-    void setActiveAll(bool on) {
-        for (auto& d : dynHost) d.active = on ? 1 : 0;
+    // Copy the device state (and flags) to the pinned mirrors and wait.
+    void readState() {
+        HIPCHK(hipMemcpyAsync(hState, dState.p, scenes.size() * sizeof(SceneState), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(hFlags, flags.p, scenes.size() * kFlagSlots * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        ++syncsInStep;
     }
 
     template <class F>
@@ -573,59 +581,60 @@ class Engine {
         evUsed = 0;
     }
 
-    void readReductions() {
-        HIPCHK(hipMemcpyAsync(hRed, red.p, scenes.size() * kRedSlots * sizeof(double), hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipEventRecord(evRed, stream));
-    }
-    void waitReductions() { HIPCHK(hipEventSynchronize(evRed)); }
-
     // ------------------------------------------------------------ neighbour search
-    // keys -> sort -> reorder (+ cell ranges) -> lists. dfsphAdvance fuses the
-    // DFSPH position update. Handles grid growth and list capacity.
+    // keys -> sort -> reorder (+ cell ranges) -> within-support lists.
+    // dfsphAdvance fuses the DFSPH position update into the keys kernel.
+    void searchOnce(bool dfsphAdvance) {
+        HIPCHK(hipMemsetAsync(flags.p, 0, scenes.size() * kFlagSlots * sizeof(int32_t), stream));
+        DevView d = view();
+        timed(K_KEYS, [&] { return launchKeys(d, dfsphAdvance ? 1 : 0, keys.p, vals.p, stream); });
+        timed(K_FILL, [&] { return launchFillRanges(fRange.p, totalCells, stream); });
+        bool swapped = false;
+        timed(K_SORT, [&] {
+            return sortPairs(sortTemp.p, sortTempBytes, keys.p, keysAlt.p, vals.p, valsAlt.p, static_cast<int>(NP),
+                             keyBits, stream, &swapped);
+        });
+        const uint32_t* ks = swapped ? keysAlt.p : keys.p;
+        const uint32_t* vs = swapped ? valsAlt.p : vals.p;
+        timed(K_REORDER, [&] { return launchReorder(d, ks, vs, xAlt.p, vAlt.p, kwsAlt.p, kvwsAlt.p, idAlt.p, stream); });
+        std::swap(x.p, xAlt.p);
+        std::swap(v.p, vAlt.p);
+        std::swap(kws.p, kwsAlt.p);
+        std::swap(kvws.p, kvwsAlt.p);
+        std::swap(id.p, idAlt.p);
+        d = view();
+        timed(K_BUILD, [&] { return launchBuildNeighbors(d, stream); });
+    }
+    // Interpret the flags of the last search (hFlags must be current). Returns
+    // true if the lists are valid; otherwise grows what overflowed and returns
+    // false (the caller repeats the search without the position update).
+    bool searchValid() {
+        bool outOfGrid = false;
+        int maxF = 0, maxB = 0;
+        for (size_t s = 0; s < scenes.size(); ++s) {
+            outOfGrid |= hFlags[s * kFlagSlots + kFlagOutOfGrid] != 0;
+            maxF = std::max(maxF, hFlags[s * kFlagSlots + kFlagMaxFluidNbr]);
+            maxB = std::max(maxB, hFlags[s * kFlagSlots + kFlagMaxBoundaryNbr]);
+        }
+        if (outOfGrid) {
+            regrid();
+            return false;
+        }
+        if (static_cast<uint32_t>(maxF) > capF || static_cast<uint32_t>(maxB) > capB) {
+            if (static_cast<uint32_t>(maxF) > capF) capF = ((static_cast<uint32_t>(maxF) * 5 / 4) + 7) & ~7u;
+            if (static_cast<uint32_t>(maxB) > capB) capB = ((static_cast<uint32_t>(maxB) * 5 / 4) + 7) & ~7u;
+            allocNeighborLists();
+            return false;
+        }
+        return true;
+    }
+    // Search and check at once (one host sync).
     void neighborSearch(bool dfsphAdvance) {
         if (!NP) return;
-        for (int attempt = 0; attempt < 4; ++attempt) {
-            HIPCHK(hipMemsetAsync(flags.p, 0, scenes.size() * kFlagSlots * sizeof(int32_t), stream));
-            DevView d = view();
-            timed(K_KEYS, [&] { return launchKeys(d, (dfsphAdvance && attempt == 0) ? 1 : 0, keys.p, vals.p, stream); });
-            timed(K_FILL, [&] { return launchFillRanges(fRange.p, totalCells, stream); });
-            bool swapped = false;
-            timed(K_SORT, [&] {
-                return sortPairs(sortTemp.p, sortTempBytes, keys.p, keysAlt.p, vals.p, valsAlt.p, static_cast<int>(NP),
-                                 keyBits, stream, &swapped);
-            });
-            const uint32_t* ks = swapped ? keysAlt.p : keys.p;
-            const uint32_t* vs = swapped ? valsAlt.p : vals.p;
-            timed(K_REORDER, [&] { return launchReorder(d, ks, vs, xAlt.p, vAlt.p, kwsAlt.p, kvwsAlt.p, idAlt.p, stream); });
-            std::swap(x.p, xAlt.p);
-            std::swap(v.p, vAlt.p);
-            std::swap(kws.p, kwsAlt.p);
-            std::swap(kvws.p, kvwsAlt.p);
-            std::swap(id.p, idAlt.p);
-            d = view();
-            timed(K_BUILD, [&] { return launchBuildNeighbors(d, stream); });
-            HIPCHK(hipMemcpyAsync(hFlags, flags.p, scenes.size() * kFlagSlots * sizeof(int32_t), hipMemcpyDeviceToHost,
-                                  stream));
-            HIPCHK(hipStreamSynchronize(stream));
-            bool outOfGrid = false;
-            int maxF = 0, maxB = 0;
-            for (size_t s = 0; s < scenes.size(); ++s) {
-                outOfGrid |= hFlags[s * kFlagSlots + kFlagOutOfGrid] != 0;
-                maxF = std::max(maxF, hFlags[s * kFlagSlots + kFlagMaxFluidNbr]);
-                maxB = std::max(maxB, hFlags[s * kFlagSlots + kFlagMaxBoundaryNbr]);
-            }
-            if (outOfGrid) {
-                regrid();
-                continue;
-            }
-            if (static_cast<uint32_t>(maxF) > capF || static_cast<uint32_t>(maxB) > capB) {
-                if (static_cast<uint32_t>(maxF) > capF) capF = ((static_cast<uint32_t>(maxF) * 5 / 4) + 7) & ~7u;
-                if (static_cast<uint32_t>(maxB) > capB) capB = ((static_cast<uint32_t>(maxB) * 5 / 4) + 7) & ~7u;
-                allocNeighborLists();
-                timed(K_BUILD, [&] { return launchBuildNeighbors(view(), stream); });
-                HIPCHK(hipStreamSynchronize(stream));
-            }
-            return;
+        for (int attempt = 0; attempt < 6; ++attempt) {
+            searchOnce(dfsphAdvance && attempt == 0);
+            readState();
+            if (searchValid()) return;
         }
         throw SphError{SPH_E_OUT_OF_GRID, "could not place particles into a grid"};
     }
@@ -655,6 +664,7 @@ class Engine {
         if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
         HIPCHK(hipSetDevice(device));
         launchesInStep = 0;
+        syncsInStep = 0;
         HIPCHK(hipEventRecord(evStart, stream));
         for (int st = 0; st < nsteps; ++st) {
             if (method == SPH_DFSPH) stepDFSPH();
@@ -680,217 +690,188 @@ class Engine {
         for (auto& s : scenes) if (s.stOn) return true;
         return false;
     }
+    int maxOver(uint32_t sph_scene_params::*field) const {
+        uint32_t m = 0;
+        for (auto& s : scenes) m = std::max(m, s.P.*field);
+        return static_cast<int>(std::min<uint32_t>(m, 1u << 30));
+    }
+
+    // Launch (A, B) iteration pairs of a Jacobi loop in chunks; the device
+    // decides per scene whether an iteration runs (flags by parity), the host
+    // only checks after each chunk whether any scene still wants one.
+    // afterFirstSync runs once, right after the first chunk's sync; if it
+    // returns false the loop is abandoned (the caller redoes the phase).
+    template <class A, class B, class C>
+    bool runLoop(int firstChunk, int bound, A&& passA, B&& passB, C&& afterFirstSync) {
+        int issued = 0;
+        int chunk = std::max(1, std::min(firstChunk, bound));
+        bool firstSync = true;
+        while (true) {
+            for (int k = 0; k < chunk && issued < bound; ++k, ++issued) {
+                passA(issued & 1);
+                passB(issued & 1);
+            }
+            readState();
+            if (firstSync) {
+                firstSync = false;
+                if (!afterFirstSync()) return false;
+            }
+            if (issued >= bound) break;
+            bool any = false;
+            for (size_t s = 0; s < scenes.size(); ++s) any |= hState[s].active[issued & 1] != 0;
+            if (!any) break;
+            chunk = 2;
+        }
+        return true;
+    }
 
     void beginStats() {
         lastStats.assign(scenes.size(), sph_step_stats{});
         for (size_t s = 0; s < scenes.size(); ++s) lastStats[s].scene = static_cast<int32_t>(s);
     }
-    void endStats() {
+    // Host mirror after a step whose last device-side bookkeeping (t += dt, loop
+    // statistics) ran after the last readState: mirror it on the host.
+    void fileStats(bool dfsph) {
         for (size_t s = 0; s < scenes.size(); ++s) {
             SceneH& sc = scenes[s];
-            sc.t = sc.t + sc.dt;  // SPHSimulation::setTime(getTime() + getTimeStep())
-            lastStats[s].t = sc.t;
-            lastStats[s].dt = sc.dt;
-        }
-    }
-    void syncDt() {
-        for (size_t s = 0; s < scenes.size(); ++s) {
-            dynHost[s].dt = scenes[s].dt;
-            dynHost[s].dt2 = scenes[s].dt * scenes[s].dt;
-        }
-    }
-
-    // Jacobi loop of one DFSPH solve, per scene (DFSPHSolver.cpp:429-514 / :744-827).
-    // density: pass A = kRhoStar, else kDivergence.
-    void dfsphLoop(bool density) {
-        const size_t S = scenes.size();
-        std::vector<int> its(S, 0);
-        std::vector<double> err(S, 0.0);
-        DevView d;
-        while (true) {
-            bool any = false;
-            for (size_t s = 0; s < S; ++s) {
-                const SceneH& sc = scenes[s];
-                bool act;
-                if (density)
-                    act = (err[s] > static_cast<double>(sc.P.eta) || its[s] < static_cast<int>(sc.P.min_iterations)) &&
-                          its[s] < static_cast<int>(sc.P.max_iterations);
-                else  // bound is maxIterations, not maxIterationsV (:746)
-                    act = (err[s] > static_cast<double>(sc.P.eta_v / sc.dt) ||
-                           its[s] < static_cast<int>(sc.P.min_iterations_v)) &&
-                          its[s] < static_cast<int>(sc.P.max_iterations);
-                dynHost[s].active = act ? 1 : 0;
-                dynHost[s].first = its[s] == 0 ? 1 : 0;
-                any |= act;
-            }
-            if (!any || !NP) break;
-            uploadDyn();
-            d = view();
-            if (density) timed(K_RHO_STAR, [&] { return launchRhoStar(d, 0, stream); });
-            else timed(K_DIVERGENCE, [&] { return launchDivergence(d, 0, stream); });
-            readReductions();
-            timed(K_VELOCITY, [&] { return launchVelocity(d, k.p, stream); });
-            waitReductions();
-            for (size_t s = 0; s < S; ++s)
-                if (dynHost[s].active) {
-                    err[s] = scenes[s].nf ? hRed[s * kRedSlots + kRedSum] / scenes[s].nf : 0.0;
-                    its[s]++;
-                }
-        }
-        for (size_t s = 0; s < S; ++s) {
-            const SceneH& sc = scenes[s];
-            if (density) {
-                scenes[s].its = its[s];
-                lastStats[s].iterations = its[s];
-                lastStats[s].avg_density_error = err[s];
-                lastStats[s].capped_density = its[s] >= static_cast<int>(sc.P.max_iterations) &&
-                                              err[s] > static_cast<double>(sc.P.eta);
+            SceneState& T = hState[s];
+            sph_step_stats& o = lastStats[s];
+            sc.dt = T.dt;
+            sc.t = T.t;
+            o.dt = T.dt;
+            o.t = T.t;
+            o.max_velocity = T.vmax;
+            if (dfsph) {
+                o.iterations = T.its;
+                o.avg_density_error = T.errD;
+                o.capped_density = T.capped & 1;
+                o.iterations_v = T.itsV;
+                o.avg_divergence_error = T.errV;
+                o.capped_divergence = (T.capped >> 1) & 1;
             } else {
-                scenes[s].itsV = its[s];
-                lastStats[s].iterations_v = its[s];
-                lastStats[s].avg_divergence_error = err[s];
-                lastStats[s].capped_divergence = its[s] >= static_cast<int>(sc.P.max_iterations) &&
-                                                 err[s] > static_cast<double>(sc.P.eta_v / sc.dt);
+                o.iterations = T.its;
+                o.avg_density_error = T.errD;
+                o.capped_density = T.capped & 1;
             }
         }
     }
 
     void stepDFSPH() {
         beginStats();
-        const size_t S = scenes.size();
-        // 1. non-pressure forces (SPHSimulation::computeNonPressureForces) with the current dt
-        syncDt();
-        setActiveAll(true);
-        uploadDyn();
+        if (!NP) return advanceEmpty();
         DevView d = view();
-        if (NP) {
-            if (anyVisc() || anyST()) timed(K_FORCES, [&] { return launchForces(d, 0, stream); });
-            if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 1, stream); });
-            else timed(K_VMAX, [&] { return launchVmax(d, stream); });
-            readReductions();
-            waitReductions();
-        }
-        // 2. updateTimeStep (DFSPHSolver.cpp:1090-1162) in FP32 on the FP64 max
-        for (size_t s = 0; s < S; ++s) {
-            SceneH& sc = scenes[s];
-            const double vmaxD = sc.nf ? hRed[s * kRedSlots + kRedMax] : 0.0;
-            lastStats[s].max_velocity = vmaxD;
-            if (!sc.P.adaptive_time_step) continue;
-            float maxVel = static_cast<float>(vmaxD);
-            if (maxVel < 1.0e-9f) maxVel = 1.0e-9f;
-            float hh = sc.dt;
-            const float cflTs = sc.P.cfl_factor * 0.4f * (2.0f * sc.r) / maxVel;
-            const int itsMax = std::max(sc.its, sc.itsV);
-            if (itsMax > 6) hh *= 0.9f;
-            else if (itsMax < 4) hh *= 1.05f;
-            hh = std::min(hh, cflTs);
-            hh = std::max(hh, sc.minTs);
-            hh = std::min(hh, sc.maxTs);
-            sc.dt = hh;
-        }
-        syncDt();
-        uploadDyn();
-        d = view();
-        if (NP) {
-            // 3. predictVelocities
-            timed(K_PREDICT, [&] { return launchPredictVelocity(d, stream); });
-            // 4. correctDensityError with densityWarmStart
-            timed(K_RHO_STAR, [&] { return launchRhoStar(d, 1, stream); });
-            timed(K_VELOCITY, [&] { return launchVelocity(d, kws.p, stream); });
-        }
-        dfsphLoop(true);
-        // 5. updatePositions fused with the keys; k_ws *= ts^2 (or cleared if the loop never ran)
-        for (size_t s = 0; s < S; ++s) {
-            dynHost[s].active = 1;
-            dynHost[s].scale = scenes[s].its > 0 ? dynHost[s].dt2 : 0.0f;
-        }
-        uploadDyn();
-        // 6-7. neighbour search
-        neighborSearch(true);
-        d = view();
-        if (NP) {
+        // 1-2. non-pressure forces, then the CFL maximum; the new dt is set on the device
+        if (anyVisc() || anyST()) timed(K_FORCES, [&] { return launchForces(d, 0, stream); });
+        if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 1, stream); });
+        else timed(K_VMAX, [&] { return launchVmax(d, stream); });
+        // 3. predictVelocities
+        timed(K_PREDICT, [&] { return launchPredictVelocity(d, stream); });
+        // 4. correctDensityError: warm start (and loop start), then the loop
+        timed(K_RHO_STAR, [&] { return launchRhoStar(d, 1, 2, stream); });
+        timed(K_VELOCITY, [&] { return launchVelocity(d, 2, stream); });
+        int first = 1;
+        for (auto& sc : scenes) first = std::max(first, std::max(static_cast<int>(sc.P.min_iterations), sc.its));
+        runLoop(first, maxOver(&sph_scene_params::max_iterations),
+                [&](int slot) { timed(K_RHO_STAR, [&] { return launchRhoStar(d, 0, slot, stream); }); },
+                [&](int slot) { timed(K_VELOCITY, [&] { return launchVelocity(d, slot, stream); }); },
+                [] { return true; });
+        for (size_t s = 0; s < scenes.size(); ++s) scenes[s].its = hState[s].it;
+        // 5-7. updatePositions fused with the keys, then the neighbour search;
+        // its validity is checked at the divergence loop's first sync
+        searchOnce(true);
+        for (int attempt = 0;; ++attempt) {
+            d = view();
+            HIPCHK(hipMemcpyAsync(snapV.p, v.p, NP * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(snapKvws.p, kvws.p, NP * sizeof(float), hipMemcpyDeviceToDevice, stream));
             // 8-9. computeDensities + computeAlpha
             timed(K_DENSITY, [&] { return launchDensityAlpha(d, 1, stream); });
-            // 10. correctDivergenceError with divergenceWarmStart
-            timed(K_DIVERGENCE, [&] { return launchDivergence(d, 1, stream); });
-            timed(K_VELOCITY, [&] { return launchVelocity(d, kvws.p, stream); });
+            // 10. correctDivergenceError: warm start (and loop start), then the loop
+            timed(K_DIVERGENCE, [&] { return launchDivergence(d, 1, 2, stream); });
+            timed(K_VELOCITY, [&] { return launchVelocity(d, 2, stream); });
+            int firstV = 1;
+            for (auto& sc : scenes) firstV = std::max(firstV, std::max(static_cast<int>(sc.P.min_iterations_v), sc.itsV));
+            const bool ok = runLoop(
+                firstV, maxOver(&sph_scene_params::max_iterations),
+                [&](int slot) { timed(K_DIVERGENCE, [&] { return launchDivergence(d, 0, slot, stream); }); },
+                [&](int slot) { timed(K_VELOCITY, [&] { return launchVelocity(d, slot, stream); }); },
+                [&] { return searchValid(); });
+            if (ok) break;
+            if (attempt >= 5) throw SphError{SPH_E_OUT_OF_GRID, "could not place particles into a grid"};
+            // the lists were invalid: undo the divergence passes and search again
+            HIPCHK(hipMemcpyAsync(v.p, snapV.p, NP * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(kvws.p, snapKvws.p, NP * sizeof(float), hipMemcpyDeviceToDevice, stream));
+            searchOnce(false);
         }
-        dfsphLoop(false);
-        for (size_t s = 0; s < S; ++s) {
-            dynHost[s].active = 1;
-            dynHost[s].scale = scenes[s].itsV > 0 ? dynHost[s].dt : 0.0f;
+        for (size_t s = 0; s < scenes.size(); ++s) scenes[s].itsV = hState[s].it;
+        timed(K_SCALE, [&] { return launchScaleDivergence(view(), stream); });
+        // kScaleDivergence filed itsV/errV and advanced t on the device; mirror it
+        for (size_t s = 0; s < scenes.size(); ++s) {
+            SceneState& T = hState[s];
+            const SceneConst_host c = hostConst(s);
+            T.itsV = T.it;
+            T.errV = T.err;
+            if (T.it >= c.maxIt && T.err > static_cast<double>(c.etaV / T.dt)) T.capped |= 2;
+            T.t = T.t + T.dt;
         }
-        uploadDyn();
-        if (NP) timed(K_SCALE, [&] { return launchScale(view(), kvws.p, stream); });
-        endStats();
+        fileStats(true);
     }
 
     void stepWCSPH() {
         beginStats();
-        syncDt();
-        setActiveAll(true);
-        uploadDyn();
+        if (!NP) return advanceEmpty();
         neighborSearch(false);
         DevView d = view();
-        if (NP) {
-            timed(K_DENSITY, [&] { return launchDensityPressure(d, stream); });
-            timed(K_FORCES, [&] { return launchForces(d, 1, stream); });
-            if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
-            timed(K_INTEGRATE, [&] { return launchIntegrate(d, stream); });
-        }
-        endStats();
+        timed(K_DENSITY, [&] { return launchDensityPressure(d, stream); });
+        timed(K_FORCES, [&] { return launchForces(d, 1, stream); });
+        if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
+        timed(K_INTEGRATE, [&] { return launchIntegrate(d, stream); });
+        for (size_t s = 0; s < scenes.size(); ++s) hState[s].t = hState[s].t + hState[s].dt;
+        fileStats(false);
     }
 
     void stepPCISPH() {
         beginStats();
-        const size_t S = scenes.size();
-        syncDt();
-        setActiveAll(true);
-        uploadDyn();
+        if (!NP) return advanceEmpty();
         neighborSearch(false);
         DevView d = view();
-        if (NP) {
-            timed(K_DENSITY, [&] { return launchDensityAlpha(d, 0, stream); });
-            if (anyVisc() || anyST()) timed(K_FORCES, [&] { return launchForces(d, 0, stream); });
-            if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
-            timed(K_PCI_INIT, [&] { return launchPciInit(d, stream); });
+        timed(K_DENSITY, [&] { return launchDensityAlpha(d, 0, stream); });
+        if (anyVisc() || anyST()) timed(K_FORCES, [&] { return launchForces(d, 0, stream); });
+        if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
+        timed(K_PCI_INIT, [&] { return launchPciInit(d, stream); });
+        int first = 1;
+        for (auto& sc : scenes) first = std::max(first, std::max(static_cast<int>(sc.P.min_iterations), sc.its));
+        runLoop(first, maxOver(&sph_scene_params::max_iterations),
+                [&](int slot) { timed(K_PCI_DENSITY, [&] { return launchPciDensity(d, slot, stream); }); },
+                [&](int slot) { timed(K_PCI_PACC, [&] { return launchPciPressureAcc(d, slot, stream); }); },
+                [] { return true; });
+        timed(K_PCI_INTEGRATE, [&] { return launchPciIntegrate(d, stream); });
+        for (size_t s = 0; s < scenes.size(); ++s) {
+            SceneState& T = hState[s];
+            const SceneConst_host c = hostConst(s);
+            scenes[s].its = T.it;
+            T.its = T.it;
+            T.errD = T.err;
+            T.capped = (T.it >= c.maxIt && T.err > static_cast<double>(c.eta)) ? 1 : 0;
+            T.t = T.t + T.dt;
         }
-        std::vector<int> its(S, 0);
-        while (true) {  // PCISPHSolver::pressureSolver (:369-401); the error is carried across steps
-            bool any = false;
-            for (size_t s = 0; s < S; ++s) {
-                const SceneH& sc = scenes[s];
-                const bool act = (sc.pciAvg > static_cast<double>(sc.P.eta) ||
-                                  its[s] < static_cast<int>(sc.P.min_iterations)) &&
-                                 its[s] < static_cast<int>(sc.P.max_iterations);
-                dynHost[s].active = act ? 1 : 0;
-                any |= act;
-            }
-            if (!any || !NP) break;
-            uploadDyn();
-            d = view();
-            timed(K_PCI_DENSITY, [&] { return launchPciDensity(d, stream); });
-            readReductions();
-            timed(K_PCI_PACC, [&] { return launchPciPressureAcc(d, stream); });
-            waitReductions();
-            for (size_t s = 0; s < S; ++s)
-                if (dynHost[s].active) {
-                    scenes[s].pciAvg = scenes[s].nf ? hRed[s * kRedSlots + kRedSum] / scenes[s].nf : 0.0;
-                    lastStats[s].max_velocity = 0;
-                    its[s]++;
-                }
-        }
-        for (size_t s = 0; s < S; ++s) {
-            scenes[s].its = its[s];
-            lastStats[s].iterations = its[s];
-            lastStats[s].avg_density_error = scenes[s].pciAvg;
-            lastStats[s].capped_density = its[s] >= static_cast<int>(scenes[s].P.max_iterations) &&
-                                          scenes[s].pciAvg > static_cast<double>(scenes[s].P.eta);
-        }
-        setActiveAll(true);
-        uploadDyn();
-        if (NP) timed(K_PCI_INTEGRATE, [&] { return launchPciIntegrate(view(), stream); });
-        endStats();
+        fileStats(false);
+    }
+
+    // No fluid particle in any scene: only time advances.
+    void advanceEmpty() {
+        for (size_t s = 0; s < scenes.size(); ++s) hState[s].t = hState[s].t + hState[s].dt;
+        HIPCHK(hipMemcpyAsync(dState.p, hState, scenes.size() * sizeof(SceneState), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        fileStats(method == SPH_DFSPH);
+    }
+
+    struct SceneConst_host {
+        int maxIt;
+        float eta, etaV;
+    };
+    SceneConst_host hostConst(size_t s) const {
+        return SceneConst_host{static_cast<int>(std::min<uint32_t>(scenes[s].P.max_iterations, 1u << 30)),
+                               scenes[s].P.eta, scenes[s].P.eta_v};
     }
 
     // ------------------------------------------------------------ I/O
@@ -975,8 +956,6 @@ class Engine {
         HIPCHK(hipMemcpy(x.p + s.fBegin, hx.data(), s.nf * sizeof(float4), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(v.p + s.fBegin, hv.data(), s.nf * sizeof(float4), hipMemcpyHostToDevice));
         if (method == SPH_DFSPH) {  // density and alpha belong to the positions
-            setActiveAll(true);
-            uploadDyn();
             neighborSearch(false);
             HIPCHK(launchDensityAlpha(view(), 1, stream));
             HIPCHK(hipStreamSynchronize(stream));
@@ -998,8 +977,6 @@ class Engine {
     int dumpNeighbors(int si, int64_t* off, int32_t* ids_, int64_t cap, int64_t* boff, int32_t* bids, int64_t bcap) {
         SceneH& s = scene(si);
         if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
-        setActiveAll(true);
-        uploadDyn();
         neighborSearch(false);
         auto ids = download(id.p, NP);
         auto cf = download(cntF.p + s.fBegin, s.nf);

@@ -30,6 +30,28 @@ __device__ __forceinline__ float dRef(float x, float y, float z) {
 }
 __device__ __forceinline__ float lenRef(float x, float y, float z) { return __fsqrt_rn(dRef(x, y, z)); }
 __device__ __forceinline__ float fastSqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
+// Build-time switches (measured variants): unroll factor of the neighbour
+// loops, non-temporal neighbour-list loads, and whether the velocity pass
+// gathers the packed (x, k) array or k first and x only for open gates.
+#ifndef SPH_UNROLL
+#define SPH_UNROLL 1
+#endif
+#ifndef SPH_NT_LISTS
+#define SPH_NT_LISTS 0
+#endif
+#ifndef SPH_XK_VELOCITY
+#define SPH_XK_VELOCITY 0
+#endif
+#define SPH_PRAGMA(x) _Pragma(#x)
+#define SPH_UNROLL_LOOP_(n) SPH_PRAGMA(unroll n)
+#define SPH_UNROLL_LOOP SPH_UNROLL_LOOP_(SPH_UNROLL)
+__device__ __forceinline__ uint32_t ldList(const uint32_t* p) {
+#if SPH_NT_LISTS
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
 __device__ __forceinline__ float fastRcp(float v) { return __builtin_amdgcn_rcpf(v); }
 
 // CubicSpline::W (CubicSpline.h:27-50)
@@ -107,12 +129,15 @@ __device__ __forceinline__ uint32_t cellKey(const SceneConst& S, int cx, int cy,
     return S.keyBase + cellLinear(S, cx, cy, cz);
 }
 
-#define SCENE_PROLOGUE                                        \
+// slot: which iteration flag of the scene state gates this launch (0/1 inside a
+// Jacobi loop, by iteration parity; 2 outside a loop, always set).
+#define SCENE_PROLOGUE(slot)                                  \
     const int sIdx = d.blockScene[blockIdx.x];                \
     const SceneConst& S = d.scenes[sIdx];                     \
-    const SceneDyn D = d.dyn[sIdx];                           \
+    const SceneState& T = d.state[sIdx];                      \
+    const float dt = T.dt, dt2 = T.dt2;                       \
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;     \
-    const bool valid = D.active && i < S.fEnd;
+    const bool valid = T.active[slot] != 0 && i < S.fEnd;
 
 // ---- FP64 deterministic reductions: block partial + last-block finish ----
 template <bool MAX>
@@ -132,15 +157,16 @@ __device__ double blockReduce(double v) {
     return t;  // valid in every thread
 }
 
-// Every block of the launch must call this exactly once (inactive blocks pass 0).
-// slot: partial slot (0 or 1). Writes red[scene * kRedSlots + redSlot] for active scenes.
-template <bool MAX>
-__device__ void finishReduction(const DevView& d, double v, int slot, int redSlot) {
+// Every block of the launch calls this exactly once (blocks with nothing to
+// add pass 0). The last block to arrive sums each scene's block partials in a
+// fixed order and hands the total to epi(scene, total) on one thread, after
+// every other block of the launch has finished its work.
+template <bool MAX, class Epi>
+__device__ void finishReduction(const DevView& d, double v, Epi&& epi) {
     const double t = blockReduce<MAX>(v);
     __shared__ bool amLast;
     if (threadIdx.x == 0) {
-        __hip_atomic_store(&d.partials[static_cast<size_t>(slot) * d.numBlocks + blockIdx.x], t, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&d.partials[blockIdx.x], t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __threadfence();
         const uint32_t ticket = atomicAdd(d.counter, 1u);
         amLast = (ticket == gridDim.x - 1);
@@ -149,26 +175,60 @@ __device__ void finishReduction(const DevView& d, double v, int slot, int redSlo
     if (!amLast) return;
     __threadfence();
     for (int s = 0; s < d.numScenes; ++s) {
-        if (!d.dyn[s].active) continue;
         const SceneConst& S = d.scenes[s];
-        double acc = MAX ? 0.0 : 0.0;
+        double acc = 0.0;
         for (uint32_t b = S.blockBegin + threadIdx.x; b < S.blockEnd; b += kBlock)
-            acc = combine<MAX>(acc, __hip_atomic_load(&d.partials[static_cast<size_t>(slot) * d.numBlocks + b],
-                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            acc = combine<MAX>(acc, __hip_atomic_load(&d.partials[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         const double tot = blockReduce<MAX>(acc);
-        if (threadIdx.x == 0) d.red[s * kRedSlots + redSlot] = tot;
+        if (threadIdx.x == 0) epi(s, tot);
     }
-    if (threadIdx.x == 0) __hip_atomic_store(d.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) {
+        __threadfence();
+        __hip_atomic_store(d.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Jacobi-loop control of one scene after the error reduction of iteration
+// `slot` (DFSPHSolver.cpp:431 / :746, PCISPHSolver.cpp:374): store the error,
+// count the iteration and decide whether the next iteration runs.
+__device__ __forceinline__ void loopEpilogue(SceneState& T, int slot, double err, double tol, int minIt, int maxIt) {
+    int next = 0;
+    if (T.active[slot]) {
+        const int it = T.it + 1;
+        T.err = err;
+        T.it = it;
+        next = ((err > tol) || it < minIt) && it < maxIt;
+    }
+    T.active[slot ^ 1] = next;
+}
+// Loop start (done by block 0 of the launch that precedes a loop; the other
+// blocks of that launch read only dt, dt2 and active[2]).
+__device__ __forceinline__ void loopStart(SceneState& T, double err, double tol, int minIt, int maxIt) {
+    T.it = 0;
+    T.err = err;
+    T.active[0] = ((err > tol) || 0 < minIt) && 0 < maxIt;
+    T.active[1] = 0;
 }
 
 // ---------------------------------------------------------------- grid
-// Cell keys of every slot. dfsphAdvance: first x += v*dt and k_ws *= scale
-// (DFSPHSolver::updatePositions :204-224 and the k_ws *= ts^2 of :516-530).
+// Cell keys of every slot. dfsphAdvance: first x += v*dt and k_ws *= ts^2
+// (DFSPHSolver::updatePositions :204-224 and :516-530; k_ws is cleared
+// instead when the density loop did not run), and block 0 files the density
+// loop's statistics.
 __global__ void __launch_bounds__(kBlock) kKeys(DevView d, int dfsphAdvance, uint32_t* keys, uint32_t* vals) {
     const int sIdx = d.blockScene[blockIdx.x];
     const SceneConst& S = d.scenes[sIdx];
-    const SceneDyn D = d.dyn[sIdx];
+    const SceneState& T = d.state[sIdx];
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (dfsphAdvance && blockIdx.x == 0) {
+        for (int s = threadIdx.x; s < d.numScenes; s += kBlock) {
+            SceneState& U = d.state[s];
+            const SceneConst& C = d.scenes[s];
+            U.its = U.it;
+            U.errD = U.err;
+            U.capped = (U.it >= C.maxIt && U.err > static_cast<double>(C.eta)) ? 1 : 0;
+        }
+    }
     vals[i] = i;
     if (i >= S.fEnd) {
         keys[i] = S.deadKey;
@@ -176,12 +236,13 @@ __global__ void __launch_bounds__(kBlock) kKeys(DevView d, int dfsphAdvance, uin
     }
     float4 p = d.x[i];
     if (dfsphAdvance) {
+        const float dt = T.dt;
         const float4 vi = d.v[i];
-        p.x += vi.x * D.dt;
-        p.y += vi.y * D.dt;
-        p.z += vi.z * D.dt;
+        p.x += vi.x * dt;
+        p.y += vi.y * dt;
+        p.z += vi.z * dt;
         d.x[i] = p;
-        d.kws[i] *= D.scale;
+        d.kws[i] *= (T.it > 0 ? T.dt2 : 0.0f);
     }
     int cx, cy, cz;
     if (!localCell(S, p, cx, cy, cz)) atomicOr(&d.flags[sIdx * kFlagSlots + kFlagOutOfGrid], 1);
@@ -220,7 +281,7 @@ __global__ void __launch_bounds__(kBlock) kReorder(DevView d, const uint32_t* ke
 // walk order: dx, dy, dz with dz fastest), filtered by |r| <= h with the
 // reference's rounding (DFSPHSolver.cpp:575). Self is included.
 __global__ void __launch_bounds__(kBlock) kBuildNeighbors(DevView d) {
-    SCENE_PROLOGUE
+    SCENE_PROLOGUE(2)
     uint32_t nf = 0, nb = 0;
     if (valid) {
         const float4 xi = d.x[i];
@@ -314,14 +375,15 @@ __global__ void kBoundaryVolume(DevView d, int scene, uint32_t bBegin, uint32_t 
 // SPHSolver::computeFluidDensities (SPHSolver.cpp:42-89) fused with
 // DFSPHSolver::computeAlpha (DFSPHSolver.cpp:109-178).
 __global__ void __launch_bounds__(kBlock) kDensityAlpha(DevView d, int withAlpha) {
-    SCENE_PROLOGUE
+    SCENE_PROLOGUE(2)
     if (!valid) return;
     const size_t NP = d.NP;
     const float4 xi = d.x[i];
     float rho = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, ssq = 0.0f;
     const uint32_t nf = min(d.cntF[i], d.capF);
+SPH_UNROLL_LOOP
     for (uint32_t s = 0; s < nf; ++s) {
-        const uint32_t j = d.nbrF[s * NP + i];
+        const uint32_t j = ldList(d.nbrF + s * NP + i);
         const float4 xj = d.x[j];
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
         const float rl = fastSqrt(rx * rx + ry * ry + rz * rz);
@@ -333,7 +395,7 @@ __global__ void __launch_bounds__(kBlock) kDensityAlpha(DevView d, int withAlpha
     }
     const uint32_t nb = S.akinci ? min(d.cntB[i], d.capB) : 0u;
     for (uint32_t s = 0; s < nb; ++s) {
-        const float4 xb = d.xb[d.nbrB[s * NP + i]];
+        const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
         const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
         const float rl = fastSqrt(rx * rx + ry * ry + rz * rz);
         rho += xb.w * cubicW(S, rl);
@@ -349,20 +411,21 @@ __global__ void __launch_bounds__(kBlock) kDensityAlpha(DevView d, int withAlpha
 
 // WCSPH density + Tait pressure (WCSPHSolver.cpp:80-95).
 __global__ void __launch_bounds__(kBlock) kDensityPressure(DevView d) {
-    SCENE_PROLOGUE
+    SCENE_PROLOGUE(2)
     if (!valid) return;
     const size_t NP = d.NP;
     const float4 xi = d.x[i];
     float rho = 0.0f;
     const uint32_t nf = min(d.cntF[i], d.capF);
+SPH_UNROLL_LOOP
     for (uint32_t s = 0; s < nf; ++s) {
-        const float4 xj = d.x[d.nbrF[s * NP + i]];
+        const float4 xj = d.x[ldList(d.nbrF + s * NP + i)];
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
         rho += S.m * cubicW(S, fastSqrt(rx * rx + ry * ry + rz * rz));
     }
     const uint32_t nb = S.akinci ? min(d.cntB[i], d.capB) : 0u;
     for (uint32_t s = 0; s < nb; ++s) {
-        const float4 xb = d.xb[d.nbrB[s * NP + i]];
+        const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
         const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
         rho += xb.w * cubicW(S, fastSqrt(rx * rx + ry * ry + rz * rz));
     }
@@ -378,7 +441,7 @@ __global__ void __launch_bounds__(kBlock) kDensityPressure(DevView d) {
 //   surface-tension normals (SurfaceTensionAkinci.cpp:13-33).
 // The acceleration is written fresh (it is zero at this point of every step).
 __global__ void __launch_bounds__(kBlock) kForces(DevView d, int pressure) {
-    SCENE_PROLOGUE
+    SCENE_PROLOGUE(2)
     if (!valid) return;
     const size_t NP = d.NP;
     const float4 xi = d.x[i];
@@ -391,8 +454,9 @@ __global__ void __launch_bounds__(kBlock) kForces(DevView d, int pressure) {
         const float pi_ = d.p[i];
         const float dpi = pi_ / (rhoi * rhoi);
         const float dpb = pi_ / (S.rho0 * S.rho0);
+SPH_UNROLL_LOOP
         for (uint32_t s = 0; s < nf; ++s) {
-            const uint32_t j = d.nbrF[s * NP + i];
+            const uint32_t j = ldList(d.nbrF + s * NP + i);
             const float4 xj = d.x[j];
             const float rhoj = d.rho[j];
             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -400,7 +464,7 @@ __global__ void __launch_bounds__(kBlock) kForces(DevView d, int pressure) {
             ax -= g * rx; ay -= g * ry; az -= g * rz;
         }
         for (uint32_t s = 0; s < nb; ++s) {
-            const float4 xb = d.xb[d.nbrB[s * NP + i]];
+            const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
             const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
             const float g = xb.w * (dpi + dpb) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
             ax -= g * rx; ay -= g * ry; az -= g * rz;
@@ -411,9 +475,10 @@ __global__ void __launch_bounds__(kBlock) kForces(DevView d, int pressure) {
     if (vm >= 0 || normals) {
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;   // viscosity sum
         float nx = 0.0f, ny = 0.0f, nz = 0.0f;   // normal sum
-        const float invdt = 1.0f / D.dt;
+        const float invdt = 1.0f / dt;
+SPH_UNROLL_LOOP
         for (uint32_t s = 0; s < nf; ++s) {
-            const uint32_t j = d.nbrF[s * NP + i];
+            const uint32_t j = ldList(d.nbrF + s * NP + i);
             const float4 xj = d.x[j];
             const float rhoj = d.rho[j];
             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -446,7 +511,7 @@ __global__ void __launch_bounds__(kBlock) kForces(DevView d, int pressure) {
         if (vm >= 0 && S.bviscOn) {
             float bx = 0.0f, by = 0.0f, bz = 0.0f;
             for (uint32_t s = 0; s < nb; ++s) {
-                const float4 xb = d.xb[d.nbrB[s * NP + i]];
+                const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
                 const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
                 const float r2 = rx * rx + ry * ry + rz * rz;
                 const float rl = fastSqrt(r2);
@@ -468,16 +533,37 @@ __global__ void __launch_bounds__(kBlock) kForces(DevView d, int pressure) {
             }
             ax += bx; ay += by; az += bz;
         }
-        if (normals) d.n[i] = make_float4(nx * S.h, ny * S.h, nz * S.h, 0.0f);
+        if (normals) d.n[i] = make_float4(nx * S.h, ny * S.h, nz * S.h, rhoi);
     }
     d.a[i] = make_float4(ax, ay, az, 0.0f);
 }
 
+// DFSPHSolver::updateTimeStep (DFSPHSolver.cpp:1090-1162) on the FP64 maximum
+// of |v + (a + g) dt|, evaluated in FP32 as the reference does. Runs in the
+// last block of the reduction, after every block has used the old dt.
+__device__ __forceinline__ void timeStepEpilogue(const DevView& d, int s, double vmax) {
+    SceneState& T = d.state[s];
+    const SceneConst& S = d.scenes[s];
+    T.vmax = vmax;
+    if (!S.adaptive || S.nf == 0) return;
+    float maxVel = static_cast<float>(vmax);
+    if (maxVel < 1.0e-9f) maxVel = 1.0e-9f;
+    float h = T.dt;
+    const float cflTs = S.cfl * 0.4f * (2.0f * S.radius) / maxVel;
+    const int its = max(T.its, T.itsV);
+    if (its > 6) h *= 0.9f;
+    else if (its < 4) h *= 1.05f;
+    h = fminf(h, cflTs);
+    h = fmaxf(h, S.minTs);
+    h = fminf(h, S.maxTs);
+    T.dt = h;
+    T.dt2 = h * h;
+}
+
 // SurfaceTensionAkinci::step pass 2 (SurfaceTensionAkinci.cpp:35-70).
-// With vmax: the FP64 max of |v + (a + g) dt| (DFSPHSolver.cpp:1096-1113)
-// over the final accelerations of this step.
+// With vmax: the CFL maximum and the new time step (DFSPH).
 __global__ void __launch_bounds__(kBlock) kSurfaceTension(DevView d, int vmax) {
-    SCENE_PROLOGUE
+    SCENE_PROLOGUE(2)
     double vm = 0.0;
     if (valid) {
         const size_t NP = d.NP;
@@ -485,17 +571,18 @@ __global__ void __launch_bounds__(kBlock) kSurfaceTension(DevView d, int vmax) {
         if (S.stOn) {
             const float4 xi = d.x[i];
             const float4 ni = d.n[i];
-            const float rhoi = d.rho[i];
+            const float rhoi = ni.w;
             const uint32_t nf = min(d.cntF[i], d.capF);
+SPH_UNROLL_LOOP
             for (uint32_t s = 0; s < nf; ++s) {
-                const uint32_t j = d.nbrF[s * NP + i];
+                const uint32_t j = ldList(d.nbrF + s * NP + i);
                 const float4 xj = d.x[j];
                 const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
                 const float dd = dRef(rx, ry, rz);
                 if (dd > 0.0f && dd <= S.dmax) {
                     const float rl = fastSqrt(dd);
-                    const float4 nj = d.n[j];
-                    const float kij = 2.0f * S.rho0 / (rhoi + d.rho[j]);
+                    const float4 nj = d.n[j];   // w = density
+                    const float kij = 2.0f * S.rho0 / (rhoi + nj.w);
                     const float c = S.st * S.m * cohesionW(S, dd, rl) / rl;
                     ai.x += kij * (-c * rx - S.st * (ni.x - nj.x));
                     ai.y += kij * (-c * ry - S.st * (ni.y - nj.y));
@@ -506,47 +593,54 @@ __global__ void __launch_bounds__(kBlock) kSurfaceTension(DevView d, int vmax) {
         }
         if (vmax) {
             const float4 vi = d.v[i];
-            const float cx = vi.x + (ai.x + S.gx) * D.dt;
-            const float cy = vi.y + (ai.y + S.gy) * D.dt;
-            const float cz = vi.z + (ai.z + S.gz) * D.dt;
+            const float cx = vi.x + (ai.x + S.gx) * dt;
+            const float cy = vi.y + (ai.y + S.gy) * dt;
+            const float cz = vi.z + (ai.z + S.gz) * dt;
             vm = sqrt(static_cast<double>(cx) * cx + static_cast<double>(cy) * cy + static_cast<double>(cz) * cz);
         }
     }
-    if (vmax) finishReduction<true>(d, vm, 0, kRedMax);
+    if (vmax) finishReduction<true>(d, vm, [&](int s, double tot) { timeStepEpilogue(d, s, tot); });
 }
 
 __global__ void __launch_bounds__(kBlock) kVmax(DevView d) {
-    SCENE_PROLOGUE
+    SCENE_PROLOGUE(2)
     double vm = 0.0;
     if (valid) {
         const float4 vi = d.v[i], ai = d.a[i];
-        const float cx = vi.x + (ai.x + S.gx) * D.dt;
-        const float cy = vi.y + (ai.y + S.gy) * D.dt;
-        const float cz = vi.z + (ai.z + S.gz) * D.dt;
+        const float cx = vi.x + (ai.x + S.gx) * dt;
+        const float cy = vi.y + (ai.y + S.gy) * dt;
+        const float cz = vi.z + (ai.z + S.gz) * dt;
         vm = sqrt(static_cast<double>(cx) * cx + static_cast<double>(cy) * cy + static_cast<double>(cz) * cz);
     }
-    finishReduction<true>(d, vm, 0, kRedMax);
+    finishReduction<true>(d, vm, [&](int s, double tot) { timeStepEpilogue(d, s, tot); });
 }
 
 // DFSPHSolver::predictVelocities (DFSPHSolver.cpp:180-202)
 __global__ void __launch_bounds__(kBlock) kPredictVelocity(DevView d) {
-    SCENE_PROLOGUE
+    SCENE_PROLOGUE(2)
     if (!valid) return;
     float4 vi = d.v[i];
     const float4 ai = d.a[i];
-    vi.x += (ai.x + S.gx) * D.dt;
-    vi.y += (ai.y + S.gy) * D.dt;
-    vi.z += (ai.z + S.gz) * D.dt;
+    vi.x += (ai.x + S.gx) * dt;
+    vi.y += (ai.y + S.gy) * dt;
+    vi.z += (ai.z + S.gz) * dt;
     d.v[i] = vi;
     d.a[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // ---------------------------------------------------------------- DFSPH solves
 // DFSPHSolver::predictDensities (DFSPHSolver.cpp:226-299). warm: the
-// warm-start rescale of densityWarmStart (:863-886); otherwise the k pass of
-// correctDensityError (:435-453) with the FP64 average error.
-__global__ void __launch_bounds__(kBlock) kRhoStar(DevView d, int warm) {
-    SCENE_PROLOGUE
+// warm-start rescale of densityWarmStart (:863-886), and block 0 starts the
+// density loop; otherwise the k pass of correctDensityError (:435-453) with
+// the FP64 average error feeding the loop control.
+__global__ void __launch_bounds__(kBlock) kRhoStar(DevView d, int warm, int slot) {
+    SCENE_PROLOGUE(slot)
+    if (warm && blockIdx.x == 0)
+        for (int s = threadIdx.x; s < d.numScenes; s += kBlock) {
+            const SceneConst& C = d.scenes[s];
+            loopStart(d.state[s], 0.0, static_cast<double>(C.eta), C.minIt, C.maxIt);
+        }
+    const bool first = T.it == 0;
     double err = 0.0;
     if (valid) {
         const size_t NP = d.NP;
@@ -554,8 +648,9 @@ __global__ void __launch_bounds__(kBlock) kRhoStar(DevView d, int warm) {
         const float4 vi = d.v[i];
         float dc = 0.0f;
         const uint32_t nf = min(d.cntF[i], d.capF);
+SPH_UNROLL_LOOP
         for (uint32_t s = 0; s < nf; ++s) {
-            const uint32_t j = d.nbrF[s * NP + i];
+            const uint32_t j = ldList(d.nbrF + s * NP + i);
             const float4 xj = d.x[j];
             const float4 vj = d.v[j];
             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -564,41 +659,58 @@ __global__ void __launch_bounds__(kBlock) kRhoStar(DevView d, int warm) {
         }
         const uint32_t nb = S.akinci ? min(d.cntB[i], d.capB) : 0u;
         for (uint32_t s = 0; s < nb; ++s) {
-            const float4 xb = d.xb[d.nbrB[s * NP + i]];
+            const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
             const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
             const float g = gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
             dc += xb.w * (g * (vi.x * rx + vi.y * ry + vi.z * rz));
         }
         const float rhoi = d.rho[i];
-        const float pd = fmaxf(rhoi + dc * D.dt, S.rho0);
+        const float pd = fmaxf(rhoi + dc * dt, S.rho0);
         d.aux[i] = pd;
+        float kOut;
         if (warm) {
-            d.kws[i] = pd > S.rho0 ? d.kws[i] / D.dt2 : 0.0f;
+            kOut = pd > S.rho0 ? d.kws[i] / dt2 : 0.0f;
+            d.kws[i] = kOut;
+            d.k[i] = kOut;
         } else {
-            const float ki = (pd - S.rho0) * d.alpha[i] / (D.dt2 * rhoi);
-            d.k[i] = ki;
-            d.kws[i] = D.first ? ki : d.kws[i] + ki;
+            kOut = (pd - S.rho0) * d.alpha[i] / (dt2 * rhoi);
+            d.k[i] = kOut;
+            d.kws[i] = first ? kOut : d.kws[i] + kOut;
             err = static_cast<double>((pd - S.rho0) / S.rho0);
         }
+        d.xk[i] = make_float4(xi.x, xi.y, xi.z, kOut);
     }
-    if (!warm) finishReduction<false>(d, err, 0, kRedSum);
+    if (!warm)
+        finishReduction<false>(d, err, [&](int s, double sum) {
+            const SceneConst& C = d.scenes[s];
+            loopEpilogue(d.state[s], slot, C.nf ? sum / C.nf : 0.0, static_cast<double>(C.eta), C.minIt, C.maxIt);
+        });
 }
 
 // The velocity correction shared by both solves and both warm starts
-// (DFSPHSolver.cpp:455-511, :788-823, :894-941, :998-1045).
-__global__ void __launch_bounds__(kBlock) kVelocity(DevView d, const float* kb) {
-    SCENE_PROLOGUE
+// (DFSPHSolver.cpp:455-511, :788-823, :894-941, :998-1045). xk = (x, k) was
+// written by the preceding kRhoStar / kDivergence launch, so each neighbour
+// costs one 16-byte gather.
+__global__ void __launch_bounds__(kBlock) kVelocity(DevView d, int slot) {
+    SCENE_PROLOGUE(slot)
     if (!valid) return;
     const size_t NP = d.NP;
-    const float4 xi = d.x[i];
-    const float ki = kb[i];
+    const float4 xi = d.xk[i];
+    const float ki = xi.w;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     const uint32_t nf = min(d.cntF[i], d.capF);
+SPH_UNROLL_LOOP
     for (uint32_t s = 0; s < nf; ++s) {
-        const uint32_t j = d.nbrF[s * NP + i];
-        const float ks = ki + kb[j];
+        const uint32_t j = ldList(d.nbrF + s * NP + i);
+#if SPH_XK_VELOCITY
+        const float4 xj = d.xk[j];
+        const float ks = ki + xj.w;
         if (fabsf(ks) > S.eps) {
+#else
+        const float ks = ki + d.k[j];
+        if (fabsf(ks) > S.eps) {   // closed gates skip the position gather
             const float4 xj = d.x[j];
+#endif
             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
             const float c = (S.m * ks) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
             sx += c * rx; sy += c * ry; sz += c * rz;
@@ -607,25 +719,34 @@ __global__ void __launch_bounds__(kBlock) kVelocity(DevView d, const float* kb) 
     if (S.akinci && fabsf(ki) > S.eps) {
         const uint32_t nb = min(d.cntB[i], d.capB);
         for (uint32_t s = 0; s < nb; ++s) {
-            const float4 xb = d.xb[d.nbrB[s * NP + i]];
+            const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
             const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
             const float c = (xb.w * ki) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
             sx += c * rx; sy += c * ry; sz += c * rz;
         }
     }
     float4 vi = d.v[i];
-    vi.x -= D.dt * sx;
-    vi.y -= D.dt * sy;
-    vi.z -= D.dt * sz;
+    vi.x -= dt * sx;
+    vi.y -= dt * sy;
+    vi.z -= dt * sz;
     d.v[i] = vi;
 }
 
 // DFSPHSolver::computeDivergenceError (DFSPHSolver.cpp:538-621); the
 // neighbour count n_i (|r| <= h, fluid + boundary, self included) is the
 // length of the within-support lists. warm: divergenceWarmStart rescale
-// (:968-990); otherwise the kv pass of correctDivergenceError (:750-768).
-__global__ void __launch_bounds__(kBlock) kDivergence(DevView d, int warm) {
-    SCENE_PROLOGUE
+// (:968-990), and block 0 starts the divergence loop; otherwise the kv pass of
+// correctDivergenceError (:750-768) with the loop control. The loop bound is
+// maxIterations, not maxIterationsV (:746).
+__global__ void __launch_bounds__(kBlock) kDivergence(DevView d, int warm, int slot) {
+    SCENE_PROLOGUE(slot)
+    if (warm && blockIdx.x == 0)
+        for (int s = threadIdx.x; s < d.numScenes; s += kBlock) {
+            const SceneConst& C = d.scenes[s];
+            SceneState& U = d.state[s];
+            loopStart(U, 0.0, static_cast<double>(C.etaV / U.dt), C.minItV, C.maxIt);
+        }
+    const bool first = T.it == 0;
     double err = 0.0;
     if (valid) {
         const size_t NP = d.NP;
@@ -634,8 +755,9 @@ __global__ void __launch_bounds__(kBlock) kDivergence(DevView d, int warm) {
         float de = 0.0f;
         const uint32_t nfTrue = d.cntF[i];
         const uint32_t nf = min(nfTrue, d.capF);
+SPH_UNROLL_LOOP
         for (uint32_t s = 0; s < nf; ++s) {
-            const uint32_t j = d.nbrF[s * NP + i];
+            const uint32_t j = ldList(d.nbrF + s * NP + i);
             const float4 xj = d.x[j];
             const float4 vj = d.v[j];
             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -648,7 +770,7 @@ __global__ void __launch_bounds__(kBlock) kDivergence(DevView d, int warm) {
             const uint32_t nb = min(nbTrue, d.capB);
             cnt += nbTrue;
             for (uint32_t s = 0; s < nb; ++s) {
-                const float4 xb = d.xb[d.nbrB[s * NP + i]];
+                const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
                 const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
                 const float g = gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
                 de += xb.w * (g * (vi.x * rx + vi.y * ry + vi.z * rz));
@@ -656,57 +778,86 @@ __global__ void __launch_bounds__(kBlock) kDivergence(DevView d, int warm) {
         }
         de = cnt < 20u ? 0.0f : fmaxf(de, 0.0f);
         d.aux[i] = de;
+        float kOut;
         if (warm) {
-            d.kvws[i] = de > 0.0f ? d.kvws[i] / D.dt : 0.0f;
+            kOut = de > 0.0f ? d.kvws[i] / dt : 0.0f;
+            d.kvws[i] = kOut;
+            d.k[i] = kOut;
         } else {
-            const float kvi = de * d.alpha[i] / (D.dt * d.rho[i]);
-            d.k[i] = kvi;
-            d.kvws[i] = D.first ? kvi : d.kvws[i] + kvi;
+            kOut = de * d.alpha[i] / (dt * d.rho[i]);
+            d.k[i] = kOut;
+            d.kvws[i] = first ? kOut : d.kvws[i] + kOut;
             err = static_cast<double>(de / S.rho0);
         }
+        d.xk[i] = make_float4(xi.x, xi.y, xi.z, kOut);
     }
-    if (!warm) finishReduction<false>(d, err, 0, kRedSum);
+    if (!warm)
+        finishReduction<false>(d, err, [&](int s, double sum) {
+            const SceneConst& C = d.scenes[s];
+            SceneState& U = d.state[s];
+            loopEpilogue(U, slot, C.nf ? sum / C.nf : 0.0, static_cast<double>(C.etaV / U.dt), C.minItV, C.maxIt);
+        });
 }
 
-// buf *= dyn.scale (k_ws *= ts^2, kv_ws *= ts; DFSPHSolver.cpp:516-530, :829-843)
-__global__ void __launch_bounds__(kBlock) kScale(DevView d, float* buf) {
-    SCENE_PROLOGUE
+// kv_ws *= ts after the divergence loop (DFSPHSolver.cpp:829-843; cleared if
+// the loop did not run); block 0 files the loop's statistics and advances t.
+__global__ void __launch_bounds__(kBlock) kScaleDivergence(DevView d) {
+    SCENE_PROLOGUE(2)
+    if (blockIdx.x == 0)
+        for (int s = threadIdx.x; s < d.numScenes; s += kBlock) {
+            SceneState& U = d.state[s];
+            const SceneConst& C = d.scenes[s];
+            U.itsV = U.it;
+            U.errV = U.err;
+            if (U.it >= C.maxIt && U.err > static_cast<double>(C.etaV / U.dt)) U.capped |= 2;
+            U.t = U.t + U.dt;
+        }
     if (!valid) return;
-    buf[i] *= D.scale;
+    d.kvws[i] *= (T.it > 0 ? dt : 0.0f);
 }
 
-// SPHSolver::integrate (SPHSolver.cpp:139-168)
+// SPHSolver::integrate (SPHSolver.cpp:139-168); block 0 advances t.
 __global__ void __launch_bounds__(kBlock) kIntegrate(DevView d) {
-    SCENE_PROLOGUE
+    SCENE_PROLOGUE(2)
+    if (blockIdx.x == 0)
+        for (int s = threadIdx.x; s < d.numScenes; s += kBlock) d.state[s].t = d.state[s].t + d.state[s].dt;
     if (!valid) return;
     float4 ai = d.a[i], vi = d.v[i], xi = d.x[i];
     ai.x += S.gx; ai.y += S.gy; ai.z += S.gz;
-    vi.x += ai.x * D.dt; vi.y += ai.y * D.dt; vi.z += ai.z * D.dt;
-    xi.x += vi.x * D.dt; xi.y += vi.y * D.dt; xi.z += vi.z * D.dt;
+    vi.x += ai.x * dt; vi.y += ai.y * dt; vi.z += ai.z * dt;
+    xi.x += vi.x * dt; xi.y += vi.y * dt; xi.z += vi.z * dt;
     d.v[i] = vi;
     d.x[i] = xi;
     d.a[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // ---------------------------------------------------------------- PCISPH
-// initPressure (PCISPHSolver.cpp:82-118) + first predictVelocityAndPosition (:120-147)
+// initPressure (PCISPHSolver.cpp:82-118) + first predictVelocityAndPosition
+// (:120-147); block 0 starts the loop with the error carried from the last
+// step (PCISPHSolver.h:18).
 __global__ void __launch_bounds__(kBlock) kPciInit(DevView d) {
-    SCENE_PROLOGUE
+    SCENE_PROLOGUE(2)
+    if (blockIdx.x == 0)
+        for (int s = threadIdx.x; s < d.numScenes; s += kBlock) {
+            const SceneConst& C = d.scenes[s];
+            SceneState& U = d.state[s];
+            loopStart(U, U.err, static_cast<double>(C.eta), C.minIt, C.maxIt);
+        }
     if (!valid) return;
     d.p[i] = 0.0f;
     d.ap[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const float4 xi = d.x[i], vi = d.v[i], ai = d.a[i];
-    const float vx = vi.x + (ai.x + S.gx) * D.dt;
-    const float vy = vi.y + (ai.y + S.gy) * D.dt;
-    const float vz = vi.z + (ai.z + S.gz) * D.dt;
-    d.xs[i] = make_float4(xi.x + vx * D.dt, xi.y + vy * D.dt, xi.z + vz * D.dt, 0.0f);
+    const float vx = vi.x + (ai.x + S.gx) * dt;
+    const float vy = vi.y + (ai.y + S.gy) * dt;
+    const float vz = vi.z + (ai.z + S.gz) * dt;
+    d.xs[i] = make_float4(xi.x + vx * dt, xi.y + vy * dt, xi.z + vz * dt, 0.0f);
 }
 
 // predictFluidDensities (PCISPHSolver.cpp:162-209): predicted positions,
 // candidates of the *current* position's 27 cells (:178); then
-// updatePressure (:246-298) with FP64 average and max error.
-__global__ void __launch_bounds__(kBlock) kPciDensity(DevView d) {
-    SCENE_PROLOGUE
+// updatePressure (:246-298) with the FP64 average error feeding the loop.
+__global__ void __launch_bounds__(kBlock) kPciDensity(DevView d, int slot) {
+    SCENE_PROLOGUE(slot)
     double err = 0.0;
     if (valid) {
         const float4 xsi = d.xs[i];
@@ -745,21 +896,16 @@ __global__ void __launch_bounds__(kBlock) kPciDensity(DevView d) {
         d.p[i] += S.pciDelta * e;
         err = static_cast<double>(e / S.rho0);
     }
-    // slot 0: sum, slot 1: max
-    const double t = err;
-    finishReduction<false>(d, t, 0, kRedSum);
-}
-__global__ void __launch_bounds__(kBlock) kPciMaxErr(DevView d) {
-    SCENE_PROLOGUE
-    double e = 0.0;
-    if (valid) e = static_cast<double>(fmaxf(0.0f, d.rho[i] - S.rho0) / S.rho0);
-    finishReduction<true>(d, e, 1, kRedMax);
+    finishReduction<false>(d, err, [&](int s, double sum) {
+        const SceneConst& C = d.scenes[s];
+        loopEpilogue(d.state[s], slot, C.nf ? sum / C.nf : 0.0, static_cast<double>(C.eta), C.minIt, C.maxIt);
+    });
 }
 
 // computePressureAcc (PCISPHSolver.cpp:300-367) at current positions with
 // the predicted densities, then the next predictVelocityAndPosition.
-__global__ void __launch_bounds__(kBlock) kPciPressureAcc(DevView d) {
-    SCENE_PROLOGUE
+__global__ void __launch_bounds__(kBlock) kPciPressureAcc(DevView d, int slot) {
+    SCENE_PROLOGUE(slot)
     if (!valid) return;
     const size_t NP = d.NP;
     const float4 xi = d.x[i];
@@ -767,8 +913,9 @@ __global__ void __launch_bounds__(kBlock) kPciPressureAcc(DevView d) {
     const float dpi = d.p[i] / (rhoi * rhoi);
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     const uint32_t nf = min(d.cntF[i], d.capF);
+SPH_UNROLL_LOOP
     for (uint32_t s = 0; s < nf; ++s) {
-        const uint32_t j = d.nbrF[s * NP + i];
+        const uint32_t j = ldList(d.nbrF + s * NP + i);
         const float4 xj = d.x[j];
         const float rhoj = d.rho[j];
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -777,28 +924,38 @@ __global__ void __launch_bounds__(kBlock) kPciPressureAcc(DevView d) {
     }
     const uint32_t nb = S.akinci ? min(d.cntB[i], d.capB) : 0u;
     for (uint32_t s = 0; s < nb; ++s) {
-        const float4 xb = d.xb[d.nbrB[s * NP + i]];
+        const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
         const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
         const float g = xb.w * dpi * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
         ax -= g * rx; ay -= g * ry; az -= g * rz;
     }
     d.ap[i] = make_float4(ax, ay, az, 0.0f);
     const float4 vi = d.v[i], ai = d.a[i];
-    const float vx = vi.x + ((ai.x + ax) + S.gx) * D.dt;
-    const float vy = vi.y + ((ai.y + ay) + S.gy) * D.dt;
-    const float vz = vi.z + ((ai.z + az) + S.gz) * D.dt;
-    d.xs[i] = make_float4(xi.x + vx * D.dt, xi.y + vy * D.dt, xi.z + vz * D.dt, 0.0f);
+    const float vx = vi.x + ((ai.x + ax) + S.gx) * dt;
+    const float vy = vi.y + ((ai.y + ay) + S.gy) * dt;
+    const float vz = vi.z + ((ai.z + az) + S.gz) * dt;
+    d.xs[i] = make_float4(xi.x + vx * dt, xi.y + vy * dt, xi.z + vz * dt, 0.0f);
 }
 
-// PCISPHSolver::integrate (PCISPHSolver.cpp:459-488)
+// PCISPHSolver::integrate (PCISPHSolver.cpp:459-488); block 0 files the
+// loop statistics and advances t.
 __global__ void __launch_bounds__(kBlock) kPciIntegrate(DevView d) {
-    SCENE_PROLOGUE
+    SCENE_PROLOGUE(2)
+    if (blockIdx.x == 0)
+        for (int s = threadIdx.x; s < d.numScenes; s += kBlock) {
+            SceneState& U = d.state[s];
+            const SceneConst& C = d.scenes[s];
+            U.its = U.it;
+            U.errD = U.err;
+            U.capped = (U.it >= C.maxIt && U.err > static_cast<double>(C.eta)) ? 1 : 0;
+            U.t = U.t + U.dt;
+        }
     if (!valid) return;
     float4 ai = d.a[i], vi = d.v[i], xi = d.x[i];
     const float4 pa = d.ap[i];
     ai.x += S.gx + pa.x; ai.y += S.gy + pa.y; ai.z += S.gz + pa.z;
-    vi.x += ai.x * D.dt; vi.y += ai.y * D.dt; vi.z += ai.z * D.dt;
-    xi.x += vi.x * D.dt; xi.y += vi.y * D.dt; xi.z += vi.z * D.dt;
+    vi.x += ai.x * dt; vi.y += ai.y * dt; vi.z += ai.z * dt;
+    xi.x += vi.x * dt; xi.y += vi.y * dt; xi.z += vi.z * dt;
     d.v[i] = vi;
     d.x[i] = xi;
     d.a[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -859,19 +1016,20 @@ hipError_t launchSurfaceTension(const DevView& d, int vmax, hipStream_t st) {
 }
 hipError_t launchVmax(const DevView& d, hipStream_t st) { LAUNCH(kVmax, d.numBlocks, d); }
 hipError_t launchPredictVelocity(const DevView& d, hipStream_t st) { LAUNCH(kPredictVelocity, d.numBlocks, d); }
-hipError_t launchRhoStar(const DevView& d, int warm, hipStream_t st) { LAUNCH(kRhoStar, d.numBlocks, d, warm); }
-hipError_t launchVelocity(const DevView& d, const float* kb, hipStream_t st) { LAUNCH(kVelocity, d.numBlocks, d, kb); }
-hipError_t launchDivergence(const DevView& d, int warm, hipStream_t st) { LAUNCH(kDivergence, d.numBlocks, d, warm); }
-hipError_t launchScale(const DevView& d, float* buf, hipStream_t st) { LAUNCH(kScale, d.numBlocks, d, buf); }
+hipError_t launchRhoStar(const DevView& d, int warm, int slot, hipStream_t st) {
+    LAUNCH(kRhoStar, d.numBlocks, d, warm, slot);
+}
+hipError_t launchVelocity(const DevView& d, int slot, hipStream_t st) { LAUNCH(kVelocity, d.numBlocks, d, slot); }
+hipError_t launchDivergence(const DevView& d, int warm, int slot, hipStream_t st) {
+    LAUNCH(kDivergence, d.numBlocks, d, warm, slot);
+}
+hipError_t launchScaleDivergence(const DevView& d, hipStream_t st) { LAUNCH(kScaleDivergence, d.numBlocks, d); }
 hipError_t launchIntegrate(const DevView& d, hipStream_t st) { LAUNCH(kIntegrate, d.numBlocks, d); }
 hipError_t launchPciInit(const DevView& d, hipStream_t st) { LAUNCH(kPciInit, d.numBlocks, d); }
-hipError_t launchPciDensity(const DevView& d, hipStream_t st) {
-    hipLaunchKernelGGL(kPciDensity, dim3(d.numBlocks), dim3(kBlock), 0, st, d);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    LAUNCH(kPciMaxErr, d.numBlocks, d);
+hipError_t launchPciDensity(const DevView& d, int slot, hipStream_t st) { LAUNCH(kPciDensity, d.numBlocks, d, slot); }
+hipError_t launchPciPressureAcc(const DevView& d, int slot, hipStream_t st) {
+    LAUNCH(kPciPressureAcc, d.numBlocks, d, slot);
 }
-hipError_t launchPciPressureAcc(const DevView& d, hipStream_t st) { LAUNCH(kPciPressureAcc, d.numBlocks, d); }
 hipError_t launchPciIntegrate(const DevView& d, hipStream_t st) { LAUNCH(kPciIntegrate, d.numBlocks, d); }
 hipError_t launchCellsOut(const DevView& d, int32_t* cells, hipStream_t st) { LAUNCH(kCellsOut, d.numBlocks, d, cells); }
 hipError_t launchCheckFinite(const DevView& d, hipStream_t st) { LAUNCH(kCheckFinite, d.numBlocks, d); }

@@ -11,7 +11,7 @@ namespace sph {
 // Device views shared by the launches of one context.
 struct DevView {
     const SceneConst* scenes;
-    const SceneDyn* dyn;
+    SceneState* state;
     const int32_t* blockScene;  // scene of each fluid block
     int32_t numScenes;
     uint32_t numBlocks;         // fluid blocks (NP / kBlock)
@@ -23,6 +23,7 @@ struct DevView {
     float4* n;     // surface-tension normals
     float4* xs;    // PCISPH predicted positions
     float4* ap;    // PCISPH pressure acceleration
+    float4* xk;    // (x, k) of the current solver pass, read by kVelocity
     float* rho;
     float* alpha;
     float* p;
@@ -44,9 +45,8 @@ struct DevView {
     uint32_t* cntB;
     uint32_t capF, capB;
     // reductions and flags
-    double* partials;   // numBlocks * 2
+    double* partials;   // numBlocks
     uint32_t* counter;
-    double* red;        // numScenes * kRedSlots
     int32_t* flags;     // numScenes * kFlagSlots
 };
 
@@ -63,15 +63,16 @@ hipError_t launchForces(const DevView& d, int pressure, hipStream_t st);
 hipError_t launchSurfaceTension(const DevView& d, int vmax, hipStream_t st);
 hipError_t launchVmax(const DevView& d, hipStream_t st);
 hipError_t launchPredictVelocity(const DevView& d, hipStream_t st);
-hipError_t launchRhoStar(const DevView& d, int warm, hipStream_t st);
-hipError_t launchVelocity(const DevView& d, const float* kbuf, hipStream_t st);
-hipError_t launchDivergence(const DevView& d, int warm, hipStream_t st);
-hipError_t launchScale(const DevView& d, float* buf, hipStream_t st);
+// slot: iteration flag (0/1 by parity inside a Jacobi loop, 2 outside)
+hipError_t launchRhoStar(const DevView& d, int warm, int slot, hipStream_t st);
+hipError_t launchVelocity(const DevView& d, int slot, hipStream_t st);
+hipError_t launchDivergence(const DevView& d, int warm, int slot, hipStream_t st);
+hipError_t launchScaleDivergence(const DevView& d, hipStream_t st);
 hipError_t launchIntegrate(const DevView& d, hipStream_t st);
 
 hipError_t launchPciInit(const DevView& d, hipStream_t st);
-hipError_t launchPciDensity(const DevView& d, hipStream_t st);
-hipError_t launchPciPressureAcc(const DevView& d, hipStream_t st);
+hipError_t launchPciDensity(const DevView& d, int slot, hipStream_t st);
+hipError_t launchPciPressureAcc(const DevView& d, int slot, hipStream_t st);
 hipError_t launchPciIntegrate(const DevView& d, hipStream_t st);
 
 hipError_t launchCellsOut(const DevView& d, int32_t* cells, hipStream_t st);
