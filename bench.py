@@ -54,6 +54,32 @@ ALGO_BYTES = {
 }
 
 
+def calibration_copy(nbytes):
+    """One device-to-device copy of `nbytes` by the HIP runtime's copy kernel:
+    the reference point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE
+    counters to bytes (it reads and writes exactly `nbytes`)."""
+    import ctypes
+
+    hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (src, dst):
+        if hip.hipMalloc(ctypes.byref(ptr), nbytes) != 0:
+            raise RuntimeError("hipMalloc(%d) failed" % nbytes)
+    try:
+        if hip.hipMemset(src, 1, nbytes) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemset failed")
+        if hip.hipMemcpy(dst, src, nbytes, 3) != 0 or hip.hipDeviceSynchronize() != 0:  # device to device
+            raise RuntimeError("hipMemcpy failed")
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    log("calibration copy: %d bytes read, %d bytes written" % (nbytes, nbytes))
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -209,15 +235,7 @@ def main():
 
     ctx.close()
     if args.calibrate_copy > 0:
-        import torch
-
-        n = args.calibrate_copy << 20
-        src = torch.ones(n, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log("calibration copy: %d bytes read, %d bytes written" % (n, n))
+        calibration_copy(args.calibrate_copy << 20)
     batched = None
     if args.batch_steps > 0:
         log("batched gain: B64, %d steps" % args.batch_steps)

@@ -69,12 +69,12 @@ struct DevBuf {
 enum KernelId {
     K_KEYS, K_SORT, K_REORDER, K_BUILD, K_DENSITY, K_FORCES, K_SURFACE_TENSION, K_VMAX, K_PREDICT, K_RHO_STAR,
     K_VELOCITY, K_DIVERGENCE, K_SCALE, K_INTEGRATE, K_PCI_INIT, K_PCI_DENSITY, K_PCI_PACC, K_PCI_INTEGRATE,
-    K_FILL, K_COUNT
+    K_FILL, K_FINISH, K_COUNT
 };
 const char* kKernelNames[K_COUNT] = {"keys", "sort", "reorder", "build_neighbors", "density", "forces",
                                      "surface_tension", "vmax", "predict_velocity", "rho_star", "velocity",
                                      "divergence", "scale", "integrate", "pci_init", "pci_density",
-                                     "pci_pressure_acc", "pci_integrate", "fill_ranges"};
+                                     "pci_pressure_acc", "pci_integrate", "fill_ranges", "finish"};
 
 // Host mirror of the reference's FP32 cubic-spline gradient (CubicSpline.h:52-74)
 // with the compiled reference's length; used for the PCISPH scaling factor.
@@ -133,6 +133,7 @@ class Engine {
     std::string lastError;
     std::vector<SceneH> scenes;
     bool initialized = false;
+    bool interleaved = false;  // velocities live in xv[2i+1]
     int method = -1;
 
     size_t NP = 0, NB = 0, totalCells = 0;
@@ -143,7 +144,7 @@ class Engine {
     DevBuf<SceneConst> dScenes;
     DevBuf<SceneState> dState;
     DevBuf<int32_t> dBlockScene;
-    DevBuf<float4> x, xAlt, v, vAlt, a, n, xs, ap, xk, xb;
+    DevBuf<float4> x, xAlt, v, vAlt, a, n, xs, ap, xk, xb, xv, xvAlt;
     DevBuf<float> rho, alpha, p, k, kws, kwsAlt, kvws, kvwsAlt, aux, vb;
     DevBuf<uint32_t> id, idAlt, keys, keysAlt, vals, valsAlt;
     DevBuf<uint2> fRange, bRange;
@@ -151,7 +152,6 @@ class Engine {
     DevBuf<double> partials;
     DevBuf<float4> snapV;   // DFSPH: v and kv_ws before the passes that follow a neighbour search
     DevBuf<float> snapKvws;
-    DevBuf<uint32_t> counter;
     DevBuf<int32_t> flags;
     DevBuf<unsigned char> sortTemp;
     size_t sortTempBytes = 0;
@@ -430,14 +430,14 @@ class Engine {
         d.numScenes = static_cast<int32_t>(scenes.size());
         d.numBlocks = numBlocks;
         d.NP = static_cast<uint32_t>(NP);
-        d.x = x.p; d.v = v.p; d.a = a.p; d.n = n.p; d.xs = xs.p; d.ap = ap.p; d.xk = xk.p;
+        d.x = x.p; d.v = v.p; d.a = a.p; d.n = n.p; d.xs = xs.p; d.ap = ap.p; d.xk = xk.p; d.xv = xv.p;
         d.rho = rho.p; d.alpha = alpha.p; d.p = p.p; d.k = k.p; d.kws = kws.p; d.kvws = kvws.p; d.aux = aux.p;
         d.id = id.p;
         d.xb = xb.p; d.vb = vb.p;
         d.fRange = fRange.p; d.bRange = bRange.p;
         d.nbrF = nbrF.p; d.cntF = cntF.p; d.nbrB = nbrB.p; d.cntB = cntB.p;
         d.capF = capF; d.capB = capB;
-        d.partials = partials.p; d.counter = counter.p; d.flags = flags.p;
+        d.partials = partials.p; d.flags = flags.p;
         return d;
     }
 
@@ -481,6 +481,11 @@ class Engine {
         dBlockScene.alloc(bs.size());
         HIPCHK(hipMemcpy(dBlockScene.p, bs.data(), bs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         for (auto* b : {&x, &xAlt, &v, &vAlt, &a, &n, &xs, &ap, &xk}) b->alloc(np);
+        interleaved = sphKernelsInterleaveVelocity() != 0;
+        if (interleaved) {
+            xv.alloc(2 * np);
+            xvAlt.alloc(2 * np);
+        }
         for (auto* b : {&rho, &alpha, &p, &k, &kws, &kwsAlt, &kvws, &kvwsAlt, &aux}) b->alloc(np);
         for (auto* b : {&id, &idAlt, &keys, &keysAlt, &vals, &valsAlt, &cntF, &cntB}) b->alloc(np);
         allocNeighborLists();
@@ -489,9 +494,7 @@ class Engine {
             snapV.alloc(np);
             snapKvws.alloc(np);
         }
-        counter.alloc(1);
         flags.alloc(scenes.size() * kFlagSlots);
-        HIPCHK(hipMemset(counter.p, 0, sizeof(uint32_t)));
         HIPCHK(hipMemset(a.p, 0, np * sizeof(float4)));
         HIPCHK(hipMemset(n.p, 0, np * sizeof(float4)));
         for (auto* b : {&rho, &alpha, &p, &k, &kws, &kvws, &aux}) HIPCHK(hipMemset(b->p, 0, np * sizeof(float)));
@@ -522,6 +525,27 @@ class Engine {
         HIPCHK(hipStreamSynchronize(stream));
     }
 
+    // Velocities of slots [first, first + count) in slot order.
+    std::vector<float4> downloadVel(uint32_t first, uint32_t count) {
+        std::vector<float4> out(count);
+        HIPCHK(hipStreamSynchronize(stream));
+        if (!count) return out;
+        if (interleaved)
+            HIPCHK(hipMemcpy2D(out.data(), sizeof(float4), xv.p + 2 * static_cast<size_t>(first) + 1, 2 * sizeof(float4),
+                               sizeof(float4), count, hipMemcpyDeviceToHost));
+        else
+            HIPCHK(hipMemcpy(out.data(), v.p + first, count * sizeof(float4), hipMemcpyDeviceToHost));
+        return out;
+    }
+    void uploadVel(uint32_t first, const std::vector<float4>& vals) {
+        if (vals.empty()) return;
+        if (interleaved)
+            HIPCHK(hipMemcpy2D(xv.p + 2 * static_cast<size_t>(first) + 1, 2 * sizeof(float4), vals.data(), sizeof(float4),
+                               sizeof(float4), vals.size(), hipMemcpyHostToDevice));
+        else
+            HIPCHK(hipMemcpy(v.p + first, vals.data(), vals.size() * sizeof(float4), hipMemcpyHostToDevice));
+    }
+
     void uploadState() {
         const size_t np = NP ? NP : 1;
         std::vector<float4> hx(np, make_float4(0, 0, 0, 0)), hv(np, make_float4(0, 0, 0, 0));
@@ -535,7 +559,7 @@ class Engine {
             }
         // padding slots: park far away from nothing (never read as neighbours)
         HIPCHK(hipMemcpy(x.p, hx.data(), np * sizeof(float4), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(v.p, hv.data(), np * sizeof(float4), hipMemcpyHostToDevice));
+        uploadVel(0, hv);
         HIPCHK(hipMemcpy(id.p, hid.data(), np * sizeof(uint32_t), hipMemcpyHostToDevice));
     }
 
@@ -569,6 +593,11 @@ class Engine {
         HIPCHK(launch());
         HIPCHK(hipEventRecord(pr.second, stream));
     }
+    // per-scene combine of the preceding launch's block partials, then the
+    // loop-control or time-step epilogue
+    void finish(const DevView& d, int kind, int slot) {
+        timed(K_FINISH, [&] { return launchFinish(d, kind, slot, stream); });
+    }
     void collectProfile() {
         if (!evUsed) return;
         HIPCHK(hipStreamSynchronize(stream));
@@ -596,9 +625,12 @@ class Engine {
         });
         const uint32_t* ks = swapped ? keysAlt.p : keys.p;
         const uint32_t* vs = swapped ? valsAlt.p : vals.p;
-        timed(K_REORDER, [&] { return launchReorder(d, ks, vs, xAlt.p, vAlt.p, kwsAlt.p, kvwsAlt.p, idAlt.p, stream); });
+        timed(K_REORDER, [&] {
+            return launchReorder(d, ks, vs, xAlt.p, vAlt.p, xvAlt.p, kwsAlt.p, kvwsAlt.p, idAlt.p, stream);
+        });
         std::swap(x.p, xAlt.p);
         std::swap(v.p, vAlt.p);
+        std::swap(xv.p, xvAlt.p);
         std::swap(kws.p, kwsAlt.p);
         std::swap(kvws.p, kvwsAlt.p);
         std::swap(id.p, idAlt.p);
@@ -764,6 +796,7 @@ class Engine {
         if (anyVisc() || anyST()) timed(K_FORCES, [&] { return launchForces(d, 0, stream); });
         if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 1, stream); });
         else timed(K_VMAX, [&] { return launchVmax(d, stream); });
+        finish(d, kFinishTimeStep, 2);
         // 3. predictVelocities
         timed(K_PREDICT, [&] { return launchPredictVelocity(d, stream); });
         // 4. correctDensityError: warm start (and loop start), then the loop
@@ -772,7 +805,10 @@ class Engine {
         int first = 1;
         for (auto& sc : scenes) first = std::max(first, std::max(static_cast<int>(sc.P.min_iterations), sc.its));
         runLoop(first, maxOver(&sph_scene_params::max_iterations),
-                [&](int slot) { timed(K_RHO_STAR, [&] { return launchRhoStar(d, 0, slot, stream); }); },
+                [&](int slot) {
+                    timed(K_RHO_STAR, [&] { return launchRhoStar(d, 0, slot, stream); });
+                    finish(d, kFinishDensity, slot);
+                },
                 [&](int slot) { timed(K_VELOCITY, [&] { return launchVelocity(d, slot, stream); }); },
                 [] { return true; });
         for (size_t s = 0; s < scenes.size(); ++s) scenes[s].its = hState[s].it;
@@ -781,7 +817,9 @@ class Engine {
         searchOnce(true);
         for (int attempt = 0;; ++attempt) {
             d = view();
-            HIPCHK(hipMemcpyAsync(snapV.p, v.p, NP * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpy2DAsync(snapV.p, sizeof(float4), interleaved ? xv.p + 1 : v.p,
+                                    (interleaved ? 2 : 1) * sizeof(float4), sizeof(float4), NP,
+                                    hipMemcpyDeviceToDevice, stream));
             HIPCHK(hipMemcpyAsync(snapKvws.p, kvws.p, NP * sizeof(float), hipMemcpyDeviceToDevice, stream));
             // 8-9. computeDensities + computeAlpha
             timed(K_DENSITY, [&] { return launchDensityAlpha(d, 1, stream); });
@@ -792,13 +830,17 @@ class Engine {
             for (auto& sc : scenes) firstV = std::max(firstV, std::max(static_cast<int>(sc.P.min_iterations_v), sc.itsV));
             const bool ok = runLoop(
                 firstV, maxOver(&sph_scene_params::max_iterations),
-                [&](int slot) { timed(K_DIVERGENCE, [&] { return launchDivergence(d, 0, slot, stream); }); },
+                [&](int slot) {
+                    timed(K_DIVERGENCE, [&] { return launchDivergence(d, 0, slot, stream); });
+                    finish(d, kFinishDivergence, slot);
+                },
                 [&](int slot) { timed(K_VELOCITY, [&] { return launchVelocity(d, slot, stream); }); },
                 [&] { return searchValid(); });
             if (ok) break;
             if (attempt >= 5) throw SphError{SPH_E_OUT_OF_GRID, "could not place particles into a grid"};
             // the lists were invalid: undo the divergence passes and search again
-            HIPCHK(hipMemcpyAsync(v.p, snapV.p, NP * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpy2DAsync(interleaved ? xv.p + 1 : v.p, (interleaved ? 2 : 1) * sizeof(float4), snapV.p,
+                                    sizeof(float4), sizeof(float4), NP, hipMemcpyDeviceToDevice, stream));
             HIPCHK(hipMemcpyAsync(kvws.p, snapKvws.p, NP * sizeof(float), hipMemcpyDeviceToDevice, stream));
             searchOnce(false);
         }
@@ -841,7 +883,10 @@ class Engine {
         int first = 1;
         for (auto& sc : scenes) first = std::max(first, std::max(static_cast<int>(sc.P.min_iterations), sc.its));
         runLoop(first, maxOver(&sph_scene_params::max_iterations),
-                [&](int slot) { timed(K_PCI_DENSITY, [&] { return launchPciDensity(d, slot, stream); }); },
+                [&](int slot) {
+                    timed(K_PCI_DENSITY, [&] { return launchPciDensity(d, slot, stream); });
+                    finish(d, kFinishDensity, slot);
+                },
                 [&](int slot) { timed(K_PCI_PACC, [&] { return launchPciPressureAcc(d, slot, stream); }); },
                 [] { return true; });
         timed(K_PCI_INTEGRATE, [&] { return launchPciIntegrate(d, stream); });
@@ -899,7 +944,7 @@ class Engine {
         auto ids = download(id.p + s.fBegin, s.nf);
         if (pos || vel) {
             auto hx = pos ? download(x.p + s.fBegin, s.nf) : std::vector<float4>();
-            auto hv = vel ? download(v.p + s.fBegin, s.nf) : std::vector<float4>();
+            auto hv = vel ? downloadVel(s.fBegin, s.nf) : std::vector<float4>();
             for (uint32_t i = 0; i < s.nf; ++i) {
                 const uint32_t o = ids[i];
                 if (o >= s.nf) throw SphError{SPH_E_STATE, "corrupt particle ids"};
@@ -947,14 +992,17 @@ class Engine {
         }
         auto ids = download(id.p + s.fBegin, s.nf);
         auto hx = download(x.p + s.fBegin, s.nf);
-        auto hv = download(v.p + s.fBegin, s.nf);
+        auto hv = downloadVel(s.fBegin, s.nf);
         for (uint32_t i = 0; i < s.nf; ++i) {
             const uint32_t o = ids[i];
             if (pos) hx[i] = make_float4(pos[3 * o], pos[3 * o + 1], pos[3 * o + 2], 0.0f);
             if (vel) hv[i] = make_float4(vel[3 * o], vel[3 * o + 1], vel[3 * o + 2], 0.0f);
         }
         HIPCHK(hipMemcpy(x.p + s.fBegin, hx.data(), s.nf * sizeof(float4), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(v.p + s.fBegin, hv.data(), s.nf * sizeof(float4), hipMemcpyHostToDevice));
+        uploadVel(s.fBegin, hv);
+        if (interleaved)  // positions are duplicated in the interleaved records
+            HIPCHK(hipMemcpy2D(xv.p + 2 * static_cast<size_t>(s.fBegin), 2 * sizeof(float4), hx.data(), sizeof(float4),
+                               sizeof(float4), s.nf, hipMemcpyHostToDevice));
         if (method == SPH_DFSPH) {  // density and alpha belong to the positions
             neighborSearch(false);
             HIPCHK(launchDensityAlpha(view(), 1, stream));

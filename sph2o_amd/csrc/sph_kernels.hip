@@ -16,7 +16,7 @@
 // operations so the neighbour sets and the cohesion branch at r = h/2 match
 // bit for bit. Inside the smooth kernel sums the hardware sqrt/rcp are used.
 // Convergence and CFL reductions are FP64 and deterministic (fixed-order
-// block partials, finished by the last block of the launch).
+// block partials, combined per scene by a one-block-per-scene kFinish launch).
 #include <hip/hip_runtime.h>
 
 #include "sph_kernels.hpp"
@@ -42,9 +42,39 @@ __device__ __forceinline__ float fastSqrt(float v) { return __builtin_amdgcn_sqr
 #ifndef SPH_XK_VELOCITY
 #define SPH_XK_VELOCITY 0
 #endif
+#ifndef SPH_XCD_REMAP
+#define SPH_XCD_REMAP 1
+#endif
+#ifndef SPH_XV
+#define SPH_XV 0
+#endif
 #define SPH_PRAGMA(x) _Pragma(#x)
 #define SPH_UNROLL_LOOP_(n) SPH_PRAGMA(unroll n)
 #define SPH_UNROLL_LOOP SPH_UNROLL_LOOP_(SPH_UNROLL)
+// Logical block of this workgroup. Workgroups are dispatched round-robin over
+// the 8 XCDs, each with its own L2; with the remap every XCD walks one
+// contiguous eighth of the (Morton-sorted) particle range, so the neighbours
+// it gathers stay in its own L2.
+__device__ __forceinline__ uint32_t logicalBlock() {
+#if SPH_XCD_REMAP
+    const uint32_t n = gridDim.x, b = blockIdx.x;
+    const uint32_t q = n >> 3, r = n & 7u, x = b & 7u, k = b >> 3;
+    return x * q + min(x, r) + k;
+#else
+    return blockIdx.x;
+#endif
+}
+// Velocities: a separate array, or interleaved with positions (xv[2i] = x,
+// xv[2i+1] = v) so passes that gather both read one cache line per neighbour.
+#if SPH_XV
+#define VEL(dv, idx) (dv).xv[2 * static_cast<size_t>(idx) + 1]
+#define POSV(dv, idx) (dv).xv[2 * static_cast<size_t>(idx)]
+#else
+#define VEL(dv, idx) (dv).v[idx]
+#define POSV(dv, idx) (dv).x[idx]
+#endif
+extern "C" __host__ int sphKernelsInterleaveVelocity() { return SPH_XV; }
+
 __device__ __forceinline__ uint32_t ldList(const uint32_t* p) {
 #if SPH_NT_LISTS
     return __builtin_nontemporal_load(p);
@@ -132,14 +162,15 @@ __device__ __forceinline__ uint32_t cellKey(const SceneConst& S, int cx, int cy,
 // slot: which iteration flag of the scene state gates this launch (0/1 inside a
 // Jacobi loop, by iteration parity; 2 outside a loop, always set).
 #define SCENE_PROLOGUE(slot)                                  \
-    const int sIdx = d.blockScene[blockIdx.x];                \
+    const uint32_t lb = logicalBlock();                       \
+    const int sIdx = d.blockScene[lb];                        \
     const SceneConst& S = d.scenes[sIdx];                     \
     const SceneState& T = d.state[sIdx];                      \
     const float dt = T.dt, dt2 = T.dt2;                       \
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;     \
+    const uint32_t i = lb * kBlock + threadIdx.x;             \
     const bool valid = T.active[slot] != 0 && i < S.fEnd;
 
-// ---- FP64 deterministic reductions: block partial + last-block finish ----
+// ---- FP64 deterministic reductions: block partials + per-scene kFinish ----
 template <bool MAX>
 __device__ __forceinline__ double combine(double a, double b) {
     return MAX ? fmax(a, b) : a + b;
@@ -157,35 +188,16 @@ __device__ double blockReduce(double v) {
     return t;  // valid in every thread
 }
 
-// Every block of the launch calls this exactly once (blocks with nothing to
-// add pass 0). The last block to arrive sums each scene's block partials in a
-// fixed order and hands the total to epi(scene, total) on one thread, after
-// every other block of the launch has finished its work.
-template <bool MAX, class Epi>
-__device__ void finishReduction(const DevView& d, double v, Epi&& epi) {
+// Block partial of a reduction, stored at the block's logical index with a
+// plain store. kFinish, launched next on the same stream, combines each
+// scene's partials in a fixed order; stream order makes them visible, so no
+// block issues a device-scope fence (on gfx950 one writes back and
+// invalidates the whole L2 of the block's XCD, which costs the neighbour
+// gathers of every other block resident there their cached lines).
+template <bool MAX>
+__device__ __forceinline__ void storePartial(const DevView& d, double v) {
     const double t = blockReduce<MAX>(v);
-    __shared__ bool amLast;
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(&d.partials[blockIdx.x], t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const uint32_t ticket = atomicAdd(d.counter, 1u);
-        amLast = (ticket == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!amLast) return;
-    __threadfence();
-    for (int s = 0; s < d.numScenes; ++s) {
-        const SceneConst& S = d.scenes[s];
-        double acc = 0.0;
-        for (uint32_t b = S.blockBegin + threadIdx.x; b < S.blockEnd; b += kBlock)
-            acc = combine<MAX>(acc, __hip_atomic_load(&d.partials[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        const double tot = blockReduce<MAX>(acc);
-        if (threadIdx.x == 0) epi(s, tot);
-    }
-    if (threadIdx.x == 0) {
-        __threadfence();
-        __hip_atomic_store(d.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (threadIdx.x == 0) d.partials[logicalBlock()] = t;
 }
 
 // Jacobi-loop control of one scene after the error reduction of iteration
@@ -216,10 +228,11 @@ __device__ __forceinline__ void loopStart(SceneState& T, double err, double tol,
 // instead when the density loop did not run), and block 0 files the density
 // loop's statistics.
 __global__ void __launch_bounds__(kBlock) kKeys(DevView d, int dfsphAdvance, uint32_t* keys, uint32_t* vals) {
-    const int sIdx = d.blockScene[blockIdx.x];
+    const uint32_t lb = logicalBlock();
+    const int sIdx = d.blockScene[lb];
     const SceneConst& S = d.scenes[sIdx];
     const SceneState& T = d.state[sIdx];
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = lb * kBlock + threadIdx.x;
     if (dfsphAdvance && blockIdx.x == 0) {
         for (int s = threadIdx.x; s < d.numScenes; s += kBlock) {
             SceneState& U = d.state[s];
@@ -237,7 +250,7 @@ __global__ void __launch_bounds__(kBlock) kKeys(DevView d, int dfsphAdvance, uin
     float4 p = d.x[i];
     if (dfsphAdvance) {
         const float dt = T.dt;
-        const float4 vi = d.v[i];
+        const float4 vi = VEL(d, i);
         p.x += vi.x * dt;
         p.y += vi.y * dt;
         p.z += vi.z * dt;
@@ -256,15 +269,21 @@ __global__ void kFillRanges(uint2* r, size_t n) {
 
 // Gather the persistent state into sorted order and record cell ranges.
 __global__ void __launch_bounds__(kBlock) kReorder(DevView d, const uint32_t* keysSorted, const uint32_t* valsSorted,
-                                                   float4* xAlt, float4* vAlt, float* kwsAlt, float* kvwsAlt,
-                                                   uint32_t* idAlt) {
-    const int sIdx = d.blockScene[blockIdx.x];
+                                                   float4* xAlt, float4* vAlt, float4* xvAlt, float* kwsAlt,
+                                                   float* kvwsAlt, uint32_t* idAlt) {
+    const uint32_t lb = logicalBlock();
+    const int sIdx = d.blockScene[lb];
     const SceneConst& S = d.scenes[sIdx];
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = lb * kBlock + threadIdx.x;
     const uint32_t o = valsSorted[i];
     const float4 p = d.x[o];
     xAlt[i] = p;
-    vAlt[i] = d.v[o];
+#if SPH_XV
+    xvAlt[2 * static_cast<size_t>(i)] = p;
+    xvAlt[2 * static_cast<size_t>(i) + 1] = VEL(d, o);
+#else
+    vAlt[i] = VEL(d, o);
+#endif
     kwsAlt[i] = d.kws[o];
     kvwsAlt[i] = d.kvws[o];
     idAlt[i] = d.id[o];
@@ -445,7 +464,7 @@ __global__ void __launch_bounds__(kBlock) kForces(DevView d, int pressure) {
     if (!valid) return;
     const size_t NP = d.NP;
     const float4 xi = d.x[i];
-    const float4 vi = d.v[i];
+    const float4 vi = VEL(d, i);
     const float rhoi = d.rho[i];
     const uint32_t nf = min(d.cntF[i], d.capF);
     const uint32_t nb = S.akinci ? min(d.cntB[i], d.capB) : 0u;
@@ -457,7 +476,7 @@ __global__ void __launch_bounds__(kBlock) kForces(DevView d, int pressure) {
 SPH_UNROLL_LOOP
         for (uint32_t s = 0; s < nf; ++s) {
             const uint32_t j = ldList(d.nbrF + s * NP + i);
-            const float4 xj = d.x[j];
+            const float4 xj = POSV(d, j);
             const float rhoj = d.rho[j];
             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
             const float g = S.m * (dpi + d.p[j] / (rhoj * rhoj)) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
@@ -479,13 +498,13 @@ SPH_UNROLL_LOOP
 SPH_UNROLL_LOOP
         for (uint32_t s = 0; s < nf; ++s) {
             const uint32_t j = ldList(d.nbrF + s * NP + i);
-            const float4 xj = d.x[j];
+            const float4 xj = POSV(d, j);
             const float rhoj = d.rho[j];
             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
             const float r2 = rx * rx + ry * ry + rz * rz;
             const float rl = fastSqrt(r2);
             if (vm >= 0) {
-                const float4 vj = d.v[j];
+                const float4 vj = VEL(d, j);
                 if (vm == 2) {  // XSPH: nu (m/rho_j) (v_j - v_i) W / dt
                     const float c = S.visc * (S.m / rhoj) * cubicW(S, rl) * invdt;
                     fx += c * (vj.x - vi.x); fy += c * (vj.y - vi.y); fz += c * (vj.z - vi.z);
@@ -539,8 +558,8 @@ SPH_UNROLL_LOOP
 }
 
 // DFSPHSolver::updateTimeStep (DFSPHSolver.cpp:1090-1162) on the FP64 maximum
-// of |v + (a + g) dt|, evaluated in FP32 as the reference does. Runs in the
-// last block of the reduction, after every block has used the old dt.
+// of |v + (a + g) dt|, evaluated in FP32 as the reference does. Runs in
+// kFinish, after every block of the reducing launch has used the old dt.
 __device__ __forceinline__ void timeStepEpilogue(const DevView& d, int s, double vmax) {
     SceneState& T = d.state[s];
     const SceneConst& S = d.scenes[s];
@@ -592,39 +611,39 @@ SPH_UNROLL_LOOP
             d.a[i] = ai;
         }
         if (vmax) {
-            const float4 vi = d.v[i];
+            const float4 vi = VEL(d, i);
             const float cx = vi.x + (ai.x + S.gx) * dt;
             const float cy = vi.y + (ai.y + S.gy) * dt;
             const float cz = vi.z + (ai.z + S.gz) * dt;
             vm = sqrt(static_cast<double>(cx) * cx + static_cast<double>(cy) * cy + static_cast<double>(cz) * cz);
         }
     }
-    if (vmax) finishReduction<true>(d, vm, [&](int s, double tot) { timeStepEpilogue(d, s, tot); });
+    if (vmax) storePartial<true>(d, vm);
 }
 
 __global__ void __launch_bounds__(kBlock) kVmax(DevView d) {
     SCENE_PROLOGUE(2)
     double vm = 0.0;
     if (valid) {
-        const float4 vi = d.v[i], ai = d.a[i];
+        const float4 vi = VEL(d, i), ai = d.a[i];
         const float cx = vi.x + (ai.x + S.gx) * dt;
         const float cy = vi.y + (ai.y + S.gy) * dt;
         const float cz = vi.z + (ai.z + S.gz) * dt;
         vm = sqrt(static_cast<double>(cx) * cx + static_cast<double>(cy) * cy + static_cast<double>(cz) * cz);
     }
-    finishReduction<true>(d, vm, [&](int s, double tot) { timeStepEpilogue(d, s, tot); });
+    storePartial<true>(d, vm);
 }
 
 // DFSPHSolver::predictVelocities (DFSPHSolver.cpp:180-202)
 __global__ void __launch_bounds__(kBlock) kPredictVelocity(DevView d) {
     SCENE_PROLOGUE(2)
     if (!valid) return;
-    float4 vi = d.v[i];
+    float4 vi = VEL(d, i);
     const float4 ai = d.a[i];
     vi.x += (ai.x + S.gx) * dt;
     vi.y += (ai.y + S.gy) * dt;
     vi.z += (ai.z + S.gz) * dt;
-    d.v[i] = vi;
+    VEL(d, i) = vi;
     d.a[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
@@ -645,14 +664,14 @@ __global__ void __launch_bounds__(kBlock) kRhoStar(DevView d, int warm, int slot
     if (valid) {
         const size_t NP = d.NP;
         const float4 xi = d.x[i];
-        const float4 vi = d.v[i];
+        const float4 vi = VEL(d, i);
         float dc = 0.0f;
         const uint32_t nf = min(d.cntF[i], d.capF);
 SPH_UNROLL_LOOP
         for (uint32_t s = 0; s < nf; ++s) {
             const uint32_t j = ldList(d.nbrF + s * NP + i);
-            const float4 xj = d.x[j];
-            const float4 vj = d.v[j];
+            const float4 xj = POSV(d, j);
+            const float4 vj = VEL(d, j);
             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
             const float g = gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
             dc += S.m * (g * ((vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz));
@@ -680,11 +699,7 @@ SPH_UNROLL_LOOP
         }
         d.xk[i] = make_float4(xi.x, xi.y, xi.z, kOut);
     }
-    if (!warm)
-        finishReduction<false>(d, err, [&](int s, double sum) {
-            const SceneConst& C = d.scenes[s];
-            loopEpilogue(d.state[s], slot, C.nf ? sum / C.nf : 0.0, static_cast<double>(C.eta), C.minIt, C.maxIt);
-        });
+    if (!warm) storePartial<false>(d, err);
 }
 
 // The velocity correction shared by both solves and both warm starts
@@ -725,11 +740,11 @@ SPH_UNROLL_LOOP
             sx += c * rx; sy += c * ry; sz += c * rz;
         }
     }
-    float4 vi = d.v[i];
+    float4 vi = VEL(d, i);
     vi.x -= dt * sx;
     vi.y -= dt * sy;
     vi.z -= dt * sz;
-    d.v[i] = vi;
+    VEL(d, i) = vi;
 }
 
 // DFSPHSolver::computeDivergenceError (DFSPHSolver.cpp:538-621); the
@@ -751,15 +766,15 @@ __global__ void __launch_bounds__(kBlock) kDivergence(DevView d, int warm, int s
     if (valid) {
         const size_t NP = d.NP;
         const float4 xi = d.x[i];
-        const float4 vi = d.v[i];
+        const float4 vi = VEL(d, i);
         float de = 0.0f;
         const uint32_t nfTrue = d.cntF[i];
         const uint32_t nf = min(nfTrue, d.capF);
 SPH_UNROLL_LOOP
         for (uint32_t s = 0; s < nf; ++s) {
             const uint32_t j = ldList(d.nbrF + s * NP + i);
-            const float4 xj = d.x[j];
-            const float4 vj = d.v[j];
+            const float4 xj = POSV(d, j);
+            const float4 vj = VEL(d, j);
             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
             const float g = gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
             de += S.m * (g * ((vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz));
@@ -791,12 +806,7 @@ SPH_UNROLL_LOOP
         }
         d.xk[i] = make_float4(xi.x, xi.y, xi.z, kOut);
     }
-    if (!warm)
-        finishReduction<false>(d, err, [&](int s, double sum) {
-            const SceneConst& C = d.scenes[s];
-            SceneState& U = d.state[s];
-            loopEpilogue(U, slot, C.nf ? sum / C.nf : 0.0, static_cast<double>(C.etaV / U.dt), C.minItV, C.maxIt);
-        });
+    if (!warm) storePartial<false>(d, err);
 }
 
 // kv_ws *= ts after the divergence loop (DFSPHSolver.cpp:829-843; cleared if
@@ -822,11 +832,11 @@ __global__ void __launch_bounds__(kBlock) kIntegrate(DevView d) {
     if (blockIdx.x == 0)
         for (int s = threadIdx.x; s < d.numScenes; s += kBlock) d.state[s].t = d.state[s].t + d.state[s].dt;
     if (!valid) return;
-    float4 ai = d.a[i], vi = d.v[i], xi = d.x[i];
+    float4 ai = d.a[i], vi = VEL(d, i), xi = d.x[i];
     ai.x += S.gx; ai.y += S.gy; ai.z += S.gz;
     vi.x += ai.x * dt; vi.y += ai.y * dt; vi.z += ai.z * dt;
     xi.x += vi.x * dt; xi.y += vi.y * dt; xi.z += vi.z * dt;
-    d.v[i] = vi;
+    VEL(d, i) = vi;
     d.x[i] = xi;
     d.a[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
@@ -846,7 +856,7 @@ __global__ void __launch_bounds__(kBlock) kPciInit(DevView d) {
     if (!valid) return;
     d.p[i] = 0.0f;
     d.ap[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float4 xi = d.x[i], vi = d.v[i], ai = d.a[i];
+    const float4 xi = d.x[i], vi = VEL(d, i), ai = d.a[i];
     const float vx = vi.x + (ai.x + S.gx) * dt;
     const float vy = vi.y + (ai.y + S.gy) * dt;
     const float vz = vi.z + (ai.z + S.gz) * dt;
@@ -896,10 +906,7 @@ __global__ void __launch_bounds__(kBlock) kPciDensity(DevView d, int slot) {
         d.p[i] += S.pciDelta * e;
         err = static_cast<double>(e / S.rho0);
     }
-    finishReduction<false>(d, err, [&](int s, double sum) {
-        const SceneConst& C = d.scenes[s];
-        loopEpilogue(d.state[s], slot, C.nf ? sum / C.nf : 0.0, static_cast<double>(C.eta), C.minIt, C.maxIt);
-    });
+    storePartial<false>(d, err);
 }
 
 // computePressureAcc (PCISPHSolver.cpp:300-367) at current positions with
@@ -930,7 +937,7 @@ SPH_UNROLL_LOOP
         ax -= g * rx; ay -= g * ry; az -= g * rz;
     }
     d.ap[i] = make_float4(ax, ay, az, 0.0f);
-    const float4 vi = d.v[i], ai = d.a[i];
+    const float4 vi = VEL(d, i), ai = d.a[i];
     const float vx = vi.x + ((ai.x + ax) + S.gx) * dt;
     const float vy = vi.y + ((ai.y + ay) + S.gy) * dt;
     const float vz = vi.z + ((ai.z + az) + S.gz) * dt;
@@ -951,21 +958,22 @@ __global__ void __launch_bounds__(kBlock) kPciIntegrate(DevView d) {
             U.t = U.t + U.dt;
         }
     if (!valid) return;
-    float4 ai = d.a[i], vi = d.v[i], xi = d.x[i];
+    float4 ai = d.a[i], vi = VEL(d, i), xi = d.x[i];
     const float4 pa = d.ap[i];
     ai.x += S.gx + pa.x; ai.y += S.gy + pa.y; ai.z += S.gz + pa.z;
     vi.x += ai.x * dt; vi.y += ai.y * dt; vi.z += ai.z * dt;
     xi.x += vi.x * dt; xi.y += vi.y * dt; xi.z += vi.z * dt;
-    d.v[i] = vi;
+    VEL(d, i) = vi;
     d.x[i] = xi;
     d.a[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // ---------------------------------------------------------------- utilities
 __global__ void __launch_bounds__(kBlock) kCellsOut(DevView d, int32_t* cells) {
-    const int sIdx = d.blockScene[blockIdx.x];
+    const uint32_t lb = logicalBlock();
+    const int sIdx = d.blockScene[lb];
     const SceneConst& S = d.scenes[sIdx];
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = lb * kBlock + threadIdx.x;
     if (i >= S.fEnd) return;
     const float4 p = d.x[i];
     cells[3 * i] = floorCell(p.x, S.invh);
@@ -974,13 +982,41 @@ __global__ void __launch_bounds__(kBlock) kCellsOut(DevView d, int32_t* cells) {
 }
 
 __global__ void __launch_bounds__(kBlock) kCheckFinite(DevView d) {
-    const int sIdx = d.blockScene[blockIdx.x];
+    const uint32_t lb = logicalBlock();
+    const int sIdx = d.blockScene[lb];
     const SceneConst& S = d.scenes[sIdx];
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = lb * kBlock + threadIdx.x;
     if (i >= S.fEnd) return;
-    const float4 p = d.x[i], v = d.v[i];
+    const float4 p = d.x[i], v = VEL(d, i);
     if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(v.x) && isfinite(v.y) && isfinite(v.z)))
         atomicOr(&d.flags[sIdx * kFlagSlots + kFlagNonFinite], 1);
+}
+
+// ---------------------------------------------------------------- reductions
+// One block per scene: the scene's block partials of the preceding launch in
+// a fixed order (thread t takes blocks t, t+256, ...; then the block tree),
+// the same order whatever other scenes share the launch, then the epilogue:
+// density / PCISPH loop control (eta), divergence loop control (etaV / dt,
+// DFSPHSolver.cpp:746), or the CFL time step.
+__global__ void __launch_bounds__(kBlock) kFinish(DevView d, int kind, int slot) {
+    const int s = blockIdx.x;
+    const SceneConst& C = d.scenes[s];
+    const bool mx = kind == kFinishTimeStep;
+    double acc = 0.0;
+    for (uint32_t b = C.blockBegin + threadIdx.x; b < C.blockEnd; b += kBlock) {
+        const double p = d.partials[b];
+        acc = mx ? fmax(acc, p) : acc + p;
+    }
+    const double tot = mx ? blockReduce<true>(acc) : blockReduce<false>(acc);
+    if (threadIdx.x != 0) return;
+    SceneState& U = d.state[s];
+    const double mean = C.nf ? tot / C.nf : 0.0;
+    if (kind == kFinishDensity)
+        loopEpilogue(U, slot, mean, static_cast<double>(C.eta), C.minIt, C.maxIt);
+    else if (kind == kFinishDivergence)
+        loopEpilogue(U, slot, mean, static_cast<double>(C.etaV / U.dt), C.minItV, C.maxIt);
+    else
+        timeStepEpilogue(d, s, tot);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -998,9 +1034,9 @@ hipError_t launchFillRanges(uint2* r, size_t n, hipStream_t st) {
     if (g == 0) return hipSuccess;
     LAUNCH(kFillRanges, g, r, n);
 }
-hipError_t launchReorder(const DevView& d, const uint32_t* ks, const uint32_t* vs, float4* xa, float4* va, float* kwa,
-                         float* kva, uint32_t* ida, hipStream_t st) {
-    LAUNCH(kReorder, d.numBlocks, d, ks, vs, xa, va, kwa, kva, ida);
+hipError_t launchReorder(const DevView& d, const uint32_t* ks, const uint32_t* vs, float4* xa, float4* va, float4* xva,
+                         float* kwa, float* kva, uint32_t* ida, hipStream_t st) {
+    LAUNCH(kReorder, d.numBlocks, d, ks, vs, xa, va, xva, kwa, kva, ida);
 }
 hipError_t launchBuildNeighbors(const DevView& d, hipStream_t st) { LAUNCH(kBuildNeighbors, d.numBlocks, d); }
 hipError_t launchBoundaryVolume(const DevView& d, int scene, uint32_t b0, uint32_t b1, hipStream_t st) {
@@ -1031,6 +1067,9 @@ hipError_t launchPciPressureAcc(const DevView& d, int slot, hipStream_t st) {
     LAUNCH(kPciPressureAcc, d.numBlocks, d, slot);
 }
 hipError_t launchPciIntegrate(const DevView& d, hipStream_t st) { LAUNCH(kPciIntegrate, d.numBlocks, d); }
+hipError_t launchFinish(const DevView& d, int kind, int slot, hipStream_t st) {
+    LAUNCH(kFinish, d.numScenes, d, kind, slot);
+}
 hipError_t launchCellsOut(const DevView& d, int32_t* cells, hipStream_t st) { LAUNCH(kCellsOut, d.numBlocks, d, cells); }
 hipError_t launchCheckFinite(const DevView& d, hipStream_t st) { LAUNCH(kCheckFinite, d.numBlocks, d); }
 

@@ -24,6 +24,7 @@ struct DevView {
     float4* xs;    // PCISPH predicted positions
     float4* ap;    // PCISPH pressure acceleration
     float4* xk;    // (x, k) of the current solver pass, read by kVelocity
+    float4* xv;    // interleaved (x, v) records when the kernels are built that way
     float* rho;
     float* alpha;
     float* p;
@@ -46,14 +47,13 @@ struct DevView {
     uint32_t capF, capB;
     // reductions and flags
     double* partials;   // numBlocks
-    uint32_t* counter;
     int32_t* flags;     // numScenes * kFlagSlots
 };
 
 hipError_t launchKeys(const DevView& d, int dfsphAdvance, uint32_t* keys, uint32_t* vals, hipStream_t st);
 hipError_t launchFillRanges(uint2* r, size_t n, hipStream_t st);
 hipError_t launchReorder(const DevView& d, const uint32_t* keysSorted, const uint32_t* valsSorted, float4* xAlt,
-                         float4* vAlt, float* kwsAlt, float* kvwsAlt, uint32_t* idAlt, hipStream_t st);
+                         float4* vAlt, float4* xvAlt, float* kwsAlt, float* kvwsAlt, uint32_t* idAlt, hipStream_t st);
 hipError_t launchBuildNeighbors(const DevView& d, hipStream_t st);
 hipError_t launchBoundaryVolume(const DevView& d, int scene, uint32_t bBegin, uint32_t bEnd, hipStream_t st);
 
@@ -75,10 +75,19 @@ hipError_t launchPciDensity(const DevView& d, int slot, hipStream_t st);
 hipError_t launchPciPressureAcc(const DevView& d, int slot, hipStream_t st);
 hipError_t launchPciIntegrate(const DevView& d, hipStream_t st);
 
+// Reductions: the launches above that reduce (non-warm rho_star / divergence,
+// surface_tension with vmax, vmax, pci_density) leave block partials; the
+// caller launches kFinish with the matching kind next on the same stream.
+enum FinishKind { kFinishDensity = 0, kFinishDivergence = 1, kFinishTimeStep = 2 };
+hipError_t launchFinish(const DevView& d, int kind, int slot, hipStream_t st);
+
 hipError_t launchCellsOut(const DevView& d, int32_t* cells, hipStream_t st);
 hipError_t launchCheckFinite(const DevView& d, hipStream_t st);
 
 }  // namespace sph
+
+// 1 if the kernels keep velocities interleaved with positions (xv), else 0.
+extern "C" int sphKernelsInterleaveVelocity();
 
 namespace sph {
 // Stable LSD radix sort of (key, value) pairs over the low `bits` key bits
