@@ -119,6 +119,54 @@ def cpu_baseline(scene_path, nf, steps, threads):
                       "steps 1-%d, step() calls only (%.1f s)" % (steps, secs)}
 
 
+def adjoint_cost(steps, every, warmup=5):
+    """A100k: gradient of L = mean |x_T - y|^2 w.r.t. v0, viscosity and surface
+    tension through `steps` WCSPH steps (100,000 particles, XSPH 0.01, Akinci
+    surface tension 0.1, dt = 2e-4: the largest of 5e-4 / 2.5e-4 / 2e-4 that
+    stays stable for k = 1000 at r = 0.005). Times the plain forward, the
+    recording forward (checkpoint every `every` steps) and the reverse sweep
+    (segment recompute + adjoint passes) on the same initial state."""
+    from sph2o_amd import scenes as SC
+    from sph2o_amd.simulation import Context
+
+    sc, _, _ = SC.a100k(0, dt=2e-4)
+    td = tempfile.mkdtemp()
+    path = os.path.join(td, "a100k.json")
+    SC.write_json(sc, path)
+    ctx = Context(0)
+    sid = ctx.load_scene_json(path)
+    ctx.init()
+    x0 = ctx.fluid_state(sid)["x"]
+    v0 = np.zeros_like(x0)
+    n = len(x0)
+    ctx.step(warmup)
+    ctx.set_fluid_state(sid, x0, v0)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.synchronize()
+    t_fwd = time.perf_counter() - t0
+    ctx.set_fluid_state(sid, x0, v0)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.forward_record(steps, every)
+    ctx.synchronize()
+    t_rec = time.perf_counter() - t0
+    xT = ctx.fluid_state(sid)["x"].astype(np.float64)
+    y = SC.adjoint_target(x0)
+    seed = (2.0 * (xT - y) / n).astype(np.float32)
+    t0 = time.perf_counter()
+    g = ctx.adjoint({sid: (seed, None)})[sid]
+    t_adj = time.perf_counter() - t0
+    ctx.close()
+    return {"workload": "A100k: WCSPH, %d fluid particles, %d steps at dt=2e-4, checkpoint every %d" % (n, steps, every),
+            "forward_s": t_fwd, "record_s": t_rec, "adjoint_s": t_adj, "record_over_forward": t_rec / t_fwd,
+            "adjoint_over_forward": t_adj / t_fwd,
+            "loss": float(np.mean(np.sum((xT - y) ** 2, axis=1))),
+            "dL_dviscosity": g["viscosity"], "dL_dsurface_tension": g["surface_tension"],
+            "dL_dv0_norm": float(np.linalg.norm(g["v0"]))}
+
+
 def batched_gain(steps, warmup=3):
     """B64: 64 DFSPH scenes (2k..200k particles) advanced together in one
     context vs the same scenes as 64 sequential single-scene contexts."""
@@ -174,6 +222,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--batch-steps", type=int, default=20, help="B64 batched-gain steps (0 = skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=200, help="A100k adjoint-cost steps (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="finish with a device-to-device copy of this size (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -241,6 +290,11 @@ def main():
         log("batched gain: B64, %d steps" % args.batch_steps)
         batched = batched_gain(args.batch_steps)
 
+    adjoint = None
+    if args.adjoint_steps > 0:
+        log("adjoint cost: A100k, %d steps" % args.adjoint_steps)
+        adjoint = adjoint_cost(args.adjoint_steps, 20)
+
     cpu = None
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
@@ -264,7 +318,8 @@ def main():
                      "avg_launch_ms": avg_ms, "launches": klaunch, "algorithmic_bytes_per_launch": bytes_per_launch},
         "batched_gain": batched["gain"] if batched else None,
         "batched": batched,
-        "adjoint_fwd_cost": None,
+        "adjoint_fwd_cost": adjoint["adjoint_over_forward"] if adjoint else None,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

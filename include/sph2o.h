@@ -164,6 +164,43 @@ const char* sph_kernel_name(int kernel_id);
 int sph_kernel_stats(const sph_ctx* ctx, int kernel_id, double* total_ms, int64_t* launches);
 int sph_reset_kernel_stats(sph_ctx* ctx);
 
+/* ---- Gradients of a WCSPH rollout. The reference has no derivatives: these
+ * entry points have no counterpart there; they serve optimisation / inverse
+ * design callers and the torch.autograd.Function in sph2o_amd/autograd.py.
+ * Supported: WCSPH (WCSPHSolver::step, WCSPHSolver.cpp:14-78) with Akinci
+ * boundaries, viscosity off or XSPH (XSPHViscosity.cpp:6-42) and Akinci
+ * surface tension (SurfaceTensionAkinci.cpp:5-71); other options give
+ * SPH_E_UNSUPPORTED. ---- */
+
+/* Sets a scene's viscosity and surface-tension coefficients (the "viscosity"
+ * and "surfaceTension" values SceneLoader.cpp:95-110 hands to the force
+ * objects). A force acts iff its method is > -1 and its coefficient is
+ * non-zero (SPHSimulation.cpp:390-394); gradients are reported for acting
+ * forces. Invalidates a recorded rollout. */
+int sph_set_coefficients(sph_ctx* ctx, int scene, float viscosity, float surface_tension);
+
+/* Advances every scene nsteps exactly as sph_step does and records the
+ * rollout: a checkpoint of the state every checkpoint_every steps, plus a tape
+ * (state, densities, pressures, normals, neighbour lists) of one segment at a
+ * time, ~(52 + 4 * list capacity) bytes per particle per step of a segment.
+ * stats as in sph_step. */
+int sph_forward_record(sph_ctx* ctx, int nsteps, int checkpoint_every, sph_step_stats* stats);
+
+/* Loss gradient w.r.t. the state at the end of the recorded rollout, per
+ * scene: dL/dx_T and dL/dv_T, N*3 floats in original order (NULL = zero). */
+int sph_adjoint_seed(sph_ctx* ctx, int scene, const float* dL_dx, const float* dL_dv);
+
+/* Reverse sweep: recomputes each segment's tape from its checkpoint and runs
+ * the adjoint of every step backwards. Afterwards the context holds the state
+ * at the end of the rollout again. */
+int sph_adjoint_run(sph_ctx* ctx);
+
+/* Results of the last sph_adjoint_run for one scene: dL/dx_0 and dL/dv_0
+ * (N*3, original order, NULL to skip) w.r.t. the state at the start of the
+ * rollout, dL/d(viscosity) and dL/d(surface tension) (NULL to skip). */
+int sph_adjoint_result(sph_ctx* ctx, int scene, float* dL_dx0, float* dL_dv0, double* dL_dviscosity,
+                       double* dL_dsurface_tension);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,6 +4,7 @@
 
 #include <chrono>
 #include <cstring>
+#include <type_traits>
 #include <memory>
 
 using namespace sphref;
@@ -60,6 +61,55 @@ double sphref_step(void* hp, int nsteps, double* stats) {
         return 0;
     });
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Overwrite positions / velocities (N*3 doubles, either may be NULL) at full
+// precision: the FP64 oracle's finite differences perturb them below FP32 ulps.
+void sphref_set_state64(void* hp, const double* x, const double* v) {
+    with(static_cast<Handle*>(hp), [&](auto& s) {
+        using R = decltype(s.x[0].x);
+        const size_t n = s.x.size();
+        for (size_t i = 0; i < n; ++i) {
+            if (x) s.x[i] = {static_cast<R>(x[3 * i]), static_cast<R>(x[3 * i + 1]), static_cast<R>(x[3 * i + 2])};
+            if (v) s.v[i] = {static_cast<R>(v[3 * i]), static_cast<R>(v[3 * i + 1]), static_cast<R>(v[3 * i + 2])};
+        }
+        return 0;
+    });
+}
+
+// WCSPH pressure branch of the next `nsteps` steps counted from init:
+// mask[k * N + i] for step k + 1 (see Sim::pBranch). nsteps = 0 clears it.
+void sphref_set_pressure_branch(void* hp, int nsteps, const uint8_t* mask) {
+    with(static_cast<Handle*>(hp), [&](auto& s) {
+        const size_t n = s.x.size();
+        s.pBranch.assign(static_cast<size_t>(nsteps), std::vector<uint8_t>(n));
+        for (int k = 0; k < nsteps; ++k) std::copy(mask + k * n, mask + (k + 1) * n, s.pBranch[k].begin());
+        return 0;
+    });
+}
+
+// FP32 positions deciding the cohesion branch of the next `nsteps` steps
+// counted from init: xyz[(k * N + i) * 3 + c] for step k + 1 (see Sim::cohPos).
+void sphref_set_cohesion_positions(void* hp, int nsteps, const float* xyz) {
+    with(static_cast<Handle*>(hp), [&](auto& s) {
+        const size_t n = s.x.size();
+        s.cohPos.assign(static_cast<size_t>(nsteps), std::vector<V3<float>>(n));
+        for (int k = 0; k < nsteps; ++k)
+            for (size_t i = 0; i < n; ++i) {
+                const float* q = xyz + (k * n + i) * 3;
+                s.cohPos[k][i] = V3<float>(q[0], q[1], q[2]);
+            }
+        return 0;
+    });
+}
+
+// Inner-branch cohesion fingerprint since init: out[0] = hash, out[1] = pairs.
+void sphref_cohesion_print(void* hp, uint64_t* out) {
+    with(static_cast<Handle*>(hp), [&](auto& s) {
+        out[0] = s.innerHash;
+        out[1] = static_cast<uint64_t>(s.innerPairs);
+        return 0;
+    });
 }
 
 // Any output pointer may be NULL. Vectors are N*3, scalars N.

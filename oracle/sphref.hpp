@@ -86,6 +86,13 @@ struct Params {
     double viscosity, bviscosity, surface_tension;
 };
 
+// SplitMix64 finaliser (a fixed bijection of 64-bit words).
+inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
 // Cell key: floor(pos/h) cast to int (HashTable.cpp:5).
 struct Key {
     int x, y, z;
@@ -145,6 +152,20 @@ class Sim {
     std::vector<V> xs, vs, ap;
     R delta = 0, pciAvg = 0, pciMax = 0;
     int steps = 0;
+    // Fingerprint of the pairs that took the inner (r <= h/2) branch of the
+    // cohesion kernel, over all steps: the reference form jumps 2x there, so
+    // finite differences are only meaningful between runs with equal prints.
+    uint64_t innerHash = 0;
+    int64_t innerPairs = 0;
+    // Optional WCSPH pressure branch per step (1: p = B((rho/rho0)^g - 1),
+    // 0: p = 0) replacing max(0, .) (WCSPHSolver.cpp:93) for steps 1..size():
+    // finite differences then follow the branch another run (an FP32 one)
+    // took at particles whose density sits within rounding of rho0.
+    std::vector<std::vector<uint8_t>> pBranch;
+    // Optional per-step FP32 positions (steps 1..size()) that decide each
+    // pair's cohesion branch (|r| <= h/2 in the reference's FP32 length)
+    // instead of this run's own positions, for the same purpose.
+    std::vector<std::vector<V3<float>>> cohPos;
 
     // ---- kernels (CubicSpline.h:27-74, ViscoK.h:29-37, Cohesion.h:20-40) ----
     void setKernels() {
@@ -186,6 +207,12 @@ class Sim {
     R laplW(const V& rv) const {
         const R mr = length(rv);
         return mr <= h ? laplC * (h - mr) : 0;
+    }
+    // Cohesion::W at length mr (0 < mr <= h) on a given branch
+    R cohesionOn(R mr, bool inner) const {
+        const R a1 = h - mr;
+        if (inner) return cohC1 * static_cast<R>(2) * (a1 * a1 * a1) * (mr * mr * mr) - cohC2;
+        return cohC1 * (a1 * a1 * a1) * (mr * mr * mr);
     }
     R cohesion(const V& rv) const {
         const R mr = length(rv);
@@ -269,6 +296,7 @@ class Sim {
         k.assign(nf, 0); kv.assign(nf, 0); kws.assign(nf, 0); kvws.assign(nf, 0);
         xs.assign(nf, V()); vs.assign(nf, V()); ap.assign(nf, V());
         t = 0; its = 0; itsV = 0; avgErr = 0; avgDiv = 0; pciAvg = 0; pciMax = 0; steps = 0;
+        innerHash = 0; innerPairs = 0;
     }
 
     void init() {
@@ -424,7 +452,12 @@ class Sim {
             ni *= h;
             n[i] = ni;
         }
-#pragma omp parallel for schedule(static)
+        uint64_t hsh = 0;
+        int64_t cnt = 0;
+        const uint64_t stepKey = static_cast<uint64_t>(steps) << 42;
+        const V3<float>* bp = steps - 1 < static_cast<int>(cohPos.size()) ? cohPos[steps - 1].data() : nullptr;
+        const float halfH = 0.5f * static_cast<float>(h);
+#pragma omp parallel for schedule(static) reduction(+ : hsh, cnt)
         for (int i = 0; i < nf; ++i) {
             const V ri = x[i], ni = n[i];
             const R di = rho[i];
@@ -434,14 +467,21 @@ class Sim {
                 const R rl = length(rij);
                 const R kij = static_cast<R>(2.0) * rho0 / (di + rho[j]);
                 if (rl > 0 && rl <= h) {
+                    const bool inner = bp ? length(bp[i] - bp[j]) <= halfH : 2 * rl <= h;
                     V acc;
-                    acc -= (gam * m * cohesion(rij)) * rij / rl;
+                    acc -= (gam * m * (bp ? cohesionOn(rl, inner) : cohesion(rij))) * rij / rl;
                     acc -= gam * (ni - n[j]);
                     ai += kij * acc;
+                    if (inner) {
+                        hsh += mix64(stepKey ^ (static_cast<uint64_t>(i) << 21) ^ static_cast<uint64_t>(j));
+                        ++cnt;
+                    }
                 }
             });
             a[i] = ai;
         }
+        innerHash += hsh;
+        innerPairs += cnt;
     }
 
     // ---- WCSPH (WCSPHSolver.cpp:14-188; SPHSolver::integrate :139-168) ----
@@ -451,9 +491,13 @@ class Sim {
         computeDensities();
         const int nf = static_cast<int>(x.size());
         const R kk = static_cast<R>(P.stiffness), gm = static_cast<R>(P.gamma);
+        const int k = steps - 1;
+        const uint8_t* mask = k < static_cast<int>(pBranch.size()) ? pBranch[k].data() : nullptr;
 #pragma omp parallel for schedule(static)
-        for (int i = 0; i < nf; ++i)
-            p[i] = std::max(static_cast<R>(0), kk * rho0 / gm * (std::pow(rho[i] / rho0, gm) - 1));
+        for (int i = 0; i < nf; ++i) {
+            const R raw = kk * rho0 / gm * (std::pow(rho[i] / rho0, gm) - 1);
+            p[i] = mask ? (mask[i] ? raw : static_cast<R>(0)) : std::max(static_cast<R>(0), raw);
+        }
         const bool ak = akinci();
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < nf; ++i) {

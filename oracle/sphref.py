@@ -72,6 +72,10 @@ def lib():
                                           ctypes.c_int64]
         L.sphref_sample_box.restype = ctypes.c_int64
         L.sphref_sample_box.argtypes = L.sphref_sample_block.argtypes
+        L.sphref_set_state64.argtypes = [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_double)]
+        L.sphref_cohesion_print.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64)]
+        L.sphref_set_pressure_branch.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_uint8)]
+        L.sphref_set_cohesion_positions.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_float)]
         _lib = L
     return _lib
 
@@ -192,6 +196,37 @@ class Oracle:
         rho = np.zeros(n); p = np.zeros(n); al = np.zeros(n)
         lib().sphref_get(self.h, *(_p(q, ctypes.c_double) for q in (x, v, a, rho, p, al)))
         return dict(x=x, v=v, a=a, rho=rho, p=p, alpha=al)
+
+    def set_state64(self, x=None, v=None):
+        """Overwrite positions / velocities (N x 3) at FP64 precision."""
+        xx = None if x is None else np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        vv = None if v is None else np.ascontiguousarray(v, np.float64).reshape(-1, 3)
+        lib().sphref_set_state64(self.h, None if xx is None else _p(xx, ctypes.c_double),
+                                 None if vv is None else _p(vv, ctypes.c_double))
+
+    def set_pressure_branch(self, masks):
+        """WCSPH: per step (rows of an nsteps x N bool array) which particles
+        take p = B((rho/rho0)^g - 1) instead of max(0, .); None clears."""
+        if masks is None:
+            lib().sphref_set_pressure_branch(self.h, 0, None)
+            return
+        m = np.ascontiguousarray(masks, np.uint8).reshape(-1, self.nf)
+        lib().sphref_set_pressure_branch(self.h, len(m), _p(m, ctypes.c_uint8))
+
+    def set_cohesion_positions(self, xs):
+        """Per step (nsteps x N x 3 float32) positions whose FP32 pair lengths
+        decide the cohesion branch at h/2; None clears."""
+        if xs is None:
+            lib().sphref_set_cohesion_positions(self.h, 0, None)
+            return
+        q = np.ascontiguousarray(xs, np.float32).reshape(-1, self.nf, 3)
+        lib().sphref_set_cohesion_positions(self.h, len(q), _p(q, ctypes.c_float))
+
+    def cohesion_print(self):
+        """(hash, count) of the pairs that took the inner cohesion branch so far."""
+        out = np.zeros(2, np.uint64)
+        lib().sphref_cohesion_print(self.h, _p(out, ctypes.c_uint64))
+        return int(out[0]), int(out[1])
 
     def boundary_volumes(self):
         out = np.zeros(max(self.nb, 1))

@@ -29,6 +29,7 @@ EXPORTS = [
     "sph_synchronize", "sph_set_fluid_state", "sph_get_fluid_state", "sph_get_boundary", "sph_get_field",
     "sph_get_time", "sph_dump_cells", "sph_dump_neighbors", "sph_last_step_timing", "sph_set_profiling",
     "sph_kernel_count", "sph_kernel_name", "sph_kernel_stats", "sph_reset_kernel_stats",
+    "sph_set_coefficients", "sph_forward_record", "sph_adjoint_seed", "sph_adjoint_run", "sph_adjoint_result",
 ]
 
 
@@ -110,6 +111,11 @@ def load():
         "sph_kernel_name": (ctypes.c_char_p, [ctypes.c_int]),
         "sph_kernel_stats": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_int64)]),
         "sph_reset_kernel_stats": (ctypes.c_int, [vp]),
+        "sph_set_coefficients": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_float, ctypes.c_float]),
+        "sph_forward_record": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, P(StepStats)]),
+        "sph_adjoint_seed": (ctypes.c_int, [vp, ctypes.c_int, fp, fp]),
+        "sph_adjoint_run": (ctypes.c_int, [vp]),
+        "sph_adjoint_result": (ctypes.c_int, [vp, ctypes.c_int, fp, fp, P(ctypes.c_double), P(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -1,0 +1,54 @@
+// Reverse-mode (adjoint) passes of the WCSPH step: launchers and the view of
+// one recorded step. Gradients flow from the state after a step back to the
+// state before it and to the XSPH viscosity and surface-tension coefficients.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "sph_common.hpp"
+
+namespace sph {
+
+struct AdjView {
+    const SceneConst* scenes;
+    const SceneState* state;   // dt of each scene (fixed in WCSPH)
+    const int32_t* blockScene;
+    int32_t numScenes;
+    uint32_t numBlocks;
+    uint32_t NP;
+    // the recorded step, in that step's cell-sorted slot order
+    const float4* x;     // positions after the step's reorder
+    const float4* v;     // velocities at the start of the step
+    const float* rho;
+    const float* p;
+    const float4* n;     // surface-tension normals (w = density)
+    const uint32_t* id;  // slot -> particle index within its scene
+    const uint32_t* nbrF;
+    const uint32_t* cntF;
+    const uint32_t* nbrB;
+    const uint32_t* cntB;
+    uint32_t capF, capB;
+    const float4* xb;    // boundary: position, w = rho0 * volume
+    // adjoints of the state after the step on entry, before it on exit,
+    // indexed fBegin + particle index (original order)
+    float4* adjX;
+    float4* adjV;
+    // per-slot work arrays
+    float4* aBar;
+    float4* xBar;
+    float4* vBar;
+    float4* nBar;
+    float* rhoBar;
+    double* partials;    // 2 * numBlocks: viscosity and surface-tension terms
+    double* acc;         // 2 per scene: dL/d(viscosity), dL/d(surface tension)
+};
+
+hipError_t launchAdjSeed(const AdjView& a, hipStream_t st);
+hipError_t launchAdjForces(const AdjView& a, hipStream_t st);
+hipError_t launchAdjNormals(const AdjView& a, hipStream_t st);
+hipError_t launchAdjDensity(const AdjView& a, hipStream_t st);
+hipError_t launchAdjFinish(const AdjView& a, hipStream_t st);
+
+}  // namespace sph

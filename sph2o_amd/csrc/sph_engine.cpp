@@ -27,6 +27,7 @@
 
 #include "../../include/sph2o.h"
 #include "scene.hpp"
+#include "sph_adjoint.hpp"
 #include "sph_kernels.hpp"
 
 using namespace sph;
@@ -52,6 +53,23 @@ template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
     void alloc(size_t count) {
         release();
         if (count == 0) count = 1;
@@ -69,12 +87,13 @@ struct DevBuf {
 enum KernelId {
     K_KEYS, K_SORT, K_REORDER, K_BUILD, K_DENSITY, K_FORCES, K_SURFACE_TENSION, K_VMAX, K_PREDICT, K_RHO_STAR,
     K_VELOCITY, K_DIVERGENCE, K_SCALE, K_INTEGRATE, K_PCI_INIT, K_PCI_DENSITY, K_PCI_PACC, K_PCI_INTEGRATE,
-    K_FILL, K_FINISH, K_COUNT
+    K_FILL, K_FINISH, K_ADJ_SEED, K_ADJ_FORCES, K_ADJ_NORMALS, K_ADJ_DENSITY, K_ADJ_FINISH, K_COUNT
 };
 const char* kKernelNames[K_COUNT] = {"keys", "sort", "reorder", "build_neighbors", "density", "forces",
                                      "surface_tension", "vmax", "predict_velocity", "rho_star", "velocity",
                                      "divergence", "scale", "integrate", "pci_init", "pci_density",
-                                     "pci_pressure_acc", "pci_integrate", "fill_ranges", "finish"};
+                                     "pci_pressure_acc", "pci_integrate", "fill_ranges", "finish",
+                                     "adj_seed", "adj_forces", "adj_normals", "adj_density", "adj_finish"};
 
 // Host mirror of the reference's FP32 cubic-spline gradient (CubicSpline.h:52-74)
 // with the compiled reference's length; used for the PCISPH scaling factor.
@@ -335,6 +354,12 @@ class Engine {
         totalCells = cells;
         keyBits = 1;
         while (keyBits < 32 && (1ull << keyBits) < keysTot) ++keyBits;
+        uploadConsts();
+        uploadBoundaryGrid();
+    }
+
+    // Per-scene constants (grid, layout, coefficients) to the device.
+    void uploadConsts() {
         std::vector<SceneConst> sc(scenes.size());
         for (size_t i = 0; i < scenes.size(); ++i) {
             const SceneH& s = scenes[i];
@@ -378,8 +403,12 @@ class Engine {
             c.adaptive = (s.P.simulation_method == SPH_DFSPH && s.P.adaptive_time_step) ? 1 : 0;
             c.nf = static_cast<int32_t>(s.nf);
         }
-        dScenes.alloc(scenes.size());
+        if (dScenes.n != scenes.size()) dScenes.alloc(scenes.size());
+        HIPCHK(hipStreamSynchronize(stream));
         HIPCHK(hipMemcpy(dScenes.p, sc.data(), sc.size() * sizeof(SceneConst), hipMemcpyHostToDevice));
+    }
+
+    void uploadBoundaryGrid() {
         // boundary grid: sort boundary particles of each scene by cell key on the host
         std::vector<uint2> br(totalCells ? totalCells : 1, make_uint2(0u, 0u));
         std::vector<float4> xbs(NB ? NB : 1);
@@ -695,6 +724,7 @@ class Engine {
     void stepAll(int nsteps, sph_step_stats* stats) {
         if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
         HIPCHK(hipSetDevice(device));
+        ++stateEpoch;
         launchesInStep = 0;
         syncsInStep = 0;
         HIPCHK(hipEventRecord(evStart, stream));
@@ -903,6 +933,344 @@ class Engine {
     }
 
     // No fluid particle in any scene: only time advances.
+    // ------------------------------------------------------------ differentiable rollout (WCSPH)
+    // A recorded rollout keeps a checkpoint (slot-order x, v, id, grids, time)
+    // at the start of every segment of `every` steps, and a tape of the last
+    // recomputed segment: per step the state after its reorder, rho, p,
+    // normals, ids and neighbour lists. The reverse sweep walks the segments
+    // from the last, recomputes a segment's tape from its checkpoint (the
+    // forward step is deterministic: stable radix sort, fixed-order sums, no
+    // atomics in values), and runs the adjoint passes step by step backwards.
+    struct GridSave {
+        int g0[3], gn[3];
+        uint32_t nCells, keyRange;
+        int morton;
+        bool operator==(const GridSave& o) const {
+            return std::equal(g0, g0 + 3, o.g0) && std::equal(gn, gn + 3, o.gn) && nCells == o.nCells &&
+                   keyRange == o.keyRange && morton == o.morton;
+        }
+    };
+    struct Checkpoint {
+        DevBuf<float4> x, v;
+        DevBuf<uint32_t> id;
+        std::vector<GridSave> grids;
+        std::vector<SceneState> state;
+        std::vector<float> t, dt;
+    };
+    struct Tape {
+        int steps = 0;
+        size_t np = 0;
+        uint32_t capF = 0, capB = 0;
+        DevBuf<float4> x, v, n;
+        DevBuf<float> rho, p;
+        DevBuf<uint32_t> id, cntF, cntB, nbrF, nbrB;
+    };
+    struct Recording {
+        bool valid = false;
+        int nsteps = 0, every = 0, segments = 0, tapeSeg = -1;
+        uint64_t epoch = 0;
+        std::vector<Checkpoint> ckpt;
+        Checkpoint last;
+    };
+    Recording rec;
+    Tape tape;
+    uint64_t stateEpoch = 0;  // bumped by every call that changes the state outside a recording
+    DevBuf<float4> adjX, adjV, aBar, xBar, vBar, nBar;
+    DevBuf<float> rhoBar;
+    DevBuf<double> adjPartials, adjAcc;
+    std::vector<std::vector<float>> seedX, seedV;
+    std::vector<std::vector<float>> gradX0, gradV0;
+    std::vector<double> gradCoef;
+    bool gradValid = false;
+
+    void copyD(void* dst, const void* src, size_t bytes) {
+        if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream));
+    }
+    void saveCheckpoint(Checkpoint& c) {
+        if (c.x.n != NP) {
+            c.x.alloc(NP);
+            c.v.alloc(NP);
+            c.id.alloc(NP);
+        }
+        copyD(c.x.p, x.p, NP * sizeof(float4));
+        copyD(c.v.p, v.p, NP * sizeof(float4));
+        copyD(c.id.p, id.p, NP * sizeof(uint32_t));
+        c.grids.resize(scenes.size());
+        c.t.resize(scenes.size());
+        c.dt.resize(scenes.size());
+        for (size_t i = 0; i < scenes.size(); ++i) {
+            const SceneH& sc = scenes[i];
+            GridSave& g = c.grids[i];
+            std::copy(sc.g0, sc.g0 + 3, g.g0);
+            std::copy(sc.gn, sc.gn + 3, g.gn);
+            g.nCells = sc.nCells;
+            g.keyRange = sc.keyRange;
+            g.morton = sc.morton;
+            c.t[i] = sc.t;
+            c.dt[i] = sc.dt;
+        }
+        c.state.assign(hState, hState + scenes.size());
+    }
+    void restoreCheckpoint(const Checkpoint& c) {
+        HIPCHK(hipStreamSynchronize(stream));
+        bool layout = false;
+        for (size_t i = 0; i < scenes.size(); ++i) {
+            SceneH& sc = scenes[i];
+            GridSave cur;
+            std::copy(sc.g0, sc.g0 + 3, cur.g0);
+            std::copy(sc.gn, sc.gn + 3, cur.gn);
+            cur.nCells = sc.nCells;
+            cur.keyRange = sc.keyRange;
+            cur.morton = sc.morton;
+            const GridSave& g = c.grids[i];
+            if (!(cur == g)) {
+                std::copy(g.g0, g.g0 + 3, sc.g0);
+                std::copy(g.gn, g.gn + 3, sc.gn);
+                sc.nCells = g.nCells;
+                sc.keyRange = g.keyRange;
+                sc.morton = g.morton;
+                layout = true;
+            }
+            sc.t = c.t[i];
+            sc.dt = c.dt[i];
+        }
+        if (layout) uploadLayout();
+        copyD(x.p, c.x.p, NP * sizeof(float4));
+        copyD(v.p, c.v.p, NP * sizeof(float4));
+        copyD(id.p, c.id.p, NP * sizeof(uint32_t));
+        std::copy(c.state.begin(), c.state.end(), hState);
+        HIPCHK(hipMemcpyAsync(dState.p, hState, scenes.size() * sizeof(SceneState), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+    void tapeAlloc(int steps) {
+        const size_t k = static_cast<size_t>(steps), np = NP;
+        tape.steps = steps;
+        tape.np = np;
+        tape.capF = capF;
+        tape.capB = capB;
+        tape.x.alloc(k * np);
+        tape.v.alloc(k * np);
+        tape.n.alloc(k * np);
+        tape.rho.alloc(k * np);
+        tape.p.alloc(k * np);
+        tape.id.alloc(k * np);
+        tape.cntF.alloc(k * np);
+        tape.cntB.alloc(k * np);
+        tape.nbrF.alloc(k * capF * np);
+        tape.nbrB.alloc(k * capB * np);
+    }
+    // The list capacity grew while a segment was recorded: widen the tape's
+    // list blocks, keeping the steps already recorded (entry s of particle i
+    // sits at s * NP + i within its step's block for any capacity).
+    void tapeGrow(int recorded) {
+        const size_t k = static_cast<size_t>(tape.steps), np = tape.np;
+        auto widen = [&](DevBuf<uint32_t>& buf, uint32_t oldCap, uint32_t newCap) {
+            if (newCap <= oldCap) return;
+            DevBuf<uint32_t> w;
+            w.alloc(k * newCap * np);
+            for (int s = 0; s < recorded; ++s)
+                copyD(w.p + s * newCap * np, buf.p + s * oldCap * np, oldCap * np * sizeof(uint32_t));
+            buf = std::move(w);
+        };
+        HIPCHK(hipStreamSynchronize(stream));
+        widen(tape.nbrF, tape.capF, capF);
+        widen(tape.nbrB, tape.capB, capB);
+        HIPCHK(hipStreamSynchronize(stream));
+        tape.capF = std::max(tape.capF, capF);
+        tape.capB = std::max(tape.capB, capB);
+    }
+    // stepWCSPH with the tape of step `slot` of the current segment.
+    void stepWCSPHRecord(int slot) {
+        beginStats();
+        neighborSearch(false);
+        if (capF > tape.capF || capB > tape.capB) tapeGrow(slot);
+        DevView d = view();
+        const size_t np = NP, o = static_cast<size_t>(slot) * np;
+        copyD(tape.x.p + o, x.p, np * sizeof(float4));
+        copyD(tape.v.p + o, v.p, np * sizeof(float4));
+        copyD(tape.id.p + o, id.p, np * sizeof(uint32_t));
+        timed(K_DENSITY, [&] { return launchDensityPressure(d, stream); });
+        timed(K_FORCES, [&] { return launchForces(d, 1, stream); });
+        if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
+        copyD(tape.rho.p + o, rho.p, np * sizeof(float));
+        copyD(tape.p.p + o, p.p, np * sizeof(float));
+        if (anyST()) copyD(tape.n.p + o, n.p, np * sizeof(float4));
+        copyD(tape.cntF.p + o, cntF.p, np * sizeof(uint32_t));
+        copyD(tape.cntB.p + o, cntB.p, np * sizeof(uint32_t));
+        copyD(tape.nbrF.p + static_cast<size_t>(slot) * tape.capF * np, nbrF.p, capF * np * sizeof(uint32_t));
+        copyD(tape.nbrB.p + static_cast<size_t>(slot) * tape.capB * np, nbrB.p, capB * np * sizeof(uint32_t));
+        timed(K_INTEGRATE, [&] { return launchIntegrate(d, stream); });
+        for (size_t s = 0; s < scenes.size(); ++s) hState[s].t = hState[s].t + hState[s].dt;
+        fileStats(false);
+    }
+    void checkDifferentiable() const {
+        if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
+        if (method != SPH_WCSPH) throw SphError{SPH_E_UNSUPPORTED, "gradients are available for WCSPH scenes only"};
+        if (interleaved) throw SphError{SPH_E_UNSUPPORTED, "gradients need separate velocity storage"};
+        if (!NP) throw SphError{SPH_E_STATE, "no fluid particles"};
+        for (auto& sc : scenes)
+            if (sc.viscOn && sc.P.viscosity_method != 2)
+                throw SphError{SPH_E_UNSUPPORTED, "gradients support XSPH viscosity (method 2) only"};
+    }
+    void forwardRecord(int nsteps, int every, sph_step_stats* stats) {
+        checkDifferentiable();
+        if (nsteps < 1 || every < 1) throw SphError{SPH_E_ARG, "nsteps and checkpoint_every must be >= 1"};
+        HIPCHK(hipSetDevice(device));
+        every = std::min(every, nsteps);
+        rec.valid = false;
+        gradValid = false;
+        rec.nsteps = nsteps;
+        rec.every = every;
+        rec.segments = (nsteps + every - 1) / every;
+        rec.ckpt.resize(static_cast<size_t>(rec.segments));
+        if (tape.steps != every || tape.np != NP || tape.capF < capF || tape.capB < capB) tapeAlloc(every);
+        launchesInStep = 0;
+        syncsInStep = 0;
+        HIPCHK(hipEventRecord(evStart, stream));
+        for (int st = 0; st < nsteps; ++st) {
+            if (st % every == 0) saveCheckpoint(rec.ckpt[static_cast<size_t>(st / every)]);
+            stepWCSPHRecord(st % every);
+            if (stats)
+                for (size_t s = 0; s < scenes.size(); ++s) stats[st * scenes.size() + s] = lastStats[s];
+        }
+        saveCheckpoint(rec.last);
+        HIPCHK(hipEventRecord(evStop, stream));
+        HIPCHK(hipEventSynchronize(evStop));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, evStart, evStop));
+        lastStepMs = ms;
+        collectProfile();
+        rec.tapeSeg = rec.segments - 1;
+        rec.epoch = ++stateEpoch;
+        rec.valid = true;
+        seedX.assign(scenes.size(), {});
+        seedV.assign(scenes.size(), {});
+    }
+    void adjointSeed(int si, const float* gx, const float* gv) {
+        SceneH& sc = scene(si);
+        if (!rec.valid || rec.epoch != stateEpoch) throw SphError{SPH_E_STATE, "no recorded rollout for this state"};
+        seedX[si].assign(3 * sc.nf, 0.0f);
+        seedV[si].assign(3 * sc.nf, 0.0f);
+        if (gx) std::copy(gx, gx + 3 * sc.nf, seedX[si].begin());
+        if (gv) std::copy(gv, gv + 3 * sc.nf, seedV[si].begin());
+    }
+    AdjView adjView(int slot) {
+        AdjView a{};
+        a.scenes = dScenes.p;
+        a.state = dState.p;
+        a.blockScene = dBlockScene.p;
+        a.numScenes = static_cast<int32_t>(scenes.size());
+        a.numBlocks = numBlocks;
+        a.NP = static_cast<uint32_t>(NP);
+        const size_t o = static_cast<size_t>(slot) * NP;
+        a.x = tape.x.p + o;
+        a.v = tape.v.p + o;
+        a.rho = tape.rho.p + o;
+        a.p = tape.p.p + o;
+        a.n = tape.n.p + o;
+        a.id = tape.id.p + o;
+        a.cntF = tape.cntF.p + o;
+        a.cntB = tape.cntB.p + o;
+        a.nbrF = tape.nbrF.p + static_cast<size_t>(slot) * tape.capF * NP;
+        a.nbrB = tape.nbrB.p + static_cast<size_t>(slot) * tape.capB * NP;
+        a.capF = tape.capF;
+        a.capB = tape.capB;
+        a.xb = xb.p;
+        a.adjX = adjX.p;
+        a.adjV = adjV.p;
+        a.aBar = aBar.p;
+        a.xBar = xBar.p;
+        a.vBar = vBar.p;
+        a.nBar = nBar.p;
+        a.rhoBar = rhoBar.p;
+        a.partials = adjPartials.p;
+        a.acc = adjAcc.p;
+        return a;
+    }
+    void adjointStep(int slot) {
+        const AdjView a = adjView(slot);
+        timed(K_ADJ_SEED, [&] { return launchAdjSeed(a, stream); });
+        timed(K_ADJ_FORCES, [&] { return launchAdjForces(a, stream); });
+        if (anyST()) timed(K_ADJ_NORMALS, [&] { return launchAdjNormals(a, stream); });
+        timed(K_ADJ_DENSITY, [&] { return launchAdjDensity(a, stream); });
+        timed(K_ADJ_FINISH, [&] { return launchAdjFinish(a, stream); });
+    }
+    void adjointRun() {
+        if (!rec.valid || rec.epoch != stateEpoch) throw SphError{SPH_E_STATE, "no recorded rollout for this state"};
+        checkDifferentiable();
+        HIPCHK(hipSetDevice(device));
+        for (auto* b : {&adjX, &adjV, &aBar, &xBar, &vBar, &nBar})
+            if (b->n != NP) b->alloc(NP);
+        if (rhoBar.n != NP) rhoBar.alloc(NP);
+        adjPartials.alloc(2 * static_cast<size_t>(numBlocks ? numBlocks : 1));
+        adjAcc.alloc(2 * scenes.size());
+        std::vector<float4> hx(NP, make_float4(0, 0, 0, 0)), hv(NP, make_float4(0, 0, 0, 0));
+        for (size_t si = 0; si < scenes.size(); ++si) {
+            const SceneH& sc = scenes[si];
+            for (uint32_t o = 0; o < sc.nf && !seedX[si].empty(); ++o) {
+                hx[sc.fBegin + o] = make_float4(seedX[si][3 * o], seedX[si][3 * o + 1], seedX[si][3 * o + 2], 0.0f);
+                hv[sc.fBegin + o] = make_float4(seedV[si][3 * o], seedV[si][3 * o + 1], seedV[si][3 * o + 2], 0.0f);
+            }
+        }
+        HIPCHK(hipMemcpy(adjX.p, hx.data(), NP * sizeof(float4), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(adjV.p, hv.data(), NP * sizeof(float4), hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(adjAcc.p, 0, 2 * scenes.size() * sizeof(double)));
+        HIPCHK(hipEventRecord(evStart, stream));
+        launchesInStep = 0;
+        for (int seg = rec.segments - 1; seg >= 0; --seg) {
+            const int first = seg * rec.every, count = std::min(rec.every, rec.nsteps - first);
+            if (rec.tapeSeg != seg) {
+                restoreCheckpoint(rec.ckpt[static_cast<size_t>(seg)]);
+                for (int j = 0; j < count; ++j) stepWCSPHRecord(j);
+                rec.tapeSeg = seg;
+            }
+            for (int j = count - 1; j >= 0; --j) adjointStep(j);
+        }
+        HIPCHK(hipEventRecord(evStop, stream));
+        HIPCHK(hipEventSynchronize(evStop));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, evStart, evStop));
+        lastStepMs = ms;
+        collectProfile();
+        hx = download(adjX.p, NP);
+        hv = download(adjV.p, NP);
+        gradCoef = download(adjAcc.p, 2 * scenes.size());
+        gradX0.assign(scenes.size(), {});
+        gradV0.assign(scenes.size(), {});
+        for (size_t si = 0; si < scenes.size(); ++si) {
+            const SceneH& sc = scenes[si];
+            gradX0[si].resize(3 * sc.nf);
+            gradV0[si].resize(3 * sc.nf);
+            for (uint32_t o = 0; o < sc.nf; ++o) {
+                const float4 q = hx[sc.fBegin + o], w = hv[sc.fBegin + o];
+                gradX0[si][3 * o] = q.x; gradX0[si][3 * o + 1] = q.y; gradX0[si][3 * o + 2] = q.z;
+                gradV0[si][3 * o] = w.x; gradV0[si][3 * o + 1] = w.y; gradV0[si][3 * o + 2] = w.z;
+            }
+        }
+        // leave the context where the recorded rollout ended
+        restoreCheckpoint(rec.last);
+        gradValid = true;
+    }
+    void adjointResult(int si, float* gx0, float* gv0, double* gnu, double* ggam) {
+        scene(si);
+        if (!gradValid) throw SphError{SPH_E_STATE, "sph_adjoint_run has not completed"};
+        if (gx0) std::copy(gradX0[si].begin(), gradX0[si].end(), gx0);
+        if (gv0) std::copy(gradV0[si].begin(), gradV0[si].end(), gv0);
+        if (gnu) *gnu = gradCoef[2 * si];
+        if (ggam) *ggam = gradCoef[2 * si + 1];
+    }
+    void setCoefficients(int si, float viscosity, float surfaceTension) {
+        SceneH& sc = scene(si);
+        if (!std::isfinite(viscosity) || !std::isfinite(surfaceTension))
+            throw SphError{SPH_E_ARG, "non-finite coefficient"};
+        sc.P.viscosity = viscosity;
+        sc.P.surface_tension = surfaceTension;
+        sc.viscOn = sc.P.viscosity_method > -1 && sc.P.viscosity != 0.0f;
+        sc.stOn = sc.P.surface_tension_method == SPH_SURFACE_TENSION_AKINCI && sc.P.surface_tension != 0.0f;
+        ++stateEpoch;
+        if (initialized) uploadConsts();
+    }
+
     void advanceEmpty() {
         for (size_t s = 0; s < scenes.size(); ++s) hState[s].t = hState[s].t + hState[s].dt;
         HIPCHK(hipMemcpyAsync(dState.p, hState, scenes.size() * sizeof(SceneState), hipMemcpyHostToDevice, stream));
@@ -981,6 +1349,7 @@ class Engine {
 
     void setState(int si, int64_t nn, const float* pos, const float* vel) {
         SceneH& s = scene(si);
+        ++stateEpoch;
         if (nn != static_cast<int64_t>(s.nf)) throw SphError{SPH_E_ARG, "particle count mismatch"};
         if (pos)
             for (int64_t i = 0; i < 3 * nn; ++i)
@@ -990,16 +1359,34 @@ class Engine {
             if (vel) std::copy(vel, vel + 3 * nn, s.vf.begin());
             return;
         }
+        // The scene restarts from the original particle order (slot fBegin + o
+        // holds particle o), carrying the DFSPH warm-start values along: the
+        // next step then sorts the same input whatever order earlier steps
+        // left (the radix sort is stable within a cell), so a state set twice
+        // steps bit-identically.
         auto ids = download(id.p + s.fBegin, s.nf);
         auto hx = download(x.p + s.fBegin, s.nf);
         auto hv = downloadVel(s.fBegin, s.nf);
+        auto kw = download(kws.p + s.fBegin, s.nf);
+        auto kv = download(kvws.p + s.fBegin, s.nf);
+        std::vector<float4> ox(s.nf), ov(s.nf);
+        std::vector<float> okw(s.nf), okv(s.nf);
+        std::vector<uint32_t> oid(s.nf);
         for (uint32_t i = 0; i < s.nf; ++i) {
             const uint32_t o = ids[i];
-            if (pos) hx[i] = make_float4(pos[3 * o], pos[3 * o + 1], pos[3 * o + 2], 0.0f);
-            if (vel) hv[i] = make_float4(vel[3 * o], vel[3 * o + 1], vel[3 * o + 2], 0.0f);
+            if (o >= s.nf) throw SphError{SPH_E_STATE, "corrupt particle ids"};
+            ox[o] = pos ? make_float4(pos[3 * o], pos[3 * o + 1], pos[3 * o + 2], 0.0f) : hx[i];
+            ov[o] = vel ? make_float4(vel[3 * o], vel[3 * o + 1], vel[3 * o + 2], 0.0f) : hv[i];
+            okw[o] = kw[i];
+            okv[o] = kv[i];
+            oid[o] = o;
         }
+        hx.swap(ox);
         HIPCHK(hipMemcpy(x.p + s.fBegin, hx.data(), s.nf * sizeof(float4), hipMemcpyHostToDevice));
-        uploadVel(s.fBegin, hv);
+        uploadVel(s.fBegin, ov);
+        HIPCHK(hipMemcpy(id.p + s.fBegin, oid.data(), s.nf * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(kws.p + s.fBegin, okw.data(), s.nf * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(kvws.p + s.fBegin, okv.data(), s.nf * sizeof(float), hipMemcpyHostToDevice));
         if (interleaved)  // positions are duplicated in the interleaved records
             HIPCHK(hipMemcpy2D(xv.p + 2 * static_cast<size_t>(s.fBegin), 2 * sizeof(float4), hx.data(), sizeof(float4),
                                sizeof(float4), s.nf, hipMemcpyHostToDevice));
@@ -1287,6 +1674,42 @@ int sph_set_profiling(sph_ctx* ctx, int on) {
             std::fill(std::begin(ctx->eng.kernelMs), std::end(ctx->eng.kernelMs), 0.0);
             std::fill(std::begin(ctx->eng.kernelLaunches), std::end(ctx->eng.kernelLaunches), 0);
         }
+        return SPH_OK;
+    });
+}
+
+int sph_set_coefficients(sph_ctx* ctx, int scene, float viscosity, float surface_tension) {
+    return guarded(ctx, [&] {
+        ctx->eng.setCoefficients(scene, viscosity, surface_tension);
+        return SPH_OK;
+    });
+}
+
+int sph_forward_record(sph_ctx* ctx, int nsteps, int checkpoint_every, sph_step_stats* stats) {
+    return guarded(ctx, [&] {
+        ctx->eng.forwardRecord(nsteps, checkpoint_every, stats);
+        return SPH_OK;
+    });
+}
+
+int sph_adjoint_seed(sph_ctx* ctx, int scene, const float* dL_dx, const float* dL_dv) {
+    return guarded(ctx, [&] {
+        ctx->eng.adjointSeed(scene, dL_dx, dL_dv);
+        return SPH_OK;
+    });
+}
+
+int sph_adjoint_run(sph_ctx* ctx) {
+    return guarded(ctx, [&] {
+        ctx->eng.adjointRun();
+        return SPH_OK;
+    });
+}
+
+int sph_adjoint_result(sph_ctx* ctx, int scene, float* dL_dx0, float* dL_dv0, double* dL_dviscosity,
+                       double* dL_dsurface_tension) {
+    return guarded(ctx, [&] {
+        ctx->eng.adjointResult(scene, dL_dx0, dL_dv0, dL_dviscosity, dL_dsurface_tension);
         return SPH_OK;
     });
 }

@@ -19,17 +19,12 @@
 // block partials, combined per scene by a one-block-per-scene kFinish launch).
 #include <hip/hip_runtime.h>
 
+#include "sph_device.hpp"
 #include "sph_kernels.hpp"
 
 namespace sph {
 
 // ---------------------------------------------------------------- helpers
-// |r|^2 as the reference's compiled glm::length forms it: fma(x,x, fma(z,z, y*y)).
-__device__ __forceinline__ float dRef(float x, float y, float z) {
-    return __fmaf_rn(x, x, __fmaf_rn(z, z, __fmul_rn(y, y)));
-}
-__device__ __forceinline__ float lenRef(float x, float y, float z) { return __fsqrt_rn(dRef(x, y, z)); }
-__device__ __forceinline__ float fastSqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
 // Build-time switches (measured variants): unroll factor of the neighbour
 // loops, non-temporal neighbour-list loads, and whether the velocity pass
 // gathers the packed (x, k) array or k first and x only for open gates.
@@ -42,28 +37,12 @@ __device__ __forceinline__ float fastSqrt(float v) { return __builtin_amdgcn_sqr
 #ifndef SPH_XK_VELOCITY
 #define SPH_XK_VELOCITY 0
 #endif
-#ifndef SPH_XCD_REMAP
-#define SPH_XCD_REMAP 1
-#endif
 #ifndef SPH_XV
 #define SPH_XV 0
 #endif
 #define SPH_PRAGMA(x) _Pragma(#x)
 #define SPH_UNROLL_LOOP_(n) SPH_PRAGMA(unroll n)
 #define SPH_UNROLL_LOOP SPH_UNROLL_LOOP_(SPH_UNROLL)
-// Logical block of this workgroup. Workgroups are dispatched round-robin over
-// the 8 XCDs, each with its own L2; with the remap every XCD walks one
-// contiguous eighth of the (Morton-sorted) particle range, so the neighbours
-// it gathers stay in its own L2.
-__device__ __forceinline__ uint32_t logicalBlock() {
-#if SPH_XCD_REMAP
-    const uint32_t n = gridDim.x, b = blockIdx.x;
-    const uint32_t q = n >> 3, r = n & 7u, x = b & 7u, k = b >> 3;
-    return x * q + min(x, r) + k;
-#else
-    return blockIdx.x;
-#endif
-}
 // Velocities: a separate array, or interleaved with positions (xv[2i] = x,
 // xv[2i+1] = v) so passes that gather both read one cache line per neighbour.
 #if SPH_XV
@@ -82,47 +61,6 @@ __device__ __forceinline__ uint32_t ldList(const uint32_t* p) {
     return *p;
 #endif
 }
-__device__ __forceinline__ float fastRcp(float v) { return __builtin_amdgcn_rcpf(v); }
-
-// CubicSpline::W (CubicSpline.h:27-50)
-__device__ __forceinline__ float cubicW(const SceneConst& S, float rl) {
-    const float q = rl * S.invh;
-    if (q > 1.0f) return 0.0f;
-    if (q <= 0.5f) {
-        const float q2 = q * q;
-        return S.kW * (6.0f * (q2 * q) - 6.0f * q2 + 1.0f);
-    }
-    const float f = 1.0f - q;
-    return S.kW * (2.0f * (f * f * f));
-}
-// CubicSpline::gradW (CubicSpline.h:52-74) as gradW = r * factor
-__device__ __forceinline__ float gradFactor(const SceneConst& S, float rl) {
-    const float q = rl * S.invh;
-    if (!(rl > 1.0e-9f) || q > 1.0f) return 0.0f;
-    const float inv = fastRcp(rl * S.h);
-    if (q <= 0.5f) return S.lW * q * (3.0f * q - 2.0f) * inv;
-    const float f = 1.0f - q;
-    return S.lW * (-f * f) * inv;
-}
-// ViscoK::laplW (ViscoK.h:29-37)
-__device__ __forceinline__ float laplW(const SceneConst& S, float rl) {
-    return rl <= S.h ? S.laplC * (S.h - rl) : 0.0f;
-}
-// Cohesion::W (Cohesion.h:20-40). The branch at r = h/2, where the
-// reference's form jumps by 2x, is decided exactly on d = |r|^2 in the
-// reference's rounding (d > dhalf <=> sqrt_rn(d) > h/2); rl is the length.
-__device__ __forceinline__ float cohesionW(const SceneConst& S, float d, float rl) {
-    if (d > S.dhalf && d <= S.dmax) {
-        const float a = S.h - rl;
-        return S.cohC1 * (a * a * a) * (rl * rl * rl);
-    }
-    if (d > 0.0f && d <= S.dhalf) {
-        const float a = S.h - rl;
-        return S.cohC1 * 2.0f * (a * a * a) * (rl * rl * rl) - S.cohC2;
-    }
-    return 0.0f;
-}
-
 __device__ __forceinline__ int floorCell(float x, float invh) { return static_cast<int>(floorf(__fmul_rn(x, invh))); }
 
 __device__ __forceinline__ uint32_t spread3(uint32_t v) {  // 10 bits -> every third bit
@@ -171,23 +109,6 @@ __device__ __forceinline__ uint32_t cellKey(const SceneConst& S, int cx, int cy,
     const bool valid = T.active[slot] != 0 && i < S.fEnd;
 
 // ---- FP64 deterministic reductions: block partials + per-scene kFinish ----
-template <bool MAX>
-__device__ __forceinline__ double combine(double a, double b) {
-    return MAX ? fmax(a, b) : a + b;
-}
-template <bool MAX>
-__device__ double blockReduce(double v) {
-    __shared__ double sm[kBlock / 64];
-    for (int off = 32; off > 0; off >>= 1) v = combine<MAX>(v, __shfl_down(v, off, 64));
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0) sm[wid] = v;
-    __syncthreads();
-    double t = sm[0];
-    for (int w = 1; w < kBlock / 64; ++w) t = combine<MAX>(t, sm[w]);
-    return t;  // valid in every thread
-}
-
 // Block partial of a reduction, stored at the block's logical index with a
 // plain store. kFinish, launched next on the same stream, combines each
 // scene's partials in a fixed order; stream order makes them visible, so no

@@ -149,14 +149,25 @@ def b64(seed=6, count=64, lo=2e3, hi=2e5):
     return out
 
 
-def a100k():
+def a100k(method=0, n=(50, 50, 40), dt=5e-4):
+    """Differentiable-rollout workload: an n[0] x n[1] x n[2] block (default
+    100,000) at r = 0.005, one spacing from the walls of a 1 x 1 x 0.6 m box,
+    fixed dt, XSPH viscosity 0.01 and Akinci surface tension 0.1. method 0:
+    WCSPH (k = 1000, gamma = 7); method 2: DFSPH with 8 density and 8
+    divergence iterations and the CFL rule off."""
     r = 0.005
     s = 2 * r
-    sc = scene_dict(2, r, [((s, s, s), (s + 0.5, s + 0.5, s + 0.4))], [((0, 0, 0), (1.0, 1.0, 0.6))], dt=5e-4,
-                    max_dt=5e-4,
+    hi = tuple(s + s * c for c in n)
+    box = (max(1.0, hi[0] + s), max(1.0, hi[1] + s), max(0.6, hi[2] + s))
+    sc = scene_dict(method, r, [((s, s, s), hi)], [((0, 0, 0), box)], dt=dt, max_dt=dt, min_dt=dt,
                     min_it=8, max_it=8, min_it_v=8, max_it_v=8, viscosity_method=2, viscosity=0.01,
                     surface_tension_method=3, surface_tension=0.1)
     return sc, None, None
+
+
+def adjoint_target(x0, seed=7):
+    """Loss target y = x0 + N(0, (0.5 s)^2) per coordinate, s = 0.01."""
+    return (np.asarray(x0, np.float64) + 0.005 * normal(seed, x0.size).reshape(x0.shape))
 
 
 def small(method, n=10, radius=0.01, box=(0.6, 0.3, 0.4), gap=0.02, **kw):

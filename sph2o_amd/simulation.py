@@ -150,6 +150,38 @@ class Context:
             self._check(rc)
             return off, ids[: off[-1]], boff, bids[: boff[-1]]
 
+    # -- gradients of a WCSPH rollout
+    def set_coefficients(self, scene, viscosity, surface_tension):
+        self._check(self.L.sph_set_coefficients(self.h, scene, float(viscosity), float(surface_tension)))
+
+    def forward_record(self, nsteps, checkpoint_every):
+        """Advance nsteps like step() and record the rollout for adjoint()."""
+        ns = self.num_scenes()
+        st = (StepStats * (max(1, nsteps) * ns))()
+        self._check(self.L.sph_forward_record(self.h, int(nsteps), int(checkpoint_every), st))
+        return [[st[k * ns + s] for s in range(ns)] for k in range(nsteps)]
+
+    def adjoint(self, seeds):
+        """seeds: {scene: (dL/dx_T, dL/dv_T)} (N x 3 arrays or None). Returns
+        {scene: dict(x0=dL/dx_0, v0=dL/dv_0, viscosity=..., surface_tension=...)}."""
+        for sc, (gx, gv) in seeds.items():
+            n = self.num_fluid(sc)
+            gx = None if gx is None else np.ascontiguousarray(gx, np.float32).reshape(n, 3)
+            gv = None if gv is None else np.ascontiguousarray(gv, np.float32).reshape(n, 3)
+            self._check(self.L.sph_adjoint_seed(self.h, sc, _fp(gx), _fp(gv)))
+        self._check(self.L.sph_adjoint_run(self.h))
+        out = {}
+        for sc in seeds:
+            n = self.num_fluid(sc)
+            gx0 = np.zeros((n, 3), np.float32)
+            gv0 = np.zeros((n, 3), np.float32)
+            gnu = ctypes.c_double()
+            gst = ctypes.c_double()
+            self._check(self.L.sph_adjoint_result(self.h, sc, _fp(gx0), _fp(gv0), ctypes.byref(gnu),
+                                                  ctypes.byref(gst)))
+            out[sc] = dict(x0=gx0, v0=gv0, viscosity=gnu.value, surface_tension=gst.value)
+        return out
+
     # -- timing
     def last_step_ms(self):
         ms = ctypes.c_double()
@@ -158,7 +190,9 @@ class Context:
         return ms.value, n.value
 
     def set_profiling(self, on):
-        self._check(self.L.sph_set_profiling(self.h, 1 if on else 0))
+        """True/False for every kernel, or an int bit mask of kernel ids."""
+        mask = int(on) if isinstance(on, int) and not isinstance(on, bool) else (1 if on else 0)
+        self._check(self.L.sph_set_profiling(self.h, mask))
 
     def kernel_stats(self):
         out = {}

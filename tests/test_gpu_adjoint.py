@@ -1,0 +1,299 @@
+"""Gradients of WCSPH rollouts (sph_forward_record / sph_adjoint_*) on the GPU,
+checked against FP64 central finite differences of the oracle.
+
+The loss is L = mean_i |x_i(T) - y_i|^2 with y = x(0) + N(0, (0.5 s)^2); the
+checked gradients are dL/dv0 and dL/dx0 (random unit directions), and
+dL/d(viscosity), dL/d(surface tension). Central differences in FP64 on the
+oracle with eps = 1e-6 (v0, coefficients) and 1e-8 (x0).
+
+The GPU differentiates its FP32 trajectory, which stays within ~1e-8 m of
+the FP64 one but resolves the reference's two non-smooth points on its own:
+  * the pressure max(0, p) (WCSPHSolver.cpp:93) has a kink at rho = rho0;
+    FP32 density sums carry ~3e-6 relative error, so a particle that close
+    to rho0 can sit on the other side than in FP64, where the one-sided
+    derivatives differ (one such particle-step moved a 20-step gradient by
+    3e-3);
+  * the cohesion kernel jumps 2x at |r| = h/2 (Cohesion.h:37), which is the
+    lattice spacing, so pairs sit there all the time; a pair whose FP32 and
+    FP64 lengths fall on different sides changes the surface-tension
+    derivative by a fixed amount from that step on (1e-2 of dL/dgamma).
+The oracle therefore evaluates its FP64 rollouts on the GPU's branches:
+every GPU step's p > 0 pattern and FP32 positions are handed to it
+(Oracle.set_pressure_branch / set_cohesion_positions; the branch of a pair is
+decided by the reference's FP32 length of the GPU positions). Every
+difference is still taken between runs whose inner-branch cohesion
+fingerprints agree, which the checker asserts.
+
+Tolerance: relative error <= 1e-3 per direction and coefficient.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import sphref as S
+from sph2o_amd import _lib, scenes as SC
+from sph2o_amd.autograd import rollout
+from sph2o_amd.simulation import Context
+
+pytestmark = pytest.mark.gpu
+
+R = 0.005
+SP = 2 * R
+
+
+def _scene(visc=True, st=True, n=(10, 10, 8), dt=5e-4):
+    hi = tuple(SP + SP * c for c in n)
+    return SC.scene_dict(0, R, [((SP, SP, SP), hi)], [((0, 0, 0), (0.2, 0.2, 0.15))], dt=dt,
+                         viscosity_method=2 if visc else -1, viscosity=0.01 if visc else 0.0,
+                         surface_tension_method=3 if st else -1, surface_tension=0.1 if st else 0.0)
+
+
+class Case:
+    """One scene on the GPU plus the FP64 oracle of the same scene."""
+
+    def __init__(self, tmp, sc, xseed=1, vseed=3):
+        path = os.path.join(tmp, "scene.json")
+        SC.write_json(sc, path)
+        self.sc = sc
+        self.ctx = Context(0)
+        self.sid = self.ctx.load_scene_json(path)
+        self.ctx.init()
+        self.xb, _ = self.ctx.boundary(self.sid, volumes=False)
+        self.x0 = SC.jitter(self.ctx.fluid_state(self.sid)["x"], 0.01 * SP, xseed)
+        self.v0 = (0.05 * SC.normal(vseed, self.x0.size).reshape(self.x0.shape)).astype(np.float32)
+        self.y = SC.adjoint_target(self.x0)
+        self.P = S.params_from_scene(sc)
+        self.nu = float(self.P.viscosity)
+        self.gam = float(self.P.surface_tension)
+        self.branch = None
+
+    def gpu_branch(self, nsteps):
+        """Per step of the GPU rollout: the p > 0 pattern and the FP32
+        positions at the start of the step (original order)."""
+        c, sid = self.ctx, self.sid
+        c.set_fluid_state(sid, self.x0, self.v0)
+        masks, xs = [], []
+        for _ in range(nsteps):
+            xs.append(c.fluid_state(sid)["x"])
+            c.step(1)
+            masks.append(c.fluid_state(sid)["p"] > 0)
+        self.branch = np.array(masks)
+        self.branch_x = np.array(xs, np.float32)
+        return self.branch
+
+    def gpu_grad(self, nsteps, every=20):
+        c, sid = self.ctx, self.sid
+        c.set_fluid_state(sid, self.x0, self.v0)
+        c.forward_record(nsteps, every)
+        xT = c.fluid_state(sid)["x"].astype(np.float64)
+        seed = (2.0 * (xT - self.y) / len(xT)).astype(np.float32)
+        return xT, c.adjoint({sid: (seed, None)})[sid]
+
+    def oracle(self, x, v, nu, gam, nsteps):
+        q = S.Params.from_buffer_copy(self.P)
+        q.viscosity = nu
+        q.surface_tension = gam
+        o = S.Oracle(q, x.astype(np.float32), v.astype(np.float32), self.xb, bits=64)
+        try:
+            o.init()
+            o.set_state64(x, v)
+            if self.branch is not None:
+                o.set_pressure_branch(self.branch[:nsteps])
+                o.set_cohesion_positions(self.branch_x[:nsteps])
+            o.step(nsteps)
+            xt = o.state()["x"]
+            return float(np.mean(np.sum((xt - self.y) ** 2, axis=1))), o.cohesion_print()
+        finally:
+            o.close()
+
+    def fd_checks(self, res, nsteps, seed=8, ndir=2):
+        """[(kind, gpu, fd)] over random directions and the coefficients."""
+        X0, V0 = self.x0.astype(np.float64), self.v0.astype(np.float64)
+        _, base = self.oracle(X0, V0, self.nu, self.gam, nsteps)
+        rng = np.random.default_rng(seed)
+        out = []
+        kinds = ["v0"] * ndir + ["x0"] * ndir
+        if self.nu:
+            kinds.append("viscosity")
+        if self.gam:
+            kinds.append("surface_tension")
+        for kind in kinds:
+            eps = 1e-8 if kind == "x0" else 1e-6
+            if kind in ("v0", "x0"):
+                d = rng.normal(size=self.x0.shape)
+                d /= np.linalg.norm(d)
+                g = float(np.sum(res[kind].astype(np.float64) * d))
+                dx = eps * d if kind == "x0" else 0.0
+                dv = eps * d if kind == "v0" else 0.0
+                lp, pp = self.oracle(X0 + dx, V0 + dv, self.nu, self.gam, nsteps)
+                lm, pm = self.oracle(X0 - dx, V0 - dv, self.nu, self.gam, nsteps)
+            else:
+                g = res[kind]
+                dn = eps if kind == "viscosity" else 0.0
+                dg = eps if kind == "surface_tension" else 0.0
+                lp, pp = self.oracle(X0, V0, self.nu + dn, self.gam + dg, nsteps)
+                lm, pm = self.oracle(X0, V0, self.nu - dn, self.gam - dg, nsteps)
+            assert pp == base and pm == base, "finite difference crossed the cohesion jump at h/2 (%s)" % kind
+            out.append((kind, g, (lp - lm) / (2 * eps)))
+        return out
+
+
+def _rel(g, fd):
+    return abs(g - fd) / max(abs(fd), 1e-30)
+
+
+@pytest.mark.parametrize("visc,st", [(True, True), (True, False), (False, True), (False, False)])
+@pytest.mark.parametrize("nsteps", [1, 5])
+def test_adjoint_matches_fp64_finite_differences(tmp_path, visc, st, nsteps):
+    case = Case(str(tmp_path), _scene(visc, st))
+    try:
+        case.gpu_branch(nsteps)
+        xT, res = case.gpu_grad(nsteps)
+        lg = float(np.mean(np.sum((xT - case.y) ** 2, axis=1)))
+        lo, _ = case.oracle(case.x0.astype(np.float64), case.v0.astype(np.float64), case.nu, case.gam, nsteps)
+        assert abs(lg - lo) <= 1e-4 * lo
+        for kind, g, fd in case.fd_checks(res, nsteps):
+            assert _rel(g, fd) <= 1e-3, (kind, g, fd)
+    finally:
+        case.ctx.close()
+
+
+def test_adjoint_long_rollout(tmp_path):
+    """20 steps, XSPH + surface tension, on the GPU's branches. Also counts
+    the particle-steps where the FP64 oracle on its own would take the other
+    side of the pressure kink (reported, not asserted)."""
+    nsteps = 20
+    case = Case(str(tmp_path), _scene(True, True), xseed=2, vseed=102)
+    try:
+        branch = case.gpu_branch(nsteps)
+        o = S.Oracle(case.P, case.x0, case.v0, case.xb, bits=64)
+        o.init()
+        flips = 0
+        for k in range(nsteps):
+            o.step(1)
+            flips += int(np.sum(branch[k] != (o.state()["p"] > 0)))
+        o.close()
+        _, res = case.gpu_grad(nsteps)
+        for kind, g, fd in case.fd_checks(res, nsteps, ndir=3):
+            assert _rel(g, fd) <= 1e-3, (kind, g, fd, "fp32/fp64 kink flips: %d" % flips)
+    finally:
+        case.ctx.close()
+
+
+def test_checkpoint_interval_does_not_change_the_gradient(tmp_path):
+    """Segment recomputation from checkpoints is bit-exact: the gradients and
+    the final state are identical for every checkpoint interval, and the
+    recorded forward equals plain stepping."""
+    case = Case(str(tmp_path), _scene(True, True))
+    try:
+        c, sid = case.ctx, case.sid
+        c.set_fluid_state(sid, case.x0, case.v0)
+        c.step(12)
+        plain = c.fluid_state(sid)
+        ref = None
+        for every in (1, 5, 12):
+            xT, res = case.gpu_grad(12, every)
+            st = c.fluid_state(sid)
+            assert np.array_equal(st["x"], plain["x"]) and np.array_equal(st["v"], plain["v"])
+            if ref is None:
+                ref = res
+                continue
+            for k in ("x0", "v0"):
+                assert np.array_equal(res[k], ref[k]), (every, k)
+            assert res["viscosity"] == ref["viscosity"] and res["surface_tension"] == ref["surface_tension"]
+    finally:
+        case.ctx.close()
+
+
+def test_batched_adjoint_equals_single_scene(tmp_path):
+    """Two scenes in one context get the gradients each gets alone, bit for bit."""
+    tmp = str(tmp_path)
+    a = _scene(True, True)
+    b = _scene(True, True, n=(8, 9, 7))
+    b["Fluid"]["FluidModel"][0]["viscosity"] = 0.03
+    b["Fluid"]["FluidModel"][0]["surfaceTension"] = 0.05
+    singles = []
+    for k, sc in enumerate((a, b)):
+        os.makedirs(os.path.join(tmp, str(k)))
+        case = Case(os.path.join(tmp, str(k)), sc, xseed=1 + k, vseed=3 + k)
+        _, res = case.gpu_grad(6, 4)
+        singles.append((case.x0, case.v0, case.y, res))
+        case.ctx.close()
+    ctx = Context(0)
+    try:
+        for k, sc in enumerate((a, b)):
+            path = os.path.join(tmp, "b%d.json" % k)
+            SC.write_json(sc, path)
+            ctx.load_scene_json(path)
+        ctx.init()
+        for k in range(2):
+            ctx.set_fluid_state(k, singles[k][0], singles[k][1])
+        ctx.forward_record(6, 4)
+        seeds = {}
+        for k in range(2):
+            xT = ctx.fluid_state(k)["x"].astype(np.float64)
+            seeds[k] = ((2.0 * (xT - singles[k][2]) / len(xT)).astype(np.float32), None)
+        res = ctx.adjoint(seeds)
+        for k in range(2):
+            ref = singles[k][3]
+            for f in ("x0", "v0"):
+                assert np.array_equal(res[k][f], ref[f]), (k, f)
+            assert res[k]["viscosity"] == ref["viscosity"]
+            assert res[k]["surface_tension"] == ref["surface_tension"]
+    finally:
+        ctx.close()
+
+
+def test_torch_autograd_function(tmp_path):
+    """loss.backward() through sph2o_amd.autograd.rollout returns the C-ABI gradients."""
+    case = Case(str(tmp_path), _scene(True, True))
+    try:
+        _, ref = case.gpu_grad(5)
+        x0 = torch.tensor(case.x0, requires_grad=True)
+        v0 = torch.tensor(case.v0, requires_grad=True)
+        nu = torch.tensor(case.nu, dtype=torch.float64, requires_grad=True)
+        gam = torch.tensor(case.gam, dtype=torch.float64, requires_grad=True)
+        xT, vT = rollout(case.ctx, case.sid, x0, v0, nu, gam, 5)
+        y = torch.tensor(case.y)
+        loss = ((xT.double() - y) ** 2).sum(dim=1).mean()
+        loss.backward()
+        assert np.array_equal(v0.grad.numpy(), ref["v0"])
+        assert np.array_equal(x0.grad.numpy(), ref["x0"])
+        assert nu.grad.item() == pytest.approx(ref["viscosity"], rel=1e-6)
+        assert gam.grad.item() == pytest.approx(ref["surface_tension"], rel=1e-6)
+    finally:
+        case.ctx.close()
+
+
+def test_adjoint_errors(tmp_path):
+    tmp = str(tmp_path)
+    case = Case(tmp, _scene(True, True))
+    try:
+        c, sid = case.ctx, case.sid
+        with pytest.raises(_lib.SPHError) as e:     # nothing recorded yet
+            c.adjoint({sid: (np.zeros_like(case.x0), None)})
+        assert e.value.code == _lib.SPH_E_STATE
+        c.forward_record(3, 2)
+        c.set_fluid_state(sid, case.x0, case.v0)    # state changed after the recording
+        with pytest.raises(_lib.SPHError) as e:
+            c.adjoint({sid: (np.zeros_like(case.x0), None)})
+        assert e.value.code == _lib.SPH_E_STATE
+        with pytest.raises(_lib.SPHError) as e:
+            c.forward_record(0, 1)
+        assert e.value.code == _lib.SPH_E_ARG
+    finally:
+        case.ctx.close()
+    dfsph = SC.small(2)
+    path = os.path.join(tmp, "dfsph.json")
+    SC.write_json(dfsph, path)
+    ctx = Context(0)
+    try:
+        ctx.load_scene_json(path)
+        ctx.init()
+        with pytest.raises(_lib.SPHError) as e:
+            ctx.forward_record(2, 1)
+        assert e.value.code == _lib.SPH_E_UNSUPPORTED
+    finally:
+        ctx.close()

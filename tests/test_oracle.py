@@ -239,3 +239,42 @@ def test_oracle_dfsph_aggregates_over_40_steps():
     ke_me = (s["v"] ** 2).sum()
     assert abs(ke_me - ke_ref) / ke_ref < 0.05
     assert np.abs(s["x"].mean(0) - g["x"].mean(0)).max() < 1e-4
+
+
+def test_fp64_finite_differences_converge_and_flag_cohesion_jumps():
+    """The checker of the GPU gradients: FP64 central differences of a 20-step
+    WCSPH rollout (XSPH + surface tension) are independent of eps over
+    1e-5..1e-7, and the inner-branch cohesion fingerprint changes when a
+    perturbation moves a pair across |r| = h/2 (here eps = 1e-4 does)."""
+    r, s = 0.005, 0.01
+    hi = (s + 10 * s, s + 10 * s, s + 8 * s)
+    sc = SC.scene_dict(0, r, [((s, s, s), hi)], [((0, 0, 0), (0.2, 0.2, 0.15))], dt=5e-4, viscosity_method=2,
+                       viscosity=0.01, surface_tension_method=3, surface_tension=0.1)
+    x0 = SC.jitter(S.sample_block((s, s, s), hi, r), 0.01 * s, 1).astype(np.float64)
+    xb = S.sample_box((0, 0, 0), (0.2, 0.2, 0.15), r)
+    v0 = 0.05 * SC.normal(3, x0.size).reshape(x0.shape)
+    y = SC.adjoint_target(x0)
+    P = S.params_from_scene(sc)
+
+    def loss(v):
+        o = S.Oracle(P, x0.astype(np.float32), v.astype(np.float32), xb, bits=64)
+        o.init()
+        o.set_state64(x0, v)
+        o.step(20)
+        L = float(np.mean(np.sum((o.state()["x"] - y) ** 2, axis=1)))
+        pr = o.cohesion_print()
+        o.close()
+        return L, pr
+
+    _, base = loss(v0)
+    assert base[1] > 0
+    d = np.random.default_rng(8).normal(size=x0.shape)
+    d /= np.linalg.norm(d)
+    fds = []
+    for eps in (1e-5, 1e-6, 1e-7):
+        (lp, pp), (lm, pm) = loss(v0 + eps * d), loss(v0 - eps * d)
+        assert pp == base and pm == base
+        fds.append((lp - lm) / (2 * eps))
+    assert max(abs(f - fds[1]) for f in fds) <= 1e-5 * abs(fds[1])
+    (_, pp), (_, pm) = loss(v0 + 1e-4 * d), loss(v0 - 1e-4 * d)
+    assert pp != base or pm != base
