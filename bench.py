@@ -119,17 +119,19 @@ def cpu_baseline(scene_path, nf, steps, threads):
                       "steps 1-%d, step() calls only (%.1f s)" % (steps, secs)}
 
 
-def adjoint_cost(steps, every, warmup=5):
-    """A100k: gradient of L = mean |x_T - y|^2 w.r.t. v0, viscosity and surface
-    tension through `steps` WCSPH steps (100,000 particles, XSPH 0.01, Akinci
-    surface tension 0.1, dt = 2e-4: the largest of 5e-4 / 2.5e-4 / 2e-4 that
-    stays stable for k = 1000 at r = 0.005). Times the plain forward, the
-    recording forward (checkpoint every `every` steps) and the reverse sweep
-    (segment recompute + adjoint passes) on the same initial state."""
+def adjoint_cost(steps, every, method, warmup=5):
+    """A100k: gradient of L = mean |x_T - y|^2 w.r.t. x0, v0, viscosity and
+    surface tension through `steps` steps of 100,000 particles (XSPH 0.01,
+    Akinci surface tension 0.1). WCSPH (k = 1000, gamma = 7) at dt = 2e-4: the
+    largest of 5e-4 / 2.5e-4 / 2e-4 that stays stable at r = 0.005. DFSPH with
+    8 density and 8 divergence iterations at a fixed dt = 5e-4. Times the plain
+    forward, the recording forward (checkpoint every `every` steps) and the
+    reverse sweep (segment recompute + adjoint passes) from the same state."""
     from sph2o_amd import scenes as SC
     from sph2o_amd.simulation import Context
 
-    sc, _, _ = SC.a100k(0, dt=2e-4)
+    dt = 2e-4 if method == 0 else 5e-4
+    sc, _, _ = SC.a100k(method, dt=dt)
     td = tempfile.mkdtemp()
     path = os.path.join(td, "a100k.json")
     SC.write_json(sc, path)
@@ -159,7 +161,9 @@ def adjoint_cost(steps, every, warmup=5):
     g = ctx.adjoint({sid: (seed, None)})[sid]
     t_adj = time.perf_counter() - t0
     ctx.close()
-    return {"workload": "A100k: WCSPH, %d fluid particles, %d steps at dt=2e-4, checkpoint every %d" % (n, steps, every),
+    name = "WCSPH" if method == 0 else "DFSPH 8+8 iterations"
+    return {"workload": "A100k: %s, %d fluid particles, %d steps at dt=%g, checkpoint every %d"
+                        % (name, n, steps, dt, every),
             "forward_s": t_fwd, "record_s": t_rec, "adjoint_s": t_adj, "record_over_forward": t_rec / t_fwd,
             "adjoint_over_forward": t_adj / t_fwd,
             "loss": float(np.mean(np.sum((xT - y) ** 2, axis=1))),
@@ -292,8 +296,9 @@ def main():
 
     adjoint = None
     if args.adjoint_steps > 0:
-        log("adjoint cost: A100k, %d steps" % args.adjoint_steps)
-        adjoint = adjoint_cost(args.adjoint_steps, 20)
+        log("adjoint cost: A100k WCSPH and DFSPH, %d steps" % args.adjoint_steps)
+        adjoint = {"wcsph": adjoint_cost(args.adjoint_steps, 20, 0),
+                   "dfsph": adjoint_cost(args.adjoint_steps, 20, 2)}
 
     cpu = None
     if not args.no_cpu:
@@ -318,7 +323,7 @@ def main():
                      "avg_launch_ms": avg_ms, "launches": klaunch, "algorithmic_bytes_per_launch": bytes_per_launch},
         "batched_gain": batched["gain"] if batched else None,
         "batched": batched,
-        "adjoint_fwd_cost": adjoint["adjoint_over_forward"] if adjoint else None,
+        "adjoint_fwd_cost": max(a["adjoint_over_forward"] for a in adjoint.values()) if adjoint else None,
         "adjoint": adjoint,
         "cpu_baseline": cpu,
     }

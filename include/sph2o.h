@@ -164,13 +164,15 @@ const char* sph_kernel_name(int kernel_id);
 int sph_kernel_stats(const sph_ctx* ctx, int kernel_id, double* total_ms, int64_t* launches);
 int sph_reset_kernel_stats(sph_ctx* ctx);
 
-/* ---- Gradients of a WCSPH rollout. The reference has no derivatives: these
- * entry points have no counterpart there; they serve optimisation / inverse
- * design callers and the torch.autograd.Function in sph2o_amd/autograd.py.
- * Supported: WCSPH (WCSPHSolver::step, WCSPHSolver.cpp:14-78) with Akinci
- * boundaries, viscosity off or XSPH (XSPHViscosity.cpp:6-42) and Akinci
- * surface tension (SurfaceTensionAkinci.cpp:5-71); other options give
- * SPH_E_UNSUPPORTED. ---- */
+/* ---- Gradients of WCSPH and DFSPH rollouts. The reference has no
+ * derivatives: these entry points have no counterpart there; they serve
+ * optimisation / inverse-design callers and the torch.autograd.Function in
+ * sph2o_amd/autograd.py. Supported: WCSPH (WCSPHSolver::step,
+ * WCSPHSolver.cpp:14-78) and DFSPH (DFSPHSolver::step, DFSPHSolver.cpp:33-107)
+ * with a fixed time step and a fixed iteration count (min_iterations ==
+ * min_iterations_v == max_iterations), Akinci boundaries, viscosity off or
+ * XSPH (XSPHViscosity.cpp:6-42) and Akinci surface tension
+ * (SurfaceTensionAkinci.cpp:5-71); other options give SPH_E_UNSUPPORTED. ---- */
 
 /* Sets a scene's viscosity and surface-tension coefficients (the "viscosity"
  * and "surfaceTension" values SceneLoader.cpp:95-110 hands to the force
@@ -200,6 +202,16 @@ int sph_adjoint_run(sph_ctx* ctx);
  * rollout, dL/d(viscosity) and dL/d(surface tension) (NULL to skip). */
 int sph_adjoint_result(sph_ctx* ctx, int scene, float* dL_dx0, float* dL_dv0, double* dL_dviscosity,
                        double* dL_dsurface_tension);
+
+/* What the recorded rollout held at one step (0-based within the rollout), in
+ * original order, for inspection and for checkers that follow the same
+ * branches: field 0 / 1 the k / rho* of DFSPH density-solve pass `pass`
+ * (0..iterations-1, `iterations` = the warm start), 2 / 3 the k / clamped
+ * divergence of divergence-solve pass `pass`, 4 the positions at the start of
+ * the step (N*3), 5 the WCSPH pressure. The step must lie in the segment the
+ * tape holds (always, when checkpoint_every >= nsteps). No counterpart in the
+ * reference. */
+int sph_recorded_field(sph_ctx* ctx, int scene, int step, int field, int pass, float* out);
 
 #ifdef __cplusplus
 }

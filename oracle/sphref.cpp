@@ -65,6 +65,8 @@ double sphref_step(void* hp, int nsteps, double* stats) {
 
 // Overwrite positions / velocities (N*3 doubles, either may be NULL) at full
 // precision: the FP64 oracle's finite differences perturb them below FP32 ulps.
+// DFSPH: rho and alpha follow the new positions and the warm-start values
+// restart at zero (as the C ABI's sph_set_fluid_state does).
 void sphref_set_state64(void* hp, const double* x, const double* v) {
     with(static_cast<Handle*>(hp), [&](auto& s) {
         using R = decltype(s.x[0].x);
@@ -73,6 +75,7 @@ void sphref_set_state64(void* hp, const double* x, const double* v) {
             if (x) s.x[i] = {static_cast<R>(x[3 * i]), static_cast<R>(x[3 * i + 1]), static_cast<R>(x[3 * i + 2])};
             if (v) s.v[i] = {static_cast<R>(v[3 * i]), static_cast<R>(v[3 * i + 1]), static_cast<R>(v[3 * i + 2])};
         }
+        if (x) s.refreshDensities();
         return 0;
     });
 }
@@ -99,6 +102,23 @@ void sphref_set_cohesion_positions(void* hp, int nsteps, const float* xyz) {
                 const float* q = xyz + (k * n + i) * 3;
                 s.cohPos[k][i] = V3<float>(q[0], q[1], q[2]);
             }
+        return 0;
+    });
+}
+
+// DFSPH branches of the next `nsteps` steps from another run (see
+// Sim::brKD): arrays [nsteps][nit + 1][N] of k and rho* (density solve) and of
+// k and the clamped divergence (divergence solve). nsteps = 0 clears them.
+void sphref_set_dfsph_branch(void* hp, int nsteps, int nit, const float* kD, const float* sD, const float* kV,
+                             const float* sV) {
+    with(static_cast<Handle*>(hp), [&](auto& s) {
+        const size_t len = static_cast<size_t>(nsteps) * (nit + 1) * s.x.size();
+        s.brSteps = nsteps;
+        s.brNit = nit;
+        s.brKD.assign(kD, kD + (nsteps ? len : 0));
+        s.brSD.assign(sD, sD + (nsteps ? len : 0));
+        s.brKV.assign(kV, kV + (nsteps ? len : 0));
+        s.brSV.assign(sV, sV + (nsteps ? len : 0));
         return 0;
     });
 }

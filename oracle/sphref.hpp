@@ -166,6 +166,17 @@ class Sim {
     // pair's cohesion branch (|r| <= h/2 in the reference's FP32 length)
     // instead of this run's own positions, for the same purpose.
     std::vector<std::vector<V3<float>>> cohPos;
+    // Optional DFSPH branches of steps 1..brSteps from another (FP32) run:
+    // per step and pass (0..nit-1 the Jacobi iterations, nit the warm start)
+    // that run's k and rho* (density solve) / clamped divergence (divergence
+    // solve), [step][pass][N]. They decide the rho* >= rho0 and d >= 0 clamps,
+    // the warm-start conditions and the |k_i + k_j| > eps, |k_i| > eps gates.
+    std::vector<float> brKD, brSD, brKV, brSV;
+    int brSteps = 0, brNit = 0;
+    const float* branch(const std::vector<float>& b, int pass) const {
+        if (steps < 1 || steps > brSteps || b.empty()) return nullptr;
+        return b.data() + (static_cast<size_t>(steps - 1) * (brNit + 1) + pass) * x.size();
+    }
 
     // ---- kernels (CubicSpline.h:27-74, ViscoK.h:29-37, Cohesion.h:20-40) ----
     void setKernels() {
@@ -299,6 +310,16 @@ class Sim {
         innerHash = 0; innerPairs = 0;
     }
 
+    // DFSPH state that belongs to the positions (rho, alpha at step start) and
+    // the warm-start history, after the positions were replaced
+    void refreshDensities() {
+        if (P.method != 2) return;
+        insertFluid();
+        computeDensities();
+        computeAlpha();
+        std::fill(kws.begin(), kws.end(), static_cast<R>(0));
+        std::fill(kvws.begin(), kvws.end(), static_cast<R>(0));
+    }
     void init() {
         if (P.method == 1) computeScalingFactor();  // PCISPHSolver.cpp:11-18
         grid.clear();
@@ -620,7 +641,7 @@ class Sim {
         hh = std::min(hh, maxTs);
         dt = hh;
     }
-    void predictDensities() {  // :226-299
+    void predictDensities(const float* clamp = nullptr) {  // :226-299
         const int nf = static_cast<int>(x.size());
         const bool ak = akinci();
 #pragma omp parallel for schedule(static)
@@ -631,14 +652,14 @@ class Sim {
                 keyOf(ri), [&](int j) { dc += m * dot(vi - v[j], gradW(ri - x[j])); },
                 [&](int b) { if (ak) dc += rho0 * Vb[b] * dot(vi, gradW(ri - xb[b])); });
             R pd = rho[i] + dc * dt;
-            rhoStar[i] = std::max(pd, rho0);
+            rhoStar[i] = clamp ? (clamp[i] > static_cast<float>(rho0) ? pd : rho0) : std::max(pd, rho0);
         }
         double s = 0;
         for (int i = 0; i < nf; ++i) s += static_cast<double>((rhoStar[i] - rho0) / rho0);
         avgErr = nf ? static_cast<R>(s / nf) : 0;
     }
     // velocity correction shared by both solves and both warm starts
-    void velocityPass(const std::vector<R>& kk) {
+    void velocityPass(const std::vector<R>& kk, const float* gate = nullptr) {
         const int nf = static_cast<int>(x.size());
         const bool ak = akinci();
 #pragma omp parallel for schedule(static)
@@ -648,9 +669,10 @@ class Sim {
             V sum;
             forFluidCandidates(keyOf(ri), [&](int j) {
                 const R ks = ki + kk[j];
-                if (std::fabs(ks) > eps) sum += (m * ks) * gradW(ri - x[j]);
+                const bool open = gate ? std::fabs(gate[i] + gate[j]) > static_cast<float>(eps) : std::fabs(ks) > eps;
+                if (open) sum += (m * ks) * gradW(ri - x[j]);
             });
-            if (std::fabs(ki) > eps && ak)
+            if ((gate ? std::fabs(gate[i]) > static_cast<float>(eps) : std::fabs(ki) > eps) && ak)
                 forBoundaryCandidates(keyOf(ri), [&](int b) { sum += (rho0 * Vb[b] * ki) * gradW(ri - xb[b]); });
             v[i] -= dt * sum;
         }
@@ -658,24 +680,24 @@ class Sim {
     void correctDensityError() {  // :414-536 with densityWarmStart :851-955
         const int nf = static_cast<int>(x.size());
         const R ts2 = dt * dt;
-        predictDensities();
+        predictDensities(branch(brSD, brNit));
         for (int i = 0; i < nf; ++i) kws[i] = rhoStar[i] > rho0 ? kws[i] / ts2 : 0;
-        velocityPass(kws);
+        velocityPass(kws, branch(brKD, brNit));
         std::fill(kws.begin(), kws.end(), static_cast<R>(0));
         its = 0;
         avgErr = 0;
         while ((avgErr > static_cast<R>(P.eta) || its < P.min_it) && its < P.max_it) {
-            predictDensities();
+            predictDensities(its < brNit ? branch(brSD, its) : nullptr);
             for (int i = 0; i < nf; ++i) {
                 k[i] = (rhoStar[i] - rho0) * alpha[i] / (ts2 * rho[i]);
                 kws[i] += k[i];
             }
-            velocityPass(k);
+            velocityPass(k, its < brNit ? branch(brKD, its) : nullptr);
             ++its;
         }
         for (int i = 0; i < nf; ++i) kws[i] *= ts2;
     }
-    void computeDivergenceError() {  // :538-621
+    void computeDivergenceError(const float* clamp = nullptr) {  // :538-621
         const int nf = static_cast<int>(x.size());
         const bool ak = akinci();
 #pragma omp parallel for schedule(static)
@@ -696,7 +718,7 @@ class Sim {
                     de += rho0 * Vb[b] * dot(vi, gradW(rr));
                     if (Policy<R>::within(rr, h)) nn++;
                 });
-            dErr[i] = nn < 20 ? 0 : std::max(de, static_cast<R>(0));
+            dErr[i] = clamp ? (clamp[i] > 0.0f ? de : static_cast<R>(0)) : nn < 20 ? 0 : std::max(de, static_cast<R>(0));
         }
         double s = 0;
         for (int i = 0; i < nf; ++i) s += static_cast<double>(dErr[i] / rho0);
@@ -704,20 +726,20 @@ class Sim {
     }
     void correctDivergenceError() {  // :730-849 with divergenceWarmStart :957-1059
         const int nf = static_cast<int>(x.size());
-        computeDivergenceError();
+        computeDivergenceError(branch(brSV, brNit));
         for (int i = 0; i < nf; ++i) kvws[i] = dErr[i] > 0 ? kvws[i] / dt : 0;
-        velocityPass(kvws);
+        velocityPass(kvws, branch(brKV, brNit));
         std::fill(kvws.begin(), kvws.end(), static_cast<R>(0));
         avgDiv = 0;
         itsV = 0;
         // the loop bound is maxIterations, not maxIterationsV (:746)
         while ((avgDiv > static_cast<R>(P.eta_v) / dt || itsV < P.min_it_v) && itsV < P.max_it) {
-            computeDivergenceError();
+            computeDivergenceError(itsV < brNit ? branch(brSV, itsV) : nullptr);
             for (int i = 0; i < nf; ++i) {
                 kv[i] = dErr[i] * alpha[i] / (dt * rho[i]);
                 kvws[i] += kv[i];
             }
-            velocityPass(kv);
+            velocityPass(kv, itsV < brNit ? branch(brKV, itsV) : nullptr);
             ++itsV;
         }
         for (int i = 0; i < nf; ++i) kvws[i] *= dt;

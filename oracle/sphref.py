@@ -76,6 +76,7 @@ def lib():
         L.sphref_cohesion_print.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64)]
         L.sphref_set_pressure_branch.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_uint8)]
         L.sphref_set_cohesion_positions.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_float)]
+        L.sphref_set_dfsph_branch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int] + [P(ctypes.c_float)] * 4
         _lib = L
     return _lib
 
@@ -221,6 +222,14 @@ class Oracle:
             return
         q = np.ascontiguousarray(xs, np.float32).reshape(-1, self.nf, 3)
         lib().sphref_set_cohesion_positions(self.h, len(q), _p(q, ctypes.c_float))
+
+    def set_dfsph_branch(self, kD, sD, kV, sV):
+        """DFSPH: branches of the next steps from another run, each array
+        [nsteps][nit + 1][N] float32 (pass nit = warm start): k and rho* of the
+        density solve, k and clamped divergence of the divergence solve."""
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (kD, sD, kV, sV)]
+        nsteps, nit1 = arrs[0].shape[0], arrs[0].shape[1]
+        lib().sphref_set_dfsph_branch(self.h, nsteps, nit1 - 1, *(_p(a, ctypes.c_float) for a in arrs))
 
     def cohesion_print(self):
         """(hash, count) of the pairs that took the inner cohesion branch so far."""

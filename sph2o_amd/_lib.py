@@ -30,6 +30,7 @@ EXPORTS = [
     "sph_get_time", "sph_dump_cells", "sph_dump_neighbors", "sph_last_step_timing", "sph_set_profiling",
     "sph_kernel_count", "sph_kernel_name", "sph_kernel_stats", "sph_reset_kernel_stats",
     "sph_set_coefficients", "sph_forward_record", "sph_adjoint_seed", "sph_adjoint_run", "sph_adjoint_result",
+    "sph_recorded_field",
 ]
 
 
@@ -116,6 +117,7 @@ def load():
         "sph_adjoint_seed": (ctypes.c_int, [vp, ctypes.c_int, fp, fp]),
         "sph_adjoint_run": (ctypes.c_int, [vp]),
         "sph_adjoint_result": (ctypes.c_int, [vp, ctypes.c_int, fp, fp, P(ctypes.c_double), P(ctypes.c_double)]),
+        "sph_recorded_field": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, fp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

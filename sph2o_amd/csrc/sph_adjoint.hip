@@ -83,19 +83,20 @@ __global__ void __launch_bounds__(kBlock) kAdjSeed(AdjView a) {
     a.vBar[i] = make_float4(vx, vy, vz, 0.0f);
     a.xBar[i] = make_float4(gx.x, gx.y, gx.z, 0.0f);
     a.aBar[i] = make_float4(dt * vx, dt * vy, dt * vz, 0.0f);
+    a.rhoBar[i] = 0.0f;
 }
 
 // Pass 1: adjoint of the pressure force, XSPH viscosity and the surface-tension
 // force given a-bar. Produces x-bar (partial), v-bar (final), normal-bar,
 // rho-bar (partial, incl. the pressure chain p(rho) and A = p / rho^2) and the
 // per-block sums of a-bar . da/dnu and a-bar . da/dgam.
-__global__ void __launch_bounds__(kBlock) kAdjForces(AdjView a) {
+__global__ void __launch_bounds__(kBlock) kAdjForces(AdjView a, int pressure) {
     ADJ_PROLOGUE
     double nuTerm = 0.0, stTerm = 0.0;
     if (valid) {
         const size_t NP = a.NP;
         const F3 xi = f3(a.x[i]), vi = f3(a.v[i]), ai = f3(a.aBar[i]), ni = f3(a.n[i]);
-        const float rhoi = a.rho[i], pi_ = a.p[i];
+        const float rhoi = a.rho[i], pi_ = pressure ? a.p[i] : 0.0f;
         const float Ai = pi_ / (rhoi * rhoi);
         const bool xsph = S.viscMethod == 2, st = S.stOn != 0;
         const float invdt = 1.0f / dt;
@@ -113,12 +114,14 @@ __global__ void __launch_bounds__(kBlock) kAdjForces(AdjView a) {
             const F3 da = ai - aj;
             const float rda = dot3(r, da);
             const float rhoj = a.rho[j];
-            // pressure: A-bar_k = -m sum (a_k - a_j) . gradW; x-bar_k -= m (A_k + A_j) H (a_k - a_j)
-            const float Aj = a.p[j] / (rhoj * rhoj);
-            gA -= S.m * G * rda;
-            const float cP = -S.m * (Ai + Aj);
-            axpy(gx, cP * G, da);
-            axpy(gx, cP * G2 * rda, r);
+            if (pressure) {
+                // A-bar_k = -m sum (a_k - a_j) . gradW; x-bar_k -= m (A_k + A_j) H (a_k - a_j)
+                const float Aj = a.p[j] / (rhoj * rhoj);
+                gA -= S.m * G * rda;
+                const float cP = -S.m * (Ai + Aj);
+                axpy(gx, cP * G, da);
+                axpy(gx, cP * G2 * rda, r);
+            }
             if (xsph) {
                 const F3 dv = f3(a.v[j]) - vi;
                 const float W = cubicW(S, rl);
@@ -163,11 +166,13 @@ __global__ void __launch_bounds__(kBlock) kAdjForces(AdjView a) {
                 const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
                 const float psi = q.w;
                 const float rai = dot3(r, ai);
-                gA -= psi * G * rai;
-                gpMirror -= psi * G * rai;
-                const float cPb = -psi * (Ai + pm);
-                axpy(gx, cPb * G, ai);
-                axpy(gx, cPb * G2 * rai, r);
+                if (pressure) {
+                    gA -= psi * G * rai;
+                    gpMirror -= psi * G * rai;
+                    const float cPb = -psi * (Ai + pm);
+                    axpy(gx, cPb * G, ai);
+                    axpy(gx, cPb * G2 * rai, r);
+                }
                 if (cb != 0.0f) {
                     const float W = cubicW(S, rl);
                     axpy(gv, -cb * (psi / rhoi) * W, ai);
@@ -185,7 +190,7 @@ __global__ void __launch_bounds__(kBlock) kAdjForces(AdjView a) {
         a.xBar[i] = make_float4(x0.x + gx.x, x0.y + gx.y, x0.z + gx.z, 0.0f);
         a.vBar[i] = make_float4(v0.x + gv.x, v0.y + gv.y, v0.z + gv.z, 0.0f);
         a.nBar[i] = f4(gn);
-        a.rhoBar[i] = gr;
+        a.rhoBar[i] += gr;
         nuTerm = static_cast<double>(nuSum) * static_cast<double>(invdt);
         stTerm = static_cast<double>(stSum);
     }
@@ -275,6 +280,251 @@ __global__ void __launch_bounds__(kBlock) kAdjFinish(AdjView a) {
     }
 }
 
+// ---------------------------------------------------------------- DFSPH
+// Reverse passes of one DFSPHSolver::step (DFSPHSolver.cpp:33-107) with a
+// fixed iteration count and a fixed dt. A step has two halves: the first at
+// the start positions x_n (forces, predictVelocities, the density solve), the
+// second at x_{n+1} = x_n + v dt after the neighbour search (densities, alpha,
+// the divergence solve). Each half is reversed in its own slot order; the
+// adjoints carried between halves and steps (x, v, the scaled k_ws / kv_ws,
+// rho and alpha at the start positions) live in original particle order.
+//
+// One Jacobi iteration is a source pass (rho* or divergence, from velocity u)
+// and a velocity pass:
+//   rho*_i = max(rho_i + dt sum_j m gradW_ij.(u_i - u_j) + dt sum_b psi_b gradW_ib.u_i, rho0)   (:226-299)
+//   k_i    = (rho*_i - rho0) alpha_i / (dt^2 rho_i),  k_ws += k                             (:435-453)
+//   d_i    = [n_i >= 20] max(sum_j m gradW_ij.(u_i - u_j) + sum_b psi_b gradW_ib.u_i, 0)    (:538-621)
+//   k_i    = d_i alpha_i / (dt rho_i),  kv_ws += k                                         (:750-768)
+//   u'_i   = u_i - dt sum_j m (k_i + k_j) gradW_ij [|k_i+k_j| > eps] - dt sum_b psi_b k_i gradW_ib [|k_i| > eps]
+// Warm starts take k = k_ws / dt^2 where rho* > rho0 (:873-884), k = kv_ws / dt
+// where d > 0 (:977-989). Gates and clamps are piecewise constant.
+
+#define ADJ_GRAD_PAIR(xi, j)                               \
+    const F3 r = xi - f3(a.x[j]);                          \
+    const float rl = fastSqrt(dot3(r, r));                 \
+    const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
+
+// Gather the carried adjoints into the slot order of one half.
+//   second half: x-bar, u-bar (= v-bar of the step's result), rho-bar and
+//     alpha-bar of the half's densities (from the next step's first half)
+//   first half: x-bar, v-bar + dt x-bar (x_{n+1} = x_n + v dt), zero rho-bar
+//   final (mode 2): x-bar, rho-bar, alpha-bar at the initial positions
+__global__ void __launch_bounds__(kBlock) kAdjHalfStart(AdjView a, int mode) {
+    ADJ_PROLOGUE
+    if (!valid) return;
+    const uint32_t o = S.fBegin + a.id[i];
+    const float4 gx = a.adjX[o];
+    a.xBar[i] = make_float4(gx.x, gx.y, gx.z, 0.0f);
+    if (mode == 0) {
+        const float4 gv = a.adjV[o];
+        a.vBar[i] = make_float4(gv.x + dt * gx.x, gv.y + dt * gx.y, gv.z + dt * gx.z, 0.0f);
+        a.rhoBar[i] = 0.0f;
+        a.alphaBar[i] = 0.0f;
+    } else {
+        if (mode == 1) a.vBar[i] = a.adjV[o];
+        a.rhoBar[i] = a.adjRho[o];
+        a.alphaBar[i] = a.adjAlpha[o];
+    }
+}
+
+// Scatter a half's results back to original order.
+__global__ void __launch_bounds__(kBlock) kAdjHalfEnd(AdjView a, int mode) {
+    ADJ_PROLOGUE
+    if (!valid) return;
+    const uint32_t o = S.fBegin + a.id[i];
+    a.adjX[o] = a.xBar[i];
+    if (mode != 2) a.adjV[o] = a.vBar[i];
+    if (mode == 0) {
+        a.adjRho[o] = a.rhoBar[i];
+        a.adjAlpha[o] = a.alphaBar[i];
+    }
+}
+
+// Adjoint of a velocity pass given u'-bar (vBar, left unchanged: u' = u - ...):
+// k-bar, x-bar from the gradient's Jacobian, then the pass kind's chain:
+// loop passes add the k_ws / kv_ws accumulation and go through k(rho*, alpha,
+// rho) / k(d, alpha, rho) to the source adjoint sBar; warm passes give the
+// adjoint of the k_ws / kv_ws that entered the step.
+__global__ void __launch_bounds__(kBlock) kAdjVelocity(AdjView a, int mode) {
+    ADJ_PROLOGUE
+    if (!valid) return;
+    const size_t NP = a.NP;
+    const F3 xi = f3(a.x[i]), ui = f3(a.vBar[i]);
+    const float ki = a.kk[i];
+    F3 gx{0.0f, 0.0f, 0.0f};
+    float kb = 0.0f;
+    const uint32_t nf = min(a.cntF[i], a.capF);
+    for (uint32_t s = 0; s < nf; ++s) {
+        const uint32_t j = a.nbrF[s * NP + i];
+        const float ks = ki + a.kk[j];
+        if (fabsf(ks) > S.eps) {
+            ADJ_GRAD_PAIR(xi, j)
+            const F3 du = ui - f3(a.vBar[j]);
+            const float rdu = dot3(r, du);
+            kb -= dt * S.m * G * rdu;
+            const float c = -dt * S.m * ks;
+            axpy(gx, c * G, du);
+            axpy(gx, c * G2 * rdu, r);
+        }
+    }
+    if (S.akinci && fabsf(ki) > S.eps) {
+        const uint32_t nb = min(a.cntB[i], a.capB);
+        for (uint32_t s = 0; s < nb; ++s) {
+            const float4 q = a.xb[a.nbrB[s * NP + i]];
+            const F3 r = xi - f3(q);
+            const float rl = fastSqrt(dot3(r, r));
+            const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
+            const float rui = dot3(r, ui);
+            kb -= dt * q.w * G * rui;
+            const float c = -dt * q.w * ki;
+            axpy(gx, c * G, ui);
+            axpy(gx, c * G2 * rui, r);
+        }
+    }
+    const float4 x0 = a.xBar[i];
+    a.xBar[i] = make_float4(x0.x + gx.x, x0.y + gx.y, x0.z + gx.z, 0.0f);
+    const uint32_t o = S.fBegin + a.id[i];
+    const float rhoi = a.rho[i], al = a.alpha[i], src = a.aux[i];
+    const float dt2 = dt * dt;
+    if (mode == kAdjDensityLoop) {
+        kb += dt2 * a.adjKws[o];
+        const float ex = src - S.rho0;
+        a.sBar[i] = src > S.rho0 ? kb * al / (dt2 * rhoi) : 0.0f;
+        a.alphaBar[i] += kb * ex / (dt2 * rhoi);
+        a.rhoBar[i] += -kb * ex * al / (dt2 * rhoi * rhoi) + a.sBar[i];
+    } else if (mode == kAdjDivergenceLoop) {
+        kb += dt * a.adjKvws[o];
+        a.sBar[i] = src > 0.0f ? kb * al / (dt * rhoi) : 0.0f;
+        a.alphaBar[i] += kb * src / (dt * rhoi);
+        a.rhoBar[i] += -kb * src * al / (dt * rhoi * rhoi);
+    } else if (mode == kAdjDensityWarm) {
+        a.adjKws[o] = src > S.rho0 ? kb / dt2 : 0.0f;
+    } else {
+        a.adjKvws[o] = src > 0.0f ? kb / dt : 0.0f;
+    }
+}
+
+// Adjoint of a source pass (rho*: c = dt, divergence: c = 1) given its
+// adjoint sBar: into u-bar (vBar) and x-bar, with u the velocity it read.
+__global__ void __launch_bounds__(kBlock) kAdjSource(AdjView a, int mode) {
+    ADJ_PROLOGUE
+    if (!valid) return;
+    const size_t NP = a.NP;
+    const float c = mode == kAdjDensityLoop ? dt : 1.0f;
+    const F3 xi = f3(a.x[i]), ui = f3(a.u[i]);
+    const float si = a.sBar[i];
+    F3 gx{0.0f, 0.0f, 0.0f}, gu{0.0f, 0.0f, 0.0f};
+    const uint32_t nf = min(a.cntF[i], a.capF);
+    for (uint32_t s = 0; s < nf; ++s) {
+        const uint32_t j = a.nbrF[s * NP + i];
+        const float ss = si + a.sBar[j];
+        if (ss == 0.0f) continue;
+        ADJ_GRAD_PAIR(xi, j)
+        const F3 du = ui - f3(a.u[j]);
+        const float w = c * S.m * ss;
+        axpy(gu, w * G, r);
+        axpy(gx, w * G, du);
+        axpy(gx, w * G2 * dot3(r, du), r);
+    }
+    if (S.akinci && si != 0.0f) {
+        const uint32_t nb = min(a.cntB[i], a.capB);
+        for (uint32_t s = 0; s < nb; ++s) {
+            const float4 q = a.xb[a.nbrB[s * NP + i]];
+            const F3 r = xi - f3(q);
+            const float rl = fastSqrt(dot3(r, r));
+            const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
+            const float w = c * q.w * si;
+            axpy(gu, w * G, r);
+            axpy(gx, w * G, ui);
+            axpy(gx, w * G2 * dot3(r, ui), r);
+        }
+    }
+    const float4 x0 = a.xBar[i], v0 = a.vBar[i];
+    a.xBar[i] = make_float4(x0.x + gx.x, x0.y + gx.y, x0.z + gx.z, 0.0f);
+    a.vBar[i] = make_float4(v0.x + gu.x, v0.y + gu.y, v0.z + gu.z, 0.0f);
+}
+
+// predictVelocities (DFSPHSolver.cpp:180-202): v* = v + (a + g) dt, so
+// a-bar = dt v*-bar and v-bar = v*-bar.
+__global__ void __launch_bounds__(kBlock) kAdjPredict(AdjView a) {
+    ADJ_PROLOGUE
+    if (!valid) return;
+    const float4 v = a.vBar[i];
+    a.aBar[i] = make_float4(dt * v.x, dt * v.y, dt * v.z, 0.0f);
+}
+
+// computeDensities + computeAlpha (DFSPHSolver.cpp:109-178), part 1:
+// alpha = rho / den (den > eps) with den = |S|^2 + sum_j |m gradW_ij|^2 and
+// S = sum_j m gradW_ij + sum_b psi_b gradW_ib. rho-bar += alpha-bar / den;
+// den-bar = -alpha-bar rho / den^2; sgBar = (2 den-bar S, den-bar).
+__global__ void __launch_bounds__(kBlock) kAdjAlphaA(AdjView a) {
+    ADJ_PROLOGUE
+    if (!valid) return;
+    const size_t NP = a.NP;
+    const F3 xi = f3(a.x[i]);
+    F3 sg{0.0f, 0.0f, 0.0f};
+    float ssq = 0.0f;
+    const uint32_t nf = min(a.cntF[i], a.capF);
+    for (uint32_t s = 0; s < nf; ++s) {
+        const uint32_t j = a.nbrF[s * NP + i];
+        const F3 r = xi - f3(a.x[j]);
+        const float g = S.m * gradFactor(S, fastSqrt(dot3(r, r)));
+        axpy(sg, g, r);
+        ssq += g * g * dot3(r, r);
+    }
+    if (S.akinci) {
+        const uint32_t nb = min(a.cntB[i], a.capB);
+        for (uint32_t s = 0; s < nb; ++s) {
+            const float4 q = a.xb[a.nbrB[s * NP + i]];
+            const F3 r = xi - f3(q);
+            axpy(sg, q.w * gradFactor(S, fastSqrt(dot3(r, r))), r);
+        }
+    }
+    const float den = dot3(sg, sg) + ssq;
+    float dbar = 0.0f;
+    if (den > S.eps) {
+        const float ab = a.alphaBar[i];
+        a.rhoBar[i] += ab / den;
+        dbar = -ab * a.rho[i] / (den * den);
+    }
+    a.sgBar[i] = make_float4(2.0f * dbar * sg.x, 2.0f * dbar * sg.y, 2.0f * dbar * sg.z, dbar);
+}
+
+// Part 2: x-bar through S, through sum |m gradW|^2 and through the density sum.
+__global__ void __launch_bounds__(kBlock) kAdjAlphaB(AdjView a) {
+    ADJ_PROLOGUE
+    if (!valid) return;
+    const size_t NP = a.NP;
+    const F3 xi = f3(a.x[i]);
+    const float4 sgi = a.sgBar[i];
+    const F3 si = f3(sgi);
+    const float rk = a.rhoBar[i];
+    F3 gx = f3(a.xBar[i]);
+    const uint32_t nf = min(a.cntF[i], a.capF);
+    for (uint32_t s = 0; s < nf; ++s) {
+        const uint32_t j = a.nbrF[s * NP + i];
+        ADJ_GRAD_PAIR(xi, j)
+        const float4 sgj = a.sgBar[j];
+        const F3 w = si - f3(sgj);
+        axpy(gx, S.m * G, w);
+        axpy(gx, S.m * G2 * dot3(r, w), r);
+        const float cq = 2.0f * S.m * S.m * (sgi.w + sgj.w) * G * (G + G2 * dot3(r, r));
+        axpy(gx, cq + S.m * (rk + a.rhoBar[j]) * G, r);
+    }
+    if (S.akinci) {
+        const uint32_t nb = min(a.cntB[i], a.capB);
+        for (uint32_t s = 0; s < nb; ++s) {
+            const float4 q = a.xb[a.nbrB[s * NP + i]];
+            const F3 r = xi - f3(q);
+            const float rl = fastSqrt(dot3(r, r));
+            const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
+            axpy(gx, q.w * G, si);
+            axpy(gx, q.w * G2 * dot3(r, si) + q.w * rk * G, r);
+        }
+    }
+    a.xBar[i] = f4(gx);
+}
+
 #define ADJ_LAUNCH(kernel, grid)                                             \
     do {                                                                     \
         if ((grid) == 0) return hipSuccess;                                  \
@@ -283,11 +533,31 @@ __global__ void __launch_bounds__(kBlock) kAdjFinish(AdjView a) {
     } while (0)
 
 hipError_t launchAdjSeed(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjSeed, a.numBlocks); }
-hipError_t launchAdjForces(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjForces, a.numBlocks); }
+hipError_t launchAdjForces(const AdjView& a, hipStream_t st) { return launchAdjForces2(a, 1, st); }
+hipError_t launchAdjForces2(const AdjView& a, int pressure, hipStream_t st) {
+    if (a.numBlocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(kAdjForces, dim3(a.numBlocks), dim3(kBlock), 0, st, a, pressure);
+    return hipGetLastError();
+}
 hipError_t launchAdjNormals(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjNormals, a.numBlocks); }
 hipError_t launchAdjDensity(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjDensity, a.numBlocks); }
 hipError_t launchAdjFinish(const AdjView& a, hipStream_t st) {
     ADJ_LAUNCH(kAdjFinish, static_cast<uint32_t>(a.numScenes));
 }
+
+#define ADJ_LAUNCH_MODE(kernel, mode)                                            \
+    do {                                                                         \
+        if (a.numBlocks == 0) return hipSuccess;                                 \
+        hipLaunchKernelGGL(kernel, dim3(a.numBlocks), dim3(kBlock), 0, st, a, mode); \
+        return hipGetLastError();                                                \
+    } while (0)
+
+hipError_t launchAdjHalfStart(const AdjView& a, int mode, hipStream_t st) { ADJ_LAUNCH_MODE(kAdjHalfStart, mode); }
+hipError_t launchAdjHalfEnd(const AdjView& a, int mode, hipStream_t st) { ADJ_LAUNCH_MODE(kAdjHalfEnd, mode); }
+hipError_t launchAdjVelocity(const AdjView& a, int mode, hipStream_t st) { ADJ_LAUNCH_MODE(kAdjVelocity, mode); }
+hipError_t launchAdjSource(const AdjView& a, int mode, hipStream_t st) { ADJ_LAUNCH_MODE(kAdjSource, mode); }
+hipError_t launchAdjPredict(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjPredict, a.numBlocks); }
+hipError_t launchAdjAlphaA(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjAlphaA, a.numBlocks); }
+hipError_t launchAdjAlphaB(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjAlphaB, a.numBlocks); }
 
 }  // namespace sph

@@ -43,12 +43,40 @@ struct AdjView {
     float* rhoBar;
     double* partials;    // 2 * numBlocks: viscosity and surface-tension terms
     double* acc;         // 2 per scene: dL/d(viscosity), dL/d(surface tension)
+    // DFSPH: one Jacobi pass of the recorded step
+    const float4* u;     // velocity the pass read
+    const float* kk;     // its k (density: 1/dt^2 units, divergence: 1/dt)
+    const float* aux;    // its rho* (density) or clamped divergence
+    const float* alpha;  // alpha at the step's positions
+    float* kBar;         // adjoint of k of the pass
+    float* sBar;         // adjoint of the pass's rho* / divergence
+    float* alphaBar;
+    float4* sgBar;       // density/alpha adjoint: 2 den-bar S (xyz), den-bar (w)
+    // original-order adjoints carried between the halves of a DFSPH step
+    float* adjKws;       // of the scaled k_ws / kv_ws leaving the step
+    float* adjKvws;
+    float* adjRho;       // of rho / alpha at the start-of-step positions
+    float* adjAlpha;
 };
+
+// DFSPH pass kinds of the adjoint kernels
+enum AdjMode { kAdjDensityLoop = 0, kAdjDivergenceLoop = 1, kAdjDensityWarm = 2, kAdjDivergenceWarm = 3 };
 
 hipError_t launchAdjSeed(const AdjView& a, hipStream_t st);
 hipError_t launchAdjForces(const AdjView& a, hipStream_t st);
 hipError_t launchAdjNormals(const AdjView& a, hipStream_t st);
 hipError_t launchAdjDensity(const AdjView& a, hipStream_t st);
 hipError_t launchAdjFinish(const AdjView& a, hipStream_t st);
+
+// DFSPH (fixed iterations, fixed dt)
+hipError_t launchAdjForces2(const AdjView& a, int pressure, hipStream_t st);
+// mode: 0 first half, 1 second half, 2 the initial densities after the last reversed step
+hipError_t launchAdjHalfStart(const AdjView& a, int mode, hipStream_t st);
+hipError_t launchAdjHalfEnd(const AdjView& a, int mode, hipStream_t st);
+hipError_t launchAdjVelocity(const AdjView& a, int mode, hipStream_t st);
+hipError_t launchAdjSource(const AdjView& a, int mode, hipStream_t st);
+hipError_t launchAdjPredict(const AdjView& a, hipStream_t st);
+hipError_t launchAdjAlphaA(const AdjView& a, hipStream_t st);
+hipError_t launchAdjAlphaB(const AdjView& a, hipStream_t st);
 
 }  // namespace sph

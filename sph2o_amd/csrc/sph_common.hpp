@@ -40,7 +40,11 @@ struct SceneConst {
     float eta, etaV, cfl, radius, minTs, maxTs;
     int32_t minIt, maxIt, minItV, adaptive;
     int32_t nf;
-    int32_t pad[3];
+    // Cells outside the dense box hash to ovMask + 1 overflow buckets: sort
+    // keys deadKey - (ovMask + 1) .. deadKey - 1, ranges at cellBase + nCells.
+    uint32_t ovMask;
+    uint32_t nCells;           // dense cells gnx * gny * gnz
+    int32_t pad;
 };
 
 // Per-scene solver state, owned by the device while a context steps: the
@@ -60,8 +64,17 @@ struct SceneState {
     int32_t pad[2];
 };
 
-// Device-side flag slots per scene (int32 each).
-enum FlagSlot { kFlagOutOfGrid = 0, kFlagMaxFluidNbr = 1, kFlagMaxBoundaryNbr = 2, kFlagNonFinite = 3, kFlagSlots = 4 };
+// Device-side flag slots per scene (int32 each): particles outside the dense
+// grid (count), largest fluid / boundary neighbour counts, non-finite
+// positions, coordinates beyond the representable cell range.
+enum FlagSlot {
+    kFlagOutOfGrid = 0,
+    kFlagMaxFluidNbr = 1,
+    kFlagMaxBoundaryNbr = 2,
+    kFlagNonFinite = 3,
+    kFlagFar = 4,
+    kFlagSlots = 5
+};
 // Reduction output slots per scene (double each).
 enum RedSlot { kRedSum = 0, kRedMax = 1, kRedSlots = 2 };
 

@@ -87,13 +87,15 @@ struct DevBuf {
 enum KernelId {
     K_KEYS, K_SORT, K_REORDER, K_BUILD, K_DENSITY, K_FORCES, K_SURFACE_TENSION, K_VMAX, K_PREDICT, K_RHO_STAR,
     K_VELOCITY, K_DIVERGENCE, K_SCALE, K_INTEGRATE, K_PCI_INIT, K_PCI_DENSITY, K_PCI_PACC, K_PCI_INTEGRATE,
-    K_FILL, K_FINISH, K_ADJ_SEED, K_ADJ_FORCES, K_ADJ_NORMALS, K_ADJ_DENSITY, K_ADJ_FINISH, K_COUNT
+    K_FILL, K_FINISH, K_ADJ_SEED, K_ADJ_FORCES, K_ADJ_NORMALS, K_ADJ_DENSITY, K_ADJ_FINISH, K_ADJ_VELOCITY,
+    K_ADJ_SOURCE, K_COUNT
 };
 const char* kKernelNames[K_COUNT] = {"keys", "sort", "reorder", "build_neighbors", "density", "forces",
                                      "surface_tension", "vmax", "predict_velocity", "rho_star", "velocity",
                                      "divergence", "scale", "integrate", "pci_init", "pci_density",
                                      "pci_pressure_acc", "pci_integrate", "fill_ranges", "finish",
-                                     "adj_seed", "adj_forces", "adj_normals", "adj_density", "adj_finish"};
+                                     "adj_seed", "adj_forces", "adj_normals", "adj_density", "adj_finish",
+                                     "adj_velocity", "adj_source"};
 
 // Host mirror of the reference's FP32 cubic-spline gradient (CubicSpline.h:52-74)
 // with the compiled reference's length; used for the PCISPH scaling factor.
@@ -140,6 +142,7 @@ struct SceneH {
     uint32_t fBegin = 0, fEnd = 0, fPadEnd = 0, blockBegin = 0, blockEnd = 0, bBegin = 0, bEnd = 0;
     int g0[3] = {0, 0, 0}, gn[3] = {1, 1, 1};
     uint32_t cellBase = 0, nCells = 0, keyBase = 0, keyRange = 0;
+    uint32_t ovBuckets = 1024;  // overflow buckets for cells outside the dense box (power of two)
     int morton = 0;
     std::vector<uint32_t> bOrder;  // sorted boundary position -> original index
 };
@@ -327,6 +330,7 @@ class Engine {
             s.morton = 0;
             s.keyRange = s.nCells + 1;
         }
+        s.keyRange += s.ovBuckets;  // dense keys, overflow buckets, dead key
     }
     void cellOfHost(const SceneH& s, const float* p, int c[3]) const {
         for (int a = 0; a < 3; ++a) c[a] = static_cast<int>(std::floor(p[a] * s.invh)) - s.g0[a];
@@ -346,7 +350,7 @@ class Engine {
         for (auto& s : scenes) {
             s.cellBase = static_cast<uint32_t>(cells);
             s.keyBase = static_cast<uint32_t>(keysTot);
-            cells += s.nCells;
+            cells += s.nCells + s.ovBuckets;
             keysTot += s.keyRange;
             if (cells > (1ull << 30) || keysTot > (1ull << 32) - 1)
                 throw SphError{SPH_E_OUT_OF_GRID, "combined grids too large"};
@@ -402,6 +406,8 @@ class Engine {
             c.minItV = static_cast<int32_t>(std::min<uint32_t>(s.P.min_iterations_v, 1u << 30));
             c.adaptive = (s.P.simulation_method == SPH_DFSPH && s.P.adaptive_time_step) ? 1 : 0;
             c.nf = static_cast<int32_t>(s.nf);
+            c.ovMask = s.ovBuckets - 1;
+            c.nCells = s.nCells;
         }
         if (dScenes.n != scenes.size()) dScenes.alloc(scenes.size());
         HIPCHK(hipStreamSynchronize(stream));
@@ -670,15 +676,24 @@ class Engine {
     // true if the lists are valid; otherwise grows what overflowed and returns
     // false (the caller repeats the search without the position update).
     bool searchValid() {
-        bool outOfGrid = false;
+        bool layout = false;
         int maxF = 0, maxB = 0;
-        for (size_t s = 0; s < scenes.size(); ++s) {
-            outOfGrid |= hFlags[s * kFlagSlots + kFlagOutOfGrid] != 0;
-            maxF = std::max(maxF, hFlags[s * kFlagSlots + kFlagMaxFluidNbr]);
-            maxB = std::max(maxB, hFlags[s * kFlagSlots + kFlagMaxBoundaryNbr]);
+        for (size_t si = 0; si < scenes.size(); ++si) {
+            const int32_t* f = hFlags + si * kFlagSlots;
+            if (f[kFlagNonFinite]) throw SphError{SPH_E_NONFINITE, "non-finite particle position"};
+            if (f[kFlagFar]) throw SphError{SPH_E_OUT_OF_GRID, "particle beyond 1e8 cells from the origin"};
+            maxF = std::max(maxF, f[kFlagMaxFluidNbr]);
+            maxB = std::max(maxB, f[kFlagMaxBoundaryNbr]);
+            // particles outside the dense box sit in overflow buckets; past half
+            // a bucket each, grow the box if that stays affordable, else the buckets
+            const uint32_t outside = static_cast<uint32_t>(std::max(0, f[kFlagOutOfGrid]));
+            if (outside > scenes[si].ovBuckets / 2) {
+                if (!growGrid(si)) growBuckets(scenes[si], outside);
+                layout = true;
+            }
         }
-        if (outOfGrid) {
-            regrid();
+        if (layout) {
+            uploadLayout();
             return false;
         }
         if (static_cast<uint32_t>(maxF) > capF || static_cast<uint32_t>(maxB) > capB) {
@@ -688,6 +703,12 @@ class Engine {
             return false;
         }
         return true;
+    }
+    static void growBuckets(SceneH& s, uint32_t outside) {
+        uint32_t b = s.ovBuckets;
+        while (b < 4 * outside && b < (1u << 24)) b <<= 1;
+        s.keyRange += b - s.ovBuckets;
+        s.ovBuckets = b;
     }
     // Search and check at once (one host sync).
     void neighborSearch(bool dfsphAdvance) {
@@ -700,25 +721,29 @@ class Engine {
         throw SphError{SPH_E_OUT_OF_GRID, "could not place particles into a grid"};
     }
 
-    // Grow the grids to the current fluid extents (rare path).
-    void regrid() {
-        std::vector<float4> hx(NP);
-        HIPCHK(hipMemcpy(hx.data(), x.p, NP * sizeof(float4), hipMemcpyDeviceToHost));
-        for (auto& s : scenes) {
-            float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
-            for (uint32_t i = s.fBegin; i < s.fEnd; ++i) {
-                const float4 q = hx[i];
-                if (!std::isfinite(q.x) || !std::isfinite(q.y) || !std::isfinite(q.z))
-                    throw SphError{SPH_E_NONFINITE, "non-finite particle position"};
-                lo[0] = std::min(lo[0], q.x); lo[1] = std::min(lo[1], q.y); lo[2] = std::min(lo[2], q.z);
-                hi[0] = std::max(hi[0], q.x); hi[1] = std::max(hi[1], q.y); hi[2] = std::max(hi[2], q.z);
-            }
-            aabb(s.xb, lo, hi);
-            if (s.nf == 0 && s.nb == 0) continue;
-            setGrid(s, lo, hi);
+    // Grow scene si's dense box to the current fluid and boundary extents when
+    // the box stays within a cell budget (the buckets take the rest); returns
+    // false, leaving the box, otherwise.
+    bool growGrid(size_t si) {
+        SceneH& s = scenes[si];
+        std::vector<float4> hx(s.nf);
+        if (s.nf) HIPCHK(hipMemcpy(hx.data(), x.p + s.fBegin, s.nf * sizeof(float4), hipMemcpyDeviceToHost));
+        float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+        for (const float4& q : hx) {
+            lo[0] = std::min(lo[0], q.x); lo[1] = std::min(lo[1], q.y); lo[2] = std::min(lo[2], q.z);
+            hi[0] = std::max(hi[0], q.x); hi[1] = std::max(hi[1], q.y); hi[2] = std::max(hi[2], q.z);
         }
-        uploadLayout();
+        aabb(s.xb, lo, hi);
+        double cells = 1.0;
+        for (int a = 0; a < 3; ++a) {
+            const double ext = std::floor(static_cast<double>(hi[a] * s.invh)) - std::floor(static_cast<double>(lo[a] * s.invh)) + 1;
+            cells *= ext + 2 * (4 + std::ceil(0.1 * ext));
+        }
+        if (cells > static_cast<double>(kMaxDenseCells)) return false;
+        setGrid(s, lo, hi);
+        return true;
     }
+    static constexpr uint32_t kMaxDenseCells = 1u << 25;
 
     // ------------------------------------------------------------ steps
     void stepAll(int nsteps, sph_step_stats* stats) {
@@ -943,27 +968,36 @@ class Engine {
     // atomics in values), and runs the adjoint passes step by step backwards.
     struct GridSave {
         int g0[3], gn[3];
-        uint32_t nCells, keyRange;
+        uint32_t nCells, keyRange, ovBuckets;
         int morton;
         bool operator==(const GridSave& o) const {
             return std::equal(g0, g0 + 3, o.g0) && std::equal(gn, gn + 3, o.gn) && nCells == o.nCells &&
-                   keyRange == o.keyRange && morton == o.morton;
+                   keyRange == o.keyRange && ovBuckets == o.ovBuckets && morton == o.morton;
         }
     };
     struct Checkpoint {
         DevBuf<float4> x, v;
         DevBuf<uint32_t> id;
+        DevBuf<float> kws, kvws;  // DFSPH warm-start state
         std::vector<GridSave> grids;
         std::vector<SceneState> state;
         std::vector<float> t, dt;
     };
+    // One half of a recorded step (WCSPH: the whole step). Per step: the
+    // state after the half's neighbour search (x, v, ids, lists), rho, p,
+    // alpha, normals; DFSPH halves also keep, per Jacobi iteration, the
+    // velocity its source pass read (u) and the pass's k and rho* / divergence
+    // (aux), the warm-start pass at index nit.
+    struct HalfTape {
+        DevBuf<float4> x, v, n, u;
+        DevBuf<float> rho, p, alpha, k, aux;
+        DevBuf<uint32_t> id, cntF, cntB, nbrF, nbrB;
+    };
     struct Tape {
-        int steps = 0;
+        int steps = 0, nit = 0, halves = 0;
         size_t np = 0;
         uint32_t capF = 0, capB = 0;
-        DevBuf<float4> x, v, n;
-        DevBuf<float> rho, p;
-        DevBuf<uint32_t> id, cntF, cntB, nbrF, nbrB;
+        HalfTape h[2];
     };
     struct Recording {
         bool valid = false;
@@ -975,8 +1009,8 @@ class Engine {
     Recording rec;
     Tape tape;
     uint64_t stateEpoch = 0;  // bumped by every call that changes the state outside a recording
-    DevBuf<float4> adjX, adjV, aBar, xBar, vBar, nBar;
-    DevBuf<float> rhoBar;
+    DevBuf<float4> adjX, adjV, aBar, xBar, vBar, nBar, sgBar;
+    DevBuf<float> rhoBar, alphaBar, kBar, sBar, adjKws, adjKvws, adjRho, adjAlpha;
     DevBuf<double> adjPartials, adjAcc;
     std::vector<std::vector<float>> seedX, seedV;
     std::vector<std::vector<float>> gradX0, gradV0;
@@ -995,6 +1029,14 @@ class Engine {
         copyD(c.x.p, x.p, NP * sizeof(float4));
         copyD(c.v.p, v.p, NP * sizeof(float4));
         copyD(c.id.p, id.p, NP * sizeof(uint32_t));
+        if (method == SPH_DFSPH) {
+            if (c.kws.n != NP) {
+                c.kws.alloc(NP);
+                c.kvws.alloc(NP);
+            }
+            copyD(c.kws.p, kws.p, NP * sizeof(float));
+            copyD(c.kvws.p, kvws.p, NP * sizeof(float));
+        }
         c.grids.resize(scenes.size());
         c.t.resize(scenes.size());
         c.dt.resize(scenes.size());
@@ -1005,6 +1047,7 @@ class Engine {
             std::copy(sc.gn, sc.gn + 3, g.gn);
             g.nCells = sc.nCells;
             g.keyRange = sc.keyRange;
+            g.ovBuckets = sc.ovBuckets;
             g.morton = sc.morton;
             c.t[i] = sc.t;
             c.dt[i] = sc.dt;
@@ -1021,6 +1064,7 @@ class Engine {
             std::copy(sc.gn, sc.gn + 3, cur.gn);
             cur.nCells = sc.nCells;
             cur.keyRange = sc.keyRange;
+            cur.ovBuckets = sc.ovBuckets;
             cur.morton = sc.morton;
             const GridSave& g = c.grids[i];
             if (!(cur == g)) {
@@ -1028,6 +1072,7 @@ class Engine {
                 std::copy(g.gn, g.gn + 3, sc.gn);
                 sc.nCells = g.nCells;
                 sc.keyRange = g.keyRange;
+                sc.ovBuckets = g.ovBuckets;
                 sc.morton = g.morton;
                 layout = true;
             }
@@ -1040,24 +1085,40 @@ class Engine {
         copyD(id.p, c.id.p, NP * sizeof(uint32_t));
         std::copy(c.state.begin(), c.state.end(), hState);
         HIPCHK(hipMemcpyAsync(dState.p, hState, scenes.size() * sizeof(SceneState), hipMemcpyHostToDevice, stream));
+        if (method == SPH_DFSPH) {
+            // The slots are in the order of the keys of x (the last search
+            // sorted them), so a fresh search keeps that order (stable sort)
+            // and rebuilds the same lists; rho and alpha follow from them.
+            copyD(kws.p, c.kws.p, NP * sizeof(float));
+            copyD(kvws.p, c.kvws.p, NP * sizeof(float));
+            neighborSearch(false);
+            HIPCHK(launchDensityAlpha(view(), 1, stream));
+        }
         HIPCHK(hipStreamSynchronize(stream));
     }
-    void tapeAlloc(int steps) {
-        const size_t k = static_cast<size_t>(steps), np = NP;
+    void tapeAlloc(int steps, int nit, int halves) {
+        const size_t k = static_cast<size_t>(steps), np = NP, it = static_cast<size_t>(nit);
         tape.steps = steps;
+        tape.nit = nit;
+        tape.halves = halves;
         tape.np = np;
         tape.capF = capF;
         tape.capB = capB;
-        tape.x.alloc(k * np);
-        tape.v.alloc(k * np);
-        tape.n.alloc(k * np);
-        tape.rho.alloc(k * np);
-        tape.p.alloc(k * np);
-        tape.id.alloc(k * np);
-        tape.cntF.alloc(k * np);
-        tape.cntB.alloc(k * np);
-        tape.nbrF.alloc(k * capF * np);
-        tape.nbrB.alloc(k * capB * np);
+        for (int hf = 0; hf < 2; ++hf) {
+            HalfTape& t = tape.h[hf];
+            if (hf >= halves) {
+                t = HalfTape{};
+                continue;
+            }
+            for (auto* b : {&t.x, &t.v, &t.n}) b->alloc(k * np);
+            for (auto* b : {&t.rho, &t.p, &t.alpha}) b->alloc(k * np);
+            for (auto* b : {&t.id, &t.cntF, &t.cntB}) b->alloc(k * np);
+            t.nbrF.alloc(k * capF * np);
+            t.nbrB.alloc(k * capB * np);
+            t.u.alloc(k * std::max<size_t>(it, 1) * np);
+            t.k.alloc(k * (it + 1) * np);
+            t.aux.alloc(k * (it + 1) * np);
+        }
     }
     // The list capacity grew while a segment was recorded: widen the tape's
     // list blocks, keeping the steps already recorded (entry s of particle i
@@ -1073,11 +1134,32 @@ class Engine {
             buf = std::move(w);
         };
         HIPCHK(hipStreamSynchronize(stream));
-        widen(tape.nbrF, tape.capF, capF);
-        widen(tape.nbrB, tape.capB, capB);
+        for (int hf = 0; hf < tape.halves; ++hf) {
+            widen(tape.h[hf].nbrF, tape.capF, capF);
+            widen(tape.h[hf].nbrB, tape.capB, capB);
+        }
         HIPCHK(hipStreamSynchronize(stream));
         tape.capF = std::max(tape.capF, capF);
         tape.capB = std::max(tape.capB, capB);
+    }
+    // x, ids and lists of the current search into a half of step `slot`
+    void recordSearch(HalfTape& t, int slot) {
+        const size_t np = NP, o = static_cast<size_t>(slot) * np;
+        copyD(t.x.p + o, x.p, np * sizeof(float4));
+        copyD(t.id.p + o, id.p, np * sizeof(uint32_t));
+        copyD(t.cntF.p + o, cntF.p, np * sizeof(uint32_t));
+        copyD(t.cntB.p + o, cntB.p, np * sizeof(uint32_t));
+        copyD(t.nbrF.p + static_cast<size_t>(slot) * tape.capF * np, nbrF.p, capF * np * sizeof(uint32_t));
+        copyD(t.nbrB.p + static_cast<size_t>(slot) * tape.capB * np, nbrB.p, capB * np * sizeof(uint32_t));
+    }
+    // k and rho* / divergence of a DFSPH pass (iteration `it`, warm = nit)
+    void recordPass(HalfTape& t, int slot, int it) {
+        const size_t np = NP, o = (static_cast<size_t>(slot) * (tape.nit + 1) + it) * np;
+        copyD(t.k.p + o, k.p, np * sizeof(float));
+        copyD(t.aux.p + o, aux.p, np * sizeof(float));
+    }
+    void recordVelocity(HalfTape& t, int slot, int it) {
+        copyD(t.u.p + (static_cast<size_t>(slot) * tape.nit + it) * NP, v.p, NP * sizeof(float4));
     }
     // stepWCSPH with the tape of step `slot` of the current segment.
     void stepWCSPHRecord(int slot) {
@@ -1085,35 +1167,118 @@ class Engine {
         neighborSearch(false);
         if (capF > tape.capF || capB > tape.capB) tapeGrow(slot);
         DevView d = view();
+        HalfTape& t = tape.h[0];
         const size_t np = NP, o = static_cast<size_t>(slot) * np;
-        copyD(tape.x.p + o, x.p, np * sizeof(float4));
-        copyD(tape.v.p + o, v.p, np * sizeof(float4));
-        copyD(tape.id.p + o, id.p, np * sizeof(uint32_t));
+        recordSearch(t, slot);
+        copyD(t.v.p + o, v.p, np * sizeof(float4));
         timed(K_DENSITY, [&] { return launchDensityPressure(d, stream); });
         timed(K_FORCES, [&] { return launchForces(d, 1, stream); });
         if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
-        copyD(tape.rho.p + o, rho.p, np * sizeof(float));
-        copyD(tape.p.p + o, p.p, np * sizeof(float));
-        if (anyST()) copyD(tape.n.p + o, n.p, np * sizeof(float4));
-        copyD(tape.cntF.p + o, cntF.p, np * sizeof(uint32_t));
-        copyD(tape.cntB.p + o, cntB.p, np * sizeof(uint32_t));
-        copyD(tape.nbrF.p + static_cast<size_t>(slot) * tape.capF * np, nbrF.p, capF * np * sizeof(uint32_t));
-        copyD(tape.nbrB.p + static_cast<size_t>(slot) * tape.capB * np, nbrB.p, capB * np * sizeof(uint32_t));
+        copyD(t.rho.p + o, rho.p, np * sizeof(float));
+        copyD(t.p.p + o, p.p, np * sizeof(float));
+        if (anyST()) copyD(t.n.p + o, n.p, np * sizeof(float4));
         timed(K_INTEGRATE, [&] { return launchIntegrate(d, stream); });
         for (size_t s = 0; s < scenes.size(); ++s) hState[s].t = hState[s].t + hState[s].dt;
         fileStats(false);
     }
-    void checkDifferentiable() const {
+    // stepDFSPH with a fixed iteration count (no loop control), recording both
+    // halves of step `slot`; the launches and their order are those of
+    // stepDFSPH, so the trajectory is the same bit for bit.
+    void stepDFSPHRecord(int slot) {
+        beginStats();
+        const int nit = tape.nit;
+        const size_t np = NP, o = static_cast<size_t>(slot) * np;
+        if (capF > tape.capF || capB > tape.capB) tapeGrow(slot);
+        HalfTape& A = tape.h[0];
+        HalfTape& B = tape.h[1];
+        DevView d = view();
+        recordSearch(A, slot);
+        copyD(A.v.p + o, v.p, np * sizeof(float4));
+        copyD(A.rho.p + o, rho.p, np * sizeof(float));
+        copyD(A.alpha.p + o, alpha.p, np * sizeof(float));
+        if (anyVisc() || anyST()) timed(K_FORCES, [&] { return launchForces(d, 0, stream); });
+        if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 1, stream); });
+        else timed(K_VMAX, [&] { return launchVmax(d, stream); });
+        finish(d, kFinishTimeStep, 2);
+        if (anyST()) copyD(A.n.p + o, n.p, np * sizeof(float4));
+        timed(K_PREDICT, [&] { return launchPredictVelocity(d, stream); });
+        timed(K_RHO_STAR, [&] { return launchRhoStar(d, 1, 2, stream); });
+        recordPass(A, slot, nit);
+        timed(K_VELOCITY, [&] { return launchVelocity(d, 2, stream); });
+        for (int it = 0; it < nit; ++it) {
+            recordVelocity(A, slot, it);
+            timed(K_RHO_STAR, [&] { return launchRhoStar(d, 0, it & 1, stream); });
+            finish(d, kFinishDensity, it & 1);
+            recordPass(A, slot, it);
+            timed(K_VELOCITY, [&] { return launchVelocity(d, it & 1, stream); });
+        }
+        readState();
+        for (size_t s = 0; s < scenes.size(); ++s) scenes[s].its = hState[s].it;
+        for (int attempt = 0;; ++attempt) {
+            searchOnce(attempt == 0);
+            readState();
+            if (searchValid()) break;
+            if (attempt >= 5) throw SphError{SPH_E_OUT_OF_GRID, "could not place particles into a grid"};
+        }
+        if (capF > tape.capF || capB > tape.capB) tapeGrow(slot + 1);
+        d = view();
+        recordSearch(B, slot);
+        timed(K_DENSITY, [&] { return launchDensityAlpha(d, 1, stream); });
+        copyD(B.rho.p + o, rho.p, np * sizeof(float));
+        copyD(B.alpha.p + o, alpha.p, np * sizeof(float));
+        timed(K_DIVERGENCE, [&] { return launchDivergence(d, 1, 2, stream); });
+        recordPass(B, slot, nit);
+        timed(K_VELOCITY, [&] { return launchVelocity(d, 2, stream); });
+        for (int it = 0; it < nit; ++it) {
+            recordVelocity(B, slot, it);
+            timed(K_DIVERGENCE, [&] { return launchDivergence(d, 0, it & 1, stream); });
+            finish(d, kFinishDivergence, it & 1);
+            recordPass(B, slot, it);
+            timed(K_VELOCITY, [&] { return launchVelocity(d, it & 1, stream); });
+        }
+        readState();
+        for (size_t s = 0; s < scenes.size(); ++s) scenes[s].itsV = hState[s].it;
+        timed(K_SCALE, [&] { return launchScaleDivergence(view(), stream); });
+        for (size_t s = 0; s < scenes.size(); ++s) {
+            SceneState& T = hState[s];
+            const SceneConst_host c = hostConst(s);
+            T.itsV = T.it;
+            T.errV = T.err;
+            if (T.it >= c.maxIt && T.err > static_cast<double>(c.etaV / T.dt)) T.capped |= 2;
+            T.t = T.t + T.dt;
+        }
+        fileStats(true);
+    }
+    void stepRecord(int slot) {
+        if (method == SPH_DFSPH) stepDFSPHRecord(slot);
+        else stepWCSPHRecord(slot);
+    }
+    // Iterations per Jacobi solve of a recorded DFSPH rollout (0 for WCSPH).
+    int checkDifferentiable() const {
         if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
-        if (method != SPH_WCSPH) throw SphError{SPH_E_UNSUPPORTED, "gradients are available for WCSPH scenes only"};
+        if (method == SPH_PCISPH) throw SphError{SPH_E_UNSUPPORTED, "gradients are available for WCSPH and DFSPH"};
         if (interleaved) throw SphError{SPH_E_UNSUPPORTED, "gradients need separate velocity storage"};
         if (!NP) throw SphError{SPH_E_STATE, "no fluid particles"};
-        for (auto& sc : scenes)
+        int nit = 0;
+        for (auto& sc : scenes) {
             if (sc.viscOn && sc.P.viscosity_method != 2)
                 throw SphError{SPH_E_UNSUPPORTED, "gradients support XSPH viscosity (method 2) only"};
+            if (method != SPH_DFSPH) continue;
+            // DFSPH: both solves run exactly maxIterations (the divergence loop
+            // is bounded by maxIterations, DFSPHSolver.cpp:746) and dt is fixed
+            const int n = static_cast<int>(sc.P.max_iterations);
+            if (n < 1 || sc.P.min_iterations != sc.P.max_iterations || sc.P.min_iterations_v != sc.P.max_iterations)
+                throw SphError{SPH_E_UNSUPPORTED,
+                               "DFSPH gradients need min_iterations == min_iterations_v == max_iterations >= 1"};
+            if (nit && n != nit) throw SphError{SPH_E_UNSUPPORTED, "DFSPH gradients need one iteration count per context"};
+            nit = n;
+            if (sc.P.adaptive_time_step && !(sc.minTs == sc.maxTs && sc.maxTs == sc.P.time_step))
+                throw SphError{SPH_E_UNSUPPORTED, "DFSPH gradients need a fixed time step (min = max = initial)"};
+        }
+        return nit;
     }
     void forwardRecord(int nsteps, int every, sph_step_stats* stats) {
-        checkDifferentiable();
+        const int nit = checkDifferentiable();
         if (nsteps < 1 || every < 1) throw SphError{SPH_E_ARG, "nsteps and checkpoint_every must be >= 1"};
         HIPCHK(hipSetDevice(device));
         every = std::min(every, nsteps);
@@ -1123,13 +1288,16 @@ class Engine {
         rec.every = every;
         rec.segments = (nsteps + every - 1) / every;
         rec.ckpt.resize(static_cast<size_t>(rec.segments));
-        if (tape.steps != every || tape.np != NP || tape.capF < capF || tape.capB < capB) tapeAlloc(every);
+        const int halves = method == SPH_DFSPH ? 2 : 1;
+        if (tape.steps != every || tape.np != NP || tape.nit != nit || tape.halves != halves || tape.capF < capF ||
+            tape.capB < capB)
+            tapeAlloc(every, nit, halves);
         launchesInStep = 0;
         syncsInStep = 0;
         HIPCHK(hipEventRecord(evStart, stream));
         for (int st = 0; st < nsteps; ++st) {
             if (st % every == 0) saveCheckpoint(rec.ckpt[static_cast<size_t>(st / every)]);
-            stepWCSPHRecord(st % every);
+            stepRecord(st % every);
             if (stats)
                 for (size_t s = 0; s < scenes.size(); ++s) stats[st * scenes.size() + s] = lastStats[s];
         }
@@ -1154,7 +1322,7 @@ class Engine {
         if (gx) std::copy(gx, gx + 3 * sc.nf, seedX[si].begin());
         if (gv) std::copy(gv, gv + 3 * sc.nf, seedV[si].begin());
     }
-    AdjView adjView(int slot) {
+    AdjView adjView(int slot, int half) {
         AdjView a{};
         a.scenes = dScenes.p;
         a.state = dState.p;
@@ -1162,17 +1330,19 @@ class Engine {
         a.numScenes = static_cast<int32_t>(scenes.size());
         a.numBlocks = numBlocks;
         a.NP = static_cast<uint32_t>(NP);
+        const HalfTape& t = tape.h[half];
         const size_t o = static_cast<size_t>(slot) * NP;
-        a.x = tape.x.p + o;
-        a.v = tape.v.p + o;
-        a.rho = tape.rho.p + o;
-        a.p = tape.p.p + o;
-        a.n = tape.n.p + o;
-        a.id = tape.id.p + o;
-        a.cntF = tape.cntF.p + o;
-        a.cntB = tape.cntB.p + o;
-        a.nbrF = tape.nbrF.p + static_cast<size_t>(slot) * tape.capF * NP;
-        a.nbrB = tape.nbrB.p + static_cast<size_t>(slot) * tape.capB * NP;
+        a.x = t.x.p + o;
+        a.v = t.v.p + o;
+        a.rho = t.rho.p + o;
+        a.p = t.p.p + o;
+        a.n = t.n.p + o;
+        a.alpha = t.alpha.p + o;
+        a.id = t.id.p + o;
+        a.cntF = t.cntF.p + o;
+        a.cntB = t.cntB.p + o;
+        a.nbrF = t.nbrF.p + static_cast<size_t>(slot) * tape.capF * NP;
+        a.nbrB = t.nbrB.p + static_cast<size_t>(slot) * tape.capB * NP;
         a.capF = tape.capF;
         a.capB = tape.capB;
         a.xb = xb.p;
@@ -1183,25 +1353,75 @@ class Engine {
         a.vBar = vBar.p;
         a.nBar = nBar.p;
         a.rhoBar = rhoBar.p;
+        a.alphaBar = alphaBar.p;
+        a.kBar = kBar.p;
+        a.sBar = sBar.p;
+        a.sgBar = sgBar.p;
+        a.adjKws = adjKws.p;
+        a.adjKvws = adjKvws.p;
+        a.adjRho = adjRho.p;
+        a.adjAlpha = adjAlpha.p;
         a.partials = adjPartials.p;
         a.acc = adjAcc.p;
         return a;
     }
+    // point a DFSPH view at iteration `it` of its half (warm start: it = nit)
+    void adjPass(AdjView& a, int slot, int half, int it) {
+        const HalfTape& t = tape.h[half];
+        const size_t ko = (static_cast<size_t>(slot) * (tape.nit + 1) + it) * NP;
+        a.kk = t.k.p + ko;
+        a.aux = t.aux.p + ko;
+        a.u = it < tape.nit ? t.u.p + (static_cast<size_t>(slot) * tape.nit + it) * NP : nullptr;
+    }
     void adjointStep(int slot) {
-        const AdjView a = adjView(slot);
+        if (method == SPH_DFSPH) return adjointStepDFSPH(slot);
+        const AdjView a = adjView(slot, 0);
         timed(K_ADJ_SEED, [&] { return launchAdjSeed(a, stream); });
         timed(K_ADJ_FORCES, [&] { return launchAdjForces(a, stream); });
         if (anyST()) timed(K_ADJ_NORMALS, [&] { return launchAdjNormals(a, stream); });
         timed(K_ADJ_DENSITY, [&] { return launchAdjDensity(a, stream); });
         timed(K_ADJ_FINISH, [&] { return launchAdjFinish(a, stream); });
     }
+    // Reverse of one DFSPH step: the second half (divergence solve, densities
+    // and alpha at x_{n+1}), then the first half (density solve, prediction,
+    // forces) at x_n.
+    void adjointStepDFSPH(int slot) {
+        const int nit = tape.nit;
+        AdjView a = adjView(slot, 1);
+        timed(K_ADJ_SEED, [&] { return launchAdjHalfStart(a, 1, stream); });
+        for (int it = nit - 1; it >= 0; --it) {
+            adjPass(a, slot, 1, it);
+            timed(K_ADJ_VELOCITY, [&] { return launchAdjVelocity(a, kAdjDivergenceLoop, stream); });
+            timed(K_ADJ_SOURCE, [&] { return launchAdjSource(a, kAdjDivergenceLoop, stream); });
+        }
+        adjPass(a, slot, 1, nit);
+        timed(K_ADJ_VELOCITY, [&] { return launchAdjVelocity(a, kAdjDivergenceWarm, stream); });
+        timed(K_ADJ_DENSITY, [&] { return launchAdjAlphaA(a, stream); });
+        timed(K_ADJ_DENSITY, [&] { return launchAdjAlphaB(a, stream); });
+        timed(K_ADJ_SEED, [&] { return launchAdjHalfEnd(a, 1, stream); });
+        a = adjView(slot, 0);
+        timed(K_ADJ_SEED, [&] { return launchAdjHalfStart(a, 0, stream); });
+        for (int it = nit - 1; it >= 0; --it) {
+            adjPass(a, slot, 0, it);
+            timed(K_ADJ_VELOCITY, [&] { return launchAdjVelocity(a, kAdjDensityLoop, stream); });
+            timed(K_ADJ_SOURCE, [&] { return launchAdjSource(a, kAdjDensityLoop, stream); });
+        }
+        adjPass(a, slot, 0, nit);
+        timed(K_ADJ_VELOCITY, [&] { return launchAdjVelocity(a, kAdjDensityWarm, stream); });
+        timed(K_ADJ_SEED, [&] { return launchAdjPredict(a, stream); });
+        if (anyVisc() || anyST()) timed(K_ADJ_FORCES, [&] { return launchAdjForces2(a, 0, stream); });
+        if (anyST()) timed(K_ADJ_NORMALS, [&] { return launchAdjNormals(a, stream); });
+        timed(K_ADJ_FINISH, [&] { return launchAdjFinish(a, stream); });
+        timed(K_ADJ_SEED, [&] { return launchAdjHalfEnd(a, 0, stream); });
+    }
     void adjointRun() {
         if (!rec.valid || rec.epoch != stateEpoch) throw SphError{SPH_E_STATE, "no recorded rollout for this state"};
         checkDifferentiable();
         HIPCHK(hipSetDevice(device));
-        for (auto* b : {&adjX, &adjV, &aBar, &xBar, &vBar, &nBar})
+        for (auto* b : {&adjX, &adjV, &aBar, &xBar, &vBar, &nBar, &sgBar})
             if (b->n != NP) b->alloc(NP);
-        if (rhoBar.n != NP) rhoBar.alloc(NP);
+        for (auto* b : {&rhoBar, &alphaBar, &kBar, &sBar, &adjKws, &adjKvws, &adjRho, &adjAlpha})
+            if (b->n != NP) b->alloc(NP);
         adjPartials.alloc(2 * static_cast<size_t>(numBlocks ? numBlocks : 1));
         adjAcc.alloc(2 * scenes.size());
         std::vector<float4> hx(NP, make_float4(0, 0, 0, 0)), hv(NP, make_float4(0, 0, 0, 0));
@@ -1215,16 +1435,25 @@ class Engine {
         HIPCHK(hipMemcpy(adjX.p, hx.data(), NP * sizeof(float4), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(adjV.p, hv.data(), NP * sizeof(float4), hipMemcpyHostToDevice));
         HIPCHK(hipMemset(adjAcc.p, 0, 2 * scenes.size() * sizeof(double)));
+        for (auto* b : {&adjKws, &adjKvws, &adjRho, &adjAlpha}) HIPCHK(hipMemset(b->p, 0, NP * sizeof(float)));
         HIPCHK(hipEventRecord(evStart, stream));
         launchesInStep = 0;
         for (int seg = rec.segments - 1; seg >= 0; --seg) {
             const int first = seg * rec.every, count = std::min(rec.every, rec.nsteps - first);
             if (rec.tapeSeg != seg) {
                 restoreCheckpoint(rec.ckpt[static_cast<size_t>(seg)]);
-                for (int j = 0; j < count; ++j) stepWCSPHRecord(j);
+                for (int j = 0; j < count; ++j) stepRecord(j);
                 rec.tapeSeg = seg;
             }
             for (int j = count - 1; j >= 0; --j) adjointStep(j);
+        }
+        if (method == SPH_DFSPH) {
+            // rho and alpha of the first step came from the initial positions
+            const AdjView a = adjView(0, 0);
+            timed(K_ADJ_SEED, [&] { return launchAdjHalfStart(a, 2, stream); });
+            timed(K_ADJ_DENSITY, [&] { return launchAdjAlphaA(a, stream); });
+            timed(K_ADJ_DENSITY, [&] { return launchAdjAlphaB(a, stream); });
+            timed(K_ADJ_SEED, [&] { return launchAdjHalfEnd(a, 2, stream); });
         }
         HIPCHK(hipEventRecord(evStop, stream));
         HIPCHK(hipEventSynchronize(evStop));
@@ -1250,6 +1479,35 @@ class Engine {
         // leave the context where the recorded rollout ended
         restoreCheckpoint(rec.last);
         gradValid = true;
+    }
+    // Values the recorded rollout held at one step (original order): see
+    // sph_recorded_field in include/sph2o.h.
+    void recordedField(int si, int step, int field, int pass, float* out) {
+        SceneH& sc = scene(si);
+        if (!rec.valid || rec.epoch != stateEpoch) throw SphError{SPH_E_STATE, "no recorded rollout for this state"};
+        if (step < 0 || step >= rec.nsteps) throw SphError{SPH_E_ARG, "step outside the recorded rollout"};
+        if (step / rec.every != rec.tapeSeg)
+            throw SphError{SPH_E_STATE, "step not on the tape (record with checkpoint_every >= nsteps)"};
+        if (field < 0 || field > 5) throw SphError{SPH_E_ARG, "unknown field"};
+        const int half = (field == 2 || field == 3) ? 1 : 0;
+        if (half >= tape.halves || (field <= 3 && method != SPH_DFSPH) || (field == 5 && method != SPH_WCSPH))
+            throw SphError{SPH_E_ARG, "field not recorded for this method"};
+        if (field <= 3 && (pass < 0 || pass > tape.nit)) throw SphError{SPH_E_ARG, "pass outside 0..iterations"};
+        const HalfTape& t = tape.h[half];
+        const size_t slot = static_cast<size_t>(step % rec.every);
+        auto ids = download(t.id.p + slot * NP + sc.fBegin, sc.nf);
+        if (field == 4) {
+            auto hx = download(t.x.p + slot * NP + sc.fBegin, sc.nf);
+            for (uint32_t i = 0; i < sc.nf; ++i) {
+                out[3 * ids[i]] = hx[i].x; out[3 * ids[i] + 1] = hx[i].y; out[3 * ids[i] + 2] = hx[i].z;
+            }
+            return;
+        }
+        const float* src = field == 5 ? t.p.p + slot * NP
+                                      : ((field == 0 || field == 2) ? t.k.p : t.aux.p) +
+                                            (slot * (tape.nit + 1) + static_cast<size_t>(pass)) * NP;
+        auto hv = download(src + sc.fBegin, sc.nf);
+        for (uint32_t i = 0; i < sc.nf; ++i) out[ids[i]] = hv[i];
     }
     void adjointResult(int si, float* gx0, float* gv0, double* gnu, double* ggam) {
         scene(si);
@@ -1360,25 +1618,21 @@ class Engine {
             return;
         }
         // The scene restarts from the original particle order (slot fBegin + o
-        // holds particle o), carrying the DFSPH warm-start values along: the
-        // next step then sorts the same input whatever order earlier steps
-        // left (the radix sort is stable within a cell), so a state set twice
-        // steps bit-identically.
+        // holds particle o) and without DFSPH warm-start history (k_ws = kv_ws
+        // = 0, as after sph_init): the next step then sorts the same input
+        // whatever earlier steps left (the radix sort is stable within a cell),
+        // so a state set twice steps bit-identically.
         auto ids = download(id.p + s.fBegin, s.nf);
         auto hx = download(x.p + s.fBegin, s.nf);
         auto hv = downloadVel(s.fBegin, s.nf);
-        auto kw = download(kws.p + s.fBegin, s.nf);
-        auto kv = download(kvws.p + s.fBegin, s.nf);
         std::vector<float4> ox(s.nf), ov(s.nf);
-        std::vector<float> okw(s.nf), okv(s.nf);
+        std::vector<float> okw(s.nf, 0.0f), okv(s.nf, 0.0f);
         std::vector<uint32_t> oid(s.nf);
         for (uint32_t i = 0; i < s.nf; ++i) {
             const uint32_t o = ids[i];
             if (o >= s.nf) throw SphError{SPH_E_STATE, "corrupt particle ids"};
             ox[o] = pos ? make_float4(pos[3 * o], pos[3 * o + 1], pos[3 * o + 2], 0.0f) : hx[i];
             ov[o] = vel ? make_float4(vel[3 * o], vel[3 * o + 1], vel[3 * o + 2], 0.0f) : hv[i];
-            okw[o] = kw[i];
-            okv[o] = kv[i];
             oid[o] = o;
         }
         hx.swap(ox);
@@ -1710,6 +1964,14 @@ int sph_adjoint_result(sph_ctx* ctx, int scene, float* dL_dx0, float* dL_dv0, do
                        double* dL_dsurface_tension) {
     return guarded(ctx, [&] {
         ctx->eng.adjointResult(scene, dL_dx0, dL_dv0, dL_dviscosity, dL_dsurface_tension);
+        return SPH_OK;
+    });
+}
+
+int sph_recorded_field(sph_ctx* ctx, int scene, int step, int field, int pass, float* out) {
+    return guarded(ctx, [&] {
+        if (!out) throw SphError{SPH_E_ARG, "out is NULL"};
+        ctx->eng.recordedField(scene, step, field, pass, out);
         return SPH_OK;
     });
 }

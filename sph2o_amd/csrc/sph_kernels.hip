@@ -97,6 +97,44 @@ __device__ __forceinline__ uint32_t cellKey(const SceneConst& S, int cx, int cy,
     return S.keyBase + cellLinear(S, cx, cy, cz);
 }
 
+// Global cell floor(x * fl(1/h)) of a position. Returns 0, or 1 for a
+// non-finite coordinate, 2 for one beyond +-1e8 cells (the cell indices and
+// their differences must stay exact in int32).
+__device__ __forceinline__ int globalCell(const SceneConst& S, float4 p, int& gx, int& gy, int& gz) {
+    gx = gy = gz = 0;
+    if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) return 1;
+    const float ax = __fmul_rn(p.x, S.invh), ay = __fmul_rn(p.y, S.invh), az = __fmul_rn(p.z, S.invh);
+    if (fabsf(ax) > 1.0e8f || fabsf(ay) > 1.0e8f || fabsf(az) > 1.0e8f) return 2;
+    gx = static_cast<int>(floorf(ax));
+    gy = static_cast<int>(floorf(ay));
+    gz = static_cast<int>(floorf(az));
+    return 0;
+}
+// Overflow bucket of a cell outside the dense box (the reference's unbounded
+// hash grid, HashTable.h:13-16, has no box; these buckets give the same sets).
+__device__ __forceinline__ uint32_t bucketOf(const SceneConst& S, int gx, int gy, int gz) {
+    uint32_t h = static_cast<uint32_t>(gx) * 0x9E3779B1u;
+    h ^= static_cast<uint32_t>(gy) * 0x85EBCA77u + (h << 6) + (h >> 2);
+    h ^= static_cast<uint32_t>(gz) * 0xC2B2AE3Du + (h << 6) + (h >> 2);
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    return h & S.ovMask;
+}
+// Where the particles of global cell g lie: a dense cell (bucket = false) or
+// the overflow bucket it hashes to, which other cells may share (scans then
+// keep only candidates whose own cell is g).
+__device__ __forceinline__ uint32_t cellSlot(const SceneConst& S, int gx, int gy, int gz, bool& bucket) {
+    const int lx = gx - S.g0x, ly = gy - S.g0y, lz = gz - S.g0z;
+    bucket = !(lx >= 0 && ly >= 0 && lz >= 0 && lx < S.gnx && ly < S.gny && lz < S.gnz);
+    return bucket ? S.cellBase + S.nCells + bucketOf(S, gx, gy, gz) : S.cellBase + cellLinear(S, lx, ly, lz);
+}
+__device__ __forceinline__ bool inCell(const SceneConst& S, float4 p, int gx, int gy, int gz) {
+    return static_cast<int>(floorf(__fmul_rn(p.x, S.invh))) == gx &&
+           static_cast<int>(floorf(__fmul_rn(p.y, S.invh))) == gy &&
+           static_cast<int>(floorf(__fmul_rn(p.z, S.invh))) == gz;
+}
+
 // slot: which iteration flag of the scene state gates this launch (0/1 inside a
 // Jacobi loop, by iteration parity; 2 outside a loop, always set).
 #define SCENE_PROLOGUE(slot)                                  \
@@ -178,9 +216,17 @@ __global__ void __launch_bounds__(kBlock) kKeys(DevView d, int dfsphAdvance, uin
         d.x[i] = p;
         d.kws[i] *= (T.it > 0 ? T.dt2 : 0.0f);
     }
-    int cx, cy, cz;
-    if (!localCell(S, p, cx, cy, cz)) atomicOr(&d.flags[sIdx * kFlagSlots + kFlagOutOfGrid], 1);
-    keys[i] = cellKey(S, cx, cy, cz);
+    int gx, gy, gz;
+    const int bad = globalCell(S, p, gx, gy, gz);
+    if (bad) atomicOr(&d.flags[sIdx * kFlagSlots + (bad == 1 ? kFlagNonFinite : kFlagFar)], 1);
+    bool bucket;
+    const uint32_t c = cellSlot(S, gx, gy, gz, bucket);
+    keys[i] = bucket ? S.deadKey - (S.ovMask + 1) + (c - S.cellBase - S.nCells)
+                     : cellKey(S, gx - S.g0x, gy - S.g0y, gz - S.g0z);
+    // particles in overflow buckets, one atomic per wave
+    const unsigned long long outside = __ballot(bucket);
+    if ((threadIdx.x & 63) == 0 && outside)
+        atomicAdd(&d.flags[sIdx * kFlagSlots + kFlagOutOfGrid], static_cast<int>(__popcll(outside)));
 }
 
 __global__ void kFillRanges(uint2* r, size_t n) {
@@ -210,9 +256,10 @@ __global__ void __launch_bounds__(kBlock) kReorder(DevView d, const uint32_t* ke
     idAlt[i] = d.id[o];
     if (i >= S.fEnd) return;
     const uint32_t key = keysSorted[i];
-    int cx, cy, cz;
-    localCell(S, p, cx, cy, cz);
-    const uint32_t c = S.cellBase + cellLinear(S, cx, cy, cz);
+    int gx, gy, gz;
+    globalCell(S, p, gx, gy, gz);
+    bool bucket;
+    const uint32_t c = cellSlot(S, gx, gy, gz, bucket);
     if (i == S.fBegin || keysSorted[i - 1] != key) d.fRange[c].x = i;
     if (i + 1 == S.fEnd || keysSorted[i + 1] != key) d.fRange[c].y = i + 1;
 }
@@ -226,27 +273,24 @@ __global__ void __launch_bounds__(kBlock) kBuildNeighbors(DevView d) {
     if (valid) {
         const float4 xi = d.x[i];
         int cx, cy, cz;
-        localCell(S, xi, cx, cy, cz);
+        globalCell(S, xi, cx, cy, cz);
         const size_t NP = d.NP;
-        for (int dx = -1; dx <= 1; ++dx) {
-            const int ax = cx + dx;
-            if (ax < 0 || ax >= S.gnx) continue;
-            for (int dy = -1; dy <= 1; ++dy) {
-                const int ay = cy + dy;
-                if (ay < 0 || ay >= S.gny) continue;
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
                 for (int dz = -1; dz <= 1; ++dz) {
-                    const int az = cz + dz;
-                    if (az < 0 || az >= S.gnz) continue;
-                    const uint32_t c = S.cellBase + cellLinear(S, ax, ay, az);
+                    const int ax = cx + dx, ay = cy + dy, az = cz + dz;
+                    bool bucket;
+                    const uint32_t c = cellSlot(S, ax, ay, az, bucket);
                     const uint2 fr = d.fRange[c];
                     for (uint32_t j = fr.x; j < fr.y; ++j) {
                         const float4 xj = d.x[j];
+                        if (bucket && !inCell(S, xj, ax, ay, az)) continue;
                         if (dRef(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) <= S.dmax) {
                             if (nf < d.capF) d.nbrF[nf * NP + i] = j;
                             ++nf;
                         }
                     }
-                    if (S.akinci) {
+                    if (S.akinci && !bucket) {   // boundary particles all lie in the dense box
                         const uint2 br = d.bRange[c];
                         for (uint32_t b = br.x; b < br.y; ++b) {
                             const float4 xb = d.xb[b];
@@ -257,8 +301,6 @@ __global__ void __launch_bounds__(kBlock) kBuildNeighbors(DevView d) {
                         }
                     }
                 }
-            }
-        }
         d.cntF[i] = nf;
         d.cntB[i] = nb;
     }
@@ -793,25 +835,22 @@ __global__ void __launch_bounds__(kBlock) kPciDensity(DevView d, int slot) {
     if (valid) {
         const float4 xsi = d.xs[i];
         int cx, cy, cz;
-        localCell(S, d.x[i], cx, cy, cz);
+        globalCell(S, d.x[i], cx, cy, cz);
         float rho = 0.0f;
-        for (int dx = -1; dx <= 1; ++dx) {
-            const int ax = cx + dx;
-            if (ax < 0 || ax >= S.gnx) continue;
-            for (int dy = -1; dy <= 1; ++dy) {
-                const int ay = cy + dy;
-                if (ay < 0 || ay >= S.gny) continue;
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
                 for (int dz = -1; dz <= 1; ++dz) {
-                    const int az = cz + dz;
-                    if (az < 0 || az >= S.gnz) continue;
-                    const uint32_t c = S.cellBase + cellLinear(S, ax, ay, az);
+                    const int ax = cx + dx, ay = cy + dy, az = cz + dz;
+                    bool bucket;
+                    const uint32_t c = cellSlot(S, ax, ay, az, bucket);
                     const uint2 fr = d.fRange[c];
                     for (uint32_t j = fr.x; j < fr.y; ++j) {
+                        if (bucket && !inCell(S, d.x[j], ax, ay, az)) continue;   // cells of current positions
                         const float4 xj = d.xs[j];
                         const float rx = xsi.x - xj.x, ry = xsi.y - xj.y, rz = xsi.z - xj.z;
                         rho += S.m * cubicW(S, fastSqrt(rx * rx + ry * ry + rz * rz));
                     }
-                    if (S.akinci) {
+                    if (S.akinci && !bucket) {
                         const uint2 br = d.bRange[c];
                         for (uint32_t b = br.x; b < br.y; ++b) {
                             const float4 xb = d.xb[b];
@@ -820,8 +859,6 @@ __global__ void __launch_bounds__(kBlock) kPciDensity(DevView d, int slot) {
                         }
                     }
                 }
-            }
-        }
         d.rho[i] = rho;
         const float e = fmaxf(0.0f, rho - S.rho0);
         d.p[i] += S.pciDelta * e;

@@ -182,6 +182,16 @@ class Context:
             out[sc] = dict(x0=gx0, v0=gv0, viscosity=gnu.value, surface_tension=gst.value)
         return out
 
+    RECORDED = {"k_density": 0, "rho_star": 1, "k_divergence": 2, "divergence": 3, "x": 4, "pressure": 5}
+
+    def recorded(self, name, step, scene=0, pass_=0):
+        """A field of the recorded rollout at `step` (see sph_recorded_field)."""
+        fid = self.RECORDED[name]
+        n = self.num_fluid(scene)
+        out = np.zeros((n, 3) if fid == 4 else n, np.float32)
+        self._check(self.L.sph_recorded_field(self.h, scene, int(step), fid, int(pass_), _fp(out)))
+        return out
+
     # -- timing
     def last_step_ms(self):
         ms = ctypes.c_double()

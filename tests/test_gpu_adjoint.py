@@ -1,7 +1,8 @@
 """Gradients of WCSPH rollouts (sph_forward_record / sph_adjoint_*) on the GPU,
 checked against FP64 central finite differences of the oracle.
 
-The loss is L = mean_i |x_i(T) - y_i|^2 with y = x(0) + N(0, (0.5 s)^2); the
+WCSPH and DFSPH (8 + 8 Jacobi iterations, fixed dt). The loss is
+L = mean_i |x_i(T) - y_i|^2 with y = x(0) + N(0, (0.5 s)^2); the
 checked gradients are dL/dv0 and dL/dx0 (random unit directions), and
 dL/d(viscosity), dL/d(surface tension). Central differences in FP64 on the
 oracle with eps = 1e-6 (v0, coefficients) and 1e-8 (x0).
@@ -17,10 +18,15 @@ the FP64 one but resolves the reference's two non-smooth points on its own:
     lattice spacing, so pairs sit there all the time; a pair whose FP32 and
     FP64 lengths fall on different sides changes the surface-tension
     derivative by a fixed amount from that step on (1e-2 of dL/dgamma).
+  * DFSPH adds the rho* >= rho0 and d >= 0 clamps, the warm-start conditions
+    and the |k_i + k_j| > 1e-5, |k_i| > 1e-5 gates (DFSPHSolver.cpp:285, :608,
+    :479, :483), all decided by FP32 values near their thresholds.
 The oracle therefore evaluates its FP64 rollouts on the GPU's branches:
-every GPU step's p > 0 pattern and FP32 positions are handed to it
-(Oracle.set_pressure_branch / set_cohesion_positions; the branch of a pair is
-decided by the reference's FP32 length of the GPU positions). Every
+every GPU step's FP32 positions and p > 0 pattern (WCSPH) or the k and
+rho* / divergence of every Jacobi pass (DFSPH, read from the recorded tape
+through sph_recorded_field) are handed to it (Oracle.set_pressure_branch /
+set_dfsph_branch / set_cohesion_positions; the branch of a pair is decided by
+the reference's FP32 length of the GPU positions). Every
 difference is still taken between runs whose inner-branch cohesion
 fingerprints agree, which the checker asserts.
 
@@ -43,9 +49,13 @@ R = 0.005
 SP = 2 * R
 
 
-def _scene(visc=True, st=True, n=(10, 10, 8), dt=5e-4):
+NIT = 8   # DFSPH: iterations of both solves (min = max)
+
+
+def _scene(visc=True, st=True, n=(10, 10, 8), dt=5e-4, method=0):
     hi = tuple(SP + SP * c for c in n)
-    return SC.scene_dict(0, R, [((SP, SP, SP), hi)], [((0, 0, 0), (0.2, 0.2, 0.15))], dt=dt,
+    return SC.scene_dict(method, R, [((SP, SP, SP), hi)], [((0, 0, 0), (0.2, 0.2, 0.15))], dt=dt, max_dt=dt,
+                         min_dt=dt, min_it=NIT, max_it=NIT, min_it_v=NIT, max_it_v=NIT,
                          viscosity_method=2 if visc else -1, viscosity=0.01 if visc else 0.0,
                          surface_tension_method=3 if st else -1, surface_tension=0.1 if st else 0.0)
 
@@ -67,13 +77,23 @@ class Case:
         self.P = S.params_from_scene(sc)
         self.nu = float(self.P.viscosity)
         self.gam = float(self.P.surface_tension)
+        self.dfsph = sc["Configuration"]["simulationMethod"] == 2
         self.branch = None
 
     def gpu_branch(self, nsteps):
-        """Per step of the GPU rollout: the p > 0 pattern and the FP32
-        positions at the start of the step (original order)."""
+        """The branches the GPU rollout took, in original order: per step the
+        FP32 positions at its start (cohesion) and, WCSPH, the p > 0 pattern;
+        DFSPH, the k and rho* / divergence of every Jacobi pass (from the tape
+        of a rollout recorded in one segment)."""
         c, sid = self.ctx, self.sid
         c.set_fluid_state(sid, self.x0, self.v0)
+        if self.dfsph:
+            c.forward_record(nsteps, nsteps)
+            rec = lambda name, k: np.array([c.recorded(name, k, sid, q) for q in range(NIT + 1)])
+            self.branch = [np.array([rec(name, k) for k in range(nsteps)])
+                           for name in ("k_density", "rho_star", "k_divergence", "divergence")]
+            self.branch_x = np.array([c.recorded("x", k, sid) for k in range(nsteps)], np.float32)
+            return self.branch
         masks, xs = [], []
         for _ in range(nsteps):
             xs.append(c.fluid_state(sid)["x"])
@@ -100,7 +120,10 @@ class Case:
             o.init()
             o.set_state64(x, v)
             if self.branch is not None:
-                o.set_pressure_branch(self.branch[:nsteps])
+                if self.dfsph:
+                    o.set_dfsph_branch(*(b[:nsteps] for b in self.branch))
+                else:
+                    o.set_pressure_branch(self.branch[:nsteps])
                 o.set_cohesion_positions(self.branch_x[:nsteps])
             o.step(nsteps)
             xt = o.state()["x"]
@@ -146,8 +169,9 @@ def _rel(g, fd):
 
 @pytest.mark.parametrize("visc,st", [(True, True), (True, False), (False, True), (False, False)])
 @pytest.mark.parametrize("nsteps", [1, 5])
-def test_adjoint_matches_fp64_finite_differences(tmp_path, visc, st, nsteps):
-    case = Case(str(tmp_path), _scene(visc, st))
+@pytest.mark.parametrize("method", [0, 2], ids=["wcsph", "dfsph"])
+def test_adjoint_matches_fp64_finite_differences(tmp_path, visc, st, nsteps, method):
+    case = Case(str(tmp_path), _scene(visc, st, method=method))
     try:
         case.gpu_branch(nsteps)
         xT, res = case.gpu_grad(nsteps)
@@ -182,11 +206,26 @@ def test_adjoint_long_rollout(tmp_path):
         case.ctx.close()
 
 
-def test_checkpoint_interval_does_not_change_the_gradient(tmp_path):
+def test_adjoint_dfsph_long_rollout(tmp_path):
+    """12 DFSPH steps (8 + 8 Jacobi iterations each), XSPH + surface tension,
+    on the GPU's branches."""
+    nsteps = 12
+    case = Case(str(tmp_path), _scene(True, True, method=2), xseed=2, vseed=102)
+    try:
+        case.gpu_branch(nsteps)
+        _, res = case.gpu_grad(nsteps)
+        for kind, g, fd in case.fd_checks(res, nsteps, ndir=3):
+            assert _rel(g, fd) <= 1e-3, (kind, g, fd)
+    finally:
+        case.ctx.close()
+
+
+@pytest.mark.parametrize("method", [0, 2], ids=["wcsph", "dfsph"])
+def test_checkpoint_interval_does_not_change_the_gradient(tmp_path, method):
     """Segment recomputation from checkpoints is bit-exact: the gradients and
     the final state are identical for every checkpoint interval, and the
     recorded forward equals plain stepping."""
-    case = Case(str(tmp_path), _scene(True, True))
+    case = Case(str(tmp_path), _scene(True, True, method=method))
     try:
         c, sid = case.ctx, case.sid
         c.set_fluid_state(sid, case.x0, case.v0)
@@ -207,11 +246,12 @@ def test_checkpoint_interval_does_not_change_the_gradient(tmp_path):
         case.ctx.close()
 
 
-def test_batched_adjoint_equals_single_scene(tmp_path):
+@pytest.mark.parametrize("method", [0, 2], ids=["wcsph", "dfsph"])
+def test_batched_adjoint_equals_single_scene(tmp_path, method):
     """Two scenes in one context get the gradients each gets alone, bit for bit."""
     tmp = str(tmp_path)
-    a = _scene(True, True)
-    b = _scene(True, True, n=(8, 9, 7))
+    a = _scene(True, True, method=method)
+    b = _scene(True, True, n=(8, 9, 7), method=method)
     b["Fluid"]["FluidModel"][0]["viscosity"] = 0.03
     b["Fluid"]["FluidModel"][0]["surfaceTension"] = 0.05
     singles = []
@@ -246,9 +286,10 @@ def test_batched_adjoint_equals_single_scene(tmp_path):
         ctx.close()
 
 
-def test_torch_autograd_function(tmp_path):
+@pytest.mark.parametrize("method", [0, 2], ids=["wcsph", "dfsph"])
+def test_torch_autograd_function(tmp_path, method):
     """loss.backward() through sph2o_amd.autograd.rollout returns the C-ABI gradients."""
-    case = Case(str(tmp_path), _scene(True, True))
+    case = Case(str(tmp_path), _scene(True, True, method=method))
     try:
         _, ref = case.gpu_grad(5)
         x0 = torch.tensor(case.x0, requires_grad=True)
@@ -285,15 +326,18 @@ def test_adjoint_errors(tmp_path):
         assert e.value.code == _lib.SPH_E_ARG
     finally:
         case.ctx.close()
-    dfsph = SC.small(2)
-    path = os.path.join(tmp, "dfsph.json")
-    SC.write_json(dfsph, path)
-    ctx = Context(0)
-    try:
-        ctx.load_scene_json(path)
-        ctx.init()
-        with pytest.raises(_lib.SPHError) as e:
-            ctx.forward_record(2, 1)
-        assert e.value.code == _lib.SPH_E_UNSUPPORTED
-    finally:
-        ctx.close()
+    # PCISPH, DFSPH with a free iteration count or an adaptive dt: not differentiable here
+    for k, sc in enumerate((SC.small(1), SC.small(2), _scene(method=2))):
+        if k == 2:
+            sc["Configuration"]["maxTimeStep"] = 1e-3
+        path = os.path.join(tmp, "u%d.json" % k)
+        SC.write_json(sc, path)
+        ctx = Context(0)
+        try:
+            ctx.load_scene_json(path)
+            ctx.init()
+            with pytest.raises(_lib.SPHError) as e:
+                ctx.forward_record(2, 1)
+            assert e.value.code == _lib.SPH_E_UNSUPPORTED
+        finally:
+            ctx.close()

@@ -299,3 +299,62 @@ def test_gpu_empty_fluid_scene_and_nonfinite(tmp_path):
     with pytest.raises(_lib.SPHError) as e:
         ctx2.step(3)
     assert e.value.code == _lib.SPH_E_NONFINITE
+
+
+def _subset_rows(ctx, sid, sc, xb, nsample, seed):
+    """Exact neighbour rows of `nsample` random particles from the FP32
+    oracle run on the particles of their 27-cell neighbourhoods only (which
+    holds every candidate of the sampled particles), against the GPU's rows."""
+    st = ctx.fluid_state(sid)
+    X = st["x"]
+    cells = ctx.cells(sid)
+    off, ids, boff, bids = ctx.neighbors(sid)
+    smp = np.random.default_rng(seed).choice(len(X), nsample, replace=False)
+    key = lambda c: (c[..., 0].astype(np.int64) * 1000003 + c[..., 1]) * 1000003 + c[..., 2]
+    need = set()
+    for d in np.stack(np.meshgrid([-1, 0, 1], [-1, 0, 1], [-1, 0, 1], indexing="ij"), -1).reshape(-1, 3):
+        need.update(key(cells[smp] + d).tolist())
+    sub = np.nonzero(np.isin(key(cells), np.fromiter(need, np.int64)))[0]
+    o = S.Oracle(S.params_from_scene(sc), X[sub], None, xb, bits=32)
+    try:
+        assert np.array_equal(o.cell_keys(), cells[sub])
+        ooff, oids, oboff, obids = o.neighbors()
+    finally:
+        o.close()
+    pos = {int(g): k for k, g in enumerate(sub)}
+    pairs = 0
+    for g in smp:
+        k = pos[int(g)]
+        a = np.sort(ids[off[g]:off[g + 1]])
+        assert np.array_equal(a, np.sort(sub[oids[ooff[k]:ooff[k + 1]]])), g
+        assert np.array_equal(np.sort(bids[boff[g]:boff[g + 1]]), np.sort(obids[oboff[k]:oboff[k + 1]])), g
+        pairs += len(a)
+    return pairs
+
+
+def test_gpu_p4m_neighbor_sets_at_scale(tmp_path):
+    """P4M: 3.9M PCISPH particles, a jittered lattice with a 1.5x-dense
+    cluster and N(0, 0.5^2) velocities. The cluster makes the step throw
+    particles out of the box at kilometres per second (the reference, on its
+    unbounded hash grid, does the same), so this is also the check of the
+    overflow buckets for cells outside the dense grid. Cell keys and fluid /
+    boundary within-support sets of 2000 random particles are exact at steps
+    0, 10 and 20."""
+    sc, x, v = SC.p4m()
+    p, _, xb = _params(sc, str(tmp_path))
+    ctx = Context(0)
+    try:
+        sid = ctx.add_scene(p, x, v, xb)
+        ctx.init()
+        assert ctx.num_fluid(sid) == len(x) > 3_900_000
+        total = _subset_rows(ctx, sid, sc, xb, 2000, 0)
+        for k in (1, 2):
+            ctx.step(10)
+            st = ctx.fluid_state(sid)
+            assert np.isfinite(st["x"]).all()
+            outside = np.any((st["x"] < -0.1) | (st["x"] > 2.1), axis=1).sum()
+            total += _subset_rows(ctx, sid, sc, xb, 2000, k)
+        assert outside > 1000   # the overflow buckets were exercised
+        assert total > 50_000
+    finally:
+        ctx.close()
