@@ -99,6 +99,42 @@ def measured_traffic(kernel):
     return None
 
 
+def pair_throughput(ctx, sid, its, steps_per_s):
+    """True neighbour pairs evaluated per second (every pass's sum over i of
+    |S_i|, fluid and boundary, at the mean list lengths of the final state)
+    and 27-cell candidate tests per second of the search. DFSPH passes per
+    step: normals and surface tension (fluid), rho* and velocity per density
+    iteration and warm start, density + alpha, divergence and velocity per
+    divergence iteration and warm start (fluid + boundary); the search tests
+    every fluid and boundary candidate of the 27 cells once."""
+    off, _, boff, _ = ctx.neighbors(sid)
+    nf_mean = float(np.diff(off).mean())
+    nb_mean = float(np.diff(boff).mean())
+    cells = ctx.cells(sid)
+    xb, _ = ctx.boundary(sid, volumes=False)
+    h = 4 * ctx.params(sid).particle_radius
+    bcells = np.floor(xb * np.float32(1.0 / h)).astype(np.int64)
+    pack = lambda c: (c[:, 0] + 2 ** 20) * 2 ** 42 + (c[:, 1] + 2 ** 20) * 2 ** 21 + (c[:, 2] + 2 ** 20)
+    uf, cf = np.unique(pack(cells.astype(np.int64)), return_counts=True)
+    ub, cb = np.unique(pack(bcells), return_counts=True)
+    cand = 0
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dz in (-1, 0, 1):
+                q = pack(cells.astype(np.int64) + np.array([dx, dy, dz]))
+                for u, c in ((uf, cf), (ub, cb)):
+                    k = np.minimum(np.searchsorted(u, q), len(u) - 1)
+                    cand += int(np.where(u[k] == q, c[k], 0).sum())
+    i_d, i_v = float(its[:, 0].mean()), float(its[:, 1].mean())
+    n = len(cells)
+    fluid_passes = 2 + 2 * (1 + i_d) + 1 + 2 * (1 + i_v)
+    both_passes = 2 * (1 + i_d) + 1 + 2 * (1 + i_v)
+    per_step = n * (fluid_passes * nf_mean + both_passes * nb_mean)
+    return {"mean_fluid_neighbours": nf_mean, "mean_boundary_neighbours": nb_mean,
+            "pairs_per_step": per_step, "pairs_per_s": per_step * steps_per_s,
+            "candidate_tests_per_step": cand, "candidate_tests_per_s": cand * steps_per_s}
+
+
 def cpu_baseline(scene_path, nf, steps, threads):
     """Time the compiled reference on the first `steps` steps of the workload."""
     from oracle import sphref as S
@@ -162,13 +198,16 @@ def adjoint_cost(steps, every, method, warmup=5):
     t_adj = time.perf_counter() - t0
     ctx.close()
     name = "WCSPH" if method == 0 else "DFSPH 8+8 iterations"
+    finite = lambda q: float(q) if np.isfinite(q) else None   # strict JSON has no inf / NaN
+    gnorm = float(np.linalg.norm(g["v0"].astype(np.float64)))
     return {"workload": "A100k: %s, %d fluid particles, %d steps at dt=%g, checkpoint every %d"
                         % (name, n, steps, dt, every),
             "forward_s": t_fwd, "record_s": t_rec, "adjoint_s": t_adj, "record_over_forward": t_rec / t_fwd,
             "adjoint_over_forward": t_adj / t_fwd,
             "loss": float(np.mean(np.sum((xT - y) ** 2, axis=1))),
-            "dL_dviscosity": g["viscosity"], "dL_dsurface_tension": g["surface_tension"],
-            "dL_dv0_norm": float(np.linalg.norm(g["v0"]))}
+            "gradient_finite": bool(np.isfinite(gnorm)),
+            "dL_dviscosity": finite(g["viscosity"]), "dL_dsurface_tension": finite(g["surface_tension"]),
+            "dL_dv0_norm": finite(gnorm)}
 
 
 def batched_gain(steps, warmup=3):
@@ -279,6 +318,7 @@ def main():
     st = ctx.fluid_state(sid)
     rho_err = float(np.mean(np.maximum(st["rho"] / 1000.0 - 1.0, 0.0)))
     its = np.array([[s[sid].iterations, s[sid].iterations_v] for s in stats])
+    pairs = pair_throughput(ctx, sid, its, args.steps / elapsed)
     value = nf * args.steps / elapsed
     own, reads_b = ALGO_BYTES.get(dominant, (0, False))
     bytes_per_launch = nf * (own + (16.0 * nb / nf if reads_b else 0.0))
@@ -317,6 +357,7 @@ def main():
         "mean_density_error": rho_err,
         "density_error_ok": rho_err <= 1e-3,
         "iterations_per_step": {"density": float(its[:, 0].mean()), "divergence": float(its[:, 1].mean())},
+        "neighbour_pairs": pairs,
         "sim_time_s": float(stats[-1][sid].t),
         "roofline": {"kernel": dominant, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
@@ -327,7 +368,7 @@ def main():
         "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    print(json.dumps(line, allow_nan=False), flush=True)
 
 
 if __name__ == "__main__":

@@ -24,11 +24,24 @@ __device__ __forceinline__ float fastRcp(float v) { return __builtin_amdgcn_rcpf
 // the 8 XCDs, each with its own L2; with the remap every XCD walks one
 // contiguous eighth of the (Morton-sorted) particle range, so the neighbours
 // it gathers stay in its own L2.
+#ifndef SPH_CU_REMAP
+#define SPH_CU_REMAP 0
+#endif
 __device__ __forceinline__ uint32_t logicalBlock() {
 #if SPH_XCD_REMAP
     const uint32_t n = gridDim.x, b = blockIdx.x;
     const uint32_t q = n >> 3, r = n & 7u, x = b & 7u, k = b >> 3;
+#if SPH_CU_REMAP
+    // second level: the k-th workgroup of an XCD is taken to land on CU k mod 32;
+    // give each CU a contiguous run of logical blocks, so the workgroups it
+    // holds at once are Morton neighbours that share their halos in L1
+    const uint32_t nx = q + (x < r ? 1u : 0u);
+    const uint32_t R = nx >> 5, rem = nx & 31u, c = k & 31u, rr = k >> 5;
+    const uint32_t kk = (rr < R || c < rem) ? c * R + min(c, rem) + rr : k;
+    return x * q + min(x, r) + kk;
+#else
     return x * q + min(x, r) + k;
+#endif
 #else
     return blockIdx.x;
 #endif
