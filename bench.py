@@ -367,6 +367,9 @@ def main():
         "adjoint_fwd_cost": max(a["adjoint_over_forward"] for a in adjoint.values()) if adjoint else None,
         "adjoint": adjoint,
         "cpu_baseline": cpu,
+        "profiled_warmup": {"steps": prof_steps,
+                            "kernel_ms": {k: round(v[0], 4) for k, v in sorted(ks.items(), key=lambda kv: -kv[1][0])},
+                            "launches": {k: v[1] for k, v in ks.items()}},
     }
     print(json.dumps(line, allow_nan=False), flush=True)
 

@@ -156,6 +156,7 @@ class Engine {
     std::vector<SceneH> scenes;
     bool initialized = false;
     bool interleaved = false;  // velocities live in xv[2i+1]
+    bool fuseFinish = true;    // Jacobi-loop reductions ride on the velocity launch
     int method = -1;
 
     size_t NP = 0, NB = 0, totalCells = 0;
@@ -164,7 +165,7 @@ class Engine {
     uint32_t capF = 64, capB = 48;
 
     DevBuf<SceneConst> dScenes;
-    DevBuf<SceneState> dState;
+    DevBuf<SceneState> dState;   // scene states, then the flags (one readback copy)
     DevBuf<int32_t> dBlockScene;
     DevBuf<float4> x, xAlt, v, vAlt, a, n, xs, ap, xk, xb, xv, xvAlt;
     DevBuf<float> rho, alpha, p, k, kws, kwsAlt, kvws, kvwsAlt, aux, vb;
@@ -174,14 +175,15 @@ class Engine {
     DevBuf<double> partials;
     DevBuf<float4> snapV;   // DFSPH: v and kv_ws before the passes that follow a neighbour search
     DevBuf<float> snapKvws;
-    DevBuf<int32_t> flags;
+    int32_t* dFlags = nullptr;   // inside dState's allocation
     DevBuf<unsigned char> sortTemp;
     size_t sortTempBytes = 0;
 
-    // pinned host mirrors of the device state and flags
+    // pinned host mirrors of the device state and flags (one allocation)
     SceneState* hState = nullptr;
     int32_t* hFlags = nullptr;
-    hipEvent_t evStart = nullptr, evStop = nullptr;
+    size_t stateBlockBytes = 0;
+    hipEvent_t evStart = nullptr, evStop = nullptr, evSync = nullptr;
     int64_t syncsInStep = 0;
 
     // profiling
@@ -203,11 +205,11 @@ class Engine {
         }
         if (evStart) (void)hipEventDestroy(evStart);
         if (evStop) (void)hipEventDestroy(evStop);
+        if (evSync) (void)hipEventDestroy(evSync);
         if (ownStream && stream) (void)hipStreamDestroy(stream);
     }
     void releasePinned() {
         if (hState) (void)hipHostFree(hState);
-        if (hFlags) (void)hipHostFree(hFlags);
         hState = nullptr;
         hFlags = nullptr;
     }
@@ -472,7 +474,7 @@ class Engine {
         d.fRange = fRange.p; d.bRange = bRange.p;
         d.nbrF = nbrF.p; d.cntF = cntF.p; d.nbrB = nbrB.p; d.cntB = cntB.p;
         d.capF = capF; d.capB = capB;
-        d.partials = partials.p; d.flags = flags.p;
+        d.partials = partials.p; d.flags = dFlags;
         return d;
     }
 
@@ -487,6 +489,7 @@ class Engine {
         }
         HIPCHK(hipEventCreate(&evStart));
         HIPCHK(hipEventCreate(&evStop));
+        HIPCHK(hipEventCreateWithFlags(&evSync, hipEventDisableTiming));
         // slots
         uint32_t off = 0, boff = 0;
         for (auto& s : scenes) {
@@ -509,7 +512,12 @@ class Engine {
         NB = boff;
         numBlocks = static_cast<uint32_t>(NP / kBlock);
         const size_t np = NP ? NP : 1;
-        dState.alloc(scenes.size());
+        // flags follow the scene states in the same allocation
+        const size_t flagStates = (scenes.size() * kFlagSlots * sizeof(int32_t) + sizeof(SceneState) - 1) /
+                                  sizeof(SceneState);
+        dState.alloc(scenes.size() + flagStates);
+        dFlags = reinterpret_cast<int32_t*>(dState.p + scenes.size());
+        stateBlockBytes = scenes.size() * sizeof(SceneState) + scenes.size() * kFlagSlots * sizeof(int32_t);
         std::vector<int32_t> bs(numBlocks ? numBlocks : 1, 0);
         for (size_t i = 0; i < scenes.size(); ++i)
             for (uint32_t b = scenes[i].blockBegin; b < scenes[i].blockEnd; ++b) bs[b] = static_cast<int32_t>(i);
@@ -517,6 +525,7 @@ class Engine {
         HIPCHK(hipMemcpy(dBlockScene.p, bs.data(), bs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         for (auto* b : {&x, &xAlt, &v, &vAlt, &a, &n, &xs, &ap, &xk}) b->alloc(np);
         interleaved = sphKernelsInterleaveVelocity() != 0;
+        fuseFinish = sphKernelsFuseFinish() != 0;
         if (interleaved) {
             xv.alloc(2 * np);
             xvAlt.alloc(2 * np);
@@ -524,17 +533,20 @@ class Engine {
         for (auto* b : {&rho, &alpha, &p, &k, &kws, &kwsAlt, &kvws, &kvwsAlt, &aux}) b->alloc(np);
         for (auto* b : {&id, &idAlt, &keys, &keysAlt, &vals, &valsAlt, &cntF, &cntB}) b->alloc(np);
         allocNeighborLists();
-        partials.alloc(static_cast<size_t>(numBlocks ? numBlocks : 1));
+        // the DFSPH source passes run sphKernelsLanesPerParticle() launch blocks per particle block
+        partials.alloc(static_cast<size_t>(numBlocks ? numBlocks : 1) *
+                       static_cast<size_t>(std::max(1, sphKernelsLanesPerParticle())));
         if (method == SPH_DFSPH) {
             snapV.alloc(np);
             snapKvws.alloc(np);
         }
-        flags.alloc(scenes.size() * kFlagSlots);
         HIPCHK(hipMemset(a.p, 0, np * sizeof(float4)));
         HIPCHK(hipMemset(n.p, 0, np * sizeof(float4)));
         for (auto* b : {&rho, &alpha, &p, &k, &kws, &kvws, &aux}) HIPCHK(hipMemset(b->p, 0, np * sizeof(float)));
-        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&hState), scenes.size() * sizeof(SceneState)));
-        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&hFlags), scenes.size() * kFlagSlots * sizeof(int32_t)));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&hState), std::max(stateBlockBytes, sizeof(SceneState))));
+        std::memset(hState, 0, stateBlockBytes);
+        hFlags = reinterpret_cast<int32_t*>(hState + scenes.size());
+        HIPCHK(hipMemset(dFlags, 0, scenes.size() * kFlagSlots * sizeof(int32_t)));
         for (size_t i = 0; i < scenes.size(); ++i) {
             SceneState& T = hState[i];
             std::memset(&T, 0, sizeof(T));
@@ -599,11 +611,26 @@ class Engine {
     }
 
     // Copy the device state (and flags) to the pinned mirrors and wait.
+    // NP float4 velocities between buffers with element strides (in float4s);
+    // unit strides take a plain linear copy.
+    void copyVel(float4* dst, size_t dstStride, const float4* src, size_t srcStride) {
+        if (dstStride == 1 && srcStride == 1)
+            HIPCHK(hipMemcpyAsync(dst, src, NP * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+        else
+            HIPCHK(hipMemcpy2DAsync(dst, dstStride * sizeof(float4), src, srcStride * sizeof(float4), sizeof(float4),
+                                    NP, hipMemcpyDeviceToDevice, stream));
+    }
+    // Copy the device state block to the pinned mirror and wait for it. The
+    // wait polls an event: a blocking wait on a millisecond of queued work
+    // sleeps, and its wake-up leaves the GPU idle for tens of microseconds
+    // before the next launches arrive.
     void readState() {
-        HIPCHK(hipMemcpyAsync(hState, dState.p, scenes.size() * sizeof(SceneState), hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipMemcpyAsync(hFlags, flags.p, scenes.size() * kFlagSlots * sizeof(int32_t), hipMemcpyDeviceToHost,
-                              stream));
-        HIPCHK(hipStreamSynchronize(stream));
+        HIPCHK(hipMemcpyAsync(hState, dState.p, stateBlockBytes, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipEventRecord(evSync, stream));
+        hipError_t e;
+        while ((e = hipEventQuery(evSync)) == hipErrorNotReady) {
+        }
+        HIPCHK(e);
         ++syncsInStep;
     }
 
@@ -633,6 +660,12 @@ class Engine {
     void finish(const DevView& d, int kind, int slot) {
         timed(K_FINISH, [&] { return launchFinish(d, kind, slot, stream); });
     }
+    // Velocity correction of a Jacobi iteration; with fuseFinish it also files
+    // the preceding pass's loop reduction of kind `kind`.
+    void velocityPass(const DevView& d, int slot, int kind) {
+        const int fin = fuseFinish ? kind : -1;
+        timed(K_VELOCITY, [&] { return launchVelocity(d, slot, fin, stream); });
+    }
     void collectProfile() {
         if (!evUsed) return;
         HIPCHK(hipStreamSynchronize(stream));
@@ -649,7 +682,7 @@ class Engine {
     // keys -> sort -> reorder (+ cell ranges) -> within-support lists.
     // dfsphAdvance fuses the DFSPH position update into the keys kernel.
     void searchOnce(bool dfsphAdvance) {
-        HIPCHK(hipMemsetAsync(flags.p, 0, scenes.size() * kFlagSlots * sizeof(int32_t), stream));
+        HIPCHK(hipMemsetAsync(dFlags, 0, scenes.size() * kFlagSlots * sizeof(int32_t), stream));
         DevView d = view();
         timed(K_KEYS, [&] { return launchKeys(d, dfsphAdvance ? 1 : 0, keys.p, vals.p, stream); });
         timed(K_FILL, [&] { return launchFillRanges(fRange.p, totalCells, stream); });
@@ -856,15 +889,15 @@ class Engine {
         timed(K_PREDICT, [&] { return launchPredictVelocity(d, stream); });
         // 4. correctDensityError: warm start (and loop start), then the loop
         timed(K_RHO_STAR, [&] { return launchRhoStar(d, 1, 2, stream); });
-        timed(K_VELOCITY, [&] { return launchVelocity(d, 2, stream); });
+        timed(K_VELOCITY, [&] { return launchVelocity(d, 2, -1, stream); });
         int first = 1;
         for (auto& sc : scenes) first = std::max(first, std::max(static_cast<int>(sc.P.min_iterations), sc.its));
         runLoop(first, maxOver(&sph_scene_params::max_iterations),
                 [&](int slot) {
                     timed(K_RHO_STAR, [&] { return launchRhoStar(d, 0, slot, stream); });
-                    finish(d, kFinishDensity, slot);
+                    if (!fuseFinish) finish(d, kFinishDensity, slot);
                 },
-                [&](int slot) { timed(K_VELOCITY, [&] { return launchVelocity(d, slot, stream); }); },
+                [&](int slot) { velocityPass(d, slot, kFinishDensity); },
                 [] { return true; });
         for (size_t s = 0; s < scenes.size(); ++s) scenes[s].its = hState[s].it;
         // 5-7. updatePositions fused with the keys, then the neighbour search;
@@ -872,30 +905,27 @@ class Engine {
         searchOnce(true);
         for (int attempt = 0;; ++attempt) {
             d = view();
-            HIPCHK(hipMemcpy2DAsync(snapV.p, sizeof(float4), interleaved ? xv.p + 1 : v.p,
-                                    (interleaved ? 2 : 1) * sizeof(float4), sizeof(float4), NP,
-                                    hipMemcpyDeviceToDevice, stream));
+            copyVel(snapV.p, 1, interleaved ? xv.p + 1 : v.p, interleaved ? 2 : 1);
             HIPCHK(hipMemcpyAsync(snapKvws.p, kvws.p, NP * sizeof(float), hipMemcpyDeviceToDevice, stream));
             // 8-9. computeDensities + computeAlpha
             timed(K_DENSITY, [&] { return launchDensityAlpha(d, 1, stream); });
             // 10. correctDivergenceError: warm start (and loop start), then the loop
             timed(K_DIVERGENCE, [&] { return launchDivergence(d, 1, 2, stream); });
-            timed(K_VELOCITY, [&] { return launchVelocity(d, 2, stream); });
+            timed(K_VELOCITY, [&] { return launchVelocity(d, 2, -1, stream); });
             int firstV = 1;
             for (auto& sc : scenes) firstV = std::max(firstV, std::max(static_cast<int>(sc.P.min_iterations_v), sc.itsV));
             const bool ok = runLoop(
                 firstV, maxOver(&sph_scene_params::max_iterations),
                 [&](int slot) {
                     timed(K_DIVERGENCE, [&] { return launchDivergence(d, 0, slot, stream); });
-                    finish(d, kFinishDivergence, slot);
+                    if (!fuseFinish) finish(d, kFinishDivergence, slot);
                 },
-                [&](int slot) { timed(K_VELOCITY, [&] { return launchVelocity(d, slot, stream); }); },
+                [&](int slot) { velocityPass(d, slot, kFinishDivergence); },
                 [&] { return searchValid(); });
             if (ok) break;
             if (attempt >= 5) throw SphError{SPH_E_OUT_OF_GRID, "could not place particles into a grid"};
             // the lists were invalid: undo the divergence passes and search again
-            HIPCHK(hipMemcpy2DAsync(interleaved ? xv.p + 1 : v.p, (interleaved ? 2 : 1) * sizeof(float4), snapV.p,
-                                    sizeof(float4), sizeof(float4), NP, hipMemcpyDeviceToDevice, stream));
+            copyVel(interleaved ? xv.p + 1 : v.p, interleaved ? 2 : 1, snapV.p, 1);
             HIPCHK(hipMemcpyAsync(kvws.p, snapKvws.p, NP * sizeof(float), hipMemcpyDeviceToDevice, stream));
             searchOnce(false);
         }
@@ -940,9 +970,12 @@ class Engine {
         runLoop(first, maxOver(&sph_scene_params::max_iterations),
                 [&](int slot) {
                     timed(K_PCI_DENSITY, [&] { return launchPciDensity(d, slot, stream); });
-                    finish(d, kFinishDensity, slot);
+                    if (!fuseFinish) finish(d, kFinishPciDensity, slot);
                 },
-                [&](int slot) { timed(K_PCI_PACC, [&] { return launchPciPressureAcc(d, slot, stream); }); },
+                [&](int slot) {
+                    const int fin = fuseFinish ? kFinishPciDensity : -1;
+                    timed(K_PCI_PACC, [&] { return launchPciPressureAcc(d, slot, fin, stream); });
+                },
                 [] { return true; });
         timed(K_PCI_INTEGRATE, [&] { return launchPciIntegrate(d, stream); });
         for (size_t s = 0; s < scenes.size(); ++s) {
@@ -1204,13 +1237,13 @@ class Engine {
         timed(K_PREDICT, [&] { return launchPredictVelocity(d, stream); });
         timed(K_RHO_STAR, [&] { return launchRhoStar(d, 1, 2, stream); });
         recordPass(A, slot, nit);
-        timed(K_VELOCITY, [&] { return launchVelocity(d, 2, stream); });
+        timed(K_VELOCITY, [&] { return launchVelocity(d, 2, -1, stream); });
         for (int it = 0; it < nit; ++it) {
             recordVelocity(A, slot, it);
             timed(K_RHO_STAR, [&] { return launchRhoStar(d, 0, it & 1, stream); });
-            finish(d, kFinishDensity, it & 1);
+            if (!fuseFinish) finish(d, kFinishDensity, it & 1);
             recordPass(A, slot, it);
-            timed(K_VELOCITY, [&] { return launchVelocity(d, it & 1, stream); });
+            velocityPass(d, it & 1, kFinishDensity);
         }
         readState();
         for (size_t s = 0; s < scenes.size(); ++s) scenes[s].its = hState[s].it;
@@ -1228,13 +1261,13 @@ class Engine {
         copyD(B.alpha.p + o, alpha.p, np * sizeof(float));
         timed(K_DIVERGENCE, [&] { return launchDivergence(d, 1, 2, stream); });
         recordPass(B, slot, nit);
-        timed(K_VELOCITY, [&] { return launchVelocity(d, 2, stream); });
+        timed(K_VELOCITY, [&] { return launchVelocity(d, 2, -1, stream); });
         for (int it = 0; it < nit; ++it) {
             recordVelocity(B, slot, it);
             timed(K_DIVERGENCE, [&] { return launchDivergence(d, 0, it & 1, stream); });
-            finish(d, kFinishDivergence, it & 1);
+            if (!fuseFinish) finish(d, kFinishDivergence, it & 1);
             recordPass(B, slot, it);
-            timed(K_VELOCITY, [&] { return launchVelocity(d, it & 1, stream); });
+            velocityPass(d, it & 1, kFinishDivergence);
         }
         readState();
         for (size_t s = 0; s < scenes.size(); ++s) scenes[s].itsV = hState[s].it;

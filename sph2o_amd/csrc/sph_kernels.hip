@@ -37,6 +37,30 @@ namespace sph {
 #ifndef SPH_XK_VELOCITY
 #define SPH_XK_VELOCITY 0
 #endif
+// Rows of the fluid neighbour list that kBuildNeighbors stages in LDS (4 B x
+// 256 lanes per row) and stores as whole 1 KB rows at the end; entries past
+// the staged rows are stored directly. 0 stores every entry directly.
+// Lanes per particle of the DFSPH Jacobi passes: the lanes of a wave take
+// 64 / L particles and split each one's neighbour list by slot residue, so one
+// gather instruction reads the neighbours of fewer cells. Measured on the
+// 1M-particle dam break (ms/step, source passes / velocity pass lanes):
+// 1/1 2.75, 2/2 2.46, 4/4 2.39, 8/8 2.65, with 4 best for the source passes
+// (kRhoStar, kDivergence) and 2 for the velocity pass.
+#ifndef SPH_LPP
+#define SPH_LPP 4
+#endif
+#ifndef SPH_LPP_VEL
+#define SPH_LPP_VEL 2
+#endif
+constexpr uint32_t kLpp = SPH_LPP;        // kRhoStar, kDivergence
+constexpr uint32_t kLppVel = SPH_LPP_VEL; // kVelocity
+static_assert(kLpp == 1 || kLpp == 2 || kLpp == 4 || kLpp == 8, "SPH_LPP must be 1, 2, 4 or 8");
+static_assert(kLppVel == 1 || kLppVel == 2 || kLppVel == 4 || kLppVel == 8, "SPH_LPP_VEL must be 1, 2, 4 or 8");
+extern "C" __host__ int sphKernelsLanesPerParticle() { return SPH_LPP; }
+
+#ifndef SPH_LIST_STAGE
+#define SPH_LIST_STAGE 24
+#endif
 #ifndef SPH_XV
 #define SPH_XV 0
 #endif
@@ -53,6 +77,12 @@ namespace sph {
 #define POSV(dv, idx) (dv).x[idx]
 #endif
 extern "C" __host__ int sphKernelsInterleaveVelocity() { return SPH_XV; }
+// Whether the Jacobi-loop reduction rides on the velocity / PCISPH pressure
+// acceleration launch (1) or gets a kFinish launch of its own (0).
+#ifndef SPH_FUSE_FINISH
+#define SPH_FUSE_FINISH 1
+#endif
+extern "C" __host__ int sphKernelsFuseFinish() { return SPH_FUSE_FINISH; }
 
 __device__ __forceinline__ uint32_t ldList(const uint32_t* p) {
 #if SPH_NT_LISTS
@@ -269,6 +299,11 @@ __global__ void __launch_bounds__(kBlock) kReorder(DevView d, const uint32_t* ke
 // reference's rounding (DFSPHSolver.cpp:575). Self is included.
 __global__ void __launch_bounds__(kBlock) kBuildNeighbors(DevView d) {
     SCENE_PROLOGUE(2)
+#if SPH_LIST_STAGE
+    // a lane finds its s-th neighbour at its own pace, so direct stores of one
+    // wave land in 64 different rows; staged rows leave as coalesced lines
+    __shared__ uint32_t sList[SPH_LIST_STAGE * kBlock];
+#endif
     uint32_t nf = 0, nb = 0;
     if (valid) {
         const float4 xi = d.x[i];
@@ -286,6 +321,10 @@ __global__ void __launch_bounds__(kBlock) kBuildNeighbors(DevView d) {
                         const float4 xj = d.x[j];
                         if (bucket && !inCell(S, xj, ax, ay, az)) continue;
                         if (dRef(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) <= S.dmax) {
+#if SPH_LIST_STAGE
+                            if (nf < SPH_LIST_STAGE) sList[nf * kBlock + threadIdx.x] = j;
+                            else
+#endif
                             if (nf < d.capF) d.nbrF[nf * NP + i] = j;
                             ++nf;
                         }
@@ -325,6 +364,17 @@ __global__ void __launch_bounds__(kBlock) kBuildNeighbors(DevView d) {
         atomicMax(&d.flags[sIdx * kFlagSlots + kFlagMaxFluidNbr], mf);
         atomicMax(&d.flags[sIdx * kFlagSlots + kFlagMaxBoundaryNbr], mb);
     }
+#if SPH_LIST_STAGE
+    // whole rows up to the block's longest list; entries past a lane's own
+    // count get ~0 (never read: every pass stops at min(count, capacity))
+    int rows = smf[0];
+    for (int w = 1; w < kBlock / 64; ++w) rows = max(rows, smf[w]);
+    rows = min(rows, min(static_cast<int>(d.capF), SPH_LIST_STAGE));
+    const size_t NPs = d.NP;
+    const uint32_t row0 = lb * kBlock + threadIdx.x;   // < NP: the grid covers the padded slots exactly
+    for (int s = 0; s < rows; ++s)
+        d.nbrF[s * NPs + row0] = static_cast<uint32_t>(s) < nf ? sList[s * kBlock + threadIdx.x] : ~0u;
+#endif
 }
 
 // AkinciBoundaryObject::computeVolume (AkinciBoundaryObject.cpp:10-34):
@@ -597,6 +647,52 @@ __global__ void __launch_bounds__(kBlock) kVmax(DevView d) {
     storePartial<true>(d, vm);
 }
 
+// Combine scene s's block partials in a fixed order and file the result (the
+// loop control of a Jacobi iteration, or the CFL time step). Called by every
+// thread of one block.
+__device__ void finishScene(const DevView& d, int s, int kind, int slot) {
+    const SceneConst& C = d.scenes[s];
+    const bool mx = kind == kFinishTimeStep;
+    // the DFSPH source passes write one partial per launch block, kLpp per particle block
+    const uint32_t mul = (kind == kFinishDensity || kind == kFinishDivergence) ? kLpp : 1u;
+    const uint32_t bBegin = C.blockBegin * mul, bEnd = C.blockEnd * mul;
+    // each thread combines partials threadIdx.x, +256, +512, ... in that order;
+    // eight loads are in flight before the first is combined
+    double acc = 0.0;
+    uint32_t b = bBegin + threadIdx.x;
+    for (; b + 7u * kBlock < bEnd; b += 8u * kBlock) {
+        double q[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) q[u] = d.partials[b + u * kBlock];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = mx ? fmax(acc, q[u]) : acc + q[u];
+    }
+    for (; b < bEnd; b += kBlock) {
+        const double p = d.partials[b];
+        acc = mx ? fmax(acc, p) : acc + p;
+    }
+    const double tot = mx ? blockReduce<true>(acc) : blockReduce<false>(acc);
+    if (threadIdx.x != 0) return;
+    SceneState& U = d.state[s];
+    const double mean = C.nf ? tot / C.nf : 0.0;
+    if (kind == kFinishDensity || kind == kFinishPciDensity)
+        loopEpilogue(U, slot, mean, static_cast<double>(C.eta), C.minIt, C.maxIt);
+    else if (kind == kFinishDivergence)
+        loopEpilogue(U, slot, mean, static_cast<double>(C.etaV / U.dt), C.minItV, C.maxIt);
+    else
+        timeStepEpilogue(d, s, tot);
+}
+
+// The same, done by the first blocks of the launch that follows a Jacobi pass
+// (kVelocityFinish, kPciPressureAcc) once their own particles are done: that
+// launch reads active[slot] and dt, which the loop epilogue leaves alone (it
+// sets active[slot ^ 1], it, err), so the reduction needs no launch of its
+// own. kind < 0: nothing to finish.
+__device__ __forceinline__ void finishInline(const DevView& d, int kind, int slot) {
+    if (kind < 0) return;
+    for (int s = blockIdx.x; s < d.numScenes; s += gridDim.x) finishScene(d, s, kind, slot);
+}
+
 // DFSPHSolver::predictVelocities (DFSPHSolver.cpp:180-202)
 __global__ void __launch_bounds__(kBlock) kPredictVelocity(DevView d) {
     SCENE_PROLOGUE(2)
@@ -611,12 +707,58 @@ __global__ void __launch_bounds__(kBlock) kPredictVelocity(DevView d) {
 }
 
 // ---------------------------------------------------------------- DFSPH solves
+// Scene prologue of a launch with L lanes per particle (grid = L x the
+// particle blocks); `sub` is the lane's share of each neighbour list.
+#define SCENE_PROLOGUE_LANES(slot, L)                                                              \
+    const uint32_t lb = logicalBlock();                                                            \
+    const int sIdx = d.blockScene[lb / (L)];                                                       \
+    const SceneConst& S = d.scenes[sIdx];                                                          \
+    const SceneState& T = d.state[sIdx];                                                           \
+    const float dt = T.dt, dt2 = T.dt2;                                                            \
+    const uint32_t sub = (threadIdx.x & 63u) / (64u / (L));                                        \
+    const uint32_t i = lb * (kBlock / (L)) + (threadIdx.x >> 6) * (64u / (L)) + threadIdx.x % (64u / (L)); \
+    const bool valid = T.active[slot] != 0 && i < S.fEnd;
+
+// Sum over the L lanes of one particle (every lane gets the same value).
+template <uint32_t L>
+__device__ __forceinline__ float lppSum(float v) {
+#pragma unroll
+    for (uint32_t off = 64u / L; off < 64u; off <<= 1) v += __shfl_xor(v, static_cast<int>(off), 64);
+    return v;
+}
+
+// This lane's share of sum_j m g_ij (v_i - v_j) . r_ij + sum_b rho0 V_b g_ib v_i . r_ib,
+// the density change rate of predictDensities (DFSPHSolver.cpp:250-283) and
+// computeDivergenceError (:560-606).
+template <uint32_t L>
+__device__ __forceinline__ float sourceShare(const DevView& d, const SceneConst& S, uint32_t i, uint32_t sub,
+                                             float4 xi, float4 vi, uint32_t nf, uint32_t nb) {
+    const size_t NP = d.NP;
+    float dc = 0.0f;
+SPH_UNROLL_LOOP
+    for (uint32_t s = sub; s < nf; s += L) {
+        const uint32_t j = ldList(d.nbrF + s * NP + i);
+        const float4 xj = POSV(d, j);
+        const float4 vj = VEL(d, j);
+        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        const float g = gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+        dc += S.m * (g * ((vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz));
+    }
+    for (uint32_t s = sub; s < nb; s += L) {
+        const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
+        const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
+        const float g = gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+        dc += xb.w * (g * (vi.x * rx + vi.y * ry + vi.z * rz));
+    }
+    return dc;
+}
+
 // DFSPHSolver::predictDensities (DFSPHSolver.cpp:226-299). warm: the
 // warm-start rescale of densityWarmStart (:863-886), and block 0 starts the
 // density loop; otherwise the k pass of correctDensityError (:435-453) with
 // the FP64 average error feeding the loop control.
 __global__ void __launch_bounds__(kBlock) kRhoStar(DevView d, int warm, int slot) {
-    SCENE_PROLOGUE(slot)
+    SCENE_PROLOGUE_LANES(slot, kLpp)
     if (warm && blockIdx.x == 0)
         for (int s = threadIdx.x; s < d.numScenes; s += kBlock) {
             const SceneConst& C = d.scenes[s];
@@ -624,28 +766,15 @@ __global__ void __launch_bounds__(kBlock) kRhoStar(DevView d, int warm, int slot
         }
     const bool first = T.it == 0;
     double err = 0.0;
+    float4 xi = make_float4(0.0f, 0.0f, 0.0f, 0.0f), vi = xi;
+    float dc = 0.0f;
     if (valid) {
-        const size_t NP = d.NP;
-        const float4 xi = d.x[i];
-        const float4 vi = VEL(d, i);
-        float dc = 0.0f;
-        const uint32_t nf = min(d.cntF[i], d.capF);
-SPH_UNROLL_LOOP
-        for (uint32_t s = 0; s < nf; ++s) {
-            const uint32_t j = ldList(d.nbrF + s * NP + i);
-            const float4 xj = POSV(d, j);
-            const float4 vj = VEL(d, j);
-            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-            const float g = gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
-            dc += S.m * (g * ((vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz));
-        }
-        const uint32_t nb = S.akinci ? min(d.cntB[i], d.capB) : 0u;
-        for (uint32_t s = 0; s < nb; ++s) {
-            const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
-            const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
-            const float g = gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
-            dc += xb.w * (g * (vi.x * rx + vi.y * ry + vi.z * rz));
-        }
+        xi = d.x[i];
+        vi = VEL(d, i);
+        dc = sourceShare<kLpp>(d, S, i, sub, xi, vi, min(d.cntF[i], d.capF), S.akinci ? min(d.cntB[i], d.capB) : 0u);
+    }
+    dc = lppSum<kLpp>(dc);
+    if (valid && sub == 0) {
         const float rhoi = d.rho[i];
         const float pd = fmaxf(rhoi + dc * dt, S.rho0);
         d.aux[i] = pd;
@@ -669,45 +798,67 @@ SPH_UNROLL_LOOP
 // (DFSPHSolver.cpp:455-511, :788-823, :894-941, :998-1045). xk = (x, k) was
 // written by the preceding kRhoStar / kDivergence launch, so each neighbour
 // costs one 16-byte gather.
-__global__ void __launch_bounds__(kBlock) kVelocity(DevView d, int slot) {
-    SCENE_PROLOGUE(slot)
-    if (!valid) return;
+// Velocity update of one Jacobi pass, kLppVel lanes per particle; shared by the
+// two kernels below.
+__device__ __forceinline__ void velocityUpdate(const DevView& d, const SceneConst& S, uint32_t i, uint32_t sub,
+                                               bool valid, float dt) {
     const size_t NP = d.NP;
-    const float4 xi = d.xk[i];
-    const float ki = xi.w;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    const uint32_t nf = min(d.cntF[i], d.capF);
+    if (valid) {
+        const float4 xi = d.xk[i];
+        const float ki = xi.w;
+        const uint32_t nf = min(d.cntF[i], d.capF);
 SPH_UNROLL_LOOP
-    for (uint32_t s = 0; s < nf; ++s) {
-        const uint32_t j = ldList(d.nbrF + s * NP + i);
+        for (uint32_t s = sub; s < nf; s += kLppVel) {
+            const uint32_t j = ldList(d.nbrF + s * NP + i);
 #if SPH_XK_VELOCITY
-        const float4 xj = d.xk[j];
-        const float ks = ki + xj.w;
-        if (fabsf(ks) > S.eps) {
+            const float4 xj = d.xk[j];
+            const float ks = ki + xj.w;
+            if (fabsf(ks) > S.eps) {
 #else
-        const float ks = ki + d.k[j];
-        if (fabsf(ks) > S.eps) {   // closed gates skip the position gather
-            const float4 xj = d.x[j];
+            const float ks = ki + d.k[j];
+            if (fabsf(ks) > S.eps) {   // closed gates skip the position gather
+                const float4 xj = d.x[j];
 #endif
-            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-            const float c = (S.m * ks) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
-            sx += c * rx; sy += c * ry; sz += c * rz;
+                const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+                const float c = (S.m * ks) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+                sx += c * rx; sy += c * ry; sz += c * rz;
+            }
+        }
+        if (S.akinci && fabsf(ki) > S.eps) {
+            const uint32_t nb = min(d.cntB[i], d.capB);
+            for (uint32_t s = sub; s < nb; s += kLppVel) {
+                const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
+                const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
+                const float c = (xb.w * ki) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+                sx += c * rx; sy += c * ry; sz += c * rz;
+            }
         }
     }
-    if (S.akinci && fabsf(ki) > S.eps) {
-        const uint32_t nb = min(d.cntB[i], d.capB);
-        for (uint32_t s = 0; s < nb; ++s) {
-            const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
-            const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
-            const float c = (xb.w * ki) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
-            sx += c * rx; sy += c * ry; sz += c * rz;
-        }
+    sx = lppSum<kLppVel>(sx);
+    sy = lppSum<kLppVel>(sy);
+    sz = lppSum<kLppVel>(sz);
+    if (valid && sub == 0) {
+        float4 vi = VEL(d, i);
+        vi.x -= dt * sx;
+        vi.y -= dt * sy;
+        vi.z -= dt * sz;
+        VEL(d, i) = vi;
     }
-    float4 vi = VEL(d, i);
-    vi.x -= dt * sx;
-    vi.y -= dt * sy;
-    vi.z -= dt * sz;
-    VEL(d, i) = vi;
+}
+
+__global__ void __launch_bounds__(kBlock) kVelocity(DevView d, int slot) {
+    SCENE_PROLOGUE_LANES(slot, kLppVel)
+    (void)dt2;
+    velocityUpdate(d, S, i, sub, valid, dt);
+}
+// The same, plus the loop reduction of the preceding pass, done by the first
+// blocks after their own particles (see finishInline).
+__global__ void __launch_bounds__(kBlock) kVelocityFinish(DevView d, int slot, int fin) {
+    SCENE_PROLOGUE_LANES(slot, kLppVel)
+    (void)dt2;
+    velocityUpdate(d, S, i, sub, valid, dt);
+    finishInline(d, fin, slot);
 }
 
 // DFSPHSolver::computeDivergenceError (DFSPHSolver.cpp:538-621); the
@@ -717,7 +868,7 @@ SPH_UNROLL_LOOP
 // correctDivergenceError (:750-768) with the loop control. The loop bound is
 // maxIterations, not maxIterationsV (:746).
 __global__ void __launch_bounds__(kBlock) kDivergence(DevView d, int warm, int slot) {
-    SCENE_PROLOGUE(slot)
+    SCENE_PROLOGUE_LANES(slot, kLpp)
     if (warm && blockIdx.x == 0)
         for (int s = threadIdx.x; s < d.numScenes; s += kBlock) {
             const SceneConst& C = d.scenes[s];
@@ -726,34 +877,19 @@ __global__ void __launch_bounds__(kBlock) kDivergence(DevView d, int warm, int s
         }
     const bool first = T.it == 0;
     double err = 0.0;
+    float4 xi = make_float4(0.0f, 0.0f, 0.0f, 0.0f), vi = xi;
+    float de = 0.0f;
+    uint32_t cnt = 0;
     if (valid) {
-        const size_t NP = d.NP;
-        const float4 xi = d.x[i];
-        const float4 vi = VEL(d, i);
-        float de = 0.0f;
+        xi = d.x[i];
+        vi = VEL(d, i);
         const uint32_t nfTrue = d.cntF[i];
-        const uint32_t nf = min(nfTrue, d.capF);
-SPH_UNROLL_LOOP
-        for (uint32_t s = 0; s < nf; ++s) {
-            const uint32_t j = ldList(d.nbrF + s * NP + i);
-            const float4 xj = POSV(d, j);
-            const float4 vj = VEL(d, j);
-            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-            const float g = gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
-            de += S.m * (g * ((vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz));
-        }
-        uint32_t cnt = nfTrue;
-        if (S.akinci) {
-            const uint32_t nbTrue = d.cntB[i];
-            const uint32_t nb = min(nbTrue, d.capB);
-            cnt += nbTrue;
-            for (uint32_t s = 0; s < nb; ++s) {
-                const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
-                const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
-                const float g = gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
-                de += xb.w * (g * (vi.x * rx + vi.y * ry + vi.z * rz));
-            }
-        }
+        const uint32_t nbTrue = S.akinci ? d.cntB[i] : 0u;
+        cnt = nfTrue + nbTrue;
+        de = sourceShare<kLpp>(d, S, i, sub, xi, vi, min(nfTrue, d.capF), min(nbTrue, d.capB));
+    }
+    de = lppSum<kLpp>(de);
+    if (valid && sub == 0) {
         de = cnt < 20u ? 0.0f : fmaxf(de, 0.0f);
         d.aux[i] = de;
         float kOut;
@@ -869,9 +1005,7 @@ __global__ void __launch_bounds__(kBlock) kPciDensity(DevView d, int slot) {
 
 // computePressureAcc (PCISPHSolver.cpp:300-367) at current positions with
 // the predicted densities, then the next predictVelocityAndPosition.
-__global__ void __launch_bounds__(kBlock) kPciPressureAcc(DevView d, int slot) {
-    SCENE_PROLOGUE(slot)
-    if (!valid) return;
+__device__ __forceinline__ void pciPressureAcc(const DevView& d, const SceneConst& S, uint32_t i, float dt) {
     const size_t NP = d.NP;
     const float4 xi = d.x[i];
     const float rhoi = d.rho[i];
@@ -900,6 +1034,12 @@ SPH_UNROLL_LOOP
     const float vy = vi.y + ((ai.y + ay) + S.gy) * dt;
     const float vz = vi.z + ((ai.z + az) + S.gz) * dt;
     d.xs[i] = make_float4(xi.x + vx * dt, xi.y + vy * dt, xi.z + vz * dt, 0.0f);
+}
+__global__ void __launch_bounds__(kBlock) kPciPressureAcc(DevView d, int slot, int fin) {
+    SCENE_PROLOGUE(slot)
+    (void)dt2;
+    if (valid) pciPressureAcc(d, S, i, dt);
+    finishInline(d, fin, slot);
 }
 
 // PCISPHSolver::integrate (PCISPHSolver.cpp:459-488); block 0 files the
@@ -957,31 +1097,26 @@ __global__ void __launch_bounds__(kBlock) kCheckFinite(DevView d) {
 // density / PCISPH loop control (eta), divergence loop control (etaV / dt,
 // DFSPHSolver.cpp:746), or the CFL time step.
 __global__ void __launch_bounds__(kBlock) kFinish(DevView d, int kind, int slot) {
-    const int s = blockIdx.x;
-    const SceneConst& C = d.scenes[s];
-    const bool mx = kind == kFinishTimeStep;
-    double acc = 0.0;
-    for (uint32_t b = C.blockBegin + threadIdx.x; b < C.blockEnd; b += kBlock) {
-        const double p = d.partials[b];
-        acc = mx ? fmax(acc, p) : acc + p;
-    }
-    const double tot = mx ? blockReduce<true>(acc) : blockReduce<false>(acc);
-    if (threadIdx.x != 0) return;
-    SceneState& U = d.state[s];
-    const double mean = C.nf ? tot / C.nf : 0.0;
-    if (kind == kFinishDensity)
-        loopEpilogue(U, slot, mean, static_cast<double>(C.eta), C.minIt, C.maxIt);
-    else if (kind == kFinishDivergence)
-        loopEpilogue(U, slot, mean, static_cast<double>(C.etaV / U.dt), C.minItV, C.maxIt);
-    else
-        timeStepEpilogue(d, s, tot);
+    finishScene(d, blockIdx.x, kind, slot);
 }
+
 
 // ---------------------------------------------------------------- launchers
 #define LAUNCH(kernel, grid, ...)                                                     \
     do {                                                                              \
         hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, st, __VA_ARGS__);     \
         return hipGetLastError();                                                     \
+    } while (0)
+// Jacobi-pass launches may reserve unused dynamic LDS to cap how many
+// workgroups share a CU (and its 32 KB vector L1); 0 leaves occupancy to the
+// register count.
+#ifndef SPH_JACOBI_LDS
+#define SPH_JACOBI_LDS 0
+#endif
+#define LAUNCH_JACOBI(kernel, grid, ...)                                                         \
+    do {                                                                                         \
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), SPH_JACOBI_LDS, st, __VA_ARGS__);   \
+        return hipGetLastError();                                                                \
     } while (0)
 
 hipError_t launchKeys(const DevView& d, int adv, uint32_t* keys, uint32_t* vals, hipStream_t st) {
@@ -1011,18 +1146,21 @@ hipError_t launchSurfaceTension(const DevView& d, int vmax, hipStream_t st) {
 hipError_t launchVmax(const DevView& d, hipStream_t st) { LAUNCH(kVmax, d.numBlocks, d); }
 hipError_t launchPredictVelocity(const DevView& d, hipStream_t st) { LAUNCH(kPredictVelocity, d.numBlocks, d); }
 hipError_t launchRhoStar(const DevView& d, int warm, int slot, hipStream_t st) {
-    LAUNCH(kRhoStar, d.numBlocks, d, warm, slot);
+    LAUNCH_JACOBI(kRhoStar, d.numBlocks * kLpp, d, warm, slot);
 }
-hipError_t launchVelocity(const DevView& d, int slot, hipStream_t st) { LAUNCH(kVelocity, d.numBlocks, d, slot); }
+hipError_t launchVelocity(const DevView& d, int slot, int fin, hipStream_t st) {
+    if (fin < 0) LAUNCH_JACOBI(kVelocity, d.numBlocks * kLppVel, d, slot);
+    LAUNCH_JACOBI(kVelocityFinish, d.numBlocks * kLppVel, d, slot, fin);
+}
 hipError_t launchDivergence(const DevView& d, int warm, int slot, hipStream_t st) {
-    LAUNCH(kDivergence, d.numBlocks, d, warm, slot);
+    LAUNCH_JACOBI(kDivergence, d.numBlocks * kLpp, d, warm, slot);
 }
 hipError_t launchScaleDivergence(const DevView& d, hipStream_t st) { LAUNCH(kScaleDivergence, d.numBlocks, d); }
 hipError_t launchIntegrate(const DevView& d, hipStream_t st) { LAUNCH(kIntegrate, d.numBlocks, d); }
 hipError_t launchPciInit(const DevView& d, hipStream_t st) { LAUNCH(kPciInit, d.numBlocks, d); }
 hipError_t launchPciDensity(const DevView& d, int slot, hipStream_t st) { LAUNCH(kPciDensity, d.numBlocks, d, slot); }
-hipError_t launchPciPressureAcc(const DevView& d, int slot, hipStream_t st) {
-    LAUNCH(kPciPressureAcc, d.numBlocks, d, slot);
+hipError_t launchPciPressureAcc(const DevView& d, int slot, int fin, hipStream_t st) {
+    LAUNCH(kPciPressureAcc, d.numBlocks, d, slot, fin);
 }
 hipError_t launchPciIntegrate(const DevView& d, hipStream_t st) { LAUNCH(kPciIntegrate, d.numBlocks, d); }
 hipError_t launchFinish(const DevView& d, int kind, int slot, hipStream_t st) {
@@ -1033,14 +1171,22 @@ hipError_t launchCheckFinite(const DevView& d, hipStream_t st) { LAUNCH(kCheckFi
 
 }  // namespace sph
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 namespace sph {
 hipError_t sortPairs(void* temp, size_t& tempBytes, uint32_t* k0, uint32_t* k1, uint32_t* v0, uint32_t* v1, int n,
                      int bits, hipStream_t st, bool* swapped) {
-    hipcub::DoubleBuffer<uint32_t> kb(k0, k1), vb(v0, v1);
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, tempBytes, kb, vb, n, 0, bits, st);
-    if (swapped) *swapped = (kb.Current() == k1);
+    // rocprim picks a block sort + merge sort up to 2^20 keys and onesweep
+    // above; SPH_SORT_ONESWEEP=1 forces onesweep (measured: about 0.5% slower
+    // per step on the 1M-particle dam break: fewer but longer launches).
+#ifndef SPH_SORT_ONESWEEP
+#define SPH_SORT_ONESWEEP 0
+#endif
+    using Config = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                              rocprim::default_config, SPH_SORT_ONESWEEP ? 0 : (1u << 20)>;
+    rocprim::double_buffer<uint32_t> kb(k0, k1), vb(v0, v1);
+    hipError_t e = rocprim::radix_sort_pairs<Config>(temp, tempBytes, kb, vb, n, 0u, static_cast<unsigned>(bits), st);
+    if (swapped) *swapped = (kb.current() == k1);
     return e;
 }
 }  // namespace sph

@@ -65,20 +65,21 @@ hipError_t launchVmax(const DevView& d, hipStream_t st);
 hipError_t launchPredictVelocity(const DevView& d, hipStream_t st);
 // slot: iteration flag (0/1 by parity inside a Jacobi loop, 2 outside)
 hipError_t launchRhoStar(const DevView& d, int warm, int slot, hipStream_t st);
-hipError_t launchVelocity(const DevView& d, int slot, hipStream_t st);
+// fin >= 0: also finish the preceding Jacobi pass's reduction (a FinishKind)
+hipError_t launchVelocity(const DevView& d, int slot, int fin, hipStream_t st);
 hipError_t launchDivergence(const DevView& d, int warm, int slot, hipStream_t st);
 hipError_t launchScaleDivergence(const DevView& d, hipStream_t st);
 hipError_t launchIntegrate(const DevView& d, hipStream_t st);
 
 hipError_t launchPciInit(const DevView& d, hipStream_t st);
 hipError_t launchPciDensity(const DevView& d, int slot, hipStream_t st);
-hipError_t launchPciPressureAcc(const DevView& d, int slot, hipStream_t st);
+hipError_t launchPciPressureAcc(const DevView& d, int slot, int fin, hipStream_t st);
 hipError_t launchPciIntegrate(const DevView& d, hipStream_t st);
 
 // Reductions: the launches above that reduce (non-warm rho_star / divergence,
 // surface_tension with vmax, vmax, pci_density) leave block partials; the
 // caller launches kFinish with the matching kind next on the same stream.
-enum FinishKind { kFinishDensity = 0, kFinishDivergence = 1, kFinishTimeStep = 2 };
+enum FinishKind { kFinishDensity = 0, kFinishDivergence = 1, kFinishTimeStep = 2, kFinishPciDensity = 3 };
 hipError_t launchFinish(const DevView& d, int kind, int slot, hipStream_t st);
 
 hipError_t launchCellsOut(const DevView& d, int32_t* cells, hipStream_t st);
@@ -88,6 +89,8 @@ hipError_t launchCheckFinite(const DevView& d, hipStream_t st);
 
 // 1 if the kernels keep velocities interleaved with positions (xv), else 0.
 extern "C" int sphKernelsInterleaveVelocity();
+extern "C" int sphKernelsFuseFinish();
+extern "C" int sphKernelsLanesPerParticle();
 
 namespace sph {
 // Stable LSD radix sort of (key, value) pairs over the low `bits` key bits
