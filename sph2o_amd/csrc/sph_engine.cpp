@@ -171,6 +171,8 @@ class Engine {
     DevBuf<float> rho, alpha, p, k, kws, kwsAlt, kvws, kvwsAlt, aux, vb;
     DevBuf<uint32_t> id, idAlt, keys, keysAlt, vals, valsAlt;
     DevBuf<uint2> fRange, bRange;
+    DevBuf<uint32_t> cellOf;     // per slot: the cell range it opened in the last reorder, or ~0
+    bool rangesDirty = true;     // fRange holds ranges cellOf does not list: clear all of it
     DevBuf<uint32_t> nbrF, cntF, nbrB, cntB;
     DevBuf<double> partials;
     DevBuf<float4> snapV;   // DFSPH: v and kv_ws before the passes that follow a neighbour search
@@ -448,6 +450,7 @@ class Engine {
         HIPCHK(hipMemcpy(xb.p, xbs.data(), xbs.size() * sizeof(float4), hipMemcpyHostToDevice));
         vb.alloc(xbs.size());
         fRange.alloc(totalCells ? totalCells : 1);
+        rangesDirty = true;
         // Akinci volumes (AkinciBoundaryObject::computeVolume), once per layout
         DevView d = view();
         for (size_t i = 0; i < scenes.size(); ++i)
@@ -471,7 +474,8 @@ class Engine {
         d.rho = rho.p; d.alpha = alpha.p; d.p = p.p; d.k = k.p; d.kws = kws.p; d.kvws = kvws.p; d.aux = aux.p;
         d.id = id.p;
         d.xb = xb.p; d.vb = vb.p;
-        d.fRange = fRange.p; d.bRange = bRange.p;
+        d.fRange = fRange.p; d.bRange = bRange.p; d.cellOf = cellOf.p;
+        d.snapV = snapV.p; d.snapKvws = snapKvws.p;
         d.nbrF = nbrF.p; d.cntF = cntF.p; d.nbrB = nbrB.p; d.cntB = cntB.p;
         d.capF = capF; d.capB = capB;
         d.partials = partials.p; d.flags = dFlags;
@@ -531,7 +535,8 @@ class Engine {
             xvAlt.alloc(2 * np);
         }
         for (auto* b : {&rho, &alpha, &p, &k, &kws, &kwsAlt, &kvws, &kvwsAlt, &aux}) b->alloc(np);
-        for (auto* b : {&id, &idAlt, &keys, &keysAlt, &vals, &valsAlt, &cntF, &cntB}) b->alloc(np);
+        for (auto* b : {&id, &idAlt, &keys, &keysAlt, &vals, &valsAlt, &cntF, &cntB, &cellOf}) b->alloc(np);
+        HIPCHK(hipMemset(cellOf.p, 0xFF, np * sizeof(uint32_t)));
         allocNeighborLists();
         // the DFSPH source passes run sphKernelsLanesPerParticle() launch blocks per particle block
         partials.alloc(static_cast<size_t>(numBlocks ? numBlocks : 1) *
@@ -684,8 +689,10 @@ class Engine {
     void searchOnce(bool dfsphAdvance) {
         HIPCHK(hipMemsetAsync(dFlags, 0, scenes.size() * kFlagSlots * sizeof(int32_t), stream));
         DevView d = view();
-        timed(K_KEYS, [&] { return launchKeys(d, dfsphAdvance ? 1 : 0, keys.p, vals.p, stream); });
-        timed(K_FILL, [&] { return launchFillRanges(fRange.p, totalCells, stream); });
+        const bool fill = rangesDirty;
+        timed(K_KEYS, [&] { return launchKeys(d, dfsphAdvance ? 1 : 0, fill ? 0 : 1, keys.p, vals.p, stream); });
+        if (fill) timed(K_FILL, [&] { return launchFillRanges(fRange.p, totalCells, stream); });
+        rangesDirty = false;   // the reorder below lists every range it writes in cellOf
         bool swapped = false;
         timed(K_SORT, [&] {
             return sortPairs(sortTemp.p, sortTempBytes, keys.p, keysAlt.p, vals.p, valsAlt.p, static_cast<int>(NP),
@@ -905,8 +912,6 @@ class Engine {
         searchOnce(true);
         for (int attempt = 0;; ++attempt) {
             d = view();
-            copyVel(snapV.p, 1, interleaved ? xv.p + 1 : v.p, interleaved ? 2 : 1);
-            HIPCHK(hipMemcpyAsync(snapKvws.p, kvws.p, NP * sizeof(float), hipMemcpyDeviceToDevice, stream));
             // 8-9. computeDensities + computeAlpha
             timed(K_DENSITY, [&] { return launchDensityAlpha(d, 1, stream); });
             // 10. correctDivergenceError: warm start (and loop start), then the loop

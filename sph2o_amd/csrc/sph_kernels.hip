@@ -56,7 +56,19 @@ constexpr uint32_t kLpp = SPH_LPP;        // kRhoStar, kDivergence
 constexpr uint32_t kLppVel = SPH_LPP_VEL; // kVelocity
 static_assert(kLpp == 1 || kLpp == 2 || kLpp == 4 || kLpp == 8, "SPH_LPP must be 1, 2, 4 or 8");
 static_assert(kLppVel == 1 || kLppVel == 2 || kLppVel == 4 || kLppVel == 8, "SPH_LPP_VEL must be 1, 2, 4 or 8");
-extern "C" __host__ int sphKernelsLanesPerParticle() { return SPH_LPP; }
+// Lanes per particle of the once-per-step neighbour passes (kDensityAlpha,
+// kForces, kSurfaceTension; kVmax follows kSurfaceTension's partial layout).
+// Left at 1: 2 lanes save 1.8% of the 1M-particle step, but their changed
+// summation order flips one divergence-loop iteration count over the
+// 40-step DFSPH trajectory against the compiled reference (88 vs 87), which
+// moves its centre of mass by 3e-4 (1e-4 bar; 3e-5 for 1 lane).
+#ifndef SPH_LPP_ONCE
+#define SPH_LPP_ONCE 1
+#endif
+constexpr uint32_t kLppOnce = SPH_LPP_ONCE;
+static_assert(kLppOnce == 1 || kLppOnce == 2 || kLppOnce == 4 || kLppOnce == 8, "SPH_LPP_ONCE must be 1, 2, 4 or 8");
+// launch blocks per particle block of the widest partial-writing launch
+extern "C" __host__ int sphKernelsLanesPerParticle() { return static_cast<int>(kLpp > kLppOnce ? kLpp : kLppOnce); }
 
 #ifndef SPH_LIST_STAGE
 #define SPH_LIST_STAGE 24
@@ -176,6 +188,26 @@ __device__ __forceinline__ bool inCell(const SceneConst& S, float4 p, int gx, in
     const uint32_t i = lb * kBlock + threadIdx.x;             \
     const bool valid = T.active[slot] != 0 && i < S.fEnd;
 
+// Scene prologue of a launch with L lanes per particle (grid = L x the
+// particle blocks); `sub` is the lane's share of each neighbour list.
+#define SCENE_PROLOGUE_LANES(slot, L)                                                              \
+    const uint32_t lb = logicalBlock();                                                            \
+    const int sIdx = d.blockScene[lb / (L)];                                                       \
+    const SceneConst& S = d.scenes[sIdx];                                                          \
+    const SceneState& T = d.state[sIdx];                                                           \
+    const float dt = T.dt, dt2 = T.dt2;                                                            \
+    const uint32_t sub = (threadIdx.x & 63u) / (64u / (L));                                        \
+    const uint32_t i = lb * (kBlock / (L)) + (threadIdx.x >> 6) * (64u / (L)) + threadIdx.x % (64u / (L)); \
+    const bool valid = T.active[slot] != 0 && i < S.fEnd;
+
+// Sum over the L lanes of one particle (every lane gets the same value).
+template <uint32_t L>
+__device__ __forceinline__ float lppSum(float v) {
+#pragma unroll
+    for (uint32_t off = 64u / L; off < 64u; off <<= 1) v += __shfl_xor(v, static_cast<int>(off), 64);
+    return v;
+}
+
 // ---- FP64 deterministic reductions: block partials + per-scene kFinish ----
 // Block partial of a reduction, stored at the block's logical index with a
 // plain store. kFinish, launched next on the same stream, combines each
@@ -216,12 +248,17 @@ __device__ __forceinline__ void loopStart(SceneState& T, double err, double tol,
 // (DFSPHSolver::updatePositions :204-224 and :516-530; k_ws is cleared
 // instead when the density loop did not run), and block 0 files the density
 // loop's statistics.
-__global__ void __launch_bounds__(kBlock) kKeys(DevView d, int dfsphAdvance, uint32_t* keys, uint32_t* vals) {
+__global__ void __launch_bounds__(kBlock) kKeys(DevView d, int dfsphAdvance, int clearCells, uint32_t* keys,
+                                                uint32_t* vals) {
     const uint32_t lb = logicalBlock();
     const int sIdx = d.blockScene[lb];
     const SceneConst& S = d.scenes[sIdx];
     const SceneState& T = d.state[sIdx];
     const uint32_t i = lb * kBlock + threadIdx.x;
+    if (clearCells) {   // the first slot of each cell of the last reorder clears its range
+        const uint32_t oc = d.cellOf[i];
+        if (oc != ~0u) d.fRange[oc] = make_uint2(0u, 0u);
+    }
     if (dfsphAdvance && blockIdx.x == 0) {
         for (int s = threadIdx.x; s < d.numScenes; s += kBlock) {
             SceneState& U = d.state[s];
@@ -281,30 +318,65 @@ __global__ void __launch_bounds__(kBlock) kReorder(DevView d, const uint32_t* ke
 #else
     vAlt[i] = VEL(d, o);
 #endif
+    const float kv = d.kvws[o];
     kwsAlt[i] = d.kws[o];
-    kvwsAlt[i] = d.kvws[o];
+    kvwsAlt[i] = kv;
     idAlt[i] = d.id[o];
-    if (i >= S.fEnd) return;
-    const uint32_t key = keysSorted[i];
-    int gx, gy, gz;
-    globalCell(S, p, gx, gy, gz);
-    bool bucket;
-    const uint32_t c = cellSlot(S, gx, gy, gz, bucket);
-    if (i == S.fBegin || keysSorted[i - 1] != key) d.fRange[c].x = i;
-    if (i + 1 == S.fEnd || keysSorted[i + 1] != key) d.fRange[c].y = i + 1;
+    if (d.snapV) {
+        d.snapV[i] = VEL(d, o);
+        d.snapKvws[i] = kv;
+    }
+    uint32_t first = ~0u;
+    if (i < S.fEnd) {
+        const uint32_t key = keysSorted[i];
+        int gx, gy, gz;
+        globalCell(S, p, gx, gy, gz);
+        bool bucket;
+        const uint32_t c = cellSlot(S, gx, gy, gz, bucket);
+        if (i == S.fBegin || keysSorted[i - 1] != key) {
+            d.fRange[c].x = i;
+            first = c;
+        }
+        if (i + 1 == S.fEnd || keysSorted[i + 1] != key) d.fRange[c].y = i + 1;
+    }
+    d.cellOf[i] = first;
 }
 
 // Within-support neighbour lists from the 27-cell candidates (HashTable.cpp:23-86
 // walk order: dx, dy, dz with dz fastest), filtered by |r| <= h with the
-// reference's rounding (DFSPHSolver.cpp:575). Self is included.
-__global__ void __launch_bounds__(kBlock) kBuildNeighbors(DevView d) {
-    SCENE_PROLOGUE(2)
+// reference's rounding (DFSPHSolver.cpp:575). Self is included. L lanes share
+// a particle: each round they test L consecutive candidates of one cell and a
+// wave ballot appends the hits in walk order, so the lists are the same for
+// every L.
+template <uint32_t L>
+__device__ __forceinline__ void buildNeighbors(const DevView& d) {
+    SCENE_PROLOGUE_LANES(2, L)
+    (void)dt;
+    (void)dt2;
+    constexpr uint32_t P = 64u / L;         // particles per wave
+    constexpr uint32_t PB = kBlock / L;     // particles per block
+    const uint32_t p = threadIdx.x % P;
+    const uint32_t tp = (threadIdx.x >> 6) * P + p;
 #if SPH_LIST_STAGE
-    // a lane finds its s-th neighbour at its own pace, so direct stores of one
-    // wave land in 64 different rows; staged rows leave as coalesced lines
-    __shared__ uint32_t sList[SPH_LIST_STAGE * kBlock];
+    // a particle finds its s-th neighbour at its own pace, so direct stores of
+    // one wave land in many different rows; staged rows leave as coalesced lines
+    __shared__ uint32_t sList[SPH_LIST_STAGE * PB];
+    __shared__ uint32_t sCnt[PB];
 #endif
     uint32_t nf = 0, nb = 0;
+    // lanes p, p + P, ... of the wave hold one particle; `before` counts the
+    // hits of its lanes below this one, `total` all of them
+    auto place = [&](bool hit, uint32_t& before, uint32_t& total) {
+        const unsigned long long m = __ballot(hit);
+        before = 0;
+        total = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < L; ++k) {
+            const uint32_t bit = static_cast<uint32_t>((m >> (p + k * P)) & 1ull);
+            before += k < sub ? bit : 0u;
+            total += bit;
+        }
+    };
     if (valid) {
         const float4 xi = d.x[i];
         int cx, cy, cz;
@@ -317,32 +389,50 @@ __global__ void __launch_bounds__(kBlock) kBuildNeighbors(DevView d) {
                     bool bucket;
                     const uint32_t c = cellSlot(S, ax, ay, az, bucket);
                     const uint2 fr = d.fRange[c];
-                    for (uint32_t j = fr.x; j < fr.y; ++j) {
-                        const float4 xj = d.x[j];
-                        if (bucket && !inCell(S, xj, ax, ay, az)) continue;
-                        if (dRef(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) <= S.dmax) {
+                    for (uint32_t base = fr.x; base < fr.y; base += L) {
+                        const uint32_t j = base + sub;
+                        bool hit = false;
+                        if (j < fr.y) {
+                            const float4 xj = d.x[j];
+                            hit = (!bucket || inCell(S, xj, ax, ay, az)) &&
+                                  dRef(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) <= S.dmax;
+                        }
+                        uint32_t before, total;
+                        place(hit, before, total);
+                        if (hit) {
+                            const uint32_t pos = nf + before;
 #if SPH_LIST_STAGE
-                            if (nf < SPH_LIST_STAGE) sList[nf * kBlock + threadIdx.x] = j;
+                            if (pos < SPH_LIST_STAGE) sList[pos * PB + tp] = j;
                             else
 #endif
-                            if (nf < d.capF) d.nbrF[nf * NP + i] = j;
-                            ++nf;
+                            if (pos < d.capF) d.nbrF[pos * NP + i] = j;
                         }
+                        nf += total;
                     }
                     if (S.akinci && !bucket) {   // boundary particles all lie in the dense box
                         const uint2 br = d.bRange[c];
-                        for (uint32_t b = br.x; b < br.y; ++b) {
-                            const float4 xb = d.xb[b];
-                            if (dRef(xi.x - xb.x, xi.y - xb.y, xi.z - xb.z) <= S.dmax) {
-                                if (nb < d.capB) d.nbrB[nb * NP + i] = b;
-                                ++nb;
+                        for (uint32_t base = br.x; base < br.y; base += L) {
+                            const uint32_t b = base + sub;
+                            bool hit = false;
+                            if (b < br.y) {
+                                const float4 xb = d.xb[b];
+                                hit = dRef(xi.x - xb.x, xi.y - xb.y, xi.z - xb.z) <= S.dmax;
                             }
+                            uint32_t before, total;
+                            place(hit, before, total);
+                            if (hit && nb + before < d.capB) d.nbrB[(nb + before) * NP + i] = b;
+                            nb += total;
                         }
                     }
                 }
-        d.cntF[i] = nf;
-        d.cntB[i] = nb;
+        if (sub == 0) {
+            d.cntF[i] = nf;
+            d.cntB[i] = nb;
+        }
     }
+#if SPH_LIST_STAGE
+    if (sub == 0) sCnt[tp] = nf;
+#endif
     // block-wide max, one atomic per block
     int mf = static_cast<int>(nf), mb = static_cast<int>(nb);
     for (int off = 32; off > 0; off >>= 1) {
@@ -365,17 +455,28 @@ __global__ void __launch_bounds__(kBlock) kBuildNeighbors(DevView d) {
         atomicMax(&d.flags[sIdx * kFlagSlots + kFlagMaxBoundaryNbr], mb);
     }
 #if SPH_LIST_STAGE
-    // whole rows up to the block's longest list; entries past a lane's own
+    // whole rows up to the block's longest list; entries past a particle's own
     // count get ~0 (never read: every pass stops at min(count, capacity))
     int rows = smf[0];
     for (int w = 1; w < kBlock / 64; ++w) rows = max(rows, smf[w]);
     rows = min(rows, min(static_cast<int>(d.capF), SPH_LIST_STAGE));
     const size_t NPs = d.NP;
-    const uint32_t row0 = lb * kBlock + threadIdx.x;   // < NP: the grid covers the padded slots exactly
-    for (int s = 0; s < rows; ++s)
-        d.nbrF[s * NPs + row0] = static_cast<uint32_t>(s) < nf ? sList[s * kBlock + threadIdx.x] : ~0u;
+    const uint32_t first = lb * PB;   // the block's particles: [first, first + PB) < NP
+    for (uint32_t e = threadIdx.x; e < static_cast<uint32_t>(rows) * PB; e += kBlock) {
+        const uint32_t row = e / PB, t = e % PB;
+        d.nbrF[row * NPs + first + t] = row < sCnt[t] ? sList[row * PB + t] : ~0u;
+    }
 #endif
 }
+// Lanes per particle of the search (same lists for every value). Measured on
+// the 1M-particle dam break: 1 lane 293 us per build, 2 lanes 336, 4 lanes 479
+// (cells hold ~8 candidates, so rounds of 2-4 add ballots without gathering
+// any better: one lane already walks each cell's contiguous slots).
+#ifndef SPH_LPP_BUILD
+#define SPH_LPP_BUILD 1
+#endif
+constexpr uint32_t kLppBuild = SPH_LPP_BUILD;
+__global__ void __launch_bounds__(kBlock) kBuildNeighbors(DevView d) { buildNeighbors<kLppBuild>(d); }
 
 // AkinciBoundaryObject::computeVolume (AkinciBoundaryObject.cpp:10-34):
 // V_b = 1 / sum of W over the boundary candidates of the 27 cells.
@@ -407,33 +508,42 @@ __global__ void kBoundaryVolume(DevView d, int scene, uint32_t bBegin, uint32_t 
 // SPHSolver::computeFluidDensities (SPHSolver.cpp:42-89) fused with
 // DFSPHSolver::computeAlpha (DFSPHSolver.cpp:109-178).
 __global__ void __launch_bounds__(kBlock) kDensityAlpha(DevView d, int withAlpha) {
-    SCENE_PROLOGUE(2)
-    if (!valid) return;
+    SCENE_PROLOGUE_LANES(2, kLppOnce)
+    (void)dt;
+    (void)dt2;
     const size_t NP = d.NP;
-    const float4 xi = d.x[i];
     float rho = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, ssq = 0.0f;
-    const uint32_t nf = min(d.cntF[i], d.capF);
+    if (valid) {
+        const float4 xi = d.x[i];
+        const uint32_t nf = min(d.cntF[i], d.capF);
 SPH_UNROLL_LOOP
-    for (uint32_t s = 0; s < nf; ++s) {
-        const uint32_t j = ldList(d.nbrF + s * NP + i);
-        const float4 xj = d.x[j];
-        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        const float rl = fastSqrt(rx * rx + ry * ry + rz * rz);
-        rho += S.m * cubicW(S, rl);
-        const float g = S.m * gradFactor(S, rl);
-        const float gx = g * rx, gy = g * ry, gz = g * rz;
-        sgx += gx; sgy += gy; sgz += gz;
-        ssq += gx * gx + gy * gy + gz * gz;
+        for (uint32_t s = sub; s < nf; s += kLppOnce) {
+            const uint32_t j = ldList(d.nbrF + s * NP + i);
+            const float4 xj = d.x[j];
+            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            const float rl = fastSqrt(rx * rx + ry * ry + rz * rz);
+            rho += S.m * cubicW(S, rl);
+            const float g = S.m * gradFactor(S, rl);
+            const float gx = g * rx, gy = g * ry, gz = g * rz;
+            sgx += gx; sgy += gy; sgz += gz;
+            ssq += gx * gx + gy * gy + gz * gz;
+        }
+        const uint32_t nb = S.akinci ? min(d.cntB[i], d.capB) : 0u;
+        for (uint32_t s = sub; s < nb; s += kLppOnce) {
+            const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
+            const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
+            const float rl = fastSqrt(rx * rx + ry * ry + rz * rz);
+            rho += xb.w * cubicW(S, rl);
+            const float g = xb.w * gradFactor(S, rl);
+            sgx += g * rx; sgy += g * ry; sgz += g * rz;
+        }
     }
-    const uint32_t nb = S.akinci ? min(d.cntB[i], d.capB) : 0u;
-    for (uint32_t s = 0; s < nb; ++s) {
-        const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
-        const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
-        const float rl = fastSqrt(rx * rx + ry * ry + rz * rz);
-        rho += xb.w * cubicW(S, rl);
-        const float g = xb.w * gradFactor(S, rl);
-        sgx += g * rx; sgy += g * ry; sgz += g * rz;
-    }
+    rho = lppSum<kLppOnce>(rho);
+    sgx = lppSum<kLppOnce>(sgx);
+    sgy = lppSum<kLppOnce>(sgy);
+    sgz = lppSum<kLppOnce>(sgz);
+    ssq = lppSum<kLppOnce>(ssq);
+    if (!valid || sub != 0) return;
     d.rho[i] = rho;
     if (withAlpha) {
         const float den = sgx * sgx + sgy * sgy + sgz * sgz + ssq;
@@ -473,21 +583,24 @@ SPH_UNROLL_LOOP
 //   surface-tension normals (SurfaceTensionAkinci.cpp:13-33).
 // The acceleration is written fresh (it is zero at this point of every step).
 __global__ void __launch_bounds__(kBlock) kForces(DevView d, int pressure) {
-    SCENE_PROLOGUE(2)
-    if (!valid) return;
+    SCENE_PROLOGUE_LANES(2, kLppOnce)
+    (void)dt2;
     const size_t NP = d.NP;
-    const float4 xi = d.x[i];
-    const float4 vi = VEL(d, i);
-    const float rhoi = d.rho[i];
-    const uint32_t nf = min(d.cntF[i], d.capF);
-    const uint32_t nb = S.akinci ? min(d.cntB[i], d.capB) : 0u;
+    // lanes of one particle take list slots sub, sub + kLppOnce, ...; invalid
+    // lanes run empty loops so that every lane reaches the cross-lane sums
+    const uint32_t ii = valid ? i : 0u;
+    const float4 xi = d.x[ii];
+    const float4 vi = VEL(d, ii);
+    const float rhoi = d.rho[ii];
+    const uint32_t nf = valid ? min(d.cntF[i], d.capF) : 0u;
+    const uint32_t nb = (valid && S.akinci) ? min(d.cntB[i], d.capB) : 0u;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     if (pressure) {
         const float pi_ = d.p[i];
         const float dpi = pi_ / (rhoi * rhoi);
         const float dpb = pi_ / (S.rho0 * S.rho0);
 SPH_UNROLL_LOOP
-        for (uint32_t s = 0; s < nf; ++s) {
+        for (uint32_t s = sub; s < nf; s += kLppOnce) {
             const uint32_t j = ldList(d.nbrF + s * NP + i);
             const float4 xj = POSV(d, j);
             const float rhoj = d.rho[j];
@@ -495,12 +608,15 @@ SPH_UNROLL_LOOP
             const float g = S.m * (dpi + d.p[j] / (rhoj * rhoj)) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
             ax -= g * rx; ay -= g * ry; az -= g * rz;
         }
-        for (uint32_t s = 0; s < nb; ++s) {
+        for (uint32_t s = sub; s < nb; s += kLppOnce) {
             const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
             const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
             const float g = xb.w * (dpi + dpb) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
             ax -= g * rx; ay -= g * ry; az -= g * rz;
         }
+        ax = lppSum<kLppOnce>(ax);
+        ay = lppSum<kLppOnce>(ay);
+        az = lppSum<kLppOnce>(az);
     }
     const int vm = S.viscMethod;
     const bool normals = S.stOn;
@@ -509,7 +625,7 @@ SPH_UNROLL_LOOP
         float nx = 0.0f, ny = 0.0f, nz = 0.0f;   // normal sum
         const float invdt = 1.0f / dt;
 SPH_UNROLL_LOOP
-        for (uint32_t s = 0; s < nf; ++s) {
+        for (uint32_t s = sub; s < nf; s += kLppOnce) {
             const uint32_t j = ldList(d.nbrF + s * NP + i);
             const float4 xj = POSV(d, j);
             const float rhoj = d.rho[j];
@@ -535,6 +651,9 @@ SPH_UNROLL_LOOP
                 nx += c * rx; ny += c * ry; nz += c * rz;
             }
         }
+        fx = lppSum<kLppOnce>(fx);
+        fy = lppSum<kLppOnce>(fy);
+        fz = lppSum<kLppOnce>(fz);
         if (vm == 0) {
             const float c = S.visc / rhoi;
             fx *= c; fy *= c; fz *= c;
@@ -542,7 +661,7 @@ SPH_UNROLL_LOOP
         ax += fx; ay += fy; az += fz;
         if (vm >= 0 && S.bviscOn) {
             float bx = 0.0f, by = 0.0f, bz = 0.0f;
-            for (uint32_t s = 0; s < nb; ++s) {
+            for (uint32_t s = sub; s < nb; s += kLppOnce) {
                 const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
                 const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
                 const float r2 = rx * rx + ry * ry + rz * rz;
@@ -559,15 +678,23 @@ SPH_UNROLL_LOOP
                     bx -= c * vi.x; by -= c * vi.y; bz -= c * vi.z;
                 }
             }
+            bx = lppSum<kLppOnce>(bx);
+            by = lppSum<kLppOnce>(by);
+            bz = lppSum<kLppOnce>(bz);
             if (vm == 0) {
                 const float c = S.bvisc / rhoi;
                 bx *= c; by *= c; bz *= c;
             }
             ax += bx; ay += by; az += bz;
         }
-        if (normals) d.n[i] = make_float4(nx * S.h, ny * S.h, nz * S.h, rhoi);
+        if (normals) {
+            nx = lppSum<kLppOnce>(nx);
+            ny = lppSum<kLppOnce>(ny);
+            nz = lppSum<kLppOnce>(nz);
+            if (valid && sub == 0) d.n[i] = make_float4(nx * S.h, ny * S.h, nz * S.h, rhoi);
+        }
     }
-    d.a[i] = make_float4(ax, ay, az, 0.0f);
+    if (valid && sub == 0) d.a[i] = make_float4(ax, ay, az, 0.0f);
 }
 
 // DFSPHSolver::updateTimeStep (DFSPHSolver.cpp:1090-1162) on the FP64 maximum
@@ -595,49 +722,57 @@ __device__ __forceinline__ void timeStepEpilogue(const DevView& d, int s, double
 // SurfaceTensionAkinci::step pass 2 (SurfaceTensionAkinci.cpp:35-70).
 // With vmax: the CFL maximum and the new time step (DFSPH).
 __global__ void __launch_bounds__(kBlock) kSurfaceTension(DevView d, int vmax) {
-    SCENE_PROLOGUE(2)
+    SCENE_PROLOGUE_LANES(2, kLppOnce)
+    (void)dt2;
     double vm = 0.0;
-    if (valid) {
+    const uint32_t ii = valid ? i : 0u;
+    float4 ai = d.a[ii];
+    if (S.stOn) {
         const size_t NP = d.NP;
-        float4 ai = d.a[i];
-        if (S.stOn) {
-            const float4 xi = d.x[i];
-            const float4 ni = d.n[i];
-            const float rhoi = ni.w;
-            const uint32_t nf = min(d.cntF[i], d.capF);
+        const float4 xi = d.x[ii];
+        const float4 ni = d.n[ii];
+        const float rhoi = ni.w;
+        const uint32_t nf = valid ? min(d.cntF[i], d.capF) : 0u;
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
 SPH_UNROLL_LOOP
-            for (uint32_t s = 0; s < nf; ++s) {
-                const uint32_t j = ldList(d.nbrF + s * NP + i);
-                const float4 xj = d.x[j];
-                const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                const float dd = dRef(rx, ry, rz);
-                if (dd > 0.0f && dd <= S.dmax) {
-                    const float rl = fastSqrt(dd);
-                    const float4 nj = d.n[j];   // w = density
-                    const float kij = 2.0f * S.rho0 / (rhoi + nj.w);
-                    const float c = S.st * S.m * cohesionW(S, dd, rl) / rl;
-                    ai.x += kij * (-c * rx - S.st * (ni.x - nj.x));
-                    ai.y += kij * (-c * ry - S.st * (ni.y - nj.y));
-                    ai.z += kij * (-c * rz - S.st * (ni.z - nj.z));
-                }
+        for (uint32_t s = sub; s < nf; s += kLppOnce) {
+            const uint32_t j = ldList(d.nbrF + s * NP + i);
+            const float4 xj = d.x[j];
+            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            const float dd = dRef(rx, ry, rz);
+            if (dd > 0.0f && dd <= S.dmax) {
+                const float rl = fastSqrt(dd);
+                const float4 nj = d.n[j];   // w = density
+                const float kij = 2.0f * S.rho0 / (rhoi + nj.w);
+                const float c = S.st * S.m * cohesionW(S, dd, rl) / rl;
+                sx += kij * (-c * rx - S.st * (ni.x - nj.x));
+                sy += kij * (-c * ry - S.st * (ni.y - nj.y));
+                sz += kij * (-c * rz - S.st * (ni.z - nj.z));
             }
-            d.a[i] = ai;
         }
-        if (vmax) {
-            const float4 vi = VEL(d, i);
-            const float cx = vi.x + (ai.x + S.gx) * dt;
-            const float cy = vi.y + (ai.y + S.gy) * dt;
-            const float cz = vi.z + (ai.z + S.gz) * dt;
-            vm = sqrt(static_cast<double>(cx) * cx + static_cast<double>(cy) * cy + static_cast<double>(cz) * cz);
-        }
+        sx = lppSum<kLppOnce>(sx);
+        sy = lppSum<kLppOnce>(sy);
+        sz = lppSum<kLppOnce>(sz);
+        ai.x += sx;
+        ai.y += sy;
+        ai.z += sz;
+        if (valid && sub == 0) d.a[i] = ai;
+    }
+    if (vmax && valid && sub == 0) {
+        const float4 vi = VEL(d, i);
+        const float cx = vi.x + (ai.x + S.gx) * dt;
+        const float cy = vi.y + (ai.y + S.gy) * dt;
+        const float cz = vi.z + (ai.z + S.gz) * dt;
+        vm = sqrt(static_cast<double>(cx) * cx + static_cast<double>(cy) * cy + static_cast<double>(cz) * cz);
     }
     if (vmax) storePartial<true>(d, vm);
 }
 
 __global__ void __launch_bounds__(kBlock) kVmax(DevView d) {
-    SCENE_PROLOGUE(2)
+    SCENE_PROLOGUE_LANES(2, kLppOnce)   // same partial layout as kSurfaceTension
+    (void)dt2;
     double vm = 0.0;
-    if (valid) {
+    if (valid && sub == 0) {
         const float4 vi = VEL(d, i), ai = d.a[i];
         const float cx = vi.x + (ai.x + S.gx) * dt;
         const float cy = vi.y + (ai.y + S.gy) * dt;
@@ -653,8 +788,11 @@ __global__ void __launch_bounds__(kBlock) kVmax(DevView d) {
 __device__ void finishScene(const DevView& d, int s, int kind, int slot) {
     const SceneConst& C = d.scenes[s];
     const bool mx = kind == kFinishTimeStep;
-    // the DFSPH source passes write one partial per launch block, kLpp per particle block
-    const uint32_t mul = (kind == kFinishDensity || kind == kFinishDivergence) ? kLpp : 1u;
+    // one partial per launch block: kLpp (DFSPH source passes) or kLppOnce
+    // (vmax of kSurfaceTension / kVmax) launch blocks per particle block
+    const uint32_t mul = (kind == kFinishDensity || kind == kFinishDivergence) ? kLpp
+                         : kind == kFinishTimeStep                             ? kLppOnce
+                                                                               : 1u;
     const uint32_t bBegin = C.blockBegin * mul, bEnd = C.blockEnd * mul;
     // each thread combines partials threadIdx.x, +256, +512, ... in that order;
     // eight loads are in flight before the first is combined
@@ -707,26 +845,6 @@ __global__ void __launch_bounds__(kBlock) kPredictVelocity(DevView d) {
 }
 
 // ---------------------------------------------------------------- DFSPH solves
-// Scene prologue of a launch with L lanes per particle (grid = L x the
-// particle blocks); `sub` is the lane's share of each neighbour list.
-#define SCENE_PROLOGUE_LANES(slot, L)                                                              \
-    const uint32_t lb = logicalBlock();                                                            \
-    const int sIdx = d.blockScene[lb / (L)];                                                       \
-    const SceneConst& S = d.scenes[sIdx];                                                          \
-    const SceneState& T = d.state[sIdx];                                                           \
-    const float dt = T.dt, dt2 = T.dt2;                                                            \
-    const uint32_t sub = (threadIdx.x & 63u) / (64u / (L));                                        \
-    const uint32_t i = lb * (kBlock / (L)) + (threadIdx.x >> 6) * (64u / (L)) + threadIdx.x % (64u / (L)); \
-    const bool valid = T.active[slot] != 0 && i < S.fEnd;
-
-// Sum over the L lanes of one particle (every lane gets the same value).
-template <uint32_t L>
-__device__ __forceinline__ float lppSum(float v) {
-#pragma unroll
-    for (uint32_t off = 64u / L; off < 64u; off <<= 1) v += __shfl_xor(v, static_cast<int>(off), 64);
-    return v;
-}
-
 // This lane's share of sum_j m g_ij (v_i - v_j) . r_ij + sum_b rho0 V_b g_ib v_i . r_ib,
 // the density change rate of predictDensities (DFSPHSolver.cpp:250-283) and
 // computeDivergenceError (:560-606).
@@ -1119,8 +1237,8 @@ __global__ void __launch_bounds__(kBlock) kFinish(DevView d, int kind, int slot)
         return hipGetLastError();                                                                \
     } while (0)
 
-hipError_t launchKeys(const DevView& d, int adv, uint32_t* keys, uint32_t* vals, hipStream_t st) {
-    LAUNCH(kKeys, d.numBlocks, d, adv, keys, vals);
+hipError_t launchKeys(const DevView& d, int adv, int clearCells, uint32_t* keys, uint32_t* vals, hipStream_t st) {
+    LAUNCH(kKeys, d.numBlocks, d, adv, clearCells, keys, vals);
 }
 hipError_t launchFillRanges(uint2* r, size_t n, hipStream_t st) {
     const size_t g = (n + kBlock - 1) / kBlock;
@@ -1131,19 +1249,21 @@ hipError_t launchReorder(const DevView& d, const uint32_t* ks, const uint32_t* v
                          float* kwa, float* kva, uint32_t* ida, hipStream_t st) {
     LAUNCH(kReorder, d.numBlocks, d, ks, vs, xa, va, xva, kwa, kva, ida);
 }
-hipError_t launchBuildNeighbors(const DevView& d, hipStream_t st) { LAUNCH(kBuildNeighbors, d.numBlocks, d); }
+hipError_t launchBuildNeighbors(const DevView& d, hipStream_t st) {
+    LAUNCH(kBuildNeighbors, d.numBlocks * kLppBuild, d);
+}
 hipError_t launchBoundaryVolume(const DevView& d, int scene, uint32_t b0, uint32_t b1, hipStream_t st) {
     const uint32_t g = (b1 - b0 + kBlock - 1) / kBlock;
     if (g == 0) return hipSuccess;
     LAUNCH(kBoundaryVolume, g, d, scene, b0, b1);
 }
-hipError_t launchDensityAlpha(const DevView& d, int wa, hipStream_t st) { LAUNCH(kDensityAlpha, d.numBlocks, d, wa); }
+hipError_t launchDensityAlpha(const DevView& d, int wa, hipStream_t st) { LAUNCH(kDensityAlpha, d.numBlocks * kLppOnce, d, wa); }
 hipError_t launchDensityPressure(const DevView& d, hipStream_t st) { LAUNCH(kDensityPressure, d.numBlocks, d); }
-hipError_t launchForces(const DevView& d, int pressure, hipStream_t st) { LAUNCH(kForces, d.numBlocks, d, pressure); }
+hipError_t launchForces(const DevView& d, int pressure, hipStream_t st) { LAUNCH(kForces, d.numBlocks * kLppOnce, d, pressure); }
 hipError_t launchSurfaceTension(const DevView& d, int vmax, hipStream_t st) {
-    LAUNCH(kSurfaceTension, d.numBlocks, d, vmax);
+    LAUNCH(kSurfaceTension, d.numBlocks * kLppOnce, d, vmax);
 }
-hipError_t launchVmax(const DevView& d, hipStream_t st) { LAUNCH(kVmax, d.numBlocks, d); }
+hipError_t launchVmax(const DevView& d, hipStream_t st) { LAUNCH(kVmax, d.numBlocks * kLppOnce, d); }
 hipError_t launchPredictVelocity(const DevView& d, hipStream_t st) { LAUNCH(kPredictVelocity, d.numBlocks, d); }
 hipError_t launchRhoStar(const DevView& d, int warm, int slot, hipStream_t st) {
     LAUNCH_JACOBI(kRhoStar, d.numBlocks * kLpp, d, warm, slot);

@@ -39,6 +39,11 @@ struct DevView {
     // grid
     uint2* fRange;
     const uint2* bRange;
+    uint32_t* cellOf;   // slot -> its cell's fRange index if it is the cell's first slot, else ~0
+    // DFSPH: copy of v and kv_ws as the reorder leaves them (undo of a divergence
+    // solve run on lists that turn out invalid); null for other methods
+    float4* snapV;
+    float* snapKvws;
     // neighbour lists, column-major: list[slot * NP + i]
     uint32_t* nbrF;
     uint32_t* cntF;
@@ -50,7 +55,10 @@ struct DevView {
     int32_t* flags;     // numScenes * kFlagSlots
 };
 
-hipError_t launchKeys(const DevView& d, int dfsphAdvance, uint32_t* keys, uint32_t* vals, hipStream_t st);
+// clearCells: first zero the ranges the last reorder wrote (d.cellOf), so no
+// separate fill of the whole range table is needed
+hipError_t launchKeys(const DevView& d, int dfsphAdvance, int clearCells, uint32_t* keys, uint32_t* vals,
+                      hipStream_t st);
 hipError_t launchFillRanges(uint2* r, size_t n, hipStream_t st);
 hipError_t launchReorder(const DevView& d, const uint32_t* keysSorted, const uint32_t* valsSorted, float4* xAlt,
                          float4* vAlt, float4* xvAlt, float* kwsAlt, float* kvwsAlt, uint32_t* idAlt, hipStream_t st);
