@@ -71,6 +71,28 @@ __device__ __forceinline__ float cohesionDeriv(const SceneConst& S, float d, flo
     const uint32_t i = lb * kBlock + threadIdx.x; \
     const bool valid = i < S.fEnd;
 
+// Lanes per particle of the adjoint Jacobi passes (kAdjVelocity, kAdjSource):
+// as in the forward passes, the L lanes of a particle split its neighbour list
+// by slot residue and their sums are combined across lanes.
+#ifndef SPH_ADJ_LPP
+#define SPH_ADJ_LPP 4
+#endif
+constexpr uint32_t kAdjLpp = SPH_ADJ_LPP;
+static_assert(kAdjLpp == 1 || kAdjLpp == 2 || kAdjLpp == 4 || kAdjLpp == 8, "SPH_ADJ_LPP must be 1, 2, 4 or 8");
+#define ADJ_PROLOGUE_LANES(L)                                                                      \
+    const uint32_t lb = logicalBlock();                                                            \
+    const int sIdx = a.blockScene[lb / (L)];                                                       \
+    const SceneConst& S = a.scenes[sIdx];                                                          \
+    const float dt = a.state[sIdx].dt;                                                             \
+    const uint32_t sub = (threadIdx.x & 63u) / (64u / (L));                                        \
+    const uint32_t i = lb * (kBlock / (L)) + (threadIdx.x >> 6) * (64u / (L)) + threadIdx.x % (64u / (L)); \
+    const bool valid = i < S.fEnd;
+__device__ __forceinline__ void lanesSum3(F3& v) {
+    v.x = lppSum<kAdjLpp>(v.x);
+    v.y = lppSum<kAdjLpp>(v.y);
+    v.z = lppSum<kAdjLpp>(v.z);
+}
+
 // Adjoint of the symplectic Euler update: v-tilde = v'-bar + dt x'-bar is the
 // adjoint of v'; a-bar = dt v-tilde; x-bar and v-bar start from x'-bar and
 // v-tilde. Reads the adjoints in original order, writes slot order.
@@ -346,15 +368,15 @@ __global__ void __launch_bounds__(kBlock) kAdjHalfEnd(AdjView a, int mode) {
 // rho) / k(d, alpha, rho) to the source adjoint sBar; warm passes give the
 // adjoint of the k_ws / kv_ws that entered the step.
 __global__ void __launch_bounds__(kBlock) kAdjVelocity(AdjView a, int mode) {
-    ADJ_PROLOGUE
-    if (!valid) return;
+    ADJ_PROLOGUE_LANES(kAdjLpp)
     const size_t NP = a.NP;
-    const F3 xi = f3(a.x[i]), ui = f3(a.vBar[i]);
-    const float ki = a.kk[i];
+    const uint32_t ii = valid ? i : 0u;   // invalid lanes run empty loops up to the lane sums
+    const F3 xi = f3(a.x[ii]), ui = f3(a.vBar[ii]);
+    const float ki = a.kk[ii];
     F3 gx{0.0f, 0.0f, 0.0f};
     float kb = 0.0f;
-    const uint32_t nf = min(a.cntF[i], a.capF);
-    for (uint32_t s = 0; s < nf; ++s) {
+    const uint32_t nf = valid ? min(a.cntF[i], a.capF) : 0u;
+    for (uint32_t s = sub; s < nf; s += kAdjLpp) {
         const uint32_t j = a.nbrF[s * NP + i];
         const float ks = ki + a.kk[j];
         if (fabsf(ks) > S.eps) {
@@ -367,9 +389,9 @@ __global__ void __launch_bounds__(kBlock) kAdjVelocity(AdjView a, int mode) {
             axpy(gx, c * G2 * rdu, r);
         }
     }
-    if (S.akinci && fabsf(ki) > S.eps) {
+    if (valid && S.akinci && fabsf(ki) > S.eps) {
         const uint32_t nb = min(a.cntB[i], a.capB);
-        for (uint32_t s = 0; s < nb; ++s) {
+        for (uint32_t s = sub; s < nb; s += kAdjLpp) {
             const float4 q = a.xb[a.nbrB[s * NP + i]];
             const F3 r = xi - f3(q);
             const float rl = fastSqrt(dot3(r, r));
@@ -381,6 +403,9 @@ __global__ void __launch_bounds__(kBlock) kAdjVelocity(AdjView a, int mode) {
             axpy(gx, c * G2 * rui, r);
         }
     }
+    lanesSum3(gx);
+    kb = lppSum<kAdjLpp>(kb);
+    if (!valid || sub != 0) return;
     const float4 x0 = a.xBar[i];
     a.xBar[i] = make_float4(x0.x + gx.x, x0.y + gx.y, x0.z + gx.z, 0.0f);
     const uint32_t o = S.fBegin + a.id[i];
@@ -407,15 +432,15 @@ __global__ void __launch_bounds__(kBlock) kAdjVelocity(AdjView a, int mode) {
 // Adjoint of a source pass (rho*: c = dt, divergence: c = 1) given its
 // adjoint sBar: into u-bar (vBar) and x-bar, with u the velocity it read.
 __global__ void __launch_bounds__(kBlock) kAdjSource(AdjView a, int mode) {
-    ADJ_PROLOGUE
-    if (!valid) return;
+    ADJ_PROLOGUE_LANES(kAdjLpp)
     const size_t NP = a.NP;
     const float c = mode == kAdjDensityLoop ? dt : 1.0f;
-    const F3 xi = f3(a.x[i]), ui = f3(a.u[i]);
-    const float si = a.sBar[i];
+    const uint32_t ii = valid ? i : 0u;   // invalid lanes run empty loops up to the lane sums
+    const F3 xi = f3(a.x[ii]), ui = f3(a.u[ii]);
+    const float si = a.sBar[ii];
     F3 gx{0.0f, 0.0f, 0.0f}, gu{0.0f, 0.0f, 0.0f};
-    const uint32_t nf = min(a.cntF[i], a.capF);
-    for (uint32_t s = 0; s < nf; ++s) {
+    const uint32_t nf = valid ? min(a.cntF[i], a.capF) : 0u;
+    for (uint32_t s = sub; s < nf; s += kAdjLpp) {
         const uint32_t j = a.nbrF[s * NP + i];
         const float ss = si + a.sBar[j];
         if (ss == 0.0f) continue;
@@ -426,9 +451,9 @@ __global__ void __launch_bounds__(kBlock) kAdjSource(AdjView a, int mode) {
         axpy(gx, w * G, du);
         axpy(gx, w * G2 * dot3(r, du), r);
     }
-    if (S.akinci && si != 0.0f) {
+    if (valid && S.akinci && si != 0.0f) {
         const uint32_t nb = min(a.cntB[i], a.capB);
-        for (uint32_t s = 0; s < nb; ++s) {
+        for (uint32_t s = sub; s < nb; s += kAdjLpp) {
             const float4 q = a.xb[a.nbrB[s * NP + i]];
             const F3 r = xi - f3(q);
             const float rl = fastSqrt(dot3(r, r));
@@ -439,6 +464,9 @@ __global__ void __launch_bounds__(kBlock) kAdjSource(AdjView a, int mode) {
             axpy(gx, w * G2 * dot3(r, ui), r);
         }
     }
+    lanesSum3(gx);
+    lanesSum3(gu);
+    if (!valid || sub != 0) return;
     const float4 x0 = a.xBar[i], v0 = a.vBar[i];
     a.xBar[i] = make_float4(x0.x + gx.x, x0.y + gx.y, x0.z + gx.z, 0.0f);
     a.vBar[i] = make_float4(v0.x + gu.x, v0.y + gu.y, v0.z + gu.z, 0.0f);
@@ -554,8 +582,16 @@ hipError_t launchAdjFinish(const AdjView& a, hipStream_t st) {
 
 hipError_t launchAdjHalfStart(const AdjView& a, int mode, hipStream_t st) { ADJ_LAUNCH_MODE(kAdjHalfStart, mode); }
 hipError_t launchAdjHalfEnd(const AdjView& a, int mode, hipStream_t st) { ADJ_LAUNCH_MODE(kAdjHalfEnd, mode); }
-hipError_t launchAdjVelocity(const AdjView& a, int mode, hipStream_t st) { ADJ_LAUNCH_MODE(kAdjVelocity, mode); }
-hipError_t launchAdjSource(const AdjView& a, int mode, hipStream_t st) { ADJ_LAUNCH_MODE(kAdjSource, mode); }
+hipError_t launchAdjVelocity(const AdjView& a, int mode, hipStream_t st) {
+    if (a.numBlocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(kAdjVelocity, dim3(a.numBlocks * kAdjLpp), dim3(kBlock), 0, st, a, mode);
+    return hipGetLastError();
+}
+hipError_t launchAdjSource(const AdjView& a, int mode, hipStream_t st) {
+    if (a.numBlocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(kAdjSource, dim3(a.numBlocks * kAdjLpp), dim3(kBlock), 0, st, a, mode);
+    return hipGetLastError();
+}
 hipError_t launchAdjPredict(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjPredict, a.numBlocks); }
 hipError_t launchAdjAlphaA(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjAlphaA, a.numBlocks); }
 hipError_t launchAdjAlphaB(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjAlphaB, a.numBlocks); }

@@ -89,6 +89,15 @@ template <bool MAX>
 __device__ __forceinline__ double combine(double a, double b) {
     return MAX ? fmax(a, b) : a + b;
 }
+// Sum over the L lanes that share one particle: lanes l, l + 64/L, l + 128/L,
+// ... of a wave (every one of them gets the same value).
+template <uint32_t L>
+__device__ __forceinline__ float lppSum(float v) {
+#pragma unroll
+    for (uint32_t off = 64u / L; off < 64u; off <<= 1) v += __shfl_xor(v, static_cast<int>(off), 64);
+    return v;
+}
+
 template <bool MAX>
 __device__ double blockReduce(double v) {
     __shared__ double sm[kBlock / 64];

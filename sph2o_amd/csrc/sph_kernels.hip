@@ -200,14 +200,6 @@ __device__ __forceinline__ bool inCell(const SceneConst& S, float4 p, int gx, in
     const uint32_t i = lb * (kBlock / (L)) + (threadIdx.x >> 6) * (64u / (L)) + threadIdx.x % (64u / (L)); \
     const bool valid = T.active[slot] != 0 && i < S.fEnd;
 
-// Sum over the L lanes of one particle (every lane gets the same value).
-template <uint32_t L>
-__device__ __forceinline__ float lppSum(float v) {
-#pragma unroll
-    for (uint32_t off = 64u / L; off < 64u; off <<= 1) v += __shfl_xor(v, static_cast<int>(off), 64);
-    return v;
-}
-
 // ---- FP64 deterministic reductions: block partials + per-scene kFinish ----
 // Block partial of a reduction, stored at the block's logical index with a
 // plain store. kFinish, launched next on the same stream, combines each
