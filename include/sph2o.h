@@ -30,6 +30,9 @@
  *   sph_dump_cells        HashTable key floor(pos/h), HashTable.cpp:5
  *   sph_dump_neighbors    HashTable::neighborhoodSearch, HashTable.cpp:23-86,
  *                         filtered by the support test of DFSPHSolver.cpp:575
+ *   sph_add_emitter       FluidObject::addEmitter, FluidObject.cpp:177-193
+ *                         (+ Emitter, Emitter.cpp:5-123)
+ *   sph_num_active        FluidObject::getNumActiveParticles, FluidObject.h:62
  */
 #ifndef SPH2O_H_
 #define SPH2O_H_
@@ -97,6 +100,24 @@ typedef struct sph_step_stats {
     int32_t reserved;
 } sph_step_stats;
 
+/* A particle emitter (EmitterInfo, SPHStructs.h; Emitter.h:14-30). Every
+ * emission places one group: type 0 a square of width x height, type 1 a disc
+ * of radius `width`, both at spacing 2r in the emitter's local xy plane, moving
+ * along its local +z at `velocity` m/s, rotated by the quaternion `rotation`
+ * (w, x, y, z) and translated to `position`. The first emission is at
+ * start_time; the next follows spacing * 1.105 * 2r / velocity seconds later,
+ * while at least one group of the num_particles budget is left. */
+typedef struct sph_emitter_params {
+    int32_t type;            /* 0 square, 1 circle */
+    int32_t reserved0;
+    int64_t num_particles;   /* total this emitter may emit */
+    float position[3];
+    float rotation[4];       /* quaternion w, x, y, z */
+    float velocity;
+    float start_time, width, height, spacing;
+    int32_t reserved[4];
+} sph_emitter_params;
+
 typedef struct sph_ctx sph_ctx;
 
 /* Context: device ordinal and a hipStream_t (NULL = a stream the context creates). */
@@ -119,9 +140,19 @@ int sph_load_scene_json(sph_ctx* ctx, const char* path);
  * SPH_E_CAPACITY if pos_cap or bpos_cap is too small (counts still written). */
 int sph_scene_from_json(const char* path, sph_scene_params* params, int64_t* nf, float* pos, int64_t pos_cap,
                         int64_t* nb, float* bpos, int64_t bpos_cap);
+/* Adds an emitter to a scene before sph_init (the scene JSON's "emitters" are
+ * added by sph_load_scene_json). The scene grows by num_particles inactive
+ * particles, numbered after the existing ones in emission order; they become
+ * active when emitted. */
+int sph_add_emitter(sph_ctx* ctx, int scene, const sph_emitter_params* emitter);
 int sph_num_scenes(const sph_ctx* ctx);
 int sph_get_params(const sph_ctx* ctx, int scene, sph_scene_params* out);
+/* All fluid particles of a scene, inactive emitter particles included
+ * (FluidObject::getSize); the state getters return that many. */
 int64_t sph_num_fluid(const sph_ctx* ctx, int scene);
+/* The active ones (the first sph_num_active in particle order); inactive
+ * particles stay at their initial (zero) position and velocity. */
+int64_t sph_num_active(const sph_ctx* ctx, int scene);
 int64_t sph_num_boundary(const sph_ctx* ctx, int scene);
 
 /* Build device state, boundary volumes, and the solver's init pass. */

@@ -163,8 +163,9 @@ int main(int argc, char** argv) {
         for (uint i = 0; i < N; ++i)
             alpha0[i] = static_cast<DFSPHSolver*>(sim->getSolver())->getAlpha(0, i);
 
-    // Per-step stats: t, dt, iterations, iterationsV, avg density err, avg div err
-    std::vector<double> stats;
+    // Per-step stats: t, dt, iterations, iterationsV, avg density err, avg div err;
+    // nact: active particles after each step (emitters)
+    std::vector<double> stats, nact;
     auto t0 = std::chrono::steady_clock::now();
     for (int s = 0; s < nsteps; ++s) {
         sim->getSolver()->step();
@@ -178,6 +179,7 @@ int main(int argc, char** argv) {
         }
         stats.insert(stats.end(), {static_cast<double>(sim->getTime()),
                                    static_cast<double>(sim->getTimeStep()), its, itsV, eD, eV});
+        nact.push_back(static_cast<double>(f->getNumActiveParticles()));
     }
     auto t1 = std::chrono::steady_clock::now();
     double secs = std::chrono::duration<double>(t1 - t0).count();
@@ -213,6 +215,7 @@ int main(int argc, char** argv) {
     dump(outdir, "p", "<f4", p, 1);
     dump(outdir, "alpha", "<f4", alpha, 1);
     dump(outdir, "stats", "<f8", stats, 6);
+    dump(outdir, "nact", "<f8", nact, 1);
     dump(outdir, "meta", "<f8", meta, 1);
 
     if (wantNbr) {

@@ -30,7 +30,7 @@ EXPORTS = [
     "sph_get_time", "sph_dump_cells", "sph_dump_neighbors", "sph_last_step_timing", "sph_set_profiling",
     "sph_kernel_count", "sph_kernel_name", "sph_kernel_stats", "sph_reset_kernel_stats",
     "sph_set_coefficients", "sph_forward_record", "sph_adjoint_seed", "sph_adjoint_run", "sph_adjoint_result",
-    "sph_recorded_field",
+    "sph_recorded_field", "sph_add_emitter", "sph_num_active",
 ]
 
 
@@ -53,6 +53,16 @@ class SceneParams(ctypes.Structure):
         ("viscosity", ctypes.c_float), ("boundary_viscosity", ctypes.c_float),
         ("surface_tension", ctypes.c_float), ("adhesion", ctypes.c_float),
         ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+class EmitterParams(ctypes.Structure):
+    """sph_emitter_params (include/sph2o.h)."""
+    _fields_ = [
+        ("type", ctypes.c_int32), ("reserved0", ctypes.c_int32), ("num_particles", ctypes.c_int64),
+        ("position", ctypes.c_float * 3), ("rotation", ctypes.c_float * 4), ("velocity", ctypes.c_float),
+        ("start_time", ctypes.c_float), ("width", ctypes.c_float), ("height", ctypes.c_float),
+        ("spacing", ctypes.c_float), ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -94,6 +104,8 @@ def load():
         "sph_num_scenes": (ctypes.c_int, [vp]),
         "sph_get_params": (ctypes.c_int, [vp, ctypes.c_int, P(SceneParams)]),
         "sph_num_fluid": (ctypes.c_int64, [vp, ctypes.c_int]),
+        "sph_num_active": (ctypes.c_int64, [vp, ctypes.c_int]),
+        "sph_add_emitter": (ctypes.c_int, [vp, ctypes.c_int, P(EmitterParams)]),
         "sph_num_boundary": (ctypes.c_int64, [vp, ctypes.c_int]),
         "sph_init": (ctypes.c_int, [vp]),
         "sph_step": (ctypes.c_int, [vp, ctypes.c_int, P(StepStats)]),

@@ -160,14 +160,20 @@ struct Reader {
         out = static_cast<uint32_t>(d < 0 ? 0 : d);
         return true;
     }
-    bool vec3(const JVal* obj, const char* key, float out[3], const std::string& where) {
+    bool vecN(const JVal* obj, const char* key, float* out, int n, const std::string& where) {
         const JVal* v = obj ? obj->find(key) : nullptr;
-        if (!v || v->kind != JVal::Arr || v->arr.size() < 3) { err = "key " + where + "." + key + " must be [x,y,z]"; return false; }
-        for (int a = 0; a < 3; ++a) {
+        if (!v || v->kind != JVal::Arr || v->arr.size() < static_cast<size_t>(n)) {
+            err = "key " + where + "." + key + " must be an array of " + std::to_string(n) + " numbers";
+            return false;
+        }
+        for (int a = 0; a < n; ++a) {
             if (v->arr[a].kind != JVal::Num) { err = "key " + where + "." + key + " must hold numbers"; return false; }
             out[a] = static_cast<float>(v->arr[a].num);
         }
         return true;
+    }
+    bool vec3(const JVal* obj, const char* key, float out[3], const std::string& where) {
+        return vecN(obj, key, out, 3, where);
     }
 };
 
@@ -247,7 +253,28 @@ int parseSceneJson(const std::string& text, SceneDesc& out, std::string& err) {
         if (!rd.vec3(&blocks->arr[j], "min", b.min, bw) || !rd.vec3(&blocks->arr[j], "max", b.max, bw)) { err = rd.err; return SPH_E_SCENE; }
         out.fluidBlocks.push_back(b);
     }
-    if (const JVal* em = m.find("emitters")) out.numEmitters = em->kind == JVal::Arr ? static_cast<int>(em->arr.size()) : 0;
+    // emitters (SceneLoader.cpp:107-141): rotation is the quaternion (w, x, y, z)
+    if (const JVal* em = m.find("emitters")) {
+        if (em->kind != JVal::Arr) { err = mw + ".emitters must be an array"; return SPH_E_SCENE; }
+        for (size_t j = 0; j < em->arr.size(); ++j) {
+            const JVal* e = &em->arr[j];
+            const std::string ew = mw + ".emitters[" + std::to_string(j) + "]";
+            sph_emitter_params q{};
+            uint32_t n = 0, type = 0;
+            if (!rd.vec3(e, "position", q.position, ew) || !rd.vecN(e, "rotation", q.rotation, 4, ew) ||
+                !rd.f32(e, "velocity", q.velocity, ew) || !rd.u32(e, "numParticles", n, ew) ||
+                !rd.f32(e, "startTime", q.start_time, ew) || !rd.u32(e, "type", type, ew) ||
+                !rd.f32(e, "width", q.width, ew) || !rd.f32(e, "height", q.height, ew) ||
+                !rd.f32(e, "spacing", q.spacing, ew)) {
+                err = rd.err;
+                return SPH_E_SCENE;
+            }
+            q.num_particles = static_cast<int64_t>(n);
+            q.type = static_cast<int32_t>(type);
+            out.emitters.push_back(q);
+        }
+        out.numEmitters = static_cast<int>(out.emitters.size());
+    }
 
     // Boundary (SceneLoader.cpp:148-209); absent sphere/geometry arrays are empty.
     if (const JVal* B = root.find("Boundary")) {

@@ -18,6 +18,7 @@ struct SceneDesc {
     std::vector<BlockDesc> fluidBlocks;   // FluidModel[0].fluidBlocks
     std::vector<BlockDesc> boxes;         // Boundary[*].box (all boundary objects)
     std::vector<SphereDesc> spheres;      // Boundary[*].sphere
+    std::vector<sph_emitter_params> emitters;  // FluidModel[0].emitters
     int numFluidModels = 0;
     int numEmitters = 0;
     int numGeometries = 0;

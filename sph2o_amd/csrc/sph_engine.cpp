@@ -145,6 +145,16 @@ struct SceneH {
     uint32_t ovBuckets = 1024;  // overflow buckets for cells outside the dense box (power of two)
     int morton = 0;
     std::vector<uint32_t> bOrder;  // sorted boundary position -> original index
+    // emitters: the first na of the nf particles are active (FluidObject's
+    // numActiveParticles); the rest wait in the emitters' budgets
+    uint32_t na = 0;
+    struct Emitter {
+        sph_emitter_params P;
+        std::vector<float> group;   // local offsets of one emission (xyz)
+        int64_t remaining = 0;      // particles left to emit
+        float next = 0.0f;          // time of the next emission
+    };
+    std::vector<Emitter> emitters;
 };
 
 class Engine {
@@ -238,6 +248,7 @@ class Engine {
         SceneH s;
         s.P = P;
         s.nf = static_cast<uint32_t>(nf);
+        s.na = s.nf;
         s.nb = static_cast<uint32_t>(nb);
         s.xf.assign(pos, pos + 3 * nf);
         s.vf.assign(3 * nf, 0.0f);
@@ -251,6 +262,126 @@ class Engine {
         scenes.push_back(std::move(s));
         method = P.simulation_method;
         return static_cast<int>(scenes.size()) - 1;
+    }
+
+    // FluidObject::addEmitter (FluidObject.cpp:177-193) with the emission
+    // group of Emitter::buildSquareGroup / buildCircleGroup (Emitter.cpp:25-70),
+    // in the reference's FP32 (and, where it converts, FP64) arithmetic.
+    void addEmitter(int si, const sph_emitter_params& E) {
+        if (initialized) throw SphError{SPH_E_STATE, "emitters must be added before sph_init"};
+        SceneH& s = scene(si);
+        if (E.type != 0 && E.type != 1) throw SphError{SPH_E_ARG, "emitter type must be 0 (square) or 1 (circle)"};
+        if (E.num_particles < 0 || static_cast<int64_t>(s.nf) + E.num_particles > (1ll << 30))
+            throw SphError{SPH_E_ARG, "emitter particle budget out of range"};
+        for (float f : {E.position[0], E.position[1], E.position[2], E.rotation[0], E.rotation[1], E.rotation[2],
+                        E.rotation[3], E.velocity, E.start_time, E.width, E.height, E.spacing})
+            if (!std::isfinite(f)) throw SphError{SPH_E_NONFINITE, "non-finite emitter parameter"};
+        SceneH::Emitter e;
+        e.P = E;
+        e.remaining = E.num_particles;
+        e.next = E.start_time;
+        const float dist = 2.0f * s.r;
+        if (E.type == 1) {
+            // disc: the centre, then rings at (i + 1) dist holding floor(2 pi R / dist) particles
+            e.group.insert(e.group.end(), {0.0f, 0.0f, 0.0f});
+            const uint32_t levels = static_cast<uint32_t>(std::floor((E.width - 0.5f * dist) / dist));
+            const float twoPi = static_cast<float>(2.0 * M_PI);
+            for (uint32_t i = 0; i < levels; ++i) {
+                const float radius = static_cast<float>(i + 1) * dist;
+                const float len = twoPi * radius;
+                const uint32_t parts = static_cast<uint32_t>(std::floor(len / dist));
+                const float inc = twoPi / static_cast<float>(parts);
+                for (uint32_t k = 0; k < parts; ++k) {
+                    const float ang = static_cast<float>(k) * inc;
+                    e.group.push_back(static_cast<float>(radius * std::sin(static_cast<double>(ang))));
+                    e.group.push_back(static_cast<float>(radius * std::cos(static_cast<double>(ang))));
+                    e.group.push_back(0.0f);
+                }
+            }
+        } else {
+            // rectangle of numX x numY particles centred on the emitter
+            const uint32_t numX = static_cast<uint32_t>(std::floor(E.width / dist));
+            const uint32_t numY = static_cast<uint32_t>(E.height / dist);
+            const float realW = static_cast<float>(numX - 1) * dist, realH = static_cast<float>(numY - 1) * dist;
+            for (uint32_t i = 0; i < numX; ++i)
+                for (uint32_t j = 0; j < numY; ++j) {
+                    e.group.push_back(static_cast<float>(realW * 0.5 - static_cast<double>(static_cast<float>(i) * dist)));
+                    e.group.push_back(static_cast<float>(realH * 0.5 - static_cast<double>(static_cast<float>(j) * dist)));
+                    e.group.push_back(0.0f);
+                }
+        }
+        s.nf += static_cast<uint32_t>(E.num_particles);
+        s.xf.resize(3 * static_cast<size_t>(s.nf), 0.0f);
+        s.vf.resize(3 * static_cast<size_t>(s.nf), 0.0f);
+        s.emitters.push_back(std::move(e));
+    }
+
+    // Rotation of v by the quaternion q = (w, x, y, z), as the reference applies
+    // its Quat4r to a point: v + 2 w (u x v) + 2 u x (u x v) with u = (x, y, z).
+    static void rotate(const float q[4], float v[3]) {
+        const float w = q[0], ux = q[1], uy = q[2], uz = q[3];
+        const float cx = uy * v[2] - uz * v[1], cy = uz * v[0] - ux * v[2], cz = ux * v[1] - uy * v[0];
+        const float dx = uy * cz - uz * cy, dy = uz * cx - ux * cz, dz = ux * cy - uy * cx;
+        v[0] = v[0] + (cx * w + dx) * 2.0f;
+        v[1] = v[1] + (cy * w + dy) * 2.0f;
+        v[2] = v[2] + (cz * w + dz) * 2.0f;
+    }
+
+    // SPHSimulation::emitParticles -> Emitter::emitParticles / emitGroup
+    // (Emitter.cpp:72-113) at the scene time t_n of the current step: every
+    // emitter whose next time has come and whose budget still holds a whole
+    // group places it in the scene's first free slots (the dead-key padding at
+    // the end of its sorted segment). dfsph marks the new particles so the
+    // keys pass skips their position update (the reference emits after
+    // updatePositions). Returns whether any scene grew.
+    bool emitAll(bool dfsph) {
+        bool any = false;
+        for (size_t si = 0; si < scenes.size(); ++si) {
+            SceneH& s = scenes[si];
+            if (s.emitters.empty()) continue;
+            const float t = hState[si].t;
+            const float dist = 2.0f * s.r;
+            std::vector<float4> nx, nv;
+            for (auto& e : s.emitters) {
+                if (!(t >= e.next)) continue;
+                const int64_t g = static_cast<int64_t>(e.group.size() / 3);
+                if (g > e.remaining) continue;
+                const float off = t - e.next;
+                for (int64_t k = 0; k < g; ++k) {
+                    float vel[3] = {0.0f, 0.0f, e.P.velocity};
+                    float pos[3];
+                    for (int a = 0; a < 3; ++a) pos[a] = e.group[3 * k + a] + off * (vel[a] + s.P.gravity[a] * off);
+                    rotate(e.P.rotation, pos);
+                    rotate(e.P.rotation, vel);
+                    for (int a = 0; a < 3; ++a) pos[a] += e.P.position[a];
+                    nx.push_back(make_float4(pos[0], pos[1], pos[2], dfsph ? 1.0f : 0.0f));
+                    nv.push_back(make_float4(vel[0], vel[1], vel[2], 0.0f));
+                }
+                e.remaining -= g;
+                e.next += e.P.spacing * 1.105f * dist / e.P.velocity;
+            }
+            if (nx.empty()) continue;
+            const uint32_t cnt = static_cast<uint32_t>(nx.size());
+            if (s.na + cnt > s.nf) throw SphError{SPH_E_STATE, "emitter budget exceeded"};
+            HIPCHK(hipStreamSynchronize(stream));
+            const uint32_t slot = s.fBegin + s.na;
+            std::vector<uint32_t> nid(cnt);
+            for (uint32_t k = 0; k < cnt; ++k) nid[k] = s.na + k;
+            const std::vector<float> zero(cnt, 0.0f);
+            HIPCHK(hipMemcpy(x.p + slot, nx.data(), cnt * sizeof(float4), hipMemcpyHostToDevice));
+            uploadVel(slot, nv);
+            HIPCHK(hipMemcpy(id.p + slot, nid.data(), cnt * sizeof(uint32_t), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(kws.p + slot, zero.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(kvws.p + slot, zero.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
+            if (interleaved)  // positions are duplicated in the interleaved records
+                HIPCHK(hipMemcpy2D(xv.p + 2 * static_cast<size_t>(slot), 2 * sizeof(float4), nx.data(), sizeof(float4),
+                                   sizeof(float4), cnt, hipMemcpyHostToDevice));
+            s.na += cnt;
+            s.fEnd = s.fBegin + s.na;
+            any = true;
+        }
+        if (any) uploadConsts();
+        return any;
     }
 
     // FP32 constants in the reference's evaluation order
@@ -409,7 +540,7 @@ class Engine {
             c.maxIt = static_cast<int32_t>(std::min<uint32_t>(s.P.max_iterations, 1u << 30));
             c.minItV = static_cast<int32_t>(std::min<uint32_t>(s.P.min_iterations_v, 1u << 30));
             c.adaptive = (s.P.simulation_method == SPH_DFSPH && s.P.adaptive_time_step) ? 1 : 0;
-            c.nf = static_cast<int32_t>(s.nf);
+            c.nf = static_cast<int32_t>(s.na);
             c.ovMask = s.ovBuckets - 1;
             c.nCells = s.nCells;
         }
@@ -498,7 +629,7 @@ class Engine {
         uint32_t off = 0, boff = 0;
         for (auto& s : scenes) {
             s.fBegin = off;
-            s.fEnd = off + s.nf;
+            s.fEnd = off + s.na;
             s.fPadEnd = off + ((s.nf + kBlock - 1) / kBlock) * kBlock;
             s.blockBegin = s.fBegin / kBlock;
             s.blockEnd = s.fPadEnd / kBlock;
@@ -507,9 +638,10 @@ class Engine {
             s.bEnd = boff + s.nb;
             boff += s.nb;
             float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
-            aabb(s.xf, lo, hi);
+            aabb(std::vector<float>(s.xf.begin(), s.xf.begin() + 3 * static_cast<size_t>(s.na)), lo, hi);
             aabb(s.xb, lo, hi);
-            if (s.nf == 0 && s.nb == 0) { lo[0] = lo[1] = lo[2] = 0; hi[0] = hi[1] = hi[2] = 0; }
+            for (auto& e : s.emitters) aabb(std::vector<float>(e.P.position, e.P.position + 3), lo, hi);
+            if (s.na == 0 && s.nb == 0 && s.emitters.empty()) { lo[0] = lo[1] = lo[2] = 0; hi[0] = hi[1] = hi[2] = 0; }
             setGrid(s, lo, hi);
         }
         NP = off;
@@ -603,7 +735,7 @@ class Engine {
         std::vector<float4> hx(np, make_float4(0, 0, 0, 0)), hv(np, make_float4(0, 0, 0, 0));
         std::vector<uint32_t> hid(np, 0);
         for (auto& s : scenes)
-            for (uint32_t i = 0; i < s.nf; ++i) {
+            for (uint32_t i = 0; i < s.na; ++i) {
                 const uint32_t slot = s.fBegin + i;
                 hx[slot] = make_float4(s.xf[3 * i], s.xf[3 * i + 1], s.xf[3 * i + 2], 0.0f);
                 hv[slot] = make_float4(s.vf[3 * i], s.vf[3 * i + 1], s.vf[3 * i + 2], 0.0f);
@@ -766,8 +898,8 @@ class Engine {
     // false, leaving the box, otherwise.
     bool growGrid(size_t si) {
         SceneH& s = scenes[si];
-        std::vector<float4> hx(s.nf);
-        if (s.nf) HIPCHK(hipMemcpy(hx.data(), x.p + s.fBegin, s.nf * sizeof(float4), hipMemcpyDeviceToHost));
+        std::vector<float4> hx(s.na);
+        if (s.na) HIPCHK(hipMemcpy(hx.data(), x.p + s.fBegin, s.na * sizeof(float4), hipMemcpyDeviceToHost));
         float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
         for (const float4& q : hx) {
             lo[0] = std::min(lo[0], q.x); lo[1] = std::min(lo[1], q.y); lo[2] = std::min(lo[2], q.z);
@@ -907,8 +1039,10 @@ class Engine {
                 [&](int slot) { velocityPass(d, slot, kFinishDensity); },
                 [] { return true; });
         for (size_t s = 0; s < scenes.size(); ++s) scenes[s].its = hState[s].it;
-        // 5-7. updatePositions fused with the keys, then the neighbour search;
-        // its validity is checked at the divergence loop's first sync
+        // 5-7. updatePositions fused with the keys (emitted particles are
+        // placed after it, DFSPHSolver.cpp:72), then the neighbour search; its
+        // validity is checked at the divergence loop's first sync
+        emitAll(true);
         searchOnce(true);
         for (int attempt = 0;; ++attempt) {
             d = view();
@@ -957,6 +1091,7 @@ class Engine {
         timed(K_FORCES, [&] { return launchForces(d, 1, stream); });
         if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
         timed(K_INTEGRATE, [&] { return launchIntegrate(d, stream); });
+        emitAll(false);   // WCSPHSolver.cpp:75, before t += dt
         for (size_t s = 0; s < scenes.size(); ++s) hState[s].t = hState[s].t + hState[s].dt;
         fileStats(false);
     }
@@ -983,6 +1118,7 @@ class Engine {
                 },
                 [] { return true; });
         timed(K_PCI_INTEGRATE, [&] { return launchPciIntegrate(d, stream); });
+        emitAll(false);   // PCISPHSolver.cpp:77, before t += dt
         for (size_t s = 0; s < scenes.size(); ++s) {
             SceneState& T = hState[s];
             const SceneConst_host c = hostConst(s);
@@ -1296,6 +1432,8 @@ class Engine {
         if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
         if (method == SPH_PCISPH) throw SphError{SPH_E_UNSUPPORTED, "gradients are available for WCSPH and DFSPH"};
         if (interleaved) throw SphError{SPH_E_UNSUPPORTED, "gradients need separate velocity storage"};
+        for (auto& sc : scenes)
+            if (!sc.emitters.empty()) throw SphError{SPH_E_UNSUPPORTED, "gradients of scenes with emitters"};
         if (!NP) throw SphError{SPH_E_STATE, "no fluid particles"};
         int nit = 0;
         for (auto& sc : scenes) {
@@ -1605,46 +1743,56 @@ class Engine {
             if (pres) std::fill(pres, pres + s.nf, 0.0f);
             return;
         }
-        auto ids = download(id.p + s.fBegin, s.nf);
+        // active particles from the device; inactive ones keep their initial values
+        const uint32_t na = s.na;
+        auto ids = download(id.p + s.fBegin, na);
         if (pos || vel) {
-            auto hx = pos ? download(x.p + s.fBegin, s.nf) : std::vector<float4>();
-            auto hv = vel ? downloadVel(s.fBegin, s.nf) : std::vector<float4>();
-            for (uint32_t i = 0; i < s.nf; ++i) {
+            auto hx = pos ? download(x.p + s.fBegin, na) : std::vector<float4>();
+            auto hv = vel ? downloadVel(s.fBegin, na) : std::vector<float4>();
+            for (uint32_t i = 0; i < na; ++i) {
                 const uint32_t o = ids[i];
-                if (o >= s.nf) throw SphError{SPH_E_STATE, "corrupt particle ids"};
+                if (o >= na) throw SphError{SPH_E_STATE, "corrupt particle ids"};
                 if (pos) { pos[3 * o] = hx[i].x; pos[3 * o + 1] = hx[i].y; pos[3 * o + 2] = hx[i].z; }
                 if (vel) { vel[3 * o] = hv[i].x; vel[3 * o + 1] = hv[i].y; vel[3 * o + 2] = hv[i].z; }
             }
         }
+        if (pos) std::copy(s.xf.begin() + 3 * static_cast<size_t>(na), s.xf.end(), pos + 3 * static_cast<size_t>(na));
+        if (vel) std::copy(s.vf.begin() + 3 * static_cast<size_t>(na), s.vf.end(), vel + 3 * static_cast<size_t>(na));
         if (dens) {
-            auto hr = download(rho.p + s.fBegin, s.nf);
-            for (uint32_t i = 0; i < s.nf; ++i) dens[ids[i]] = hr[i];
+            auto hr = download(rho.p + s.fBegin, na);
+            for (uint32_t i = 0; i < na; ++i) dens[ids[i]] = hr[i];
+            std::fill(dens + na, dens + s.nf, 0.0f);
         }
         if (pres) {
-            auto hp = download(p.p + s.fBegin, s.nf);
-            for (uint32_t i = 0; i < s.nf; ++i) pres[ids[i]] = hp[i];
+            auto hp = download(p.p + s.fBegin, na);
+            for (uint32_t i = 0; i < na; ++i) pres[ids[i]] = hp[i];
+            std::fill(pres + na, pres + s.nf, 0.0f);
         }
     }
 
     void getField(int si, int field, float* out) {
         SceneH& s = scene(si);
         if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
-        auto ids = download(id.p + s.fBegin, s.nf);
+        const uint32_t na = s.na;   // inactive particles read 0
+        auto ids = download(id.p + s.fBegin, na);
         if (field == 1 || field == 5) {
-            auto hv = download((field == 1 ? a.p : n.p) + s.fBegin, s.nf);
-            for (uint32_t i = 0; i < s.nf; ++i) {
+            auto hv = download((field == 1 ? a.p : n.p) + s.fBegin, na);
+            std::fill(out, out + 3 * static_cast<size_t>(s.nf), 0.0f);
+            for (uint32_t i = 0; i < na; ++i) {
                 out[3 * ids[i]] = hv[i].x; out[3 * ids[i] + 1] = hv[i].y; out[3 * ids[i] + 2] = hv[i].z;
             }
             return;
         }
         const float* src = field == 0 ? alpha.p : field == 2 ? aux.p : field == 3 ? kws.p : field == 4 ? kvws.p : nullptr;
         if (!src) throw SphError{SPH_E_ARG, "unknown field"};
-        auto hv = download(src + s.fBegin, s.nf);
-        for (uint32_t i = 0; i < s.nf; ++i) out[ids[i]] = hv[i];
+        auto hv = download(src + s.fBegin, na);
+        std::fill(out, out + s.nf, 0.0f);
+        for (uint32_t i = 0; i < na; ++i) out[ids[i]] = hv[i];
     }
 
     void setState(int si, int64_t nn, const float* pos, const float* vel) {
         SceneH& s = scene(si);
+        if (!s.emitters.empty()) throw SphError{SPH_E_UNSUPPORTED, "setting the state of a scene with emitters"};
         ++stateEpoch;
         if (nn != static_cast<int64_t>(s.nf)) throw SphError{SPH_E_ARG, "particle count mismatch"};
         if (pos)
@@ -1695,9 +1843,10 @@ class Engine {
         DevBuf<int32_t> dc;
         dc.alloc(3 * (NP ? NP : 1));
         HIPCHK(launchCellsOut(view(), dc.p, stream));
-        auto hc = download(dc.p + 3 * static_cast<size_t>(s.fBegin), 3 * static_cast<size_t>(s.nf));
-        auto ids = download(id.p + s.fBegin, s.nf);
-        for (uint32_t i = 0; i < s.nf; ++i)
+        auto hc = download(dc.p + 3 * static_cast<size_t>(s.fBegin), 3 * static_cast<size_t>(s.na));
+        auto ids = download(id.p + s.fBegin, s.na);
+        std::fill(cells, cells + 3 * static_cast<size_t>(s.nf), 0);   // inactive particles: no cell
+        for (uint32_t i = 0; i < s.na; ++i)
             for (int a3 = 0; a3 < 3; ++a3) cells[3 * ids[i] + a3] = hc[3 * i + a3];
     }
 
@@ -1706,21 +1855,23 @@ class Engine {
         if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
         neighborSearch(false);
         auto ids = download(id.p, NP);
-        auto cf = download(cntF.p + s.fBegin, s.nf);
-        auto cb = download(cntB.p + s.fBegin, s.nf);
+        // active particles only; inactive ones get empty lists
+        const uint32_t na = s.na;
+        auto cf = download(cntF.p + s.fBegin, na);
+        auto cb = download(cntB.p + s.fBegin, na);
         auto lf = download(nbrF.p, static_cast<size_t>(capF) * NP);
         auto lb = download(nbrB.p, static_cast<size_t>(capB) * NP);
-        std::vector<uint32_t> slotOf(s.nf);
-        for (uint32_t i = 0; i < s.nf; ++i) slotOf[ids[s.fBegin + i]] = s.fBegin + i;
+        std::vector<uint32_t> slotOf(na);
+        for (uint32_t i = 0; i < na; ++i) slotOf[ids[s.fBegin + i]] = s.fBegin + i;
         off[0] = 0;
         boff[0] = 0;
         for (uint32_t o = 0; o < s.nf; ++o) {
-            const uint32_t sl = slotOf[o] - s.fBegin;
-            off[o + 1] = off[o] + cf[sl];
-            boff[o + 1] = boff[o] + cb[sl];
+            const uint32_t sl = o < na ? slotOf[o] - s.fBegin : 0u;
+            off[o + 1] = off[o] + (o < na ? cf[sl] : 0u);
+            boff[o + 1] = boff[o] + (o < na ? cb[sl] : 0u);
         }
         if (off[s.nf] > cap || boff[s.nf] > bcap) return SPH_E_CAPACITY;
-        for (uint32_t o = 0; o < s.nf; ++o) {
+        for (uint32_t o = 0; o < na; ++o) {
             const uint32_t slot = slotOf[o];
             const uint32_t sl = slot - s.fBegin;
             for (uint32_t q = 0; q < cf[sl]; ++q) ids_[off[o] + q] = static_cast<int32_t>(ids[lf[static_cast<size_t>(q) * NP + slot]]);
@@ -1812,7 +1963,8 @@ int sph_add_scene(sph_ctx* ctx, const sph_scene_params* params, int64_t nf, cons
 namespace {
 // SPHSimulation::importScene (SPHSimulation.cpp:299-481) minus the object
 // factory: JSON -> parameters + sampled fluid and boundary particles.
-void readScene(const char* path, sph_scene_params& params, std::vector<float>& xf, std::vector<float>& xbv) {
+void readScene(const char* path, sph_scene_params& params, std::vector<float>& xf, std::vector<float>& xbv,
+               std::vector<sph_emitter_params>* emitters = nullptr) {
     if (!path) throw SphError{SPH_E_ARG, "path is NULL"};
     std::ifstream f(path);
     if (!f) throw SphError{SPH_E_SCENE, std::string("cannot read ") + path};
@@ -1822,7 +1974,6 @@ void readScene(const char* path, sph_scene_params& params, std::vector<float>& x
     std::string err;
     const int rc = parseSceneJson(ss.str(), desc, err);
     if (rc != SPH_OK) throw SphError{rc, err};
-    if (desc.numEmitters > 0) throw SphError{SPH_E_UNSUPPORTED, "emitters are not supported"};
     if (desc.numGeometries > 0) throw SphError{SPH_E_UNSUPPORTED, "mesh (geometry) boundaries are not supported"};
     if (desc.params.boundary_method != SPH_BOUNDARY_AKINCI && (!desc.boxes.empty() || !desc.spheres.empty()))
         throw SphError{SPH_E_UNSUPPORTED, "only Akinci boundaries are supported"};
@@ -1832,6 +1983,7 @@ void readScene(const char* path, sph_scene_params& params, std::vector<float>& x
         for (auto& s : desc.spheres) sampleSphereSurface(s, desc.params.particle_radius, xbv);
     }
     params = desc.params;
+    if (emitters) *emitters = desc.emitters;
 }
 }  // namespace
 
@@ -1839,9 +1991,12 @@ int sph_load_scene_json(sph_ctx* ctx, const char* path) {
     return guarded(ctx, [&] {
         sph_scene_params params;
         std::vector<float> xf, xbv;
-        readScene(path, params, xf, xbv);
-        return ctx->eng.addScene(params, static_cast<int64_t>(xf.size() / 3), xf.data(), nullptr,
-                                 static_cast<int64_t>(xbv.size() / 3), xbv.data());
+        std::vector<sph_emitter_params> em;
+        readScene(path, params, xf, xbv, &em);
+        const int id = ctx->eng.addScene(params, static_cast<int64_t>(xf.size() / 3), xf.data(), nullptr,
+                                         static_cast<int64_t>(xbv.size() / 3), xbv.data());
+        for (auto& e : em) ctx->eng.addEmitter(id, e);   // SPHSimulation.cpp:373-385
+        return id;
     });
 }
 
@@ -1851,7 +2006,9 @@ int sph_scene_from_json(const char* path, sph_scene_params* params, int64_t* nf,
     return guarded(&dummy, [&] {
         if (!params || !nf || !nb) throw SphError{SPH_E_ARG, "NULL output"};
         std::vector<float> xf, xbv;
-        readScene(path, *params, xf, xbv);
+        std::vector<sph_emitter_params> em;
+        readScene(path, *params, xf, xbv, &em);
+        if (!em.empty()) throw SphError{SPH_E_UNSUPPORTED, "the scene has emitters: load it with sph_load_scene_json"};
         *nf = static_cast<int64_t>(xf.size() / 3);
         *nb = static_cast<int64_t>(xbv.size() / 3);
         if (*nf > pos_cap || *nb > bpos_cap) return SPH_E_CAPACITY;
@@ -1871,6 +2028,17 @@ int sph_get_params(const sph_ctx* ctx, int scene, sph_scene_params* out) {
 int64_t sph_num_fluid(const sph_ctx* ctx, int scene) {
     if (!ctx || scene < 0 || static_cast<size_t>(scene) >= ctx->eng.scenes.size()) return SPH_E_ARG;
     return ctx->eng.scenes[scene].nf;
+}
+int64_t sph_num_active(const sph_ctx* ctx, int scene) {
+    if (!ctx || scene < 0 || static_cast<size_t>(scene) >= ctx->eng.scenes.size()) return SPH_E_ARG;
+    return ctx->eng.scenes[scene].na;
+}
+int sph_add_emitter(sph_ctx* ctx, int scene, const sph_emitter_params* emitter) {
+    return guarded(ctx, [&] {
+        if (!emitter) throw SphError{SPH_E_ARG, "emitter is NULL"};
+        ctx->eng.addEmitter(scene, *emitter);
+        return SPH_OK;
+    });
 }
 int64_t sph_num_boundary(const sph_ctx* ctx, int scene) {
     if (!ctx || scene < 0 || static_cast<size_t>(scene) >= ctx->eng.scenes.size()) return SPH_E_ARG;

@@ -267,11 +267,16 @@ __global__ void __launch_bounds__(kBlock) kKeys(DevView d, int dfsphAdvance, int
     }
     float4 p = d.x[i];
     if (dfsphAdvance) {
-        const float dt = T.dt;
-        const float4 vi = VEL(d, i);
-        p.x += vi.x * dt;
-        p.y += vi.y * dt;
-        p.z += vi.z * dt;
+        // w != 0 marks a particle emitted in this step: the reference emits
+        // after updatePositions (DFSPHSolver.cpp:72), so it does not move yet
+        if (p.w == 0.0f) {
+            const float dt = T.dt;
+            const float4 vi = VEL(d, i);
+            p.x += vi.x * dt;
+            p.y += vi.y * dt;
+            p.z += vi.z * dt;
+        }
+        p.w = 0.0f;
         d.x[i] = p;
         d.kws[i] *= (T.it > 0 ? T.dt2 : 0.0f);
     }

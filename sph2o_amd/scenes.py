@@ -66,6 +66,33 @@ def scene_dict(method, radius, fluid_blocks, boxes, dt=1e-3, max_dt=1e-3, min_dt
                           "geometry": []}]}
 
 
+def add_emitter(scene, position, rotation=(1.0, 0.0, 0.0, 0.0), velocity=1.0, num_particles=70, start_time=0.0,
+                type=1, width=0.03, height=0.03, spacing=1.0):
+    """Append an emitter of the reference schema (SceneLoader.cpp:125-141) to
+    FluidModel[0]; rotation is the quaternion (w, x, y, z). Returns the scene."""
+    fm = scene["Fluid"]["FluidModel"][0]
+    fm.setdefault("emitters", []).append(
+        {"position": list(position), "rotation": list(rotation), "velocity": velocity, "numParticles": num_particles,
+         "startTime": start_time, "type": type, "width": width, "height": height, "spacing": spacing})
+    return scene
+
+
+def emitter_scenes():
+    """Small emitter scenes (10^3-particle block at r = 0.01 in a 0.6 x 0.3 x 0.4 m
+    box, like `small`): a disc emitter shooting along -x (quaternion of a -90
+    degree turn about y) and a square one shooting along +y (+90 degrees about x),
+    keyed by the solver they run."""
+    q_mx = (0.70710678, 0.0, -0.70710678, 0.0)
+    q_py = (0.70710678, -0.70710678, 0.0, 0.0)
+    dfsph = add_emitter(small(2, viscosity_method=2, viscosity=0.01, surface_tension_method=3, surface_tension=0.1),
+                        (0.5, 0.15, 0.25), q_mx, velocity=1.0, num_particles=35, type=1, width=0.03)
+    wcsph = add_emitter(small(0, viscosity_method=2, viscosity=0.01), (0.45, 0.05, 0.3), q_py, velocity=2.0,
+                        num_particles=24, type=0, width=0.06, height=0.04, spacing=1.0, start_time=2e-4)
+    pcisph = add_emitter(small(1, viscosity_method=2, viscosity=0.01), (0.5, 0.15, 0.25), q_mx, velocity=1.0,
+                         num_particles=21, type=1, width=0.03, start_time=1e-3)
+    return {"dfsph": dfsph, "wcsph": wcsph, "pcisph": pcisph}
+
+
 def write_json(scene, path):
     with open(path, "w") as f:
         json.dump(scene, f)

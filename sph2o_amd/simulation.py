@@ -70,6 +70,22 @@ class Context:
     def num_fluid(self, scene=0):
         return self._check(self.L.sph_num_fluid(self.h, scene))
 
+    def num_active(self, scene=0):
+        """Active fluid particles (the first num_active of the num_fluid)."""
+        return self._check(self.L.sph_num_active(self.h, scene))
+
+    def add_emitter(self, scene, type, num_particles, position, rotation=(1.0, 0.0, 0.0, 0.0), velocity=1.0,
+                    start_time=0.0, width=0.1, height=0.1, spacing=1.0):
+        """FluidObject::addEmitter before init: type 0 square / 1 circle,
+        rotation the quaternion (w, x, y, z)."""
+        e = _lib.EmitterParams()
+        e.type = int(type)
+        e.num_particles = int(num_particles)
+        e.position[:] = [float(c) for c in position]
+        e.rotation[:] = [float(c) for c in rotation]
+        e.velocity, e.start_time, e.width, e.height, e.spacing = velocity, start_time, width, height, spacing
+        return self._check(self.L.sph_add_emitter(self.h, scene, ctypes.byref(e)))
+
     def num_boundary(self, scene=0):
         return self._check(self.L.sph_num_boundary(self.h, scene))
 

@@ -19,6 +19,12 @@ Fixtures:
                          fluid/boundary positions, volumes, one DFSPH step
   ref_example_wcsph.npz  the same scene stepped twice by WCSPH (no DFSPH
                          gates): a tight check of the force passes
+  ref_emit_*.npz         emitter scenes (sph2o_amd.scenes.emitter_scenes) run
+                         from the reference's own sampling: the state after
+                         the first emission and after a longer run, with the
+                         active count after every step
+
+``python tests/golden/make_golden.py emitters`` writes only the emitter files.
 """
 import json
 import os
@@ -159,9 +165,29 @@ def example_scene_wcsph():
          bvol=out["bvol"], x=out["x"], v=out["v"], rho=out["rho"], stats=out["stats"])
 
 
+def emitters():
+    """Emitter scenes: state right after the first emission (step `first`) and
+    after `nsteps`, from the reference's own lattice (no jitter: the driver's
+    --init needs the full particle count, emitted ones included)."""
+    # DFSPH runs to just past its third emission (step 25): at step 29 the
+    # divergence loop of the two builds stops one iteration apart (a count
+    # decided at the tolerance), after which dt and the states part ways
+    plan = {"dfsph": (1, 26), "wcsph": (3, 120), "pcisph": (3, 100)}
+    for tag, sc in SC.emitter_scenes().items():
+        first, nsteps = plan[tag]
+        a = S.run_reference(sc, first)
+        b = S.run_reference(sc, nsteps)
+        save("ref_emit_%s.npz" % tag, scene=scene_json(sc), first=np.array(first), nsteps=np.array(nsteps),
+             x0=a["x0"], bx=a["bx"], x1=a["x"], v1=a["v"], nact1=a["nact"], x=b["x"], v=b["v"], rho=b["rho"],
+             stats=b["stats"], nact=b["nact"], meta=b["meta"])
+
+
 def main():
     if not S.reference_available():
         raise SystemExit("compiled reference missing: make -C oracle ref")
+    if sys.argv[1:] == ["emitters"]:
+        emitters()
+        return
     cell_keys()
     support_pairs()
     neighbors("lattice", 0.0)
@@ -177,6 +203,7 @@ def main():
     trajectory("wcsph20_std", SC.small(0, viscosity_method=0, viscosity=0.5, boundary_viscosity=0.1), 20)
     example_scene()
     example_scene_wcsph()
+    emitters()
 
 
 if __name__ == "__main__":
