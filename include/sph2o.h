@@ -159,6 +159,16 @@ int64_t sph_num_boundary(const sph_ctx* ctx, int scene);
 int sph_init(sph_ctx* ctx);
 /* Advance every scene nsteps. stats: NULL or nsteps*num_scenes entries (step-major). */
 int sph_step(sph_ctx* ctx, int nsteps, sph_step_stats* stats);
+/* Exact resume. sph_save_state writes the whole initialised context (every
+ * scene's particles, time, time step, solver warm-start and loop state,
+ * emitter progress) to a binary file; sph_load_state reads it into an
+ * initialised context built from the same scenes, after which sph_step
+ * continues bit-identically. The reference has no working counterpart:
+ * SPHSimulation::importFluidState ends in exit(0) (SPHSimulation.cpp:548) and
+ * SceneLoader::writeFluid (SceneLoader.cpp:211-275) keeps only positions and
+ * velocities. */
+int sph_save_state(sph_ctx* ctx, const char* path);
+int sph_load_state(sph_ctx* ctx, const char* path);
 int sph_synchronize(sph_ctx* ctx);
 
 /* State I/O in original order; any pointer may be NULL. */

@@ -30,7 +30,7 @@ EXPORTS = [
     "sph_get_time", "sph_dump_cells", "sph_dump_neighbors", "sph_last_step_timing", "sph_set_profiling",
     "sph_kernel_count", "sph_kernel_name", "sph_kernel_stats", "sph_reset_kernel_stats",
     "sph_set_coefficients", "sph_forward_record", "sph_adjoint_seed", "sph_adjoint_run", "sph_adjoint_result",
-    "sph_recorded_field", "sph_add_emitter", "sph_num_active",
+    "sph_recorded_field", "sph_add_emitter", "sph_num_active", "sph_save_state", "sph_load_state",
 ]
 
 
@@ -105,6 +105,8 @@ def load():
         "sph_get_params": (ctypes.c_int, [vp, ctypes.c_int, P(SceneParams)]),
         "sph_num_fluid": (ctypes.c_int64, [vp, ctypes.c_int]),
         "sph_num_active": (ctypes.c_int64, [vp, ctypes.c_int]),
+        "sph_save_state": (ctypes.c_int, [vp, ctypes.c_char_p]),
+        "sph_load_state": (ctypes.c_int, [vp, ctypes.c_char_p]),
         "sph_add_emitter": (ctypes.c_int, [vp, ctypes.c_int, P(EmitterParams)]),
         "sph_num_boundary": (ctypes.c_int64, [vp, ctypes.c_int]),
         "sph_init": (ctypes.c_int, [vp]),

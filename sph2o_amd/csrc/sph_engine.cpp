@@ -1270,6 +1270,149 @@ class Engine {
         }
         HIPCHK(hipStreamSynchronize(stream));
     }
+    // ------------------------------------------------------------ state files
+    // The whole context for an exact resume: per scene its grid layout, time,
+    // solver state and emitter progress, then the slot arrays in slot order.
+    // A DFSPH load re-runs the search and the density/alpha pass on the loaded
+    // slots, as a checkpoint restore does (same keys, stable sort: the same
+    // lists, densities and alphas as the saved run had).
+    static constexpr char kStateMagic[8] = {'S', 'P', 'H', '2', 'O', 'S', 'T', '1'};
+    void saveState(const char* path) {
+        if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
+        if (!path) throw SphError{SPH_E_ARG, "path is NULL"};
+        HIPCHK(hipStreamSynchronize(stream));
+        std::ofstream f(path, std::ios::binary);
+        if (!f) throw SphError{SPH_E_ARG, std::string("cannot write ") + path};
+        auto put = [&](const void* q, size_t n) { f.write(static_cast<const char*>(q), static_cast<std::streamsize>(n)); };
+        const uint32_t ns = static_cast<uint32_t>(scenes.size());
+        const int32_t m = method;
+        const uint64_t np = NP;
+        put(kStateMagic, 8);
+        put(&ns, 4);
+        put(&m, 4);
+        put(&np, 8);
+        for (size_t i = 0; i < scenes.size(); ++i) {
+            const SceneH& sc = scenes[i];
+            const uint32_t counts[4] = {sc.nf, sc.na, sc.nb, static_cast<uint32_t>(sc.emitters.size())};
+            put(counts, sizeof(counts));
+            const int32_t grid[7] = {sc.g0[0], sc.g0[1], sc.g0[2], sc.gn[0], sc.gn[1], sc.gn[2], sc.morton};
+            const uint32_t cells[3] = {sc.nCells, sc.keyRange, sc.ovBuckets};
+            put(grid, sizeof(grid));
+            put(cells, sizeof(cells));
+            const float tt[2] = {sc.t, sc.dt};
+            const int32_t its[2] = {sc.its, sc.itsV};
+            put(tt, sizeof(tt));
+            put(its, sizeof(its));
+            put(&hState[i], sizeof(SceneState));
+            for (const auto& e : sc.emitters) {
+                put(&e.remaining, 8);
+                put(&e.next, 4);
+            }
+        }
+        auto hx = download(x.p, NP);
+        auto hv = downloadVel(0, static_cast<uint32_t>(NP));
+        auto hid = download(id.p, NP);
+        auto hkw = download(kws.p, NP);
+        auto hkv = download(kvws.p, NP);
+        put(hx.data(), NP * sizeof(float4));
+        put(hv.data(), NP * sizeof(float4));
+        put(hid.data(), NP * sizeof(uint32_t));
+        put(hkw.data(), NP * sizeof(float));
+        put(hkv.data(), NP * sizeof(float));
+        if (!f) throw SphError{SPH_E_ARG, std::string("cannot write ") + path};
+    }
+    void loadState(const char* path) {
+        if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
+        if (!path) throw SphError{SPH_E_ARG, "path is NULL"};
+        std::ifstream f(path, std::ios::binary);
+        if (!f) throw SphError{SPH_E_ARG, std::string("cannot read ") + path};
+        auto get = [&](void* q, size_t n) {
+            f.read(static_cast<char*>(q), static_cast<std::streamsize>(n));
+            if (!f) throw SphError{SPH_E_ARG, "truncated state file"};
+        };
+        char magic[8];
+        uint32_t ns = 0;
+        int32_t m = 0;
+        uint64_t np = 0;
+        get(magic, 8);
+        if (std::memcmp(magic, kStateMagic, 8) != 0) throw SphError{SPH_E_ARG, "not a state file"};
+        get(&ns, 4);
+        get(&m, 4);
+        get(&np, 8);
+        if (ns != scenes.size() || m != method || np != NP)
+            throw SphError{SPH_E_ARG, "state file does not match the context's scenes"};
+        HIPCHK(hipStreamSynchronize(stream));
+        bool layout = false;
+        std::vector<SceneState> st(scenes.size());
+        for (size_t i = 0; i < scenes.size(); ++i) {
+            SceneH& sc = scenes[i];
+            uint32_t counts[4];
+            get(counts, sizeof(counts));
+            if (counts[0] != sc.nf || counts[2] != sc.nb || counts[3] != sc.emitters.size() || counts[1] > sc.nf)
+                throw SphError{SPH_E_ARG, "state file does not match the context's scenes"};
+            int32_t grid[7];
+            uint32_t cells[3];
+            get(grid, sizeof(grid));
+            get(cells, sizeof(cells));
+            GridSave cur, g;
+            std::copy(sc.g0, sc.g0 + 3, cur.g0);
+            std::copy(sc.gn, sc.gn + 3, cur.gn);
+            cur.nCells = sc.nCells; cur.keyRange = sc.keyRange; cur.ovBuckets = sc.ovBuckets; cur.morton = sc.morton;
+            std::copy(grid, grid + 3, g.g0);
+            std::copy(grid + 3, grid + 6, g.gn);
+            g.morton = grid[6];
+            g.nCells = cells[0]; g.keyRange = cells[1]; g.ovBuckets = cells[2];
+            if (!(cur == g)) {
+                std::copy(g.g0, g.g0 + 3, sc.g0);
+                std::copy(g.gn, g.gn + 3, sc.gn);
+                sc.nCells = g.nCells; sc.keyRange = g.keyRange; sc.ovBuckets = g.ovBuckets; sc.morton = g.morton;
+                layout = true;
+            }
+            float tt[2];
+            int32_t its[2];
+            get(tt, sizeof(tt));
+            get(its, sizeof(its));
+            sc.t = tt[0];
+            sc.dt = tt[1];
+            sc.its = its[0];
+            sc.itsV = its[1];
+            get(&st[i], sizeof(SceneState));
+            for (auto& e : sc.emitters) {
+                get(&e.remaining, 8);
+                get(&e.next, 4);
+            }
+            sc.na = counts[1];
+            sc.fEnd = sc.fBegin + sc.na;
+        }
+        std::vector<float4> hx(NP), hv(NP);
+        std::vector<uint32_t> hid(NP);
+        std::vector<float> hkw(NP), hkv(NP);
+        get(hx.data(), NP * sizeof(float4));
+        get(hv.data(), NP * sizeof(float4));
+        get(hid.data(), NP * sizeof(uint32_t));
+        get(hkw.data(), NP * sizeof(float));
+        get(hkv.data(), NP * sizeof(float));
+        if (layout) uploadLayout();
+        else uploadConsts();
+        rangesDirty = true;
+        HIPCHK(hipMemcpy(x.p, hx.data(), NP * sizeof(float4), hipMemcpyHostToDevice));
+        uploadVel(0, hv);
+        if (interleaved)  // positions are duplicated in the interleaved records
+            HIPCHK(hipMemcpy2D(xv.p, 2 * sizeof(float4), hx.data(), sizeof(float4), sizeof(float4), NP,
+                               hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(id.p, hid.data(), NP * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(kws.p, hkw.data(), NP * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(kvws.p, hkv.data(), NP * sizeof(float), hipMemcpyHostToDevice));
+        std::copy(st.begin(), st.end(), hState);
+        HIPCHK(hipMemcpy(dState.p, hState, scenes.size() * sizeof(SceneState), hipMemcpyHostToDevice));
+        ++stateEpoch;
+        if (method == SPH_DFSPH) {
+            neighborSearch(false);
+            HIPCHK(launchDensityAlpha(view(), 1, stream));
+        }
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
     void tapeAlloc(int steps, int nit, int halves) {
         const size_t k = static_cast<size_t>(steps), np = NP, it = static_cast<size_t>(nit);
         tape.steps = steps;
@@ -2032,6 +2175,18 @@ int64_t sph_num_fluid(const sph_ctx* ctx, int scene) {
 int64_t sph_num_active(const sph_ctx* ctx, int scene) {
     if (!ctx || scene < 0 || static_cast<size_t>(scene) >= ctx->eng.scenes.size()) return SPH_E_ARG;
     return ctx->eng.scenes[scene].na;
+}
+int sph_save_state(sph_ctx* ctx, const char* path) {
+    return guarded(ctx, [&] {
+        ctx->eng.saveState(path);
+        return SPH_OK;
+    });
+}
+int sph_load_state(sph_ctx* ctx, const char* path) {
+    return guarded(ctx, [&] {
+        ctx->eng.loadState(path);
+        return SPH_OK;
+    });
 }
 int sph_add_emitter(sph_ctx* ctx, int scene, const sph_emitter_params* emitter) {
     return guarded(ctx, [&] {

@@ -70,6 +70,15 @@ class Context:
     def num_fluid(self, scene=0):
         return self._check(self.L.sph_num_fluid(self.h, scene))
 
+    def save_state(self, path):
+        """Whole-context state file for an exact resume (sph_save_state)."""
+        self._check(self.L.sph_save_state(self.h, str(path).encode()))
+
+    def load_state(self, path):
+        """Resume from a state file written by save_state on a context built from
+        the same scenes (after init)."""
+        self._check(self.L.sph_load_state(self.h, str(path).encode()))
+
     def num_active(self, scene=0):
         """Active fluid particles (the first num_active of the num_fluid)."""
         return self._check(self.L.sph_num_active(self.h, scene))
