@@ -33,6 +33,8 @@
  *   sph_add_emitter       FluidObject::addEmitter, FluidObject.cpp:177-193
  *                         (+ Emitter, Emitter.cpp:5-123)
  *   sph_num_active        FluidObject::getNumActiveParticles, FluidObject.h:62
+ *   sph_add_cube_boundary CubeBoundaryObject, CubeBoundaryObject.cpp:4-88,
+ *                         created by SPHSimulation::importScene, :457-474
  */
 #ifndef SPH2O_H_
 #define SPH2O_H_
@@ -59,6 +61,7 @@ extern "C" {
 #define SPH_PCISPH 1
 #define SPH_DFSPH 2
 #define SPH_BOUNDARY_NONE (-1)
+#define SPH_BOUNDARY_CUBE 0
 #define SPH_BOUNDARY_AKINCI 2
 #define SPH_VISCOSITY_STANDARD 0
 #define SPH_VISCOSITY_ARTIFICIAL 1
@@ -69,7 +72,7 @@ extern "C" {
  * (SPHStructs.h:11-64) as the reference stores them (FP32). */
 typedef struct sph_scene_params {
     int32_t simulation_method;      /* SPH_WCSPH / SPH_PCISPH / SPH_DFSPH */
-    int32_t boundary_method;        /* SPH_BOUNDARY_AKINCI or SPH_BOUNDARY_NONE */
+    int32_t boundary_method;        /* SPH_BOUNDARY_AKINCI, SPH_BOUNDARY_CUBE or SPH_BOUNDARY_NONE */
     int32_t viscosity_method;       /* -1 or SPH_VISCOSITY_* */
     int32_t surface_tension_method; /* -1 or SPH_SURFACE_TENSION_AKINCI */
     int32_t adhesion_method;        /* must be -1 */
@@ -145,6 +148,15 @@ int sph_scene_from_json(const char* path, sph_scene_params* params, int64_t* nf,
  * particles, numbered after the existing ones in emission order; they become
  * active when emitted. */
 int sph_add_emitter(sph_ctx* ctx, int scene, const sph_emitter_params* emitter);
+/* Adds a CubeBoundaryObject (CubeBoundaryObject.cpp:10-88) to a scene of
+ * boundary method SPH_BOUNDARY_CUBE before sph_init: at the end of every step
+ * the fluid is clamped into [min, max]; a clamped coordinate's outward
+ * velocity is scaled by -normal_fct and the other two components by
+ * tangential_fct. Scene JSONs of boundary method 0 get one per Boundary entry
+ * (its first box) with both factors 0, as the reference's loader leaves them
+ * (SceneLoader.cpp:168-172 reads them only for the PCISPH method). At most 4. */
+int sph_add_cube_boundary(sph_ctx* ctx, int scene, const float* min, const float* max, float normal_fct,
+                          float tangential_fct);
 int sph_num_scenes(const sph_ctx* ctx);
 int sph_get_params(const sph_ctx* ctx, int scene, sph_scene_params* out);
 /* All fluid particles of a scene, inactive emitter particles included

@@ -84,6 +84,11 @@ struct Params {
     double stiffness, gamma;
     double eta, eta_v;
     double viscosity, bviscosity, surface_tension;
+    // Akinci adhesion (AdhesionForce.cpp:5-32) and cube boundary objects
+    // (CubeBoundaryObject.cpp:10-88, boundary_method 0): lo xyz, hi xyz, normalFct, tangentialFct
+    int32_t adhesion_on, n_cubes;
+    double adhesion;
+    double cubes[4][8];
 };
 
 // SplitMix64 finaliser (a fixed bijection of 64-bit words).
@@ -391,6 +396,58 @@ class Sim {
             else if (P.viscosity_method == 0) standardVisc();
         }
         if (stOn) surfaceTension();
+        if (P.adhesion_on && akinci()) adhesion();
+    }
+    // AdhesionForce::step (AdhesionForce.cpp:5-32) with Adhesion::W as the
+    // reference returns it (Adhesion.h:25-35, no valueCoef factor):
+    // a -= beta rho0 V_b A(|r|) r / |r|, A = (-4 r^2/h + 6 r - 2h)^(1/4) on h/2 < |r| <= h
+    void adhesion() {
+        const int nf = static_cast<int>(x.size());
+        const R beta = static_cast<R>(P.adhesion);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < nf; ++i) {
+            const V ri = x[i];
+            V ai = a[i];
+            forBoundaryCandidates(keyOf(ri), [&](int b) {
+                const V rib = ri - xb[b];
+                const R rl = length(rib);
+                if (rl > 0 && 2 * rl > h && rl <= h) {
+                    const R val = static_cast<R>(-4.0) * rl * rl / h + static_cast<R>(6.0) * rl - static_cast<R>(2.0) * h;
+                    const R w = std::pow(val, static_cast<R>(0.25));
+                    ai -= (beta * rho0 * Vb[b] * w) * rib / rl;
+                }
+            });
+            a[i] = ai;
+        }
+    }
+    // CubeBoundaryObject::correctPositionAndVelocity (CubeBoundaryObject.cpp:10-88),
+    // every cube object in order, at the end of the step
+    void cubeBoundary() {
+        const int nf = static_cast<int>(x.size());
+        for (int c = 0; c < P.n_cubes; ++c) {
+            const double* q = P.cubes[c];
+            const R nfac = static_cast<R>(q[6]), tfac = static_cast<R>(q[7]);
+            for (int i = 0; i < nf; ++i) {
+                R* pc[3] = {&x[i].x, &x[i].y, &x[i].z};
+                R* vc[3] = {&v[i].x, &v[i].y, &v[i].z};
+                for (int ax = 0; ax < 3; ++ax) {
+                    const R lo = static_cast<R>(q[ax]), hi = static_cast<R>(q[3 + ax]);
+                    R* o1 = vc[(ax + 1) % 3];
+                    R* o2 = vc[(ax + 2) % 3];
+                    if (*pc[ax] < lo) {
+                        *pc[ax] = lo;
+                        if (*vc[ax] < 0) *vc[ax] *= -nfac;
+                        *o1 *= tfac;
+                        *o2 *= tfac;
+                    } else if (*pc[ax] > hi) {
+                        *pc[ax] = hi;
+                        if (*vc[ax] > 0) *vc[ax] *= -nfac;
+                        *o1 *= tfac;
+                        *o2 *= tfac;
+                    }
+                }
+            }
+        }
     }
     // XSPHViscosity::step (XSPHViscosity.cpp:6-42)
     void xsph() {
@@ -533,6 +590,7 @@ class Sim {
         }
         forces();
         integrateEuler(nullptr);
+        cubeBoundary();   // WCSPHSolver.cpp:51-61
         t += dt;
     }
     void integrateEuler(const std::vector<V>* extra) {
@@ -614,6 +672,7 @@ class Sim {
             ++its;
         }
         integrateEuler(&ap);
+        cubeBoundary();   // PCISPHSolver.cpp:53-61
         t += dt;
     }
 
@@ -761,6 +820,7 @@ class Sim {
         computeDensities();
         computeAlpha();
         correctDivergenceError();
+        cubeBoundary();   // DFSPHSolver.cpp:92-102
         t += dt;
     }
 

@@ -38,6 +38,9 @@ class Params(ctypes.Structure):
         ("eta", ctypes.c_double), ("eta_v", ctypes.c_double),
         ("viscosity", ctypes.c_double), ("bviscosity", ctypes.c_double),
         ("surface_tension", ctypes.c_double),
+        ("adhesion_on", ctypes.c_int32), ("n_cubes", ctypes.c_int32),
+        ("adhesion", ctypes.c_double),
+        ("cubes", (ctypes.c_double * 8) * 4),
     ]
 
 
@@ -122,6 +125,17 @@ def params_from_scene(scene, adaptive_dt=True):
     p.viscosity = f32(fm.get("viscosity", 0.0)) if p.viscosity_method > -1 else 0.0
     p.bviscosity = f32(fm.get("boundaryViscosity", 0.0)) if p.viscosity_method > -1 else 0.0
     p.surface_tension = f32(fm.get("surfaceTension", 0.0)) if p.st_method > -1 else 0.0
+    # adhesion exists for the Akinci method (4) with a nonzero coefficient (SPHSimulation.cpp:396-397,
+    # FluidObject::setAdhesionForce); cube objects: the first box of every Boundary entry, both
+    # factors 0 as the reference's loader leaves them (SceneLoader.cpp:168-172, SPHSimulation.cpp:457-474)
+    adh = f32(fm.get("adhesion", 0.0)) if int(fl.get("adhesionMethod", -1)) > -1 else 0.0
+    p.adhesion_on = int(int(fl.get("adhesionMethod", -1)) == 4 and adh != 0.0)
+    p.adhesion = adh
+    if p.boundary_method == 0:
+        boxes = [b["box"][0] for b in scene.get("Boundary", []) if b.get("box")]
+        p.n_cubes = len(boxes)
+        for k, b in enumerate(boxes[:4]):
+            p.cubes[k][:] = [f32(q) for q in list(b["min"]) + list(b["max"])] + [0.0, 0.0]
     return p
 
 

@@ -31,6 +31,7 @@ EXPORTS = [
     "sph_kernel_count", "sph_kernel_name", "sph_kernel_stats", "sph_reset_kernel_stats",
     "sph_set_coefficients", "sph_forward_record", "sph_adjoint_seed", "sph_adjoint_run", "sph_adjoint_result",
     "sph_recorded_field", "sph_add_emitter", "sph_num_active", "sph_save_state", "sph_load_state",
+    "sph_add_cube_boundary",
 ]
 
 
@@ -106,6 +107,7 @@ def load():
         "sph_num_fluid": (ctypes.c_int64, [vp, ctypes.c_int]),
         "sph_num_active": (ctypes.c_int64, [vp, ctypes.c_int]),
         "sph_save_state": (ctypes.c_int, [vp, ctypes.c_char_p]),
+        "sph_add_cube_boundary": (ctypes.c_int, [vp, ctypes.c_int, fp, fp, ctypes.c_float, ctypes.c_float]),
         "sph_load_state": (ctypes.c_int, [vp, ctypes.c_char_p]),
         "sph_add_emitter": (ctypes.c_int, [vp, ctypes.c_int, P(EmitterParams)]),
         "sph_num_boundary": (ctypes.c_int64, [vp, ctypes.c_int]),

@@ -289,6 +289,7 @@ int parseSceneJson(const std::string& text, SceneDesc& out, std::string& err) {
                     const std::string w = bw + ".box[" + std::to_string(j) + "]";
                     if (!rd.vec3(&bx->arr[j], "min", b.min, w) || !rd.vec3(&bx->arr[j], "max", b.max, w)) { err = rd.err; return SPH_E_SCENE; }
                     out.boxes.push_back(b);
+                    if (j == 0) out.cubes.push_back(b);   // the cube method keeps one box per object
                 }
             }
             if (const JVal* sp = bo.find("sphere")) {

@@ -17,6 +17,8 @@ struct SceneDesc {
     sph_scene_params params{};
     std::vector<BlockDesc> fluidBlocks;   // FluidModel[0].fluidBlocks
     std::vector<BlockDesc> boxes;         // Boundary[*].box (all boundary objects)
+    std::vector<BlockDesc> cubes;         // Boundary[i].box[0]: the cube objects of boundary method 0
+                                          // (SPHSimulation.cpp:457-474)
     std::vector<SphereDesc> spheres;      // Boundary[*].sphere
     std::vector<sph_emitter_params> emitters;  // FluidModel[0].emitters
     int numFluidModels = 0;

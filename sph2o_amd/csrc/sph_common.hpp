@@ -9,6 +9,8 @@ constexpr int kBlock = 256;  // particles per workgroup (4 wave64)
 
 // Per-scene constants, uploaded at init / regrid. One scene = one SPH2O
 // single-fluid scene with its own grid, particle ranges and coefficients.
+constexpr int kMaxCubes = 4;   // cube boundary objects per scene
+
 struct SceneConst {
     // grid: cell key = floor(x * invh), stored relative to the origin cell
     float h, invh;
@@ -44,7 +46,13 @@ struct SceneConst {
     // keys deadKey - (ovMask + 1) .. deadKey - 1, ranges at cellBase + nCells.
     uint32_t ovMask;
     uint32_t nCells;           // dense cells gnx * gny * gnz
-    int32_t pad;
+    // Akinci adhesion to the boundary (AdhesionForce.cpp:5-32): on if adhOn
+    int32_t adhOn;
+    float adh;                 // beta
+    // cube boundary objects (CubeBoundaryObject.cpp:10-88), applied in order
+    int32_t nCubes;
+    float cubeLo[kMaxCubes][3], cubeHi[kMaxCubes][3];
+    float cubeNormal[kMaxCubes], cubeTang[kMaxCubes];
 };
 
 // Per-scene solver state, owned by the device while a context steps: the

@@ -88,14 +88,14 @@ enum KernelId {
     K_KEYS, K_SORT, K_REORDER, K_BUILD, K_DENSITY, K_FORCES, K_SURFACE_TENSION, K_VMAX, K_PREDICT, K_RHO_STAR,
     K_VELOCITY, K_DIVERGENCE, K_SCALE, K_INTEGRATE, K_PCI_INIT, K_PCI_DENSITY, K_PCI_PACC, K_PCI_INTEGRATE,
     K_FILL, K_FINISH, K_ADJ_SEED, K_ADJ_FORCES, K_ADJ_NORMALS, K_ADJ_DENSITY, K_ADJ_FINISH, K_ADJ_VELOCITY,
-    K_ADJ_SOURCE, K_COUNT
+    K_ADJ_SOURCE, K_CUBE, K_COUNT
 };
 const char* kKernelNames[K_COUNT] = {"keys", "sort", "reorder", "build_neighbors", "density", "forces",
                                      "surface_tension", "vmax", "predict_velocity", "rho_star", "velocity",
                                      "divergence", "scale", "integrate", "pci_init", "pci_density",
                                      "pci_pressure_acc", "pci_integrate", "fill_ranges", "finish",
                                      "adj_seed", "adj_forces", "adj_normals", "adj_density", "adj_finish",
-                                     "adj_velocity", "adj_source"};
+                                     "adj_velocity", "adj_source", "cube_boundary"};
 
 // Host mirror of the reference's FP32 cubic-spline gradient (CubicSpline.h:52-74)
 // with the compiled reference's length; used for the PCISPH scaling factor.
@@ -155,6 +155,11 @@ struct SceneH {
         float next = 0.0f;          // time of the next emission
     };
     std::vector<Emitter> emitters;
+    bool adhOn = false;   // Akinci adhesion (adhesion method 4, coefficient != 0, Akinci boundary)
+    struct Cube {
+        float lo[3], hi[3], normal, tang;
+    };
+    std::vector<Cube> cubes;   // cube boundary objects (boundary method 0)
 };
 
 class Engine {
@@ -238,8 +243,6 @@ class Engine {
         if (P.boundary_method != SPH_BOUNDARY_AKINCI && !(P.boundary_method == SPH_BOUNDARY_NONE) &&
             !(nb == 0 && (P.boundary_method == 0)))
             throw SphError{SPH_E_UNSUPPORTED, "only Akinci boundaries are supported"};
-        if (P.adhesion_method > -1 && P.adhesion != 0.0f)
-            throw SphError{SPH_E_UNSUPPORTED, "adhesion force is not supported"};
         if (P.viscosity_method > 2 || P.viscosity_method < -1)
             throw SphError{SPH_E_ARG, "viscosity_method must be -1, 0, 1 or 2"};
         if (!(P.particle_radius > 0.0f)) throw SphError{SPH_E_ARG, "particle_radius must be > 0"};
@@ -314,6 +317,33 @@ class Engine {
         s.xf.resize(3 * static_cast<size_t>(s.nf), 0.0f);
         s.vf.resize(3 * static_cast<size_t>(s.nf), 0.0f);
         s.emitters.push_back(std::move(e));
+    }
+
+    // A CubeBoundaryObject (CubeBoundaryObject.cpp, SPHSimulation.cpp:457-474):
+    // the fluid is clamped into [lo, hi] at the end of every step.
+    void addCube(int si, const float lo[3], const float hi[3], float normalFct, float tangFct) {
+        if (initialized) throw SphError{SPH_E_STATE, "cube boundaries must be added before sph_init"};
+        SceneH& s = scene(si);
+        if (s.P.boundary_method != 0) throw SphError{SPH_E_ARG, "cube boundaries need boundary method 0"};
+        if (s.cubes.size() >= static_cast<size_t>(kMaxCubes))
+            throw SphError{SPH_E_UNSUPPORTED, "at most 4 cube boundary objects per scene"};
+        SceneH::Cube c;
+        for (int a = 0; a < 3; ++a) {
+            if (!std::isfinite(lo[a]) || !std::isfinite(hi[a])) throw SphError{SPH_E_NONFINITE, "non-finite cube"};
+            c.lo[a] = lo[a];
+            c.hi[a] = hi[a];
+        }
+        c.normal = normalFct;
+        c.tang = tangFct;
+        s.cubes.push_back(c);
+    }
+    bool anyCube() const {
+        for (auto& s : scenes) if (!s.cubes.empty()) return true;
+        return false;
+    }
+    bool anyAdh() const {
+        for (auto& s : scenes) if (s.adhOn) return true;
+        return false;
     }
 
     // Rotation of v by the quaternion q = (w, x, y, z), as the reference applies
@@ -412,6 +442,8 @@ class Engine {
         s.viscOn = P.viscosity_method > -1 && P.viscosity != 0.0f;
         s.stOn = P.surface_tension_method == SPH_SURFACE_TENSION_AKINCI && P.surface_tension != 0.0f;
         s.akinci = P.boundary_method == SPH_BOUNDARY_AKINCI && s.nb > 0;
+        // FluidObject::setAdhesionForce creates the force for the Akinci method (4) only
+        s.adhOn = P.adhesion_method == 4 && P.adhesion != 0.0f && s.akinci;
         if (P.simulation_method == SPH_PCISPH) {  // PCISPHSolver.cpp:403-436 with the initial dt
             const float dist = 2.0f * s.r;
             float sg[3] = {0, 0, 0}, ssq = 0;
@@ -543,6 +575,17 @@ class Engine {
             c.nf = static_cast<int32_t>(s.na);
             c.ovMask = s.ovBuckets - 1;
             c.nCells = s.nCells;
+            c.adhOn = s.adhOn ? 1 : 0;
+            c.adh = s.P.adhesion;
+            c.nCubes = static_cast<int32_t>(s.cubes.size());
+            for (size_t k = 0; k < s.cubes.size(); ++k) {
+                for (int a = 0; a < 3; ++a) {
+                    c.cubeLo[k][a] = s.cubes[k].lo[a];
+                    c.cubeHi[k][a] = s.cubes[k].hi[a];
+                }
+                c.cubeNormal[k] = s.cubes[k].normal;
+                c.cubeTang[k] = s.cubes[k].tang;
+            }
         }
         if (dScenes.n != scenes.size()) dScenes.alloc(scenes.size());
         HIPCHK(hipStreamSynchronize(stream));
@@ -641,7 +684,14 @@ class Engine {
             aabb(std::vector<float>(s.xf.begin(), s.xf.begin() + 3 * static_cast<size_t>(s.na)), lo, hi);
             aabb(s.xb, lo, hi);
             for (auto& e : s.emitters) aabb(std::vector<float>(e.P.position, e.P.position + 3), lo, hi);
-            if (s.na == 0 && s.nb == 0 && s.emitters.empty()) { lo[0] = lo[1] = lo[2] = 0; hi[0] = hi[1] = hi[2] = 0; }
+            for (auto& c : s.cubes) {
+                aabb(std::vector<float>(c.lo, c.lo + 3), lo, hi);
+                aabb(std::vector<float>(c.hi, c.hi + 3), lo, hi);
+            }
+            if (s.na == 0 && s.nb == 0 && s.emitters.empty() && s.cubes.empty()) {
+                lo[0] = lo[1] = lo[2] = 0;
+                hi[0] = hi[1] = hi[2] = 0;
+            }
             setGrid(s, lo, hi);
         }
         NP = off;
@@ -1021,7 +1071,7 @@ class Engine {
         DevView d = view();
         // 1-2. non-pressure forces, then the CFL maximum; the new dt is set on the device
         if (anyVisc() || anyST()) timed(K_FORCES, [&] { return launchForces(d, 0, stream); });
-        if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 1, stream); });
+        if (anyST() || anyAdh()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 1, stream); });
         else timed(K_VMAX, [&] { return launchVmax(d, stream); });
         finish(d, kFinishTimeStep, 2);
         // 3. predictVelocities
@@ -1070,6 +1120,7 @@ class Engine {
         }
         for (size_t s = 0; s < scenes.size(); ++s) scenes[s].itsV = hState[s].it;
         timed(K_SCALE, [&] { return launchScaleDivergence(view(), stream); });
+        if (anyCube()) timed(K_CUBE, [&] { return launchCubeBoundary(view(), stream); });   // DFSPHSolver.cpp:92-102
         // kScaleDivergence filed itsV/errV and advanced t on the device; mirror it
         for (size_t s = 0; s < scenes.size(); ++s) {
             SceneState& T = hState[s];
@@ -1089,8 +1140,9 @@ class Engine {
         DevView d = view();
         timed(K_DENSITY, [&] { return launchDensityPressure(d, stream); });
         timed(K_FORCES, [&] { return launchForces(d, 1, stream); });
-        if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
+        if (anyST() || anyAdh()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
         timed(K_INTEGRATE, [&] { return launchIntegrate(d, stream); });
+        if (anyCube()) timed(K_CUBE, [&] { return launchCubeBoundary(d, stream); });   // WCSPHSolver.cpp:51-61
         emitAll(false);   // WCSPHSolver.cpp:75, before t += dt
         for (size_t s = 0; s < scenes.size(); ++s) hState[s].t = hState[s].t + hState[s].dt;
         fileStats(false);
@@ -1103,7 +1155,7 @@ class Engine {
         DevView d = view();
         timed(K_DENSITY, [&] { return launchDensityAlpha(d, 0, stream); });
         if (anyVisc() || anyST()) timed(K_FORCES, [&] { return launchForces(d, 0, stream); });
-        if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
+        if (anyST() || anyAdh()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
         timed(K_PCI_INIT, [&] { return launchPciInit(d, stream); });
         int first = 1;
         for (auto& sc : scenes) first = std::max(first, std::max(static_cast<int>(sc.P.min_iterations), sc.its));
@@ -1118,6 +1170,7 @@ class Engine {
                 },
                 [] { return true; });
         timed(K_PCI_INTEGRATE, [&] { return launchPciIntegrate(d, stream); });
+        if (anyCube()) timed(K_CUBE, [&] { return launchCubeBoundary(d, stream); });   // PCISPHSolver.cpp:53-61
         emitAll(false);   // PCISPHSolver.cpp:77, before t += dt
         for (size_t s = 0; s < scenes.size(); ++s) {
             SceneState& T = hState[s];
@@ -1575,8 +1628,11 @@ class Engine {
         if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
         if (method == SPH_PCISPH) throw SphError{SPH_E_UNSUPPORTED, "gradients are available for WCSPH and DFSPH"};
         if (interleaved) throw SphError{SPH_E_UNSUPPORTED, "gradients need separate velocity storage"};
-        for (auto& sc : scenes)
+        for (auto& sc : scenes) {
             if (!sc.emitters.empty()) throw SphError{SPH_E_UNSUPPORTED, "gradients of scenes with emitters"};
+            if (sc.adhOn || !sc.cubes.empty())
+                throw SphError{SPH_E_UNSUPPORTED, "gradients of scenes with adhesion or cube boundaries"};
+        }
         if (!NP) throw SphError{SPH_E_STATE, "no fluid particles"};
         int nit = 0;
         for (auto& sc : scenes) {
@@ -2107,7 +2163,7 @@ namespace {
 // SPHSimulation::importScene (SPHSimulation.cpp:299-481) minus the object
 // factory: JSON -> parameters + sampled fluid and boundary particles.
 void readScene(const char* path, sph_scene_params& params, std::vector<float>& xf, std::vector<float>& xbv,
-               std::vector<sph_emitter_params>* emitters = nullptr) {
+               std::vector<sph_emitter_params>* emitters = nullptr, std::vector<BlockDesc>* cubes = nullptr) {
     if (!path) throw SphError{SPH_E_ARG, "path is NULL"};
     std::ifstream f(path);
     if (!f) throw SphError{SPH_E_SCENE, std::string("cannot read ") + path};
@@ -2118,8 +2174,9 @@ void readScene(const char* path, sph_scene_params& params, std::vector<float>& x
     const int rc = parseSceneJson(ss.str(), desc, err);
     if (rc != SPH_OK) throw SphError{rc, err};
     if (desc.numGeometries > 0) throw SphError{SPH_E_UNSUPPORTED, "mesh (geometry) boundaries are not supported"};
-    if (desc.params.boundary_method != SPH_BOUNDARY_AKINCI && (!desc.boxes.empty() || !desc.spheres.empty()))
-        throw SphError{SPH_E_UNSUPPORTED, "only Akinci boundaries are supported"};
+    if (desc.params.boundary_method != SPH_BOUNDARY_AKINCI && desc.params.boundary_method != SPH_BOUNDARY_CUBE &&
+        (!desc.boxes.empty() || !desc.spheres.empty()))
+        throw SphError{SPH_E_UNSUPPORTED, "only Akinci and cube boundaries are supported"};
     for (auto& b : desc.fluidBlocks) sampleFluidBlock(b, desc.params.particle_radius, xf);
     if (desc.params.boundary_method == SPH_BOUNDARY_AKINCI) {
         for (auto& b : desc.boxes) sampleBoxSurface(b, desc.params.particle_radius, xbv);
@@ -2127,6 +2184,11 @@ void readScene(const char* path, sph_scene_params& params, std::vector<float>& x
     }
     params = desc.params;
     if (emitters) *emitters = desc.emitters;
+    if (desc.params.boundary_method == SPH_BOUNDARY_CUBE) {
+        if (!cubes && !desc.cubes.empty())
+            throw SphError{SPH_E_UNSUPPORTED, "the scene has cube boundaries: load it with sph_load_scene_json"};
+        if (cubes) *cubes = desc.cubes;
+    }
 }
 }  // namespace
 
@@ -2135,10 +2197,12 @@ int sph_load_scene_json(sph_ctx* ctx, const char* path) {
         sph_scene_params params;
         std::vector<float> xf, xbv;
         std::vector<sph_emitter_params> em;
-        readScene(path, params, xf, xbv, &em);
+        std::vector<BlockDesc> cubes;
+        readScene(path, params, xf, xbv, &em, &cubes);
         const int id = ctx->eng.addScene(params, static_cast<int64_t>(xf.size() / 3), xf.data(), nullptr,
                                          static_cast<int64_t>(xbv.size() / 3), xbv.data());
         for (auto& e : em) ctx->eng.addEmitter(id, e);   // SPHSimulation.cpp:373-385
+        for (auto& c : cubes) ctx->eng.addCube(id, c.min, c.max, 0.0f, 0.0f);   // SPHSimulation.cpp:457-474
         return id;
     });
 }
@@ -2175,6 +2239,14 @@ int64_t sph_num_fluid(const sph_ctx* ctx, int scene) {
 int64_t sph_num_active(const sph_ctx* ctx, int scene) {
     if (!ctx || scene < 0 || static_cast<size_t>(scene) >= ctx->eng.scenes.size()) return SPH_E_ARG;
     return ctx->eng.scenes[scene].na;
+}
+int sph_add_cube_boundary(sph_ctx* ctx, int scene, const float* min, const float* max, float normal_fct,
+                          float tangential_fct) {
+    return guarded(ctx, [&] {
+        if (!min || !max) throw SphError{SPH_E_ARG, "NULL box"};
+        ctx->eng.addCube(scene, min, max, normal_fct, tangential_fct);
+        return SPH_OK;
+    });
 }
 int sph_save_state(sph_ctx* ctx, const char* path) {
     return guarded(ctx, [&] {

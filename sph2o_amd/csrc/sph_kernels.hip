@@ -755,6 +755,38 @@ SPH_UNROLL_LOOP
         ai.z += sz;
         if (valid && sub == 0) d.a[i] = ai;
     }
+    // AdhesionForce::step (AdhesionForce.cpp:5-32), the force after surface
+    // tension in the reference's order: a -= beta rho0 V_b A(|r|) r / |r| over
+    // the boundary neighbours with h/2 < |r| <= h, A = (-4 r^2/h + 6 r - 2h)^(1/4)
+    // as Adhesion::W returns it (Adhesion.h:25-35: without its valueCoef).
+    if (S.adhOn) {
+        const size_t NP = d.NP;
+        const float4 xi = d.x[ii];
+        const uint32_t nb = valid ? min(d.cntB[i], d.capB) : 0u;
+        const float bc = S.adh * S.rho0;
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        for (uint32_t s = sub; s < nb; s += kLppOnce) {
+            const uint32_t b = ldList(d.nbrB + s * NP + i);
+            const float4 xb = d.xb[b];
+            const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
+            const float dd = dRef(rx, ry, rz);
+            if (dd > S.dhalf && dd <= S.dmax) {
+                const float rl = __fsqrt_rn(dd);
+                const float val = -4.0f * rl * rl / S.h + 6.0f * rl - 2.0f * S.h;
+                const float c = bc * d.vb[b] * powf(val, 0.25f);
+                ax -= c * rx / rl;
+                ay -= c * ry / rl;
+                az -= c * rz / rl;
+            }
+        }
+        ax = lppSum<kLppOnce>(ax);
+        ay = lppSum<kLppOnce>(ay);
+        az = lppSum<kLppOnce>(az);
+        ai.x += ax;
+        ai.y += ay;
+        ai.z += az;
+        if (valid && sub == 0) d.a[i] = ai;
+    }
     if (vmax && valid && sub == 0) {
         const float4 vi = VEL(d, i);
         const float cx = vi.x + (ai.x + S.gx) * dt;
@@ -1059,6 +1091,42 @@ __global__ void __launch_bounds__(kBlock) kIntegrate(DevView d) {
 // initPressure (PCISPHSolver.cpp:82-118) + first predictVelocityAndPosition
 // (:120-147); block 0 starts the loop with the error carried from the last
 // step (PCISPHSolver.h:18).
+// CubeBoundaryObject::correctPositionAndVelocity (CubeBoundaryObject.cpp:10-88)
+// for each of the scene's cube objects in order: per axis, a position below
+// min / above max is clamped, an outward normal velocity scaled by
+// -normalFct, and the two other components by tangentialFct.
+__global__ void __launch_bounds__(kBlock) kCubeBoundary(DevView d) {
+    SCENE_PROLOGUE(2)
+    (void)dt;
+    (void)dt2;
+    if (!valid || S.nCubes == 0) return;
+    float4 p = d.x[i], v = VEL(d, i);
+    float* pc[3] = {&p.x, &p.y, &p.z};
+    float* vc[3] = {&v.x, &v.y, &v.z};
+    for (int c = 0; c < S.nCubes; ++c) {
+        const float nf = S.cubeNormal[c], tf = S.cubeTang[c];
+        for (int a = 0; a < 3; ++a) {
+            const int b = (a + 1) % 3, e = (a + 2) % 3;
+            if (*pc[a] < S.cubeLo[c][a]) {
+                *pc[a] = S.cubeLo[c][a];
+                if (*vc[a] < 0.0f) *vc[a] *= -nf;
+                *vc[b] *= tf;
+                *vc[e] *= tf;
+            } else if (*pc[a] > S.cubeHi[c][a]) {
+                *pc[a] = S.cubeHi[c][a];
+                if (*vc[a] > 0.0f) *vc[a] *= -nf;
+                *vc[b] *= tf;
+                *vc[e] *= tf;
+            }
+        }
+    }
+    d.x[i] = p;
+#if SPH_XV
+    d.xv[2 * static_cast<size_t>(i)] = p;
+#endif
+    VEL(d, i) = v;
+}
+
 __global__ void __launch_bounds__(kBlock) kPciInit(DevView d) {
     SCENE_PROLOGUE(2)
     if (blockIdx.x == 0)
@@ -1284,6 +1352,7 @@ hipError_t launchFinish(const DevView& d, int kind, int slot, hipStream_t st) {
     LAUNCH(kFinish, d.numScenes, d, kind, slot);
 }
 hipError_t launchCellsOut(const DevView& d, int32_t* cells, hipStream_t st) { LAUNCH(kCellsOut, d.numBlocks, d, cells); }
+hipError_t launchCubeBoundary(const DevView& d, hipStream_t st) { LAUNCH(kCubeBoundary, d.numBlocks, d); }
 hipError_t launchCheckFinite(const DevView& d, hipStream_t st) { LAUNCH(kCheckFinite, d.numBlocks, d); }
 
 }  // namespace sph

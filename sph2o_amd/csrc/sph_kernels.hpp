@@ -63,6 +63,8 @@ hipError_t launchFillRanges(uint2* r, size_t n, hipStream_t st);
 hipError_t launchReorder(const DevView& d, const uint32_t* keysSorted, const uint32_t* valsSorted, float4* xAlt,
                          float4* vAlt, float4* xvAlt, float* kwsAlt, float* kvwsAlt, uint32_t* idAlt, hipStream_t st);
 hipError_t launchBuildNeighbors(const DevView& d, hipStream_t st);
+// cube boundary objects of every scene (no-op for scenes without)
+hipError_t launchCubeBoundary(const DevView& d, hipStream_t st);
 hipError_t launchBoundaryVolume(const DevView& d, int scene, uint32_t bBegin, uint32_t bEnd, hipStream_t st);
 
 hipError_t launchDensityAlpha(const DevView& d, int withAlpha, hipStream_t st);

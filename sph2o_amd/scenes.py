@@ -93,6 +93,28 @@ def emitter_scenes():
     return {"dfsph": dfsph, "wcsph": wcsph, "pcisph": pcisph}
 
 
+def boundary_option_scenes():
+    """Small scenes (10^3 block at r = 0.01, like `small`) for the force and
+    boundary options around the step: Akinci adhesion (method 4, beta = 5) with
+    WCSPH and DFSPH, and cube boundary objects (boundary method 0, no boundary
+    particles) with WCSPH (a wall at x = 0.2 cutting the block's last layer)
+    and DFSPH (a floor at z = 0.025 under the falling block)."""
+    out = {}
+    for tag, method in (("adh_wcsph", 0), ("adh_dfsph", 2)):
+        extra = dict(surface_tension_method=3, surface_tension=0.1) if method == 2 else {}
+        sc = small(method, viscosity_method=2, viscosity=0.01, **extra)
+        sc["Fluid"]["adhesionMethod"] = 4
+        sc["Fluid"]["FluidModel"][0]["adhesion"] = 5.0
+        out[tag] = sc
+    for tag, method, lo, hi in (("cube_wcsph", 0, (0.0, 0.0, 0.025), (0.2, 0.3, 0.4)),
+                                ("cube_dfsph", 2, (0.0, 0.0, 0.025), (0.6, 0.3, 0.4))):
+        sc = small(method, viscosity_method=2, viscosity=0.01)
+        sc["Configuration"]["boundaryMethod"] = 0
+        sc["Boundary"] = [{"box": [{"min": list(lo), "max": list(hi)}], "sphere": [], "geometry": []}]
+        out[tag] = sc
+    return out
+
+
 def write_json(scene, path):
     with open(path, "w") as f:
         json.dump(scene, f)

@@ -24,7 +24,12 @@ Fixtures:
                          the first emission and after a longer run, with the
                          active count after every step
 
-``python tests/golden/make_golden.py emitters`` writes only the emitter files.
+  ref_opt_*.npz          Akinci adhesion and cube-boundary scenes
+                         (sph2o_amd.scenes.boundary_option_scenes): the state
+                         after one step and after a short run
+
+``python tests/golden/make_golden.py emitters`` (or ``options``) writes only
+the emitter (or adhesion / cube) files.
 """
 import json
 import os
@@ -182,11 +187,25 @@ def emitters():
              stats=b["stats"], nact=b["nact"], meta=b["meta"])
 
 
+def boundary_options():
+    """Adhesion and cube-boundary scenes (sph2o_amd.scenes.boundary_option_scenes)
+    from the reference's own lattice: the state after 1 step and after a run."""
+    plan = {"adh_wcsph": 20, "adh_dfsph": 20, "cube_wcsph": 20, "cube_dfsph": 40}
+    for tag, sc in SC.boundary_option_scenes().items():
+        a = S.run_reference(sc, 1)
+        b = S.run_reference(sc, plan[tag])
+        save("ref_opt_%s.npz" % tag, scene=scene_json(sc), nsteps=np.array(plan[tag]), x0=a["x0"], bx=a["bx"],
+             x1=a["x"], v1=a["v"], a1=a["a"], rho1=a["rho"], x=b["x"], v=b["v"], rho=b["rho"], stats=b["stats"])
+
+
 def main():
     if not S.reference_available():
         raise SystemExit("compiled reference missing: make -C oracle ref")
     if sys.argv[1:] == ["emitters"]:
         emitters()
+        return
+    if sys.argv[1:] == ["options"]:
+        boundary_options()
         return
     cell_keys()
     support_pairs()
@@ -204,6 +223,7 @@ def main():
     example_scene()
     example_scene_wcsph()
     emitters()
+    boundary_options()
 
 
 if __name__ == "__main__":
