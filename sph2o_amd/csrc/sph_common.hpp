@@ -46,6 +46,10 @@ struct SceneConst {
     // keys deadKey - (ovMask + 1) .. deadKey - 1, ranges at cellBase + nCells.
     uint32_t ovMask;
     uint32_t nCells;           // dense cells gnx * gny * gnz
+    // counting sort: the scene's sort positions ("order indices") start at
+    // ordBase: dense cells in key order, then the overflow buckets, then one
+    // for the padding slots
+    uint32_t ordBase;
     // Akinci adhesion to the boundary (AdhesionForce.cpp:5-32): on if adhOn
     int32_t adhOn;
     float adh;                 // beta
@@ -81,7 +85,8 @@ enum FlagSlot {
     kFlagMaxBoundaryNbr = 2,
     kFlagNonFinite = 3,
     kFlagFar = 4,
-    kFlagSlots = 5
+    kFlagBigCell = 5,   // a cell held more particles than the counting sort orders in place
+    kFlagSlots = 6
 };
 // Reduction output slots per scene (double each).
 enum RedSlot { kRedSum = 0, kRedMax = 1, kRedSlots = 2 };

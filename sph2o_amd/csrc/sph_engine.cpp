@@ -141,7 +141,7 @@ struct SceneH {
     // layout
     uint32_t fBegin = 0, fEnd = 0, fPadEnd = 0, blockBegin = 0, blockEnd = 0, bBegin = 0, bEnd = 0;
     int g0[3] = {0, 0, 0}, gn[3] = {1, 1, 1};
-    uint32_t cellBase = 0, nCells = 0, keyBase = 0, keyRange = 0;
+    uint32_t cellBase = 0, nCells = 0, keyBase = 0, keyRange = 0, ordBase = 0;
     uint32_t ovBuckets = 1024;  // overflow buckets for cells outside the dense box (power of two)
     int morton = 0;
     std::vector<uint32_t> bOrder;  // sorted boundary position -> original index
@@ -187,6 +187,12 @@ class Engine {
     DevBuf<uint32_t> id, idAlt, keys, keysAlt, vals, valsAlt;
     DevBuf<uint2> fRange, bRange;
     DevBuf<uint32_t> cellOf;     // per slot: the cell range it opened in the last reorder, or ~0
+    // counting sort of the keys (the default; radix sort when a cell overflows it)
+    DevBuf<uint32_t> rankOf, ordToCell, counts, starts, li;
+    DevBuf<unsigned char> scanTemp;
+    size_t scanTempBytes = 0;
+    uint32_t ordTotal = 0;
+    bool radixOnce = false;      // the next search uses the radix sort (a cell was too crowded)
     bool rangesDirty = true;     // fRange holds ranges cellOf does not list: clear all of it
     DevBuf<uint32_t> nbrF, cntF, nbrB, cntB;
     DevBuf<double> partials;
@@ -525,8 +531,56 @@ class Engine {
         totalCells = cells;
         keyBits = 1;
         while (keyBits < 32 && (1ull << keyBits) < keysTot) ++keyBits;
+        uploadOrderTables();
         uploadConsts();
         uploadBoundaryGrid();
+    }
+
+    // Counting-sort tables of the layout: each scene's order indices are its
+    // dense cells in key order (Morton or linear), then its overflow buckets,
+    // then one entry for its padding slots.
+    void uploadOrderTables() {
+        std::vector<uint32_t> rank(totalCells ? totalCells : 1, 0u);
+        std::vector<uint32_t> o2c;
+        uint32_t ob = 0;
+        for (auto& sc : scenes) {
+            sc.ordBase = ob;
+            std::vector<uint32_t> lins(sc.nCells);
+            std::iota(lins.begin(), lins.end(), 0u);
+            if (sc.morton) {
+                std::vector<uint64_t> kl(sc.nCells);
+                const uint32_t nx = static_cast<uint32_t>(sc.gn[0]), ny = static_cast<uint32_t>(sc.gn[1]);
+                for (uint32_t l = 0; l < sc.nCells; ++l) {
+                    const uint32_t cx = l % nx, cy = (l / nx) % ny, cz = l / (nx * ny);
+                    const uint64_t key = spread3h(cx) | (spread3h(cy) << 1) | (spread3h(cz) << 2);
+                    kl[l] = (key << 32) | l;
+                }
+                std::sort(kl.begin(), kl.end());
+                for (uint32_t r = 0; r < sc.nCells; ++r) lins[r] = static_cast<uint32_t>(kl[r] & 0xffffffffu);
+            }
+            for (uint32_t r = 0; r < sc.nCells; ++r) {
+                rank[sc.cellBase + lins[r]] = r;
+                o2c.push_back(sc.cellBase + lins[r]);
+            }
+            for (uint32_t b = 0; b < sc.ovBuckets; ++b) {
+                rank[sc.cellBase + sc.nCells + b] = sc.nCells + b;
+                o2c.push_back(sc.cellBase + sc.nCells + b);
+            }
+            o2c.push_back(~0u);   // padding slots
+            ob += sc.nCells + sc.ovBuckets + 1;
+        }
+        ordTotal = ob;
+        rankOf.alloc(rank.size());
+        HIPCHK(hipMemcpy(rankOf.p, rank.data(), rank.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        ordToCell.alloc(o2c.size() ? o2c.size() : 1);
+        if (!o2c.empty())
+            HIPCHK(hipMemcpy(ordToCell.p, o2c.data(), o2c.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        counts.alloc(static_cast<size_t>(ordTotal) + 1);
+        starts.alloc(static_cast<size_t>(ordTotal) + 1);
+        HIPCHK(hipMemset(counts.p, 0, (static_cast<size_t>(ordTotal) + 1) * sizeof(uint32_t)));
+        scanTempBytes = 0;
+        HIPCHK(scanCounts(nullptr, scanTempBytes, counts.p, starts.p, ordTotal + 1, stream));
+        scanTemp.alloc(scanTempBytes + 256);
     }
 
     // Per-scene constants (grid, layout, coefficients) to the device.
@@ -575,6 +629,7 @@ class Engine {
             c.nf = static_cast<int32_t>(s.na);
             c.ovMask = s.ovBuckets - 1;
             c.nCells = s.nCells;
+            c.ordBase = s.ordBase;
             c.adhOn = s.adhOn ? 1 : 0;
             c.adh = s.P.adhesion;
             c.nCubes = static_cast<int32_t>(s.cubes.size());
@@ -649,6 +704,7 @@ class Engine {
         d.id = id.p;
         d.xb = xb.p; d.vb = vb.p;
         d.fRange = fRange.p; d.bRange = bRange.p; d.cellOf = cellOf.p;
+        d.rankOf = rankOf.p; d.ordToCell = ordToCell.p; d.counts = counts.p; d.starts = starts.p; d.li = li.p;
         d.snapV = snapV.p; d.snapKvws = snapKvws.p;
         d.nbrF = nbrF.p; d.cntF = cntF.p; d.nbrB = nbrB.p; d.cntB = cntB.p;
         d.capF = capF; d.capB = capB;
@@ -717,7 +773,7 @@ class Engine {
             xvAlt.alloc(2 * np);
         }
         for (auto* b : {&rho, &alpha, &p, &k, &kws, &kwsAlt, &kvws, &kvwsAlt, &aux}) b->alloc(np);
-        for (auto* b : {&id, &idAlt, &keys, &keysAlt, &vals, &valsAlt, &cntF, &cntB, &cellOf}) b->alloc(np);
+        for (auto* b : {&id, &idAlt, &keys, &keysAlt, &vals, &valsAlt, &cntF, &cntB, &cellOf, &li}) b->alloc(np);
         HIPCHK(hipMemset(cellOf.p, 0xFF, np * sizeof(uint32_t)));
         allocNeighborLists();
         // the DFSPH source passes run sphKernelsLanesPerParticle() launch blocks per particle block
@@ -871,20 +927,39 @@ class Engine {
     void searchOnce(bool dfsphAdvance) {
         HIPCHK(hipMemsetAsync(dFlags, 0, scenes.size() * kFlagSlots * sizeof(int32_t), stream));
         DevView d = view();
-        const bool fill = rangesDirty;
-        timed(K_KEYS, [&] { return launchKeys(d, dfsphAdvance ? 1 : 0, fill ? 0 : 1, keys.p, vals.p, stream); });
-        if (fill) timed(K_FILL, [&] { return launchFillRanges(fRange.p, totalCells, stream); });
-        rangesDirty = false;   // the reorder below lists every range it writes in cellOf
-        bool swapped = false;
-        timed(K_SORT, [&] {
-            return sortPairs(sortTemp.p, sortTempBytes, keys.p, keysAlt.p, vals.p, valsAlt.p, static_cast<int>(NP),
-                             keyBits, stream, &swapped);
-        });
-        const uint32_t* ks = swapped ? keysAlt.p : keys.p;
-        const uint32_t* vs = swapped ? valsAlt.p : vals.p;
-        timed(K_REORDER, [&] {
-            return launchReorder(d, ks, vs, xAlt.p, vAlt.p, xvAlt.p, kwsAlt.p, kvwsAlt.p, idAlt.p, stream);
-        });
+        if (!radixOnce) {
+            // counting sort: keys = order indices counted per index, then an
+            // exclusive scan, the scatter, and the per-index fix-up that also
+            // writes every cell range (so no range fill is needed)
+            timed(K_KEYS, [&] { return launchKeys(d, dfsphAdvance ? 1 : 0, 0, 1, keys.p, vals.p, stream); });
+            timed(K_SORT, [&] {
+                hipError_t e = scanCounts(scanTemp.p, scanTempBytes, counts.p, starts.p, ordTotal + 1, stream);
+                if (e == hipSuccess) e = launchScatter(d, keys.p, vals.p, stream);
+                if (e == hipSuccess) e = launchOrderFix(d, vals.p, ordTotal, stream);
+                return e;
+            });
+            rangesDirty = true;   // cellOf does not list these ranges: a radix search fills first
+            timed(K_REORDER, [&] {
+                return launchReorder(d, keys.p, vals.p, xAlt.p, vAlt.p, xvAlt.p, kwsAlt.p, kvwsAlt.p, idAlt.p, 0,
+                                     stream);
+            });
+        } else {
+            radixOnce = false;
+            const bool fill = rangesDirty;
+            timed(K_KEYS, [&] { return launchKeys(d, dfsphAdvance ? 1 : 0, fill ? 0 : 1, 0, keys.p, vals.p, stream); });
+            if (fill) timed(K_FILL, [&] { return launchFillRanges(fRange.p, totalCells, stream); });
+            rangesDirty = false;   // the reorder below lists every range it writes in cellOf
+            bool swapped = false;
+            timed(K_SORT, [&] {
+                return sortPairs(sortTemp.p, sortTempBytes, keys.p, keysAlt.p, vals.p, valsAlt.p, static_cast<int>(NP),
+                                 keyBits, stream, &swapped);
+            });
+            const uint32_t* ks = swapped ? keysAlt.p : keys.p;
+            const uint32_t* vs = swapped ? valsAlt.p : vals.p;
+            timed(K_REORDER, [&] {
+                return launchReorder(d, ks, vs, xAlt.p, vAlt.p, xvAlt.p, kwsAlt.p, kvwsAlt.p, idAlt.p, 1, stream);
+            });
+        }
         std::swap(x.p, xAlt.p);
         std::swap(v.p, vAlt.p);
         std::swap(xv.p, xvAlt.p);
@@ -903,6 +978,7 @@ class Engine {
         for (size_t si = 0; si < scenes.size(); ++si) {
             const int32_t* f = hFlags + si * kFlagSlots;
             if (f[kFlagNonFinite]) throw SphError{SPH_E_NONFINITE, "non-finite particle position"};
+            if (f[kFlagBigCell]) radixOnce = true;   // redo this search with the radix sort
             if (f[kFlagFar]) throw SphError{SPH_E_OUT_OF_GRID, "particle beyond 1e8 cells from the origin"};
             maxF = std::max(maxF, f[kFlagMaxFluidNbr]);
             maxB = std::max(maxB, f[kFlagMaxBoundaryNbr]);
@@ -924,6 +1000,7 @@ class Engine {
             allocNeighborLists();
             return false;
         }
+        if (radixOnce) return false;   // a cell was too crowded for the counting sort's in-place fix-up
         return true;
     }
     static void growBuckets(SceneH& s, uint32_t outside) {

@@ -240,8 +240,8 @@ __device__ __forceinline__ void loopStart(SceneState& T, double err, double tol,
 // (DFSPHSolver::updatePositions :204-224 and :516-530; k_ws is cleared
 // instead when the density loop did not run), and block 0 files the density
 // loop's statistics.
-__global__ void __launch_bounds__(kBlock) kKeys(DevView d, int dfsphAdvance, int clearCells, uint32_t* keys,
-                                                uint32_t* vals) {
+__global__ void __launch_bounds__(kBlock) kKeys(DevView d, int dfsphAdvance, int clearCells, int counting,
+                                                uint32_t* keys, uint32_t* vals) {
     const uint32_t lb = logicalBlock();
     const int sIdx = d.blockScene[lb];
     const SceneConst& S = d.scenes[sIdx];
@@ -260,48 +260,131 @@ __global__ void __launch_bounds__(kBlock) kKeys(DevView d, int dfsphAdvance, int
             U.capped = (U.it >= C.maxIt && U.err > static_cast<double>(C.eta)) ? 1 : 0;
         }
     }
-    vals[i] = i;
-    if (i >= S.fEnd) {
-        keys[i] = S.deadKey;
-        return;
-    }
-    float4 p = d.x[i];
-    if (dfsphAdvance) {
-        // w != 0 marks a particle emitted in this step: the reference emits
-        // after updatePositions (DFSPHSolver.cpp:72), so it does not move yet
-        if (p.w == 0.0f) {
-            const float dt = T.dt;
-            const float4 vi = VEL(d, i);
-            p.x += vi.x * dt;
-            p.y += vi.y * dt;
-            p.z += vi.z * dt;
+    const bool live = i < S.fEnd;
+    uint32_t key = S.deadKey, ord = S.ordBase + S.nCells + S.ovMask + 1;   // padding slots
+    bool bucket = false;
+    if (live) {
+        float4 p = d.x[i];
+        if (dfsphAdvance) {
+            // w != 0 marks a particle emitted in this step: the reference emits
+            // after updatePositions (DFSPHSolver.cpp:72), so it does not move yet
+            if (p.w == 0.0f) {
+                const float dt = T.dt;
+                const float4 vi = VEL(d, i);
+                p.x += vi.x * dt;
+                p.y += vi.y * dt;
+                p.z += vi.z * dt;
+            }
+            p.w = 0.0f;
+            d.x[i] = p;
+            d.kws[i] *= (T.it > 0 ? T.dt2 : 0.0f);
         }
-        p.w = 0.0f;
-        d.x[i] = p;
-        d.kws[i] *= (T.it > 0 ? T.dt2 : 0.0f);
-    }
-    int gx, gy, gz;
-    const int bad = globalCell(S, p, gx, gy, gz);
-    if (bad) atomicOr(&d.flags[sIdx * kFlagSlots + (bad == 1 ? kFlagNonFinite : kFlagFar)], 1);
-    bool bucket;
-    const uint32_t c = cellSlot(S, gx, gy, gz, bucket);
-    keys[i] = bucket ? S.deadKey - (S.ovMask + 1) + (c - S.cellBase - S.nCells)
+        int gx, gy, gz;
+        const int bad = globalCell(S, p, gx, gy, gz);
+        if (bad) atomicOr(&d.flags[sIdx * kFlagSlots + (bad == 1 ? kFlagNonFinite : kFlagFar)], 1);
+        const uint32_t c = cellSlot(S, gx, gy, gz, bucket);
+        key = bucket ? S.deadKey - (S.ovMask + 1) + (c - S.cellBase - S.nCells)
                      : cellKey(S, gx - S.g0x, gy - S.g0y, gz - S.g0z);
+        if (counting) ord = S.ordBase + d.rankOf[c];
+    }
     // particles in overflow buckets, one atomic per wave
     const unsigned long long outside = __ballot(bucket);
     if ((threadIdx.x & 63) == 0 && outside)
         atomicAdd(&d.flags[sIdx * kFlagSlots + kFlagOutOfGrid], static_cast<int>(__popcll(outside)));
+    if (!counting) {
+        keys[i] = key;
+        vals[i] = i;
+        return;
+    }
+    // Counting sort: slots come in the last search's order, so a wave holds
+    // runs of equal order indices; the first lane of each run counts the run
+    // with one atomic and hands its base to the others. The bases follow the
+    // atomics' order; kOrderFix restores ascending slots within each index.
+    // Padding slots are the scene's last fPadEnd - fEnd slots in both orders:
+    // they keep their relative order (li = i - fEnd) and the first one counts
+    // them all, so their order index needs no fix-up.
+    keys[i] = ord;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t prev = __shfl_up(ord, 1u, 64);
+    const bool head = live && (lane == 0 || prev != ord);
+    const unsigned long long hm = __ballot(head);
+    const unsigned long long upTo = hm & ((2ull << lane) - 1ull);   // heads at or below this lane
+    const uint32_t h = upTo ? 63u - static_cast<uint32_t>(__clzll(upTo)) : lane;
+    uint32_t base = 0;
+    if (head) {
+        // the run ends at the next head or at the first padding lane
+        const unsigned long long after = lane == 63u ? 0ull : (hm >> (lane + 1u));
+        uint32_t next = after ? lane + 1u + static_cast<uint32_t>(__ffsll(static_cast<long long>(after)) - 1) : 64u;
+        const uint32_t liveEnd = S.fEnd > i ? (S.fEnd - i < 64u - lane ? lane + (S.fEnd - i) : 64u) : lane;
+        next = min(next, liveEnd);
+        base = atomicAdd(&d.counts[ord], next - lane);
+    }
+    base = __shfl(base, static_cast<int>(h), 64);
+    if (live) {
+        d.li[i] = base + (lane - h);
+    } else {
+        d.li[i] = i - S.fEnd;
+        if (i == S.fEnd) atomicAdd(&d.counts[ord], S.fPadEnd - S.fEnd);
+    }
 }
-
 __global__ void kFillRanges(uint2* r, size_t n) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) r[i] = make_uint2(0u, 0u);
 }
 
+// Counting sort, step 2: slot i's place in the sorted order.
+__global__ void __launch_bounds__(kBlock) kScatter(DevView d, const uint32_t* keys, uint32_t* perm) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;   // over all NP slots
+    perm[d.starts[keys[i]] + d.li[i]] = i;
+}
+
+// Counting sort, step 3, one thread per order index: ascending slots within
+// the index (the order a stable radix sort of the same keys gives), the cell
+// range, and a zero count for the next search.
+constexpr uint32_t kFixInRegs = 32, kFixMax = 256;
+__global__ void __launch_bounds__(kBlock) kOrderFix(DevView d, uint32_t* perm, uint32_t ordTotal) {
+    const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
+    if (o >= ordTotal) return;
+    const uint32_t b = d.starts[o], e = d.starts[o + 1], n = e - b;
+    d.counts[o] = 0u;
+    const uint32_t cell = d.ordToCell[o];   // ~0: the scene's padding slots, already in order
+    if (n > 1 && cell != ~0u) {
+        if (n <= kFixInRegs) {
+            uint32_t q[kFixInRegs];
+            for (uint32_t k = 0; k < n; ++k) q[k] = perm[b + k];
+            for (uint32_t k = 1; k < n; ++k) {
+                const uint32_t v = q[k];
+                uint32_t j = k;
+                while (j > 0 && q[j - 1] > v) {
+                    q[j] = q[j - 1];
+                    --j;
+                }
+                q[j] = v;
+            }
+            for (uint32_t k = 0; k < n; ++k) perm[b + k] = q[k];
+        } else if (n <= kFixMax) {
+            for (uint32_t k = b + 1; k < e; ++k) {
+                const uint32_t v = perm[k];
+                uint32_t j = k;
+                while (j > b && perm[j - 1] > v) {
+                    perm[j] = perm[j - 1];
+                    --j;
+                }
+                perm[j] = v;
+            }
+        } else {
+            // a crowded cell: leave it to the radix sort (the engine redoes the search)
+            const uint32_t s = b < d.NP ? static_cast<uint32_t>(d.blockScene[b / kBlock]) : 0u;
+            atomicOr(&d.flags[s * kFlagSlots + kFlagBigCell], 1);
+        }
+    }
+    if (cell != ~0u) d.fRange[cell] = n ? make_uint2(b, e) : make_uint2(0u, 0u);
+}
+
 // Gather the persistent state into sorted order and record cell ranges.
 __global__ void __launch_bounds__(kBlock) kReorder(DevView d, const uint32_t* keysSorted, const uint32_t* valsSorted,
                                                    float4* xAlt, float4* vAlt, float4* xvAlt, float* kwsAlt,
-                                                   float* kvwsAlt, uint32_t* idAlt) {
+                                                   float* kvwsAlt, uint32_t* idAlt, int ranges) {
     const uint32_t lb = logicalBlock();
     const int sIdx = d.blockScene[lb];
     const SceneConst& S = d.scenes[sIdx];
@@ -323,6 +406,7 @@ __global__ void __launch_bounds__(kBlock) kReorder(DevView d, const uint32_t* ke
         d.snapV[i] = VEL(d, o);
         d.snapKvws[i] = kv;
     }
+    if (!ranges) return;
     uint32_t first = ~0u;
     if (i < S.fEnd) {
         const uint32_t key = keysSorted[i];
@@ -1302,8 +1386,17 @@ __global__ void __launch_bounds__(kBlock) kFinish(DevView d, int kind, int slot)
         return hipGetLastError();                                                                \
     } while (0)
 
-hipError_t launchKeys(const DevView& d, int adv, int clearCells, uint32_t* keys, uint32_t* vals, hipStream_t st) {
-    LAUNCH(kKeys, d.numBlocks, d, adv, clearCells, keys, vals);
+hipError_t launchKeys(const DevView& d, int adv, int clearCells, int counting, uint32_t* keys, uint32_t* vals,
+                      hipStream_t st) {
+    LAUNCH(kKeys, d.numBlocks, d, adv, clearCells, counting, keys, vals);
+}
+hipError_t launchScatter(const DevView& d, const uint32_t* keys, uint32_t* perm, hipStream_t st) {
+    LAUNCH(kScatter, d.numBlocks, d, keys, perm);
+}
+hipError_t launchOrderFix(const DevView& d, uint32_t* perm, uint32_t ordTotal, hipStream_t st) {
+    const uint32_t g = (ordTotal + kBlock - 1) / kBlock;
+    if (g == 0) return hipSuccess;
+    LAUNCH(kOrderFix, g, d, perm, ordTotal);
 }
 hipError_t launchFillRanges(uint2* r, size_t n, hipStream_t st) {
     const size_t g = (n + kBlock - 1) / kBlock;
@@ -1311,8 +1404,8 @@ hipError_t launchFillRanges(uint2* r, size_t n, hipStream_t st) {
     LAUNCH(kFillRanges, g, r, n);
 }
 hipError_t launchReorder(const DevView& d, const uint32_t* ks, const uint32_t* vs, float4* xa, float4* va, float4* xva,
-                         float* kwa, float* kva, uint32_t* ida, hipStream_t st) {
-    LAUNCH(kReorder, d.numBlocks, d, ks, vs, xa, va, xva, kwa, kva, ida);
+                         float* kwa, float* kva, uint32_t* ida, int ranges, hipStream_t st) {
+    LAUNCH(kReorder, d.numBlocks, d, ks, vs, xa, va, xva, kwa, kva, ida, ranges);
 }
 hipError_t launchBuildNeighbors(const DevView& d, hipStream_t st) {
     LAUNCH(kBuildNeighbors, d.numBlocks * kLppBuild, d);
@@ -1358,8 +1451,14 @@ hipError_t launchCheckFinite(const DevView& d, hipStream_t st) { LAUNCH(kCheckFi
 }  // namespace sph
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 namespace sph {
+hipError_t scanCounts(void* temp, size_t& tempBytes, const uint32_t* counts, uint32_t* starts, uint32_t n,
+                      hipStream_t st) {
+    return rocprim::exclusive_scan(temp, tempBytes, counts, starts, 0u, static_cast<size_t>(n), rocprim::plus<uint32_t>(),
+                                   st);
+}
 hipError_t sortPairs(void* temp, size_t& tempBytes, uint32_t* k0, uint32_t* k1, uint32_t* v0, uint32_t* v1, int n,
                      int bits, hipStream_t st, bool* swapped) {
     // rocprim picks a block sort + merge sort up to 2^20 keys and onesweep

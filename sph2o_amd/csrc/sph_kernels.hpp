@@ -40,6 +40,15 @@ struct DevView {
     uint2* fRange;
     const uint2* bRange;
     uint32_t* cellOf;   // slot -> its cell's fRange index if it is the cell's first slot, else ~0
+    // counting sort (per layout): fRange index -> rank within the scene's order
+    // indices; order index -> fRange index (~0 for the padding entry); per
+    // search: particles per order index, their exclusive prefix sums, and
+    // each slot's place within its order index
+    const uint32_t* rankOf;
+    const uint32_t* ordToCell;
+    uint32_t* counts;
+    uint32_t* starts;
+    uint32_t* li;
     // DFSPH: copy of v and kv_ws as the reorder leaves them (undo of a divergence
     // solve run on lists that turn out invalid); null for other methods
     float4* snapV;
@@ -56,12 +65,26 @@ struct DevView {
 };
 
 // clearCells: first zero the ranges the last reorder wrote (d.cellOf), so no
-// separate fill of the whole range table is needed
-hipError_t launchKeys(const DevView& d, int dfsphAdvance, int clearCells, uint32_t* keys, uint32_t* vals,
+// separate fill of the whole range table is needed. counting: keys are order
+// indices, counted into d.counts with each slot's place in d.li (counting
+// sort); otherwise radix-sort keys with vals = slot.
+hipError_t launchKeys(const DevView& d, int dfsphAdvance, int clearCells, int counting, uint32_t* keys,
+                      uint32_t* vals, hipStream_t st);
+// counting sort, after the exclusive scan of d.counts into d.starts: slot i
+// goes to perm[starts[keys[i]] + li[i]] (kScatter); then every order index
+// puts its slots back in ascending (stable) order, writes its cell range and
+// zeroes its count for the next search (kOrderFix; cells above 256 particles
+// raise kFlagBigCell instead and the search is redone by radix sort)
+hipError_t launchScatter(const DevView& d, const uint32_t* keys, uint32_t* perm, hipStream_t st);
+hipError_t launchOrderFix(const DevView& d, uint32_t* perm, uint32_t ordTotal, hipStream_t st);
+hipError_t scanCounts(void* temp, size_t& tempBytes, const uint32_t* counts, uint32_t* starts, uint32_t n,
                       hipStream_t st);
 hipError_t launchFillRanges(uint2* r, size_t n, hipStream_t st);
+// ranges: record the cell ranges (radix path; the counting sort's kOrderFix
+// writes them itself)
 hipError_t launchReorder(const DevView& d, const uint32_t* keysSorted, const uint32_t* valsSorted, float4* xAlt,
-                         float4* vAlt, float4* xvAlt, float* kwsAlt, float* kvwsAlt, uint32_t* idAlt, hipStream_t st);
+                         float4* vAlt, float4* xvAlt, float* kwsAlt, float* kvwsAlt, uint32_t* idAlt, int ranges,
+                         hipStream_t st);
 hipError_t launchBuildNeighbors(const DevView& d, hipStream_t st);
 // cube boundary objects of every scene (no-op for scenes without)
 hipError_t launchCubeBoundary(const DevView& d, hipStream_t st);
