@@ -2130,6 +2130,7 @@ class Engine {
         SceneH& s = scene(si);
         if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
         neighborSearch(false);
+        collectProfile();   // the search's kernels count in the profile like a step's
         auto ids = download(id.p, NP);
         // active particles only; inactive ones get empty lists
         const uint32_t na = s.na;

@@ -85,6 +85,42 @@ def test_gpu_support_test_matches_reference_pairs(tmp_path):
     assert np.array_equal(member, g["member"])
 
 
+def test_gpu_crowded_cell_falls_back_to_radix_sort_and_stays_exact(tmp_path):
+    # 400 particles inside one cell (h = 0.04) plus a lattice block: the counting
+    # sort orders at most 256 per cell in place, so that search is redone with the
+    # radix sort. The sets must be exact there, and again after the crowd is
+    # spread out (the counting sort takes over again) and gathered once more.
+    r = 0.01
+    rng = np.random.default_rng(5)
+    block = SC.lattice((0.05, 0.05, 0.05), (6, 6, 6), 2 * r, r)
+    crowd = (0.281 + 0.038 * rng.random((400, 3))).astype(np.float32)   # cell [0.28, 0.32)^3
+    spread = (0.250 + 0.25 * rng.random((400, 3))).astype(np.float32)
+    sc = SC.small(0)
+    p, _, xb = _params(sc, str(tmp_path))
+    ctx = Context(0)
+    sid = ctx.add_scene(p, np.concatenate([block, crowd]), None, xb)
+    ctx.init()
+    for part, radix in ((crowd, True), (spread, False), (crowd[::-1].copy(), True)):
+        x = np.concatenate([block, part])
+        ctx.set_fluid_state(sid, x, np.zeros_like(x))
+        o = S.Oracle(S.params_from_scene(sc), x, None, xb, bits=32)
+        ctx.reset_kernel_stats()
+        ctx.set_profiling(True)
+        off, ids, boff, bids = ctx.neighbors(sid)
+        # the radix path fills the range table first; the counting sort never does
+        fills = ctx.kernel_stats().get("fill_ranges", (0.0, 0))[1]
+        ctx.set_profiling(False)
+        assert (fills > 0) == radix
+        ooff, oids, oboff, obids = o.neighbors()
+        assert np.array_equal(off, ooff) and np.array_equal(boff, oboff)
+        for a_, b_ in zip(_sorted_rows(off, ids), _sorted_rows(ooff, oids)):
+            assert np.array_equal(a_, b_)
+        for a_, b_ in zip(_sorted_rows(boff, bids), _sorted_rows(oboff, obids)):
+            assert np.array_equal(a_, b_)
+        assert np.array_equal(ctx.cells(sid), o.cell_keys())
+        o.close()
+
+
 def test_gpu_neighbor_sets_match_oracle_dense_and_far(tmp_path):
     # a 1.5x-dense jittered cluster (lists grow past their first capacity) plus a
     # particle far outside the box moving away (the grid must grow); sets stay exact
