@@ -423,6 +423,12 @@ __global__ void __launch_bounds__(kBlock) kReorder(DevView d, const uint32_t* ke
     d.cellOf[i] = first;
 }
 
+// Candidates each lane of the search loads per round (1 = one at a time).
+#ifndef SPH_BUILD_UNROLL
+#define SPH_BUILD_UNROLL 4
+#endif
+constexpr uint32_t kBuildUnroll = SPH_BUILD_UNROLL;
+
 // Within-support neighbour lists from the 27-cell candidates (HashTable.cpp:23-86
 // walk order: dx, dy, dz with dz fastest), filtered by |r| <= h with the
 // reference's rounding (DFSPHSolver.cpp:575). Self is included. L lanes share
@@ -463,49 +469,69 @@ __device__ __forceinline__ void buildNeighbors(const DevView& d) {
         int cx, cy, cz;
         globalCell(S, xi, cx, cy, cz);
         const size_t NP = d.NP;
-        for (int dx = -1; dx <= 1; ++dx)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dz = -1; dz <= 1; ++dz) {
-                    const int ax = cx + dx, ay = cy + dy, az = cz + dz;
-                    bool bucket;
-                    const uint32_t c = cellSlot(S, ax, ay, az, bucket);
-                    const uint2 fr = d.fRange[c];
-                    for (uint32_t base = fr.x; base < fr.y; base += L) {
-                        const uint32_t j = base + sub;
-                        bool hit = false;
-                        if (j < fr.y) {
-                            const float4 xj = d.x[j];
-                            hit = (!bucket || inCell(S, xj, ax, ay, az)) &&
-                                  dRef(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) <= S.dmax;
-                        }
-                        uint32_t before, total;
-                        place(hit, before, total);
-                        if (hit) {
-                            const uint32_t pos = nf + before;
+        // walk cell q = 9 (dx + 1) + 3 (dy + 1) + (dz + 1); the ranges of cell
+        // q + 1 are loaded before cell q is scanned, and each lane loads U
+        // candidates per round before testing them in order, so a list costs
+        // a few rounds of load latency instead of one per candidate
+        constexpr uint32_t U = kBuildUnroll;
+        auto cellAt = [&](int q, int& ax, int& ay, int& az, bool& bucket) {
+            ax = cx + q / 9 - 1;
+            ay = cy + (q / 3) % 3 - 1;
+            az = cz + q % 3 - 1;
+            return cellSlot(S, ax, ay, az, bucket);
+        };
+        int axN, ayN, azN;
+        bool bucketN;
+        uint32_t cN = cellAt(0, axN, ayN, azN, bucketN);
+        uint2 frN = d.fRange[cN];
+        uint2 brN = S.akinci && !bucketN ? d.bRange[cN] : make_uint2(0u, 0u);
+        for (int q = 0; q < 27; ++q) {
+            const int ax = axN, ay = ayN, az = azN;
+            const bool bucket = bucketN;
+            const uint2 fr = frN, br = brN;
+            if (q + 1 < 27) {
+                cN = cellAt(q + 1, axN, ayN, azN, bucketN);
+                frN = d.fRange[cN];
+                // boundary particles all lie in the dense box
+                brN = S.akinci && !bucketN ? d.bRange[cN] : make_uint2(0u, 0u);
+            }
+            for (uint32_t base = fr.x; base < fr.y; base += U * L) {
+                float4 xj[U];
+#pragma unroll
+                for (uint32_t k = 0; k < U; ++k) xj[k] = d.x[min(base + k * L + sub, fr.y - 1u)];
+#pragma unroll
+                for (uint32_t k = 0; k < U; ++k) {
+                    const uint32_t j = base + k * L + sub;
+                    const bool hit = j < fr.y && (!bucket || inCell(S, xj[k], ax, ay, az)) &&
+                                     dRef(xi.x - xj[k].x, xi.y - xj[k].y, xi.z - xj[k].z) <= S.dmax;
+                    uint32_t before, total;
+                    place(hit, before, total);
+                    if (hit) {
+                        const uint32_t pos = nf + before;
 #if SPH_LIST_STAGE
-                            if (pos < SPH_LIST_STAGE) sList[pos * PB + tp] = j;
-                            else
+                        if (pos < SPH_LIST_STAGE) sList[pos * PB + tp] = j;
+                        else
 #endif
-                            if (pos < d.capF) d.nbrF[pos * NP + i] = j;
-                        }
-                        nf += total;
+                        if (pos < d.capF) d.nbrF[pos * NP + i] = j;
                     }
-                    if (S.akinci && !bucket) {   // boundary particles all lie in the dense box
-                        const uint2 br = d.bRange[c];
-                        for (uint32_t base = br.x; base < br.y; base += L) {
-                            const uint32_t b = base + sub;
-                            bool hit = false;
-                            if (b < br.y) {
-                                const float4 xb = d.xb[b];
-                                hit = dRef(xi.x - xb.x, xi.y - xb.y, xi.z - xb.z) <= S.dmax;
-                            }
-                            uint32_t before, total;
-                            place(hit, before, total);
-                            if (hit && nb + before < d.capB) d.nbrB[(nb + before) * NP + i] = b;
-                            nb += total;
-                        }
-                    }
+                    nf += total;
                 }
+            }
+            for (uint32_t base = br.x; base < br.y; base += U * L) {
+                float4 xb[U];
+#pragma unroll
+                for (uint32_t k = 0; k < U; ++k) xb[k] = d.xb[min(base + k * L + sub, br.y - 1u)];
+#pragma unroll
+                for (uint32_t k = 0; k < U; ++k) {
+                    const uint32_t b = base + k * L + sub;
+                    const bool hit = b < br.y && dRef(xi.x - xb[k].x, xi.y - xb[k].y, xi.z - xb[k].z) <= S.dmax;
+                    uint32_t before, total;
+                    place(hit, before, total);
+                    if (hit && nb + before < d.capB) d.nbrB[(nb + before) * NP + i] = b;
+                    nb += total;
+                }
+            }
+        }
         if (sub == 0) {
             d.cntF[i] = nf;
             d.cntB[i] = nb;
