@@ -78,6 +78,12 @@ __device__ __forceinline__ float cohesionDeriv(const SceneConst& S, float d, flo
 #define SPH_ADJ_LPP 4
 #endif
 constexpr uint32_t kAdjLpp = SPH_ADJ_LPP;
+// List slots a lane of kAdjVelocity / kAdjSource loads per round (their
+// gathers then overlap instead of waiting one slot at a time).
+#ifndef SPH_ADJ_BATCH
+#define SPH_ADJ_BATCH 2
+#endif
+constexpr uint32_t kAdjBatch = SPH_ADJ_BATCH;
 static_assert(kAdjLpp == 1 || kAdjLpp == 2 || kAdjLpp == 4 || kAdjLpp == 8, "SPH_ADJ_LPP must be 1, 2, 4 or 8");
 #define ADJ_PROLOGUE_LANES(L)                                                                      \
     const uint32_t lb = logicalBlock();                                                            \
@@ -376,18 +382,38 @@ __global__ void __launch_bounds__(kBlock) kAdjVelocity(AdjView a, int mode) {
     F3 gx{0.0f, 0.0f, 0.0f};
     float kb = 0.0f;
     const uint32_t nf = valid ? min(a.cntF[i], a.capF) : 0u;
-    for (uint32_t s = sub; s < nf; s += kAdjLpp) {
-        const uint32_t j = a.nbrF[s * NP + i];
-        const float ks = ki + a.kk[j];
-        if (fabsf(ks) > S.eps) {
-            ADJ_GRAD_PAIR(xi, j)
-            const F3 du = ui - f3(a.vBar[j]);
-            const float rdu = dot3(r, du);
-            kb -= dt * S.m * G * rdu;
-            const float c = -dt * S.m * ks;
-            axpy(gx, c * G, du);
-            axpy(gx, c * G2 * rdu, r);
+    // kAdjBatch slots per round: list entries, then k_j, then the open pairs'
+    // x_j and v-bar_j, then the sums in slot order
+    for (uint32_t s0 = sub; s0 < nf; s0 += kAdjBatch * kAdjLpp) {
+        uint32_t j[kAdjBatch];
+        float ks[kAdjBatch];
+        float4 xj[kAdjBatch], vj[kAdjBatch];
+        bool open[kAdjBatch];
+#pragma unroll
+        for (uint32_t q = 0; q < kAdjBatch; ++q) j[q] = a.nbrF[static_cast<size_t>(min(s0 + q * kAdjLpp, nf - 1u)) * NP + i];
+#pragma unroll
+        for (uint32_t q = 0; q < kAdjBatch; ++q) ks[q] = ki + a.kk[j[q]];
+#pragma unroll
+        for (uint32_t q = 0; q < kAdjBatch; ++q) {
+            open[q] = s0 + q * kAdjLpp < nf && fabsf(ks[q]) > S.eps;
+            if (open[q]) {
+                xj[q] = a.x[j[q]];
+                vj[q] = a.vBar[j[q]];
+            }
         }
+#pragma unroll
+        for (uint32_t q = 0; q < kAdjBatch; ++q)
+            if (open[q]) {
+                const F3 r = xi - f3(xj[q]);
+                const float rl = fastSqrt(dot3(r, r));
+                const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
+                const F3 du = ui - f3(vj[q]);
+                const float rdu = dot3(r, du);
+                kb -= dt * S.m * G * rdu;
+                const float c = -dt * S.m * ks[q];
+                axpy(gx, c * G, du);
+                axpy(gx, c * G2 * rdu, r);
+            }
     }
     if (valid && S.akinci && fabsf(ki) > S.eps) {
         const uint32_t nb = min(a.cntB[i], a.capB);
@@ -440,16 +466,35 @@ __global__ void __launch_bounds__(kBlock) kAdjSource(AdjView a, int mode) {
     const float si = a.sBar[ii];
     F3 gx{0.0f, 0.0f, 0.0f}, gu{0.0f, 0.0f, 0.0f};
     const uint32_t nf = valid ? min(a.cntF[i], a.capF) : 0u;
-    for (uint32_t s = sub; s < nf; s += kAdjLpp) {
-        const uint32_t j = a.nbrF[s * NP + i];
-        const float ss = si + a.sBar[j];
-        if (ss == 0.0f) continue;
-        ADJ_GRAD_PAIR(xi, j)
-        const F3 du = ui - f3(a.u[j]);
-        const float w = c * S.m * ss;
-        axpy(gu, w * G, r);
-        axpy(gx, w * G, du);
-        axpy(gx, w * G2 * dot3(r, du), r);
+    for (uint32_t s0 = sub; s0 < nf; s0 += kAdjBatch * kAdjLpp) {   // as in kAdjVelocity
+        uint32_t j[kAdjBatch];
+        float ss[kAdjBatch];
+        float4 xj[kAdjBatch], uj[kAdjBatch];
+        bool open[kAdjBatch];
+#pragma unroll
+        for (uint32_t q = 0; q < kAdjBatch; ++q) j[q] = a.nbrF[static_cast<size_t>(min(s0 + q * kAdjLpp, nf - 1u)) * NP + i];
+#pragma unroll
+        for (uint32_t q = 0; q < kAdjBatch; ++q) ss[q] = si + a.sBar[j[q]];
+#pragma unroll
+        for (uint32_t q = 0; q < kAdjBatch; ++q) {
+            open[q] = s0 + q * kAdjLpp < nf && ss[q] != 0.0f;
+            if (open[q]) {
+                xj[q] = a.x[j[q]];
+                uj[q] = a.u[j[q]];
+            }
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < kAdjBatch; ++q)
+            if (open[q]) {
+                const F3 r = xi - f3(xj[q]);
+                const float rl = fastSqrt(dot3(r, r));
+                const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
+                const F3 du = ui - f3(uj[q]);
+                const float w = c * S.m * ss[q];
+                axpy(gu, w * G, r);
+                axpy(gx, w * G, du);
+                axpy(gx, w * G2 * dot3(r, du), r);
+            }
     }
     if (valid && S.akinci && si != 0.0f) {
         const uint32_t nb = min(a.cntB[i], a.capB);
