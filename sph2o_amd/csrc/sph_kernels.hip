@@ -481,20 +481,28 @@ __device__ __forceinline__ void buildNeighbors(const DevView& d) {
     // lanes p, p + P, ... of the wave hold one particle; `before` counts the
     // hits of its lanes below this one, `total` all of them
     auto place = [&](bool hit, uint32_t& before, uint32_t& total) {
-        const unsigned long long m = __ballot(hit);
-        before = 0;
-        total = 0;
+        if constexpr (L == 1) {
+            before = 0;
+            total = hit ? 1u : 0u;
+        } else {
+            const unsigned long long m = __ballot(hit);
+            before = 0;
+            total = 0;
 #pragma unroll
-        for (uint32_t k = 0; k < L; ++k) {
-            const uint32_t bit = static_cast<uint32_t>((m >> (p + k * P)) & 1ull);
-            before += k < sub ? bit : 0u;
-            total += bit;
+            for (uint32_t k = 0; k < L; ++k) {
+                const uint32_t bit = static_cast<uint32_t>((m >> (p + k * P)) & 1ull);
+                before += k < sub ? bit : 0u;
+                total += bit;
+            }
         }
     };
     if (valid) {
+        // the scene's constants as values: the list stores below could alias
+        // the scene table, which would make every candidate reload them
+        const SceneConst Sv = S;
         const float4 xi = d.x[i];
         int cx, cy, cz;
-        globalCell(S, xi, cx, cy, cz);
+        globalCell(Sv, xi, cx, cy, cz);
         const size_t NP = d.NP;
         // walk cell q = 9 (dx + 1) + 3 (dy + 1) + (dz + 1); the ranges of cell
         // q + 1 are loaded before cell q is scanned, and each lane loads U
@@ -505,13 +513,13 @@ __device__ __forceinline__ void buildNeighbors(const DevView& d) {
             ax = cx + q / 9 - 1;
             ay = cy + (q / 3) % 3 - 1;
             az = cz + q % 3 - 1;
-            return cellSlot(S, ax, ay, az, bucket);
+            return cellSlot(Sv, ax, ay, az, bucket);
         };
         int axN, ayN, azN;
         bool bucketN;
         uint32_t cN = cellAt(0, axN, ayN, azN, bucketN);
         uint2 frN = d.fRange[cN];
-        uint2 brN = S.akinci && !bucketN ? d.bRange[cN] : make_uint2(0u, 0u);
+        uint2 brN = Sv.akinci && !bucketN ? d.bRange[cN] : make_uint2(0u, 0u);
         for (int q = 0; q < 27; ++q) {
             const int ax = axN, ay = ayN, az = azN;
             const bool bucket = bucketN;
@@ -520,7 +528,7 @@ __device__ __forceinline__ void buildNeighbors(const DevView& d) {
                 cN = cellAt(q + 1, axN, ayN, azN, bucketN);
                 frN = d.fRange[cN];
                 // boundary particles all lie in the dense box
-                brN = S.akinci && !bucketN ? d.bRange[cN] : make_uint2(0u, 0u);
+                brN = Sv.akinci && !bucketN ? d.bRange[cN] : make_uint2(0u, 0u);
             }
             for (uint32_t base = fr.x; base < fr.y; base += U * L) {
                 float4 xj[U];
@@ -529,8 +537,8 @@ __device__ __forceinline__ void buildNeighbors(const DevView& d) {
 #pragma unroll
                 for (uint32_t k = 0; k < U; ++k) {
                     const uint32_t j = base + k * L + sub;
-                    const bool hit = j < fr.y && (!bucket || inCell(S, xj[k], ax, ay, az)) &&
-                                     dRef(xi.x - xj[k].x, xi.y - xj[k].y, xi.z - xj[k].z) <= S.dmax;
+                    const bool hit = j < fr.y && (!bucket || inCell(Sv, xj[k], ax, ay, az)) &&
+                                     dRef(xi.x - xj[k].x, xi.y - xj[k].y, xi.z - xj[k].z) <= Sv.dmax;
                     uint32_t before, total;
                     place(hit, before, total);
                     if (hit) {
@@ -551,7 +559,7 @@ __device__ __forceinline__ void buildNeighbors(const DevView& d) {
 #pragma unroll
                 for (uint32_t k = 0; k < U; ++k) {
                     const uint32_t b = base + k * L + sub;
-                    const bool hit = b < br.y && dRef(xi.x - xb[k].x, xi.y - xb[k].y, xi.z - xb[k].z) <= S.dmax;
+                    const bool hit = b < br.y && dRef(xi.x - xb[k].x, xi.y - xb[k].y, xi.z - xb[k].z) <= Sv.dmax;
                     uint32_t before, total;
                     place(hit, before, total);
                     if (hit && nb + before < d.capB) d.nbrB[(nb + before) * NP + i] = b;
