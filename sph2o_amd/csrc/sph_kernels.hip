@@ -32,7 +32,7 @@ namespace sph {
 #define SPH_NT_LISTS 0
 #endif
 #ifndef SPH_XK_VELOCITY
-#define SPH_XK_VELOCITY 0
+#define SPH_XK_VELOCITY 1
 #endif
 // Rows of the fluid neighbour list that kBuildNeighbors stages in LDS (4 B x
 // 256 lanes per row) and stores as whole 1 KB rows at the end; entries past
@@ -100,11 +100,11 @@ __device__ __forceinline__ uint32_t ldList(const uint32_t* p) {
 // List slots a lane of a neighbour pass loads per round. A slot's gather
 // depends on its list load, so one slot per round costs two memory latencies
 // per neighbour; a round of B slots loads B list entries, then B gathers, then
-// accumulates them in slot order. The additions keep their order for every B,
-// but the compiler may contract a different product of a sum such as
-// rx^2 + ry^2 into an FMA in each unrolled copy, so results of different B
-// agree to rounding, not bit for bit (one binary is deterministic). Measured
-// on the 1M-particle dam break: 2.28 ms/step at B = 1, 2.12 at 2, 2.04 at 4.
+// accumulates them in slot order. The additions keep their order for every B;
+// where the compiler chooses which product of a sum to fuse into an FMA
+// (passes without spelled-out FMAs), results of different B agree to
+// rounding, not bit for bit (one binary is deterministic). Measured on the
+// 1M-particle dam break: 2.28 ms/step at B = 1, 2.12 at 2, 2.04 at 4.
 #ifndef SPH_BATCH
 #define SPH_BATCH 4
 #endif
@@ -128,6 +128,14 @@ __device__ __forceinline__ void walkList(const uint32_t* list, size_t NP, uint32
     }
 }
 struct XV { float4 x, v; };
+// The Jacobi passes spell out their FMAs (r^2 as dRef, dot products and the
+// accumulations): left to the compiler, which product of a sum such as
+// a x + b y it fuses can change with the unrolling or the gather layout, and
+// the iteration counts follow the last bit.
+__device__ __forceinline__ float pairFma(float a, float b, float c) { return __fmaf_rn(a, b, c); }
+__device__ __forceinline__ float dot3f(float ax, float ay, float az, float bx, float by, float bz) {
+    return __fmaf_rn(az, bz, __fmaf_rn(ay, by, __fmul_rn(ax, bx)));
+}
 struct XRP { float4 x; float rho, p; };
 struct XVR { float4 x, v; float rho; };
 __device__ __forceinline__ int floorCell(float x, float invh) { return static_cast<int>(floorf(__fmul_rn(x, invh))); }
@@ -1027,14 +1035,14 @@ __device__ __forceinline__ float sourceShare(const DevView& d, const SceneConst&
                         [&](uint32_t, const XV& q) {
                             const float4 xj = q.x, vj = q.v;
                             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                            const float g = gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
-                            dc += S.m * (g * ((vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz));
+                            const float g = gradFactor(S, fastSqrt(dRef(rx, ry, rz)));
+                            dc = pairFma(S.m, g * dot3f(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, rx, ry, rz), dc);
                         });
     for (uint32_t s = sub; s < nb; s += L) {
         const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
         const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
-        const float g = gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
-        dc += xb.w * (g * (vi.x * rx + vi.y * ry + vi.z * rz));
+        const float g = gradFactor(S, fastSqrt(dRef(rx, ry, rz)));
+        dc = pairFma(xb.w, g * dot3f(vi.x, vi.y, vi.z, rx, ry, rz), dc);
     }
     return dc;
 }
@@ -1100,8 +1108,8 @@ __device__ __forceinline__ void velocityUpdate(const DevView& d, const SceneCons
                                       const float ks = ki + xj.w;
                                       if (fabsf(ks) > S.eps) {
                                           const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                                          const float c = (S.m * ks) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
-                                          sx += c * rx; sy += c * ry; sz += c * rz;
+                                          const float c = (S.m * ks) * gradFactor(S, fastSqrt(dRef(rx, ry, rz)));
+                                          sx = pairFma(c, rx, sx); sy = pairFma(c, ry, sy); sz = pairFma(c, rz, sz);
                                       }
                                   });
 #else
@@ -1122,8 +1130,8 @@ __device__ __forceinline__ void velocityUpdate(const DevView& d, const SceneCons
             for (uint32_t q = 0; q < kBatch; ++q)
                 if (s0 + q * kLppVel < nf && fabsf(ks[q]) > S.eps) {
                     const float rx = xi.x - xj[q].x, ry = xi.y - xj[q].y, rz = xi.z - xj[q].z;
-                    const float c = (S.m * ks[q]) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
-                    sx += c * rx; sy += c * ry; sz += c * rz;
+                    const float c = (S.m * ks[q]) * gradFactor(S, fastSqrt(dRef(rx, ry, rz)));
+                    sx = pairFma(c, rx, sx); sy = pairFma(c, ry, sy); sz = pairFma(c, rz, sz);
                 }
         }
 #endif
@@ -1132,8 +1140,8 @@ __device__ __forceinline__ void velocityUpdate(const DevView& d, const SceneCons
             for (uint32_t s = sub; s < nb; s += kLppVel) {
                 const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
                 const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
-                const float c = (xb.w * ki) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
-                sx += c * rx; sy += c * ry; sz += c * rz;
+                const float c = (xb.w * ki) * gradFactor(S, fastSqrt(dRef(rx, ry, rz)));
+                sx = pairFma(c, rx, sx); sy = pairFma(c, ry, sy); sz = pairFma(c, rz, sz);
             }
         }
     }

@@ -7,9 +7,16 @@ Tolerances (FP32 GPU vs FP64 oracle / FP32 reference), stated per check:
   WCSPH / PCISPH trajectories (20 steps): |dx| <= 1e-6 m, rms(dv)/rms(v) <= 2e-3,
       |drho|/rho0 <= 1e-5, PCISPH iterations per step equal
   DFSPH one step: iterations equal, rms(dv)/rms(v) <= 1e-3
-  DFSPH 40 steps: iteration totals within 5%, mean density error within 20%,
-      kinetic energy within 5%, centre of mass within 1e-4 m (per-particle
-      states decorrelate through the reference's |k| > 1e-5 gates)
+  DFSPH 40 steps: per-step iteration counts equal over the leading steps
+      whose loops the reference stopped clear of the thresholds (tests/
+      conftest.py decided_steps: 24 of 40 here); iteration totals within 5%,
+      mean density error within 20%, kinetic energy within 5%, centre of mass
+      within 5e-4 m (2.5% of the particle spacing). Per-particle states
+      decorrelate through the reference's |k| > 1e-5 gates, and past step 24
+      the loops stop within the spread of legitimate FP32 evaluations: builds
+      that differ only in which product of r^2 the compiler fuses into an FMA
+      land 2e-5 to 2.2e-4 m from the reference's centre of mass (the FP64
+      oracle 5.5e-5, the FP32 oracle 3.1e-5)
   batched scenes: bit-identical to the same scenes stepped alone
 """
 import json
@@ -18,7 +25,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import golden
+from conftest import decided_steps, golden
 from oracle import sphref as S
 from sph2o_amd import _lib, scenes as SC
 from sph2o_amd.simulation import Context
@@ -223,6 +230,9 @@ def test_gpu_dfsph_aggregates(tmp_path, tag):
     st = ctx.fluid_state(sid)
     ref = g["stats"]
     its = np.array([[s[sid].iterations, s[sid].iterations_v] for s in stats])
+    k = decided_steps(ref, sc)
+    assert k >= 10
+    assert np.array_equal(its[:k], ref[:k, 2:4].astype(int))
     assert abs(its[:, 0].sum() - ref[:, 2].sum()) <= 0.05 * ref[:, 2].sum() + 2
     assert abs(its[:, 1].sum() - ref[:, 3].sum()) <= 0.05 * ref[:, 3].sum() + 2
     assert abs(stats[-1][sid].t - ref[-1, 0]) / ref[-1, 0] < 0.05
@@ -230,7 +240,7 @@ def test_gpu_dfsph_aggregates(tmp_path, tag):
     e_me = np.mean(np.maximum(st["rho"] / 1000 - 1, 0))
     assert abs(e_me - e_ref) < 0.2 * e_ref + 1e-5
     assert abs((st["v"] ** 2).sum() - (g["v"] ** 2).sum()) / (g["v"] ** 2).sum() < 0.05
-    assert np.abs(st["x"].mean(0) - g["x"].mean(0)).max() < 1e-4
+    assert np.abs(st["x"].mean(0) - g["x"].mean(0)).max() < 5e-4
 
 
 def test_gpu_reference_example_scene_forces(tmp_path):
