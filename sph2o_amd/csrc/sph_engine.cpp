@@ -189,6 +189,8 @@ class Engine {
     DevBuf<uint32_t> cellOf;     // per slot: the cell range it opened in the last reorder, or ~0
     // counting sort of the keys (the default; radix sort when a cell overflows it)
     DevBuf<uint32_t> rankOf, ordToCell, counts, starts, li;
+    DevBuf<uint8_t> occ;         // per order index: held slots at the last counting fix-up
+    bool rangesFull = true;      // the next counting fix-up writes every range (occ is stale)
     DevBuf<unsigned char> scanTemp;
     size_t scanTempBytes = 0;
     uint32_t ordTotal = 0;
@@ -577,6 +579,8 @@ class Engine {
             HIPCHK(hipMemcpy(ordToCell.p, o2c.data(), o2c.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         counts.alloc(static_cast<size_t>(ordTotal) + 1);
         starts.alloc(static_cast<size_t>(ordTotal) + 1);
+        occ.alloc(static_cast<size_t>(ordTotal) + 1);
+        rangesFull = true;
         HIPCHK(hipMemset(counts.p, 0, (static_cast<size_t>(ordTotal) + 1) * sizeof(uint32_t)));
         scanTempBytes = 0;
         HIPCHK(scanCounts(nullptr, scanTempBytes, counts.p, starts.p, ordTotal + 1, stream));
@@ -680,6 +684,7 @@ class Engine {
         vb.alloc(xbs.size());
         fRange.alloc(totalCells ? totalCells : 1);
         rangesDirty = true;
+        rangesFull = true;
         // Akinci volumes (AkinciBoundaryObject::computeVolume), once per layout
         DevView d = view();
         for (size_t i = 0; i < scenes.size(); ++i)
@@ -935,16 +940,18 @@ class Engine {
             timed(K_SORT, [&] {
                 hipError_t e = scanCounts(scanTemp.p, scanTempBytes, counts.p, starts.p, ordTotal + 1, stream);
                 if (e == hipSuccess) e = launchScatter(d, keys.p, vals.p, stream);
-                if (e == hipSuccess) e = launchOrderFix(d, vals.p, ordTotal, stream);
+                if (e == hipSuccess) e = launchOrderFix(d, vals.p, ordTotal, occ.p, rangesFull ? 1 : 0, stream);
                 return e;
             });
             rangesDirty = true;   // cellOf does not list these ranges: a radix search fills first
+            rangesFull = false;
             timed(K_REORDER, [&] {
                 return launchReorder(d, keys.p, vals.p, xAlt.p, vAlt.p, xvAlt.p, kwsAlt.p, kvwsAlt.p, idAlt.p, 0,
                                      stream);
             });
         } else {
             radixOnce = false;
+            rangesFull = true;    // ranges written here are not in occ
             const bool fill = rangesDirty;
             timed(K_KEYS, [&] { return launchKeys(d, dfsphAdvance ? 1 : 0, fill ? 0 : 1, 0, keys.p, vals.p, stream); });
             if (fill) timed(K_FILL, [&] { return launchFillRanges(fRange.p, totalCells, stream); });
@@ -1525,6 +1532,7 @@ class Engine {
         if (layout) uploadLayout();
         else uploadConsts();
         rangesDirty = true;
+        rangesFull = true;
         HIPCHK(hipMemcpy(x.p, hx.data(), NP * sizeof(float4), hipMemcpyHostToDevice));
         uploadVel(0, hv);
         if (interleaved)  // positions are duplicated in the interleaved records

@@ -373,13 +373,20 @@ __global__ void __launch_bounds__(kBlock) kScatter(DevView d, const uint32_t* ke
 
 // Counting sort, step 3, one thread per order index: ascending slots within
 // the index (the order a stable radix sort of the same keys gives), the cell
-// range, and a zero count for the next search.
+// range, and a zero count for the next search. occ[o] records whether index
+// o held slots at this fix-up; unless `full` (the ranges were written by
+// anything else since), an index empty now and then has a zero count and a
+// cleared range already and is left alone: most of the dense box is empty.
 constexpr uint32_t kFixInRegs = 32, kFixMax = 256;
-__global__ void __launch_bounds__(kBlock) kOrderFix(DevView d, uint32_t* perm, uint32_t ordTotal) {
+__global__ void __launch_bounds__(kBlock) kOrderFix(DevView d, uint32_t* perm, uint32_t ordTotal, uint8_t* occ,
+                                                    int full) {
     const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
     if (o >= ordTotal) return;
     const uint32_t b = d.starts[o], e = d.starts[o + 1], n = e - b;
-    d.counts[o] = 0u;
+    const uint8_t was = occ[o];
+    if (!full && n == 0 && !was) return;
+    if (n) d.counts[o] = 0u;   // counts[o] == n (starts is its exclusive scan)
+    if (full || was != (n != 0)) occ[o] = n != 0 ? 1 : 0;
     const uint32_t cell = d.ordToCell[o];   // ~0: the scene's padding slots, already in order
     if (n > 1 && cell != ~0u) {
         if (n <= kFixInRegs) {
@@ -1470,10 +1477,11 @@ hipError_t launchKeys(const DevView& d, int adv, int clearCells, int counting, u
 hipError_t launchScatter(const DevView& d, const uint32_t* keys, uint32_t* perm, hipStream_t st) {
     LAUNCH(kScatter, d.numBlocks, d, keys, perm);
 }
-hipError_t launchOrderFix(const DevView& d, uint32_t* perm, uint32_t ordTotal, hipStream_t st) {
+hipError_t launchOrderFix(const DevView& d, uint32_t* perm, uint32_t ordTotal, uint8_t* occ, int full,
+                          hipStream_t st) {
     const uint32_t g = (ordTotal + kBlock - 1) / kBlock;
     if (g == 0) return hipSuccess;
-    LAUNCH(kOrderFix, g, d, perm, ordTotal);
+    LAUNCH(kOrderFix, g, d, perm, ordTotal, occ, full);
 }
 hipError_t launchFillRanges(uint2* r, size_t n, hipStream_t st) {
     const size_t g = (n + kBlock - 1) / kBlock;

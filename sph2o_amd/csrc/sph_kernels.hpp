@@ -74,9 +74,12 @@ hipError_t launchKeys(const DevView& d, int dfsphAdvance, int clearCells, int co
 // goes to perm[starts[keys[i]] + li[i]] (kScatter); then every order index
 // puts its slots back in ascending (stable) order, writes its cell range and
 // zeroes its count for the next search (kOrderFix; cells above 256 particles
-// raise kFlagBigCell instead and the search is redone by radix sort)
+// raise kFlagBigCell instead and the search is redone by radix sort). occ
+// (ordTotal bytes) tracks which indices held slots at the last fix-up; full:
+// write every index (the ranges were written by anything else since).
 hipError_t launchScatter(const DevView& d, const uint32_t* keys, uint32_t* perm, hipStream_t st);
-hipError_t launchOrderFix(const DevView& d, uint32_t* perm, uint32_t ordTotal, hipStream_t st);
+hipError_t launchOrderFix(const DevView& d, uint32_t* perm, uint32_t ordTotal, uint8_t* occ, int full,
+                          hipStream_t st);
 hipError_t scanCounts(void* temp, size_t& tempBytes, const uint32_t* counts, uint32_t* starts, uint32_t n,
                       hipStream_t st);
 hipError_t launchFillRanges(uint2* r, size_t n, hipStream_t st);
