@@ -110,19 +110,43 @@ constexpr uint32_t kBatch = SPH_BATCH;
 // Lane `sub` of L visits slots sub, sub + L, ... < n of particle i's list
 // (row-major, NP entries per row): gather(j) per slot, then consume(j, g) in
 // slot order. Slots past n load slot n - 1 (a valid entry) and are not consumed.
-template <uint32_t L, uint32_t B, class G, class C>
+// Pre: load the next round's list entries while this round's gathers are in
+// flight (B more registers; pays where a lane has several rounds).
+template <uint32_t L, uint32_t B, bool Pre = false, class G, class C>
 __device__ __forceinline__ void walkList(const uint32_t* list, size_t NP, uint32_t i, uint32_t sub, uint32_t n,
                                          G&& gather, C&& consume) {
-    for (uint32_t s0 = sub; s0 < n; s0 += B * L) {
-        uint32_t j[B];
+    if constexpr (!Pre) {
+        for (uint32_t s0 = sub; s0 < n; s0 += B * L) {
+            uint32_t j[B];
 #pragma unroll
-        for (uint32_t k = 0; k < B; ++k) j[k] = ldList(list + static_cast<size_t>(min(s0 + k * L, n - 1u)) * NP + i);
+            for (uint32_t k = 0; k < B; ++k) j[k] = ldList(list + static_cast<size_t>(min(s0 + k * L, n - 1u)) * NP + i);
+            decltype(gather(0u)) g[B];
+#pragma unroll
+            for (uint32_t k = 0; k < B; ++k) g[k] = gather(j[k]);
+#pragma unroll
+            for (uint32_t k = 0; k < B; ++k)
+                if (s0 + k * L < n) consume(j[k], g[k]);
+        }
+        return;
+    }
+    if (sub >= n) return;
+    uint32_t j[B];
+#pragma unroll
+    for (uint32_t k = 0; k < B; ++k) j[k] = ldList(list + static_cast<size_t>(min(sub + k * L, n - 1u)) * NP + i);
+    for (uint32_t s0 = sub; s0 < n; s0 += B * L) {
         decltype(gather(0u)) g[B];
 #pragma unroll
         for (uint32_t k = 0; k < B; ++k) g[k] = gather(j[k]);
+        const uint32_t s1 = s0 + B * L;
+        uint32_t jn[B];
+#pragma unroll
+        for (uint32_t k = 0; k < B; ++k)
+            jn[k] = s1 < n ? ldList(list + static_cast<size_t>(min(s1 + k * L, n - 1u)) * NP + i) : 0u;
 #pragma unroll
         for (uint32_t k = 0; k < B; ++k)
             if (s0 + k * L < n) consume(j[k], g[k]);
+#pragma unroll
+        for (uint32_t k = 0; k < B; ++k) j[k] = jn[k];
     }
 }
 struct XV { float4 x, v; };
@@ -671,7 +695,7 @@ __global__ void __launch_bounds__(kBlock) kDensityAlpha(DevView d, int withAlpha
     if (valid) {
         const float4 xi = d.x[i];
         const uint32_t nf = min(d.cntF[i], d.capF);
-        walkList<kLppOnce, kBatch>(d.nbrF, NP, i, sub, nf, [&](uint32_t j) { return d.x[j]; },
+        walkList<kLppOnce, kBatch, true>(d.nbrF, NP, i, sub, nf, [&](uint32_t j) { return d.x[j]; },
                                    [&](uint32_t, const float4& xj) {
                                        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
                                        const float rl = fastSqrt(rx * rx + ry * ry + rz * rz);
@@ -778,7 +802,7 @@ __global__ void __launch_bounds__(kBlock) kForces(DevView d, int pressure) {
         auto gatherXVR = [&](uint32_t j) {
             return XVR{POSV(d, j), vm >= 0 ? VEL(d, j) : make_float4(0.0f, 0.0f, 0.0f, 0.0f), d.rho[j]};
         };
-        walkList<kLppOnce, kBatch>(d.nbrF, NP, i, sub, nf, gatherXVR, [&](uint32_t, const XVR& q) {
+        walkList<kLppOnce, kBatch, true>(d.nbrF, NP, i, sub, nf, gatherXVR, [&](uint32_t, const XVR& q) {
             const float4 xj = q.x;
             const float rhoj = q.rho;
             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -1108,7 +1132,7 @@ __device__ __forceinline__ void velocityUpdate(const DevView& d, const SceneCons
         const float ki = xi.w;
         const uint32_t nf = min(d.cntF[i], d.capF);
 #if SPH_XK_VELOCITY
-        walkList<kLppVel, kBatch>(d.nbrF, NP, i, sub, nf, [&](uint32_t j) { return d.xk[j]; },
+        walkList<kLppVel, kBatch, true>(d.nbrF, NP, i, sub, nf, [&](uint32_t j) { return d.xk[j]; },
                                   [&](uint32_t, const float4& xj) {
                                       const float ks = ki + xj.w;
                                       if (fabsf(ks) > S.eps) {
