@@ -107,6 +107,17 @@ __device__ __forceinline__ uint32_t ldList(const uint32_t* p) {
 #define SPH_BATCH 4
 #endif
 constexpr uint32_t kBatch = SPH_BATCH;
+// The source passes' (kRhoStar, kDivergence) slots per round and prefetch:
+// over the 1M-particle run's timed steps, kRhoStar 86.7 us per launch at 4
+// slots without prefetch, 85.2 at 2 with (bit-identical sums: the passes
+// spell out their FMAs).
+#ifndef SPH_BATCH_SRC
+#define SPH_BATCH_SRC 2
+#endif
+#ifndef SPH_PRE_SRC
+#define SPH_PRE_SRC 1
+#endif
+constexpr uint32_t kBatchSrc = SPH_BATCH_SRC;
 // Lane `sub` of L visits slots sub, sub + L, ... < n of particle i's list
 // (row-major, NP entries per row): gather(j) per slot, then consume(j, g) in
 // slot order. Slots past n load slot n - 1 (a valid entry) and are not consumed.
@@ -1060,7 +1071,7 @@ __device__ __forceinline__ float sourceShare(const DevView& d, const SceneConst&
                                              float4 xi, float4 vi, uint32_t nf, uint32_t nb) {
     const size_t NP = d.NP;
     float dc = 0.0f;
-    walkList<L, kBatch>(d.nbrF, NP, i, sub, nf, [&](uint32_t j) { return XV{POSV(d, j), VEL(d, j)}; },
+    walkList<L, kBatchSrc, SPH_PRE_SRC != 0>(d.nbrF, NP, i, sub, nf, [&](uint32_t j) { return XV{POSV(d, j), VEL(d, j)}; },
                         [&](uint32_t, const XV& q) {
                             const float4 xj = q.x, vj = q.v;
                             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
