@@ -170,7 +170,6 @@ class Engine {
     std::string lastError;
     std::vector<SceneH> scenes;
     bool initialized = false;
-    bool interleaved = false;  // velocities live in xv[2i+1]
     bool fuseFinish = true;    // Jacobi-loop reductions ride on the velocity launch
     int method = -1;
 
@@ -182,7 +181,7 @@ class Engine {
     DevBuf<SceneConst> dScenes;
     DevBuf<SceneState> dState;   // scene states, then the flags (one readback copy)
     DevBuf<int32_t> dBlockScene;
-    DevBuf<float4> x, xAlt, v, vAlt, a, n, xs, ap, xk, xb, xv, xvAlt;
+    DevBuf<float4> x, xAlt, v, vAlt, a, n, xs, ap, xk, xb;
     DevBuf<float> rho, alpha, p, k, kws, kwsAlt, kvws, kvwsAlt, aux, vb;
     DevBuf<uint32_t> id, idAlt, keys, keysAlt, vals, valsAlt;
     DevBuf<uint2> fRange, bRange;
@@ -411,9 +410,6 @@ class Engine {
             HIPCHK(hipMemcpy(id.p + slot, nid.data(), cnt * sizeof(uint32_t), hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(kws.p + slot, zero.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(kvws.p + slot, zero.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
-            if (interleaved)  // positions are duplicated in the interleaved records
-                HIPCHK(hipMemcpy2D(xv.p + 2 * static_cast<size_t>(slot), 2 * sizeof(float4), nx.data(), sizeof(float4),
-                                   sizeof(float4), cnt, hipMemcpyHostToDevice));
             s.na += cnt;
             s.fEnd = s.fBegin + s.na;
             any = true;
@@ -704,7 +700,7 @@ class Engine {
         d.numScenes = static_cast<int32_t>(scenes.size());
         d.numBlocks = numBlocks;
         d.NP = static_cast<uint32_t>(NP);
-        d.x = x.p; d.v = v.p; d.a = a.p; d.n = n.p; d.xs = xs.p; d.ap = ap.p; d.xk = xk.p; d.xv = xv.p;
+        d.x = x.p; d.v = v.p; d.a = a.p; d.n = n.p; d.xs = xs.p; d.ap = ap.p; d.xk = xk.p;
         d.rho = rho.p; d.alpha = alpha.p; d.p = p.p; d.k = k.p; d.kws = kws.p; d.kvws = kvws.p; d.aux = aux.p;
         d.id = id.p;
         d.xb = xb.p; d.vb = vb.p;
@@ -771,12 +767,7 @@ class Engine {
         dBlockScene.alloc(bs.size());
         HIPCHK(hipMemcpy(dBlockScene.p, bs.data(), bs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         for (auto* b : {&x, &xAlt, &v, &vAlt, &a, &n, &xs, &ap, &xk}) b->alloc(np);
-        interleaved = sphKernelsInterleaveVelocity() != 0;
         fuseFinish = sphKernelsFuseFinish() != 0;
-        if (interleaved) {
-            xv.alloc(2 * np);
-            xvAlt.alloc(2 * np);
-        }
         for (auto* b : {&rho, &alpha, &p, &k, &kws, &kwsAlt, &kvws, &kvwsAlt, &aux}) b->alloc(np);
         for (auto* b : {&id, &idAlt, &keys, &keysAlt, &vals, &valsAlt, &cntF, &cntB, &cellOf, &li}) b->alloc(np);
         HIPCHK(hipMemset(cellOf.p, 0xFF, np * sizeof(uint32_t)));
@@ -825,20 +816,12 @@ class Engine {
         std::vector<float4> out(count);
         HIPCHK(hipStreamSynchronize(stream));
         if (!count) return out;
-        if (interleaved)
-            HIPCHK(hipMemcpy2D(out.data(), sizeof(float4), xv.p + 2 * static_cast<size_t>(first) + 1, 2 * sizeof(float4),
-                               sizeof(float4), count, hipMemcpyDeviceToHost));
-        else
-            HIPCHK(hipMemcpy(out.data(), v.p + first, count * sizeof(float4), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(out.data(), v.p + first, count * sizeof(float4), hipMemcpyDeviceToHost));
         return out;
     }
     void uploadVel(uint32_t first, const std::vector<float4>& vals) {
         if (vals.empty()) return;
-        if (interleaved)
-            HIPCHK(hipMemcpy2D(xv.p + 2 * static_cast<size_t>(first) + 1, 2 * sizeof(float4), vals.data(), sizeof(float4),
-                               sizeof(float4), vals.size(), hipMemcpyHostToDevice));
-        else
-            HIPCHK(hipMemcpy(v.p + first, vals.data(), vals.size() * sizeof(float4), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(v.p + first, vals.data(), vals.size() * sizeof(float4), hipMemcpyHostToDevice));
     }
 
     void uploadState() {
@@ -858,15 +841,9 @@ class Engine {
         HIPCHK(hipMemcpy(id.p, hid.data(), np * sizeof(uint32_t), hipMemcpyHostToDevice));
     }
 
-    // Copy the device state (and flags) to the pinned mirrors and wait.
-    // NP float4 velocities between buffers with element strides (in float4s);
-    // unit strides take a plain linear copy.
-    void copyVel(float4* dst, size_t dstStride, const float4* src, size_t srcStride) {
-        if (dstStride == 1 && srcStride == 1)
-            HIPCHK(hipMemcpyAsync(dst, src, NP * sizeof(float4), hipMemcpyDeviceToDevice, stream));
-        else
-            HIPCHK(hipMemcpy2DAsync(dst, dstStride * sizeof(float4), src, srcStride * sizeof(float4), sizeof(float4),
-                                    NP, hipMemcpyDeviceToDevice, stream));
+    // NP float4 velocities from one buffer to another, on the engine's stream.
+    void copyVel(float4* dst, const float4* src) {
+        HIPCHK(hipMemcpyAsync(dst, src, NP * sizeof(float4), hipMemcpyDeviceToDevice, stream));
     }
     // Copy the device state block to the pinned mirror and wait for it. The
     // wait polls an event: a blocking wait on a millisecond of queued work
@@ -946,8 +923,7 @@ class Engine {
             rangesDirty = true;   // cellOf does not list these ranges: a radix search fills first
             rangesFull = false;
             timed(K_REORDER, [&] {
-                return launchReorder(d, keys.p, vals.p, xAlt.p, vAlt.p, xvAlt.p, kwsAlt.p, kvwsAlt.p, idAlt.p, 0,
-                                     stream);
+                return launchReorder(d, keys.p, vals.p, xAlt.p, vAlt.p, kwsAlt.p, kvwsAlt.p, idAlt.p, 0, stream);
             });
         } else {
             radixOnce = false;
@@ -964,12 +940,11 @@ class Engine {
             const uint32_t* ks = swapped ? keysAlt.p : keys.p;
             const uint32_t* vs = swapped ? valsAlt.p : vals.p;
             timed(K_REORDER, [&] {
-                return launchReorder(d, ks, vs, xAlt.p, vAlt.p, xvAlt.p, kwsAlt.p, kvwsAlt.p, idAlt.p, 1, stream);
+                return launchReorder(d, ks, vs, xAlt.p, vAlt.p, kwsAlt.p, kvwsAlt.p, idAlt.p, 1, stream);
             });
         }
         std::swap(x.p, xAlt.p);
         std::swap(v.p, vAlt.p);
-        std::swap(xv.p, xvAlt.p);
         std::swap(kws.p, kwsAlt.p);
         std::swap(kvws.p, kvwsAlt.p);
         std::swap(id.p, idAlt.p);
@@ -1198,7 +1173,7 @@ class Engine {
             if (ok) break;
             if (attempt >= 5) throw SphError{SPH_E_OUT_OF_GRID, "could not place particles into a grid"};
             // the lists were invalid: undo the divergence passes and search again
-            copyVel(interleaved ? xv.p + 1 : v.p, interleaved ? 2 : 1, snapV.p, 1);
+            copyVel(v.p, snapV.p);
             HIPCHK(hipMemcpyAsync(kvws.p, snapKvws.p, NP * sizeof(float), hipMemcpyDeviceToDevice, stream));
             searchOnce(false);
         }
@@ -1535,9 +1510,6 @@ class Engine {
         rangesFull = true;
         HIPCHK(hipMemcpy(x.p, hx.data(), NP * sizeof(float4), hipMemcpyHostToDevice));
         uploadVel(0, hv);
-        if (interleaved)  // positions are duplicated in the interleaved records
-            HIPCHK(hipMemcpy2D(xv.p, 2 * sizeof(float4), hx.data(), sizeof(float4), sizeof(float4), NP,
-                               hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(id.p, hid.data(), NP * sizeof(uint32_t), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(kws.p, hkw.data(), NP * sizeof(float), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(kvws.p, hkv.data(), NP * sizeof(float), hipMemcpyHostToDevice));
@@ -1712,7 +1684,6 @@ class Engine {
     int checkDifferentiable() const {
         if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
         if (method == SPH_PCISPH) throw SphError{SPH_E_UNSUPPORTED, "gradients are available for WCSPH and DFSPH"};
-        if (interleaved) throw SphError{SPH_E_UNSUPPORTED, "gradients need separate velocity storage"};
         for (auto& sc : scenes) {
             if (!sc.emitters.empty()) throw SphError{SPH_E_UNSUPPORTED, "gradients of scenes with emitters"};
             if (sc.adhOn || !sc.cubes.empty())
@@ -2111,9 +2082,6 @@ class Engine {
         HIPCHK(hipMemcpy(id.p + s.fBegin, oid.data(), s.nf * sizeof(uint32_t), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(kws.p + s.fBegin, okw.data(), s.nf * sizeof(float), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(kvws.p + s.fBegin, okv.data(), s.nf * sizeof(float), hipMemcpyHostToDevice));
-        if (interleaved)  // positions are duplicated in the interleaved records
-            HIPCHK(hipMemcpy2D(xv.p + 2 * static_cast<size_t>(s.fBegin), 2 * sizeof(float4), hx.data(), sizeof(float4),
-                               sizeof(float4), s.nf, hipMemcpyHostToDevice));
         if (method == SPH_DFSPH) {  // density and alpha belong to the positions
             neighborSearch(false);
             HIPCHK(launchDensityAlpha(view(), 1, stream));

@@ -68,19 +68,10 @@ extern "C" __host__ int sphKernelsLanesPerParticle() { return static_cast<int>(k
 #ifndef SPH_LIST_STAGE
 #define SPH_LIST_STAGE 24
 #endif
-#ifndef SPH_XV
-#define SPH_XV 0
-#endif
-// Velocities: a separate array, or interleaved with positions (xv[2i] = x,
-// xv[2i+1] = v) so passes that gather both read one cache line per neighbour.
-#if SPH_XV
-#define VEL(dv, idx) (dv).xv[2 * static_cast<size_t>(idx) + 1]
-#define POSV(dv, idx) (dv).xv[2 * static_cast<size_t>(idx)]
-#else
+// Velocity and position of slot idx (separate arrays: interleaved (x, v)
+// records measured slower once the passes ran several lanes per particle).
 #define VEL(dv, idx) (dv).v[idx]
 #define POSV(dv, idx) (dv).x[idx]
-#endif
-extern "C" __host__ int sphKernelsInterleaveVelocity() { return SPH_XV; }
 // Whether the Jacobi-loop reduction rides on the velocity / PCISPH pressure
 // acceleration launch (1) or gets a kFinish launch of its own (0).
 #ifndef SPH_FUSE_FINISH
@@ -458,8 +449,8 @@ __global__ void __launch_bounds__(kBlock) kOrderFix(DevView d, uint32_t* perm, u
 
 // Gather the persistent state into sorted order and record cell ranges.
 __global__ void __launch_bounds__(kBlock) kReorder(DevView d, const uint32_t* keysSorted, const uint32_t* valsSorted,
-                                                   float4* xAlt, float4* vAlt, float4* xvAlt, float* kwsAlt,
-                                                   float* kvwsAlt, uint32_t* idAlt, int ranges) {
+                                                   float4* xAlt, float4* vAlt, float* kwsAlt, float* kvwsAlt,
+                                                   uint32_t* idAlt, int ranges) {
     const uint32_t lb = logicalBlock();
     const int sIdx = d.blockScene[lb];
     const SceneConst& S = d.scenes[sIdx];
@@ -467,12 +458,7 @@ __global__ void __launch_bounds__(kBlock) kReorder(DevView d, const uint32_t* ke
     const uint32_t o = valsSorted[i];
     const float4 p = d.x[o];
     xAlt[i] = p;
-#if SPH_XV
-    xvAlt[2 * static_cast<size_t>(i)] = p;
-    xvAlt[2 * static_cast<size_t>(i) + 1] = VEL(d, o);
-#else
     vAlt[i] = VEL(d, o);
-#endif
     const float kv = d.kvws[o];
     kwsAlt[i] = d.kws[o];
     kvwsAlt[i] = kv;
@@ -1324,9 +1310,6 @@ __global__ void __launch_bounds__(kBlock) kCubeBoundary(DevView d) {
         }
     }
     d.x[i] = p;
-#if SPH_XV
-    d.xv[2 * static_cast<size_t>(i)] = p;
-#endif
     VEL(d, i) = v;
 }
 
@@ -1523,9 +1506,9 @@ hipError_t launchFillRanges(uint2* r, size_t n, hipStream_t st) {
     if (g == 0) return hipSuccess;
     LAUNCH(kFillRanges, g, r, n);
 }
-hipError_t launchReorder(const DevView& d, const uint32_t* ks, const uint32_t* vs, float4* xa, float4* va, float4* xva,
-                         float* kwa, float* kva, uint32_t* ida, int ranges, hipStream_t st) {
-    LAUNCH(kReorder, d.numBlocks, d, ks, vs, xa, va, xva, kwa, kva, ida, ranges);
+hipError_t launchReorder(const DevView& d, const uint32_t* ks, const uint32_t* vs, float4* xa, float4* va, float* kwa,
+                         float* kva, uint32_t* ida, int ranges, hipStream_t st) {
+    LAUNCH(kReorder, d.numBlocks, d, ks, vs, xa, va, kwa, kva, ida, ranges);
 }
 hipError_t launchBuildNeighbors(const DevView& d, hipStream_t st) {
     LAUNCH(kBuildNeighbors, d.numBlocks * kLppBuild, d);

@@ -24,7 +24,6 @@ struct DevView {
     float4* xs;    // PCISPH predicted positions
     float4* ap;    // PCISPH pressure acceleration
     float4* xk;    // (x, k) of the current solver pass, read by kVelocity
-    float4* xv;    // interleaved (x, v) records when the kernels are built that way
     float* rho;
     float* alpha;
     float* p;
@@ -86,8 +85,7 @@ hipError_t launchFillRanges(uint2* r, size_t n, hipStream_t st);
 // ranges: record the cell ranges (radix path; the counting sort's kOrderFix
 // writes them itself)
 hipError_t launchReorder(const DevView& d, const uint32_t* keysSorted, const uint32_t* valsSorted, float4* xAlt,
-                         float4* vAlt, float4* xvAlt, float* kwsAlt, float* kvwsAlt, uint32_t* idAlt, int ranges,
-                         hipStream_t st);
+                         float4* vAlt, float* kwsAlt, float* kvwsAlt, uint32_t* idAlt, int ranges, hipStream_t st);
 hipError_t launchBuildNeighbors(const DevView& d, hipStream_t st);
 // cube boundary objects of every scene (no-op for scenes without)
 hipError_t launchCubeBoundary(const DevView& d, hipStream_t st);
@@ -123,8 +121,6 @@ hipError_t launchCheckFinite(const DevView& d, hipStream_t st);
 
 }  // namespace sph
 
-// 1 if the kernels keep velocities interleaved with positions (xv), else 0.
-extern "C" int sphKernelsInterleaveVelocity();
 extern "C" int sphKernelsFuseFinish();
 extern "C" int sphKernelsLanesPerParticle();
 
