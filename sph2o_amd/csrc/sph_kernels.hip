@@ -1342,28 +1342,60 @@ __global__ void __launch_bounds__(kBlock) kPciDensity(DevView d, int slot) {
         int cx, cy, cz;
         globalCell(S, d.x[i], cx, cy, cz);
         float rho = 0.0f;
-        for (int dx = -1; dx <= 1; ++dx)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dz = -1; dz <= 1; ++dz) {
-                    const int ax = cx + dx, ay = cy + dy, az = cz + dz;
-                    bool bucket;
-                    const uint32_t c = cellSlot(S, ax, ay, az, bucket);
-                    const uint2 fr = d.fRange[c];
-                    for (uint32_t j = fr.x; j < fr.y; ++j) {
-                        if (bucket && !inCell(S, d.x[j], ax, ay, az)) continue;   // cells of current positions
-                        const float4 xj = d.xs[j];
-                        const float rx = xsi.x - xj.x, ry = xsi.y - xj.y, rz = xsi.z - xj.z;
-                        rho += S.m * cubicW(S, fastSqrt(rx * rx + ry * ry + rz * rz));
-                    }
-                    if (S.akinci && !bucket) {
-                        const uint2 br = d.bRange[c];
-                        for (uint32_t b = br.x; b < br.y; ++b) {
-                            const float4 xb = d.xb[b];
-                            const float rx = xsi.x - xb.x, ry = xsi.y - xb.y, rz = xsi.z - xb.z;
-                            rho += xb.w * cubicW(S, fastSqrt(rx * rx + ry * ry + rz * rz));
-                        }
-                    }
+        // as in the search: the next cell's ranges load ahead, U candidates per
+        // round; r^2 and the sums are spelled out (dRef, pairFma) so that the
+        // unrolled copies round alike
+        constexpr uint32_t U = kBuildUnroll;
+        auto cellAt = [&](int q, int& ax, int& ay, int& az, bool& bucket) {
+            ax = cx + q / 9 - 1;
+            ay = cy + (q / 3) % 3 - 1;
+            az = cz + q % 3 - 1;
+            return cellSlot(S, ax, ay, az, bucket);
+        };
+        int axN, ayN, azN;
+        bool bucketN;
+        uint32_t cN = cellAt(0, axN, ayN, azN, bucketN);
+        uint2 frN = d.fRange[cN];
+        uint2 brN = S.akinci && !bucketN ? d.bRange[cN] : make_uint2(0u, 0u);
+        for (int q = 0; q < 27; ++q) {
+            const int ax = axN, ay = ayN, az = azN;
+            const bool bucket = bucketN;
+            const uint2 fr = frN, br = brN;
+            if (q + 1 < 27) {
+                cN = cellAt(q + 1, axN, ayN, azN, bucketN);
+                frN = d.fRange[cN];
+                brN = S.akinci && !bucketN ? d.bRange[cN] : make_uint2(0u, 0u);
+            }
+            if (!bucket) {
+                for (uint32_t base = fr.x; base < fr.y; base += U) {
+                    float4 xj[U];
+#pragma unroll
+                    for (uint32_t k = 0; k < U; ++k) xj[k] = d.xs[min(base + k, fr.y - 1u)];
+#pragma unroll
+                    for (uint32_t k = 0; k < U; ++k)
+                        if (base + k < fr.y)
+                            rho = pairFma(S.m, cubicW(S, fastSqrt(dRef(xsi.x - xj[k].x, xsi.y - xj[k].y, xsi.z - xj[k].z))),
+                                          rho);
                 }
+            } else {
+                // an overflow bucket holds several cells: keep the candidates
+                // whose current position lies in this one, and gather only those
+                for (uint32_t j = fr.x; j < fr.y; ++j) {
+                    if (!inCell(S, d.x[j], ax, ay, az)) continue;
+                    const float4 xj = d.xs[j];
+                    rho = pairFma(S.m, cubicW(S, fastSqrt(dRef(xsi.x - xj.x, xsi.y - xj.y, xsi.z - xj.z))), rho);
+                }
+            }
+            for (uint32_t base = br.x; base < br.y; base += U) {
+                float4 xb[U];
+#pragma unroll
+                for (uint32_t k = 0; k < U; ++k) xb[k] = d.xb[min(base + k, br.y - 1u)];
+#pragma unroll
+                for (uint32_t k = 0; k < U; ++k)
+                    if (base + k < br.y)
+                        rho = pairFma(xb[k].w, cubicW(S, fastSqrt(dRef(xsi.x - xb[k].x, xsi.y - xb[k].y, xsi.z - xb[k].z))), rho);
+            }
+        }
         d.rho[i] = rho;
         const float e = fmaxf(0.0f, rho - S.rho0);
         d.p[i] += S.pciDelta * e;
