@@ -58,13 +58,16 @@ __device__ __forceinline__ float cubicW(const SceneConst& S, float rl) {
     return S.kW * (2.0f * (f * f * f));
 }
 // CubicSpline::gradW (CubicSpline.h:52-74) as gradW = r * factor
+// Both branches are evaluated and selected (the same operations as branching
+// on q, without divergent branches inside the neighbour loops).
 __device__ __forceinline__ float gradFactor(const SceneConst& S, float rl) {
     const float q = rl * S.invh;
-    if (!(rl > 1.0e-9f) || q > 1.0f) return 0.0f;
     const float inv = fastRcp(rl * S.h);
-    if (q <= 0.5f) return S.lW * q * (3.0f * q - 2.0f) * inv;
     const float f = 1.0f - q;
-    return S.lW * (-f * f) * inv;
+    const float near = S.lW * q * (3.0f * q - 2.0f);
+    const float far = S.lW * (-f * f);
+    const float g = (q <= 0.5f ? near : far) * inv;
+    return (rl > 1.0e-9f && !(q > 1.0f)) ? g : 0.0f;
 }
 // ViscoK::laplW (ViscoK.h:29-37)
 __device__ __forceinline__ float laplW(const SceneConst& S, float rl) {
