@@ -191,6 +191,15 @@ int sph_get_boundary(sph_ctx* ctx, int scene, float* pos, float* volume);
  * 3 k_ws, 4 kv_ws, 5 surface normals xyz (N*3) */
 int sph_get_field(sph_ctx* ctx, int scene, int field, float* out);
 int sph_get_time(const sph_ctx* ctx, int scene, double* t, double* dt);
+/* Viewer feed: the data the reference's viewer pulls each frame (System.cpp:
+ * 119-125) — the instanced vec4 positions (SimulationObject::getPositions,
+ * SimulationObject.h:16, w = 1) and the scalar field |v|
+ * (SPHSimulation::updateScalarField, SPHSimulation.cpp:638-657) — for the
+ * scene's sph_num_active particles in original order, written by a kernel
+ * into device buffers (pos4: 4 floats per particle, speed: 1; either may be
+ * NULL), ordered on the context's stream. A GL or Vulkan buffer mapped into
+ * HIP is such a buffer. Pointers the device cannot address give SPH_E_ARG. */
+int sph_viewer_feed(sph_ctx* ctx, int scene, float* pos4, float* speed);
 
 /* Neighbour search results at the current positions (original ids).
  * cells: N*3 int32 cell keys. neighbours: CSR offsets[N+1] into ids; each

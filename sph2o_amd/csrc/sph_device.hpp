@@ -13,7 +13,10 @@ namespace sph {
 __device__ __forceinline__ float dRef(float x, float y, float z) {
     return __fmaf_rn(x, x, __fmaf_rn(z, z, __fmul_rn(y, y)));
 }
-__device__ __forceinline__ float lenRef(float x, float y, float z) { return __fsqrt_rn(dRef(x, y, z)); }
+// sqrtf, not __fsqrt_rn: for gfx950 the compiler lowers sqrtf to the
+// v_sqrt_f32 estimate plus an FMA correction (correctly rounded, as the
+// reference's sqrtss), while __fsqrt_rn becomes the bare estimate (1 ulp).
+__device__ __forceinline__ float lenRef(float x, float y, float z) { return sqrtf(dRef(x, y, z)); }
 __device__ __forceinline__ float fastSqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
 __device__ __forceinline__ float fastRcp(float v) { return __builtin_amdgcn_rcpf(v); }
 

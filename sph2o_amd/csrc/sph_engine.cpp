@@ -2045,6 +2045,24 @@ class Engine {
         for (uint32_t i = 0; i < na; ++i) out[ids[i]] = hv[i];
     }
 
+    // Positions (x, y, z, 1) and |v| of the scene's active particles in
+    // original order into device buffers, on the engine's stream. Buffers the
+    // device cannot address (pageable host memory) are refused, not faulted on.
+    void viewerFeed(int si, float* pos4, float* speed) {
+        SceneH& s = scene(si);
+        if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
+        for (const void* ptr : {static_cast<const void*>(pos4), static_cast<const void*>(speed)}) {
+            if (!ptr) continue;
+            hipPointerAttribute_t at{};
+            if (hipPointerGetAttributes(&at, ptr) != hipSuccess || at.devicePointer == nullptr) {
+                (void)hipGetLastError();
+                throw SphError{SPH_E_ARG, "viewer feed buffers must be device memory"};
+            }
+        }
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(launchViewerFeed(view(), si, s.na, reinterpret_cast<float4*>(pos4), speed, stream));
+    }
+
     void setState(int si, int64_t nn, const float* pos, const float* vel) {
         SceneH& s = scene(si);
         if (!s.emitters.empty()) throw SphError{SPH_E_UNSUPPORTED, "setting the state of a scene with emitters"};
@@ -2373,6 +2391,13 @@ int sph_get_field(sph_ctx* ctx, int scene, int field, float* out) {
     return guarded(ctx, [&] {
         if (!out) throw SphError{SPH_E_ARG, "out is NULL"};
         ctx->eng.getField(scene, field, out);
+        return SPH_OK;
+    });
+}
+
+int sph_viewer_feed(sph_ctx* ctx, int scene, float* pos4, float* speed) {
+    return guarded(ctx, [&] {
+        ctx->eng.viewerFeed(scene, pos4, speed);
         return SPH_OK;
     });
 }

@@ -946,7 +946,7 @@ __global__ void __launch_bounds__(kBlock) kSurfaceTension(DevView d, int vmax) {
             const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
             const float dd = dRef(rx, ry, rz);
             if (dd > S.dhalf && dd <= S.dmax) {
-                const float rl = __fsqrt_rn(dd);
+                const float rl = sqrtf(dd);   // correctly rounded (see lenRef)
                 const float val = -4.0f * rl * rl / S.h + 6.0f * rl - 2.0f * S.h;
                 const float c = bc * d.vb[b] * powf(val, 0.25f);
                 ax -= c * rx / rl;
@@ -1582,6 +1582,32 @@ hipError_t launchFinish(const DevView& d, int kind, int slot, hipStream_t st) {
 hipError_t launchCellsOut(const DevView& d, int32_t* cells, hipStream_t st) { LAUNCH(kCellsOut, d.numBlocks, d, cells); }
 hipError_t launchCubeBoundary(const DevView& d, hipStream_t st) { LAUNCH(kCubeBoundary, d.numBlocks, d); }
 hipError_t launchCheckFinite(const DevView& d, hipStream_t st) { LAUNCH(kCheckFinite, d.numBlocks, d); }
+
+// The viewer's per-frame data (System.cpp:119-125): the vec4 positions it
+// instances (SimulationObject::getPositions, SimulationObject.h:16; w = 1 as
+// FluidObject::build sets it, FluidObject.cpp:55) and the scalar field |v|
+// (SPHSimulation::updateScalarField, SPHSimulation.cpp:638-657, glm::length in
+// the reference's FMA rounding), in original order, for the scene's active
+// particles, written straight into the caller's device buffers.
+__global__ void __launch_bounds__(kBlock) kViewerFeed(DevView d, int scene, float4* pos, float* speed) {
+    const SceneConst& S = d.scenes[scene];
+    const uint32_t i = S.fBegin + blockIdx.x * kBlock + threadIdx.x;
+    if (i >= S.fEnd) return;
+    const uint32_t o = d.id[i];
+    if (pos) {
+        const float4 p = d.x[i];
+        pos[o] = make_float4(p.x, p.y, p.z, 1.0f);
+    }
+    if (speed) {
+        const float4 v = VEL(d, i);
+        speed[o] = lenRef(v.x, v.y, v.z);
+    }
+}
+hipError_t launchViewerFeed(const DevView& d, int scene, uint32_t count, float4* pos, float* speed, hipStream_t st) {
+    const uint32_t g = (count + kBlock - 1) / kBlock;
+    if (g == 0) return hipSuccess;
+    LAUNCH(kViewerFeed, g, d, scene, pos, speed);
+}
 
 }  // namespace sph
 

@@ -144,6 +144,29 @@ class Context:
         self._check(self.L.sph_get_field(self.h, scene, fid, _fp(out)))
         return out
 
+    def viewer_feed_into(self, scene, pos4_ptr, speed_ptr):
+        """sph_viewer_feed into caller-owned device buffers (integer device
+        addresses, 0 to skip one): vec4 positions (w = 1) and |v| of the active
+        particles in original order, ordered on the context's stream."""
+        self._check(self.L.sph_viewer_feed(self.h, scene, ctypes.c_void_p(pos4_ptr or None),
+                                           ctypes.c_void_p(speed_ptr or None)))
+
+    def viewer_feed(self, scene=0):
+        """The viewer's per-frame data as host arrays: (n, 4) positions with
+        w = 1 and (n,) |v| (SPHSimulation::updateScalarField) of the active
+        particles in original order, written on the GPU into device buffers
+        and copied back."""
+        from ._hip import DeviceBuffer
+        n = self.num_active(scene)
+        pos4 = np.zeros((n, 4), np.float32)
+        speed = np.zeros(n, np.float32)
+        with DeviceBuffer(max(1, pos4.nbytes)) as dp, DeviceBuffer(max(1, speed.nbytes)) as ds:
+            self.viewer_feed_into(scene, dp.ptr, ds.ptr)
+            self.synchronize()
+            dp.copy_to(pos4)
+            ds.copy_to(speed)
+        return pos4, speed
+
     def time(self, scene=0):
         t = ctypes.c_double()
         dt = ctypes.c_double()
