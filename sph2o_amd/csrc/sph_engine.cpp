@@ -753,6 +753,9 @@ class Engine {
         }
         NP = off;
         NB = boff;
+        // the Jacobi passes address particle records with 32-bit byte offsets
+        if (NP >= (static_cast<size_t>(1) << 28))
+            throw SphError{SPH_E_UNSUPPORTED, "more than 2^28 particle slots in one context"};
         numBlocks = static_cast<uint32_t>(NP / kBlock);
         const size_t np = NP ? NP : 1;
         // flags follow the scene states in the same allocation

@@ -108,6 +108,12 @@ constexpr uint32_t kBatch = SPH_BATCH;
 #ifndef SPH_PRE_SRC
 #define SPH_PRE_SRC 1
 #endif
+// The Jacobi passes gather neighbour records through raw buffer loads (1:
+// one 32-bit offset per neighbour, the buffer's size bounding every load) or
+// global loads (0: a 64-bit address per record). Same sums bit for bit.
+#ifndef SPH_BUFFER_GATHER
+#define SPH_BUFFER_GATHER 1
+#endif
 constexpr uint32_t kBatchSrc = SPH_BATCH_SRC;
 // Lane `sub` of L visits slots sub, sub + L, ... < n of particle i's list
 // (row-major, NP entries per row): gather(j) per slot, then consume(j, g) in
@@ -1057,7 +1063,21 @@ __device__ __forceinline__ float sourceShare(const DevView& d, const SceneConst&
                                              float4 xi, float4 vi, uint32_t nf, uint32_t nb) {
     const size_t NP = d.NP;
     float dc = 0.0f;
-    walkList<L, kBatchSrc, SPH_PRE_SRC != 0>(d.nbrF, NP, i, sub, nf, [&](uint32_t j) { return XV{POSV(d, j), VEL(d, j)}; },
+#if SPH_BUFFER_GATHER
+    // raw buffer loads: one 32-bit offset per neighbour for both records
+    // instead of two 64-bit addresses (num_records bounds every load)
+    const uint32_t bytes = static_cast<uint32_t>(NP * sizeof(float4));
+    const __amdgpu_buffer_rsrc_t rX = __builtin_amdgcn_make_buffer_rsrc(d.x, 0, bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rV = __builtin_amdgcn_make_buffer_rsrc(d.v, 0, bytes, 0x00020000);
+    auto gather = [&](uint32_t j) {
+        const uint32_t off = j * static_cast<uint32_t>(sizeof(float4));
+        return XV{__builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rX, off, 0, 0)),
+                  __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rV, off, 0, 0))};
+    };
+#else
+    auto gather = [&](uint32_t j) { return XV{POSV(d, j), VEL(d, j)}; };
+#endif
+    walkList<L, kBatchSrc, SPH_PRE_SRC != 0>(d.nbrF, NP, i, sub, nf, gather,
                         [&](uint32_t, const XV& q) {
                             const float4 xj = q.x, vj = q.v;
                             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -1129,7 +1149,17 @@ __device__ __forceinline__ void velocityUpdate(const DevView& d, const SceneCons
         const float ki = xi.w;
         const uint32_t nf = min(d.cntF[i], d.capF);
 #if SPH_XK_VELOCITY
-        walkList<kLppVel, kBatch, true>(d.nbrF, NP, i, sub, nf, [&](uint32_t j) { return d.xk[j]; },
+#if SPH_BUFFER_GATHER
+        const __amdgpu_buffer_rsrc_t rXK =
+            __builtin_amdgcn_make_buffer_rsrc(d.xk, 0, static_cast<uint32_t>(NP * sizeof(float4)), 0x00020000);
+        auto gatherXK = [&](uint32_t j) {
+            return __builtin_bit_cast(
+                float4, __builtin_amdgcn_raw_buffer_load_b128(rXK, j * static_cast<uint32_t>(sizeof(float4)), 0, 0));
+        };
+#else
+        auto gatherXK = [&](uint32_t j) { return d.xk[j]; };
+#endif
+        walkList<kLppVel, kBatch, true>(d.nbrF, NP, i, sub, nf, gatherXK,
                                   [&](uint32_t, const float4& xj) {
                                       const float ks = ki + xj.w;
                                       if (fabsf(ks) > S.eps) {
