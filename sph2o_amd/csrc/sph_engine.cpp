@@ -879,6 +879,22 @@ class Engine {
         auto& pr = evPool[evUsed];
         evKernel[evUsed] = kid;
         ++evUsed;
+        // one-kernel launches of sph_kernels.hip take the events into the
+        // launch (kernel start / end); the sort (several kernels, rocPRIM's
+        // among them) and the adjoint passes are bracketed on the stream
+        const bool inLaunch = kid != K_SORT && !(kid >= K_ADJ_SEED && kid <= K_ADJ_SOURCE);
+        if (inLaunch) {
+            setLaunchEvents(pr.first, pr.second);
+            const hipError_t e = launch();
+            const bool used = launchEventsUsed();
+            setLaunchEvents(nullptr, nullptr);
+            HIPCHK(e);
+            if (!used) {   // nothing launched: an empty interval
+                HIPCHK(hipEventRecord(pr.first, stream));
+                HIPCHK(hipEventRecord(pr.second, stream));
+            }
+            return;
+        }
         HIPCHK(hipEventRecord(pr.first, stream));
         HIPCHK(launch());
         HIPCHK(hipEventRecord(pr.second, stream));

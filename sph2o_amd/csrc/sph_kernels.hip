@@ -19,6 +19,8 @@
 // block partials, combined per scene by a one-block-per-scene kFinish launch).
 #include <hip/hip_runtime.h>
 
+#include <hip/hip_ext.h>
+
 #include "sph_device.hpp"
 #include "sph_kernels.hpp"
 
@@ -1533,22 +1535,36 @@ __global__ void __launch_bounds__(kBlock) kFinish(DevView d, int kind, int slot)
 
 
 // ---------------------------------------------------------------- launchers
-#define LAUNCH(kernel, grid, ...)                                                     \
-    do {                                                                              \
-        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, st, __VA_ARGS__);     \
-        return hipGetLastError();                                                     \
+// Profiling events for the next launch (set by the engine around one launch):
+// hipExtLaunchKernelGGL has the runtime record them at the kernel's own
+// start and end, so a launch's time excludes the dispatch around it.
+static thread_local hipEvent_t tlEvStart = nullptr, tlEvStop = nullptr;
+static thread_local bool tlEvUsed = false;
+void setLaunchEvents(hipEvent_t start, hipEvent_t stop) {
+    tlEvStart = start;
+    tlEvStop = stop;
+    tlEvUsed = false;
+}
+bool launchEventsUsed() { return tlEvUsed; }
+#define LAUNCH_SHMEM(kernel, grid, shmem, ...)                                                               \
+    do {                                                                                                     \
+        if (tlEvStart) {                                                                                     \
+            hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), shmem, st, tlEvStart, tlEvStop, 0,        \
+                                  __VA_ARGS__);                                                              \
+            tlEvUsed = true;                                                                                 \
+        } else {                                                                                             \
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), shmem, st, __VA_ARGS__);                    \
+        }                                                                                                    \
+        return hipGetLastError();                                                                            \
     } while (0)
+#define LAUNCH(kernel, grid, ...) LAUNCH_SHMEM(kernel, grid, 0, __VA_ARGS__)
 // Jacobi-pass launches may reserve unused dynamic LDS to cap how many
 // workgroups share a CU (and its 32 KB vector L1); 0 leaves occupancy to the
 // register count.
 #ifndef SPH_JACOBI_LDS
 #define SPH_JACOBI_LDS 0
 #endif
-#define LAUNCH_JACOBI(kernel, grid, ...)                                                         \
-    do {                                                                                         \
-        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), SPH_JACOBI_LDS, st, __VA_ARGS__);   \
-        return hipGetLastError();                                                                \
-    } while (0)
+#define LAUNCH_JACOBI(kernel, grid, ...) LAUNCH_SHMEM(kernel, grid, SPH_JACOBI_LDS, __VA_ARGS__)
 
 hipError_t launchKeys(const DevView& d, int adv, int clearCells, int counting, uint32_t* keys, uint32_t* vals,
                       hipStream_t st) {

@@ -118,6 +118,11 @@ hipError_t launchFinish(const DevView& d, int kind, int slot, hipStream_t st);
 
 hipError_t launchCellsOut(const DevView& d, int32_t* cells, hipStream_t st);
 hipError_t launchCheckFinite(const DevView& d, hipStream_t st);
+// Profiling: the next launch made through this file's launchers (one kernel)
+// records start / stop at the kernel's own start and end; launchEventsUsed()
+// tells whether a launch took them. Per host thread; clear with nullptrs.
+void setLaunchEvents(hipEvent_t start, hipEvent_t stop);
+bool launchEventsUsed();
 // count = the scene's active particles; pos (count float4) and speed (count
 // floats) are device buffers indexed by original id, either may be null
 hipError_t launchViewerFeed(const DevView& d, int scene, uint32_t count, float4* pos, float* speed, hipStream_t st);
