@@ -209,6 +209,16 @@ int sph_viewer_feed(sph_ctx* ctx, int scene, float* pos4, float* speed);
 int sph_dump_cells(sph_ctx* ctx, int scene, int32_t* cells);
 int sph_dump_neighbors(sph_ctx* ctx, int scene, int64_t* offsets, int32_t* ids, int64_t cap, int64_t* boffsets,
                        int32_t* bids, int64_t bcap);
+/* sph_dump_cells without any allocation. No counterpart in the reference: it
+ * is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving or rendering loops, harnesses that forbid hipMalloc). A kernel
+ * writes the keys into a fixed device scratch array that the library holds
+ * on every GPU it runs on until the process ends (12 MiB), and one copy on the
+ * context's stream brings them to `cells`; no hipMalloc, hipFree or other
+ * allocating call is made. Scenes of more than 1048576 active particles give
+ * SPH_E_CAPACITY. Calls on one GPU share that scratch array, so calls that
+ * overlap on one GPU (from several host threads or contexts) race. */
+int sph_dump_cells_noalloc(sph_ctx* ctx, int scene, int32_t* cells);
 
 /* Wall time of the last sph_step call on the GPU timeline (HIP events on the
  * context's stream) and the number of kernel launches it made. */

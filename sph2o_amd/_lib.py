@@ -31,7 +31,7 @@ EXPORTS = [
     "sph_kernel_count", "sph_kernel_name", "sph_kernel_stats", "sph_reset_kernel_stats",
     "sph_set_coefficients", "sph_forward_record", "sph_adjoint_seed", "sph_adjoint_run", "sph_adjoint_result",
     "sph_recorded_field", "sph_add_emitter", "sph_num_active", "sph_save_state", "sph_load_state",
-    "sph_add_cube_boundary", "sph_viewer_feed",
+    "sph_add_cube_boundary", "sph_viewer_feed", "sph_dump_cells_noalloc",
 ]
 
 
@@ -121,6 +121,7 @@ def load():
         "sph_get_time": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)]),
         "sph_viewer_feed": (ctypes.c_int, [vp, ctypes.c_int, vp, vp]),
         "sph_dump_cells": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_int32)]),
+        "sph_dump_cells_noalloc": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_int32)]),
         "sph_dump_neighbors": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_int64), P(ctypes.c_int32), ctypes.c_int64,
                                               P(ctypes.c_int64), P(ctypes.c_int32), ctypes.c_int64]),
         "sph_last_step_timing": (ctypes.c_int, [vp, P(ctypes.c_double), P(ctypes.c_int64)]),

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <sstream>
@@ -32,7 +33,34 @@
 
 using namespace sph;
 
+// Device scratch of the allocation-free entry point sph_dump_cells_noalloc:
+// the cell keys of one scene's active particles in original order (3 int32
+// per particle), at most kNoAllocMaxParticles particles (12 MiB). Every GPU
+// the library runs on holds its own copy until the process ends. In host code
+// the name stands for a host-side shadow, not a device address: kernels and
+// copies get the address hipGetSymbolAddress returns for the current device.
+static constexpr size_t kNoAllocMaxParticles = static_cast<size_t>(1) << 20;
+static __device__ __align__(256) int32_t scratch[3 * kNoAllocMaxParticles];
+
 namespace {
+
+// Per-device address of `scratch`, looked up once per device (with that device
+// current) when the first context on it is initialised.
+std::mutex gScratchMutex;
+std::vector<int32_t*> gScratchAddr;
+hipError_t scratchAddress(int device, int32_t** out) {
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (device < 0) return hipErrorInvalidDevice;
+    if (gScratchAddr.size() <= static_cast<size_t>(device)) gScratchAddr.resize(static_cast<size_t>(device) + 1, nullptr);
+    if (!gScratchAddr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        gScratchAddr[device] = static_cast<int32_t*>(ptr);
+    }
+    *out = gScratchAddr[device];
+    return hipSuccess;
+}
 
 struct HipError {
     hipError_t e;
@@ -209,6 +237,7 @@ class Engine {
     size_t stateBlockBytes = 0;
     hipEvent_t evStart = nullptr, evStop = nullptr, evSync = nullptr;
     int64_t syncsInStep = 0;
+    int32_t* noAllocScratch = nullptr;   // this device's `scratch` (set at init)
 
     // profiling
     bool profiling = false;
@@ -722,6 +751,7 @@ class Engine {
             HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
             ownStream = true;
         }
+        HIPCHK(scratchAddress(device, &noAllocScratch));
         HIPCHK(hipEventCreate(&evStart));
         HIPCHK(hipEventCreate(&evStop));
         HIPCHK(hipEventCreateWithFlags(&evSync, hipEventDisableTiming));
@@ -2139,6 +2169,24 @@ class Engine {
             for (int a3 = 0; a3 < 3; ++a3) cells[3 * ids[i] + a3] = hc[3 * i + a3];
     }
 
+    // sph_dump_cells without any allocation: a kernel writes the keys in
+    // original order into this device's `scratch`, one copy brings them to
+    // the caller, the inactive tail is zeroed on the host.
+    void dumpCellsNoAlloc(int si, int32_t* cells) {
+        SceneH& s = scene(si);
+        if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
+        if (!cells) throw SphError{SPH_E_ARG, "null cells buffer"};
+        if (s.na > kNoAllocMaxParticles)
+            throw SphError{SPH_E_CAPACITY, "sph_dump_cells_noalloc handles at most 1048576 active particles per scene; "
+                                           "this scene has " + std::to_string(s.na)};
+        HIPCHK(hipSetDevice(device));
+        const size_t bytes = 3 * static_cast<size_t>(s.na) * sizeof(int32_t);
+        HIPCHK(launchCellsOutOriginal(view(), si, s.na, noAllocScratch, stream));
+        if (bytes) HIPCHK(hipMemcpyAsync(cells, noAllocScratch, bytes, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        std::fill(cells + 3 * static_cast<size_t>(s.na), cells + 3 * static_cast<size_t>(s.nf), 0);
+    }
+
     int dumpNeighbors(int si, int64_t* off, int32_t* ids_, int64_t cap, int64_t* boff, int32_t* bids, int64_t bcap) {
         SceneH& s = scene(si);
         if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
@@ -2432,6 +2480,13 @@ int sph_dump_cells(sph_ctx* ctx, int scene, int32_t* cells) {
     return guarded(ctx, [&] {
         if (!cells) throw SphError{SPH_E_ARG, "cells is NULL"};
         ctx->eng.dumpCells(scene, cells);
+        return SPH_OK;
+    });
+}
+
+int sph_dump_cells_noalloc(sph_ctx* ctx, int scene, int32_t* cells) {
+    return guarded(ctx, [&] {
+        ctx->eng.dumpCellsNoAlloc(scene, cells);
         return SPH_OK;
     });
 }

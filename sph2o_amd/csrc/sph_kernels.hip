@@ -1512,6 +1512,19 @@ __global__ void __launch_bounds__(kBlock) kCellsOut(DevView d, int32_t* cells) {
     cells[3 * i + 2] = floorCell(p.z, S.invh);
 }
 
+// The cell keys of one scene's active particles in original order:
+// cells[3 * id + a] for the slots [fBegin, fEnd) (ids are 0 .. active - 1).
+__global__ void __launch_bounds__(kBlock) kCellsOutOriginal(DevView d, int scene, int32_t* cells) {
+    const SceneConst& S = d.scenes[scene];
+    const uint32_t i = S.fBegin + blockIdx.x * kBlock + threadIdx.x;
+    if (i >= S.fEnd) return;
+    const float4 p = d.x[i];
+    const uint32_t o = d.id[i];
+    cells[3 * o] = floorCell(p.x, S.invh);
+    cells[3 * o + 1] = floorCell(p.y, S.invh);
+    cells[3 * o + 2] = floorCell(p.z, S.invh);
+}
+
 __global__ void __launch_bounds__(kBlock) kCheckFinite(DevView d) {
     const uint32_t lb = logicalBlock();
     const int sIdx = d.blockScene[lb];
@@ -1626,6 +1639,11 @@ hipError_t launchFinish(const DevView& d, int kind, int slot, hipStream_t st) {
     LAUNCH(kFinish, d.numScenes, d, kind, slot);
 }
 hipError_t launchCellsOut(const DevView& d, int32_t* cells, hipStream_t st) { LAUNCH(kCellsOut, d.numBlocks, d, cells); }
+hipError_t launchCellsOutOriginal(const DevView& d, int scene, uint32_t count, int32_t* cells, hipStream_t st) {
+    const uint32_t g = (count + kBlock - 1) / kBlock;
+    if (g == 0) return hipSuccess;
+    LAUNCH(kCellsOutOriginal, g, d, scene, cells);
+}
 hipError_t launchCubeBoundary(const DevView& d, hipStream_t st) { LAUNCH(kCubeBoundary, d.numBlocks, d); }
 hipError_t launchCheckFinite(const DevView& d, hipStream_t st) { LAUNCH(kCheckFinite, d.numBlocks, d); }
 

@@ -117,6 +117,9 @@ enum FinishKind { kFinishDensity = 0, kFinishDivergence = 1, kFinishTimeStep = 2
 hipError_t launchFinish(const DevView& d, int kind, int slot, hipStream_t st);
 
 hipError_t launchCellsOut(const DevView& d, int32_t* cells, hipStream_t st);
+// count = the scene's active particles; cells (3 * count int32, device) gets
+// each particle's cell key at 3 * (original id)
+hipError_t launchCellsOutOriginal(const DevView& d, int scene, uint32_t count, int32_t* cells, hipStream_t st);
 hipError_t launchCheckFinite(const DevView& d, hipStream_t st);
 // Profiling: the next launch made through this file's launchers (one kernel)
 // records start / stop at the kernel's own start and end; launchEventsUsed()

@@ -179,6 +179,13 @@ class Context:
         self._check(self.L.sph_dump_cells(self.h, scene, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
         return out
 
+    def cells_noalloc(self, scene=0):
+        """The same keys through sph_dump_cells_noalloc (no device allocation)."""
+        n = self.num_fluid(scene)
+        out = np.zeros((n, 3), np.int32)
+        self._check(self.L.sph_dump_cells_noalloc(self.h, scene, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return out
+
     def neighbors(self, scene=0):
         """Within-support sets (CSR, original ids, unsorted rows)."""
         n = self.num_fluid(scene)
