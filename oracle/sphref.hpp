@@ -103,9 +103,16 @@ struct Key {
     int x, y, z;
     bool operator==(const Key& o) const { return x == o.x && y == o.y && z == o.z; }
 };
-struct KeyHash {  // HashTable.h:13-16
+// HashTable.h:13-16 forms the same sum in int, which overflows (undefined
+// behaviour) for cells from about x = 20, y = 7, z = 8 on: inlined into two
+// loops the compiler may then hash one key two ways, and a lookup misses a
+// cell that holds particles. The sum is taken modulo 2^32 here; the hash only
+// places buckets, so results do not depend on it.
+struct KeyHash {
     size_t operator()(const Key& k) const {
-        return static_cast<size_t>(k.x * 73856093 + k.y * 19349663 + k.z * 83492791);
+        const uint32_t s = static_cast<uint32_t>(k.x) * 73856093u + static_cast<uint32_t>(k.y) * 19349663u +
+                           static_cast<uint32_t>(k.z) * 83492791u;
+        return static_cast<size_t>(s);
     }
 };
 

@@ -162,6 +162,27 @@ def test_neighbor_sets_brute_force():
         assert set(ids[off[i]:off[i + 1]].tolist()) == set(np.nonzero(d[i] <= 0.04)[0].tolist())
 
 
+@pytest.mark.parametrize("bits", [32, 64])
+def test_far_cells_keep_their_particles(bits):
+    # Cells whose hash sum overflows int32 (x >= 20, y >= 7, z >= 8 at h = 0.04):
+    # every particle still finds itself (rho >= m W(0)) and its brute-force
+    # neighbours, on a cloud spread over 4 m that a WCSPH step integrates.
+    rng = np.random.default_rng(8)
+    x = np.concatenate([rng.uniform(0.0, 4.0, (3000, 3)), rng.uniform(3.0, 3.2, (1500, 3))]).astype(np.float32)
+    p = S.params_from_scene(SC.small(0))
+    p.boundary_method = -1
+    o = S.Oracle(p, x, None, np.zeros((0, 3), np.float32), bits=bits)
+    off, ids, _, _ = o.neighbors()
+    d = np.sqrt(((x[:, None, :].astype(np.float64) - x[None, :, :]) ** 2).sum(-1))
+    for i in range(0, len(x), 7):
+        assert set(ids[off[i]:off[i + 1]].tolist()) == set(np.nonzero(d[i] <= 0.04)[0].tolist()), i
+    o.init()
+    o.step(1)
+    m = 1000.0 * 0.02 ** 3
+    assert o.state()["rho"].min() >= 0.999999 * m * 8 / (np.pi * 0.04 ** 3)
+    o.close()
+
+
 # ---------------------------------------------------------------- sampling
 def test_sampling_matches_reference_example_scene():
     g = golden("ref_example_scene.npz")
