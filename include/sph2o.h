@@ -75,7 +75,7 @@ typedef struct sph_scene_params {
     int32_t boundary_method;        /* SPH_BOUNDARY_AKINCI, SPH_BOUNDARY_CUBE or SPH_BOUNDARY_NONE */
     int32_t viscosity_method;       /* -1 or SPH_VISCOSITY_* */
     int32_t surface_tension_method; /* -1 or SPH_SURFACE_TENSION_AKINCI */
-    int32_t adhesion_method;        /* must be -1 */
+    int32_t adhesion_method;        /* -1 or 4 (Akinci adhesion, AdhesionForce.cpp:5-32; acts with Akinci boundaries) */
     uint32_t min_iterations, max_iterations;
     uint32_t min_iterations_v, max_iterations_v;
     int32_t adaptive_time_step;     /* DFSPH: 1 = CFL + iteration rule of the reference (default) */
@@ -266,7 +266,9 @@ int sph_adjoint_seed(sph_ctx* ctx, int scene, const float* dL_dx, const float* d
 
 /* Reverse sweep: recomputes each segment's tape from its checkpoint and runs
  * the adjoint of every step backwards. Afterwards the context holds the state
- * at the end of the rollout again. */
+ * at the end of the rollout again. Returns SPH_E_NONFINITE when a gradient of
+ * any scene is NaN or infinite (the sweep has completed and
+ * sph_adjoint_result still returns the values). */
 int sph_adjoint_run(sph_ctx* ctx);
 
 /* Results of the last sph_adjoint_run for one scene: dL/dx_0 and dL/dv_0

@@ -865,7 +865,8 @@ class Sim {
 };
 
 // ---- scene sampling (FluidObject.cpp:30-61, AkinciBoundaryObject.cpp:97-153) ----
-// Fluid block lattice: ppedge = floor((max-min)/(2r) + 1e-5), x = (i,j,k)*2r + (min + r), k fastest.
+// Fluid block lattice: ppedge = floor((max-min)/(2r) + 1e-5), x = (i,j,k)*2r + (min + r), k fastest,
+// rounded as the shipped -O3 build rounds it: x as a multiply then an add, y and z as one FMA.
 inline void sampleBlock(const float mn[3], const float mx[3], float radius, std::vector<float>& out) {
     const float dist = 2.0f * radius;
     unsigned pp[3];
@@ -876,7 +877,8 @@ inline void sampleBlock(const float mn[3], const float mx[3], float radius, std:
                 const unsigned idx[3] = {i, j, kk};
                 for (int a = 0; a < 3; ++a) {
                     const float base = mn[a] + radius;
-                    out.push_back(static_cast<float>(idx[a]) * dist + base);
+                    const float t = static_cast<float>(idx[a]);
+                    out.push_back(a == 0 ? t * dist + base : std::fma(t, dist, base));
                 }
             }
 }
@@ -893,7 +895,11 @@ inline void sampleBox(const float mn[3], const float mx[3], float radius, std::v
             for (unsigned kk = 0; kk <= nn[2]; kk++)
                 if (i == 0 || i == nn[0] || j == 0 || j == nn[1] || kk == 0 || kk == nn[2]) {
                     const unsigned idx[3] = {i, j, kk};
-                    for (int a = 0; a < 3; ++a) out.push_back(static_cast<float>(idx[a]) * d[a] + mn[a]);
+                    // rounded as the shipped -O3 build rounds it: z as one FMA
+                    for (int a = 0; a < 3; ++a) {
+                        const float t = static_cast<float>(idx[a]);
+                        out.push_back(a == 2 ? std::fma(t, d[a], mn[a]) : t * d[a] + mn[a]);
+                    }
                 }
 }
 

@@ -1,9 +1,11 @@
-"""Differentiable WCSPH rollouts as a ``torch.autograd.Function``.
+"""Differentiable WCSPH and fixed-iteration DFSPH rollouts as a
+``torch.autograd.Function``.
 
     x_T, v_T = rollout(context, scene, x0, v0, viscosity, surface_tension,
                        nsteps, checkpoint_every=20)
 
-The forward pass runs ``nsteps`` WCSPH steps of the HIP path from (x0, v0)
+The forward pass runs ``nsteps`` steps of the scene's solver (WCSPH, or
+DFSPH with a fixed time step and min = max iterations) on the HIP path from (x0, v0)
 through ``sph_forward_record`` (checkpoints every ``checkpoint_every`` steps);
 the backward pass runs the adjoint sweep of libsph2o.so (``sph_adjoint_run``)
 and returns dL/dx0, dL/dv0, dL/d(viscosity) and dL/d(surface tension). The
@@ -13,15 +15,16 @@ backward are the HIP kernels. One recorded rollout per context at a time: a
 second forward on the same context before the backward of the first makes
 that backward fail with ``SPH_E_STATE``.
 
-The reference (WCSPHSolver.cpp:14-78) has no gradients; the forward step is
-the same as ``Context.step`` on a WCSPH scene.
+The reference (WCSPHSolver.cpp:14-78, DFSPHSolver.cpp:33-107) has no
+gradients; the forward step is the same as ``Context.step``. A non-finite
+gradient raises ``SPHError`` (``SPH_E_NONFINITE``) in the backward pass.
 """
 import torch
 
-__all__ = ["WCSPHRollout", "rollout"]
+__all__ = ["SPHRollout", "WCSPHRollout", "rollout"]
 
 
-class WCSPHRollout(torch.autograd.Function):
+class SPHRollout(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x0, v0, viscosity, surface_tension, context, scene, nsteps, checkpoint_every):
         context.set_fluid_state(scene, x0.detach().cpu().numpy(), v0.detach().cpu().numpy())
@@ -49,10 +52,14 @@ class WCSPHRollout(torch.autograd.Function):
 
 
 def rollout(context, scene, x0, v0, viscosity, surface_tension, nsteps, checkpoint_every=20):
-    """x_T, v_T after `nsteps` WCSPH steps; differentiable in x0, v0 (N x 3
+    """x_T, v_T after `nsteps` steps; differentiable in x0, v0 (N x 3
     tensors) and the viscosity / surface-tension coefficients (scalars or
     0-d tensors)."""
     nu = viscosity if torch.is_tensor(viscosity) else torch.tensor(float(viscosity), dtype=torch.float64)
     gam = surface_tension if torch.is_tensor(surface_tension) else torch.tensor(float(surface_tension),
                                                                                dtype=torch.float64)
-    return WCSPHRollout.apply(x0, v0, nu, gam, context, scene, nsteps, checkpoint_every)
+    return SPHRollout.apply(x0, v0, nu, gam, context, scene, nsteps, checkpoint_every)
+
+
+# the name under which the WCSPH-only first version was published
+WCSPHRollout = SPHRollout

@@ -314,12 +314,15 @@ void sampleFluidBlock(const BlockDesc& b, float radius, std::vector<float>& xyz)
     for (int a = 0; a < 3; ++a) pp[a] = static_cast<unsigned>(std::floor((b.max[a] - b.min[a]) / dist + 1e-5f));
     float base[3];
     for (int a = 0; a < 3; ++a) base[a] = b.min[a] + radius;
+    // pos = (i, j, k) * dist + (min + r) as the shipped -O3 build rounds it
+    // (pinned against its 1M-particle lattice): x with two roundings, y and
+    // z contracted into one FMA each
     for (unsigned i = 0; i < pp[0]; ++i)
         for (unsigned j = 0; j < pp[1]; ++j)
             for (unsigned k = 0; k < pp[2]; ++k) {
                 xyz.push_back(static_cast<float>(i) * dist + base[0]);
-                xyz.push_back(static_cast<float>(j) * dist + base[1]);
-                xyz.push_back(static_cast<float>(k) * dist + base[2]);
+                xyz.push_back(std::fma(static_cast<float>(j), dist, base[1]));
+                xyz.push_back(std::fma(static_cast<float>(k), dist, base[2]));
             }
 }
 
@@ -334,9 +337,11 @@ void sampleBoxSurface(const BlockDesc& b, float radius, std::vector<float>& xyz)
         for (unsigned j = 0; j <= n[1]; j++)
             for (unsigned k = 0; k <= n[2]; k++)
                 if (i == 0 || i == n[0] || j == 0 || j == n[1] || k == 0 || k == n[2]) {
+                    // (i d.x, j d.y, k d.z) + min as the shipped -O3 build rounds
+                    // it: z contracted into one FMA, x and y two roundings
                     xyz.push_back(static_cast<float>(i) * d[0] + b.min[0]);
                     xyz.push_back(static_cast<float>(j) * d[1] + b.min[1]);
-                    xyz.push_back(static_cast<float>(k) * d[2] + b.min[2]);
+                    xyz.push_back(std::fma(static_cast<float>(k), d[2], b.min[2]));
                 }
 }
 

@@ -755,7 +755,20 @@ class Engine {
         HIPCHK(hipEventCreate(&evStart));
         HIPCHK(hipEventCreate(&evStop));
         HIPCHK(hipEventCreateWithFlags(&evSync, hipEventDisableTiming));
-        // slots
+        // slots; the Jacobi passes address particle records with 32-bit byte
+        // offsets, so the padded total must stay below 2^28 (checked before
+        // any 32-bit slot index is formed)
+        {
+            uint64_t total = 0, btotal = 0;
+            for (auto& s : scenes) {
+                total += (static_cast<uint64_t>(s.nf) + kBlock - 1) / kBlock * kBlock;
+                btotal += s.nb;
+            }
+            if (total >= (static_cast<uint64_t>(1) << 28))
+                throw SphError{SPH_E_UNSUPPORTED, "more than 2^28 particle slots in one context"};
+            if (btotal >= (static_cast<uint64_t>(1) << 31))
+                throw SphError{SPH_E_UNSUPPORTED, "more than 2^31 boundary particles in one context"};
+        }
         uint32_t off = 0, boff = 0;
         for (auto& s : scenes) {
             s.fBegin = off;
@@ -783,9 +796,6 @@ class Engine {
         }
         NP = off;
         NB = boff;
-        // the Jacobi passes address particle records with 32-bit byte offsets
-        if (NP >= (static_cast<size_t>(1) << 28))
-            throw SphError{SPH_E_UNSUPPORTED, "more than 2^28 particle slots in one context"};
         numBlocks = static_cast<uint32_t>(NP / kBlock);
         const size_t np = NP ? NP : 1;
         // flags follow the scene states in the same allocation
@@ -1959,6 +1969,18 @@ class Engine {
         // leave the context where the recorded rollout ended
         restoreCheckpoint(rec.last);
         gradValid = true;
+        // the forward refuses non-finite states (SPH_E_NONFINITE); so does the
+        // reverse sweep, after the results are stored for inspection
+        int bad = -1;
+        for (size_t si = 0; si < scenes.size() && bad < 0; ++si) {
+            bool ok = std::isfinite(gradCoef[2 * si]) && std::isfinite(gradCoef[2 * si + 1]);
+            for (float f : gradX0[si]) ok = ok && std::isfinite(f);
+            for (float f : gradV0[si]) ok = ok && std::isfinite(f);
+            if (!ok) bad = static_cast<int>(si);
+        }
+        if (bad >= 0)
+            throw SphError{SPH_E_NONFINITE, "non-finite gradient in scene " + std::to_string(bad) +
+                                                " (results kept for sph_adjoint_result)"};
     }
     // Values the recorded rollout held at one step (original order): see
     // sph_recorded_field in include/sph2o.h.

@@ -24,12 +24,19 @@ Fixtures:
                          the first emission and after a longer run, with the
                          active count after every step
 
+  ref_w8k.npz            the W8k dam break (8,000 particles, WCSPH): the state
+                         after 20 steps and box aggregates after 2,000
+  ref_d1m.npz            the D1M dam break (1M particles, DFSPH): per-step
+                         stats of 3 steps, 4,000 sampled particles after
+                         steps 1 and 3
+  ref_b64_smallest.npz   the smallest B64 scene alone: 1 step and 10 steps
+
   ref_opt_*.npz          Akinci adhesion and cube-boundary scenes
                          (sph2o_amd.scenes.boundary_option_scenes): the state
                          after one step and after a short run
 
-``python tests/golden/make_golden.py emitters`` (or ``options``) writes only
-the emitter (or adhesion / cube) files.
+``python tests/golden/make_golden.py emitters`` (or ``options``, ``w8k``,
+``d1m``, ``b64``) writes only those files.
 """
 import json
 import os
@@ -198,6 +205,59 @@ def boundary_options():
              x1=a["x"], v1=a["v"], a1=a["a"], rho1=a["rho"], x=b["x"], v=b["v"], rho=b["rho"], stats=b["stats"])
 
 
+def _box_aggregates(x, v, rho, box):
+    """inside count, kinetic sum |v|^2, centre of mass and mean density of the
+    particles still inside the box."""
+    inside = np.all((x >= 0) & (x <= np.asarray(box, np.float32)), axis=1)
+    return np.array([inside.sum(), (v.astype(np.float64) ** 2).sum(), *x[inside].astype(np.float64).mean(0),
+                     rho[inside].astype(np.float64).mean()])
+
+
+def w8k():
+    """The W8k workload (sph2o_amd.scenes.w8k: 20^3 jittered lattice flush in the
+    corner of a 0.8 x 0.4 x 0.4 m box, WCSPH k = 1000, XSPH 0.01, dt = 1e-4):
+    the state after 20 steps, and the box aggregates after the full 2,000."""
+    sc, x0, _ = SC.w8k()
+    a = S.run_reference(sc, 20, x=x0, threads=8)
+    b = S.run_reference(sc, 2000, x=x0, threads=8, timeout=3600)
+    save("ref_w8k.npz", scene=scene_json(sc), x0=x0, bx=a["bx"], bvol=a["bvol"], nsteps=np.array(20), x=a["x"],
+         v=a["v"], rho=a["rho"], long_steps=np.array(2000),
+         long_aggregates=_box_aggregates(b["x"], b["v"], b["rho"], (0.8, 0.4, 0.4)))
+
+
+def d1m(nsteps=3, nsample=4000):
+    """The D1M dam break (sph2o_amd.scenes.d1m: 1,000,000 fluid + 400,002
+    boundary particles, DFSPH with surface tension, CFL dt): per-step stats of
+    the first `nsteps` steps and the state of `nsample` random particles
+    (their sampled positions and initial densities, and `nsample` sampled
+    boundary positions) after step 1 and after step `nsteps` (8 OpenMP
+    threads)."""
+    sc, _, _ = SC.d1m()
+    a = S.run_reference(sc, 1, threads=8, timeout=3600)
+    b = S.run_reference(sc, nsteps, threads=8, timeout=3600)
+    idx = np.sort(np.random.default_rng(12).choice(len(a["x0"]), nsample, replace=False))
+    save("ref_d1m.npz", scene=scene_json(sc), nsteps=np.array(nsteps), threads=np.array(8), idx=idx,
+         x0=a["x0"][idx], bx=a["bx"][idx % len(a["bx"])], bidx=idx % len(a["bx"]), rho0=a["rho0"][idx], x1=a["x"][idx], v1=a["v"][idx], rho1=a["rho"][idx], stats1=a["stats"],
+         x=b["x"][idx], v=b["v"][idx], rho=b["rho"][idx], stats=b["stats"],
+         mean_density_error=np.array(np.mean(np.maximum(b["rho"].astype(np.float64) / 1000.0 - 1.0, 0.0))))
+
+
+def b64_smallest(nsteps=10):
+    """The smallest scene of the B64 sweep (sph2o_amd.scenes.b64) run alone:
+    the state after one step and the stats and state after `nsteps`."""
+    scenes = SC.b64()
+    sizes = []
+    for sc in scenes:
+        fb = sc["Fluid"]["FluidModel"][0]["fluidBlocks"][0]
+        sizes.append(round((fb["max"][0] - fb["min"][0]) / 0.01) ** 3)
+    k = int(np.argmin(sizes))
+    sc = scenes[k]
+    a = S.run_reference(sc, 1)
+    b = S.run_reference(sc, nsteps)
+    save("ref_b64_smallest.npz", scene=scene_json(sc), index=np.array(k), nsteps=np.array(nsteps), x0=a["x0"],
+         bx=a["bx"], x1=a["x"], v1=a["v"], stats1=a["stats"], x=b["x"], v=b["v"], rho=b["rho"], stats=b["stats"])
+
+
 def main():
     if not S.reference_available():
         raise SystemExit("compiled reference missing: make -C oracle ref")
@@ -206,6 +266,15 @@ def main():
         return
     if sys.argv[1:] == ["options"]:
         boundary_options()
+        return
+    if sys.argv[1:] == ["w8k"]:
+        w8k()
+        return
+    if sys.argv[1:] == ["d1m"]:
+        d1m()
+        return
+    if sys.argv[1:] == ["b64"]:
+        b64_smallest()
         return
     cell_keys()
     support_pairs()
@@ -224,6 +293,9 @@ def main():
     example_scene_wcsph()
     emitters()
     boundary_options()
+    w8k()
+    d1m()
+    b64_smallest()
 
 
 if __name__ == "__main__":

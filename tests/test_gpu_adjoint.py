@@ -131,16 +131,34 @@ class Case:
         finally:
             o.close()
 
-    def fd_checks(self, res, nsteps, seed=8, ndir=2):
-        """[(kind, gpu, fd)] over random directions and the coefficients."""
+    def fd_checks(self, res, nsteps, seed=8, ndir=2, kinds=None, smooth=None):
+        """[(kind, gpu, fd)] over random directions and the coefficients
+        (`kinds`: which of v0, x0 (ndir directions each), viscosity,
+        surface_tension; default all).
+
+        smooth: None takes central differences at one eps per kind and asserts
+        that no run crossed a cohesion jump (fingerprints). A decreasing
+        sequence of eps instead tests the loss for jumps, whatever their cause
+        (on large scenes the FP64 rollouts also meet non-smooth points that
+        the branches handed over do not fix): at each eps the forward and
+        backward differences f+ = (L(+eps) - L(0)) / eps and
+        f- = (L(0) - L(-eps)) / eps are formed; a jump J inside (0, eps] on
+        one side moves that side's quotient by J / eps. The first eps at
+        which f+ and f- agree within 1e-3 of their mean gives the central
+        difference; failing that, a side whose quotient agrees within 1e-4 at
+        two successive eps is smooth over the larger interval and gives its
+        one-sided difference at the smaller eps (error eps L'' / 2)."""
         X0, V0 = self.x0.astype(np.float64), self.v0.astype(np.float64)
-        _, base = self.oracle(X0, V0, self.nu, self.gam, nsteps)
+        L0, base = self.oracle(X0, V0, self.nu, self.gam, nsteps)
+        if smooth is not None:
+            return self._fd_smooth(res, nsteps, seed, ndir, kinds, smooth, X0, V0, L0)
         rng = np.random.default_rng(seed)
         out = []
-        kinds = ["v0"] * ndir + ["x0"] * ndir
-        if self.nu:
+        want = set(kinds or ("v0", "x0", "viscosity", "surface_tension"))
+        kinds = [k for k in ["v0"] * ndir + ["x0"] * ndir if k in want]
+        if self.nu and "viscosity" in want:
             kinds.append("viscosity")
-        if self.gam:
+        if self.gam and "surface_tension" in want:
             kinds.append("surface_tension")
         for kind in kinds:
             eps = 1e-8 if kind == "x0" else 1e-6
@@ -160,6 +178,52 @@ class Case:
                 lm, pm = self.oracle(X0, V0, self.nu - dn, self.gam - dg, nsteps)
             assert pp == base and pm == base, "finite difference crossed the cohesion jump at h/2 (%s)" % kind
             out.append((kind, g, (lp - lm) / (2 * eps)))
+        return out
+
+
+    def _fd_smooth(self, res, nsteps, seed, ndir, kinds, epss, X0, V0, L0):
+        rng = np.random.default_rng(seed)
+        want = set(kinds or ("v0", "x0", "viscosity", "surface_tension"))
+        todo = [k for k in ["v0"] * ndir + ["x0"] * ndir if k in want]
+        todo += [k for k, on in (("viscosity", self.nu), ("surface_tension", self.gam)) if on and k in want]
+        out = []
+        for kind in todo:
+            d = None
+            if kind in ("v0", "x0"):
+                d = rng.normal(size=self.x0.shape)
+                d /= np.linalg.norm(d)
+                g = float(np.sum(res[kind].astype(np.float64) * d))
+            else:
+                g = res[kind]
+            tried = []
+            found = None
+            for eps in epss:
+                def loss(sgn):
+                    e = sgn * eps
+                    if kind == "v0":
+                        return self.oracle(X0, V0 + e * d, self.nu, self.gam, nsteps)[0]
+                    if kind == "x0":
+                        return self.oracle(X0 + e * d, V0, self.nu, self.gam, nsteps)[0]
+                    if kind == "viscosity":
+                        return self.oracle(X0, V0, self.nu + e, self.gam, nsteps)[0]
+                    return self.oracle(X0, V0, self.nu, self.gam + e, nsteps)[0]
+                fp, fm = (loss(1) - L0) / eps, (L0 - loss(-1)) / eps
+                c = 0.5 * (fp + fm)
+                tried.append((eps, fp, fm))
+                if abs(fp - fm) <= 1e-3 * abs(c):
+                    found = c
+                    break
+                if len(tried) > 1:
+                    (_, fp0, fm0), (_, fp1, fm1) = tried[-2], tried[-1]
+                    if abs(fp1 - fp0) <= 1e-4 * abs(fp1):
+                        found = fp1
+                        break
+                    if abs(fm1 - fm0) <= 1e-4 * abs(fm1):
+                        found = fm1
+                        break
+            if found is None:
+                raise AssertionError("no eps with a smooth loss for %s: %s" % (kind, tried))
+            out.append((kind, g, found))
         return out
 
 

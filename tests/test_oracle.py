@@ -193,9 +193,13 @@ def test_sampling_matches_reference_example_scene():
     xf = S.sample_block(blk["min"], blk["max"], r)
     xb = S.sample_box(box["min"], box["max"], r)
     assert xf.shape == g["x0"].shape and xb.shape == g["bx"].shape
-    # same points to within one FP32 rounding of the reference's contracted multiply-adds
-    assert np.abs(xf - g["x0"]).max() <= 1e-7
-    assert np.abs(xb - g["bx"]).max() <= 1e-7
+    # bit for bit, with the reference's contracted multiply-adds (FMA on the
+    # lattice's y and z and on the box's z); a plain multiply-add on every axis
+    # misses a third of the box's z coordinates by one ulp
+    assert np.array_equal(xf, g["x0"]) and np.array_equal(xb, g["bx"])
+    d = golden("ref_d1m.npz")
+    blk = json.loads(str(d["scene"]))["Fluid"]["FluidModel"][0]["fluidBlocks"][0]
+    assert np.array_equal(S.sample_block(blk["min"], blk["max"], 0.005)[d["idx"]], d["x0"])
 
 
 # ---------------------------------------------------------------- solver steps
