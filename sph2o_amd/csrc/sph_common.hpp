@@ -28,6 +28,9 @@ struct SceneConst {
     // physics (reference FP32 values)
     float m, rho0, gx, gy, gz;
     float kW, lW, laplC, cohC1, cohC2;  // CubicSpline k,l; ViscoK laplCoef; Cohesion valueCoef1/2
+    // gradW = G(|r|) r with G = gA |r| + gB for |r| <= h/2 and gC (h - |r|)^2 / |r|
+    // beyond: 3 lW / h^3, -2 lW / h^2, -lW / h^3 (CubicSpline.h:52-74 rearranged)
+    float gA, gB, gC;
     float visc, bvisc, st;
     int32_t viscMethod;        // -1 off, 0 standard, 1 artificial, 2 XSPH
     int32_t stOn, bviscOn, akinci;

@@ -72,6 +72,21 @@ __device__ __forceinline__ float gradFactor(const SceneConst& S, float rl) {
     const float g = (q <= 0.5f ? near : far) * inv;
     return (rl > 1.0e-9f && !(q > 1.0f)) ? g : 0.0f;
 }
+// The same factor from d = |r|^2 with one reciprocal square root: for
+// q <= 1/2 the factor lW q (3q - 2) / (h |r|) is linear in |r|, gA |r| + gB,
+// and beyond it is gC (h - |r|)^2 / |r|. A self pair (d = 0) gets a finite
+// factor (its r = 0 makes the term zero), q > 1 gets 0.
+__device__ __forceinline__ float gradFactorSq(const SceneConst& S, float d) {
+    const float dm = fmaxf(d, 1.0e-30f);
+    const float rinv = __builtin_amdgcn_rsqf(dm);
+    const float rl = dm * rinv;
+    const float q = rl * S.invh;
+    const float t = S.h - rl;
+    const float near = __fmaf_rn(S.gA, rl, S.gB);
+    const float far = (S.gC * (t * t)) * rinv;
+    const float g = q <= 0.5f ? near : far;
+    return q > 1.0f ? 0.0f : g;
+}
 // ViscoK::laplW (ViscoK.h:29-37)
 __device__ __forceinline__ float laplW(const SceneConst& S, float rl) {
     return rl <= S.h ? S.laplC * (S.h - rl) : 0.0f;

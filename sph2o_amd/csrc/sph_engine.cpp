@@ -633,6 +633,12 @@ class Engine {
             c.m = s.m; c.rho0 = s.rho0;
             c.gx = s.P.gravity[0]; c.gy = s.P.gravity[1]; c.gz = s.P.gravity[2];
             c.kW = s.kW; c.lW = s.lW; c.laplC = s.laplC; c.cohC1 = s.cohC1; c.cohC2 = s.cohC2;
+            {
+                const float h3 = s.h * s.h * s.h;
+                c.gA = 3.0f * s.lW / h3;
+                c.gB = -2.0f * s.lW / (s.h * s.h);
+                c.gC = -(s.lW / h3);
+            }
             c.visc = s.P.viscosity; c.bvisc = s.P.boundary_viscosity; c.st = s.P.surface_tension;
             c.viscMethod = s.viscOn ? s.P.viscosity_method : -1;
             c.stOn = s.stOn ? 1 : 0;

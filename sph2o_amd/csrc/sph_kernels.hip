@@ -27,14 +27,9 @@
 namespace sph {
 
 // ---------------------------------------------------------------- helpers
-// Build-time switches (measured variants): non-temporal neighbour-list loads,
-// and whether the velocity pass gathers the packed (x, k) array or k first and
-// x only for open gates.
+// Build-time switch (a measured variant): non-temporal neighbour-list loads.
 #ifndef SPH_NT_LISTS
 #define SPH_NT_LISTS 0
-#endif
-#ifndef SPH_XK_VELOCITY
-#define SPH_XK_VELOCITY 1
 #endif
 // Rows of the fluid neighbour list that kBuildNeighbors stages in LDS (4 B x
 // 256 lanes per row) and stores as whole 1 KB rows at the end; entries past
@@ -100,23 +95,11 @@ __device__ __forceinline__ uint32_t ldList(const uint32_t* p) {
 #define SPH_BATCH 4
 #endif
 constexpr uint32_t kBatch = SPH_BATCH;
-// The source passes' (kRhoStar, kDivergence) slots per round and prefetch:
-// over the 1M-particle run's timed steps, kRhoStar 86.7 us per launch at 4
-// slots without prefetch, 85.2 at 2 with (bit-identical sums: the passes
-// spell out their FMAs).
-#ifndef SPH_BATCH_SRC
-#define SPH_BATCH_SRC 2
-#endif
+// Whether the source passes (kRhoStar, kDivergence) load the next round's
+// list entries while a round's gathers are in flight.
 #ifndef SPH_PRE_SRC
 #define SPH_PRE_SRC 1
 #endif
-// The Jacobi passes gather neighbour records through raw buffer loads (1:
-// one 32-bit offset per neighbour, the buffer's size bounding every load) or
-// global loads (0: a 64-bit address per record). Same sums bit for bit.
-#ifndef SPH_BUFFER_GATHER
-#define SPH_BUFFER_GATHER 1
-#endif
-constexpr uint32_t kBatchSrc = SPH_BATCH_SRC;
 // Lane `sub` of L visits slots sub, sub + L, ... < n of particle i's list
 // (row-major, NP entries per row): gather(j) per slot, then consume(j, g) in
 // slot order. Slots past n load slot n - 1 (a valid entry) and are not consumed.
@@ -157,6 +140,55 @@ __device__ __forceinline__ void walkList(const uint32_t* list, size_t NP, uint32
             if (s0 + k * L < n) consume(j[k], g[k]);
 #pragma unroll
         for (uint32_t k = 0; k < B; ++k) j[k] = jn[k];
+    }
+}
+// Rows [n, n rounded up to kPadRows) of every fluid list hold the particle
+// itself (kBuildNeighbors writes them), whose term in the Jacobi passes is
+// exactly zero (r = 0). A lane of those passes therefore walks whole rounds
+// of kPadRows = L x B rows with no per-row test, and since every lane of a
+// wave is at the same round, a row's list address is a wave-uniform base
+// plus the lane's fixed 32-bit offset (global loads with a scalar base: no
+// vector arithmetic per row). Pre: the next round's entries are loaded while
+// this round's gathers are in flight.
+constexpr uint32_t kPadRows = 8;
+static_assert(SPH_LIST_STAGE % kPadRows == 0, "the staged rows end on a round boundary");
+template <uint32_t L, uint32_t B, bool Pre, class G, class C>
+__device__ __forceinline__ void walkRows(const uint32_t* list, uint32_t NP, uint32_t i, uint32_t sub, uint32_t n,
+                                         G&& gather, C&& consume) {
+    static_assert(L * B == kPadRows, "a round covers kPadRows rows");
+    const uint32_t rounds = (n + kPadRows - 1u) / kPadRows;
+    if (rounds == 0) return;
+    // rows r kPadRows + k L .. + L - 1 as one buffer (a scalar descriptor per
+    // row group; the lane reads at its fixed offset within it, < L NP 4 bytes)
+    const uint32_t vo = (sub * NP + i) * static_cast<uint32_t>(sizeof(uint32_t));
+    const uint32_t groupBytes = L * NP * static_cast<uint32_t>(sizeof(uint32_t));
+    auto load = [&](uint32_t r, uint32_t k) {
+        const uint32_t* rowK = list + (static_cast<size_t>(r) * kPadRows + k * L) * NP;
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(rowK), 0, groupBytes, 0x00020000);
+        return __builtin_amdgcn_raw_buffer_load_b32(rs, vo, 0, 0);
+    };
+    uint32_t j[B];
+#pragma unroll
+    for (uint32_t k = 0; k < B; ++k) j[k] = load(0, k);
+    for (uint32_t r = 0; r < rounds; ++r) {
+        decltype(gather(0u)) g[B];
+#pragma unroll
+        for (uint32_t k = 0; k < B; ++k) g[k] = gather(j[k]);
+        if constexpr (Pre) {
+            if (r + 1 < rounds) {
+#pragma unroll
+                for (uint32_t k = 0; k < B; ++k) j[k] = load(r + 1, k);
+            }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < B; ++k) consume(g[k]);
+        if constexpr (!Pre) {
+            if (r + 1 < rounds) {
+#pragma unroll
+                for (uint32_t k = 0; k < B; ++k) j[k] = load(r + 1, k);
+            }
+        }
     }
 }
 struct XV { float4 x, v; };
@@ -612,6 +644,14 @@ __device__ __forceinline__ void buildNeighbors(const DevView& d) {
         if (sub == 0) {
             d.cntF[i] = nf;
             d.cntB[i] = nb;
+            // rows [nf, nf rounded up to kPadRows) hold the particle itself
+            // (the staged rows get theirs below); capF is a multiple of kPadRows
+            const uint32_t padEnd = min((nf + kPadRows - 1u) / kPadRows * kPadRows, d.capF);
+#if SPH_LIST_STAGE
+            for (uint32_t r = max(nf, static_cast<uint32_t>(SPH_LIST_STAGE)); r < padEnd; ++r) d.nbrF[r * NP + i] = i;
+#else
+            for (uint32_t r = nf; r < padEnd; ++r) d.nbrF[r * NP + i] = i;
+#endif
         }
     }
 #if SPH_LIST_STAGE
@@ -639,16 +679,21 @@ __device__ __forceinline__ void buildNeighbors(const DevView& d) {
         atomicMax(&d.flags[sIdx * kFlagSlots + kFlagMaxBoundaryNbr], mb);
     }
 #if SPH_LIST_STAGE
-    // whole rows up to the block's longest list; entries past a particle's own
-    // count get ~0 (never read: every pass stops at min(count, capacity))
+    // whole rows up to the block's longest list rounded up to kPadRows; rows
+    // of a particle from its count to the next multiple of kPadRows hold the
+    // particle itself (a zero term in the Jacobi passes), later ones ~0 (never
+    // read: every pass stops at min(count, capacity) rounded up)
     int rows = smf[0];
     for (int w = 1; w < kBlock / 64; ++w) rows = max(rows, smf[w]);
+    rows = (rows + static_cast<int>(kPadRows) - 1) / static_cast<int>(kPadRows) * static_cast<int>(kPadRows);
     rows = min(rows, min(static_cast<int>(d.capF), SPH_LIST_STAGE));
     const size_t NPs = d.NP;
     const uint32_t first = lb * PB;   // the block's particles: [first, first + PB) < NP
     for (uint32_t e = threadIdx.x; e < static_cast<uint32_t>(rows) * PB; e += kBlock) {
         const uint32_t row = e / PB, t = e % PB;
-        d.nbrF[row * NPs + first + t] = row < sCnt[t] ? sList[row * PB + t] : ~0u;
+        const uint32_t c = sCnt[t];
+        const uint32_t pad = (c + kPadRows - 1u) / kPadRows * kPadRows;
+        d.nbrF[row * NPs + first + t] = row < c ? sList[row * PB + t] : (row < pad ? first + t : ~0u);
     }
 #endif
 }
@@ -1065,7 +1110,6 @@ __device__ __forceinline__ float sourceShare(const DevView& d, const SceneConst&
                                              float4 xi, float4 vi, uint32_t nf, uint32_t nb) {
     const size_t NP = d.NP;
     float dc = 0.0f;
-#if SPH_BUFFER_GATHER
     // raw buffer loads: one 32-bit offset per neighbour for both records
     // instead of two 64-bit addresses (num_records bounds every load)
     const uint32_t bytes = static_cast<uint32_t>(NP * sizeof(float4));
@@ -1076,20 +1120,16 @@ __device__ __forceinline__ float sourceShare(const DevView& d, const SceneConst&
         return XV{__builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rX, off, 0, 0)),
                   __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rV, off, 0, 0))};
     };
-#else
-    auto gather = [&](uint32_t j) { return XV{POSV(d, j), VEL(d, j)}; };
-#endif
-    walkList<L, kBatchSrc, SPH_PRE_SRC != 0>(d.nbrF, NP, i, sub, nf, gather,
-                        [&](uint32_t, const XV& q) {
-                            const float4 xj = q.x, vj = q.v;
-                            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                            const float g = gradFactor(S, fastSqrt(dRef(rx, ry, rz)));
-                            dc = pairFma(S.m, g * dot3f(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, rx, ry, rz), dc);
-                        });
+    walkRows<L, kPadRows / L, SPH_PRE_SRC != 0>(d.nbrF, d.NP, i, sub, nf, gather, [&](const XV& q) {
+        const float4 xj = q.x, vj = q.v;
+        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        const float g = gradFactorSq(S, dRef(rx, ry, rz));
+        dc = pairFma(S.m, g * dot3f(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, rx, ry, rz), dc);
+    });
     for (uint32_t s = sub; s < nb; s += L) {
         const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
         const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
-        const float g = gradFactor(S, fastSqrt(dRef(rx, ry, rz)));
+        const float g = gradFactorSq(S, dRef(rx, ry, rz));
         dc = pairFma(xb.w, g * dot3f(vi.x, vi.y, vi.z, rx, ry, rz), dc);
     }
     return dc;
@@ -1150,55 +1190,27 @@ __device__ __forceinline__ void velocityUpdate(const DevView& d, const SceneCons
         const float4 xi = d.xk[i];
         const float ki = xi.w;
         const uint32_t nf = min(d.cntF[i], d.capF);
-#if SPH_XK_VELOCITY
-#if SPH_BUFFER_GATHER
         const __amdgpu_buffer_rsrc_t rXK =
             __builtin_amdgcn_make_buffer_rsrc(d.xk, 0, static_cast<uint32_t>(NP * sizeof(float4)), 0x00020000);
         auto gatherXK = [&](uint32_t j) {
             return __builtin_bit_cast(
                 float4, __builtin_amdgcn_raw_buffer_load_b128(rXK, j * static_cast<uint32_t>(sizeof(float4)), 0, 0));
         };
-#else
-        auto gatherXK = [&](uint32_t j) { return d.xk[j]; };
-#endif
-        walkList<kLppVel, kBatch, true>(d.nbrF, NP, i, sub, nf, gatherXK,
-                                  [&](uint32_t, const float4& xj) {
-                                      const float ks = ki + xj.w;
-                                      if (fabsf(ks) > S.eps) {
-                                          const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                                          const float c = (S.m * ks) * gradFactor(S, fastSqrt(dRef(rx, ry, rz)));
-                                          sx = pairFma(c, rx, sx); sy = pairFma(c, ry, sy); sz = pairFma(c, rz, sz);
-                                      }
-                                  });
-#else
-        // k_j first; the position gather only for open gates
-        for (uint32_t s0 = sub; s0 < nf; s0 += kBatch * kLppVel) {
-            uint32_t j[kBatch];
-            float ks[kBatch];
-            float4 xj[kBatch];
-#pragma unroll
-            for (uint32_t q = 0; q < kBatch; ++q)
-                j[q] = ldList(d.nbrF + static_cast<size_t>(min(s0 + q * kLppVel, nf - 1u)) * NP + i);
-#pragma unroll
-            for (uint32_t q = 0; q < kBatch; ++q) ks[q] = ki + d.k[j[q]];
-#pragma unroll
-            for (uint32_t q = 0; q < kBatch; ++q)
-                if (s0 + q * kLppVel < nf && fabsf(ks[q]) > S.eps) xj[q] = d.x[j[q]];
-#pragma unroll
-            for (uint32_t q = 0; q < kBatch; ++q)
-                if (s0 + q * kLppVel < nf && fabsf(ks[q]) > S.eps) {
-                    const float rx = xi.x - xj[q].x, ry = xi.y - xj[q].y, rz = xi.z - xj[q].z;
-                    const float c = (S.m * ks[q]) * gradFactor(S, fastSqrt(dRef(rx, ry, rz)));
-                    sx = pairFma(c, rx, sx); sy = pairFma(c, ry, sy); sz = pairFma(c, rz, sz);
-                }
-        }
-#endif
+        // a closed gate (|k_i + k_j| <= eps) adds c = 0: the same sums as
+        // skipping the pair, without a branch per pair
+        walkRows<kLppVel, kPadRows / kLppVel, true>(d.nbrF, d.NP, i, sub, nf, gatherXK, [&](const float4& xj) {
+            const float ks = ki + xj.w;
+            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            const float c0 = (S.m * ks) * gradFactorSq(S, dRef(rx, ry, rz));
+            const float c = fabsf(ks) > S.eps ? c0 : 0.0f;
+            sx = pairFma(c, rx, sx); sy = pairFma(c, ry, sy); sz = pairFma(c, rz, sz);
+        });
         if (S.akinci && fabsf(ki) > S.eps) {
             const uint32_t nb = min(d.cntB[i], d.capB);
             for (uint32_t s = sub; s < nb; s += kLppVel) {
                 const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
                 const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
-                const float c = (xb.w * ki) * gradFactor(S, fastSqrt(dRef(rx, ry, rz)));
+                const float c = (xb.w * ki) * gradFactorSq(S, dRef(rx, ry, rz));
                 sx = pairFma(c, rx, sx); sy = pairFma(c, ry, sy); sz = pairFma(c, rz, sz);
             }
         }
