@@ -6,6 +6,11 @@
 namespace sph {
 
 constexpr int kBlock = 256;  // particles per workgroup (4 wave64)
+// DFSPH Jacobi tiles: groups of kTileGroup consecutive slots (one launch block
+// of the 4-lanes-per-particle passes) stage the records of at most kTileMax
+// distinct neighbours in LDS
+constexpr uint32_t kTileGroup = 64;
+constexpr uint32_t kTileMax = 512;
 
 // Per-scene constants, uploaded at init / regrid. One scene = one SPH2O
 // single-fluid scene with its own grid, particle ranges and coefficients.

@@ -116,14 +116,14 @@ enum KernelId {
     K_KEYS, K_SORT, K_REORDER, K_BUILD, K_DENSITY, K_FORCES, K_SURFACE_TENSION, K_VMAX, K_PREDICT, K_RHO_STAR,
     K_VELOCITY, K_DIVERGENCE, K_SCALE, K_INTEGRATE, K_PCI_INIT, K_PCI_DENSITY, K_PCI_PACC, K_PCI_INTEGRATE,
     K_FILL, K_FINISH, K_ADJ_SEED, K_ADJ_FORCES, K_ADJ_NORMALS, K_ADJ_DENSITY, K_ADJ_FINISH, K_ADJ_VELOCITY,
-    K_ADJ_SOURCE, K_CUBE, K_COUNT
+    K_ADJ_SOURCE, K_CUBE, K_TILES, K_COUNT
 };
 const char* kKernelNames[K_COUNT] = {"keys", "sort", "reorder", "build_neighbors", "density", "forces",
                                      "surface_tension", "vmax", "predict_velocity", "rho_star", "velocity",
                                      "divergence", "scale", "integrate", "pci_init", "pci_density",
                                      "pci_pressure_acc", "pci_integrate", "fill_ranges", "finish",
                                      "adj_seed", "adj_forces", "adj_normals", "adj_density", "adj_finish",
-                                     "adj_velocity", "adj_source", "cube_boundary"};
+                                     "adj_velocity", "adj_source", "cube_boundary", "tile_lists"};
 
 // Host mirror of the reference's FP32 cubic-spline gradient (CubicSpline.h:52-74)
 // with the compiled reference's length; used for the PCISPH scaling factor.
@@ -224,6 +224,11 @@ class Engine {
     bool radixOnce = false;      // the next search uses the radix sort (a cell was too crowded)
     bool rangesDirty = true;     // fRange holds ranges cellOf does not list: clear all of it
     DevBuf<uint32_t> nbrF, cntF, nbrB, cntB;
+    // DFSPH: per group of kTileGroup slots the fluid slots its lists name
+    // (kTileMax per group), their count (~0: too many, the group gathers from
+    // HBM), and the lists as 16-bit indices into that set, two per word
+    DevBuf<uint32_t> tileSlot, tileCount, nbrT;
+    bool tiles = false;
     DevBuf<double> partials;
     DevBuf<float4> snapV;   // DFSPH: v and kv_ws before the passes that follow a neighbour search
     DevBuf<float> snapKvws;
@@ -724,6 +729,7 @@ class Engine {
 
     void allocNeighborLists() {
         nbrF.alloc(static_cast<size_t>(capF) * NP);
+        if (tiles) nbrT.alloc(static_cast<size_t>(capF / 2) * NP);
         nbrB.alloc(static_cast<size_t>(capB) * NP);
     }
 
@@ -744,6 +750,9 @@ class Engine {
         d.snapV = snapV.p; d.snapKvws = snapKvws.p;
         d.nbrF = nbrF.p; d.cntF = cntF.p; d.nbrB = nbrB.p; d.cntB = cntB.p;
         d.capF = capF; d.capB = capB;
+        d.tileSlot = tiles ? tileSlot.p : nullptr;
+        d.tileCount = tiles ? tileCount.p : nullptr;
+        d.nbrT = tiles ? nbrT.p : nullptr;
         d.partials = partials.p; d.flags = dFlags;
         return d;
     }
@@ -820,6 +829,12 @@ class Engine {
         for (auto* b : {&rho, &alpha, &p, &k, &kws, &kwsAlt, &kvws, &kvwsAlt, &aux}) b->alloc(np);
         for (auto* b : {&id, &idAlt, &keys, &keysAlt, &vals, &valsAlt, &cntF, &cntB, &cellOf, &li}) b->alloc(np);
         HIPCHK(hipMemset(cellOf.p, 0xFF, np * sizeof(uint32_t)));
+        tiles = method == SPH_DFSPH && sphKernelsTiles() != 0;
+        if (tiles) {
+            const size_t groups = np / kTileGroup + 1;
+            tileSlot.alloc(groups * kTileMax);
+            tileCount.alloc(groups);
+        }
         allocNeighborLists();
         // the DFSPH source passes run sphKernelsLanesPerParticle() launch blocks per particle block
         partials.alloc(static_cast<size_t>(numBlocks ? numBlocks : 1) *
@@ -1015,6 +1030,7 @@ class Engine {
         std::swap(id.p, idAlt.p);
         d = view();
         timed(K_BUILD, [&] { return launchBuildNeighbors(d, stream); });
+        if (tiles) timed(K_TILES, [&] { return launchTileLists(d, stream); });
     }
     // Interpret the flags of the last search (hFlags must be current). Returns
     // true if the lists are valid; otherwise grows what overflowed and returns

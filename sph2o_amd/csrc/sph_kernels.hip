@@ -191,6 +191,53 @@ __device__ __forceinline__ void walkRows(const uint32_t* list, uint32_t NP, uint
         }
     }
 }
+// The tile of launch block g (logical block = tile group: 64 particles of a
+// 4-lanes-per-particle pass): stages A[slot] (and B[slot]) of its tile's
+// slots into tA (tB) and returns true; false (nothing staged) when the
+// context has no tiles, the block's scene sits this pass out, or the group's
+// tile overflowed. Every condition is uniform over the block.
+__device__ __forceinline__ bool stageTile(const DevView& d, uint32_t g, bool active, const float4* A, float4* tA,
+                                          const float4* B, float4* tB) {
+    if (!d.tileCount || !active) return false;
+    const uint32_t T = d.tileCount[g];
+    if (T > kTileMax) return false;
+    const uint32_t* slots = d.tileSlot + static_cast<size_t>(g) * kTileMax;
+    const uint32_t bytes = d.NP * static_cast<uint32_t>(sizeof(float4));
+    const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(A), 0, bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rB =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(B ? B : A), 0, bytes, 0x00020000);
+    for (uint32_t t = threadIdx.x; t < T; t += kBlock) {
+        const uint32_t off = slots[t] * static_cast<uint32_t>(sizeof(float4));
+        tA[t] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rA, off, 0, 0));
+        if (B) tB[t] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rB, off, 0, 0));
+    }
+    __syncthreads();
+    return true;
+}
+// Tile-index walk of particle i's list by lane `sub` of 4: word row 4R + sub
+// holds rows 8R + sub and 8R + sub + 4, the rows walkRows<4, 2> gives the lane
+// in the same order, so the sums match the untiled walk bit for bit.
+template <class C>
+__device__ __forceinline__ void walkTile(const DevView& d, uint32_t i, uint32_t sub, uint32_t n, C&& consume) {
+    const uint32_t rounds = (n + kPadRows - 1u) / kPadRows;
+    if (rounds == 0) return;
+    const uint32_t NP = d.NP;
+    const uint32_t vo = (sub * NP + i) * static_cast<uint32_t>(sizeof(uint32_t));
+    const uint32_t groupBytes = 4u * NP * static_cast<uint32_t>(sizeof(uint32_t));
+    auto load = [&](uint32_t R) {
+        const uint32_t* row = d.nbrT + static_cast<size_t>(R) * 4u * NP;
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(row), 0, groupBytes, 0x00020000);
+        return __builtin_amdgcn_raw_buffer_load_b32(rs, vo, 0, 0);
+    };
+    uint32_t w = load(0);
+    for (uint32_t R = 0; R < rounds; ++R) {
+        const uint32_t cur = w;
+        if (R + 1 < rounds) w = load(R + 1);
+        consume(cur & 0xffffu);
+        consume(cur >> 16);
+    }
+}
 struct XV { float4 x, v; };
 // The Jacobi passes spell out their FMAs (r^2 as dRef, dot products and the
 // accumulations): left to the compiler, which product of a sum such as
@@ -707,6 +754,122 @@ __device__ __forceinline__ void buildNeighbors(const DevView& d) {
 constexpr uint32_t kLppBuild = SPH_LPP_BUILD;
 __global__ void __launch_bounds__(kBlock) kBuildNeighbors(DevView d) { buildNeighbors<kLppBuild>(d); }
 
+// ---------------------------------------------------------------- DFSPH tiles
+// The Jacobi passes of one launch block (kTileGroup = 64 particles, 4 lanes
+// each) read the records of a few hundred distinct neighbours, each of them
+// several times, gathered one list entry at a time from L2/HBM. A tile is the
+// set of distinct fluid slots the group's lists name: the passes stage their
+// records in LDS with one wide cooperative load, then walk 16-bit indices
+// into the tile. Tiles are rebuilt with the lists (kTileLists, one block per
+// group): a block-wide open-addressing hash set in LDS collects the slots,
+// a scan over the table numbers them, and the lists are rewritten as tile
+// indices. A group with more than kTileMax distinct neighbours is marked
+// (count ~0) and its passes gather from HBM as before; the row order of both
+// paths is the same, so their sums are the same. Needs 4 lanes per particle
+// in all three Jacobi passes (SPH_LPP_VEL=4). Measured on the 1M-particle dam
+// break (ms per step; per-launch us of kRhoStar / kDivergence / velocity):
+// tiles 1.83 (66.8 / 75.8 / 41.5, and 109 us per tile build, which also
+// slows the once-per-step passes that follow it), no tiles with 4 velocity
+// lanes 1.74 (71.1 / 80.5 / 41.6), no tiles with 2 velocity lanes 1.71
+// (71.1 / 80.4 / 38.4): off by default.
+#ifndef SPH_TILES
+#define SPH_TILES 0
+#endif
+extern "C" __host__ int sphKernelsTiles() { return SPH_TILES; }
+constexpr uint32_t kTileHash = 2 * kTileMax;   // hash slots (load factor <= 1/2)
+static_assert((kTileHash & (kTileHash - 1)) == 0, "hash size is a power of two");
+static_assert(!SPH_TILES || (kTileGroup * SPH_LPP == kBlock && SPH_LPP == 4 && SPH_LPP_VEL == 4),
+              "a tile group is one launch block of the 4-lanes-per-particle passes");
+__device__ __forceinline__ uint32_t tileHash(uint32_t j) { return (j * 0x9E3779B1u) >> 22; }   // 10 bits
+static_assert(kTileHash == 1024, "tileHash yields 10 bits");
+
+__global__ void __launch_bounds__(kBlock) kTileLists(DevView d) {
+    __shared__ uint32_t key[kTileHash];
+    __shared__ uint16_t tix[kTileHash];
+    __shared__ uint32_t nUnique, overflow;
+    __shared__ uint32_t wsum[kBlock / 64];
+    const uint32_t g = blockIdx.x;
+    const uint32_t first = g * kTileGroup;
+    const SceneConst& S = d.scenes[d.blockScene[first / kBlock]];
+    const size_t NP = d.NP;
+    for (uint32_t h = threadIdx.x; h < kTileHash; h += kBlock) key[h] = ~0u;
+    if (threadIdx.x == 0) {
+        nUnique = 0;
+        overflow = 0;
+    }
+    __syncthreads();
+    // four threads per particle, rows q, q + 4, ... (self-padding rows name
+    // the particle itself, which its own list holds already)
+    const uint32_t p = threadIdx.x % kTileGroup, q = threadIdx.x / kTileGroup;
+    const uint32_t i = first + p;
+    const uint32_t n = i < S.fEnd ? min(d.cntF[i], d.capF) : 0u;
+    const uint32_t nPad = (n + kPadRows - 1u) / kPadRows * kPadRows;
+    for (uint32_t r = q; r < n; r += 4) {
+        const uint32_t j = d.nbrF[r * NP + i];
+        uint32_t h = tileHash(j);
+        for (uint32_t probe = 0;; ++probe) {
+            if (probe == kTileHash) {
+                overflow = 1;
+                break;
+            }
+            const uint32_t old = atomicCAS(&key[h], ~0u, j);
+            if (old == ~0u) {
+                atomicAdd(&nUnique, 1u);
+                break;
+            }
+            if (old == j) break;
+            h = (h + 1u) & (kTileHash - 1u);
+        }
+    }
+    __syncthreads();
+    const uint32_t T = nUnique;
+    if (overflow || T > kTileMax) {
+        if (threadIdx.x == 0) d.tileCount[g] = ~0u;
+        return;
+    }
+    // number the occupied slots in table order: thread t owns slots 8t .. 8t + 7
+    constexpr uint32_t kPer = kTileHash / kBlock;
+    uint32_t own = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) own += key[threadIdx.x * kPer + k] != ~0u ? 1u : 0u;
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    uint32_t incl = own;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t t = incl - own;
+    for (uint32_t k = 0; k < w; ++k) t += wsum[k];
+    uint32_t* slots = d.tileSlot + static_cast<size_t>(g) * kTileMax;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        const uint32_t h = threadIdx.x * kPer + k;
+        const uint32_t j = key[h];
+        if (j != ~0u) {
+            tix[h] = static_cast<uint16_t>(t);
+            slots[t] = j;
+            ++t;
+        }
+    }
+    __syncthreads();
+    auto lookup = [&](uint32_t j) {
+        uint32_t h = tileHash(j);
+        while (key[h] != j) h = (h + 1u) & (kTileHash - 1u);   // every j of the group is in the table
+        return static_cast<uint32_t>(tix[h]);
+    };
+    // word w of a particle: rows 8 (w / 4) + w % 4 and that row + 4
+    for (uint32_t wd = q; wd < nPad / 2; wd += 4) {
+        const uint32_t r0 = (wd >> 2) * kPadRows + (wd & 3u);
+        const uint32_t t0 = lookup(d.nbrF[r0 * NP + i]);
+        const uint32_t t1 = lookup(d.nbrF[(r0 + 4u) * NP + i]);
+        d.nbrT[wd * NP + i] = t0 | (t1 << 16);
+    }
+    if (threadIdx.x == 0) d.tileCount[g] = T;
+}
+
 // AkinciBoundaryObject::computeVolume (AkinciBoundaryObject.cpp:10-34):
 // V_b = 1 / sum of W over the boundary candidates of the 27 cells.
 __global__ void kBoundaryVolume(DevView d, int scene, uint32_t bBegin, uint32_t bEnd) {
@@ -1107,25 +1270,31 @@ __global__ void __launch_bounds__(kBlock) kPredictVelocity(DevView d) {
 // computeDivergenceError (:560-606).
 template <uint32_t L>
 __device__ __forceinline__ float sourceShare(const DevView& d, const SceneConst& S, uint32_t i, uint32_t sub,
-                                             float4 xi, float4 vi, uint32_t nf, uint32_t nb) {
+                                             float4 xi, float4 vi, uint32_t nf, uint32_t nb, const float4* tX,
+                                             const float4* tV) {
     const size_t NP = d.NP;
     float dc = 0.0f;
-    // raw buffer loads: one 32-bit offset per neighbour for both records
-    // instead of two 64-bit addresses (num_records bounds every load)
-    const uint32_t bytes = static_cast<uint32_t>(NP * sizeof(float4));
-    const __amdgpu_buffer_rsrc_t rX = __builtin_amdgcn_make_buffer_rsrc(d.x, 0, bytes, 0x00020000);
-    const __amdgpu_buffer_rsrc_t rV = __builtin_amdgcn_make_buffer_rsrc(d.v, 0, bytes, 0x00020000);
-    auto gather = [&](uint32_t j) {
-        const uint32_t off = j * static_cast<uint32_t>(sizeof(float4));
-        return XV{__builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rX, off, 0, 0)),
-                  __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rV, off, 0, 0))};
-    };
-    walkRows<L, kPadRows / L, SPH_PRE_SRC != 0>(d.nbrF, d.NP, i, sub, nf, gather, [&](const XV& q) {
-        const float4 xj = q.x, vj = q.v;
+    auto pair = [&](float4 xj, float4 vj) {
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
         const float g = gradFactorSq(S, dRef(rx, ry, rz));
         dc = pairFma(S.m, g * dot3f(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, rx, ry, rz), dc);
-    });
+    };
+    if (tX) {
+        walkTile(d, i, sub, nf, [&](uint32_t t) { pair(tX[t], tV[t]); });
+    } else {
+        // raw buffer loads: one 32-bit offset per neighbour for both records
+        // instead of two 64-bit addresses (num_records bounds every load)
+        const uint32_t bytes = static_cast<uint32_t>(NP * sizeof(float4));
+        const __amdgpu_buffer_rsrc_t rX = __builtin_amdgcn_make_buffer_rsrc(d.x, 0, bytes, 0x00020000);
+        const __amdgpu_buffer_rsrc_t rV = __builtin_amdgcn_make_buffer_rsrc(d.v, 0, bytes, 0x00020000);
+        auto gather = [&](uint32_t j) {
+            const uint32_t off = j * static_cast<uint32_t>(sizeof(float4));
+            return XV{__builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rX, off, 0, 0)),
+                      __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rV, off, 0, 0))};
+        };
+        walkRows<L, kPadRows / L, SPH_PRE_SRC != 0>(d.nbrF, d.NP, i, sub, nf, gather,
+                                                    [&](const XV& q) { pair(q.x, q.v); });
+    }
     for (uint32_t s = sub; s < nb; s += L) {
         const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
         const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
@@ -1147,13 +1316,16 @@ __global__ void __launch_bounds__(kBlock) kRhoStar(DevView d, int warm, int slot
             loopStart(d.state[s], 0.0, static_cast<double>(C.eta), C.minIt, C.maxIt);
         }
     const bool first = T.it == 0;
+    __shared__ float4 tX[kTileMax], tV[kTileMax];
+    const bool tiled = stageTile(d, lb, T.active[slot] != 0, d.x, tX, d.v, tV);
     double err = 0.0;
     float4 xi = make_float4(0.0f, 0.0f, 0.0f, 0.0f), vi = xi;
     float dc = 0.0f;
     if (valid) {
         xi = d.x[i];
         vi = VEL(d, i);
-        dc = sourceShare<kLpp>(d, S, i, sub, xi, vi, min(d.cntF[i], d.capF), S.akinci ? min(d.cntB[i], d.capB) : 0u);
+        dc = sourceShare<kLpp>(d, S, i, sub, xi, vi, min(d.cntF[i], d.capF), S.akinci ? min(d.cntB[i], d.capB) : 0u,
+                               tiled ? tX : nullptr, tV);
     }
     dc = lppSum<kLpp>(dc);
     if (valid && sub == 0) {
@@ -1183,7 +1355,7 @@ __global__ void __launch_bounds__(kBlock) kRhoStar(DevView d, int warm, int slot
 // Velocity update of one Jacobi pass, kLppVel lanes per particle; shared by the
 // two kernels below.
 __device__ __forceinline__ void velocityUpdate(const DevView& d, const SceneConst& S, uint32_t i, uint32_t sub,
-                                               bool valid, float dt) {
+                                               bool valid, float dt, const float4* tXK) {
     const size_t NP = d.NP;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     if (valid) {
@@ -1198,13 +1370,15 @@ __device__ __forceinline__ void velocityUpdate(const DevView& d, const SceneCons
         };
         // a closed gate (|k_i + k_j| <= eps) adds c = 0: the same sums as
         // skipping the pair, without a branch per pair
-        walkRows<kLppVel, kPadRows / kLppVel, true>(d.nbrF, d.NP, i, sub, nf, gatherXK, [&](const float4& xj) {
+        auto pair = [&](const float4& xj) {
             const float ks = ki + xj.w;
             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
             const float c0 = (S.m * ks) * gradFactorSq(S, dRef(rx, ry, rz));
             const float c = fabsf(ks) > S.eps ? c0 : 0.0f;
             sx = pairFma(c, rx, sx); sy = pairFma(c, ry, sy); sz = pairFma(c, rz, sz);
-        });
+        };
+        if (tXK) walkTile(d, i, sub, nf, [&](uint32_t t) { pair(tXK[t]); });
+        else walkRows<kLppVel, kPadRows / kLppVel, true>(d.nbrF, d.NP, i, sub, nf, gatherXK, pair);
         if (S.akinci && fabsf(ki) > S.eps) {
             const uint32_t nb = min(d.cntB[i], d.capB);
             for (uint32_t s = sub; s < nb; s += kLppVel) {
@@ -1230,14 +1404,18 @@ __device__ __forceinline__ void velocityUpdate(const DevView& d, const SceneCons
 __global__ void __launch_bounds__(kBlock) kVelocity(DevView d, int slot) {
     SCENE_PROLOGUE_LANES(slot, kLppVel)
     (void)dt2;
-    velocityUpdate(d, S, i, sub, valid, dt);
+    __shared__ float4 tXK[kTileMax];
+    const bool tiled = stageTile(d, lb, T.active[slot] != 0, d.xk, tXK, nullptr, nullptr);
+    velocityUpdate(d, S, i, sub, valid, dt, tiled ? tXK : nullptr);
 }
 // The same, plus the loop reduction of the preceding pass, done by the first
 // blocks after their own particles (see finishInline).
 __global__ void __launch_bounds__(kBlock) kVelocityFinish(DevView d, int slot, int fin) {
     SCENE_PROLOGUE_LANES(slot, kLppVel)
     (void)dt2;
-    velocityUpdate(d, S, i, sub, valid, dt);
+    __shared__ float4 tXK[kTileMax];
+    const bool tiled = stageTile(d, lb, T.active[slot] != 0, d.xk, tXK, nullptr, nullptr);
+    velocityUpdate(d, S, i, sub, valid, dt, tiled ? tXK : nullptr);
     finishInline(d, fin, slot);
 }
 
@@ -1256,6 +1434,8 @@ __global__ void __launch_bounds__(kBlock) kDivergence(DevView d, int warm, int s
             loopStart(U, 0.0, static_cast<double>(C.etaV / U.dt), C.minItV, C.maxIt);
         }
     const bool first = T.it == 0;
+    __shared__ float4 tX[kTileMax], tV[kTileMax];
+    const bool tiled = stageTile(d, lb, T.active[slot] != 0, d.x, tX, d.v, tV);
     double err = 0.0;
     float4 xi = make_float4(0.0f, 0.0f, 0.0f, 0.0f), vi = xi;
     float de = 0.0f;
@@ -1266,7 +1446,8 @@ __global__ void __launch_bounds__(kBlock) kDivergence(DevView d, int warm, int s
         const uint32_t nfTrue = d.cntF[i];
         const uint32_t nbTrue = S.akinci ? d.cntB[i] : 0u;
         cnt = nfTrue + nbTrue;
-        de = sourceShare<kLpp>(d, S, i, sub, xi, vi, min(nfTrue, d.capF), min(nbTrue, d.capB));
+        de = sourceShare<kLpp>(d, S, i, sub, xi, vi, min(nfTrue, d.capF), min(nbTrue, d.capB), tiled ? tX : nullptr,
+                               tV);
     }
     de = lppSum<kLpp>(de);
     if (valid && sub == 0) {
@@ -1615,6 +1796,9 @@ hipError_t launchReorder(const DevView& d, const uint32_t* ks, const uint32_t* v
 }
 hipError_t launchBuildNeighbors(const DevView& d, hipStream_t st) {
     LAUNCH(kBuildNeighbors, d.numBlocks * kLppBuild, d);
+}
+hipError_t launchTileLists(const DevView& d, hipStream_t st) {
+    LAUNCH(kTileLists, d.numBlocks * (kBlock / kTileGroup), d);
 }
 hipError_t launchBoundaryVolume(const DevView& d, int scene, uint32_t b0, uint32_t b1, hipStream_t st) {
     const uint32_t g = (b1 - b0 + kBlock - 1) / kBlock;

@@ -58,6 +58,14 @@ struct DevView {
     uint32_t* nbrB;
     uint32_t* cntB;
     uint32_t capF, capB;
+    // DFSPH tiles (null elsewhere): per group g of kTileGroup slots,
+    // tileSlot[g * kTileMax + t] (t < tileCount[g]) are the fluid slots its
+    // lists name; tileCount[g] = ~0 if more than kTileMax. nbrT: the fluid
+    // lists as 16-bit tile indices, word [w * NP + i] = rows 8 (w / 4) + w % 4
+    // (low half) and that row + 4 (high half)
+    uint32_t* tileSlot;
+    uint32_t* tileCount;
+    uint32_t* nbrT;
     // reductions and flags
     double* partials;   // numBlocks
     int32_t* flags;     // numScenes * kFlagSlots
@@ -87,6 +95,8 @@ hipError_t launchFillRanges(uint2* r, size_t n, hipStream_t st);
 hipError_t launchReorder(const DevView& d, const uint32_t* keysSorted, const uint32_t* valsSorted, float4* xAlt,
                          float4* vAlt, float* kwsAlt, float* kvwsAlt, uint32_t* idAlt, int ranges, hipStream_t st);
 hipError_t launchBuildNeighbors(const DevView& d, hipStream_t st);
+// DFSPH tiles of the lists just built (d.tileSlot, d.tileCount, d.nbrT)
+hipError_t launchTileLists(const DevView& d, hipStream_t st);
 // cube boundary objects of every scene (no-op for scenes without)
 hipError_t launchCubeBoundary(const DevView& d, hipStream_t st);
 hipError_t launchBoundaryVolume(const DevView& d, int scene, uint32_t bBegin, uint32_t bEnd, hipStream_t st);
@@ -133,6 +143,8 @@ hipError_t launchViewerFeed(const DevView& d, int scene, uint32_t count, float4*
 }  // namespace sph
 
 extern "C" int sphKernelsFuseFinish();
+// whether the DFSPH Jacobi passes stage LDS tiles (1) or gather from HBM (0)
+extern "C" int sphKernelsTiles();
 extern "C" int sphKernelsLanesPerParticle();
 
 namespace sph {
