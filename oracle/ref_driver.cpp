@@ -19,8 +19,13 @@
 //     FluidObject::build with caller-supplied ones (importFluidState is dead
 //     code: it calls exit(0), SPHSimulation.cpp:548).
 //
+//   * with --frames, steps through SPHSimulation::step() itself with frame
+//     output on (activateSave), as `SPH2O -c scene.json -g false` does: the
+//     time window, the frame cadence and SceneLoader::writeFluid's files
+//     (<scene>_NNNN.json next to the scene) are the reference's own.
+//
 // Usage:
-//   sph2o_ref <scene.json> <nsteps> <outdir> [--init state.bin] [--nbr]
+//   sph2o_ref <scene.json> <nsteps> <outdir> [--init state.bin] [--nbr] [--frames]
 // state.bin: int32 N, then N*3 float32 positions, then N*3 float32 velocities.
 // Outputs are .npy files in <outdir> (see dump_* below).
 
@@ -101,10 +106,11 @@ int main(int argc, char** argv) {
     const int nsteps = std::atoi(argv[2]);
     const std::string outdir = argv[3];
     std::string initPath;
-    bool wantNbr = false;
+    bool wantNbr = false, frames = false;
     for (int a = 4; a < argc; ++a) {
         if (!std::strcmp(argv[a], "--init") && a + 1 < argc) initPath = argv[++a];
         else if (!std::strcmp(argv[a], "--nbr")) wantNbr = true;
+        else if (!std::strcmp(argv[a], "--frames")) frames = true;
     }
 
     SPHSimulation* sim = SPHSimulation::getCurrent();
@@ -167,8 +173,16 @@ int main(int argc, char** argv) {
     // nact: active particles after each step (emitters)
     std::vector<double> stats, nact;
     auto t0 = std::chrono::steady_clock::now();
+    // --frames: per step whether SPHSimulation::step() wrote a frame, and the frame counter after it
+    std::vector<double> saved;
+    if (frames) sim->activateSave(true);
     for (int s = 0; s < nsteps; ++s) {
-        sim->getSolver()->step();
+        if (frames) {
+            saved.push_back(sim->step() ? 1.0 : 0.0);
+            saved.push_back(static_cast<double>(sim->getFrame()));
+        } else {
+            sim->getSolver()->step();
+        }
         double its = 0, itsV = 0, eD = 0, eV = 0;
         if (method == SPHSimulation::DFSPH_METHOD) {
             auto* p = static_cast<DFPeek*>(static_cast<DFSPHSolver*>(sim->getSolver()));
@@ -217,6 +231,7 @@ int main(int argc, char** argv) {
     dump(outdir, "stats", "<f8", stats, 6);
     dump(outdir, "nact", "<f8", nact, 1);
     dump(outdir, "meta", "<f8", meta, 1);
+    if (frames) dump(outdir, "saved", "<f8", saved, 2);
 
     if (wantNbr) {
         // Rebuild the grid at the final positions with the reference's own

@@ -297,9 +297,11 @@ def reference_available():
     return os.path.exists(REF_BIN)
 
 
-def run_reference(scene, nsteps, x=None, v=None, neighbors=False, threads=1, timeout=600):
+def run_reference(scene, nsteps, x=None, v=None, neighbors=False, threads=1, timeout=600, frames=False):
     """Run the compiled reference on `scene` for `nsteps` steps. Returns a dict
-    of arrays (see oracle/ref_driver.cpp)."""
+    of arrays (see oracle/ref_driver.cpp); with frames=True the steps go
+    through SPHSimulation::step() with frame output on, and the dict also
+    holds "frames": {file name: parsed JSON} of the frames it wrote."""
     if not reference_available():
         raise RuntimeError("compiled reference not built (make -C oracle ref)")
     with tempfile.TemporaryDirectory() as td:
@@ -317,6 +319,8 @@ def run_reference(scene, nsteps, x=None, v=None, neighbors=False, threads=1, tim
             args += ["--init", ip]
         if neighbors:
             args.append("--nbr")
+        if frames:
+            args.append("--frames")
         env = dict(os.environ, OMP_NUM_THREADS=str(threads))
         r = subprocess.run(args, env=env, capture_output=True, text=True, timeout=timeout)
         if r.returncode != 0:
@@ -325,4 +329,10 @@ def run_reference(scene, nsteps, x=None, v=None, neighbors=False, threads=1, tim
         for fn in os.listdir(td):
             if fn.endswith(".npy"):
                 out[fn[:-4]] = np.load(os.path.join(td, fn))
+        if frames:
+            out["frames"] = {}
+            for fn in sorted(os.listdir(td)):
+                if fn.startswith("scene_") and fn.endswith(".json"):
+                    with open(os.path.join(td, fn)) as f:
+                        out["frames"][fn] = json.load(f)
         return out

@@ -13,7 +13,7 @@
 
 #include "../../include/sph2o.h"
 
-static void writeFrame(const std::string& scene, unsigned frame, int64_t n, double t, double dt,
+static void writeFrame(const std::string& scene, unsigned frame, int64_t n, int64_t active, double t, double dt,
                        const std::vector<float>& x, const std::vector<float>& v) {
     std::string num = std::to_string(frame);
     while (num.size() < 4) num = "0" + num;  // SceneLoader.h:20 MAX_DIGITS
@@ -29,7 +29,7 @@ static void writeFrame(const std::string& scene, unsigned frame, int64_t n, doub
         return;
     }
     std::fprintf(f, "{\"numParticles\":%lld,\"numActiveParticles\":%lld,\"time\":%.9g,\"timeStep\":%.9g,\"positions\":[",
-                 static_cast<long long>(n), static_cast<long long>(n), t, dt);
+                 static_cast<long long>(n), static_cast<long long>(active), t, dt);
     for (int64_t i = 0; i < n; ++i)
         std::fprintf(f, "%s[%.9g,%.9g,%.9g]", i ? "," : "", x[3 * i], x[3 * i + 1], x[3 * i + 2]);
     std::fprintf(f, "],\"velocities\":[");
@@ -73,7 +73,8 @@ int main(int argc, char** argv) {
     sph_get_params(ctx, sid, &p);
     const int64_t n = sph_num_fluid(ctx, sid);
     std::vector<float> x(3 * n), v(3 * n);
-    double t = 0, dt = 0, lastSave = 0;
+    double t = 0, dt = 0;
+    float lastSave = 0.0f;   // TimeHelper::lastTimeSave, a Real
     unsigned frame = 1;
     long done = 0;
     // SPHSimulation::step gate (SPHSimulation.cpp:163) and TimeHelper::hasToSave (TimeHelper.h:100-110)
@@ -85,11 +86,13 @@ int main(int argc, char** argv) {
         }
         ++done;
         sph_get_time(ctx, sid, &t, &dt);
-        if (t - lastSave >= 1.0 / p.fps) {
-            lastSave = static_cast<double>(frame) / p.fps;
+        // time - lastTimeSave in FP32 against 1.0 / fps in FP64, then
+        // lastTimeSave = (Real) frame / fps in FP32, as the reference rounds them
+        if (static_cast<double>(static_cast<float>(t) - lastSave) >= 1.0 / static_cast<double>(p.fps)) {
+            lastSave = static_cast<float>(frame) / p.fps;
             if (save) {
                 sph_get_fluid_state(ctx, sid, x.data(), v.data(), nullptr, nullptr);
-                writeFrame(scene, frame, n, t, dt, x, v);
+                writeFrame(scene, frame, n, sph_num_active(ctx, sid), t, dt, x, v);
             }
             ++frame;
         }

@@ -1514,6 +1514,9 @@ class Engine {
         put(hkv.data(), NP * sizeof(float));
         if (!f) throw SphError{SPH_E_ARG, std::string("cannot write ") + path};
     }
+    // The whole file is parsed into temporaries and validated before any of
+    // the context changes: a truncated or inconsistent file leaves it as it
+    // was (same host and device state, same next step).
     void loadState(const char* path) {
         if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
         if (!path) throw SphError{SPH_E_ARG, "path is NULL"};
@@ -1523,6 +1526,7 @@ class Engine {
             f.read(static_cast<char*>(q), static_cast<std::streamsize>(n));
             if (!f) throw SphError{SPH_E_ARG, "truncated state file"};
         };
+        auto bad = [](const char* what) { return SphError{SPH_E_ARG, std::string("invalid state file: ") + what}; };
         char magic[8];
         uint32_t ns = 0;
         int32_t m = 0;
@@ -1534,48 +1538,80 @@ class Engine {
         get(&np, 8);
         if (ns != scenes.size() || m != method || np != NP)
             throw SphError{SPH_E_ARG, "state file does not match the context's scenes"};
-        HIPCHK(hipStreamSynchronize(stream));
-        bool layout = false;
-        std::vector<SceneState> st(scenes.size());
+        struct SceneLoad {
+            uint32_t na = 0;
+            GridSave g{};
+            float t = 0, dt = 0;
+            int32_t its = 0, itsV = 0;
+            SceneState st{};
+            std::vector<std::pair<int64_t, float>> emit;
+        };
+        std::vector<SceneLoad> L(scenes.size());
         for (size_t i = 0; i < scenes.size(); ++i) {
-            SceneH& sc = scenes[i];
+            const SceneH& sc = scenes[i];
+            SceneLoad& q = L[i];
             uint32_t counts[4];
             get(counts, sizeof(counts));
             if (counts[0] != sc.nf || counts[2] != sc.nb || counts[3] != sc.emitters.size() || counts[1] > sc.nf)
                 throw SphError{SPH_E_ARG, "state file does not match the context's scenes"};
+            q.na = counts[1];
             int32_t grid[7];
             uint32_t cells[3];
             get(grid, sizeof(grid));
             get(cells, sizeof(cells));
-            GridSave cur, g;
-            std::copy(sc.g0, sc.g0 + 3, cur.g0);
-            std::copy(sc.gn, sc.gn + 3, cur.gn);
-            cur.nCells = sc.nCells; cur.keyRange = sc.keyRange; cur.ovBuckets = sc.ovBuckets; cur.morton = sc.morton;
-            std::copy(grid, grid + 3, g.g0);
-            std::copy(grid + 3, grid + 6, g.gn);
-            g.morton = grid[6];
-            g.nCells = cells[0]; g.keyRange = cells[1]; g.ovBuckets = cells[2];
-            if (!(cur == g)) {
-                std::copy(g.g0, g.g0 + 3, sc.g0);
-                std::copy(g.gn, g.gn + 3, sc.gn);
-                sc.nCells = g.nCells; sc.keyRange = g.keyRange; sc.ovBuckets = g.ovBuckets; sc.morton = g.morton;
-                layout = true;
+            std::copy(grid, grid + 3, q.g.g0);
+            std::copy(grid + 3, grid + 6, q.g.gn);
+            q.g.morton = grid[6];
+            q.g.nCells = cells[0];
+            q.g.keyRange = cells[1];
+            q.g.ovBuckets = cells[2];
+            // the grid must be one setGrid / growBuckets could have made
+            uint64_t nc = 1;
+            int maxDim = 1;
+            for (int a = 0; a < 3; ++a) {
+                if (q.g.gn[a] < 1 || q.g.gn[a] > (1 << 20) || q.g.g0[a] < -1000000000 ||
+                    static_cast<int64_t>(q.g.g0[a]) + q.g.gn[a] > 1000000000)
+                    throw bad("grid extent");
+                nc *= static_cast<uint64_t>(q.g.gn[a]);
+                maxDim = std::max(maxDim, q.g.gn[a]);
             }
+            if (nc != q.g.nCells || nc > (static_cast<uint64_t>(1) << 28)) throw bad("cell count");
+            const uint32_t ob = q.g.ovBuckets;
+            if (ob == 0 || (ob & (ob - 1)) != 0 || ob > (1u << 24)) throw bad("overflow buckets");
+            int bits = 0;
+            while ((1 << bits) < maxDim) ++bits;
+            uint64_t keys;
+            if (q.g.morton == 1) {
+                if (bits > 10) throw bad("Morton grid");
+                keys = (static_cast<uint64_t>(1) << (3 * bits)) + 1;
+            } else if (q.g.morton == 0) {
+                keys = q.g.nCells + 1ull;
+            } else {
+                throw bad("key layout");
+            }
+            if (keys + ob != q.g.keyRange) throw bad("key range");
             float tt[2];
             int32_t its[2];
             get(tt, sizeof(tt));
             get(its, sizeof(its));
-            sc.t = tt[0];
-            sc.dt = tt[1];
-            sc.its = its[0];
-            sc.itsV = its[1];
-            get(&st[i], sizeof(SceneState));
-            for (auto& e : sc.emitters) {
-                get(&e.remaining, 8);
-                get(&e.next, 4);
+            q.t = tt[0];
+            q.dt = tt[1];
+            q.its = its[0];
+            q.itsV = its[1];
+            if (!std::isfinite(q.t) || !std::isfinite(q.dt) || !(q.dt > 0.0f)) throw bad("time");
+            get(&q.st, sizeof(SceneState));
+            if (!(q.st.dt > 0.0f) || !std::isfinite(q.st.t)) throw bad("scene state");
+            int64_t left = 0;
+            for (size_t e = 0; e < sc.emitters.size(); ++e) {
+                int64_t rem = 0;
+                float next = 0;
+                get(&rem, 8);
+                get(&next, 4);
+                if (rem < 0 || rem > sc.emitters[e].P.num_particles || !std::isfinite(next)) throw bad("emitter");
+                left += rem;
+                q.emit.emplace_back(rem, next);
             }
-            sc.na = counts[1];
-            sc.fEnd = sc.fBegin + sc.na;
+            if (static_cast<int64_t>(q.na) + left != static_cast<int64_t>(sc.nf)) throw bad("active count");
         }
         std::vector<float4> hx(NP), hv(NP);
         std::vector<uint32_t> hid(NP);
@@ -1585,6 +1621,47 @@ class Engine {
         get(hid.data(), NP * sizeof(uint32_t));
         get(hkw.data(), NP * sizeof(float));
         get(hkv.data(), NP * sizeof(float));
+        // every scene's active slots hold a permutation of its active ids
+        for (size_t i = 0; i < scenes.size(); ++i) {
+            const SceneH& sc = scenes[i];
+            std::vector<uint8_t> seen(L[i].na, 0);
+            for (uint32_t k = 0; k < L[i].na; ++k) {
+                const uint32_t o = hid[sc.fBegin + k];
+                if (o >= L[i].na || seen[o]) throw bad("particle ids");
+                seen[o] = 1;
+                const float4 q = hx[sc.fBegin + k];
+                if (!std::isfinite(q.x) || !std::isfinite(q.y) || !std::isfinite(q.z)) throw bad("positions");
+            }
+        }
+        // commit
+        HIPCHK(hipStreamSynchronize(stream));
+        bool layout = false;
+        for (size_t i = 0; i < scenes.size(); ++i) {
+            SceneH& sc = scenes[i];
+            const SceneLoad& q = L[i];
+            GridSave cur;
+            std::copy(sc.g0, sc.g0 + 3, cur.g0);
+            std::copy(sc.gn, sc.gn + 3, cur.gn);
+            cur.nCells = sc.nCells; cur.keyRange = sc.keyRange; cur.ovBuckets = sc.ovBuckets; cur.morton = sc.morton;
+            if (!(cur == q.g)) {
+                std::copy(q.g.g0, q.g.g0 + 3, sc.g0);
+                std::copy(q.g.gn, q.g.gn + 3, sc.gn);
+                sc.nCells = q.g.nCells; sc.keyRange = q.g.keyRange; sc.ovBuckets = q.g.ovBuckets;
+                sc.morton = q.g.morton;
+                layout = true;
+            }
+            sc.t = q.t;
+            sc.dt = q.dt;
+            sc.its = q.its;
+            sc.itsV = q.itsV;
+            for (size_t e = 0; e < sc.emitters.size(); ++e) {
+                sc.emitters[e].remaining = q.emit[e].first;
+                sc.emitters[e].next = q.emit[e].second;
+            }
+            sc.na = q.na;
+            sc.fEnd = sc.fBegin + sc.na;
+            hState[i] = q.st;
+        }
         if (layout) uploadLayout();
         else uploadConsts();
         rangesDirty = true;
@@ -1594,7 +1671,6 @@ class Engine {
         HIPCHK(hipMemcpy(id.p, hid.data(), NP * sizeof(uint32_t), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(kws.p, hkw.data(), NP * sizeof(float), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(kvws.p, hkv.data(), NP * sizeof(float), hipMemcpyHostToDevice));
-        std::copy(st.begin(), st.end(), hState);
         HIPCHK(hipMemcpy(dState.p, hState, scenes.size() * sizeof(SceneState), hipMemcpyHostToDevice));
         ++stateEpoch;
         if (method == SPH_DFSPH) {

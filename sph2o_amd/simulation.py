@@ -283,20 +283,50 @@ class SPHSimulation:
     def __init__(self, device=0):
         self.ctx = Context(device)
         self.scene = None
+        self.name = ""
         self.frame = 1
-        self.last_save = 0.0
+        self.last_save = np.float32(0.0)
+        self.active_save = False
 
     def importScene(self, path):
         try:
             self.scene = self.ctx.load_scene_json(path)
         except SPHError:
             return False
+        self.name = str(path)
         return True
+
+    def activateSave(self, active):
+        """Write a frame file at every frame step (SPHSimulation.h:131)."""
+        self.active_save = bool(active)
+
+    def getName(self):
+        return self.name
+
+    def writeFluid(self):
+        """SceneLoader::writeFluid (SceneLoader.cpp:211-275): <scene>_NNNN.json
+        (frame number zero-padded to 4 digits, SceneLoader.h:20) with
+        numParticles, numActiveParticles, time, timeStep and every particle's
+        position and velocity."""
+        import json
+        num = "%04d" % self.frame
+        path = self.name[:-len(".json")] + "_" + num + ".json" if self.name.endswith(".json") else \
+            self.name + "_" + num + ".json"
+        st = self.ctx.fluid_state(self.scene)
+        t, dt = self.ctx.time(self.scene)
+        state = {"numParticles": int(self.ctx.num_fluid(self.scene)),
+                 "numActiveParticles": int(self.ctx.num_active(self.scene)),
+                 "time": float(np.float32(t)), "timeStep": float(np.float32(dt)),
+                 "positions": st["x"].astype(np.float64).tolist(),
+                 "velocities": st["v"].astype(np.float64).tolist()}
+        with open(path, "w") as f:
+            json.dump(state, f)
+        return path
 
     def init(self):
         self.ctx.init()
         self.frame = 1
-        self.last_save = 0.0
+        self.last_save = np.float32(0.0)
 
     def getTime(self):
         return self.ctx.time(self.scene)[0]
@@ -325,8 +355,12 @@ class SPHSimulation:
         if np.float32(p.start_time) <= t <= np.float32(p.end_time):
             self.last_stats = self.ctx.step(1)[0][self.scene]
             t = np.float32(self.getTime())
-            if t - np.float32(self.last_save) >= 1.0 / np.float32(p.fps):
-                self.last_save = np.float32(self.frame) / np.float32(p.fps)
+            # TimeHelper::hasToSave: time - lastTimeSave in FP32 against 1.0 / fps
+            # in FP64; lastTimeSave = (Real) frame / fps in FP32
+            if float(np.float32(t - self.last_save)) >= 1.0 / float(np.float32(p.fps)):
+                self.last_save = np.float32(np.float32(self.frame) / np.float32(p.fps))
+                if self.active_save:
+                    self.writeFluid()
                 self.frame += 1
                 return True
         return False

@@ -30,13 +30,16 @@ Fixtures:
                          stats of 3 steps, 4,000 sampled particles after
                          steps 1 and 3
   ref_b64_smallest.npz   the smallest B64 scene alone: 1 step and 10 steps
+  ref_frames_wcsph.npz   frame output of SPHSimulation::step() (the WCSPH
+                         emitter scene at 1000 fps, 120 steps): the steps that
+                         wrote a frame and every frame file's name and fields
 
   ref_opt_*.npz          Akinci adhesion and cube-boundary scenes
                          (sph2o_amd.scenes.boundary_option_scenes): the state
                          after one step and after a short run
 
 ``python tests/golden/make_golden.py emitters`` (or ``options``, ``w8k``,
-``d1m``, ``b64``) writes only those files.
+``d1m``, ``b64``, ``frames``) writes only those files.
 """
 import json
 import os
@@ -258,6 +261,24 @@ def b64_smallest(nsteps=10):
          bx=a["bx"], x1=a["x"], v1=a["v"], stats1=a["stats"], x=b["x"], v=b["v"], rho=b["rho"], stats=b["stats"])
 
 
+def frame_output(nsteps=120):
+    """SPHSimulation::step() with frame output on (SceneLoader::writeFluid) for
+    the WCSPH emitter scene at 1000 frames per second: per step whether a
+    frame was written and the frame counter, and every frame's file name,
+    header fields, positions and velocities."""
+    sc = SC.emitter_scenes()["wcsph"]
+    sc["Configuration"]["fps"] = 1000.0
+    r = S.run_reference(sc, nsteps, frames=True)
+    names = sorted(r["frames"])
+    fr = [r["frames"][n] for n in names]
+    save("ref_frames_wcsph.npz", scene=scene_json(sc), nsteps=np.array(nsteps), saved=r["saved"],
+         names=np.array(names), num_particles=np.array([f["numParticles"] for f in fr]),
+         num_active=np.array([f["numActiveParticles"] for f in fr]),
+         time=np.array([f["time"] for f in fr]), time_step=np.array([f["timeStep"] for f in fr]),
+         positions=np.array([f["positions"] for f in fr], np.float32),
+         velocities=np.array([f["velocities"] for f in fr], np.float32))
+
+
 def main():
     if not S.reference_available():
         raise SystemExit("compiled reference missing: make -C oracle ref")
@@ -275,6 +296,9 @@ def main():
         return
     if sys.argv[1:] == ["b64"]:
         b64_smallest()
+        return
+    if sys.argv[1:] == ["frames"]:
+        frame_output()
         return
     cell_keys()
     support_pairs()
@@ -296,6 +320,7 @@ def main():
     w8k()
     d1m()
     b64_smallest()
+    frame_output()
 
 
 if __name__ == "__main__":

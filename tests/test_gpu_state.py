@@ -79,3 +79,42 @@ def test_gpu_state_file_must_match_the_scenes(tmp_path):
         f.write(b"NOTSTATE")
     with pytest.raises(_lib.SPHError):
         a.load_state(path)
+
+
+def test_gpu_rejected_state_file_leaves_the_context_unchanged(tmp_path):
+    """A truncated file, a duplicated particle id and an impossible grid are
+    all refused before anything is applied: the context then steps exactly
+    as an untouched twin does."""
+    tmp = str(tmp_path)
+    scenes = CASES["dfsph"]
+    a, _ = _make(tmp, scenes)
+    a.step(5)
+    good = os.path.join(tmp, "state.bin")
+    a.save_state(good)
+    with open(good, "rb") as f:
+        data = bytearray(f.read())
+    NP = 1024   # 1,000 particles padded to whole 256-slot blocks
+    header, scene_rec = 24, 16 + 28 + 12 + 8 + 8 + 80   # counts, grid, cells, t/dt, its, SceneState
+    ids_at = header + scene_rec + 2 * NP * 16
+    dup = bytearray(data)
+    dup[ids_at + 4:ids_at + 8] = dup[ids_at:ids_at + 4]
+    grid = bytearray(data)
+    grid[header + 16 + 12:header + 16 + 16] = (0).to_bytes(4, "little")   # gn.x = 0
+    variants = {"truncated": data[:len(data) // 2], "duplicate_id": dup, "grid": grid}
+    b, ids_b = _make(tmp, scenes)
+    c, ids_c = _make(tmp, scenes)
+    b.step(3)
+    c.step(3)
+    for name, blob in variants.items():
+        path = os.path.join(tmp, name + ".bin")
+        with open(path, "wb") as f:
+            f.write(bytes(blob))
+        with pytest.raises(_lib.SPHError) as e:
+            b.load_state(path)
+        assert e.value.code == _lib.SPH_E_ARG, name
+    sb, sc_ = b.step(6), c.step(6)
+    for (xb, vb, rb, nb), (xc, vc, rc, nc) in zip(_snap(b, ids_b), _snap(c, ids_c)):
+        assert nb == nc and np.array_equal(xb, xc) and np.array_equal(vb, vc) and np.array_equal(rb, rc)
+    assert [(s[0].t, s[0].iterations, s[0].iterations_v) for s in sb] == \
+           [(s[0].t, s[0].iterations, s[0].iterations_v) for s in sc_]
+    b.load_state(good)   # the good file still loads afterwards
