@@ -135,8 +135,28 @@ def pair_throughput(ctx, sid, its, steps_per_s):
             "candidate_tests_per_step": cand, "candidate_tests_per_s": cand * steps_per_s}
 
 
+def host_info():
+    """CPU model, logical CPUs and the compiler that built the CPU baselines."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cc = subprocess.run(["g++", "--version"], capture_output=True, text=True, timeout=10).stdout.splitlines()[0]
+    except (OSError, IndexError, subprocess.SubprocessError):
+        cc = "unknown"
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "compiler": cc}
+
+
 def cpu_baseline(scene_path, nf, steps, threads):
-    """Time the compiled reference on the first `steps` steps of the workload."""
+    """Time the compiled reference (the original sources, built here by
+    oracle/Makefile with the shipped -O3 -ffast-math flags and OpenMP) on
+    steps 1..`steps` of the workload, step() calls only."""
     from oracle import sphref as S
 
     if not S.reference_available():
@@ -151,8 +171,30 @@ def cpu_baseline(scene_path, nf, steps, threads):
         meta = np.load(os.path.join(td, "meta.npy"))
     secs = float(meta[3])
     return {"value": nf * steps / secs, "unit": "particle-steps/s", "cores": threads, "kind": "reference",
-            "sample": "original SPH2O DFSPH step (OpenMP, -O3 -ffast-math) on the same 1M-particle scene, "
-                      "steps 1-%d, step() calls only (%.1f s)" % (steps, secs)}
+            "sample": "original SPH2O DFSPH step (%s, -O3 -ffast-math) on the same 1M-particle scene, "
+                      "steps 1-%d, step() calls only (%.1f s)"
+                      % ("OpenMP, %d threads" % threads if threads > 1 else "1 thread", steps, secs)}
+
+
+def cpu_port(scene_path, nf, steps, threads):
+    """The oracle's FP32 restatement of the reference step (oracle/sphref.hpp:
+    the same hash grid and pass sequence, -O2, OpenMP) on the same steps."""
+    from oracle import sphref as S
+    from sph2o_amd import _lib
+
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    p, x, xb = _lib.scene_from_json(scene_path)
+    with open(scene_path) as f:
+        sc = json.load(f)
+    o = S.Oracle(S.params_from_scene(sc), x, None, xb, bits=32)
+    o.init()
+    t0 = time.perf_counter()
+    o.step(steps)
+    secs = time.perf_counter() - t0
+    o.close()
+    return {"value": nf * steps / secs, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": "FP32 restatement (oracle/sphref.hpp) of the DFSPH step on the same 1M-particle scene, "
+                      "steps 1-%d (%.1f s)" % (steps, secs)}
 
 
 def adjoint_cost(steps, every, method, warmup=5):
@@ -193,20 +235,30 @@ def adjoint_cost(steps, every, method, warmup=5):
     xT = ctx.fluid_state(sid)["x"].astype(np.float64)
     y = SC.adjoint_target(x0)
     seed = (2.0 * (xT - y) / n).astype(np.float32)
+    from sph2o_amd._lib import SPH_E_NONFINITE, SPHError
+
     t0 = time.perf_counter()
-    g = ctx.adjoint({sid: (seed, None)})[sid]
+    try:
+        g = ctx.adjoint({sid: (seed, None)})[sid]
+    except SPHError as e:   # the sweep completed; its gradient is not finite
+        if e.code != SPH_E_NONFINITE:
+            raise
+        g = None
     t_adj = time.perf_counter() - t0
     ctx.close()
     name = "WCSPH" if method == 0 else "DFSPH 8+8 iterations"
-    finite = lambda q: float(q) if np.isfinite(q) else None   # strict JSON has no inf / NaN
-    gnorm = float(np.linalg.norm(g["v0"].astype(np.float64)))
+    finite = lambda q: float(q) if q is not None and np.isfinite(q) else None   # strict JSON has no inf / NaN
+    gnorm = float(np.linalg.norm(g["v0"].astype(np.float64))) if g is not None else None
     return {"workload": "A100k: %s, %d fluid particles, %d steps at dt=%g, checkpoint every %d"
                         % (name, n, steps, dt, every),
             "forward_s": t_fwd, "record_s": t_rec, "adjoint_s": t_adj, "record_over_forward": t_rec / t_fwd,
             "adjoint_over_forward": t_adj / t_fwd,
+            # the whole gradient: the recording forward plus the reverse sweep
+            "gradient_over_forward": (t_rec + t_adj) / t_fwd,
             "loss": float(np.mean(np.sum((xT - y) ** 2, axis=1))),
-            "gradient_finite": bool(np.isfinite(gnorm)),
-            "dL_dviscosity": finite(g["viscosity"]), "dL_dsurface_tension": finite(g["surface_tension"]),
+            "gradient_finite": g is not None and bool(np.isfinite(gnorm)),
+            "dL_dviscosity": finite(g["viscosity"] if g else None),
+            "dL_dsurface_tension": finite(g["surface_tension"] if g else None),
             "dL_dv0_norm": finite(gnorm)}
 
 
@@ -264,6 +316,7 @@ def main():
     ap.add_argument("--n", type=int, default=100, help="fluid lattice edge (100 -> 1M particles)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps-serial", type=int, default=2, help="steps of the 1-thread reference leg (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=20, help="B64 batched-gain steps (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=200, help="A100k adjoint-cost steps (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
@@ -341,10 +394,18 @@ def main():
                    "dfsph": adjoint_cost(args.adjoint_steps, 20, 2)}
 
     cpu = None
+    cpu_all = None
     if not args.no_cpu:
+        # the host CPUs one GPU's job may use: OMP_NUM_THREADS (16 per GPU on
+        # the MI355X hosts, whatever os.cpu_count() reports there)
         threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
-        log("cpu baseline: %d steps of the reference on %d threads" % (args.cpu_steps, threads))
+        log("cpu baseline: %d steps of the reference on %d threads, %d on 1 thread, %d of the FP32 port"
+            % (args.cpu_steps, threads, args.cpu_steps_serial, args.cpu_steps))
         cpu = cpu_baseline(scene_path, nf, args.cpu_steps, threads)
+        cpu_all = {"host": host_info(), "reference_all_threads": cpu,
+                   "reference_1_thread": cpu_baseline(scene_path, nf, args.cpu_steps_serial, 1)
+                   if args.cpu_steps_serial > 0 else None,
+                   "port_all_threads": cpu_port(scene_path, nf, args.cpu_steps, threads)}
 
     line = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -364,9 +425,11 @@ def main():
                      "avg_launch_ms": avg_ms, "launches": klaunch, "algorithmic_bytes_per_launch": bytes_per_launch},
         "batched_gain": batched["gain"] if batched else None,
         "batched": batched,
-        "adjoint_fwd_cost": max(a["adjoint_over_forward"] for a in adjoint.values()) if adjoint else None,
+        # (recording forward + reverse sweep) / plain forward, the larger of the two methods
+        "adjoint_fwd_cost": max(a["gradient_over_forward"] for a in adjoint.values()) if adjoint else None,
         "adjoint": adjoint,
         "cpu_baseline": cpu,
+        "cpu_baselines": cpu_all,
         "profiled_warmup": {"steps": prof_steps,
                             "kernel_ms": {k: round(v[0], 4) for k, v in sorted(ks.items(), key=lambda kv: -kv[1][0])},
                             "launches": {k: v[1] for k, v in ks.items()}},

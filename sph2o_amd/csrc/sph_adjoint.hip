@@ -73,17 +73,12 @@ __device__ __forceinline__ float cohesionDeriv(const SceneConst& S, float d, flo
 
 // Lanes per particle of the adjoint Jacobi passes (kAdjVelocity, kAdjSource):
 // as in the forward passes, the L lanes of a particle split its neighbour list
-// by slot residue and their sums are combined across lanes.
+// by slot residue and their sums are combined across lanes; each lane loads
+// kPadRows / L slots per round.
 #ifndef SPH_ADJ_LPP
 #define SPH_ADJ_LPP 4
 #endif
 constexpr uint32_t kAdjLpp = SPH_ADJ_LPP;
-// List slots a lane of kAdjVelocity / kAdjSource loads per round (their
-// gathers then overlap instead of waiting one slot at a time).
-#ifndef SPH_ADJ_BATCH
-#define SPH_ADJ_BATCH 2
-#endif
-constexpr uint32_t kAdjBatch = SPH_ADJ_BATCH;
 static_assert(kAdjLpp == 1 || kAdjLpp == 2 || kAdjLpp == 4 || kAdjLpp == 8, "SPH_ADJ_LPP must be 1, 2, 4 or 8");
 #define ADJ_PROLOGUE_LANES(L)                                                                      \
     const uint32_t lb = logicalBlock();                                                            \
@@ -382,39 +377,35 @@ __global__ void __launch_bounds__(kBlock) kAdjVelocity(AdjView a, int mode) {
     F3 gx{0.0f, 0.0f, 0.0f};
     float kb = 0.0f;
     const uint32_t nf = valid ? min(a.cntF[i], a.capF) : 0u;
-    // kAdjBatch slots per round: list entries, then k_j, then the open pairs'
-    // x_j and v-bar_j, then the sums in slot order
-    for (uint32_t s0 = sub; s0 < nf; s0 += kAdjBatch * kAdjLpp) {
-        uint32_t j[kAdjBatch];
-        float ks[kAdjBatch];
-        float4 xj[kAdjBatch], vj[kAdjBatch];
-        bool open[kAdjBatch];
-#pragma unroll
-        for (uint32_t q = 0; q < kAdjBatch; ++q) j[q] = a.nbrF[static_cast<size_t>(min(s0 + q * kAdjLpp, nf - 1u)) * NP + i];
-#pragma unroll
-        for (uint32_t q = 0; q < kAdjBatch; ++q) ks[q] = ki + a.kk[j[q]];
-#pragma unroll
-        for (uint32_t q = 0; q < kAdjBatch; ++q) {
-            open[q] = s0 + q * kAdjLpp < nf && fabsf(ks[q]) > S.eps;
-            if (open[q]) {
-                xj[q] = a.x[j[q]];
-                vj[q] = a.vBar[j[q]];
-            }
-        }
-#pragma unroll
-        for (uint32_t q = 0; q < kAdjBatch; ++q)
-            if (open[q]) {
-                const F3 r = xi - f3(xj[q]);
-                const float rl = fastSqrt(dot3(r, r));
-                const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
-                const F3 du = ui - f3(vj[q]);
-                const float rdu = dot3(r, du);
-                kb -= dt * S.m * G * rdu;
-                const float c = -dt * S.m * ks[q];
-                axpy(gx, c * G, du);
-                axpy(gx, c * G2 * rdu, r);
-            }
-    }
+    // whole rounds of the self-padded list (a self pair adds exactly zero):
+    // per slot k_j, x_j and v-bar_j at once, then the sums in slot order
+    struct KXV { float k; float4 x, v; };
+    const uint32_t bytes = static_cast<uint32_t>(NP * sizeof(float4));
+    const __amdgpu_buffer_rsrc_t rX = __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(a.x), 0, bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rV = __builtin_amdgcn_make_buffer_rsrc(a.vBar, 0, bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rK =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(a.kk), 0, bytes / 4u, 0x00020000);
+    walkRows<kAdjLpp, kPadRows / kAdjLpp, true>(
+        a.nbrF, a.NP, ii, sub, nf,
+        [&](uint32_t j) {
+            const uint32_t off = j * static_cast<uint32_t>(sizeof(float4));
+            return KXV{__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rK, j * 4u, 0, 0)),
+                       __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rX, off, 0, 0)),
+                       __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rV, off, 0, 0))};
+        },
+        [&](const KXV& q) {
+            const float ks = ki + q.k;
+            if (!(fabsf(ks) > S.eps)) return;
+            const F3 r = xi - f3(q.x);
+            const float rl = fastSqrt(dot3(r, r));
+            const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
+            const F3 du = ui - f3(q.v);
+            const float rdu = dot3(r, du);
+            kb -= dt * S.m * G * rdu;
+            const float c = -dt * S.m * ks;
+            axpy(gx, c * G, du);
+            axpy(gx, c * G2 * rdu, r);
+        });
     if (valid && S.akinci && fabsf(ki) > S.eps) {
         const uint32_t nb = min(a.cntB[i], a.capB);
         for (uint32_t s = sub; s < nb; s += kAdjLpp) {
@@ -466,36 +457,31 @@ __global__ void __launch_bounds__(kBlock) kAdjSource(AdjView a, int mode) {
     const float si = a.sBar[ii];
     F3 gx{0.0f, 0.0f, 0.0f}, gu{0.0f, 0.0f, 0.0f};
     const uint32_t nf = valid ? min(a.cntF[i], a.capF) : 0u;
-    for (uint32_t s0 = sub; s0 < nf; s0 += kAdjBatch * kAdjLpp) {   // as in kAdjVelocity
-        uint32_t j[kAdjBatch];
-        float ss[kAdjBatch];
-        float4 xj[kAdjBatch], uj[kAdjBatch];
-        bool open[kAdjBatch];
-#pragma unroll
-        for (uint32_t q = 0; q < kAdjBatch; ++q) j[q] = a.nbrF[static_cast<size_t>(min(s0 + q * kAdjLpp, nf - 1u)) * NP + i];
-#pragma unroll
-        for (uint32_t q = 0; q < kAdjBatch; ++q) ss[q] = si + a.sBar[j[q]];
-#pragma unroll
-        for (uint32_t q = 0; q < kAdjBatch; ++q) {
-            open[q] = s0 + q * kAdjLpp < nf && ss[q] != 0.0f;
-            if (open[q]) {
-                xj[q] = a.x[j[q]];
-                uj[q] = a.u[j[q]];
-            }
-        }
-#pragma unroll
-        for (uint32_t q = 0; q < kAdjBatch; ++q)
-            if (open[q]) {
-                const F3 r = xi - f3(xj[q]);
-                const float rl = fastSqrt(dot3(r, r));
-                const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
-                const F3 du = ui - f3(uj[q]);
-                const float w = c * S.m * ss[q];
-                axpy(gu, w * G, r);
-                axpy(gx, w * G, du);
-                axpy(gx, w * G2 * dot3(r, du), r);
-            }
-    }
+    struct SXU { float s; float4 x, u; };   // as in kAdjVelocity
+    const uint32_t bytes = static_cast<uint32_t>(NP * sizeof(float4));
+    const __amdgpu_buffer_rsrc_t rX = __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(a.x), 0, bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rU = __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(a.u), 0, bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rS = __builtin_amdgcn_make_buffer_rsrc(a.sBar, 0, bytes / 4u, 0x00020000);
+    walkRows<kAdjLpp, kPadRows / kAdjLpp, true>(
+        a.nbrF, a.NP, ii, sub, nf,
+        [&](uint32_t j) {
+            const uint32_t off = j * static_cast<uint32_t>(sizeof(float4));
+            return SXU{__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rS, j * 4u, 0, 0)),
+                       __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rX, off, 0, 0)),
+                       __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rU, off, 0, 0))};
+        },
+        [&](const SXU& q) {
+            const float ss = si + q.s;
+            if (ss == 0.0f) return;
+            const F3 r = xi - f3(q.x);
+            const float rl = fastSqrt(dot3(r, r));
+            const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
+            const F3 du = ui - f3(q.u);
+            const float w = c * S.m * ss;
+            axpy(gu, w * G, r);
+            axpy(gx, w * G, du);
+            axpy(gx, w * G2 * dot3(r, du), r);
+        });
     if (valid && S.akinci && si != 0.0f) {
         const uint32_t nb = min(a.cntB[i], a.capB);
         for (uint32_t s = sub; s < nb; s += kAdjLpp) {

@@ -10,6 +10,9 @@ constexpr int kBlock = 256;  // particles per workgroup (4 wave64)
 // of the 4-lanes-per-particle passes) stage the records of at most kTileMax
 // distinct neighbours in LDS
 constexpr uint32_t kTileGroup = 64;
+// Rows [n, n rounded up to kPadRows) of every particle's fluid list hold the
+// particle itself (a zero term in the Jacobi passes and their adjoints)
+constexpr uint32_t kPadRows = 8;
 constexpr uint32_t kTileMax = 512;
 
 // Per-scene constants, uploaded at init / regrid. One scene = one SPH2O

@@ -106,6 +106,54 @@ __device__ __forceinline__ float cohesionW(const SceneConst& S, float d, float r
     return 0.0f;
 }
 
+// Rows [n, n rounded up to kPadRows) of every fluid list hold the particle
+// itself (kBuildNeighbors writes them; tapes copy them), whose term in the
+// Jacobi passes and their adjoints is exactly zero (r = 0). A lane of those
+// passes therefore walks whole rounds of kPadRows = L x B rows with no
+// per-row test, and since every lane of a wave is at the same round, a row's
+// list address is a wave-uniform base plus the lane's fixed 32-bit offset
+// (buffer loads through a scalar descriptor: no vector arithmetic per row).
+// Pre: the next round's entries are loaded while this round's gathers are in
+// flight.
+template <uint32_t L, uint32_t B, bool Pre, class G, class C>
+__device__ __forceinline__ void walkRows(const uint32_t* list, uint32_t NP, uint32_t i, uint32_t sub, uint32_t n,
+                                         G&& gather, C&& consume) {
+    static_assert(L * B == kPadRows, "a round covers kPadRows rows");
+    const uint32_t rounds = (n + kPadRows - 1u) / kPadRows;
+    if (rounds == 0) return;
+    // rows r kPadRows + k L .. + L - 1 as one buffer (a scalar descriptor per
+    // row group; the lane reads at its fixed offset within it, < L NP 4 bytes)
+    const uint32_t vo = (sub * NP + i) * static_cast<uint32_t>(sizeof(uint32_t));
+    const uint32_t groupBytes = L * NP * static_cast<uint32_t>(sizeof(uint32_t));
+    auto load = [&](uint32_t r, uint32_t k) {
+        const uint32_t* rowK = list + (static_cast<size_t>(r) * kPadRows + k * L) * NP;
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(rowK), 0, groupBytes, 0x00020000);
+        return __builtin_amdgcn_raw_buffer_load_b32(rs, vo, 0, 0);
+    };
+    uint32_t j[B];
+#pragma unroll
+    for (uint32_t k = 0; k < B; ++k) j[k] = load(0, k);
+    for (uint32_t r = 0; r < rounds; ++r) {
+        decltype(gather(0u)) g[B];
+#pragma unroll
+        for (uint32_t k = 0; k < B; ++k) g[k] = gather(j[k]);
+        if constexpr (Pre) {
+            if (r + 1 < rounds) {
+#pragma unroll
+                for (uint32_t k = 0; k < B; ++k) j[k] = load(r + 1, k);
+            }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < B; ++k) consume(g[k]);
+        if constexpr (!Pre) {
+            if (r + 1 < rounds) {
+#pragma unroll
+                for (uint32_t k = 0; k < B; ++k) j[k] = load(r + 1, k);
+            }
+        }
+    }
+}
 template <bool MAX>
 __device__ __forceinline__ double combine(double a, double b) {
     return MAX ? fmax(a, b) : a + b;

@@ -116,14 +116,14 @@ enum KernelId {
     K_KEYS, K_SORT, K_REORDER, K_BUILD, K_DENSITY, K_FORCES, K_SURFACE_TENSION, K_VMAX, K_PREDICT, K_RHO_STAR,
     K_VELOCITY, K_DIVERGENCE, K_SCALE, K_INTEGRATE, K_PCI_INIT, K_PCI_DENSITY, K_PCI_PACC, K_PCI_INTEGRATE,
     K_FILL, K_FINISH, K_ADJ_SEED, K_ADJ_FORCES, K_ADJ_NORMALS, K_ADJ_DENSITY, K_ADJ_FINISH, K_ADJ_VELOCITY,
-    K_ADJ_SOURCE, K_CUBE, K_TILES, K_COUNT
+    K_ADJ_SOURCE, K_CUBE, K_TILES, K_TAPE, K_COUNT
 };
 const char* kKernelNames[K_COUNT] = {"keys", "sort", "reorder", "build_neighbors", "density", "forces",
                                      "surface_tension", "vmax", "predict_velocity", "rho_star", "velocity",
                                      "divergence", "scale", "integrate", "pci_init", "pci_density",
                                      "pci_pressure_acc", "pci_integrate", "fill_ranges", "finish",
                                      "adj_seed", "adj_forces", "adj_normals", "adj_density", "adj_finish",
-                                     "adj_velocity", "adj_source", "cube_boundary", "tile_lists"};
+                                     "adj_velocity", "adj_source", "cube_boundary", "tile_lists", "tape_copy"};
 
 // Host mirror of the reference's FP32 cubic-spline gradient (CubicSpline.h:52-74)
 // with the compiled reference's length; used for the PCISPH scaling factor.
@@ -748,6 +748,7 @@ class Engine {
         d.fRange = fRange.p; d.bRange = bRange.p; d.cellOf = cellOf.p;
         d.rankOf = rankOf.p; d.ordToCell = ordToCell.p; d.counts = counts.p; d.starts = starts.p; d.li = li.p;
         d.snapV = snapV.p; d.snapKvws = snapKvws.p;
+        d.vRec = nullptr;
         d.nbrF = nbrF.p; d.cntF = cntF.p; d.nbrB = nbrB.p; d.cntB = cntB.p;
         d.capF = capF; d.capB = capB;
         d.tileSlot = tiles ? tileSlot.p : nullptr;
@@ -1387,6 +1388,27 @@ class Engine {
     void copyD(void* dst, const void* src, size_t bytes) {
         if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream));
     }
+    // Tape copies gathered into batched launches (whole 16-byte words; every
+    // tape array is a multiple of 256 slots of 4- or 16-byte elements).
+    CopyBatch pendingCopies{};
+    void tapeCopy(void* dst, const void* src, size_t bytes) {
+        if (!bytes) return;
+        if ((bytes & 15u) || (reinterpret_cast<uintptr_t>(dst) & 15u) || (reinterpret_cast<uintptr_t>(src) & 15u)) {
+            copyD(dst, src, bytes);
+            return;
+        }
+        if (pendingCopies.n == kCopyBatchMax) flushCopies();
+        const int c = pendingCopies.n++;
+        pendingCopies.src[c] = static_cast<const uint4*>(src);
+        pendingCopies.dst[c] = static_cast<uint4*>(dst);
+        pendingCopies.words[c] = bytes / 16u;
+    }
+    void flushCopies() {
+        if (!pendingCopies.n) return;
+        const CopyBatch b = pendingCopies;
+        pendingCopies = CopyBatch{};
+        timed(K_TAPE, [&] { return launchCopyBatch(b, stream); });
+    }
     void saveCheckpoint(Checkpoint& c) {
         if (c.x.n != NP) {
             c.x.alloc(NP);
@@ -1708,6 +1730,7 @@ class Engine {
     // list blocks, keeping the steps already recorded (entry s of particle i
     // sits at s * NP + i within its step's block for any capacity).
     void tapeGrow(int recorded) {
+        flushCopies();   // no batched copy may target the blocks replaced here
         const size_t k = static_cast<size_t>(tape.steps), np = tape.np;
         auto widen = [&](DevBuf<uint32_t>& buf, uint32_t oldCap, uint32_t newCap) {
             if (newCap <= oldCap) return;
@@ -1726,24 +1749,52 @@ class Engine {
         tape.capF = std::max(tape.capF, capF);
         tape.capB = std::max(tape.capB, capB);
     }
-    // x, ids and lists of the current search into a half of step `slot`
+    // x, ids and lists of the current search into a half of step `slot`; of
+    // the lists only the rows some particle uses (the longest list of the
+    // search, from the flags the host read after it, rounded up to whole
+    // kPadRows rounds: the self-padding rows the adjoint Jacobi passes walk)
     void recordSearch(HalfTape& t, int slot) {
         const size_t np = NP, o = static_cast<size_t>(slot) * np;
-        copyD(t.x.p + o, x.p, np * sizeof(float4));
-        copyD(t.id.p + o, id.p, np * sizeof(uint32_t));
-        copyD(t.cntF.p + o, cntF.p, np * sizeof(uint32_t));
-        copyD(t.cntB.p + o, cntB.p, np * sizeof(uint32_t));
-        copyD(t.nbrF.p + static_cast<size_t>(slot) * tape.capF * np, nbrF.p, capF * np * sizeof(uint32_t));
-        copyD(t.nbrB.p + static_cast<size_t>(slot) * tape.capB * np, nbrB.p, capB * np * sizeof(uint32_t));
+        uint32_t rowsF = 0, rowsB = 0;
+        for (size_t si = 0; si < scenes.size(); ++si) {
+            rowsF = std::max(rowsF, static_cast<uint32_t>(std::max(0, hFlags[si * kFlagSlots + kFlagMaxFluidNbr])));
+            rowsB = std::max(rowsB, static_cast<uint32_t>(std::max(0, hFlags[si * kFlagSlots + kFlagMaxBoundaryNbr])));
+        }
+        rowsF = std::min((rowsF + kPadRows - 1u) / kPadRows * kPadRows, capF);   // with the self-padding rows
+        rowsB = std::min(rowsB, capB);
+        tapeCopy(t.x.p + o, x.p, np * sizeof(float4));
+        tapeCopy(t.id.p + o, id.p, np * sizeof(uint32_t));
+        tapeCopy(t.cntF.p + o, cntF.p, np * sizeof(uint32_t));
+        tapeCopy(t.cntB.p + o, cntB.p, np * sizeof(uint32_t));
+        tapeCopy(t.nbrF.p + static_cast<size_t>(slot) * tape.capF * np, nbrF.p, rowsF * np * sizeof(uint32_t));
+        tapeCopy(t.nbrB.p + static_cast<size_t>(slot) * tape.capB * np, nbrB.p, rowsB * np * sizeof(uint32_t));
     }
-    // k and rho* / divergence of a DFSPH pass (iteration `it`, warm = nit)
-    void recordPass(HalfTape& t, int slot, int it) {
-        const size_t np = NP, o = (static_cast<size_t>(slot) * (tape.nit + 1) + it) * np;
-        copyD(t.k.p + o, k.p, np * sizeof(float));
-        copyD(t.aux.p + o, aux.p, np * sizeof(float));
+    // A DFSPH pass of iteration `it` (warm start: nit) writes its k and
+    // rho* / divergence straight into the tape (no pass reads them back)
+    DevView passView(const DevView& d, HalfTape& t, int slot, int it) {
+        const size_t o = (static_cast<size_t>(slot) * (tape.nit + 1) + static_cast<size_t>(it)) * NP;
+        DevView q = d;
+        q.k = t.k.p + o;
+        q.aux = t.aux.p + o;
+        return q;
     }
-    void recordVelocity(HalfTape& t, int slot, int it) {
-        copyD(t.u.p + (static_cast<size_t>(slot) * tape.nit + it) * NP, v.p, NP * sizeof(float4));
+    // a velocity pass that also stores its result at `rec` (the velocity the
+    // next source pass reads)
+    static DevView velView(const DevView& d, float4* rec) {
+        DevView q = d;
+        q.vRec = rec;
+        return q;
+    }
+    float4* uSlot(HalfTape& t, int slot, int it) {
+        return t.u.p + (static_cast<size_t>(slot) * tape.nit + static_cast<size_t>(it)) * NP;
+    }
+    // The tape block holding the search fields (x, ids, counts, lists) of a
+    // half: a DFSPH step's first half runs on the search its predecessor's
+    // second half ran on, so for slot > 0 that is recorded once, in the
+    // second half of slot - 1 (with the densities and alpha it computed).
+    std::pair<const HalfTape*, size_t> searchHalf(int slot, int half) const {
+        if (method == SPH_DFSPH && half == 0 && slot > 0) return {&tape.h[1], static_cast<size_t>(slot - 1)};
+        return {&tape.h[half], static_cast<size_t>(slot)};
     }
     // stepWCSPH with the tape of step `slot` of the current segment.
     void stepWCSPHRecord(int slot) {
@@ -1754,13 +1805,14 @@ class Engine {
         HalfTape& t = tape.h[0];
         const size_t np = NP, o = static_cast<size_t>(slot) * np;
         recordSearch(t, slot);
-        copyD(t.v.p + o, v.p, np * sizeof(float4));
+        tapeCopy(t.v.p + o, v.p, np * sizeof(float4));
         timed(K_DENSITY, [&] { return launchDensityPressure(d, stream); });
         timed(K_FORCES, [&] { return launchForces(d, 1, stream); });
         if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
-        copyD(t.rho.p + o, rho.p, np * sizeof(float));
-        copyD(t.p.p + o, p.p, np * sizeof(float));
-        if (anyST()) copyD(t.n.p + o, n.p, np * sizeof(float4));
+        tapeCopy(t.rho.p + o, rho.p, np * sizeof(float));
+        tapeCopy(t.p.p + o, p.p, np * sizeof(float));
+        if (anyST()) tapeCopy(t.n.p + o, n.p, np * sizeof(float4));
+        flushCopies();   // before the integration moves x and v
         timed(K_INTEGRATE, [&] { return launchIntegrate(d, stream); });
         for (size_t s = 0; s < scenes.size(); ++s) hState[s].t = hState[s].t + hState[s].dt;
         fileStats(false);
@@ -1776,25 +1828,28 @@ class Engine {
         HalfTape& A = tape.h[0];
         HalfTape& B = tape.h[1];
         DevView d = view();
-        recordSearch(A, slot);
-        copyD(A.v.p + o, v.p, np * sizeof(float4));
-        copyD(A.rho.p + o, rho.p, np * sizeof(float));
-        copyD(A.alpha.p + o, alpha.p, np * sizeof(float));
+        // the first half's search fields, rho and alpha are the second half's
+        // of slot - 1 (searchHalf), and its start velocity was stored by the
+        // last velocity pass of slot - 1; only a segment's first step copies
+        if (slot == 0) {
+            recordSearch(A, slot);
+            tapeCopy(A.v.p + o, v.p, np * sizeof(float4));
+            tapeCopy(A.rho.p + o, rho.p, np * sizeof(float));
+            tapeCopy(A.alpha.p + o, alpha.p, np * sizeof(float));
+        }
         if (anyVisc() || anyST()) timed(K_FORCES, [&] { return launchForces(d, 0, stream); });
         if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 1, stream); });
         else timed(K_VMAX, [&] { return launchVmax(d, stream); });
         finish(d, kFinishTimeStep, 2);
-        if (anyST()) copyD(A.n.p + o, n.p, np * sizeof(float4));
+        if (anyST()) tapeCopy(A.n.p + o, n.p, np * sizeof(float4));
+        flushCopies();   // before predictVelocities changes v
         timed(K_PREDICT, [&] { return launchPredictVelocity(d, stream); });
-        timed(K_RHO_STAR, [&] { return launchRhoStar(d, 1, 2, stream); });
-        recordPass(A, slot, nit);
-        timed(K_VELOCITY, [&] { return launchVelocity(d, 2, -1, stream); });
+        timed(K_RHO_STAR, [&] { return launchRhoStar(passView(d, A, slot, nit), 1, 2, stream); });
+        timed(K_VELOCITY, [&] { return launchVelocity(velView(d, uSlot(A, slot, 0)), 2, -1, stream); });
         for (int it = 0; it < nit; ++it) {
-            recordVelocity(A, slot, it);
-            timed(K_RHO_STAR, [&] { return launchRhoStar(d, 0, it & 1, stream); });
+            timed(K_RHO_STAR, [&] { return launchRhoStar(passView(d, A, slot, it), 0, it & 1, stream); });
             if (!fuseFinish) finish(d, kFinishDensity, it & 1);
-            recordPass(A, slot, it);
-            velocityPass(d, it & 1, kFinishDensity);
+            velocityPass(velView(d, it + 1 < nit ? uSlot(A, slot, it + 1) : nullptr), it & 1, kFinishDensity);
         }
         readState();
         for (size_t s = 0; s < scenes.size(); ++s) scenes[s].its = hState[s].it;
@@ -1808,18 +1863,20 @@ class Engine {
         d = view();
         recordSearch(B, slot);
         timed(K_DENSITY, [&] { return launchDensityAlpha(d, 1, stream); });
-        copyD(B.rho.p + o, rho.p, np * sizeof(float));
-        copyD(B.alpha.p + o, alpha.p, np * sizeof(float));
-        timed(K_DIVERGENCE, [&] { return launchDivergence(d, 1, 2, stream); });
-        recordPass(B, slot, nit);
-        timed(K_VELOCITY, [&] { return launchVelocity(d, 2, -1, stream); });
+        tapeCopy(B.rho.p + o, rho.p, np * sizeof(float));
+        tapeCopy(B.alpha.p + o, alpha.p, np * sizeof(float));
+        timed(K_DIVERGENCE, [&] { return launchDivergence(passView(d, B, slot, nit), 1, 2, stream); });
+        timed(K_VELOCITY, [&] { return launchVelocity(velView(d, uSlot(B, slot, 0)), 2, -1, stream); });
+        // the last velocity pass leaves the next step's start velocity
+        float4* nextV = slot + 1 < tape.steps ? A.v.p + o + np : nullptr;
         for (int it = 0; it < nit; ++it) {
-            recordVelocity(B, slot, it);
-            timed(K_DIVERGENCE, [&] { return launchDivergence(d, 0, it & 1, stream); });
+            timed(K_DIVERGENCE, [&] { return launchDivergence(passView(d, B, slot, it), 0, it & 1, stream); });
             if (!fuseFinish) finish(d, kFinishDivergence, it & 1);
-            recordPass(B, slot, it);
-            velocityPass(d, it & 1, kFinishDivergence);
+            velocityPass(velView(d, it + 1 < nit ? uSlot(B, slot, it + 1) : nextV), it & 1, kFinishDivergence);
         }
+        // aux of the context is the last pass's (sph_get_field reads it)
+        tapeCopy(aux.p, B.aux.p + (o * (nit + 1) + static_cast<size_t>(nit - 1) * np), np * sizeof(float));
+        flushCopies();   // x, lists, rho and alpha stay until the next step's keys pass
         readState();
         for (size_t s = 0; s < scenes.size(); ++s) scenes[s].itsV = hState[s].it;
         timed(K_SCALE, [&] { return launchScaleDivergence(view(), stream); });
@@ -1920,17 +1977,20 @@ class Engine {
         a.NP = static_cast<uint32_t>(NP);
         const HalfTape& t = tape.h[half];
         const size_t o = static_cast<size_t>(slot) * NP;
-        a.x = t.x.p + o;
+        const auto sh = searchHalf(slot, half);
+        const HalfTape& ts = *sh.first;
+        const size_t os = sh.second * NP;
+        a.x = ts.x.p + os;
         a.v = t.v.p + o;
-        a.rho = t.rho.p + o;
+        a.rho = ts.rho.p + os;
         a.p = t.p.p + o;
         a.n = t.n.p + o;
-        a.alpha = t.alpha.p + o;
-        a.id = t.id.p + o;
-        a.cntF = t.cntF.p + o;
-        a.cntB = t.cntB.p + o;
-        a.nbrF = t.nbrF.p + static_cast<size_t>(slot) * tape.capF * NP;
-        a.nbrB = t.nbrB.p + static_cast<size_t>(slot) * tape.capB * NP;
+        a.alpha = ts.alpha.p + os;
+        a.id = ts.id.p + os;
+        a.cntF = ts.cntF.p + os;
+        a.cntB = ts.cntB.p + os;
+        a.nbrF = ts.nbrF.p + sh.second * tape.capF * NP;
+        a.nbrB = ts.nbrB.p + sh.second * tape.capB * NP;
         a.capF = tape.capF;
         a.capB = tape.capB;
         a.xb = xb.p;
@@ -2095,9 +2155,10 @@ class Engine {
         if (field <= 3 && (pass < 0 || pass > tape.nit)) throw SphError{SPH_E_ARG, "pass outside 0..iterations"};
         const HalfTape& t = tape.h[half];
         const size_t slot = static_cast<size_t>(step % rec.every);
-        auto ids = download(t.id.p + slot * NP + sc.fBegin, sc.nf);
+        const auto sh = searchHalf(static_cast<int>(slot), half);
+        auto ids = download(sh.first->id.p + sh.second * NP + sc.fBegin, sc.nf);
         if (field == 4) {
-            auto hx = download(t.x.p + slot * NP + sc.fBegin, sc.nf);
+            auto hx = download(sh.first->x.p + sh.second * NP + sc.fBegin, sc.nf);
             for (uint32_t i = 0; i < sc.nf; ++i) {
                 out[3 * ids[i]] = hx[i].x; out[3 * ids[i] + 1] = hx[i].y; out[3 * ids[i] + 2] = hx[i].z;
             }

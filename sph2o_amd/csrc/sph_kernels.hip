@@ -76,6 +76,7 @@ extern "C" __host__ int sphKernelsLanesPerParticle() { return static_cast<int>(k
 #endif
 extern "C" __host__ int sphKernelsFuseFinish() { return SPH_FUSE_FINISH; }
 
+static_assert(SPH_LIST_STAGE % kPadRows == 0, "the staged rows end on a round boundary");
 __device__ __forceinline__ uint32_t ldList(const uint32_t* p) {
 #if SPH_NT_LISTS
     return __builtin_nontemporal_load(p);
@@ -140,55 +141,6 @@ __device__ __forceinline__ void walkList(const uint32_t* list, size_t NP, uint32
             if (s0 + k * L < n) consume(j[k], g[k]);
 #pragma unroll
         for (uint32_t k = 0; k < B; ++k) j[k] = jn[k];
-    }
-}
-// Rows [n, n rounded up to kPadRows) of every fluid list hold the particle
-// itself (kBuildNeighbors writes them), whose term in the Jacobi passes is
-// exactly zero (r = 0). A lane of those passes therefore walks whole rounds
-// of kPadRows = L x B rows with no per-row test, and since every lane of a
-// wave is at the same round, a row's list address is a wave-uniform base
-// plus the lane's fixed 32-bit offset (global loads with a scalar base: no
-// vector arithmetic per row). Pre: the next round's entries are loaded while
-// this round's gathers are in flight.
-constexpr uint32_t kPadRows = 8;
-static_assert(SPH_LIST_STAGE % kPadRows == 0, "the staged rows end on a round boundary");
-template <uint32_t L, uint32_t B, bool Pre, class G, class C>
-__device__ __forceinline__ void walkRows(const uint32_t* list, uint32_t NP, uint32_t i, uint32_t sub, uint32_t n,
-                                         G&& gather, C&& consume) {
-    static_assert(L * B == kPadRows, "a round covers kPadRows rows");
-    const uint32_t rounds = (n + kPadRows - 1u) / kPadRows;
-    if (rounds == 0) return;
-    // rows r kPadRows + k L .. + L - 1 as one buffer (a scalar descriptor per
-    // row group; the lane reads at its fixed offset within it, < L NP 4 bytes)
-    const uint32_t vo = (sub * NP + i) * static_cast<uint32_t>(sizeof(uint32_t));
-    const uint32_t groupBytes = L * NP * static_cast<uint32_t>(sizeof(uint32_t));
-    auto load = [&](uint32_t r, uint32_t k) {
-        const uint32_t* rowK = list + (static_cast<size_t>(r) * kPadRows + k * L) * NP;
-        const __amdgpu_buffer_rsrc_t rs =
-            __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(rowK), 0, groupBytes, 0x00020000);
-        return __builtin_amdgcn_raw_buffer_load_b32(rs, vo, 0, 0);
-    };
-    uint32_t j[B];
-#pragma unroll
-    for (uint32_t k = 0; k < B; ++k) j[k] = load(0, k);
-    for (uint32_t r = 0; r < rounds; ++r) {
-        decltype(gather(0u)) g[B];
-#pragma unroll
-        for (uint32_t k = 0; k < B; ++k) g[k] = gather(j[k]);
-        if constexpr (Pre) {
-            if (r + 1 < rounds) {
-#pragma unroll
-                for (uint32_t k = 0; k < B; ++k) j[k] = load(r + 1, k);
-            }
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < B; ++k) consume(g[k]);
-        if constexpr (!Pre) {
-            if (r + 1 < rounds) {
-#pragma unroll
-                for (uint32_t k = 0; k < B; ++k) j[k] = load(r + 1, k);
-            }
-        }
     }
 }
 // The tile of launch block g (logical block = tile group: 64 particles of a
@@ -1398,6 +1350,7 @@ __device__ __forceinline__ void velocityUpdate(const DevView& d, const SceneCons
         vi.y -= dt * sy;
         vi.z -= dt * sz;
         VEL(d, i) = vi;
+        if (d.vRec) d.vRec[i] = vi;   // a recorded rollout's tape
     }
 }
 
@@ -1835,6 +1788,27 @@ hipError_t launchFinish(const DevView& d, int kind, int slot, hipStream_t st) {
     LAUNCH(kFinish, d.numScenes, d, kind, slot);
 }
 hipError_t launchCellsOut(const DevView& d, int32_t* cells, hipStream_t st) { LAUNCH(kCellsOut, d.numBlocks, d, cells); }
+// copy blockIdx.y of the batch, grid-strided over its words
+__global__ void __launch_bounds__(kBlock) kCopyBatch(CopyBatch b) {
+    const int c = static_cast<int>(blockIdx.y);
+    if (c >= b.n) return;
+    const uint4* src = b.src[c];
+    uint4* dst = b.dst[c];
+    const uint64_t n = b.words[c];
+    for (uint64_t w = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; w < n;
+         w += static_cast<uint64_t>(gridDim.x) * kBlock)
+        dst[w] = src[w];
+}
+hipError_t launchCopyBatch(const CopyBatch& b, hipStream_t st) {
+    if (b.n <= 0) return hipSuccess;
+    uint64_t most = 0;
+    for (int c = 0; c < b.n; ++c) most = b.words[c] > most ? b.words[c] : most;
+    if (most == 0) return hipSuccess;
+    const uint64_t want = (most + kBlock - 1) / kBlock;
+    const uint32_t gx = static_cast<uint32_t>(want < 2048 ? want : 2048);
+    hipLaunchKernelGGL(kCopyBatch, dim3(gx, static_cast<uint32_t>(b.n)), dim3(kBlock), 0, st, b);
+    return hipGetLastError();
+}
 hipError_t launchCellsOutOriginal(const DevView& d, int scene, uint32_t count, int32_t* cells, hipStream_t st) {
     const uint32_t g = (count + kBlock - 1) / kBlock;
     if (g == 0) return hipSuccess;

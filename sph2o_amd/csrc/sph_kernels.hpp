@@ -51,6 +51,8 @@ struct DevView {
     // DFSPH: copy of v and kv_ws as the reorder leaves them (undo of a divergence
     // solve run on lists that turn out invalid); null for other methods
     float4* snapV;
+    // recording (null otherwise): the velocity pass also stores its result here
+    float4* vRec;
     float* snapKvws;
     // neighbour lists, column-major: list[slot * NP + i]
     uint32_t* nbrF;
@@ -136,6 +138,16 @@ hipError_t launchCheckFinite(const DevView& d, hipStream_t st);
 // tells whether a launch took them. Per host thread; clear with nullptrs.
 void setLaunchEvents(hipEvent_t start, hipEvent_t stop);
 bool launchEventsUsed();
+// Up to kCopyBatchMax device-to-device copies of whole 16-byte words in one
+// launch (the tape of a recorded rollout takes several per step).
+constexpr int kCopyBatchMax = 8;
+struct CopyBatch {
+    const uint4* src[kCopyBatchMax];
+    uint4* dst[kCopyBatchMax];
+    uint64_t words[kCopyBatchMax];
+    int32_t n;
+};
+hipError_t launchCopyBatch(const CopyBatch& b, hipStream_t st);
 // count = the scene's active particles; pos (count float4) and speed (count
 // floats) are device buffers indexed by original id, either may be null
 hipError_t launchViewerFeed(const DevView& d, int scene, uint32_t count, float4* pos, float* speed, hipStream_t st);
