@@ -308,6 +308,56 @@ def batched_gain(steps, warmup=3):
             "mean_iterations_per_scene_step": {"batched": its_b, "sequential": float(np.mean(its_s))}}
 
 
+def b64_timed(steps):
+    """The B64 context alone: 3 warmup steps, a 0.5 s pause (a marker in a
+    kernel trace), then `steps` timed steps. Prints the wall time."""
+    from sph2o_amd import scenes as SC
+    from sph2o_amd.simulation import Context
+
+    td = tempfile.mkdtemp()
+    ctx = Context(0)
+    for k, sc in enumerate(SC.b64()):
+        p = os.path.join(td, "b%02d.json" % k)
+        SC.write_json(sc, p)
+        ctx.load_scene_json(p)
+    ctx.init()
+    ctx.step(3)
+    ctx.synchronize()
+    time.sleep(0.5)
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.synchronize()
+    t = time.perf_counter() - t0
+    ctx.close()
+    print(json.dumps({"steps": steps, "wall_s": t, "ms_per_step": 1000.0 * t / steps}), flush=True)
+
+
+def kernel_busy(trace_dir, steps):
+    """From a rocprofv3 --kernel-trace CSV of `--b64-steps`: the dispatches
+    after the marker pause, their span and the union of their intervals."""
+    import csv
+    import glob
+    from collections import Counter
+
+    rows = list(csv.DictReader(open(glob.glob(os.path.join(trace_dir, "*kernel_trace.csv"))[0])))
+    iv = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]) for r in rows)
+    gap, k = max((iv[q + 1][0] - iv[q][1], q) for q in range(len(iv) - 1))
+    seg = iv[k + 1:]
+    busy, (cs, ce) = 0, seg[0][:2]
+    for s_, e_, _ in seg[1:]:
+        if s_ > ce:
+            busy += ce - cs
+            cs, ce = s_, e_
+        else:
+            ce = max(ce, e_)
+    busy += ce - cs
+    span = seg[-1][1] - seg[0][0]
+    print(json.dumps({"marker_gap_ms": gap / 1e6, "dispatches": len(seg), "span_ms": span / 1e6,
+                      "kernel_busy_ms": busy / 1e6, "busy_fraction": busy / span,
+                      "dispatches_per_step": len(seg) / float(steps),
+                      "top": Counter(n.split("(")[0] for _, _, n in seg).most_common(8)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -321,7 +371,17 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200, help="A100k adjoint-cost steps (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="finish with a device-to-device copy of this size (HBM counter calibration under rocprofv3)")
+    ap.add_argument("--b64-steps", type=int, default=0,
+                    help="only time this many steps of the B64 context after a marker pause (for a kernel trace)")
+    ap.add_argument("--kernel-busy", metavar="TRACE_DIR",
+                    help="summarise a rocprofv3 kernel trace of --b64-steps (no GPU use)")
     args = ap.parse_args()
+    if args.kernel_busy:
+        kernel_busy(args.kernel_busy, args.b64_steps or 300)
+        return
+    if args.b64_steps > 0:
+        b64_timed(args.b64_steps)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
