@@ -229,4 +229,11 @@ int64_t sphref_sample_box(const float* mn, const float* mx, float radius, float*
     return static_cast<int64_t>(v.size() / 3);
 }
 
+int64_t sphref_sample_sphere(const float* c, float R, float radius, float* out, int64_t cap) {
+    std::vector<float> v;
+    sampleSphere(c, R, radius, v);
+    std::memcpy(out, v.data(), std::min<int64_t>(cap * 3, v.size()) * sizeof(float));
+    return static_cast<int64_t>(v.size() / 3);
+}
+
 }  // extern "C"

@@ -883,11 +883,13 @@ inline void sampleBlock(const float mn[3], const float mx[3], float radius, std:
             }
 }
 // Box surface: n = ceil(|max-min|/(2r)) per axis, d = l/n, emit (i,j,k)*d + min with an index on a face.
+// The shipped build forms l/(2r) (a vector over one scalar) as l * fl(1/(2r)).
 inline void sampleBox(const float mn[3], const float mx[3], float radius, std::vector<float>& out) {
+    const float inv = 1.0f / (2.0f * radius);
     float l[3], nn[3], d[3];
     for (int a = 0; a < 3; ++a) {
         l[a] = std::fabs(mn[a] - mx[a]);
-        nn[a] = std::ceil(l[a] / (2.0f * radius));
+        nn[a] = std::ceil(l[a] * inv);
         d[a] = l[a] / nn[a];
     }
     for (unsigned i = 0; i <= nn[0]; i++)
@@ -901,6 +903,35 @@ inline void sampleBox(const float mn[3], const float mx[3], float radius, std::v
                         out.push_back(a == 2 ? std::fma(t, d[a], mn[a]) : t * d[a] + mn[a]);
                     }
                 }
+}
+
+// Sphere surface (AkinciBoundaryObject.cpp:125-153): N = floor(4 pi R^2 / (pi r^2) 1.5)
+// points in m_theta = round(pi / sqrt(4 pi / N)) rings at theta = pi (m + 1/2) / m_theta,
+// ring m holding m_phi = round(2 pi sin(theta) / d_phi) points at phi = 2 pi n / m_phi.
+// Rounding of the shipped build: both divisions by the ring counts as products with
+// the count's float reciprocal, sin/cos in double on the float angles, x and y
+// placed with one FMA, z with a multiply and an add.
+inline void sampleSphere(const float c[3], float R, float radius, std::vector<float>& out) {
+    const unsigned N = static_cast<unsigned>(std::floor(4.0 * M_PI * R * R / (M_PI * radius * radius) * 1.5));
+    if (N == 0) return;
+    const float area = static_cast<float>(4.0 * M_PI) / static_cast<float>(N);
+    const unsigned rings = static_cast<unsigned>(std::round(M_PI / std::sqrt(area)));
+    if (rings == 0) return;
+    const float dPhi = area / (static_cast<float>(M_PI) / static_cast<float>(rings));
+    const float rRings = 1.0f / static_cast<float>(rings);
+    for (unsigned m = 0; m < rings; ++m) {
+        const float th = static_cast<float>(M_PI * (m + 0.5)) * rRings;
+        const double sth = std::sin(static_cast<double>(th));
+        const unsigned per = static_cast<unsigned>(std::round(2.0 * M_PI * sth / dPhi));
+        if (per == 0) continue;
+        const float rPer = 1.0f / static_cast<float>(per);
+        for (unsigned n = 0; n < per; ++n) {
+            const double ph = static_cast<double>((static_cast<float>(2.0 * M_PI) * static_cast<float>(n)) * rPer);
+            const float u[3] = {static_cast<float>(sth * std::cos(ph)), static_cast<float>(sth * std::sin(ph)),
+                                static_cast<float>(std::cos(static_cast<double>(th)))};
+            for (int a = 0; a < 3; ++a) out.push_back(a < 2 ? std::fma(R, u[a], c[a]) : R * u[a] + c[a]);
+        }
+    }
 }
 
 }  // namespace sphref

@@ -75,6 +75,9 @@ def lib():
                                           ctypes.c_int64]
         L.sphref_sample_box.restype = ctypes.c_int64
         L.sphref_sample_box.argtypes = L.sphref_sample_block.argtypes
+        L.sphref_sample_sphere.restype = ctypes.c_int64
+        L.sphref_sample_sphere.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_float, ctypes.c_float,
+                                           ctypes.POINTER(ctypes.c_float), ctypes.c_int64]
         L.sphref_set_state64.argtypes = [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_double)]
         L.sphref_cohesion_print.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64)]
         L.sphref_set_pressure_branch.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_uint8)]
@@ -160,6 +163,18 @@ def sample_box(mn, mx, radius):
         out = np.zeros((cap, 3), np.float32)
         n = lib().sphref_sample_box(_p(mn, ctypes.c_float), _p(mx, ctypes.c_float), np.float32(radius),
                                     _p(out, ctypes.c_float), cap)
+        if n <= cap:
+            return out[:n]
+        cap = int(n)
+
+
+def sample_sphere(centre, R, radius):
+    c = np.asarray(centre, np.float32)
+    cap = 1
+    while True:
+        out = np.zeros((cap, 3), np.float32)
+        n = lib().sphref_sample_sphere(_p(c, ctypes.c_float), np.float32(R), np.float32(radius),
+                                       _p(out, ctypes.c_float), cap)
         if n <= cap:
             return out[:n]
         cap = int(n)

@@ -288,6 +288,7 @@ int parseSceneJson(const std::string& text, SceneDesc& out, std::string& err) {
                     BlockDesc b;
                     const std::string w = bw + ".box[" + std::to_string(j) + "]";
                     if (!rd.vec3(&bx->arr[j], "min", b.min, w) || !rd.vec3(&bx->arr[j], "max", b.max, w)) { err = rd.err; return SPH_E_SCENE; }
+                    out.shapeOrder.push_back(static_cast<int>(out.boxes.size()));
                     out.boxes.push_back(b);
                     if (j == 0) out.cubes.push_back(b);   // the cube method keeps one box per object
                 }
@@ -298,6 +299,7 @@ int parseSceneJson(const std::string& text, SceneDesc& out, std::string& err) {
                     SphereDesc s;
                     const std::string w = bw + ".sphere[" + std::to_string(j) + "]";
                     if (!rd.vec3(&sp->arr[j], "pos", s.pos, w) || !rd.f32(&sp->arr[j], "radius", s.radius, w)) { err = rd.err; return SPH_E_SCENE; }
+                    out.shapeOrder.push_back(-1 - static_cast<int>(out.spheres.size()));
                     out.spheres.push_back(s);
                 }
             }
@@ -327,10 +329,15 @@ void sampleFluidBlock(const BlockDesc& b, float radius, std::vector<float>& xyz)
 }
 
 void sampleBoxSurface(const BlockDesc& b, float radius, std::vector<float>& xyz) {
+    // n = ceil(l / 2r) with the vector-by-scalar division done, as the shipped
+    // -O3 -ffast-math build does it, as l times one rounded reciprocal: for
+    // l = 0.06, r = 0.01 that is a tie that rounds to 3 (a true division gives
+    // 3.0000002, so 4)
+    const float inv2r = 1.0f / (2.0f * radius);
     float l[3], n[3], d[3];
     for (int a = 0; a < 3; ++a) {
         l[a] = std::fabs(b.min[a] - b.max[a]);
-        n[a] = std::ceil(l[a] / (2.0f * radius));
+        n[a] = std::ceil(l[a] * inv2r);
         d[a] = l[a] / n[a];
     }
     for (unsigned i = 0; i <= n[0]; i++)
@@ -346,26 +353,36 @@ void sampleBoxSurface(const BlockDesc& b, float radius, std::vector<float>& xyz)
 }
 
 void sampleSphereSurface(const SphereDesc& s, float radius, std::vector<float>& xyz) {
+    // Rounded as the shipped -O3 -ffast-math build rounds it (pinned point by
+    // point against its samples): the divisions of theta by m_theta and of
+    // phi by m_phi as products with the rounded reciprocal of the loop-invariant
+    // count; sin / cos in double on the float angles, the product rounded
+    // once to float; x and y placed with one FMA each, z with a multiply and
+    // an add.
     const double sep = 1.5;
     const unsigned numParts = static_cast<unsigned>(
         std::floor(4.0 * M_PI * s.radius * s.radius / (M_PI * radius * radius) * sep));
     if (numParts == 0) return;
-    const float a = static_cast<float>(4.0 * M_PI) / numParts;
+    const float a = static_cast<float>(4.0 * M_PI) / static_cast<float>(numParts);
     const float d = std::sqrt(a);
     const unsigned mTheta = static_cast<unsigned>(std::round(M_PI / d));
     if (mTheta == 0) return;
-    const float dTheta = static_cast<float>(M_PI) / mTheta;
+    const float dTheta = static_cast<float>(M_PI) / static_cast<float>(mTheta);
     const float dPhi = a / dTheta;
+    const float invMTheta = 1.0f / static_cast<float>(mTheta);
     for (unsigned m = 0; m < mTheta; ++m) {
-        const float theta = static_cast<float>(M_PI * (m + 0.5)) / mTheta;
-        const unsigned mPhi = static_cast<unsigned>(std::round(2.0 * M_PI * std::sin(theta) / dPhi));
+        const float theta = static_cast<float>(M_PI * (m + 0.5)) * invMTheta;
+        const double st = std::sin(static_cast<double>(theta)), ct = std::cos(static_cast<double>(theta));
+        const unsigned mPhi = static_cast<unsigned>(std::round(2.0 * M_PI * st / dPhi));
+        if (mPhi == 0) continue;
+        const float invMPhi = 1.0f / static_cast<float>(mPhi);
         for (unsigned n = 0; n < mPhi; ++n) {
-            const float phi = static_cast<float>(2.0 * M_PI) * n / mPhi;
-            const float x = std::sin(theta) * std::cos(phi);
-            const float y = std::sin(theta) * std::sin(phi);
-            const float z = std::cos(theta);
-            xyz.push_back(s.radius * x + s.pos[0]);
-            xyz.push_back(s.radius * y + s.pos[1]);
+            const float phi = (static_cast<float>(2.0 * M_PI) * static_cast<float>(n)) * invMPhi;
+            const float x = static_cast<float>(st * std::cos(static_cast<double>(phi)));
+            const float y = static_cast<float>(st * std::sin(static_cast<double>(phi)));
+            const float z = static_cast<float>(ct);
+            xyz.push_back(std::fma(s.radius, x, s.pos[0]));
+            xyz.push_back(std::fma(s.radius, y, s.pos[1]));
             xyz.push_back(s.radius * z + s.pos[2]);
         }
     }

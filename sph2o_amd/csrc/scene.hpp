@@ -20,6 +20,10 @@ struct SceneDesc {
     std::vector<BlockDesc> cubes;         // Boundary[i].box[0]: the cube objects of boundary method 0
                                           // (SPHSimulation.cpp:457-474)
     std::vector<SphereDesc> spheres;      // Boundary[*].sphere
+    // Akinci sampling order (SPHSimulation.cpp:408-424): object by object,
+    // each object's boxes, then its spheres; an entry >= 0 is boxes[e], one
+    // < 0 is spheres[-1 - e]
+    std::vector<int> shapeOrder;
     std::vector<sph_emitter_params> emitters;  // FluidModel[0].emitters
     int numFluidModels = 0;
     int numEmitters = 0;

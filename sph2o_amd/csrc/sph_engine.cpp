@@ -2499,8 +2499,10 @@ void readScene(const char* path, sph_scene_params& params, std::vector<float>& x
         throw SphError{SPH_E_UNSUPPORTED, "only Akinci and cube boundaries are supported"};
     for (auto& b : desc.fluidBlocks) sampleFluidBlock(b, desc.params.particle_radius, xf);
     if (desc.params.boundary_method == SPH_BOUNDARY_AKINCI) {
-        for (auto& b : desc.boxes) sampleBoxSurface(b, desc.params.particle_radius, xbv);
-        for (auto& s : desc.spheres) sampleSphereSurface(s, desc.params.particle_radius, xbv);
+        for (const int e : desc.shapeOrder) {
+            if (e >= 0) sampleBoxSurface(desc.boxes[e], desc.params.particle_radius, xbv);
+            else sampleSphereSurface(desc.spheres[-1 - e], desc.params.particle_radius, xbv);
+        }
     }
     params = desc.params;
     if (emitters) *emitters = desc.emitters;

@@ -115,6 +115,29 @@ def boundary_option_scenes():
     return out
 
 
+def sphere_scenes():
+    """Akinci sphere boundaries (AkinciBoundaryObject::sampleSphere): a 10^3
+    block at r = 0.01 starting 2 cm above a sphere of radius 5 cm, inside a
+    0.5 x 0.3 x 0.4 m box; a second boundary object holds a pillar box and a
+    sphere of radius 3 cm, so the sampling order (object by object, boxes
+    before spheres) is part of what is checked. WCSPH (XSPH 0.01) and DFSPH
+    (XSPH 0.01, surface tension 0.1)."""
+    out = {}
+    for tag, method in (("wcsph", 0), ("dfsph", 2)):
+        extra = dict(surface_tension_method=3, surface_tension=0.1) if method == 2 else {}
+        sc = small(method, viscosity_method=2, viscosity=0.01, **extra)
+        fb = sc["Fluid"]["FluidModel"][0]["fluidBlocks"][0]
+        fb["min"], fb["max"] = [0.1, 0.05, 0.14], [0.3, 0.25, 0.34]
+        sc["Boundary"] = [
+            {"box": [{"min": [0.0, 0.0, 0.0], "max": [0.5, 0.3, 0.4]}],
+             "sphere": [{"pos": [0.2, 0.15, 0.07], "radius": 0.05}], "geometry": []},
+            {"box": [{"min": [0.38, 0.1, 0.0], "max": [0.44, 0.2, 0.1]}],
+             "sphere": [{"pos": [0.41, 0.15, 0.2], "radius": 0.03}], "geometry": []},
+        ]
+        out[tag] = sc
+    return out
+
+
 def write_json(scene, path):
     with open(path, "w") as f:
         json.dump(scene, f)

@@ -34,12 +34,16 @@ Fixtures:
                          emitter scene at 1000 fps, 120 steps): the steps that
                          wrote a frame and every frame file's name and fields
 
+  ref_sphere_*.npz       Akinci sphere boundaries (two boundary objects, box
+                         and sphere each): sampled boundary, volumes, the
+                         state after one step and after a short run
+
   ref_opt_*.npz          Akinci adhesion and cube-boundary scenes
                          (sph2o_amd.scenes.boundary_option_scenes): the state
                          after one step and after a short run
 
 ``python tests/golden/make_golden.py emitters`` (or ``options``, ``w8k``,
-``d1m``, ``b64``, ``frames``) writes only those files.
+``d1m``, ``b64``, ``frames``, ``spheres``) writes only those files.
 """
 import json
 import os
@@ -208,6 +212,19 @@ def boundary_options():
              x1=a["x"], v1=a["v"], a1=a["a"], rho1=a["rho"], x=b["x"], v=b["v"], rho=b["rho"], stats=b["stats"])
 
 
+def spheres():
+    """Sphere-boundary scenes (sph2o_amd.scenes.sphere_scenes) from the
+    reference's own lattice: the sampled boundary (positions in object order,
+    Akinci volumes), the state after 1 step and after a run."""
+    plan = {"wcsph": 20, "dfsph": 10}
+    for tag, sc in SC.sphere_scenes().items():
+        a = S.run_reference(sc, 1)
+        b = S.run_reference(sc, plan[tag])
+        save("ref_sphere_%s.npz" % tag, scene=scene_json(sc), nsteps=np.array(plan[tag]), x0=a["x0"], bx=a["bx"],
+             bvol=a["bvol"], rho0=a["rho0"], x1=a["x"], v1=a["v"], rho1=a["rho"], x=b["x"], v=b["v"], rho=b["rho"],
+             stats=b["stats"])
+
+
 def _box_aggregates(x, v, rho, box):
     """inside count, kinetic sum |v|^2, centre of mass and mean density of the
     particles still inside the box."""
@@ -300,6 +317,9 @@ def main():
     if sys.argv[1:] == ["frames"]:
         frame_output()
         return
+    if sys.argv[1:] == ["spheres"]:
+        spheres()
+        return
     cell_keys()
     support_pairs()
     neighbors("lattice", 0.0)
@@ -321,6 +341,7 @@ def main():
     d1m()
     b64_smallest()
     frame_output()
+    spheres()
 
 
 if __name__ == "__main__":

@@ -71,6 +71,18 @@ def test_scene_reader_d1m_lattice_is_bit_exact():
     assert np.array_equal(xb[g["bidx"]], g["bx"])
 
 
+def test_scene_reader_sphere_boundaries_are_bit_exact(tmp_path):
+    # AkinciBoundaryObject::sampleSphere as the shipped build rounds it, and
+    # the boundary sampled object by object (box, sphere, box, sphere)
+    for tag in ("wcsph", "dfsph"):
+        g = golden("ref_sphere_%s.npz" % tag)
+        path = os.path.join(str(tmp_path), "sphere_%s.json" % tag)
+        SC.write_json(json.loads(str(g["scene"])), path)
+        p, x, xb = _lib.scene_from_json(path)
+        assert np.array_equal(x, g["x0"])
+        assert xb.shape == g["bx"].shape and np.array_equal(xb, g["bx"])
+
+
 def _write(tmp, sc):
     p = os.path.join(tmp, "scene.json")
     with open(p, "w") as f:

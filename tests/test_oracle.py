@@ -202,6 +202,46 @@ def test_sampling_matches_reference_example_scene():
     assert np.array_equal(S.sample_block(blk["min"], blk["max"], 0.005)[d["idx"]], d["x0"])
 
 
+def test_sampling_matches_reference_spheres_in_object_order():
+    # two boundary objects, each a box then a sphere (SPHSimulation.cpp:408-424):
+    # the reference's boundary is box 0, sphere 0, box 1, sphere 1, bit for bit
+    for tag in ("wcsph", "dfsph"):
+        g = golden("ref_sphere_%s.npz" % tag)
+        sc = json.loads(str(g["scene"]))
+        r = sc["Configuration"]["particleRadius"]
+        parts = []
+        for obj in sc["Boundary"]:
+            parts += [S.sample_box(b["min"], b["max"], r) for b in obj["box"]]
+            parts += [S.sample_sphere(s["pos"], s["radius"], r) for s in obj["sphere"]]
+        xb = np.concatenate(parts)
+        assert xb.shape == g["bx"].shape and np.array_equal(xb, g["bx"])
+    # the 6 cm pillar: ceil(0.06 * fl(1/0.02)) = 3 (a tie rounded to even; a true
+    # division gives 3.0000002, so 4), hence 4*6*6 - 2*4*4 = 112 surface points, not 132
+    assert len(S.sample_box((0.38, 0.1, 0.0), (0.44, 0.2, 0.1), 0.01)) == 112
+
+
+@pytest.mark.parametrize("R", [0.03, 0.05, 0.1, 0.25])
+def test_sphere_sampling_geometry(R):
+    # the points lie on the sphere, their count is the requested
+    # N = floor(4 R^2 / r^2 * 1.5) to within a few percent, the rings are
+    # mirrored about the equator and each ring is evenly spaced in phi
+    c = np.array([0.3, -0.2, 0.5], np.float32)
+    p = S.sample_sphere(c, R, 0.01).astype(np.float64)
+    d = p - c.astype(np.float64)
+    assert np.abs(np.linalg.norm(d, axis=1) - R).max() <= 4e-7 * (R + 0.6)
+    N = np.floor(4 * R * R / 1e-4 * 1.5)
+    assert abs(len(p) - N) <= max(3, 0.05 * N)
+    z = np.round(d[:, 2] / R, 5)
+    zs, counts = np.unique(z, return_counts=True)
+    assert np.allclose(zs, -zs[::-1], atol=2e-5) and np.array_equal(counts, counts[::-1])
+    for zr in zs:
+        ring = d[z == zr]
+        phi = np.sort(np.mod(np.arctan2(ring[:, 1], ring[:, 0]), 2 * np.pi))
+        assert abs(phi[0]) < 1e-6
+        if len(ring) > 1:
+            assert np.allclose(np.diff(phi), 2 * np.pi / len(ring), atol=1e-5)
+
+
 # ---------------------------------------------------------------- solver steps
 def _traj(tag):
     g = golden("ref_traj_%s.npz" % tag)
