@@ -99,6 +99,27 @@ def measured_traffic(kernel):
     return None
 
 
+# VALU issue peak of MI355X: 256 CUs, each issuing one wave64 VALU instruction
+# per cycle (4 SIMDs, 4 cycles per wave64 instruction), at 2.4 GHz
+VALU_PEAK_WAVE_INSTS = 256 * 2.4e9
+
+
+def measured_valu(kernel):
+    """VALU wave-instructions per launch of `kernel` on this workload, from the
+    committed rocprofv3 SQ_INSTS_VALU summary
+    (profiles/d1m_neighbour_pass_counters.md, lines `valu <kernel> <per_launch>`),
+    or None."""
+    path = os.path.join(ROOT, "profiles", "d1m_neighbour_pass_counters.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.strip().strip("`").split()
+            if len(parts) == 3 and parts[0] == "valu" and parts[1] == kernel:
+                return float(parts[2])
+    return None
+
+
 def pair_throughput(ctx, sid, its, steps_per_s):
     """True neighbour pairs evaluated per second (every pass's sum over i of
     |S_i|, fluid and boundary, at the mean list lengths of the final state)
@@ -438,6 +459,8 @@ def main():
     avg_ms = kms / klaunch if klaunch else float("nan")
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if klaunch else None
     traffic = measured_traffic(dominant)
+    valu = measured_valu(dominant)
+    valu_frac = valu / (avg_ms * 1e-3) / VALU_PEAK_WAVE_INSTS if (valu and klaunch) else None
 
     ctx.close()
     if args.calibrate_copy > 0:
@@ -482,7 +505,11 @@ def main():
         "sim_time_s": float(stats[-1][sid].t),
         "roofline": {"kernel": dominant, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
-                     "avg_launch_ms": avg_ms, "launches": klaunch, "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "avg_launch_ms": avg_ms, "launches": klaunch, "algorithmic_bytes_per_launch": bytes_per_launch,
+                     # SQ_INSTS_VALU per launch (counter run of this workload) over this run's launch time,
+                     # against the VALU issue peak
+                     "valu_issue": {"wave_insts_per_launch": valu, "peak_wave_insts_per_s": VALU_PEAK_WAVE_INSTS,
+                                    "frac": valu_frac}},
         "batched_gain": batched["gain"] if batched else None,
         "batched": batched,
         # (recording forward + reverse sweep) / plain forward, the larger of the two methods
