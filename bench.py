@@ -283,6 +283,98 @@ def adjoint_cost(steps, every, method, warmup=5):
             "dL_dv0_norm": finite(gnorm)}
 
 
+def ref_leg(sc, steps, threads, x=None, v=None, what="step"):
+    """The compiled reference on `steps` steps of a scene (step() calls only),
+    optionally from given positions / velocities, or None where it is absent."""
+    from oracle import sphref as S
+
+    if not S.reference_available():
+        return None
+    r = S.run_reference(sc, steps, x=x, v=v, threads=threads, timeout=900)
+    n = int(r["meta"][0])
+    secs = float(r["meta"][3])
+    return {"value": n * steps / secs, "unit": "particle-steps/s", "cores": threads, "kind": "reference",
+            "sample": "original SPH2O %s (%s, -O3 -ffast-math), steps 1-%d, step() calls only (%.1f s)"
+                      % (what, "OpenMP, %d threads" % threads if threads > 1 else "1 thread", steps, secs)}
+
+
+def port_leg(sc, steps, threads, x, v=None, xb=None):
+    """The oracle's FP32 restatement on the same steps."""
+    from oracle import sphref as S
+
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    o = S.Oracle(S.params_from_scene(sc), x, v, xb, bits=32)
+    o.init()
+    t0 = time.perf_counter()
+    o.step(steps)
+    secs = time.perf_counter() - t0
+    o.close()
+    return {"value": len(x) * steps / secs, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": "FP32 restatement (oracle/sphref.hpp), steps 1-%d (%.1f s)" % (steps, secs)}
+
+
+def gpu_leg(sc, x, v, windows):
+    """One context on `sc` from positions x (velocities v); times each
+    (first, last) step window in turn (the windows follow each other)."""
+    from sph2o_amd import _lib, scenes as SC
+    from sph2o_amd.simulation import Context
+
+    td = tempfile.mkdtemp()
+    path = os.path.join(td, "s.json")
+    SC.write_json(sc, path)
+    p, xs, xb = _lib.scene_from_json(path)
+    ctx = Context(0)
+    sid = ctx.add_scene(p, xs if x is None else x, v, xb)
+    ctx.init()
+    n = ctx.num_fluid(sid)
+    out = []
+    done = 0
+    for a, b in windows:
+        if a - 1 > done:
+            ctx.step(a - 1 - done)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        st = ctx.step(b - a + 1)
+        ctx.synchronize()
+        secs = time.perf_counter() - t0
+        done = b
+        its = float(np.mean([row[sid].iterations for row in st]))
+        out.append({"steps": "%d-%d" % (a, b), "value": n * (b - a + 1) / secs, "unit": "particle-steps/s",
+                    "ms_per_step": 1000.0 * secs / (b - a + 1), "iterations_per_step": its})
+    ctx.close()
+    return n, out
+
+
+def configs_table(threads):
+    """Every workload's GPU throughput next to the CPU legs that fit the
+    box's time budget (the figures of the measured-results table)."""
+    from sph2o_amd import scenes as SC
+
+    from sph2o_amd import _lib
+
+    rows = []
+    sc, x, _ = SC.w8k()
+    td = tempfile.mkdtemp()
+    path = os.path.join(td, "w8k.json")
+    SC.write_json(sc, path)
+    _, _, xb = _lib.scene_from_json(path)
+    log("configs: W8k GPU, 2,000 steps")
+    n, g = gpu_leg(sc, x, None, [(1, 2000)])
+    log("configs: W8k CPU legs")
+    rows.append({"workload": "W8k: WCSPH dam break, 8,000 particles, dt = 1e-4, 2,000 steps", "particles": n,
+                 "gpu": g, "cpu_reference_all_threads": ref_leg(sc, 200, threads, x=x, what="WCSPH step"),
+                 "cpu_reference_1_thread": ref_leg(sc, 200, 1, x=x, what="WCSPH step"),
+                 "cpu_port_all_threads": port_leg(sc, 200, threads, x, xb=xb)})
+    sc, x, v = SC.p4m()
+    log("configs: P4M GPU, steps 1-20 and 21-200")
+    n, g = gpu_leg(sc, x, v, [(1, 20), (21, 200)])
+    log("configs: P4M CPU leg")
+    rows.append({"workload": "P4M: PCISPH neighbour-search stress case, %d particles (jittered lattice + 1.5x "
+                             "cluster, Gaussian velocities)" % n, "particles": n, "gpu": g,
+                 "cpu_reference_all_threads": ref_leg(sc, 2, threads, x=x, v=v, what="PCISPH step")})
+    return rows
+
+
 def batched_gain(steps, warmup=3):
     """B64: 64 DFSPH scenes (2k..200k particles) advanced together in one
     context vs the same scenes as 64 sequential single-scene contexts."""
@@ -394,6 +486,8 @@ def main():
                     help="finish with a device-to-device copy of this size (HBM counter calibration under rocprofv3)")
     ap.add_argument("--b64-steps", type=int, default=0,
                     help="only time this many steps of the B64 context after a marker pause (for a kernel trace)")
+    ap.add_argument("--configs", action="store_true",
+                    help="print the W8k and P4M GPU / CPU figures instead of the bench line")
     ap.add_argument("--kernel-busy", metavar="TRACE_DIR",
                     help="summarise a rocprofv3 kernel trace of --b64-steps (no GPU use)")
     args = ap.parse_args()
@@ -402,6 +496,11 @@ def main():
         return
     if args.b64_steps > 0:
         b64_timed(args.b64_steps)
+        return
+    if args.configs:
+        threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+        for row in configs_table(threads):
+            print(json.dumps(row, allow_nan=False), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",

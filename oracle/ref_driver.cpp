@@ -17,7 +17,8 @@
 //     PCISPHSolver.cpp:374);
 //   * optionally replaces the lattice positions/velocities built by
 //     FluidObject::build with caller-supplied ones (importFluidState is dead
-//     code: it calls exit(0), SPHSimulation.cpp:548).
+//     code: it calls exit(0), SPHSimulation.cpp:548); a scene whose fluid
+//     blocks sample no particle takes the supplied state's particle count.
 //
 //   * with --frames, steps through SPHSimulation::step() itself with frame
 //     output on (activateSave), as `SPH2O -c scene.json -g false` does: the
@@ -34,6 +35,7 @@
 #include "SPH2O/SPHSolver/PCISPHSolver.h"
 #include "SPH2O/SPHSolver/WCSPHSolver.h"
 #include "SPH2O/BoundaryObject/AkinciBoundaryObject.h"
+#include "SPH2O/BoundaryObject/PCISPHBoundaryObject.h"
 #include "SPH2O/HashTable.h"
 
 #include <chrono>
@@ -61,6 +63,11 @@ struct PCIPeek : public PCISPHSolver {
     Real errD() const { return avgDensityError; }
     Real delta() const { return scalingFactor.empty() ? 0 : scalingFactor[0]; }
     void zeroUninit() { avgDensityError = 0; maxDensityError = 0; }
+};
+
+// Read-only peek at a PCISPH boundary object's per-particle normals.
+struct PBPeek : public PCISPHBoundaryObject {
+    const Vector3r& normal(uint i) const { return n[i]; }
 };
 
 void write_npy(const std::string& path, const char* descr, const std::vector<size_t>& shape,
@@ -120,12 +127,20 @@ int main(int argc, char** argv) {
         return 4;
     }
     FluidObject* f = sim->getFluidModel(0);
-    const uint N = f->getSize();
+    uint N = f->getSize();
 
     if (!initPath.empty()) {
         std::ifstream in(initPath, std::ios::binary);
         int32_t n = 0;
         in.read(reinterpret_cast<char*>(&n), 4);
+        if (N == 0 && n > 0) {
+            // a positions-only workload (its fluid block samples no particle):
+            // the fluid object takes the state's count; SPHSimulation::init
+            // then gives every particle the reference's mass
+            f->resize(static_cast<uint>(n));
+            f->setNumActiveParticles(static_cast<uint>(n));
+            N = static_cast<uint>(n);
+        }
         if (static_cast<uint>(n) != N) {
             std::fprintf(stderr, "init state has %d particles, scene has %u\n", n, N);
             return 5;
@@ -158,6 +173,22 @@ int main(int argc, char** argv) {
             }
         }
     }
+    // PCISPH boundary objects: positions and normals as sampled (object order)
+    std::vector<float> pbx, pbn;
+    auto pciBoundary = [&](std::vector<float>* dens) {
+        for (uint o = 0; o < sim->numberBoundaryModels(); ++o) {
+            auto* b = static_cast<PCISPHBoundaryObject*>(sim->getBoundaryModel(o));
+            for (uint i = 0; i < b->getSize(); ++i) {
+                if (dens) { dens->push_back(b->getDensity(i)); continue; }
+                const Vector3r& p = b->getPosition(i);
+                const Vector3r& q = static_cast<PBPeek*>(b)->normal(i);
+                pbx.insert(pbx.end(), {p.x, p.y, p.z});
+                pbn.insert(pbn.end(), {q.x, q.y, q.z});
+            }
+        }
+    };
+    const bool pciB = sim->getBoundaryHandlingMethod() == SPHSimulation::PCISPH_BOUNDARY_METHOD;
+    if (pciB) pciBoundary(nullptr);
     std::vector<float> x0;
     for (uint i = 0; i < N; ++i) {
         const Vector3r& p = f->getPosition(i);
@@ -232,6 +263,13 @@ int main(int argc, char** argv) {
     dump(outdir, "nact", "<f8", nact, 1);
     dump(outdir, "meta", "<f8", meta, 1);
     if (frames) dump(outdir, "saved", "<f8", saved, 2);
+    if (pciB) {
+        std::vector<float> pbrho;
+        pciBoundary(&pbrho);
+        dump(outdir, "pbx", "<f4", pbx, 3);
+        dump(outdir, "pbn", "<f4", pbn, 3);
+        dump(outdir, "pbrho", "<f4", pbrho, 1);
+    }
 
     if (wantNbr) {
         // Rebuild the grid at the final positions with the reference's own
