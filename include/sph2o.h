@@ -62,6 +62,7 @@ extern "C" {
 #define SPH_DFSPH 2
 #define SPH_BOUNDARY_NONE (-1)
 #define SPH_BOUNDARY_CUBE 0
+#define SPH_BOUNDARY_PCISPH 1
 #define SPH_BOUNDARY_AKINCI 2
 #define SPH_VISCOSITY_STANDARD 0
 #define SPH_VISCOSITY_ARTIFICIAL 1
@@ -72,7 +73,7 @@ extern "C" {
  * (SPHStructs.h:11-64) as the reference stores them (FP32). */
 typedef struct sph_scene_params {
     int32_t simulation_method;      /* SPH_WCSPH / SPH_PCISPH / SPH_DFSPH */
-    int32_t boundary_method;        /* SPH_BOUNDARY_AKINCI, SPH_BOUNDARY_CUBE or SPH_BOUNDARY_NONE */
+    int32_t boundary_method;        /* SPH_BOUNDARY_AKINCI, SPH_BOUNDARY_PCISPH, SPH_BOUNDARY_CUBE or SPH_BOUNDARY_NONE */
     int32_t viscosity_method;       /* -1 or SPH_VISCOSITY_* */
     int32_t surface_tension_method; /* -1 or SPH_SURFACE_TENSION_AKINCI */
     int32_t adhesion_method;        /* -1 or 4 (Akinci adhesion, AdhesionForce.cpp:5-32; acts with Akinci boundaries) */
@@ -157,6 +158,30 @@ int sph_add_emitter(sph_ctx* ctx, int scene, const sph_emitter_params* emitter);
  * (SceneLoader.cpp:168-172 reads them only for the PCISPH method). At most 4. */
 int sph_add_cube_boundary(sph_ctx* ctx, int scene, const float* min, const float* max, float normal_fct,
                           float tangential_fct);
+/* The PCISPH boundary object (boundary method SPH_BOUNDARY_PCISPH,
+ * PCISPHBoundaryObject.h:6-48) of a scene added by sph_add_scene, before
+ * sph_init: the normal of each of its nb boundary particles (n, N*3 float32,
+ * as PCISPHBoundaryObject::addCube / addSphere set them) and the object's
+ * normalFct / tangentialFct (SPHSimulation.cpp:428-455). Boundary particles
+ * get mass density0 (2r)^3, a density and a pressure, and push the fluid with
+ * the pressure term of each solver; after integration the fluid is moved out
+ * of the boundary's rest radius h/2 along the averaged normals and its
+ * velocity reflected (correctPositions / correctVelocities,
+ * PCISPHBoundaryObject.cpp:218-373), and PCISPH corrects its predicted
+ * positions each iteration (correctPredPositions, :375-447). WCSPH and PCISPH
+ * only (DFSPH: SPH_E_UNSUPPORTED); sph_load_scene_json sets this from the
+ * scene's Boundary entry. */
+int sph_set_boundary_normals(sph_ctx* ctx, int scene, int64_t nb, const float* normals, float normal_fct,
+                             float tangential_fct);
+/* The PCISPH boundary's normals and factors of a scene JSON without a context
+ * or a GPU (the companion of sph_scene_from_json, whose boundary positions
+ * they follow); SPH_E_CAPACITY if cap (in particles) is too small. */
+int sph_scene_boundary_normals_from_json(const char* path, int64_t* nb, float* normals, int64_t cap, float* normal_fct,
+                                         float* tangential_fct);
+/* Densities and pressures of a PCISPH boundary object's particles after the
+ * last step (PCISPHBoundaryObject::getDensity / getPressure), in the order of
+ * sph_get_boundary. */
+int sph_get_boundary_state(sph_ctx* ctx, int scene, float* density, float* pressure);
 int sph_num_scenes(const sph_ctx* ctx);
 int sph_get_params(const sph_ctx* ctx, int scene, sph_scene_params* out);
 /* All fluid particles of a scene, inactive emitter particles included

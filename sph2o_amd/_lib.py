@@ -32,6 +32,7 @@ EXPORTS = [
     "sph_set_coefficients", "sph_forward_record", "sph_adjoint_seed", "sph_adjoint_run", "sph_adjoint_result",
     "sph_recorded_field", "sph_add_emitter", "sph_num_active", "sph_save_state", "sph_load_state",
     "sph_add_cube_boundary", "sph_viewer_feed", "sph_dump_cells_noalloc",
+    "sph_set_boundary_normals", "sph_scene_boundary_normals_from_json", "sph_get_boundary_state",
 ]
 
 
@@ -108,6 +109,11 @@ def load():
         "sph_num_active": (ctypes.c_int64, [vp, ctypes.c_int]),
         "sph_save_state": (ctypes.c_int, [vp, ctypes.c_char_p]),
         "sph_add_cube_boundary": (ctypes.c_int, [vp, ctypes.c_int, fp, fp, ctypes.c_float, ctypes.c_float]),
+        "sph_set_boundary_normals": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int64, fp, ctypes.c_float,
+                                                    ctypes.c_float]),
+        "sph_scene_boundary_normals_from_json": (ctypes.c_int, [ctypes.c_char_p, P(ctypes.c_int64), fp, ctypes.c_int64,
+                                                                fp, fp]),
+        "sph_get_boundary_state": (ctypes.c_int, [vp, ctypes.c_int, fp, fp]),
         "sph_load_state": (ctypes.c_int, [vp, ctypes.c_char_p]),
         "sph_add_emitter": (ctypes.c_int, [vp, ctypes.c_int, P(EmitterParams)]),
         "sph_num_boundary": (ctypes.c_int64, [vp, ctypes.c_int]),
@@ -171,3 +177,23 @@ def scene_from_json(path):
     if rc != SPH_OK:
         raise SPHError(rc, "sph_scene_from_json failed for %s" % path)
     return p, x[: nf.value], xb[: nb.value]
+
+
+def boundary_normals_from_json(path):
+    """The PCISPH boundary's normals (Nb, 3) and (normalFct, tangFct) of a
+    scene JSON of boundary method 1, without a GPU."""
+    import numpy as np
+    L = load()
+    nb = ctypes.c_int64(0)
+    nf, tf = ctypes.c_float(0), ctypes.c_float(0)
+    fpp = ctypes.POINTER(ctypes.c_float)
+    rc = L.sph_scene_boundary_normals_from_json(str(path).encode(), ctypes.byref(nb), None, 0, ctypes.byref(nf),
+                                                ctypes.byref(tf))
+    if rc not in (SPH_OK, SPH_E_CAPACITY):
+        raise SPHError(rc, "sph_scene_boundary_normals_from_json failed for %s" % path)
+    out = np.zeros((max(1, nb.value), 3), np.float32)
+    rc = L.sph_scene_boundary_normals_from_json(str(path).encode(), ctypes.byref(nb), out.ctypes.data_as(fpp), len(out),
+                                                ctypes.byref(nf), ctypes.byref(tf))
+    if rc != SPH_OK:
+        raise SPHError(rc, "sph_scene_boundary_normals_from_json failed for %s" % path)
+    return out[: nb.value], (nf.value, tf.value)

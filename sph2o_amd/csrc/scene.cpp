@@ -279,15 +279,26 @@ int parseSceneJson(const std::string& text, SceneDesc& out, std::string& err) {
     // Boundary (SceneLoader.cpp:148-209); absent sphere/geometry arrays are empty.
     if (const JVal* B = root.find("Boundary")) {
         if (B->kind != JVal::Arr) { err = "Boundary must be an array"; return SPH_E_SCENE; }
+        const bool pci = P.boundary_method == 1;
+        out.numBoundaryObjects = static_cast<int>(B->arr.size());
         for (size_t i = 0; i < B->arr.size(); ++i) {
             const JVal& bo = B->arr[i];
             const std::string bw = "Boundary[" + std::to_string(i) + "]";
+            if (pci) {   // read for the PCISPH method only (SceneLoader.cpp:168-172)
+                float nf = 0.0f, tf = 0.0f;
+                if (!rd.f32(&bo, "normalFct", nf, bw) || !rd.f32(&bo, "tangFct", tf, bw)) { err = rd.err; return SPH_E_SCENE; }
+                out.normalFct.push_back(nf);
+                out.tangFct.push_back(tf);
+            }
             if (const JVal* bx = bo.find("box")) {
                 if (bx->kind != JVal::Arr) { err = bw + ".box must be an array"; return SPH_E_SCENE; }
                 for (size_t j = 0; j < bx->arr.size(); ++j) {
                     BlockDesc b;
                     const std::string w = bw + ".box[" + std::to_string(j) + "]";
                     if (!rd.vec3(&bx->arr[j], "min", b.min, w) || !rd.vec3(&bx->arr[j], "max", b.max, w)) { err = rd.err; return SPH_E_SCENE; }
+                    int32_t inv = 0;
+                    if (pci && !rd.i32(&bx->arr[j], "inverted", inv, w)) { err = rd.err; return SPH_E_SCENE; }
+                    out.boxInverted.push_back(inv != 0);
                     out.shapeOrder.push_back(static_cast<int>(out.boxes.size()));
                     out.boxes.push_back(b);
                     if (j == 0) out.cubes.push_back(b);   // the cube method keeps one box per object
@@ -299,6 +310,9 @@ int parseSceneJson(const std::string& text, SceneDesc& out, std::string& err) {
                     SphereDesc s;
                     const std::string w = bw + ".sphere[" + std::to_string(j) + "]";
                     if (!rd.vec3(&sp->arr[j], "pos", s.pos, w) || !rd.f32(&sp->arr[j], "radius", s.radius, w)) { err = rd.err; return SPH_E_SCENE; }
+                    int32_t inv = 0;
+                    if (pci && !rd.i32(&sp->arr[j], "inverted", inv, w)) { err = rd.err; return SPH_E_SCENE; }
+                    out.sphereInverted.push_back(inv != 0);
                     out.shapeOrder.push_back(-1 - static_cast<int>(out.spheres.size()));
                     out.spheres.push_back(s);
                 }
@@ -384,6 +398,84 @@ void sampleSphereSurface(const SphereDesc& s, float radius, std::vector<float>& 
             xyz.push_back(std::fma(s.radius, x, s.pos[0]));
             xyz.push_back(std::fma(s.radius, y, s.pos[1]));
             xyz.push_back(s.radius * z + s.pos[2]);
+        }
+    }
+}
+
+bool samplePciBox(const BlockDesc& b, float supportRadius, bool inverted, std::vector<float>& xyz,
+                  std::vector<float>& nrm) {
+    // l = abs(int(min - max)) as the shipped build resolves the unqualified abs
+    // (the int overload: the extent truncates to whole metres)
+    float l[3], d[3];
+    unsigned n[3];
+    const float spacing = 0.45f * supportRadius;
+    for (int a = 0; a < 3; ++a) {
+        const int t = static_cast<int>(b.min[a] - b.max[a]);
+        l[a] = static_cast<float>(t < 0 ? -t : t);
+        if (l[a] == 0.0f) return false;
+        n[a] = static_cast<unsigned>(std::ceil(l[a] / spacing));
+        d[a] = l[a] / static_cast<float>(n[a]);
+    }
+    // face normals (+x at i = 0, -x at i = n, +y at j = 0, -y at j = n, +z at
+    // k = 0, -z at k = n); an edge point takes normalize(0.5 (a + b)), a corner
+    // normalize((a + b + c) / 3), as the reference's table of 26 cases does
+    auto unit = [](float x, float y, float z, float* o) {
+        const float q = 1.0f / std::sqrt(x * x + y * y + z * z);
+        o[0] = x * q; o[1] = y * q; o[2] = z * q;
+    };
+    const float sgn = inverted ? 1.0f : -1.0f;
+    for (unsigned i = 0; i <= n[0]; i++)
+        for (unsigned j = 0; j <= n[1]; j++)
+            for (unsigned k = 0; k <= n[2]; k++) {
+                const float fx = i == 0 ? 1.0f : (i == n[0] ? -1.0f : 0.0f);
+                const float fy = j == 0 ? 1.0f : (j == n[1] ? -1.0f : 0.0f);
+                const float fz = k == 0 ? 1.0f : (k == n[2] ? -1.0f : 0.0f);
+                const int faces = (fx != 0.0f) + (fy != 0.0f) + (fz != 0.0f);
+                if (faces == 0) continue;
+                // (i dx, j dy, k dz) + min as the shipped -O3 build rounds it: z as one FMA
+                xyz.push_back(static_cast<float>(i) * d[0] + b.min[0]);
+                xyz.push_back(static_cast<float>(j) * d[1] + b.min[1]);
+                xyz.push_back(std::fma(static_cast<float>(k), d[2], b.min[2]));
+                float o[3] = {fx, fy, fz};
+                if (faces == 2) unit(fx * 0.5f, fy * 0.5f, fz * 0.5f, o);
+                else if (faces == 3) unit(fx / 3.0f, fy / 3.0f, fz / 3.0f, o);
+                for (int a = 0; a < 3; ++a) nrm.push_back(sgn * o[a]);
+            }
+    return true;
+}
+
+void samplePciSphere(const SphereDesc& s, float radius, bool inverted, std::vector<float>& xyz,
+                     std::vector<float>& nrm) {
+    // the count in FP32 (PCISPHBoundaryObject.cpp:192), then the points as
+    // sampleSphereSurface places them (the same rounding of the shipped build)
+    const float sep = 1.5f;
+    const float pi = static_cast<float>(M_PI);
+    const unsigned numParts = static_cast<unsigned>(
+        std::floor(static_cast<float>(4.0 * M_PI) * s.radius * s.radius / (pi * radius * radius) * sep));
+    if (numParts == 0) return;
+    const float a = static_cast<float>(4.0 * M_PI) / static_cast<float>(numParts);
+    const float d = std::sqrt(a);
+    const unsigned mTheta = static_cast<unsigned>(std::round(M_PI / d));
+    if (mTheta == 0) return;
+    const float dTheta = pi / static_cast<float>(mTheta);
+    const float dPhi = a / dTheta;
+    const float invMTheta = 1.0f / static_cast<float>(mTheta);
+    const float sgn = inverted ? 1.0f : -1.0f;
+    for (unsigned m = 0; m < mTheta; ++m) {
+        const float theta = static_cast<float>(M_PI * (m + 0.5)) * invMTheta;
+        const double st = std::sin(static_cast<double>(theta)), ct = std::cos(static_cast<double>(theta));
+        const int mPhi = static_cast<int>(std::round(2.0 * M_PI * st / dPhi));
+        if (mPhi <= 0) continue;
+        const float invMPhi = 1.0f / static_cast<float>(mPhi);
+        for (int n = 0; n < mPhi; ++n) {
+            const float phi = (static_cast<float>(2.0 * M_PI) * static_cast<float>(n)) * invMPhi;
+            const float p[3] = {std::fma(s.radius, static_cast<float>(st * std::cos(static_cast<double>(phi))), s.pos[0]),
+                                std::fma(s.radius, static_cast<float>(st * std::sin(static_cast<double>(phi))), s.pos[1]),
+                                s.radius * static_cast<float>(ct) + s.pos[2]};
+            for (int q = 0; q < 3; ++q) {
+                xyz.push_back(p[q]);
+                nrm.push_back(sgn * (s.pos[q] - p[q]));
+            }
         }
     }
 }

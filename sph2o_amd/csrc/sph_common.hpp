@@ -64,6 +64,12 @@ struct SceneConst {
     // Akinci adhesion to the boundary (AdhesionForce.cpp:5-32): on if adhOn
     int32_t adhOn;
     float adh;                 // beta
+    // PCISPH boundary object (PCISPHBoundaryObject.cpp): on if pciB; normalFct,
+    // tangentialFct, the rest radius h/2 of the collision corrections, Poly6's
+    // and Spiky's coefficients 315 / (64 pi h^9) and 45 / (pi h^6) and h^2, as
+    // Poly6.h / Spiky.h round them
+    int32_t pciB;
+    float nFct, tFct, restR, poly6C, spikyC, h2;
     // cube boundary objects (CubeBoundaryObject.cpp:10-88), applied in order
     int32_t nCubes;
     float cubeLo[kMaxCubes][3], cubeHi[kMaxCubes][3];

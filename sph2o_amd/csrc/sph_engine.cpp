@@ -116,14 +116,15 @@ enum KernelId {
     K_KEYS, K_SORT, K_REORDER, K_BUILD, K_DENSITY, K_FORCES, K_SURFACE_TENSION, K_VMAX, K_PREDICT, K_RHO_STAR,
     K_VELOCITY, K_DIVERGENCE, K_SCALE, K_INTEGRATE, K_PCI_INIT, K_PCI_DENSITY, K_PCI_PACC, K_PCI_INTEGRATE,
     K_FILL, K_FINISH, K_ADJ_SEED, K_ADJ_FORCES, K_ADJ_NORMALS, K_ADJ_DENSITY, K_ADJ_FINISH, K_ADJ_VELOCITY,
-    K_ADJ_SOURCE, K_CUBE, K_TILES, K_TAPE, K_COUNT
+    K_ADJ_SOURCE, K_CUBE, K_TILES, K_TAPE, K_BDENSITY, K_COUNT
 };
 const char* kKernelNames[K_COUNT] = {"keys", "sort", "reorder", "build_neighbors", "density", "forces",
                                      "surface_tension", "vmax", "predict_velocity", "rho_star", "velocity",
                                      "divergence", "scale", "integrate", "pci_init", "pci_density",
                                      "pci_pressure_acc", "pci_integrate", "fill_ranges", "finish",
                                      "adj_seed", "adj_forces", "adj_normals", "adj_density", "adj_finish",
-                                     "adj_velocity", "adj_source", "cube_boundary", "tile_lists", "tape_copy"};
+                                     "adj_velocity", "adj_source", "cube_boundary", "tile_lists", "tape_copy",
+                                     "boundary_density"};
 
 // Host mirror of the reference's FP32 cubic-spline gradient (CubicSpline.h:52-74)
 // with the compiled reference's length; used for the PCISPH scaling factor.
@@ -188,6 +189,14 @@ struct SceneH {
         float lo[3], hi[3], normal, tang;
     };
     std::vector<Cube> cubes;   // cube boundary objects (boundary method 0)
+    // PCISPH boundary object (boundary method 1): normals in original order
+    bool pciB = false, normalsSet = false;
+    std::vector<float> bn;
+    float nFct = 0.0f, tFct = 0.0f;
+    // the dense box the cell-first marks were last laid out for
+    bool markedLayout = false;
+    int markG0[3] = {0, 0, 0}, markGn[3] = {0, 0, 0};
+    uint32_t markBase = 0;
 };
 
 class Engine {
@@ -210,6 +219,12 @@ class Engine {
     DevBuf<SceneState> dState;   // scene states, then the flags (one readback copy)
     DevBuf<int32_t> dBlockScene;
     DevBuf<float4> x, xAlt, v, vAlt, a, n, xs, ap, xk, xb;
+    // PCISPH boundary objects: boundary normals, densities, pressures; predicted
+    // velocities and the last iteration's predictions (PCISPH); cell marks
+    DevBuf<float4> bn, vs, xsP, vsP;
+    DevBuf<float> bRho, bP;
+    DevBuf<uint32_t> cellFirst;
+    uint32_t stepNo = 0;   // steps taken since sph_init (the hash table's clock)
     DevBuf<float> rho, alpha, p, k, kws, kwsAlt, kvws, kvwsAlt, aux, vb;
     DevBuf<uint32_t> id, idAlt, keys, keysAlt, vals, valsAlt;
     DevBuf<uint2> fRange, bRange;
@@ -281,9 +296,11 @@ class Engine {
         if (nf > (1ll << 30)) throw SphError{SPH_E_ARG, "too many fluid particles"};
         if (P.simulation_method < 0 || P.simulation_method > 2)
             throw SphError{SPH_E_ARG, "simulation_method must be 0, 1 or 2"};
-        if (P.boundary_method != SPH_BOUNDARY_AKINCI && !(P.boundary_method == SPH_BOUNDARY_NONE) &&
-            !(nb == 0 && (P.boundary_method == 0)))
-            throw SphError{SPH_E_UNSUPPORTED, "only Akinci boundaries are supported"};
+        if (P.boundary_method != SPH_BOUNDARY_AKINCI && P.boundary_method != SPH_BOUNDARY_PCISPH &&
+            !(P.boundary_method == SPH_BOUNDARY_NONE) && !(nb == 0 && (P.boundary_method == 0)))
+            throw SphError{SPH_E_UNSUPPORTED, "only Akinci and PCISPH boundaries are supported"};
+        if (P.boundary_method == SPH_BOUNDARY_PCISPH && nb > 0 && P.simulation_method == SPH_DFSPH)
+            throw SphError{SPH_E_UNSUPPORTED, "PCISPH boundary objects are supported with WCSPH and PCISPH, not DFSPH"};
         if (P.viscosity_method > 2 || P.viscosity_method < -1)
             throw SphError{SPH_E_ARG, "viscosity_method must be -1, 0, 1 or 2"};
         if (!(P.particle_radius > 0.0f)) throw SphError{SPH_E_ARG, "particle_radius must be > 0"};
@@ -362,6 +379,37 @@ class Engine {
 
     // A CubeBoundaryObject (CubeBoundaryObject.cpp, SPHSimulation.cpp:457-474):
     // the fluid is clamped into [lo, hi] at the end of every step.
+    void setBoundaryNormals(int si, int64_t nb, const float* normals, float normalFct, float tangFct) {
+        if (initialized) throw SphError{SPH_E_STATE, "boundary normals must be set before sph_init"};
+        SceneH& s = scene(si);
+        if (s.P.boundary_method != SPH_BOUNDARY_PCISPH)
+            throw SphError{SPH_E_ARG, "boundary normals belong to boundary method SPH_BOUNDARY_PCISPH"};
+        if (nb != static_cast<int64_t>(s.nb) || (nb > 0 && !normals))
+            throw SphError{SPH_E_ARG, "one normal per boundary particle"};
+        if (!std::isfinite(normalFct) || !std::isfinite(tangFct)) throw SphError{SPH_E_NONFINITE, "non-finite factor"};
+        s.bn.assign(normals, normals + 3 * nb);
+        for (float q : s.bn)
+            if (!std::isfinite(q)) throw SphError{SPH_E_NONFINITE, "non-finite boundary normal"};
+        s.nFct = normalFct;
+        s.tFct = tangFct;
+        s.normalsSet = true;
+    }
+    void getBoundaryState(int si, float* density, float* pressure) {
+        if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
+        SceneH& s = scene(si);
+        if (!s.pciB) throw SphError{SPH_E_ARG, "the scene has no PCISPH boundary object"};
+        HIPCHK(hipStreamSynchronize(stream));
+        std::vector<float> r(s.nb), q(s.nb);
+        if (s.nb) {
+            HIPCHK(hipMemcpy(r.data(), bRho.p + s.bBegin, s.nb * sizeof(float), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(q.data(), bP.p + s.bBegin, s.nb * sizeof(float), hipMemcpyDeviceToHost));
+        }
+        for (uint32_t k = 0; k < s.nb; ++k) {
+            if (density) density[s.bOrder[k]] = r[k];
+            if (pressure) pressure[s.bOrder[k]] = q[k];
+        }
+    }
+
     void addCube(int si, const float lo[3], const float hi[3], float normalFct, float tangFct) {
         if (initialized) throw SphError{SPH_E_STATE, "cube boundaries must be added before sph_init"};
         SceneH& s = scene(si);
@@ -480,6 +528,7 @@ class Engine {
         s.viscOn = P.viscosity_method > -1 && P.viscosity != 0.0f;
         s.stOn = P.surface_tension_method == SPH_SURFACE_TENSION_AKINCI && P.surface_tension != 0.0f;
         s.akinci = P.boundary_method == SPH_BOUNDARY_AKINCI && s.nb > 0;
+        s.pciB = P.boundary_method == SPH_BOUNDARY_PCISPH && s.nb > 0;
         // FluidObject::setAdhesionForce creates the force for the Akinci method (4) only
         s.adhOn = P.adhesion_method == 4 && P.adhesion != 0.0f && s.akinci;
         if (P.simulation_method == SPH_PCISPH) {  // PCISPHSolver.cpp:403-436 with the initial dt
@@ -647,8 +696,21 @@ class Engine {
             c.visc = s.P.viscosity; c.bvisc = s.P.boundary_viscosity; c.st = s.P.surface_tension;
             c.viscMethod = s.viscOn ? s.P.viscosity_method : -1;
             c.stOn = s.stOn ? 1 : 0;
-            c.akinci = s.akinci ? 1 : 0;
+            // boundary particles in the densities and neighbour lists: Akinci ones
+            // and PCISPH boundary objects alike (their w carries the weight)
+            c.akinci = (s.akinci || s.pciB) ? 1 : 0;
             c.bviscOn = (s.viscOn && s.P.boundary_viscosity != 0.0f && s.akinci) ? 1 : 0;
+            c.pciB = s.pciB ? 1 : 0;
+            c.nFct = s.nFct;
+            c.tFct = s.tFct;
+            c.restR = 0.5f * s.h;   // PCISPHBoundaryObject.cpp:224
+            {   // Poly6.h:20-27, Spiky.h:20-27
+                const float h = s.h;
+                const float h6 = h * h * h * h * h * h, h9 = h6 * h * h * h;
+                c.poly6C = 315.0f / (static_cast<float>(64.0 * M_PI) * h9);
+                c.spikyC = 45.0f / (static_cast<float>(M_PI) * h6);
+                c.h2 = h * h;
+            }
             c.gamma = s.P.gamma;
             c.wcB = s.P.gamma != 0.0f ? s.P.stiffness * s.rho0 / s.P.gamma : 0.0f;
             c.pciDelta = s.pciDelta;
@@ -691,6 +753,7 @@ class Engine {
         // boundary grid: sort boundary particles of each scene by cell key on the host
         std::vector<uint2> br(totalCells ? totalCells : 1, make_uint2(0u, 0u));
         std::vector<float4> xbs(NB ? NB : 1);
+        std::vector<float4> bns(anyPciB() ? xbs.size() : 0);
         for (auto& s : scenes) {
             s.bOrder.resize(s.nb);
             std::vector<uint32_t> key(s.nb), lin(s.nb);
@@ -707,7 +770,10 @@ class Engine {
             for (uint32_t q = 0; q < s.nb; ++q) {
                 const uint32_t o = s.bOrder[q];
                 const uint32_t g = s.bBegin + q;
-                xbs[g] = make_float4(s.xb[3 * o], s.xb[3 * o + 1], s.xb[3 * o + 2], 0.0f);
+                // a PCISPH boundary particle weighs its mass density0 (2r)^3
+                // (SPHSimulation.cpp:434-436); Akinci ones get rho0 V_b below
+                xbs[g] = make_float4(s.xb[3 * o], s.xb[3 * o + 1], s.xb[3 * o + 2], s.pciB ? s.m : 0.0f);
+                if (s.pciB) bns[g] = make_float4(s.bn[3 * o], s.bn[3 * o + 1], s.bn[3 * o + 2], 0.0f);
                 uint2& r = br[s.cellBase + lin[o]];
                 if (r.y == 0) r.x = g;
                 r.y = g + 1;
@@ -718,6 +784,14 @@ class Engine {
         xb.alloc(xbs.size());
         HIPCHK(hipMemcpy(xb.p, xbs.data(), xbs.size() * sizeof(float4), hipMemcpyHostToDevice));
         vb.alloc(xbs.size());
+        if (anyPciB()) {
+            bn.alloc(bns.size());
+            HIPCHK(hipMemcpy(bn.p, bns.data(), bns.size() * sizeof(float4), hipMemcpyHostToDevice));
+            bRho.alloc(xbs.size());
+            bP.alloc(xbs.size());
+            HIPCHK(hipMemset(bP.p, 0, xbs.size() * sizeof(float)));
+            layoutCellMarks(br);
+        }
         fRange.alloc(totalCells ? totalCells : 1);
         rangesDirty = true;
         rangesFull = true;
@@ -725,6 +799,54 @@ class Engine {
         DevView d = view();
         for (size_t i = 0; i < scenes.size(); ++i)
             if (scenes[i].akinci) HIPCHK(launchBoundaryVolume(d, static_cast<int>(i), scenes[i].bBegin, scenes[i].bEnd, stream));
+    }
+
+    bool anyPciB() const {
+        for (auto& s : scenes)
+            if (s.pciB) return true;
+        return false;
+    }
+    // The cells the reference's hash table holds (a cell exists from the first
+    // search that inserted a particle into it, or from the search after a
+    // collision correction looked it up): per cell slot the first step from
+    // which it exists. Boundary cells exist from the start
+    // (SPHSolver::insertBoundaryParticles); on a new dense box the marks of
+    // the old one move to the same global cells; overflow buckets count as
+    // existing (no boundary particle is within their reach).
+    void layoutCellMarks(const std::vector<uint2>& br) {
+        std::vector<uint32_t> old;
+        if (cellFirst.n) {
+            old.resize(cellFirst.n);
+            HIPCHK(hipMemcpy(old.data(), cellFirst.p, old.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        }
+        std::vector<uint32_t> m(totalCells ? totalCells : 1, 0u);
+        for (auto& s : scenes) {
+            if (!s.pciB) continue;
+            for (uint32_t c = 0; c < s.nCells; ++c) m[s.cellBase + c] = br[s.cellBase + c].y > 0 ? 0u : ~0u;
+            if (s.markedLayout) {
+                const uint32_t oldCells = static_cast<uint32_t>(s.markGn[0]) * s.markGn[1] * s.markGn[2];
+                for (uint32_t c = 0; c < oldCells && s.markBase + c < old.size(); ++c) {
+                    const uint32_t f = old[s.markBase + c];
+                    if (f == ~0u) continue;
+                    const int l[3] = {static_cast<int>(c % s.markGn[0]), static_cast<int>((c / s.markGn[0]) % s.markGn[1]),
+                                      static_cast<int>(c / (static_cast<uint32_t>(s.markGn[0]) * s.markGn[1]))};
+                    int q[3];
+                    bool in = true;
+                    for (int a = 0; a < 3; ++a) {
+                        q[a] = l[a] + s.markG0[a] - s.g0[a];
+                        in = in && q[a] >= 0 && q[a] < s.gn[a];
+                    }
+                    if (!in) continue;
+                    uint32_t& t = m[s.cellBase + static_cast<uint32_t>((q[2] * s.gn[1] + q[1]) * s.gn[0] + q[0])];
+                    t = std::min(t, f);
+                }
+            }
+            s.markedLayout = true;
+            for (int a = 0; a < 3; ++a) { s.markG0[a] = s.g0[a]; s.markGn[a] = s.gn[a]; }
+            s.markBase = s.cellBase;
+        }
+        cellFirst.alloc(m.size());
+        HIPCHK(hipMemcpy(cellFirst.p, m.data(), m.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     }
 
     void allocNeighborLists() {
@@ -745,6 +867,9 @@ class Engine {
         d.rho = rho.p; d.alpha = alpha.p; d.p = p.p; d.k = k.p; d.kws = kws.p; d.kvws = kvws.p; d.aux = aux.p;
         d.id = id.p;
         d.xb = xb.p; d.vb = vb.p;
+        d.bn = bn.p; d.bRho = bRho.p; d.bP = bP.p; d.vs = vs.p; d.xsP = xsP.p; d.vsP = vsP.p;
+        d.cellFirst = cellFirst.p;
+        d.stepNo = stepNo;
         d.fRange = fRange.p; d.bRange = bRange.p; d.cellOf = cellOf.p;
         d.rankOf = rankOf.p; d.ordToCell = ordToCell.p; d.counts = counts.p; d.starts = starts.p; d.li = li.p;
         d.snapV = snapV.p; d.snapKvws = snapKvws.p;
@@ -762,6 +887,9 @@ class Engine {
     void init() {
         if (initialized) throw SphError{SPH_E_STATE, "sph_init called twice"};
         if (scenes.empty()) throw SphError{SPH_E_STATE, "no scene added"};
+        for (auto& s : scenes)
+            if (s.pciB && !s.normalsSet)
+                throw SphError{SPH_E_STATE, "a PCISPH boundary scene needs its normals (sph_set_boundary_normals)"};
         HIPCHK(hipSetDevice(device));
         if (!stream) {
             HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -826,6 +954,12 @@ class Engine {
         dBlockScene.alloc(bs.size());
         HIPCHK(hipMemcpy(dBlockScene.p, bs.data(), bs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         for (auto* b : {&x, &xAlt, &v, &vAlt, &a, &n, &xs, &ap, &xk}) b->alloc(np);
+        if (method == SPH_PCISPH && anyPciB())
+            for (auto* b : {&vs, &xsP, &vsP}) {
+                b->alloc(np);
+                HIPCHK(hipMemset(b->p, 0, np * sizeof(float4)));
+            }
+        stepNo = 0;
         fuseFinish = sphKernelsFuseFinish() != 0;
         for (auto* b : {&rho, &alpha, &p, &k, &kws, &kwsAlt, &kvws, &kvwsAlt, &aux}) b->alloc(np);
         for (auto* b : {&id, &idAlt, &keys, &keysAlt, &vals, &valsAlt, &cntF, &cntB, &cellOf, &li}) b->alloc(np);
@@ -1280,12 +1414,19 @@ class Engine {
         neighborSearch(false);
         DevView d = view();
         timed(K_DENSITY, [&] { return launchDensityPressure(d, stream); });
+        // boundary densities and pressures of PCISPH boundary objects (SPHSolver.cpp:91-123, WCSPHSolver.cpp:97-115)
+        for (size_t si = 0; si < scenes.size(); ++si)
+            if (scenes[si].pciB)
+                timed(K_BDENSITY, [&] {
+                    return launchBoundaryDensity(d, static_cast<int>(si), scenes[si].bBegin, scenes[si].bEnd, 0, 2, stream);
+                });
         timed(K_FORCES, [&] { return launchForces(d, 1, stream); });
         if (anyST() || anyAdh()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
-        timed(K_INTEGRATE, [&] { return launchIntegrate(d, stream); });
+        timed(K_INTEGRATE, [&] { return launchIntegrate(d, stream); });   // + PCISPH boundary corrections
         if (anyCube()) timed(K_CUBE, [&] { return launchCubeBoundary(d, stream); });   // WCSPHSolver.cpp:51-61
         emitAll(false);   // WCSPHSolver.cpp:75, before t += dt
         for (size_t s = 0; s < scenes.size(); ++s) hState[s].t = hState[s].t + hState[s].dt;
+        ++stepNo;
         fileStats(false);
     }
 
@@ -1297,12 +1438,21 @@ class Engine {
         timed(K_DENSITY, [&] { return launchDensityAlpha(d, 0, stream); });
         if (anyVisc() || anyST()) timed(K_FORCES, [&] { return launchForces(d, 0, stream); });
         if (anyST() || anyAdh()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
+        // initPressure zeroes the boundary pressures (PCISPHSolver.cpp:100-117)
+        if (anyPciB()) HIPCHK(hipMemsetAsync(bP.p, 0, bP.n * sizeof(float), stream));
         timed(K_PCI_INIT, [&] { return launchPciInit(d, stream); });
         int first = 1;
         for (auto& sc : scenes) first = std::max(first, std::max(static_cast<int>(sc.P.min_iterations), sc.its));
         runLoop(first, maxOver(&sph_scene_params::max_iterations),
                 [&](int slot) {
                     timed(K_PCI_DENSITY, [&] { return launchPciDensity(d, slot, stream); });
+                    // predictBoundaryDensities + the boundary part of updatePressure (PCISPHSolver.cpp:211-297)
+                    for (size_t si = 0; si < scenes.size(); ++si)
+                        if (scenes[si].pciB)
+                            timed(K_BDENSITY, [&] {
+                                return launchBoundaryDensity(d, static_cast<int>(si), scenes[si].bBegin, scenes[si].bEnd,
+                                                             1, slot, stream);
+                            });
                     if (!fuseFinish) finish(d, kFinishPciDensity, slot);
                 },
                 [&](int slot) {
@@ -1322,6 +1472,7 @@ class Engine {
             T.capped = (T.it >= c.maxIt && T.err > static_cast<double>(c.eta)) ? 1 : 0;
             T.t = T.t + T.dt;
         }
+        ++stepNo;
         fileStats(false);
     }
 
@@ -1493,6 +1644,8 @@ class Engine {
     // lists, densities and alphas as the saved run had).
     static constexpr char kStateMagic[8] = {'S', 'P', 'H', '2', 'O', 'S', 'T', '1'};
     void saveState(const char* path) {
+        if (anyPciB())   // the hash table's cell history (cellFirst) is not part of the file format
+            throw SphError{SPH_E_UNSUPPORTED, "state files of PCISPH boundary scenes are not supported"};
         if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
         if (!path) throw SphError{SPH_E_ARG, "path is NULL"};
         HIPCHK(hipStreamSynchronize(stream));
@@ -1540,6 +1693,7 @@ class Engine {
     // the context changes: a truncated or inconsistent file leaves it as it
     // was (same host and device state, same next step).
     void loadState(const char* path) {
+        if (anyPciB()) throw SphError{SPH_E_UNSUPPORTED, "state files of PCISPH boundary scenes are not supported"};
         if (!initialized) throw SphError{SPH_E_STATE, "sph_init not called"};
         if (!path) throw SphError{SPH_E_ARG, "path is NULL"};
         std::ifstream f(path, std::ios::binary);
@@ -1900,8 +2054,8 @@ class Engine {
         if (method == SPH_PCISPH) throw SphError{SPH_E_UNSUPPORTED, "gradients are available for WCSPH and DFSPH"};
         for (auto& sc : scenes) {
             if (!sc.emitters.empty()) throw SphError{SPH_E_UNSUPPORTED, "gradients of scenes with emitters"};
-            if (sc.adhOn || !sc.cubes.empty())
-                throw SphError{SPH_E_UNSUPPORTED, "gradients of scenes with adhesion or cube boundaries"};
+            if (sc.adhOn || !sc.cubes.empty() || sc.pciB)
+                throw SphError{SPH_E_UNSUPPORTED, "gradients of scenes with adhesion, cube or PCISPH boundaries"};
         }
         if (!NP) throw SphError{SPH_E_STATE, "no fluid particles"};
         int nit = 0;
@@ -2482,8 +2636,16 @@ int sph_add_scene(sph_ctx* ctx, const sph_scene_params* params, int64_t nf, cons
 namespace {
 // SPHSimulation::importScene (SPHSimulation.cpp:299-481) minus the object
 // factory: JSON -> parameters + sampled fluid and boundary particles.
+// The boundary of a scene of boundary method 1: per-particle normals and the
+// object's normalFct / tangFct (SPHSimulation.cpp:428-455).
+struct PciBoundaryDesc {
+    std::vector<float> normals;
+    float normalFct = 0.0f, tangFct = 0.0f;
+};
+
 void readScene(const char* path, sph_scene_params& params, std::vector<float>& xf, std::vector<float>& xbv,
-               std::vector<sph_emitter_params>* emitters = nullptr, std::vector<BlockDesc>* cubes = nullptr) {
+               std::vector<sph_emitter_params>* emitters = nullptr, std::vector<BlockDesc>* cubes = nullptr,
+               PciBoundaryDesc* pci = nullptr) {
     if (!path) throw SphError{SPH_E_ARG, "path is NULL"};
     std::ifstream f(path);
     if (!f) throw SphError{SPH_E_SCENE, std::string("cannot read ") + path};
@@ -2495,14 +2657,39 @@ void readScene(const char* path, sph_scene_params& params, std::vector<float>& x
     if (rc != SPH_OK) throw SphError{rc, err};
     if (desc.numGeometries > 0) throw SphError{SPH_E_UNSUPPORTED, "mesh (geometry) boundaries are not supported"};
     if (desc.params.boundary_method != SPH_BOUNDARY_AKINCI && desc.params.boundary_method != SPH_BOUNDARY_CUBE &&
-        (!desc.boxes.empty() || !desc.spheres.empty()))
-        throw SphError{SPH_E_UNSUPPORTED, "only Akinci and cube boundaries are supported"};
+        desc.params.boundary_method != SPH_BOUNDARY_PCISPH && (!desc.boxes.empty() || !desc.spheres.empty()))
+        throw SphError{SPH_E_UNSUPPORTED, "only Akinci, PCISPH and cube boundaries are supported"};
     for (auto& b : desc.fluidBlocks) sampleFluidBlock(b, desc.params.particle_radius, xf);
     if (desc.params.boundary_method == SPH_BOUNDARY_AKINCI) {
         for (const int e : desc.shapeOrder) {
             if (e >= 0) sampleBoxSurface(desc.boxes[e], desc.params.particle_radius, xbv);
             else sampleSphereSurface(desc.spheres[-1 - e], desc.params.particle_radius, xbv);
         }
+    }
+    if (desc.params.boundary_method == SPH_BOUNDARY_PCISPH) {
+        // PCISPHBoundaryObject::correctPositions walks the boundary neighbours of
+        // every object with its own object's arrays (forsame_boundary_neighbors,
+        // SPHSimulation.h:42-47), which is defined for one object only
+        if (desc.numBoundaryObjects > 1)
+            throw SphError{SPH_E_UNSUPPORTED, "the PCISPH boundary method is defined for one boundary object"};
+        PciBoundaryDesc pb;
+        if (desc.numBoundaryObjects == 1) {
+            pb.normalFct = desc.normalFct[0];
+            pb.tangFct = desc.tangFct[0];
+        }
+        const float supportRadius = 4.0f * desc.params.particle_radius;
+        for (const int e : desc.shapeOrder) {
+            if (e >= 0) {
+                if (!samplePciBox(desc.boxes[e], supportRadius, desc.boxInverted[e] != 0, xbv, pb.normals))
+                    throw SphError{SPH_E_UNSUPPORTED,
+                                   "PCISPH boundary box under 1 m on an axis: the reference's sampler (integer abs of the "
+                                   "extent, PCISPHBoundaryObject.cpp:62-64) samples one particle at 0/0 there"};
+            } else {
+                samplePciSphere(desc.spheres[-1 - e], desc.params.particle_radius, desc.sphereInverted[-1 - e] != 0, xbv,
+                                pb.normals);
+            }
+        }
+        if (pci) *pci = pb;
     }
     params = desc.params;
     if (emitters) *emitters = desc.emitters;
@@ -2520,9 +2707,13 @@ int sph_load_scene_json(sph_ctx* ctx, const char* path) {
         std::vector<float> xf, xbv;
         std::vector<sph_emitter_params> em;
         std::vector<BlockDesc> cubes;
-        readScene(path, params, xf, xbv, &em, &cubes);
+        PciBoundaryDesc pb;
+        readScene(path, params, xf, xbv, &em, &cubes, &pb);
         const int id = ctx->eng.addScene(params, static_cast<int64_t>(xf.size() / 3), xf.data(), nullptr,
                                          static_cast<int64_t>(xbv.size() / 3), xbv.data());
+        if (params.boundary_method == SPH_BOUNDARY_PCISPH)   // SPHSimulation.cpp:428-455
+            ctx->eng.setBoundaryNormals(id, static_cast<int64_t>(pb.normals.size() / 3), pb.normals.data(), pb.normalFct,
+                                        pb.tangFct);
         for (auto& e : em) ctx->eng.addEmitter(id, e);   // SPHSimulation.cpp:373-385
         for (auto& c : cubes) ctx->eng.addCube(id, c.min, c.max, 0.0f, 0.0f);   // SPHSimulation.cpp:457-474
         return id;
@@ -2543,6 +2734,42 @@ int sph_scene_from_json(const char* path, sph_scene_params* params, int64_t* nf,
         if (*nf > pos_cap || *nb > bpos_cap) return SPH_E_CAPACITY;
         if (pos && !xf.empty()) std::memcpy(pos, xf.data(), xf.size() * sizeof(float));
         if (bpos && !xbv.empty()) std::memcpy(bpos, xbv.data(), xbv.size() * sizeof(float));
+        return SPH_OK;
+    });
+}
+
+int sph_scene_boundary_normals_from_json(const char* path, int64_t* nb, float* normals, int64_t cap, float* normal_fct,
+                                         float* tangential_fct) {
+    sph_ctx dummy;
+    return guarded(&dummy, [&] {
+        if (!nb || !normal_fct || !tangential_fct) throw SphError{SPH_E_ARG, "NULL output"};
+        sph_scene_params params;
+        std::vector<float> xf, xbv;
+        std::vector<sph_emitter_params> em;
+        PciBoundaryDesc pb;
+        readScene(path, params, xf, xbv, &em, nullptr, &pb);
+        if (params.boundary_method != SPH_BOUNDARY_PCISPH)
+            throw SphError{SPH_E_ARG, "the scene's boundary method is not SPH_BOUNDARY_PCISPH"};
+        *nb = static_cast<int64_t>(pb.normals.size() / 3);
+        *normal_fct = pb.normalFct;
+        *tangential_fct = pb.tangFct;
+        if (*nb > cap) return SPH_E_CAPACITY;
+        if (normals && *nb) std::memcpy(normals, pb.normals.data(), pb.normals.size() * sizeof(float));
+        return SPH_OK;
+    });
+}
+
+int sph_set_boundary_normals(sph_ctx* ctx, int scene, int64_t nb, const float* normals, float normal_fct,
+                             float tangential_fct) {
+    return guarded(ctx, [&] {
+        ctx->eng.setBoundaryNormals(scene, nb, normals, normal_fct, tangential_fct);
+        return SPH_OK;
+    });
+}
+
+int sph_get_boundary_state(sph_ctx* ctx, int scene, float* density, float* pressure) {
+    return guarded(ctx, [&] {
+        ctx->eng.getBoundaryState(scene, density, pressure);
         return SPH_OK;
     });
 }

@@ -383,6 +383,9 @@ __global__ void __launch_bounds__(kBlock) kKeys(DevView d, int dfsphAdvance, int
         const uint32_t c = cellSlot(S, gx, gy, gz, bucket);
         key = bucket ? S.deadKey - (S.ovMask + 1) + (c - S.cellBase - S.nCells)
                      : cellKey(S, gx - S.g0x, gy - S.g0y, gz - S.g0z);
+        // the reference's hash table keeps every cell a particle was inserted in
+        // (HashTable::clear empties cells but never erases them)
+        if (S.pciB && !bucket && !bad && d.cellFirst[c] > d.stepNo) atomicMin(d.cellFirst + c, d.stepNo);
         if (counting) ord = S.ordBase + d.rankOf[c];
     }
     // particles in overflow buckets, one atomic per wave
@@ -848,6 +851,145 @@ __global__ void kBoundaryVolume(DevView d, int scene, uint32_t bBegin, uint32_t 
     d.xb[b].w = S.rho0 * vol;
 }
 
+// ---------------------------------------------------------------- PCISPH boundary objects
+// Poly6::W (Poly6.h:31-43): |r| as glm::length rounds it, then squared.
+__device__ __forceinline__ float poly6W(const SceneConst& S, float rl) {
+    const float r2 = rl * rl;
+    if (!(r2 <= S.h2)) return 0.0f;
+    const float diff = S.h2 - r2;
+    return S.poly6C * diff * diff * diff;
+}
+// Spiky::gradW (Spiky.h:44-56) as gradW = r * factor.
+__device__ __forceinline__ float spikyFactor(const SceneConst& S, float rl) {
+    if (!(rl != 0.0f && rl <= S.h)) return 0.0f;
+    const float diff = S.h - rl;
+    return -(S.spikyC * diff * diff / rl);
+}
+
+// One thread per boundary particle of one scene: the fluid particles of its
+// 27 cells (forall_fluid_neighbors), then the boundary particles of the same
+// cells (forall_boundary_neighbors), SPHSolver.cpp:91-123 (current
+// positions, cubic spline; Tait pressure of WCSPHSolver.cpp:97-115) or
+// PCISPHSolver.cpp:211-244 (predicted positions, Poly6; pressure update of
+// :284-297).
+__global__ void kBoundaryDensity(DevView d, int scene, uint32_t bBegin, uint32_t bEnd, int predicted, int slot) {
+    const SceneConst& S = d.scenes[scene];
+    if (!d.state[scene].active[slot]) return;
+    const uint32_t b = bBegin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= bEnd) return;
+    const float4 xb = d.xb[b];
+    int cx, cy, cz;
+    globalCell(S, xb, cx, cy, cz);
+    float rho = 0.0f;
+    for (int q = 0; q < 27; ++q) {
+        bool bucket;
+        const uint32_t c = cellSlot(S, cx + q / 9 - 1, cy + (q / 3) % 3 - 1, cz + q % 3 - 1, bucket);
+        if (bucket) continue;   // boundary cells lie 4 or more cells inside the dense box
+        const uint2 fr = d.fRange[c];
+        for (uint32_t j = fr.x; j < fr.y; ++j) {
+            const float4 xj = predicted ? d.xs[j] : d.x[j];
+            const float rl = lenRef(xb.x - xj.x, xb.y - xj.y, xb.z - xj.z);
+            rho += S.m * (predicted ? poly6W(S, rl) : cubicW(S, rl));
+        }
+    }
+    for (int q = 0; q < 27; ++q) {
+        bool bucket;
+        const uint32_t c = cellSlot(S, cx + q / 9 - 1, cy + (q / 3) % 3 - 1, cz + q % 3 - 1, bucket);
+        if (bucket) continue;
+        const uint2 br = d.bRange[c];
+        for (uint32_t j = br.x; j < br.y; ++j) {
+            const float4 xc = d.xb[j];
+            const float rl = lenRef(xb.x - xc.x, xb.y - xc.y, xb.z - xc.z);
+            rho += xc.w * (predicted ? poly6W(S, rl) : cubicW(S, rl));
+        }
+    }
+    d.bRho[b] = rho;
+    if (predicted) d.bP[b] += S.pciDelta * fmaxf(0.0f, rho - S.rho0);
+    else d.bP[b] = fmaxf(0.0f, S.wcB * (powf(rho / S.rho0, S.gamma) - 1.0f));
+}
+
+// correctPredPositions (PCISPHBoundaryObject.cpp:375-428) of slot i: boundary
+// particles within the rest radius of its current position x (all of them
+// are in its boundary list, which holds those within h) push its predicted
+// position p out along the weighted average of their normals.
+__device__ __forceinline__ void pciCorrectPredicted(const DevView& d, const SceneConst& S, uint32_t i, float4 x,
+                                                    float& px, float& py, float& pz) {
+    const uint32_t nb = min(d.cntB[i], d.capB);
+    float ws = 0.0f, w2 = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
+    for (uint32_t s = 0; s < nb; ++s) {
+        const uint32_t bi = ldList(d.nbrB + static_cast<size_t>(s) * d.NP + i);
+        const float4 xb = d.xb[bi];
+        if (!(lenRef(x.x - xb.x, x.y - xb.y, x.z - xb.z) < S.restR)) continue;
+        const float dl = lenRef(px - xb.x, py - xb.y, pz - xb.z);
+        const float w = fmaxf(0.0f, (S.restR - dl) / S.restR);
+        ws += w;
+        w2 += w * (S.restR - dl);
+        const float4 n = d.bn[bi];
+        nx += w * n.x; ny += w * n.y; nz += w * n.z;
+    }
+    if (ws > 0.0f) {
+        const float q = 1.0f / sqrtf(nx * nx + ny * ny + nz * nz);   // glm::normalize
+        const float c = 1.0f / ws * w2;
+        px += c * (nx * q); py += c * (ny * q); pz += c * (nz * q);
+    }
+}
+
+// correctPositions + correctVelocities (PCISPHBoundaryObject.cpp:218-373)
+// of slot i after the integration: the boundary particles of the 27 cells
+// around its new cell that lie within the rest radius of its new position x
+// move it out along the weighted average n of their normals (weights from
+// the distance of `from` to them: x itself for WCSPH, the last predicted
+// position for PCISPH), and the velocity `base` (v, or the last predicted
+// velocity for PCISPH) is split along n into normal and tangential parts,
+// v = tangentialFct vt - normalFct vn. A cell the reference's hash table does
+// not hold yet (no particle in it at any search so far) has no neighbour
+// lists: no correction, as there; its first access makes it exist from the
+// next search on. Without a weighted normal the reference's normal is
+// normalize(0), not a number, so the velocity test `length(n) > 0` fails and
+// v stays.
+__device__ __forceinline__ void pciCorrect(const DevView& d, const SceneConst& S, float4& x, float4& v, float4 from,
+                                           float4 base) {
+    int cx, cy, cz;
+    if (globalCell(S, x, cx, cy, cz)) return;
+    bool bucket;
+    const uint32_t home = cellSlot(S, cx, cy, cz, bucket);
+    bool exists = true;
+    if (!bucket) {
+        const uint32_t f = d.cellFirst[home];
+        exists = f <= d.stepNo;
+        if (f > d.stepNo + 1u) atomicMin(d.cellFirst + home, d.stepNo + 1u);
+    }
+    float ws = 0.0f, w2 = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
+    if (exists)
+        for (int q = 0; q < 27; ++q) {
+            bool nbucket;
+            const uint32_t c = cellSlot(S, cx + q / 9 - 1, cy + (q / 3) % 3 - 1, cz + q % 3 - 1, nbucket);
+            if (nbucket) continue;   // no boundary particle lies outside the dense box
+            const uint2 br = d.bRange[c];
+            for (uint32_t j = br.x; j < br.y; ++j) {
+                const float4 xb = d.xb[j];
+                if (!(lenRef(x.x - xb.x, x.y - xb.y, x.z - xb.z) < S.restR)) continue;
+                const float dl = lenRef(from.x - xb.x, from.y - xb.y, from.z - xb.z);
+                const float w = fmaxf(0.0f, (S.restR - dl) / S.restR);
+                ws += w;
+                w2 += w * (S.restR - dl);
+                const float4 n = d.bn[j];
+                nx += w * n.x; ny += w * n.y; nz += w * n.z;
+            }
+        }
+    if (!(ws > 0.0f)) return;   // n = normalize(0): no position or velocity change
+    const float q = 1.0f / sqrtf(nx * nx + ny * ny + nz * nz);
+    nx *= q; ny *= q; nz *= q;
+    const float c = 1.0f / ws * w2;
+    x.x += c * nx; x.y += c * ny; x.z += c * nz;
+    if (!(sqrtf(nx * nx + ny * ny + nz * nz) > 0.0f)) return;
+    const float dn = base.x * nx + base.y * ny + base.z * nz;
+    const float vnx = dn * nx, vny = dn * ny, vnz = dn * nz;
+    v.x = S.tFct * (base.x - vnx) - S.nFct * vnx;
+    v.y = S.tFct * (base.y - vny) - S.nFct * vny;
+    v.z = S.tFct * (base.z - vnz) - S.nFct * vnz;
+}
+
 // ---------------------------------------------------------------- density
 // SPHSolver::computeFluidDensities (SPHSolver.cpp:42-89) fused with
 // DFSPHSolver::computeAlpha (DFSPHSolver.cpp:109-178).
@@ -949,9 +1091,17 @@ __global__ void __launch_bounds__(kBlock) kForces(DevView d, int pressure) {
                 ax -= g * rx; ay -= g * ry; az -= g * rz;
             });
         for (uint32_t s = sub; s < nb; s += kLppOnce) {
-            const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
+            const uint32_t bi = ldList(d.nbrB + s * NP + i);
+            const float4 xb = d.xb[bi];
             const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
-            const float g = xb.w * (dpi + dpb) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+            // Akinci: rho0 V_b p_i / rho0^2 (WCSPHSolver.cpp:176-186); a PCISPH
+            // boundary particle: m_b (p_i / rho_i^2 + p_b / rho_b^2) (:160-172)
+            float coef = dpi + dpb;
+            if (S.pciB) {
+                const float rb = d.bRho[bi];
+                coef = dpi + d.bP[bi] / (rb * rb);
+            }
+            const float g = xb.w * coef * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
             ax -= g * rx; ay -= g * ry; az -= g * rz;
         }
         ax = lppSum<kLppOnce>(ax);
@@ -1449,6 +1599,7 @@ __global__ void __launch_bounds__(kBlock) kIntegrate(DevView d) {
     ai.x += S.gx; ai.y += S.gy; ai.z += S.gz;
     vi.x += ai.x * dt; vi.y += ai.y * dt; vi.z += ai.z * dt;
     xi.x += vi.x * dt; xi.y += vi.y * dt; xi.z += vi.z * dt;
+    if (S.pciB) pciCorrect(d, S, xi, vi, xi, vi);   // WCSPHSolver.cpp:62-71
     VEL(d, i) = vi;
     d.x[i] = xi;
     d.a[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1506,7 +1657,12 @@ __global__ void __launch_bounds__(kBlock) kPciInit(DevView d) {
     const float vx = vi.x + (ai.x + S.gx) * dt;
     const float vy = vi.y + (ai.y + S.gy) * dt;
     const float vz = vi.z + (ai.z + S.gz) * dt;
-    d.xs[i] = make_float4(xi.x + vx * dt, xi.y + vy * dt, xi.z + vz * dt, 0.0f);
+    float px = xi.x + vx * dt, py = xi.y + vy * dt, pz = xi.z + vz * dt;
+    if (S.pciB) {   // PCISPHSolver.cpp:333-341
+        pciCorrectPredicted(d, S, i, xi, px, py, pz);
+        d.vs[i] = make_float4(vx, vy, vz, 0.0f);
+    }
+    d.xs[i] = make_float4(px, py, pz, 0.0f);
 }
 
 // predictFluidDensities (PCISPHSolver.cpp:162-209): predicted positions,
@@ -1602,9 +1758,16 @@ __device__ __forceinline__ void pciPressureAcc(const DevView& d, const SceneCons
                         });
     const uint32_t nb = S.akinci ? min(d.cntB[i], d.capB) : 0u;
     for (uint32_t s = 0; s < nb; ++s) {
-        const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
+        const uint32_t bi = ldList(d.nbrB + s * NP + i);
+        const float4 xb = d.xb[bi];
         const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
-        const float g = xb.w * dpi * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+        float g;
+        if (S.pciB) {   // m_b (p_i / rho_i^2 + p_b / rho_b^2) Spiky::gradW (:339-351)
+            const float rb = d.bRho[bi];
+            g = xb.w * (dpi + d.bP[bi] / (rb * rb)) * spikyFactor(S, lenRef(rx, ry, rz));
+        } else {
+            g = xb.w * dpi * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+        }
         ax -= g * rx; ay -= g * ry; az -= g * rz;
     }
     d.ap[i] = make_float4(ax, ay, az, 0.0f);
@@ -1612,7 +1775,16 @@ __device__ __forceinline__ void pciPressureAcc(const DevView& d, const SceneCons
     const float vx = vi.x + ((ai.x + ax) + S.gx) * dt;
     const float vy = vi.y + ((ai.y + ay) + S.gy) * dt;
     const float vz = vi.z + ((ai.z + az) + S.gz) * dt;
-    d.xs[i] = make_float4(xi.x + vx * dt, xi.y + vy * dt, xi.z + vz * dt, 0.0f);
+    float px = xi.x + vx * dt, py = xi.y + vy * dt, pz = xi.z + vz * dt;
+    if (S.pciB) {
+        // keep the prediction this iteration used (the last one, when the loop
+        // stops here, is what correctPositions / correctVelocities read)
+        d.xsP[i] = d.xs[i];
+        d.vsP[i] = d.vs[i];
+        pciCorrectPredicted(d, S, i, xi, px, py, pz);
+        d.vs[i] = make_float4(vx, vy, vz, 0.0f);
+    }
+    d.xs[i] = make_float4(px, py, pz, 0.0f);
 }
 __global__ void __launch_bounds__(kBlock) kPciPressureAcc(DevView d, int slot, int fin) {
     SCENE_PROLOGUE(slot)
@@ -1640,6 +1812,7 @@ __global__ void __launch_bounds__(kBlock) kPciIntegrate(DevView d) {
     ai.x += S.gx + pa.x; ai.y += S.gy + pa.y; ai.z += S.gz + pa.z;
     vi.x += ai.x * dt; vi.y += ai.y * dt; vi.z += ai.z * dt;
     xi.x += vi.x * dt; xi.y += vi.y * dt; xi.z += vi.z * dt;
+    if (S.pciB) pciCorrect(d, S, xi, vi, d.xsP[i], d.vsP[i]);   // PCISPHSolver.cpp:64-73
     VEL(d, i) = vi;
     d.x[i] = xi;
     d.a[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1757,6 +1930,12 @@ hipError_t launchBoundaryVolume(const DevView& d, int scene, uint32_t b0, uint32
     const uint32_t g = (b1 - b0 + kBlock - 1) / kBlock;
     if (g == 0) return hipSuccess;
     LAUNCH(kBoundaryVolume, g, d, scene, b0, b1);
+}
+hipError_t launchBoundaryDensity(const DevView& d, int scene, uint32_t b0, uint32_t b1, int predicted, int slot,
+                                 hipStream_t st) {
+    const uint32_t g = (b1 - b0 + kBlock - 1) / kBlock;
+    if (g == 0) return hipSuccess;
+    LAUNCH(kBoundaryDensity, g, d, scene, b0, b1, predicted, slot);
 }
 hipError_t launchDensityAlpha(const DevView& d, int wa, hipStream_t st) { LAUNCH(kDensityAlpha, d.numBlocks * kLppOnce, d, wa); }
 hipError_t launchDensityPressure(const DevView& d, hipStream_t st) { LAUNCH(kDensityPressure, d.numBlocks, d); }

@@ -32,9 +32,23 @@ struct DevView {
     float* kvws;   // divergence warm-start accumulator
     float* aux;    // predicted density / divergence error
     uint32_t* id;  // original particle index of each slot
-    // boundary (sorted), w = density0 * volume
+    // boundary (sorted), w = density0 * volume (Akinci) or the particle mass
+    // density0 (2r)^3 (PCISPH boundary objects)
     float4* xb;
     float* vb;
+    // PCISPH boundary objects (null otherwise): per boundary particle its
+    // normal, density and pressure; per fluid slot the predicted velocity and
+    // the last iteration's predicted position and velocity (PCISPH only); per
+    // cell slot the first step from which the reference's hash table holds the
+    // cell (~0: not yet); the index of the step in progress
+    float4* bn;
+    float* bRho;
+    float* bP;
+    float4* vs;
+    float4* xsP;
+    float4* vsP;
+    uint32_t* cellFirst;
+    uint32_t stepNo;
     // grid
     uint2* fRange;
     const uint2* bRange;
@@ -102,6 +116,14 @@ hipError_t launchTileLists(const DevView& d, hipStream_t st);
 // cube boundary objects of every scene (no-op for scenes without)
 hipError_t launchCubeBoundary(const DevView& d, hipStream_t st);
 hipError_t launchBoundaryVolume(const DevView& d, int scene, uint32_t bBegin, uint32_t bEnd, hipStream_t st);
+// Densities (and pressures) of a PCISPH boundary object's particles: predicted
+// = 0 is SPHSolver::computeBoundaryDensities + WCSPHSolver::computeBoundaryPressures
+// (cubic spline at current positions, Tait pressure), predicted = 1 is
+// PCISPHSolver::predictBoundaryDensities + the boundary part of updatePressure
+// (Poly6 at predicted positions, p += delta max(0, rho - rho0)); gated by the
+// scene's iteration flag `slot`.
+hipError_t launchBoundaryDensity(const DevView& d, int scene, uint32_t bBegin, uint32_t bEnd, int predicted, int slot,
+                                 hipStream_t st);
 
 hipError_t launchDensityAlpha(const DevView& d, int withAlpha, hipStream_t st);
 hipError_t launchDensityPressure(const DevView& d, hipStream_t st);

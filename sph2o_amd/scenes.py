@@ -138,6 +138,40 @@ def sphere_scenes():
     return out
 
 
+def pcisph_boundary_scene(method, boxes=(), spheres=(), block=((0.0, 0.0, 0.0), (0.2, 0.2, 0.2)), normal_fct=0.3,
+                          tang_fct=0.95):
+    """A scene of boundary method 1 (PCISPHBoundaryObject): one boundary
+    object with the given boxes ((min, max, inverted)) and spheres
+    ((centre, radius, inverted)), a block at r = 0.01, XSPH 0.01."""
+    sc = small(method, viscosity_method=2, viscosity=0.01)
+    sc["Configuration"]["boundaryMethod"] = 1
+    fb = sc["Fluid"]["FluidModel"][0]["fluidBlocks"][0]
+    fb["min"], fb["max"] = list(block[0]), list(block[1])
+    sc["Boundary"] = [{"normalFct": normal_fct, "tangFct": tang_fct,
+                       "box": [{"min": list(lo), "max": list(hi), "inverted": inv} for lo, hi, inv in boxes],
+                       "sphere": [{"pos": list(c), "radius": rad, "inverted": inv} for c, rad, inv in spheres],
+                       "geometry": []}]
+    return sc
+
+
+def pcisph_boundary_scenes():
+    """Boundary method 1 with its defined samples (boxes of at least 1 m per
+    axis, spheres): a 1 m container box with the block flush in its corner
+    (WCSPH) or one particle radius off the walls (PCISPH: flush, the
+    reference's first PCISPH step throws particles out at 1,156 m/s), the
+    block resting against a sphere obstacle in the same object as the box
+    (WCSPH), and a block inside an inverted sphere of radius 0.2 m (PCISPH)."""
+    box = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), True)
+    return {
+        "wcsph_box": pcisph_boundary_scene(0, [box]),
+        "pcisph_box": pcisph_boundary_scene(1, [box], block=((0.01, 0.01, 0.01), (0.21, 0.21, 0.21))),
+        "wcsph_obstacle": pcisph_boundary_scene(0, [box], [((0.57, 0.5, 0.1), 0.065, False)],
+                                                block=((0.3, 0.4, 0.0), (0.5, 0.6, 0.2))),
+        "pcisph_sphere": pcisph_boundary_scene(1, [], [((0.2, 0.2, 0.2), 0.2, True)],
+                                               block=((0.13, 0.13, 0.03), (0.27, 0.27, 0.17))),
+    }
+
+
 def write_json(scene, path):
     with open(path, "w") as f:
         json.dump(scene, f)

@@ -135,6 +135,21 @@ class Context:
         self._check(self.L.sph_get_boundary(self.h, scene, _fp(xb) if nb else None, _fp(vol) if (volumes and nb) else None))
         return xb, vol
 
+    def set_boundary_normals(self, scene, normals, normal_fct, tangential_fct):
+        """The PCISPH boundary object of a scene added by add_scene (boundary
+        method 1): one normal per boundary particle and the object's factors."""
+        nrm = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        self._check(self.L.sph_set_boundary_normals(self.h, scene, len(nrm), _fp(nrm), float(normal_fct),
+                                                    float(tangential_fct)))
+
+    def boundary_state(self, scene=0):
+        """Densities and pressures of a PCISPH boundary object's particles."""
+        nb = self.num_boundary(scene)
+        rho = np.zeros(max(1, nb), np.float32)
+        p = np.zeros(max(1, nb), np.float32)
+        self._check(self.L.sph_get_boundary_state(self.h, scene, _fp(rho), _fp(p)))
+        return rho[:nb], p[:nb]
+
     FIELDS = {"alpha": (0, 1), "a": (1, 3), "aux": (2, 1), "k_ws": (3, 1), "kv_ws": (4, 1), "normal": (5, 3)}
 
     def field(self, name, scene=0):

@@ -38,12 +38,22 @@ Fixtures:
                          and sphere each): sampled boundary, volumes, the
                          state after one step and after a short run
 
+  ref_pcisph_boundary_samples.npz
+                         PCISPH boundary objects as the reference samples
+                         them: positions and normals of boxes (non-integer
+                         corners, a truncated extent), spheres, and the
+                         degenerate sub-metre box
+  ref_pcib_*.npz         PCISPH boundary runs (WCSPH and PCISPH, a container
+                         box, a sphere obstacle, an inverted sphere): state
+                         and boundary densities after 1 step and a short run
+
   ref_opt_*.npz          Akinci adhesion and cube-boundary scenes
                          (sph2o_amd.scenes.boundary_option_scenes): the state
                          after one step and after a short run
 
 ``python tests/golden/make_golden.py emitters`` (or ``options``, ``w8k``,
-``d1m``, ``b64``, ``frames``, ``spheres``) writes only those files.
+``d1m``, ``b64``, ``frames``, ``spheres``,
+``pcisph_boundary``) writes only those files.
 """
 import json
 import os
@@ -225,6 +235,37 @@ def spheres():
              stats=b["stats"])
 
 
+def pcisph_boundary():
+    """Boundary method 1 (PCISPHBoundaryObject). Samples: boxes with
+    non-integer corners (the extents truncate to whole metres), a box with
+    two spheres in one object, and the sub-metre box the reference samples as
+    one particle at 0/0. Runs (sph2o_amd.scenes.pcisph_boundary_scenes, the
+    reference's own lattice): the state and boundary densities after 1 step
+    and after a short run."""
+    samples = {
+        "offset": SC.pcisph_boundary_scene(1, [((0.13, 0.07, 0.011), (1.37, 1.21, 1.49), False)]),
+        "wide": SC.pcisph_boundary_scene(1, [((-0.3, 0.05, 0.2), (2.0, 1.2, 3.37), False)]),
+        "truncated": SC.pcisph_boundary_scene(1, [((0.0, 0.0, 0.0), (1.5, 1.2, 1.0), True)]),
+        "spheres": SC.pcisph_boundary_scene(1, [((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), True)],
+                                            [((0.5, 0.5, 0.5), 0.1, False), ((0.3, 0.3, 0.3), 0.05, True)]),
+        "degenerate": SC.pcisph_boundary_scene(1, [((0.0, 0.0, 0.0), (0.6, 0.3, 0.4), True)]),
+    }
+    arrays = {}
+    for tag, sc in samples.items():
+        r = S.run_reference(sc, 0)
+        arrays[tag + "_scene"] = scene_json(sc)
+        arrays[tag + "_bx"] = r["pbx"]
+        arrays[tag + "_bn"] = r["pbn"]
+    save("ref_pcisph_boundary_samples.npz", **arrays)
+    plan = {"wcsph_box": 20, "pcisph_box": 20, "wcsph_obstacle": 20, "pcisph_sphere": 40}
+    for tag, sc in SC.pcisph_boundary_scenes().items():
+        a = S.run_reference(sc, 1)
+        b = S.run_reference(sc, plan[tag])
+        save("ref_pcib_%s.npz" % tag, scene=scene_json(sc), nsteps=np.array(plan[tag]), x0=a["x0"], bx=a["pbx"],
+             bn=a["pbn"], x1=a["x"], v1=a["v"], rho1=a["rho"], brho1=a["pbrho"], stats1=a["stats"], x=b["x"],
+             v=b["v"], rho=b["rho"], brho=b["pbrho"], stats=b["stats"])
+
+
 def _box_aggregates(x, v, rho, box):
     """inside count, kinetic sum |v|^2, centre of mass and mean density of the
     particles still inside the box."""
@@ -320,6 +361,9 @@ def main():
     if sys.argv[1:] == ["spheres"]:
         spheres()
         return
+    if sys.argv[1:] == ["pcisph_boundary"]:
+        pcisph_boundary()
+        return
     cell_keys()
     support_pairs()
     neighbors("lattice", 0.0)
@@ -342,6 +386,7 @@ def main():
     b64_smallest()
     frame_output()
     spheres()
+    pcisph_boundary()
 
 
 if __name__ == "__main__":

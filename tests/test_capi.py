@@ -83,6 +83,66 @@ def test_scene_reader_sphere_boundaries_are_bit_exact(tmp_path):
         assert xb.shape == g["bx"].shape and np.array_equal(xb, g["bx"])
 
 
+def test_scene_reader_pcisph_boundary_samples_are_bit_exact(tmp_path):
+    # PCISPHBoundaryObject::sampleCube / sampleSphere as the shipped build
+    # samples them: extents truncated to whole metres (the integer abs), z
+    # placed with one FMA, face / edge / corner normals negated unless
+    # inverted, sphere normals centre - point
+    g = golden("ref_pcisph_boundary_samples.npz")
+    for tag in ("offset", "wide", "truncated", "spheres"):
+        sc = json.loads(str(g[tag + "_scene"]))
+        path = os.path.join(str(tmp_path), tag + ".json")
+        SC.write_json(sc, path)
+        p, x, xb = _lib.scene_from_json(path)
+        nrm, (nf, tf) = _lib.boundary_normals_from_json(path)
+        assert p.boundary_method == 1
+        assert xb.shape == g[tag + "_bx"].shape and np.array_equal(xb, g[tag + "_bx"]), tag
+        assert np.array_equal(nrm, g[tag + "_bn"]), tag
+        assert (nf, tf) == (np.float32(0.3), np.float32(0.95))
+    # the truncation: the 1.5 x 1.2 x 1.0 m box is sampled as a 1 m cube
+    xb = g["truncated_bx"]
+    assert np.array_equal(xb.min(0), [0, 0, 0]) and np.array_equal(xb.max(0), [1, 1, 1])
+    # box normals are unit vectors; a sphere's are centre - point (length R):
+    # the "spheres" object is a 1 m box (as many points as the truncated one)
+    # and spheres of radius 0.1 and 0.05, in that order
+    for tag in ("offset", "wide", "truncated"):
+        assert np.abs(np.linalg.norm(g[tag + "_bn"].astype(np.float64), axis=1) - 1.0).max() < 1e-6, tag
+    nb = len(g["truncated_bx"])
+    lens = np.linalg.norm(g["spheres_bn"].astype(np.float64), axis=1)
+    assert np.abs(lens[:nb] - 1.0).max() < 1e-6
+    big = np.abs(lens[nb:] - 0.1) < 1e-6
+    k = int(np.argmin(big))   # first point of the second sphere
+    assert big[:k].all() and k > 0 and np.abs(lens[nb + k:] - 0.05).max() < 1e-6
+
+
+def test_scene_reader_refuses_undefined_pcisph_boundaries(tmp_path):
+    g = golden("ref_pcisph_boundary_samples.npz")
+    # the reference samples a sub-metre box as one particle at 0/0
+    assert len(g["degenerate_bx"]) == 1 and not np.isfinite(g["degenerate_bx"]).all()
+    path = os.path.join(str(tmp_path), "degenerate.json")
+    SC.write_json(json.loads(str(g["degenerate_scene"])), path)
+    with pytest.raises(_lib.SPHError) as e:
+        _lib.scene_from_json(path)
+    assert e.value.code == _lib.SPH_E_UNSUPPORTED
+    # two boundary objects: correctPositions indexes one object's arrays with
+    # every object's neighbour ids (forsame_boundary_neighbors)
+    sc = SC.pcisph_boundary_scene(0, [((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), True)])
+    sc["Boundary"].append(json.loads(json.dumps(sc["Boundary"][0])))
+    path = os.path.join(str(tmp_path), "two.json")
+    SC.write_json(sc, path)
+    with pytest.raises(_lib.SPHError) as e:
+        _lib.scene_from_json(path)
+    assert e.value.code == _lib.SPH_E_UNSUPPORTED
+    # normalFct / tangFct / inverted are required keys for boundary method 1
+    sc = SC.pcisph_boundary_scene(0, [((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), True)])
+    del sc["Boundary"][0]["box"][0]["inverted"]
+    path = os.path.join(str(tmp_path), "noinv.json")
+    SC.write_json(sc, path)
+    with pytest.raises(_lib.SPHError) as e:
+        _lib.scene_from_json(path)
+    assert e.value.code == _lib.SPH_E_SCENE
+
+
 def _write(tmp, sc):
     p = os.path.join(tmp, "scene.json")
     with open(p, "w") as f:

@@ -1,0 +1,158 @@
+"""PCISPH boundary objects (boundary method 1, PCISPHBoundaryObject.cpp) on
+the HIP path, against the compiled reference's runs of the same scene JSON
+(tests/golden/ref_pcib_*.npz): boundary densities and pressures
+(SPHSolver.cpp:91-123, WCSPHSolver.cpp:97-115, PCISPHSolver.cpp:211-297), the
+boundary pressure terms (WCSPHSolver.cpp:160-172 cubic spline,
+PCISPHSolver.cpp:339-351 Spiky), the predicted-position correction of every
+PCISPH iteration and the position / velocity corrections after the
+integration (PCISPHBoundaryObject.cpp:218-447).
+
+Scenes: a 1 m container box with a block flush in its corner (WCSPH) or one
+particle radius off the walls (PCISPH); the block against a sphere obstacle
+in the box's object (WCSPH); a block inside an inverted sphere (PCISPH).
+
+Tolerances, stated per check:
+  sampling: boundary positions bit-identical in the reference's order
+  one step: |dx| <= 1e-6 m, rms(dv)/rms(v) <= 1e-4, fluid and boundary
+      densities within 5e-6 rho0, PCISPH iteration count equal
+  the run (20 steps; the inverted sphere 40): per-step time and time step
+      equal (relative 1e-6), PCISPH iteration counts equal at every step,
+      |dx| <= 1e-5 m, rms(dv)/rms(v) <= 1e-3, boundary densities within 1e-4
+      rho0 (measured: 1.4e-7 to 3.3e-6 m, 2e-6 to 2e-5, 5e-7 rho0)
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import golden
+from sph2o_amd import _lib, scenes as SC
+from sph2o_amd._lib import SPH_E_STATE, SPH_E_UNSUPPORTED, SPHError
+from sph2o_amd.simulation import Context
+
+pytestmark = pytest.mark.gpu
+
+TAGS = ["wcsph_box", "pcisph_box", "wcsph_obstacle", "pcisph_sphere"]
+
+
+def _rms_rel(a, b):
+    d = a - b
+    return float(np.sqrt((d ** 2).mean() / max(1e-30, (b ** 2).mean())))
+
+
+def _load(tmp, sc):
+    path = os.path.join(tmp, "scene.json")
+    SC.write_json(sc, path)
+    ctx = Context(0)
+    sid = ctx.load_scene_json(path)
+    ctx.init()
+    return ctx, sid
+
+
+@pytest.mark.parametrize("tag", TAGS)
+def test_gpu_pcisph_boundary_one_step(tmp_path, tag):
+    g = golden("ref_pcib_%s.npz" % tag)
+    ctx, sid = _load(str(tmp_path), json.loads(str(g["scene"])))
+    try:
+        xb, _ = ctx.boundary(sid)
+        assert np.array_equal(xb, g["bx"])
+        s = ctx.step(1)[0][sid]
+        st = ctx.fluid_state(sid)
+        brho, _ = ctx.boundary_state(sid)
+        assert s.iterations == int(g["stats1"][0, 2])
+        assert np.abs(st["x"] - g["x1"]).max() <= 1e-6
+        assert _rms_rel(st["v"], g["v1"]) <= 1e-4
+        assert np.abs(st["rho"] - g["rho1"]).max() <= 5e-6 * 1000.0
+        assert np.abs(brho - g["brho1"]).max() <= 5e-6 * 1000.0
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("tag", TAGS)
+def test_gpu_pcisph_boundary_run(tmp_path, tag):
+    g = golden("ref_pcib_%s.npz" % tag)
+    ctx, sid = _load(str(tmp_path), json.loads(str(g["scene"])))
+    try:
+        stats = [ctx.step(1)[0][sid] for _ in range(int(g["nsteps"]))]
+        ref = g["stats"]
+        assert np.allclose([s.t for s in stats], ref[:, 0], rtol=1e-6, atol=0)
+        assert np.allclose([s.dt for s in stats], ref[:, 1], rtol=1e-6, atol=0)
+        assert np.array_equal([s.iterations for s in stats], ref[:, 2].astype(int))
+        st = ctx.fluid_state(sid)
+        brho, _ = ctx.boundary_state(sid)
+        assert np.abs(st["x"] - g["x"]).max() <= 1e-5
+        assert _rms_rel(st["v"], g["v"]) <= 1e-3
+        assert np.abs(brho - g["brho"]).max() <= 1e-4 * 1000.0
+    finally:
+        ctx.close()
+
+
+def test_gpu_pcisph_boundary_batched_scene_equals_alone(tmp_path):
+    # a PCISPH boundary scene and an Akinci one in one context: every scene
+    # bit-identical to itself stepped alone
+    sph = json.loads(str(golden("ref_pcib_pcisph_sphere.npz")["scene"]))
+    ak = SC.small(1, viscosity_method=2, viscosity=0.01)
+    paths = []
+    for k, sc in enumerate((sph, ak)):
+        p = os.path.join(str(tmp_path), "s%d.json" % k)
+        SC.write_json(sc, p)
+        paths.append(p)
+    both = Context(0)
+    for p in paths:
+        both.load_scene_json(p)
+    both.init()
+    both.step(10)
+    try:
+        for k, p in enumerate(paths):
+            one = Context(0)
+            one.load_scene_json(p)
+            one.init()
+            one.step(10)
+            a, b = both.fluid_state(k), one.fluid_state(0)
+            assert np.array_equal(a["x"], b["x"]) and np.array_equal(a["v"], b["v"])
+            one.close()
+    finally:
+        both.close()
+
+
+def test_gpu_pcisph_boundary_unsupported_cases(tmp_path):
+    box = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), True)
+    # DFSPH has no PCISPH boundary path here
+    with pytest.raises(SPHError) as e:
+        _load(str(tmp_path), SC.pcisph_boundary_scene(2, [box]))
+    assert e.value.code == SPH_E_UNSUPPORTED
+    # normals are required before sph_init
+    sc = SC.pcisph_boundary_scene(0, [box])
+    path = os.path.join(str(tmp_path), "w.json")
+    SC.write_json(sc, path)
+    p, x, xb = _lib.scene_from_json(path)
+    ctx = Context(0)
+    try:
+        sid = ctx.add_scene(p, x, None, xb)
+        with pytest.raises(SPHError) as e:
+            ctx.init()
+        assert e.value.code == SPH_E_STATE
+    finally:
+        ctx.close()
+    # with them, the add_scene path equals the JSON path; state files and
+    # gradients are refused
+    nrm, (nf, tf) = _lib.boundary_normals_from_json(path)
+    ctx = Context(0)
+    ref, rid = _load(str(tmp_path), sc)
+    try:
+        sid = ctx.add_scene(p, x, None, xb)
+        ctx.set_boundary_normals(sid, nrm, nf, tf)
+        ctx.init()
+        ctx.step(3)
+        ref.step(3)
+        assert np.array_equal(ctx.fluid_state(sid)["x"], ref.fluid_state(rid)["x"])
+        with pytest.raises(SPHError) as e:
+            ctx.save_state(os.path.join(str(tmp_path), "state.bin"))
+        assert e.value.code == SPH_E_UNSUPPORTED
+        with pytest.raises(SPHError) as e:
+            ctx.forward_record(2, 1)
+        assert e.value.code == SPH_E_UNSUPPORTED
+    finally:
+        ctx.close()
+        ref.close()
