@@ -225,6 +225,8 @@ class Engine {
     DevBuf<float> bRho, bP;
     DevBuf<uint32_t> cellFirst;
     uint32_t stepNo = 0;   // steps taken since sph_init (the hash table's clock)
+    uint32_t markNo = 0;   // the step from which the cells of the search in progress exist
+    DevBuf<float4> nc, ncAlt;   // DFSPH: normal of the last correctPositions per slot
     DevBuf<float> rho, alpha, p, k, kws, kwsAlt, kvws, kvwsAlt, aux, vb;
     DevBuf<uint32_t> id, idAlt, keys, keysAlt, vals, valsAlt;
     DevBuf<uint2> fRange, bRange;
@@ -299,8 +301,6 @@ class Engine {
         if (P.boundary_method != SPH_BOUNDARY_AKINCI && P.boundary_method != SPH_BOUNDARY_PCISPH &&
             !(P.boundary_method == SPH_BOUNDARY_NONE) && !(nb == 0 && (P.boundary_method == 0)))
             throw SphError{SPH_E_UNSUPPORTED, "only Akinci and PCISPH boundaries are supported"};
-        if (P.boundary_method == SPH_BOUNDARY_PCISPH && nb > 0 && P.simulation_method == SPH_DFSPH)
-            throw SphError{SPH_E_UNSUPPORTED, "PCISPH boundary objects are supported with WCSPH and PCISPH, not DFSPH"};
         if (P.viscosity_method > 2 || P.viscosity_method < -1)
             throw SphError{SPH_E_ARG, "viscosity_method must be -1, 0, 1 or 2"};
         if (!(P.particle_radius > 0.0f)) throw SphError{SPH_E_ARG, "particle_radius must be > 0"};
@@ -870,6 +870,9 @@ class Engine {
         d.bn = bn.p; d.bRho = bRho.p; d.bP = bP.p; d.vs = vs.p; d.xsP = xsP.p; d.vsP = vsP.p;
         d.cellFirst = cellFirst.p;
         d.stepNo = stepNo;
+        d.markNo = markNo;
+        d.nc = nc.p;
+        d.ncAlt = ncAlt.p;
         d.fRange = fRange.p; d.bRange = bRange.p; d.cellOf = cellOf.p;
         d.rankOf = rankOf.p; d.ordToCell = ordToCell.p; d.counts = counts.p; d.starts = starts.p; d.li = li.p;
         d.snapV = snapV.p; d.snapKvws = snapKvws.p;
@@ -959,7 +962,13 @@ class Engine {
                 b->alloc(np);
                 HIPCHK(hipMemset(b->p, 0, np * sizeof(float4)));
             }
+        if (method == SPH_DFSPH && anyPciB())   // FluidObject's normals start at zero
+            for (auto* b : {&nc, &ncAlt}) {
+                b->alloc(np);
+                HIPCHK(hipMemset(b->p, 0, np * sizeof(float4)));
+            }
         stepNo = 0;
+        markNo = 0;
         fuseFinish = sphKernelsFuseFinish() != 0;
         for (auto* b : {&rho, &alpha, &p, &k, &kws, &kwsAlt, &kvws, &kvwsAlt, &aux}) b->alloc(np);
         for (auto* b : {&id, &idAlt, &keys, &keysAlt, &vals, &valsAlt, &cntF, &cntB, &cellOf, &li}) b->alloc(np);
@@ -1006,6 +1015,7 @@ class Engine {
         if (method == SPH_DFSPH && NP) {  // DFSPHSolver::init (DFSPHSolver.cpp:22-31)
             neighborSearch(false);
             timed(K_DENSITY, [&] { return launchDensityAlpha(view(), 1, stream); });
+            boundaryDensities();
         }
         HIPCHK(hipStreamSynchronize(stream));
     }
@@ -1163,6 +1173,7 @@ class Engine {
         std::swap(kws.p, kwsAlt.p);
         std::swap(kvws.p, kvwsAlt.p);
         std::swap(id.p, idAlt.p);
+        if (nc.p) std::swap(nc.p, ncAlt.p);
         d = view();
         timed(K_BUILD, [&] { return launchBuildNeighbors(d, stream); });
         if (tiles) timed(K_TILES, [&] { return launchTileLists(d, stream); });
@@ -1340,9 +1351,22 @@ class Engine {
         }
     }
 
+    // SPHSolver::computeBoundaryDensities of PCISPH boundary objects at the
+    // current positions (DFSPH computes them with the fluid's; none of its
+    // passes reads them)
+    void boundaryDensities() {
+        const DevView d = view();
+        for (size_t si = 0; si < scenes.size(); ++si)
+            if (scenes[si].pciB)
+                timed(K_BDENSITY, [&] {
+                    return launchBoundaryDensity(d, static_cast<int>(si), scenes[si].bBegin, scenes[si].bEnd, 0, 2, stream);
+                });
+    }
+
     void stepDFSPH() {
         beginStats();
         if (!NP) return advanceEmpty();
+        markNo = stepNo + 1;   // the mid-step search follows this step's position update
         DevView d = view();
         // 1-2. non-pressure forces, then the CFL maximum; the new dt is set on the device
         if (anyVisc() || anyST()) timed(K_FORCES, [&] { return launchForces(d, 0, stream); });
@@ -1373,6 +1397,7 @@ class Engine {
             d = view();
             // 8-9. computeDensities + computeAlpha
             timed(K_DENSITY, [&] { return launchDensityAlpha(d, 1, stream); });
+            if (anyPciB()) boundaryDensities();
             // 10. correctDivergenceError: warm start (and loop start), then the loop
             timed(K_DIVERGENCE, [&] { return launchDivergence(d, 1, 2, stream); });
             timed(K_VELOCITY, [&] { return launchVelocity(d, 2, -1, stream); });
@@ -1405,12 +1430,14 @@ class Engine {
             if (T.it >= c.maxIt && T.err > static_cast<double>(c.etaV / T.dt)) T.capped |= 2;
             T.t = T.t + T.dt;
         }
+        ++stepNo;
         fileStats(true);
     }
 
     void stepWCSPH() {
         beginStats();
         if (!NP) return advanceEmpty();
+        markNo = stepNo;
         neighborSearch(false);
         DevView d = view();
         timed(K_DENSITY, [&] { return launchDensityPressure(d, stream); });
@@ -1433,6 +1460,7 @@ class Engine {
     void stepPCISPH() {
         beginStats();
         if (!NP) return advanceEmpty();
+        markNo = stepNo;
         neighborSearch(false);
         DevView d = view();
         timed(K_DENSITY, [&] { return launchDensityAlpha(d, 0, stream); });

@@ -275,6 +275,78 @@ __device__ __forceinline__ bool inCell(const SceneConst& S, float4 p, int gx, in
            static_cast<int>(floorf(__fmul_rn(p.z, S.invh))) == gz;
 }
 
+// correctPositions (PCISPHBoundaryObject.cpp:218-315) of one fluid particle
+// after its position update: the boundary particles of the 27 cells around
+// its new cell that lie within the rest radius of its new position x move it
+// out along the weighted average of their normals, weights from the distance
+// of `from` to them (x itself for WCSPH and DFSPH, the last predicted
+// position for PCISPH). A cell the reference's hash table does not hold yet
+// (no particle in it at any search so far) has no neighbour lists: no
+// correction, as there; its first access makes it exist from the next search
+// on. Returns the unit normal n, or false where the reference's normal is
+// normalize(0) (no weight), which fails every later `length(n) > 0` test.
+__device__ __forceinline__ bool pciCollide(const DevView& d, const SceneConst& S, float4& x, float4 from, float& nx,
+                                           float& ny, float& nz) {
+    nx = ny = nz = 0.0f;
+    int cx, cy, cz;
+    if (globalCell(S, x, cx, cy, cz)) return false;
+    bool bucket;
+    const uint32_t home = cellSlot(S, cx, cy, cz, bucket);
+    if (!bucket) {
+        // the lists in use are those of the last search, which (WCSPH, PCISPH:
+        // this step's; DFSPH: the previous step's mid-step one) holds the cells
+        // marked up to stepNo; a cell looked up now joins at the next search
+        const uint32_t f = d.cellFirst[home];
+        if (f > d.stepNo + 1u) atomicMin(d.cellFirst + home, d.stepNo + 1u);
+        if (f > d.stepNo) return false;
+    }
+    float ws = 0.0f, w2 = 0.0f;
+    for (int q = 0; q < 27; ++q) {
+        bool nbucket;
+        const uint32_t c = cellSlot(S, cx + q / 9 - 1, cy + (q / 3) % 3 - 1, cz + q % 3 - 1, nbucket);
+        if (nbucket) continue;   // no boundary particle lies outside the dense box
+        const uint2 br = d.bRange[c];
+        for (uint32_t j = br.x; j < br.y; ++j) {
+            const float4 xb = d.xb[j];
+            if (!(lenRef(x.x - xb.x, x.y - xb.y, x.z - xb.z) < S.restR)) continue;
+            const float dl = lenRef(from.x - xb.x, from.y - xb.y, from.z - xb.z);
+            const float w = fmaxf(0.0f, (S.restR - dl) / S.restR);
+            ws += w;
+            w2 += w * (S.restR - dl);
+            const float4 n = d.bn[j];
+            nx += w * n.x; ny += w * n.y; nz += w * n.z;
+        }
+    }
+    if (!(ws > 0.0f)) {
+        nx = ny = nz = 0.0f;
+        return false;
+    }
+    const float q = 1.0f / sqrtf(nx * nx + ny * ny + nz * nz);   // glm::normalize
+    nx *= q; ny *= q; nz *= q;
+    const float c = 1.0f / ws * w2;
+    x.x += c * nx; x.y += c * ny; x.z += c * nz;
+    return true;
+}
+// correctVelocities (PCISPHBoundaryObject.cpp:317-373): with length(n) > 0,
+// the velocity `base` (v itself, or PCISPH's last predicted velocity) split
+// along n gives v = tangentialFct vt - normalFct vn (n as the fluid's normal
+// array holds it: not always a unit vector).
+__device__ __forceinline__ void pciReflect(const SceneConst& S, float4& v, float4 base, float nx, float ny, float nz) {
+    if (!(sqrtf(nx * nx + ny * ny + nz * nz) > 0.0f)) return;
+    const float dn = base.x * nx + base.y * ny + base.z * nz;
+    const float vnx = dn * nx, vny = dn * ny, vnz = dn * nz;
+    v.x = S.tFct * (base.x - vnx) - S.nFct * vnx;
+    v.y = S.tFct * (base.y - vny) - S.nFct * vny;
+    v.z = S.tFct * (base.z - vnz) - S.nFct * vnz;
+}
+// WCSPH and PCISPH: both corrections after the integration.
+__device__ __forceinline__ void pciCorrect(const DevView& d, const SceneConst& S, float4& x, float4& v, float4 from,
+                                           float4 base) {
+    float nx, ny, nz;
+    if (pciCollide(d, S, x, from, nx, ny, nz)) pciReflect(S, v, base, nx, ny, nz);
+}
+
+
 // slot: which iteration flag of the scene state gates this launch (0/1 inside a
 // Jacobi loop, by iteration parity; 2 outside a loop, always set).
 #define SCENE_PROLOGUE(slot)                                  \
@@ -372,6 +444,15 @@ __global__ void __launch_bounds__(kBlock) kKeys(DevView d, int dfsphAdvance, int
                 p.x += vi.x * dt;
                 p.y += vi.y * dt;
                 p.z += vi.z * dt;
+                if (S.pciB) {
+                    // correctPositions after updatePositions (DFSPHSolver.cpp:60-68);
+                    // its normal is what the next step's correctVelocities reads
+                    float nx, ny, nz;
+                    pciCollide(d, S, p, p, nx, ny, nz);
+                    d.nc[i] = make_float4(nx, ny, nz, 0.0f);
+                }
+            } else if (S.pciB) {
+                d.nc[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);   // emitted after the correction
             }
             p.w = 0.0f;
             d.x[i] = p;
@@ -385,7 +466,7 @@ __global__ void __launch_bounds__(kBlock) kKeys(DevView d, int dfsphAdvance, int
                      : cellKey(S, gx - S.g0x, gy - S.g0y, gz - S.g0z);
         // the reference's hash table keeps every cell a particle was inserted in
         // (HashTable::clear empties cells but never erases them)
-        if (S.pciB && !bucket && !bad && d.cellFirst[c] > d.stepNo) atomicMin(d.cellFirst + c, d.stepNo);
+        if (S.pciB && !bucket && !bad && d.cellFirst[c] > d.markNo) atomicMin(d.cellFirst + c, d.markNo);
         if (counting) ord = S.ordBase + d.rankOf[c];
     }
     // particles in overflow buckets, one atomic per wave
@@ -505,6 +586,7 @@ __global__ void __launch_bounds__(kBlock) kReorder(DevView d, const uint32_t* ke
     kwsAlt[i] = d.kws[o];
     kvwsAlt[i] = kv;
     idAlt[i] = d.id[o];
+    if (d.nc) d.ncAlt[i] = d.nc[o];
     if (d.snapV) {
         d.snapV[i] = VEL(d, o);
         d.snapKvws[i] = kv;
@@ -934,62 +1016,6 @@ __device__ __forceinline__ void pciCorrectPredicted(const DevView& d, const Scen
     }
 }
 
-// correctPositions + correctVelocities (PCISPHBoundaryObject.cpp:218-373)
-// of slot i after the integration: the boundary particles of the 27 cells
-// around its new cell that lie within the rest radius of its new position x
-// move it out along the weighted average n of their normals (weights from
-// the distance of `from` to them: x itself for WCSPH, the last predicted
-// position for PCISPH), and the velocity `base` (v, or the last predicted
-// velocity for PCISPH) is split along n into normal and tangential parts,
-// v = tangentialFct vt - normalFct vn. A cell the reference's hash table does
-// not hold yet (no particle in it at any search so far) has no neighbour
-// lists: no correction, as there; its first access makes it exist from the
-// next search on. Without a weighted normal the reference's normal is
-// normalize(0), not a number, so the velocity test `length(n) > 0` fails and
-// v stays.
-__device__ __forceinline__ void pciCorrect(const DevView& d, const SceneConst& S, float4& x, float4& v, float4 from,
-                                           float4 base) {
-    int cx, cy, cz;
-    if (globalCell(S, x, cx, cy, cz)) return;
-    bool bucket;
-    const uint32_t home = cellSlot(S, cx, cy, cz, bucket);
-    bool exists = true;
-    if (!bucket) {
-        const uint32_t f = d.cellFirst[home];
-        exists = f <= d.stepNo;
-        if (f > d.stepNo + 1u) atomicMin(d.cellFirst + home, d.stepNo + 1u);
-    }
-    float ws = 0.0f, w2 = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
-    if (exists)
-        for (int q = 0; q < 27; ++q) {
-            bool nbucket;
-            const uint32_t c = cellSlot(S, cx + q / 9 - 1, cy + (q / 3) % 3 - 1, cz + q % 3 - 1, nbucket);
-            if (nbucket) continue;   // no boundary particle lies outside the dense box
-            const uint2 br = d.bRange[c];
-            for (uint32_t j = br.x; j < br.y; ++j) {
-                const float4 xb = d.xb[j];
-                if (!(lenRef(x.x - xb.x, x.y - xb.y, x.z - xb.z) < S.restR)) continue;
-                const float dl = lenRef(from.x - xb.x, from.y - xb.y, from.z - xb.z);
-                const float w = fmaxf(0.0f, (S.restR - dl) / S.restR);
-                ws += w;
-                w2 += w * (S.restR - dl);
-                const float4 n = d.bn[j];
-                nx += w * n.x; ny += w * n.y; nz += w * n.z;
-            }
-        }
-    if (!(ws > 0.0f)) return;   // n = normalize(0): no position or velocity change
-    const float q = 1.0f / sqrtf(nx * nx + ny * ny + nz * nz);
-    nx *= q; ny *= q; nz *= q;
-    const float c = 1.0f / ws * w2;
-    x.x += c * nx; x.y += c * ny; x.z += c * nz;
-    if (!(sqrtf(nx * nx + ny * ny + nz * nz) > 0.0f)) return;
-    const float dn = base.x * nx + base.y * ny + base.z * nz;
-    const float vnx = dn * nx, vny = dn * ny, vnz = dn * nz;
-    v.x = S.tFct * (base.x - vnx) - S.nFct * vnx;
-    v.y = S.tFct * (base.y - vny) - S.nFct * vny;
-    v.z = S.tFct * (base.z - vnz) - S.nFct * vnz;
-}
-
 // ---------------------------------------------------------------- density
 // SPHSolver::computeFluidDensities (SPHSolver.cpp:42-89) fused with
 // DFSPHSolver::computeAlpha (DFSPHSolver.cpp:109-178).
@@ -1362,6 +1388,13 @@ __global__ void __launch_bounds__(kBlock) kPredictVelocity(DevView d) {
     vi.x += (ai.x + S.gx) * dt;
     vi.y += (ai.y + S.gy) * dt;
     vi.z += (ai.z + S.gz) * dt;
+    if (S.pciB) {
+        // correctVelocities after predictVelocities (DFSPHSolver.cpp:46-51) with
+        // the fluid's normals as they stand: the last correctPositions' (surface
+        // tension keeps its normals in an array of its own, SurfaceTensionAkinci.h:15)
+        const float4 n = d.nc[i];
+        pciReflect(S, vi, vi, n.x, n.y, n.z);
+    }
     VEL(d, i) = vi;
     d.a[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }

@@ -49,6 +49,11 @@ struct DevView {
     float4* vsP;
     uint32_t* cellFirst;
     uint32_t stepNo;
+    uint32_t markNo;   // the step a search's cells exist from (DFSPH's mid-step search: the next)
+    // DFSPH: the fluid's normal (FluidObject::n) as the last correctPositions left it,
+    // per slot (and the reorder's target)
+    float4* nc;
+    float4* ncAlt;
     // grid
     uint2* fRange;
     const uint2* bRange;

@@ -139,11 +139,11 @@ def sphere_scenes():
 
 
 def pcisph_boundary_scene(method, boxes=(), spheres=(), block=((0.0, 0.0, 0.0), (0.2, 0.2, 0.2)), normal_fct=0.3,
-                          tang_fct=0.95):
+                          tang_fct=0.95, **kw):
     """A scene of boundary method 1 (PCISPHBoundaryObject): one boundary
     object with the given boxes ((min, max, inverted)) and spheres
     ((centre, radius, inverted)), a block at r = 0.01, XSPH 0.01."""
-    sc = small(method, viscosity_method=2, viscosity=0.01)
+    sc = small(method, viscosity_method=2, viscosity=0.01, **kw)
     sc["Configuration"]["boundaryMethod"] = 1
     fb = sc["Fluid"]["FluidModel"][0]["fluidBlocks"][0]
     fb["min"], fb["max"] = list(block[0]), list(block[1])
@@ -160,7 +160,10 @@ def pcisph_boundary_scenes():
     (WCSPH) or one particle radius off the walls (PCISPH: flush, the
     reference's first PCISPH step throws particles out at 1,156 m/s), the
     block resting against a sphere obstacle in the same object as the box
-    (WCSPH), and a block inside an inverted sphere of radius 0.2 m (PCISPH)."""
+    (WCSPH), and a block inside an inverted sphere of radius 0.2 m (PCISPH);
+    DFSPH in the box one particle radius off the walls (flush, the
+    reference's DFSPH explodes as well) and in the
+    sphere with surface tension."""
     box = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), True)
     return {
         "wcsph_box": pcisph_boundary_scene(0, [box]),
@@ -169,6 +172,10 @@ def pcisph_boundary_scenes():
                                                 block=((0.3, 0.4, 0.0), (0.5, 0.6, 0.2))),
         "pcisph_sphere": pcisph_boundary_scene(1, [], [((0.2, 0.2, 0.2), 0.2, True)],
                                                block=((0.13, 0.13, 0.03), (0.27, 0.27, 0.17))),
+        "dfsph_box": pcisph_boundary_scene(2, [box], block=((0.01, 0.01, 0.01), (0.21, 0.21, 0.21))),
+        "dfsph_sphere_st": pcisph_boundary_scene(2, [], [((0.2, 0.2, 0.2), 0.2, True)],
+                                                 block=((0.13, 0.13, 0.03), (0.27, 0.27, 0.17)),
+                                                 surface_tension_method=3, surface_tension=0.1),
     }
 
 

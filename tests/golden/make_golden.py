@@ -257,7 +257,8 @@ def pcisph_boundary():
         arrays[tag + "_bx"] = r["pbx"]
         arrays[tag + "_bn"] = r["pbn"]
     save("ref_pcisph_boundary_samples.npz", **arrays)
-    plan = {"wcsph_box": 20, "pcisph_box": 20, "wcsph_obstacle": 20, "pcisph_sphere": 40}
+    plan = {"wcsph_box": 20, "pcisph_box": 20, "wcsph_obstacle": 20, "pcisph_sphere": 40, "dfsph_box": 10,
+            "dfsph_sphere_st": 20}
     for tag, sc in SC.pcisph_boundary_scenes().items():
         a = S.run_reference(sc, 1)
         b = S.run_reference(sc, plan[tag])

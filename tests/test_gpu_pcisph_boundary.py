@@ -8,17 +8,27 @@ PCISPH iteration and the position / velocity corrections after the
 integration (PCISPHBoundaryObject.cpp:218-447).
 
 Scenes: a 1 m container box with a block flush in its corner (WCSPH) or one
-particle radius off the walls (PCISPH); the block against a sphere obstacle
-in the box's object (WCSPH); a block inside an inverted sphere (PCISPH).
+particle radius off the walls (PCISPH, DFSPH); the block against a sphere
+obstacle in the box's object (WCSPH); a block inside an inverted sphere
+(PCISPH, and DFSPH with surface tension). DFSPH corrects the predicted
+velocities with the normals the last correctPositions left
+(DFSPHSolver.cpp:46-68).
 
 Tolerances, stated per check:
   sampling: boundary positions bit-identical in the reference's order
-  one step: |dx| <= 1e-6 m, rms(dv)/rms(v) <= 1e-4, fluid and boundary
-      densities within 5e-6 rho0, PCISPH iteration count equal
-  the run (20 steps; the inverted sphere 40): per-step time and time step
-      equal (relative 1e-6), PCISPH iteration counts equal at every step,
-      |dx| <= 1e-5 m, rms(dv)/rms(v) <= 1e-3, boundary densities within 1e-4
-      rho0 (measured: 1.4e-7 to 3.3e-6 m, 2e-6 to 2e-5, 5e-7 rho0)
+  one step: |dx| <= 1e-6 m, rms(dv)/rms(v) <= 1e-4 (DFSPH with surface
+      tension on the lattice, at the |k| gates: 2e-3; measured 5.3e-4),
+      fluid and boundary densities within 5e-6 rho0, iteration counts equal
+  the run (20 steps; the inverted PCISPH sphere 40, the DFSPH box 10): per-
+      step time and time step equal (relative 1e-6; DFSPH's CFL time step
+      1e-5), iteration counts equal at every step, |dx| <= 1e-5 m,
+      rms(dv)/rms(v) <= 1e-3, boundary densities within 1e-4 rho0 (measured:
+      1.4e-7 to 3.3e-6 m, 2e-6 to 6e-5, 5e-7 rho0). DFSPH with surface
+      tension: density iteration counts equal at every step, divergence ones
+      over the first 12 steps and in total within 5% + 2 (from step 17 a
+      count is decided by a first-iteration divergence error at etaV / dt:
+      the reference does 2 iterations at step 18 and 1 at 19, this path 1
+      and 2), kinetic energy within 5%, centre of mass within 5e-4 m.
 """
 import json
 import os
@@ -33,7 +43,7 @@ from sph2o_amd.simulation import Context
 
 pytestmark = pytest.mark.gpu
 
-TAGS = ["wcsph_box", "pcisph_box", "wcsph_obstacle", "pcisph_sphere"]
+TAGS = ["wcsph_box", "pcisph_box", "wcsph_obstacle", "pcisph_sphere", "dfsph_box", "dfsph_sphere_st"]
 
 
 def _rms_rel(a, b):
@@ -60,9 +70,9 @@ def test_gpu_pcisph_boundary_one_step(tmp_path, tag):
         s = ctx.step(1)[0][sid]
         st = ctx.fluid_state(sid)
         brho, _ = ctx.boundary_state(sid)
-        assert s.iterations == int(g["stats1"][0, 2])
+        assert (s.iterations, s.iterations_v) == (int(g["stats1"][0, 2]), int(g["stats1"][0, 3]))
         assert np.abs(st["x"] - g["x1"]).max() <= 1e-6
-        assert _rms_rel(st["v"], g["v1"]) <= 1e-4
+        assert _rms_rel(st["v"], g["v1"]) <= (2e-3 if tag == "dfsph_sphere_st" else 1e-4)
         assert np.abs(st["rho"] - g["rho1"]).max() <= 5e-6 * 1000.0
         assert np.abs(brho - g["brho1"]).max() <= 5e-6 * 1000.0
     finally:
@@ -72,14 +82,24 @@ def test_gpu_pcisph_boundary_one_step(tmp_path, tag):
 @pytest.mark.parametrize("tag", TAGS)
 def test_gpu_pcisph_boundary_run(tmp_path, tag):
     g = golden("ref_pcib_%s.npz" % tag)
-    ctx, sid = _load(str(tmp_path), json.loads(str(g["scene"])))
+    sc = json.loads(str(g["scene"]))
+    ctx, sid = _load(str(tmp_path), sc)
     try:
         stats = [ctx.step(1)[0][sid] for _ in range(int(g["nsteps"]))]
         ref = g["stats"]
-        assert np.allclose([s.t for s in stats], ref[:, 0], rtol=1e-6, atol=0)
-        assert np.allclose([s.dt for s in stats], ref[:, 1], rtol=1e-6, atol=0)
+        rtol = 1e-5 if tag.startswith("dfsph") else 1e-6
+        assert np.allclose([s.t for s in stats], ref[:, 0], rtol=rtol, atol=0)
+        assert np.allclose([s.dt for s in stats], ref[:, 1], rtol=rtol, atol=0)
         assert np.array_equal([s.iterations for s in stats], ref[:, 2].astype(int))
         st = ctx.fluid_state(sid)
+        if tag == "dfsph_sphere_st":
+            itv = np.array([s.iterations_v for s in stats])
+            assert np.array_equal(itv[:12], ref[:12, 3].astype(int))
+            assert abs(itv.sum() - ref[:, 3].sum()) <= 0.05 * ref[:, 3].sum() + 2
+            assert abs((st["v"] ** 2).sum() - (g["v"] ** 2).sum()) / (g["v"] ** 2).sum() < 0.05
+            assert np.abs(st["x"].mean(0) - g["x"].mean(0)).max() < 5e-4
+            return
+        assert np.array_equal([s.iterations_v for s in stats], ref[:, 3].astype(int))
         brho, _ = ctx.boundary_state(sid)
         assert np.abs(st["x"] - g["x"]).max() <= 1e-5
         assert _rms_rel(st["v"], g["v"]) <= 1e-3
@@ -118,10 +138,6 @@ def test_gpu_pcisph_boundary_batched_scene_equals_alone(tmp_path):
 
 def test_gpu_pcisph_boundary_unsupported_cases(tmp_path):
     box = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), True)
-    # DFSPH has no PCISPH boundary path here
-    with pytest.raises(SPHError) as e:
-        _load(str(tmp_path), SC.pcisph_boundary_scene(2, [box]))
-    assert e.value.code == SPH_E_UNSUPPORTED
     # normals are required before sph_init
     sc = SC.pcisph_boundary_scene(0, [box])
     path = os.path.join(str(tmp_path), "w.json")
