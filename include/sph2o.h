@@ -167,10 +167,13 @@ int sph_add_cube_boundary(sph_ctx* ctx, int scene, const float* min, const float
  * the pressure term of each solver; after integration the fluid is moved out
  * of the boundary's rest radius h/2 along the averaged normals and its
  * velocity reflected (correctPositions / correctVelocities,
- * PCISPHBoundaryObject.cpp:218-373), and PCISPH corrects its predicted
- * positions each iteration (correctPredPositions, :375-447). WCSPH and PCISPH
- * only (DFSPH: SPH_E_UNSUPPORTED); sph_load_scene_json sets this from the
- * scene's Boundary entry. */
+ * PCISPHBoundaryObject.cpp:218-373, at each solver's points: WCSPH and
+ * PCISPH after the integration, DFSPH after predictVelocities and after
+ * updatePositions), and PCISPH corrects its predicted positions each
+ * iteration (correctPredPositions, :375-447). One boundary object per scene;
+ * boxes must span at least 1 m per axis (the reference truncates extents to
+ * whole metres). sph_load_scene_json sets this from the scene's Boundary
+ * entry. State files and gradients of such scenes give SPH_E_UNSUPPORTED. */
 int sph_set_boundary_normals(sph_ctx* ctx, int scene, int64_t nb, const float* normals, float normal_fct,
                              float tangential_fct);
 /* The PCISPH boundary's normals and factors of a scene JSON without a context
