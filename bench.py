@@ -345,14 +345,63 @@ def gpu_leg(sc, x, v, windows):
     return n, out
 
 
+def d1m_flush_leg(threads, steps=20, ref_steps=3):
+    """D1M as the survey places it, flush against the walls: the per-step
+    statistics of the first `ref_steps` steps on the GPU and in the compiled
+    reference, and the GPU's time and mean density error over `steps`."""
+    from oracle import sphref as S
+    from sph2o_amd import scenes as SC
+    from sph2o_amd.simulation import Context
+
+    sc, _, _ = SC.d1m(gap=0.0)
+    td = tempfile.mkdtemp()
+    path = os.path.join(td, "d1m_flush.json")
+    SC.write_json(sc, path)
+    from sph2o_amd._lib import SPH_E_NONFINITE, SPHError
+
+    ctx = Context(0)
+    sid = ctx.load_scene_json(path)
+    ctx.init()
+    n = ctx.num_fluid(sid)
+    ctx.synchronize()
+    row = lambda s: [s.t, s.dt, s.iterations, s.iterations_v, s.avg_density_error, s.avg_divergence_error]
+    rows, nonfinite, secs, rho = [], None, 0.0, None
+    for k in range(steps):
+        t0 = time.perf_counter()
+        try:
+            rows.append(row(ctx.step(1)[0][sid]))
+        except SPHError as e:
+            if e.code != SPH_E_NONFINITE:
+                raise
+            nonfinite = k + 1
+            break
+        ctx.synchronize()
+        secs += time.perf_counter() - t0
+        rho = ctx.fluid_state(sid)["rho"]
+    ctx.close()
+    done = len(rows)
+    out = {"workload": "D1M flush: the D1M dam break with the block flush against the walls (first layer r from the "
+                       "boundary particles), %d fluid particles" % n, "particles": n,
+           "gpu": {"steps": "1-%d" % done, "value": n * done / secs if done else None, "unit": "particle-steps/s",
+                   "ms_per_step": 1000.0 * secs / done if done else None,
+                   "nonfinite_at_step": nonfinite,
+                   "mean_density_error": float(np.mean(np.maximum(rho / 1000.0 - 1.0, 0.0))) if rho is not None else None,
+                   "steps_stats": rows},
+           "reference_first_steps": None}
+    if S.reference_available():
+        r = S.run_reference(sc, ref_steps, threads=threads, timeout=900)
+        out["reference_first_steps"] = r["stats"].tolist()
+    return out
+
+
 def configs_table(threads):
     """Every workload's GPU throughput next to the CPU legs that fit the
-    box's time budget (the figures of the measured-results table)."""
+    box's time budget (the figures of the measured-results table), one row
+    at a time."""
     from sph2o_amd import scenes as SC
 
     from sph2o_amd import _lib
 
-    rows = []
     sc, x, _ = SC.w8k()
     td = tempfile.mkdtemp()
     path = os.path.join(td, "w8k.json")
@@ -361,7 +410,7 @@ def configs_table(threads):
     log("configs: W8k GPU, 2,000 steps")
     n, g = gpu_leg(sc, x, None, [(1, 2000)])
     log("configs: W8k CPU legs")
-    rows.append({"workload": "W8k: WCSPH dam break, 8,000 particles, dt = 1e-4, 2,000 steps", "particles": n,
+    yield ({"workload": "W8k: WCSPH dam break, 8,000 particles, dt = 1e-4, 2,000 steps", "particles": n,
                  "gpu": g, "cpu_reference_all_threads": ref_leg(sc, 200, threads, x=x, what="WCSPH step"),
                  "cpu_reference_1_thread": ref_leg(sc, 200, 1, x=x, what="WCSPH step"),
                  "cpu_port_all_threads": port_leg(sc, 200, threads, x, xb=xb)})
@@ -369,10 +418,11 @@ def configs_table(threads):
     log("configs: P4M GPU, steps 1-20 and 21-200")
     n, g = gpu_leg(sc, x, v, [(1, 20), (21, 200)])
     log("configs: P4M CPU leg")
-    rows.append({"workload": "P4M: PCISPH neighbour-search stress case, %d particles (jittered lattice + 1.5x "
+    yield ({"workload": "P4M: PCISPH neighbour-search stress case, %d particles (jittered lattice + 1.5x "
                              "cluster, Gaussian velocities)" % n, "particles": n, "gpu": g,
                  "cpu_reference_all_threads": ref_leg(sc, 2, threads, x=x, v=v, what="PCISPH step")})
-    return rows
+    log("configs: D1M flush, GPU 20 steps, reference 3 steps")
+    yield d1m_flush_leg(threads)
 
 
 def batched_gain(steps, warmup=3):
@@ -487,7 +537,7 @@ def main():
     ap.add_argument("--b64-steps", type=int, default=0,
                     help="only time this many steps of the B64 context after a marker pause (for a kernel trace)")
     ap.add_argument("--configs", action="store_true",
-                    help="print the W8k and P4M GPU / CPU figures instead of the bench line")
+                    help="print the W8k, P4M and flush-D1M GPU / CPU figures instead of the bench line")
     ap.add_argument("--kernel-busy", metavar="TRACE_DIR",
                     help="summarise a rocprofv3 kernel trace of --b64-steps (no GPU use)")
     args = ap.parse_args()

@@ -204,18 +204,19 @@ def w8k():
     return sc, jitter(x, 0.01 * 2 * r, 1), None
 
 
-def d1m(n=100, surface_tension=0.1):
+def d1m(n=100, surface_tension=0.1, gap=None):
     """DFSPH dam break: n^3 fluid at r = 0.005 in a 4 x 2 x 2 m box (n=100: 1M).
 
     The block sits in a corner one lattice spacing s = 2r away from the three
-    walls, so its first layer is 1.5 s from the boundary particles. Flush
-    against the walls (first layer at r) the reference itself diverges: its
-    first step stops at 100 density iterations with 1.07% error and dt
-    collapses to 4.7e-5 s in step 2."""
+    walls (gap=None), so its first layer is 1.5 s from the boundary particles.
+    Flush against the walls (gap=0: first layer at r) the reference itself
+    diverges: its first step stops at 100 density iterations with 1.07% error
+    and dt collapses to 4.7e-5 s in step 2."""
     r = 0.005
     s = 2 * r
+    g = s if gap is None else gap
     L = n * s
-    sc = scene_dict(2, r, [((s, s, s), (s + L, s + L, s + L))], [((0, 0, 0), (4.0 * L, 2.0 * L, 2.0 * L))], dt=1e-3,
+    sc = scene_dict(2, r, [((g, g, g), (g + L, g + L, g + L))], [((0, 0, 0), (4.0 * L, 2.0 * L, 2.0 * L))], dt=1e-3,
                     max_dt=1e-3, eta=1e-4, eta_v=1e-3, min_it=2, max_it=100, min_it_v=1, max_it_v=100,
                     surface_tension_method=3, surface_tension=surface_tension)
     return sc, None, None

@@ -383,9 +383,11 @@ def test_gpu_p4m_neighbor_sets_at_scale(tmp_path):
     cluster and N(0, 0.5^2) velocities. The cluster makes the step throw
     particles out of the box at kilometres per second (the reference, on its
     unbounded hash grid, does the same), so this is also the check of the
-    overflow buckets for cells outside the dense grid. Cell keys and fluid /
-    boundary within-support sets of 2000 random particles are exact at steps
-    0, 10 and 20."""
+    overflow buckets for cells outside the dense grid. Over the workload's
+    200 steps, every 50 steps (and at steps 10 and 20, while the cluster
+    bursts): every particle's cell key equals floor(x * fl(1/h)) in FP32 (the
+    reference's, HashTable.cpp:5), and the fluid / boundary within-support
+    sets of 2000 random particles equal the FP32 oracle's."""
     sc, x, v = SC.p4m()
     p, _, xb = _params(sc, str(tmp_path))
     ctx = Context(0)
@@ -393,13 +395,18 @@ def test_gpu_p4m_neighbor_sets_at_scale(tmp_path):
         sid = ctx.add_scene(p, x, v, xb)
         ctx.init()
         assert ctx.num_fluid(sid) == len(x) > 3_900_000
+        inv_h = np.float32(1.0) / np.float32(4 * np.float32(sc["Configuration"]["particleRadius"]))
         total = _subset_rows(ctx, sid, sc, xb, 2000, 0)
-        for k in (1, 2):
-            ctx.step(10)
+        done, outside = 0, 0
+        for k, step in enumerate((10, 20, 50, 100, 150, 200), start=1):
+            ctx.step(step - done)
+            done = step
             st = ctx.fluid_state(sid)
             assert np.isfinite(st["x"]).all()
-            outside = np.any((st["x"] < -0.1) | (st["x"] > 2.1), axis=1).sum()
+            outside = max(outside, int(np.any((st["x"] < -0.1) | (st["x"] > 2.1), axis=1).sum()))
             total += _subset_rows(ctx, sid, sc, xb, 2000, k)
+            cells = ctx.cells(sid)
+            assert np.array_equal(cells, np.floor(st["x"] * inv_h).astype(np.int64)), step
         assert outside > 1000   # the overflow buckets were exercised
         assert total > 50_000
     finally:
