@@ -62,8 +62,11 @@ static_assert(kLppOnce == 1 || kLppOnce == 2 || kLppOnce == 4 || kLppOnce == 8, 
 // launch blocks per particle block of the widest partial-writing launch
 extern "C" __host__ int sphKernelsLanesPerParticle() { return static_cast<int>(kLpp > kLppOnce ? kLpp : kLppOnce); }
 
+// List rows the search stages in LDS per workgroup (the rest are stored
+// directly). 16 rows leave room for 8 waves per SIMD; on D1M 8 / 16 / 24 / 32
+// rows gave 1.720 / 1.681 / 1.690 / 1.712 ms per step (the same lists).
 #ifndef SPH_LIST_STAGE
-#define SPH_LIST_STAGE 24
+#define SPH_LIST_STAGE 16
 #endif
 // Velocity and position of slot idx (separate arrays: interleaved (x, v)
 // records measured slower once the passes ran several lanes per particle).
