@@ -51,6 +51,25 @@ __device__ __forceinline__ float gradFactor2(const SceneConst& S, float rl) {
     return S.lW * (1.0f - q * q) / (h3 * (q * q) * rl);
 }
 
+// G and G2 together from d = |r|^2 with one reciprocal square root, in the
+// forward Jacobi passes' form (gradFactorSq): near, G = gA |r| + gB and
+// G2 = gA / |r|; far, G = gC (h - |r|)^2 / |r| and G2 = -gC (h^2 - d) / |r|^3.
+// d is clamped to 1e-30 as in the forward, so a self pair (r = 0) gets finite
+// factors (the near branch) and a term of exactly zero; q > 1 gets 0.
+__device__ __forceinline__ void gradFactors(const SceneConst& S, float d, float& G, float& G2) {
+    const float dm = fmaxf(d, 1.0e-30f);
+    const float rinv = __builtin_amdgcn_rsqf(dm);
+    const float rl = dm * rinv;
+    const float q = rl * S.invh;
+    const float t = S.h - rl;
+    const bool near = q <= 0.5f, in = !(q > 1.0f);
+    const float gNear = __fmaf_rn(S.gA, rl, S.gB), g2Near = S.gA * rinv;
+    const float gFar = (S.gC * (t * t)) * rinv;
+    const float g2Far = (-S.gC * (S.h * S.h - dm)) * (rinv * rinv * rinv);
+    G = in ? (near ? gNear : gFar) : 0.0f;
+    G2 = in ? (near ? g2Near : g2Far) : 0.0f;
+}
+
 // dC/dr of the cohesion kernel in the reference's form (factor 2 inside h/2,
 // the unscaled constant has no derivative); branch decided on d as forward.
 __device__ __forceinline__ float cohesionDeriv(const SceneConst& S, float d, float rl) {
@@ -394,15 +413,17 @@ __global__ void __launch_bounds__(kBlock) kAdjVelocity(AdjView a, int mode) {
                        __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rV, off, 0, 0))};
         },
         [&](const KXV& q) {
+            // a closed gate (|k_i + k_j| <= eps) adds exact zeros: the same sums
+            // as skipping the pair, without a branch per pair
             const float ks = ki + q.k;
-            if (!(fabsf(ks) > S.eps)) return;
+            const bool on = fabsf(ks) > S.eps;
             const F3 r = xi - f3(q.x);
-            const float rl = fastSqrt(dot3(r, r));
-            const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
+            float G, G2;
+            gradFactors(S, dRef(r.x, r.y, r.z), G, G2);
             const F3 du = ui - f3(q.v);
             const float rdu = dot3(r, du);
-            kb -= dt * S.m * G * rdu;
-            const float c = -dt * S.m * ks;
+            kb -= on ? dt * S.m * G * rdu : 0.0f;
+            const float c = on ? -dt * S.m * ks : 0.0f;
             axpy(gx, c * G, du);
             axpy(gx, c * G2 * rdu, r);
         });
@@ -411,8 +432,8 @@ __global__ void __launch_bounds__(kBlock) kAdjVelocity(AdjView a, int mode) {
         for (uint32_t s = sub; s < nb; s += kAdjLpp) {
             const float4 q = a.xb[a.nbrB[s * NP + i]];
             const F3 r = xi - f3(q);
-            const float rl = fastSqrt(dot3(r, r));
-            const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
+            float G, G2;
+            gradFactors(S, dRef(r.x, r.y, r.z), G, G2);
             const float rui = dot3(r, ui);
             kb -= dt * q.w * G * rui;
             const float c = -dt * q.w * ki;
@@ -471,11 +492,11 @@ __global__ void __launch_bounds__(kBlock) kAdjSource(AdjView a, int mode) {
                        __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rU, off, 0, 0))};
         },
         [&](const SXU& q) {
+            // s_i + s_j = 0 gives w = 0 and exact zeros: no branch per pair
             const float ss = si + q.s;
-            if (ss == 0.0f) return;
             const F3 r = xi - f3(q.x);
-            const float rl = fastSqrt(dot3(r, r));
-            const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
+            float G, G2;
+            gradFactors(S, dRef(r.x, r.y, r.z), G, G2);
             const F3 du = ui - f3(q.u);
             const float w = c * S.m * ss;
             axpy(gu, w * G, r);
@@ -487,8 +508,8 @@ __global__ void __launch_bounds__(kBlock) kAdjSource(AdjView a, int mode) {
         for (uint32_t s = sub; s < nb; s += kAdjLpp) {
             const float4 q = a.xb[a.nbrB[s * NP + i]];
             const F3 r = xi - f3(q);
-            const float rl = fastSqrt(dot3(r, r));
-            const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
+            float G, G2;
+            gradFactors(S, dRef(r.x, r.y, r.z), G, G2);
             const float w = c * q.w * si;
             axpy(gu, w * G, r);
             axpy(gx, w * G, ui);

@@ -281,6 +281,9 @@ class Engine {
         if (evStart) (void)hipEventDestroy(evStart);
         if (evStop) (void)hipEventDestroy(evStop);
         if (evSync) (void)hipEventDestroy(evSync);
+        if (evAdjDone) (void)hipEventDestroy(evAdjDone);
+        if (evRecDone) (void)hipEventDestroy(evRecDone);
+        if (adjStream) (void)hipStreamDestroy(adjStream);
         if (ownStream && stream) (void)hipStreamDestroy(stream);
     }
     void releasePinned() {
@@ -1101,9 +1104,11 @@ class Engine {
             }
             return;
         }
-        HIPCHK(hipEventRecord(pr.first, stream));
+        // the adjoint passes run on adjS (the reverse sweep's second stream)
+        const hipStream_t es = (kid >= K_ADJ_SEED && kid <= K_ADJ_SOURCE && adjS) ? adjS : stream;
+        HIPCHK(hipEventRecord(pr.first, es));
         HIPCHK(launch());
-        HIPCHK(hipEventRecord(pr.second, stream));
+        HIPCHK(hipEventRecord(pr.second, es));
     }
     // per-scene combine of the preceding launch's block partials, then the
     // loop-control or time-step epilogue
@@ -1555,6 +1560,20 @@ class Engine {
     };
     Recording rec;
     Tape tape;
+    // The reverse sweep runs two segments at once: the adjoint passes of one
+    // segment read its tape (tapeRd) on adjStream while the forward recomputes
+    // the segment before it into `tape` on the engine's stream; the tapes swap
+    // at each segment boundary. adjTape is the tape adjView / adjPass read and
+    // adjS the stream the adjoint passes go to (the engine's stream when the
+    // sweep has one segment or no room for the second tape). The adjoint
+    // passes read scene constants and dt from their own copies (dScenesAdj,
+    // dStateAdj), which the concurrent forward never writes.
+    Tape tapeRd;
+    const Tape* adjTape = &tape;
+    hipStream_t adjStream = nullptr, adjS = nullptr;
+    hipEvent_t evAdjDone = nullptr, evRecDone = nullptr;
+    DevBuf<SceneConst> dScenesAdj;
+    DevBuf<SceneState> dStateAdj;
     uint64_t stateEpoch = 0;  // bumped by every call that changes the state outside a recording
     DevBuf<float4> adjX, adjV, aBar, xBar, vBar, nBar, sgBar;
     DevBuf<float> rhoBar, alphaBar, kBar, sBar, adjKws, adjKvws, adjRho, adjAlpha;
@@ -1884,16 +1903,17 @@ class Engine {
         HIPCHK(hipStreamSynchronize(stream));
     }
 
-    void tapeAlloc(int steps, int nit, int halves) {
+    void tapeAlloc(int steps, int nit, int halves) { tapeAllocInto(tape, steps, nit, halves, capF, capB); }
+    void tapeAllocInto(Tape& tp, int steps, int nit, int halves, uint32_t cF, uint32_t cB) {
         const size_t k = static_cast<size_t>(steps), np = NP, it = static_cast<size_t>(nit);
-        tape.steps = steps;
-        tape.nit = nit;
-        tape.halves = halves;
-        tape.np = np;
-        tape.capF = capF;
-        tape.capB = capB;
+        tp.steps = steps;
+        tp.nit = nit;
+        tp.halves = halves;
+        tp.np = np;
+        tp.capF = cF;
+        tp.capB = cB;
         for (int hf = 0; hf < 2; ++hf) {
-            HalfTape& t = tape.h[hf];
+            HalfTape& t = tp.h[hf];
             if (hf >= halves) {
                 t = HalfTape{};
                 continue;
@@ -1901,12 +1921,21 @@ class Engine {
             for (auto* b : {&t.x, &t.v, &t.n}) b->alloc(k * np);
             for (auto* b : {&t.rho, &t.p, &t.alpha}) b->alloc(k * np);
             for (auto* b : {&t.id, &t.cntF, &t.cntB}) b->alloc(k * np);
-            t.nbrF.alloc(k * capF * np);
-            t.nbrB.alloc(k * capB * np);
+            t.nbrF.alloc(k * cF * np);
+            t.nbrB.alloc(k * cB * np);
             t.u.alloc(k * std::max<size_t>(it, 1) * np);
             t.k.alloc(k * (it + 1) * np);
             t.aux.alloc(k * (it + 1) * np);
         }
+    }
+    // Device bytes of a tape of these dimensions (tapeAllocInto's arrays).
+    static size_t tapeBytes(const Tape& tp) {
+        const size_t k = static_cast<size_t>(tp.steps), np = tp.np, it = static_cast<size_t>(tp.nit);
+        const size_t perHalf = k * np *
+                               (3 * sizeof(float4) + 3 * sizeof(float) + 3 * sizeof(uint32_t) +
+                                (static_cast<size_t>(tp.capF) + tp.capB) * sizeof(uint32_t) +
+                                std::max<size_t>(it, 1) * sizeof(float4) + 2 * (it + 1) * sizeof(float));
+        return perHalf * static_cast<size_t>(tp.halves);
     }
     // The list capacity grew while a segment was recorded: widen the tape's
     // list blocks, keeping the steps already recorded (entry s of particle i
@@ -1974,9 +2003,9 @@ class Engine {
     // half: a DFSPH step's first half runs on the search its predecessor's
     // second half ran on, so for slot > 0 that is recorded once, in the
     // second half of slot - 1 (with the densities and alpha it computed).
-    std::pair<const HalfTape*, size_t> searchHalf(int slot, int half) const {
-        if (method == SPH_DFSPH && half == 0 && slot > 0) return {&tape.h[1], static_cast<size_t>(slot - 1)};
-        return {&tape.h[half], static_cast<size_t>(slot)};
+    std::pair<const HalfTape*, size_t> searchHalf(const Tape& tp, int slot, int half) const {
+        if (method == SPH_DFSPH && half == 0 && slot > 0) return {&tp.h[1], static_cast<size_t>(slot - 1)};
+        return {&tp.h[half], static_cast<size_t>(slot)};
     }
     // stepWCSPH with the tape of step `slot` of the current segment.
     void stepWCSPHRecord(int slot) {
@@ -2151,15 +2180,16 @@ class Engine {
     }
     AdjView adjView(int slot, int half) {
         AdjView a{};
-        a.scenes = dScenes.p;
-        a.state = dState.p;
+        a.scenes = dScenesAdj.p;
+        a.state = dStateAdj.p;
         a.blockScene = dBlockScene.p;
         a.numScenes = static_cast<int32_t>(scenes.size());
         a.numBlocks = numBlocks;
         a.NP = static_cast<uint32_t>(NP);
-        const HalfTape& t = tape.h[half];
+        const Tape& tp = *adjTape;
+        const HalfTape& t = tp.h[half];
         const size_t o = static_cast<size_t>(slot) * NP;
-        const auto sh = searchHalf(slot, half);
+        const auto sh = searchHalf(tp, slot, half);
         const HalfTape& ts = *sh.first;
         const size_t os = sh.second * NP;
         a.x = ts.x.p + os;
@@ -2171,10 +2201,10 @@ class Engine {
         a.id = ts.id.p + os;
         a.cntF = ts.cntF.p + os;
         a.cntB = ts.cntB.p + os;
-        a.nbrF = ts.nbrF.p + sh.second * tape.capF * NP;
-        a.nbrB = ts.nbrB.p + sh.second * tape.capB * NP;
-        a.capF = tape.capF;
-        a.capB = tape.capB;
+        a.nbrF = ts.nbrF.p + sh.second * tp.capF * NP;
+        a.nbrB = ts.nbrB.p + sh.second * tp.capB * NP;
+        a.capF = tp.capF;
+        a.capB = tp.capB;
         a.xb = xb.p;
         a.adjX = adjX.p;
         a.adjV = adjV.p;
@@ -2197,52 +2227,75 @@ class Engine {
     }
     // point a DFSPH view at iteration `it` of its half (warm start: it = nit)
     void adjPass(AdjView& a, int slot, int half, int it) {
-        const HalfTape& t = tape.h[half];
-        const size_t ko = (static_cast<size_t>(slot) * (tape.nit + 1) + it) * NP;
+        const Tape& tp = *adjTape;
+        const HalfTape& t = tp.h[half];
+        const size_t ko = (static_cast<size_t>(slot) * (tp.nit + 1) + it) * NP;
         a.kk = t.k.p + ko;
         a.aux = t.aux.p + ko;
-        a.u = it < tape.nit ? t.u.p + (static_cast<size_t>(slot) * tape.nit + it) * NP : nullptr;
+        a.u = it < tp.nit ? t.u.p + (static_cast<size_t>(slot) * tp.nit + it) * NP : nullptr;
     }
     void adjointStep(int slot) {
         if (method == SPH_DFSPH) return adjointStepDFSPH(slot);
         const AdjView a = adjView(slot, 0);
-        timed(K_ADJ_SEED, [&] { return launchAdjSeed(a, stream); });
-        timed(K_ADJ_FORCES, [&] { return launchAdjForces(a, stream); });
-        if (anyST()) timed(K_ADJ_NORMALS, [&] { return launchAdjNormals(a, stream); });
-        timed(K_ADJ_DENSITY, [&] { return launchAdjDensity(a, stream); });
-        timed(K_ADJ_FINISH, [&] { return launchAdjFinish(a, stream); });
+        timed(K_ADJ_SEED, [&] { return launchAdjSeed(a, adjS); });
+        timed(K_ADJ_FORCES, [&] { return launchAdjForces(a, adjS); });
+        if (anyST()) timed(K_ADJ_NORMALS, [&] { return launchAdjNormals(a, adjS); });
+        timed(K_ADJ_DENSITY, [&] { return launchAdjDensity(a, adjS); });
+        timed(K_ADJ_FINISH, [&] { return launchAdjFinish(a, adjS); });
     }
     // Reverse of one DFSPH step: the second half (divergence solve, densities
     // and alpha at x_{n+1}), then the first half (density solve, prediction,
     // forces) at x_n.
     void adjointStepDFSPH(int slot) {
-        const int nit = tape.nit;
+        const int nit = adjTape->nit;
         AdjView a = adjView(slot, 1);
-        timed(K_ADJ_SEED, [&] { return launchAdjHalfStart(a, 1, stream); });
+        timed(K_ADJ_SEED, [&] { return launchAdjHalfStart(a, 1, adjS); });
         for (int it = nit - 1; it >= 0; --it) {
             adjPass(a, slot, 1, it);
-            timed(K_ADJ_VELOCITY, [&] { return launchAdjVelocity(a, kAdjDivergenceLoop, stream); });
-            timed(K_ADJ_SOURCE, [&] { return launchAdjSource(a, kAdjDivergenceLoop, stream); });
+            timed(K_ADJ_VELOCITY, [&] { return launchAdjVelocity(a, kAdjDivergenceLoop, adjS); });
+            timed(K_ADJ_SOURCE, [&] { return launchAdjSource(a, kAdjDivergenceLoop, adjS); });
         }
         adjPass(a, slot, 1, nit);
-        timed(K_ADJ_VELOCITY, [&] { return launchAdjVelocity(a, kAdjDivergenceWarm, stream); });
-        timed(K_ADJ_DENSITY, [&] { return launchAdjAlphaA(a, stream); });
-        timed(K_ADJ_DENSITY, [&] { return launchAdjAlphaB(a, stream); });
-        timed(K_ADJ_SEED, [&] { return launchAdjHalfEnd(a, 1, stream); });
+        timed(K_ADJ_VELOCITY, [&] { return launchAdjVelocity(a, kAdjDivergenceWarm, adjS); });
+        timed(K_ADJ_DENSITY, [&] { return launchAdjAlphaA(a, adjS); });
+        timed(K_ADJ_DENSITY, [&] { return launchAdjAlphaB(a, adjS); });
+        timed(K_ADJ_SEED, [&] { return launchAdjHalfEnd(a, 1, adjS); });
         a = adjView(slot, 0);
-        timed(K_ADJ_SEED, [&] { return launchAdjHalfStart(a, 0, stream); });
+        timed(K_ADJ_SEED, [&] { return launchAdjHalfStart(a, 0, adjS); });
         for (int it = nit - 1; it >= 0; --it) {
             adjPass(a, slot, 0, it);
-            timed(K_ADJ_VELOCITY, [&] { return launchAdjVelocity(a, kAdjDensityLoop, stream); });
-            timed(K_ADJ_SOURCE, [&] { return launchAdjSource(a, kAdjDensityLoop, stream); });
+            timed(K_ADJ_VELOCITY, [&] { return launchAdjVelocity(a, kAdjDensityLoop, adjS); });
+            timed(K_ADJ_SOURCE, [&] { return launchAdjSource(a, kAdjDensityLoop, adjS); });
         }
         adjPass(a, slot, 0, nit);
-        timed(K_ADJ_VELOCITY, [&] { return launchAdjVelocity(a, kAdjDensityWarm, stream); });
-        timed(K_ADJ_SEED, [&] { return launchAdjPredict(a, stream); });
-        if (anyVisc() || anyST()) timed(K_ADJ_FORCES, [&] { return launchAdjForces2(a, 0, stream); });
-        if (anyST()) timed(K_ADJ_NORMALS, [&] { return launchAdjNormals(a, stream); });
-        timed(K_ADJ_FINISH, [&] { return launchAdjFinish(a, stream); });
-        timed(K_ADJ_SEED, [&] { return launchAdjHalfEnd(a, 0, stream); });
+        timed(K_ADJ_VELOCITY, [&] { return launchAdjVelocity(a, kAdjDensityWarm, adjS); });
+        timed(K_ADJ_SEED, [&] { return launchAdjPredict(a, adjS); });
+        if (anyVisc() || anyST()) timed(K_ADJ_FORCES, [&] { return launchAdjForces2(a, 0, adjS); });
+        if (anyST()) timed(K_ADJ_NORMALS, [&] { return launchAdjNormals(a, adjS); });
+        timed(K_ADJ_FINISH, [&] { return launchAdjFinish(a, adjS); });
+        timed(K_ADJ_SEED, [&] { return launchAdjHalfEnd(a, 0, adjS); });
+    }
+    // The reverse sweep's second stream, its events and a second tape of the
+    // recorded tape's dimensions (in tapeRd); false, for a sweep on one stream,
+    // where device memory cannot hold the second tape.
+    bool prepareOverlap() {
+        if (!adjStream) {
+            HIPCHK(hipStreamCreateWithFlags(&adjStream, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&evAdjDone, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&evRecDone, hipEventDisableTiming));
+        }
+        Tape& sp = tapeRd;
+        if (sp.steps == tape.steps && sp.np == tape.np && sp.nit == tape.nit && sp.halves == tape.halves &&
+            sp.capF == tape.capF && sp.capB == tape.capB)
+            return true;
+        HIPCHK(hipStreamSynchronize(stream));
+        sp = Tape{};
+        size_t freeB = 0, totalB = 0;
+        HIPCHK(hipMemGetInfo(&freeB, &totalB));
+        const size_t need = tapeBytes(tape);
+        if (need + need / 8 + (static_cast<size_t>(1) << 30) > freeB) return false;
+        tapeAllocInto(sp, tape.steps, tape.nit, tape.halves, tape.capF, tape.capB);
+        return true;
     }
     void adjointRun() {
         if (!rec.valid || rec.epoch != stateEpoch) throw SphError{SPH_E_STATE, "no recorded rollout for this state"};
@@ -2268,22 +2321,23 @@ class Engine {
         for (auto* b : {&adjKws, &adjKvws, &adjRho, &adjAlpha}) HIPCHK(hipMemset(b->p, 0, NP * sizeof(float)));
         HIPCHK(hipEventRecord(evStart, stream));
         launchesInStep = 0;
-        for (int seg = rec.segments - 1; seg >= 0; --seg) {
-            const int first = seg * rec.every, count = std::min(rec.every, rec.nsteps - first);
-            if (rec.tapeSeg != seg) {
-                restoreCheckpoint(rec.ckpt[static_cast<size_t>(seg)]);
-                for (int j = 0; j < count; ++j) stepRecord(j);
-                rec.tapeSeg = seg;
-            }
-            for (int j = count - 1; j >= 0; --j) adjointStep(j);
-        }
-        if (method == SPH_DFSPH) {
-            // rho and alpha of the first step came from the initial positions
-            const AdjView a = adjView(0, 0);
-            timed(K_ADJ_SEED, [&] { return launchAdjHalfStart(a, 2, stream); });
-            timed(K_ADJ_DENSITY, [&] { return launchAdjAlphaA(a, stream); });
-            timed(K_ADJ_DENSITY, [&] { return launchAdjAlphaB(a, stream); });
-            timed(K_ADJ_SEED, [&] { return launchAdjHalfEnd(a, 2, stream); });
+        // the adjoint passes' own copies of the scene constants and states
+        if (dScenesAdj.n != scenes.size()) dScenesAdj.alloc(scenes.size());
+        if (dStateAdj.n != scenes.size()) dStateAdj.alloc(scenes.size());
+        copyD(dScenesAdj.p, dScenes.p, scenes.size() * sizeof(SceneConst));
+        copyD(dStateAdj.p, dState.p, scenes.size() * sizeof(SceneState));
+        const bool overlap = rec.segments > 1 && prepareOverlap();
+        try {
+            sweep(overlap);
+        } catch (...) {
+            // leave no adjoint work in flight and no tape marked as holding a
+            // segment it may not hold
+            if (adjStream) (void)hipStreamSynchronize(adjStream);
+            (void)hipStreamSynchronize(stream);
+            adjS = stream;
+            adjTape = &tape;
+            rec.tapeSeg = -1;
+            throw;
         }
         HIPCHK(hipEventRecord(evStop, stream));
         HIPCHK(hipEventSynchronize(evStop));
@@ -2291,8 +2345,91 @@ class Engine {
         HIPCHK(hipEventElapsedTime(&ms, evStart, evStop));
         lastStepMs = ms;
         collectProfile();
-        hx = download(adjX.p, NP);
-        hv = download(adjV.p, NP);
+        finishAdjoint();
+    }
+    // The reverse sweep over every segment (see adjTape / adjS above); with
+    // `overlap` the adjoint passes run on adjStream beside the recomputes.
+    void sweep(bool overlap) {
+        if (!overlap) {
+            adjS = stream;
+            adjTape = &tape;
+            for (int seg = rec.segments - 1; seg >= 0; --seg) {
+                const int first = seg * rec.every, count = std::min(rec.every, rec.nsteps - first);
+                if (rec.tapeSeg != seg) {
+                    restoreCheckpoint(rec.ckpt[static_cast<size_t>(seg)]);
+                    for (int j = 0; j < count; ++j) stepRecord(j);
+                    rec.tapeSeg = seg;
+                }
+                for (int j = count - 1; j >= 0; --j) adjointStep(j);
+            }
+        } else {
+            // Segment seg's adjoint (adjStream, from tapeRd) runs beside the
+            // recompute of segment seg - 1 (the engine's stream, into `tape`).
+            // The recompute waits until the adjoint of seg + 1 has finished
+            // reading `tape`; the adjoint of seg - 1 waits for its recompute.
+            // A few adjoint steps are queued before the recompute's restore,
+            // whose host waits would otherwise leave adjStream idle.
+            const int last = rec.segments - 1;
+            if (rec.tapeSeg != last) {   // a second sweep over the same recording
+                restoreCheckpoint(rec.ckpt[static_cast<size_t>(last)]);
+                for (int j = 0; j < rec.nsteps - last * rec.every; ++j) stepRecord(j);
+                flushCopies();
+                rec.tapeSeg = last;
+            }
+            adjS = adjStream;
+            adjTape = &tapeRd;
+            std::swap(tape, tapeRd);   // tapeRd: the last segment
+            rec.tapeSeg = -1;          // `tape` holds no segment until a recompute fills it
+            HIPCHK(hipEventRecord(evRecDone, stream));   // the set-up above
+            HIPCHK(hipStreamWaitEvent(adjStream, evRecDone, 0));
+            HIPCHK(hipEventRecord(evAdjDone, adjStream));
+            constexpr int kLead = 2;
+            for (int seg = rec.segments - 1; seg >= 0; --seg) {
+                const int count = std::min(rec.every, rec.nsteps - seg * rec.every);
+                const int countPrev = seg > 0 ? rec.every : 0;   // segments before the last are whole
+                int j = count - 1, k = 0;
+                for (int l = 0; l < kLead && j >= 0; ++l) adjointStep(j--);
+                if (countPrev) {
+                    HIPCHK(hipStreamWaitEvent(stream, evAdjDone, 0));
+                    restoreCheckpoint(rec.ckpt[static_cast<size_t>(seg - 1)]);
+                }
+                while (j >= 0 || k < countPrev) {
+                    if (j >= 0) adjointStep(j--);
+                    if (k < countPrev) stepRecord(k++);
+                }
+                HIPCHK(hipEventRecord(evAdjDone, adjStream));
+                if (countPrev) {
+                    flushCopies();
+                    HIPCHK(hipEventRecord(evRecDone, stream));
+                    HIPCHK(hipStreamWaitEvent(adjStream, evRecDone, 0));
+                    std::swap(tape, tapeRd);   // tapeRd: segment seg - 1; tape: segment seg
+                    rec.tapeSeg = seg;
+                }
+            }
+        }
+        if (method == SPH_DFSPH) {
+            // rho and alpha of the first step came from the initial positions
+            const AdjView a = adjView(0, 0);
+            timed(K_ADJ_SEED, [&] { return launchAdjHalfStart(a, 2, adjS); });
+            timed(K_ADJ_DENSITY, [&] { return launchAdjAlphaA(a, adjS); });
+            timed(K_ADJ_DENSITY, [&] { return launchAdjAlphaB(a, adjS); });
+            timed(K_ADJ_SEED, [&] { return launchAdjHalfEnd(a, 2, adjS); });
+        }
+        if (overlap) {
+            // the engine's stream continues after the adjoint passes; the tape
+            // of segment 0 becomes `tape` again (sph_recorded_field reads it)
+            HIPCHK(hipEventRecord(evAdjDone, adjStream));
+            HIPCHK(hipStreamWaitEvent(stream, evAdjDone, 0));
+            std::swap(tape, tapeRd);
+            rec.tapeSeg = 0;
+        }
+        adjS = stream;
+        adjTape = &tape;
+    }
+    // Gradients to the host (original order), the context back where the
+    // recorded rollout ended, and the finiteness check.
+    void finishAdjoint() {
+        std::vector<float4> hx = download(adjX.p, NP), hv = download(adjV.p, NP);
         gradCoef = download(adjAcc.p, 2 * scenes.size());
         gradX0.assign(scenes.size(), {});
         gradV0.assign(scenes.size(), {});
@@ -2337,7 +2474,7 @@ class Engine {
         if (field <= 3 && (pass < 0 || pass > tape.nit)) throw SphError{SPH_E_ARG, "pass outside 0..iterations"};
         const HalfTape& t = tape.h[half];
         const size_t slot = static_cast<size_t>(step % rec.every);
-        const auto sh = searchHalf(static_cast<int>(slot), half);
+        const auto sh = searchHalf(tape, static_cast<int>(slot), half);
         auto ids = download(sh.first->id.p + sh.second * NP + sc.fBegin, sc.nf);
         if (field == 4) {
             auto hx = download(sh.first->x.p + sh.second * NP + sc.fBegin, sc.nf);

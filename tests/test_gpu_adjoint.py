@@ -288,7 +288,10 @@ def test_adjoint_dfsph_long_rollout(tmp_path):
 def test_checkpoint_interval_does_not_change_the_gradient(tmp_path, method):
     """Segment recomputation from checkpoints is bit-exact: the gradients and
     the final state are identical for every checkpoint interval, and the
-    recorded forward equals plain stepping."""
+    recorded forward equals plain stepping. One segment (every = 12) runs the
+    sweep on one stream; several segments run each segment's adjoint passes
+    beside the recompute of the segment before it, on two streams, and a
+    second sweep over the same recording gives the same gradients again."""
     case = Case(str(tmp_path), _scene(True, True, method=method))
     try:
         c, sid = case.ctx, case.sid
@@ -300,6 +303,12 @@ def test_checkpoint_interval_does_not_change_the_gradient(tmp_path, method):
             xT, res = case.gpu_grad(12, every)
             st = c.fluid_state(sid)
             assert np.array_equal(st["x"], plain["x"]) and np.array_equal(st["v"], plain["v"])
+            if every == 5:
+                seed = (2.0 * (xT - case.y) / len(xT)).astype(np.float32)
+                again = c.adjoint({sid: (seed, None)})[sid]
+                for k in ("x0", "v0"):
+                    assert np.array_equal(again[k], res[k]), ("second sweep", k)
+                assert again["viscosity"] == res["viscosity"]
             if ref is None:
                 ref = res
                 continue
