@@ -120,6 +120,14 @@ def measured_valu(kernel):
     return None
 
 
+def search_info(ctx):
+    """The search's lanes per particle in a context and the fastest list build
+    its tuning rounds saw per count (ms; None for a count not timed)."""
+    lanes, tuned, ms = ctx.search_lanes()
+    return {"lanes": lanes, "tuned": tuned,
+            "build_ms": {str(k): (float(v) if np.isfinite(v) else None) for k, v in ms.items()}}
+
+
 def pair_throughput(ctx, sid, its, steps_per_s):
     """True neighbour pairs evaluated per second (every pass's sum over i of
     |S_i|, fluid and boundary, at the mean list lengths of the final state)
@@ -340,7 +348,8 @@ def gpu_leg(sc, x, v, windows):
         done = b
         its = float(np.mean([row[sid].iterations for row in st]))
         out.append({"steps": "%d-%d" % (a, b), "value": n * (b - a + 1) / secs, "unit": "particle-steps/s",
-                    "ms_per_step": 1000.0 * secs / (b - a + 1), "iterations_per_step": its})
+                    "ms_per_step": 1000.0 * secs / (b - a + 1), "iterations_per_step": its,
+                    "search_lanes": search_info(ctx)})
     ctx.close()
     return n, out
 
@@ -425,9 +434,11 @@ def configs_table(threads):
     yield d1m_flush_leg(threads)
 
 
-def batched_gain(steps, warmup=3):
+def batched_gain(steps, warmup=12):
     """B64: 64 DFSPH scenes (2k..200k particles) advanced together in one
-    context vs the same scenes as 64 sequential single-scene contexts."""
+    context vs the same scenes as 64 sequential single-scene contexts. The
+    warmup steps cover each context's search tuning (12 timed searches: the
+    one at init and one per step)."""
     from sph2o_amd import scenes as SC
     from sph2o_amd.simulation import Context
 
@@ -449,9 +460,11 @@ def batched_gain(steps, warmup=3):
     batch.synchronize()
     tb = time.perf_counter() - t0
     its_b = float(np.mean([[x.iterations + x.iterations_v for x in row] for row in st]))
+    search_b = search_info(batch)
     batch.close()
     ts = 0.0
     its_s = []
+    lanes_s = {}
     for p in paths:
         c = Context(0)
         c.load_scene_json(p)
@@ -463,12 +476,15 @@ def batched_gain(steps, warmup=3):
         c.synchronize()
         ts += time.perf_counter() - t0
         its_s.append(np.mean([row[0].iterations + row[0].iterations_v for row in st]))
+        ln = str(c.search_lanes()[0])
+        lanes_s[ln] = lanes_s.get(ln, 0) + 1
         c.close()
     b_ps = total * steps / tb
     s_ps = total * steps / ts
     return {"scenes": len(paths), "fluid_particles": total, "steps": steps, "warmup": warmup,
             "batched_particle_steps_per_s": b_ps, "sequential_particle_steps_per_s": s_ps, "gain": b_ps / s_ps,
-            "mean_iterations_per_scene_step": {"batched": its_b, "sequential": float(np.mean(its_s))}}
+            "mean_iterations_per_scene_step": {"batched": its_b, "sequential": float(np.mean(its_s))},
+            "search_lanes": {"batched": search_b, "sequential_contexts_by_lanes": lanes_s}}
 
 
 def b64_timed(steps):
@@ -610,6 +626,7 @@ def main():
     traffic = measured_traffic(dominant)
     valu = measured_valu(dominant)
     valu_frac = valu / (avg_ms * 1e-3) / VALU_PEAK_WAVE_INSTS if (valu and klaunch) else None
+    search = search_info(ctx)
 
     ctx.close()
     if args.calibrate_copy > 0:
@@ -651,6 +668,7 @@ def main():
         "density_error_ok": rho_err <= 1e-3,
         "iterations_per_step": {"density": float(its[:, 0].mean()), "divergence": float(its[:, 1].mean())},
         "neighbour_pairs": pairs,
+        "search_lanes": search,
         "sim_time_s": float(stats[-1][sid].t),
         "roofline": {"kernel": dominant, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,

@@ -264,6 +264,19 @@ const char* sph_kernel_name(int kernel_id);
 int sph_kernel_stats(const sph_ctx* ctx, int kernel_id, double* total_ms, int64_t* launches);
 int sph_reset_kernel_stats(sph_ctx* ctx);
 
+/* Lanes per particle of the neighbour search's list build (no reference
+ * counterpart: a launch shape of this library's search, which builds the same
+ * lists for every value, the lists HashTable::neighborhoodSearch gives,
+ * HashTable.cpp:23-86). lanes 1, 2, 4 or 8 pins the count; 0 (the default)
+ * lets the context time each count on its first 12 searches (three rounds of
+ * 1, 2, 4, 8) and keep the fastest. */
+int sph_set_search_lanes(sph_ctx* ctx, int lanes);
+/* The count the search uses; tuned = 1 once the timing rounds have picked it
+ * (0 while they run or when the count is pinned); trial_ms (4 floats, may be
+ * NULL) the fastest build seen with 1, 2, 4 and 8 lanes, in ms (infinity for
+ * a count not timed). */
+int sph_get_search_lanes(const sph_ctx* ctx, int* lanes, int* tuned, float* trial_ms);
+
 /* ---- Gradients of WCSPH and DFSPH rollouts. The reference has no
  * derivatives: these entry points have no counterpart there; they serve
  * optimisation / inverse-design callers and the torch.autograd.Function in

@@ -33,6 +33,7 @@ EXPORTS = [
     "sph_recorded_field", "sph_add_emitter", "sph_num_active", "sph_save_state", "sph_load_state",
     "sph_add_cube_boundary", "sph_viewer_feed", "sph_dump_cells_noalloc",
     "sph_set_boundary_normals", "sph_scene_boundary_normals_from_json", "sph_get_boundary_state",
+    "sph_set_search_lanes", "sph_get_search_lanes",
 ]
 
 
@@ -132,6 +133,8 @@ def load():
                                               P(ctypes.c_int64), P(ctypes.c_int32), ctypes.c_int64]),
         "sph_last_step_timing": (ctypes.c_int, [vp, P(ctypes.c_double), P(ctypes.c_int64)]),
         "sph_set_profiling": (ctypes.c_int, [vp, ctypes.c_int]),
+        "sph_set_search_lanes": (ctypes.c_int, [vp, ctypes.c_int]),
+        "sph_get_search_lanes": (ctypes.c_int, [vp, P(ctypes.c_int), P(ctypes.c_int), fp]),
         "sph_kernel_count": (ctypes.c_int, []),
         "sph_kernel_name": (ctypes.c_char_p, [ctypes.c_int]),
         "sph_kernel_stats": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_int64)]),

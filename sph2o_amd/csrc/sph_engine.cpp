@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -259,6 +260,18 @@ class Engine {
     size_t stateBlockBytes = 0;
     hipEvent_t evStart = nullptr, evStop = nullptr, evSync = nullptr;
     int64_t syncsInStep = 0;
+    // Lanes per particle of the search (buildLists): searchLanesFixed > 0 pins
+    // the count (sph_set_search_lanes), 0 lets the tuner pick searchLanes.
+    int searchLanesFixed = 0, searchLanes = 1;
+    struct SearchTuner {
+        static constexpr int kCands = 4, kRounds = 3;
+        static constexpr int kLanes[kCands] = {1, 2, 4, 8};
+        int trial = 0, pendingIdx = 0;
+        bool done = false, pending = false;
+        float best[kCands] = {std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity(),
+                              std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity()};
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+    } tuner;
     int32_t* noAllocScratch = nullptr;   // this device's `scratch` (set at init)
 
     // profiling
@@ -281,6 +294,8 @@ class Engine {
         if (evStart) (void)hipEventDestroy(evStart);
         if (evStop) (void)hipEventDestroy(evStop);
         if (evSync) (void)hipEventDestroy(evSync);
+        if (tuner.e0) (void)hipEventDestroy(tuner.e0);
+        if (tuner.e1) (void)hipEventDestroy(tuner.e1);
         if (evAdjDone) (void)hipEventDestroy(evAdjDone);
         if (evRecDone) (void)hipEventDestroy(evRecDone);
         if (adjStream) (void)hipStreamDestroy(adjStream);
@@ -1180,8 +1195,69 @@ class Engine {
         std::swap(id.p, idAlt.p);
         if (nc.p) std::swap(nc.p, ncAlt.p);
         d = view();
-        timed(K_BUILD, [&] { return launchBuildNeighbors(d, stream); });
+        buildLists(d);
         if (tiles) timed(K_TILES, [&] { return launchTileLists(d, stream); });
+    }
+    // The list build of a search, with the lanes per particle the context
+    // uses. Until that count is settled (searchLanesFixed == 0), each of the
+    // context's first searches times one candidate with an event pair taken
+    // into the launch, in turn 1, 2, 4, 8, three rounds; the fastest time of
+    // each is kept and the count with the least is used from then on. Every
+    // count builds the same lists, so the tuning leaves the trajectory alone.
+    void buildLists(const DevView& d) {
+        tuneCollect();
+        if (searchLanesFixed || tuner.done) {
+            const int lanes = searchLanesFixed ? searchLanesFixed : searchLanes;
+            timed(K_BUILD, [&] { return launchBuildNeighbors(d, lanes, stream); });
+            return;
+        }
+        if (!tuner.e0) {
+            HIPCHK(hipEventCreate(&tuner.e0));
+            HIPCHK(hipEventCreate(&tuner.e1));
+        }
+        const int idx = tuner.trial % SearchTuner::kCands;
+        ++launchesInStep;
+        setLaunchEvents(tuner.e0, tuner.e1);
+        const hipError_t e = launchBuildNeighbors(d, SearchTuner::kLanes[idx], stream);
+        const bool used = launchEventsUsed();
+        setLaunchEvents(nullptr, nullptr);
+        HIPCHK(e);
+        tuner.pending = used;
+        tuner.pendingIdx = idx;
+    }
+    // File the timing of the last tuning launch. Every search is followed by a
+    // state readback on the same stream before the next one starts, so the
+    // launch has finished by the time this runs.
+    void tuneCollect() {
+        if (!tuner.pending) return;
+        tuner.pending = false;
+        HIPCHK(hipEventSynchronize(tuner.e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, tuner.e0, tuner.e1));
+        const int idx = tuner.pendingIdx;
+        tuner.best[idx] = std::min(tuner.best[idx], ms);
+        if (profiling && ((profileMask >> K_BUILD) & 1ull)) {
+            kernelMs[K_BUILD] += ms;
+            kernelLaunches[K_BUILD] += 1;
+        }
+        if (++tuner.trial < SearchTuner::kCands * SearchTuner::kRounds) return;
+        int best = 0;
+        for (int c = 1; c < SearchTuner::kCands; ++c)
+            if (tuner.best[c] < tuner.best[best]) best = c;
+        searchLanes = SearchTuner::kLanes[best];
+        tuner.done = true;
+    }
+    void setSearchLanes(int lanes) {
+        if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8)
+            throw SphError{SPH_E_ARG, "search lanes must be 0 (tune), 1, 2, 4 or 8"};
+        if (stream) HIPCHK(hipStreamSynchronize(stream));
+        searchLanesFixed = lanes;
+        if (lanes == 0) {   // tune afresh
+            tuner.trial = 0;
+            tuner.done = false;
+            tuner.pending = false;
+            std::fill(tuner.best, tuner.best + SearchTuner::kCands, std::numeric_limits<float>::infinity());
+        }
     }
     // Interpret the flags of the last search (hFlags must be current). Returns
     // true if the lists are valid; otherwise grows what overflowed and returns
@@ -3091,6 +3167,23 @@ int sph_set_profiling(sph_ctx* ctx, int on) {
         }
         return SPH_OK;
     });
+}
+
+int sph_set_search_lanes(sph_ctx* ctx, int lanes) {
+    return guarded(ctx, [&] {
+        ctx->eng.setSearchLanes(lanes);
+        return SPH_OK;
+    });
+}
+
+int sph_get_search_lanes(const sph_ctx* ctx, int* lanes, int* tuned, float* trial_ms) {
+    if (!ctx) return SPH_E_ARG;
+    const auto& e = ctx->eng;
+    if (lanes) *lanes = e.searchLanesFixed ? e.searchLanesFixed : e.searchLanes;
+    if (tuned) *tuned = e.searchLanesFixed ? 0 : (e.tuner.done ? 1 : 0);
+    if (trial_ms)
+        for (int c = 0; c < 4; ++c) trial_ms[c] = e.tuner.best[c];
+    return SPH_OK;
 }
 
 int sph_set_coefficients(sph_ctx* ctx, int scene, float viscosity, float surface_tension) {

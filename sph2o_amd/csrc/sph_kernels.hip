@@ -784,15 +784,16 @@ __device__ __forceinline__ void buildNeighbors(const DevView& d) {
     }
 #endif
 }
-// Lanes per particle of the search (same lists for every value). Measured on
-// the 1M-particle dam break: 1 lane 293 us per build, 2 lanes 336, 4 lanes 479
-// (cells hold ~8 candidates, so rounds of 2-4 add ballots without gathering
-// any better: one lane already walks each cell's contiguous slots).
-#ifndef SPH_LPP_BUILD
-#define SPH_LPP_BUILD 1
-#endif
-constexpr uint32_t kLppBuild = SPH_LPP_BUILD;
-__global__ void __launch_bounds__(kBlock) kBuildNeighbors(DevView d) { buildNeighbors<kLppBuild>(d); }
+// Lanes per particle of the search: 1, 2, 4 or 8, the same lists for each.
+// Where a launch fills the chip one lane is fastest (the 1M-particle dam
+// break: 1 lane 293 us per build, 2 lanes 336, 4 lanes 479; cells hold ~8
+// candidates, so rounds of 2-4 add ballots without gathering any better: one
+// lane already walks each cell's contiguous slots). A scene of a few thousand
+// particles puts one lane per particle on a few dozen CUs, where more lanes
+// finish sooner. The engine picks the count per context by timing each on
+// its first searches (Engine::buildLists).
+template <uint32_t L>
+__global__ void __launch_bounds__(kBlock) kBuildNeighbors(DevView d) { buildNeighbors<L>(d); }
 
 // ---------------------------------------------------------------- DFSPH tiles
 // The Jacobi passes of one launch block (kTileGroup = 64 particles, 4 lanes
@@ -1956,8 +1957,13 @@ hipError_t launchReorder(const DevView& d, const uint32_t* ks, const uint32_t* v
                          float* kva, uint32_t* ida, int ranges, hipStream_t st) {
     LAUNCH(kReorder, d.numBlocks, d, ks, vs, xa, va, kwa, kva, ida, ranges);
 }
-hipError_t launchBuildNeighbors(const DevView& d, hipStream_t st) {
-    LAUNCH(kBuildNeighbors, d.numBlocks * kLppBuild, d);
+hipError_t launchBuildNeighbors(const DevView& d, int lanes, hipStream_t st) {
+    switch (lanes) {
+    case 2: LAUNCH(kBuildNeighbors<2>, d.numBlocks * 2, d);
+    case 4: LAUNCH(kBuildNeighbors<4>, d.numBlocks * 4, d);
+    case 8: LAUNCH(kBuildNeighbors<8>, d.numBlocks * 8, d);
+    default: LAUNCH(kBuildNeighbors<1>, d.numBlocks, d);
+    }
 }
 hipError_t launchTileLists(const DevView& d, hipStream_t st) {
     LAUNCH(kTileLists, d.numBlocks * (kBlock / kTileGroup), d);

@@ -115,7 +115,8 @@ hipError_t launchFillRanges(uint2* r, size_t n, hipStream_t st);
 // writes them itself)
 hipError_t launchReorder(const DevView& d, const uint32_t* keysSorted, const uint32_t* valsSorted, float4* xAlt,
                          float4* vAlt, float* kwsAlt, float* kvwsAlt, uint32_t* idAlt, int ranges, hipStream_t st);
-hipError_t launchBuildNeighbors(const DevView& d, hipStream_t st);
+// lanes per particle: 1, 2, 4 or 8 (the same lists for each)
+hipError_t launchBuildNeighbors(const DevView& d, int lanes, hipStream_t st);
 // DFSPH tiles of the lists just built (d.tileSlot, d.tileCount, d.nbrT)
 hipError_t launchTileLists(const DevView& d, hipStream_t st);
 // cube boundary objects of every scene (no-op for scenes without)

@@ -287,6 +287,20 @@ class Context:
     def reset_kernel_stats(self):
         self._check(self.L.sph_reset_kernel_stats(self.h))
 
+    def set_search_lanes(self, lanes):
+        """Lanes per particle of the search's list build: 1, 2, 4 or 8 pins
+        it, 0 times each on the first searches and keeps the fastest (every
+        count builds the same lists)."""
+        self._check(self.L.sph_set_search_lanes(self.h, int(lanes)))
+
+    def search_lanes(self):
+        """(lanes in use, tuned, {lanes: fastest build ms}) of the search."""
+        lanes = ctypes.c_int()
+        tuned = ctypes.c_int()
+        ms = (ctypes.c_float * 4)()
+        self._check(self.L.sph_get_search_lanes(self.h, ctypes.byref(lanes), ctypes.byref(tuned), ms))
+        return lanes.value, bool(tuned.value), {c: float(m) for c, m in zip((1, 2, 4, 8), ms)}
+
 
 class SPHSimulation:
     """Drop-in for the reference's ``SPHSimulation`` on one scene
