@@ -120,6 +120,27 @@ def measured_valu(kernel):
     return None
 
 
+# Engine clock of the busy-cycle counters (MI355X peak engine clock)
+ENGINE_HZ = 2.4e9
+
+
+def measured_ta_busy(kernel):
+    """Busy cycles per launch of the texture-address unit (one per CU, every
+    vector memory instruction passes it) for `kernel` on this workload, from
+    the committed rocprofv3 TA_BUSY_avr summary
+    (profiles/d1m_vmem_pipeline_counters.md, lines `ta_busy <kernel> <cycles>`),
+    or None."""
+    path = os.path.join(ROOT, "profiles", "d1m_vmem_pipeline_counters.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.strip().strip("`").split()
+            if len(parts) == 3 and parts[0] == "ta_busy" and parts[1] == kernel:
+                return float(parts[2])
+    return None
+
+
 def search_info(ctx):
     """The search's lanes per particle in a context and the fastest list build
     its tuning rounds saw per count (ms; None for a count not timed)."""
@@ -626,6 +647,8 @@ def main():
     traffic = measured_traffic(dominant)
     valu = measured_valu(dominant)
     valu_frac = valu / (avg_ms * 1e-3) / VALU_PEAK_WAVE_INSTS if (valu and klaunch) else None
+    ta = measured_ta_busy(dominant)
+    ta_frac = ta / (avg_ms * 1e-3 * ENGINE_HZ) if (ta and klaunch) else None
     search = search_info(ctx)
 
     ctx.close()
@@ -676,7 +699,11 @@ def main():
                      # SQ_INSTS_VALU per launch (counter run of this workload) over this run's launch time,
                      # against the VALU issue peak
                      "valu_issue": {"wave_insts_per_launch": valu, "peak_wave_insts_per_s": VALU_PEAK_WAVE_INSTS,
-                                    "frac": valu_frac}},
+                                    "frac": valu_frac},
+                     # TA_BUSY_avr per launch (counter run of this workload) over this run's launch time at
+                     # the engine clock: the vector-memory address unit, the busiest unit of the neighbour
+                     # passes (scattered 64-lane gathers, ~37 L1 lines each)
+                     "vmem": {"ta_busy_cycles_per_launch": ta, "engine_hz": ENGINE_HZ, "ta_busy_frac": ta_frac}},
         "batched_gain": batched["gain"] if batched else None,
         "batched": batched,
         # (recording forward + reverse sweep) / plain forward, the larger of the two methods
