@@ -2370,7 +2370,13 @@ class Engine {
         HIPCHK(hipMemGetInfo(&freeB, &totalB));
         const size_t need = tapeBytes(tape);
         if (need + need / 8 + (static_cast<size_t>(1) << 30) > freeB) return false;
-        tapeAllocInto(sp, tape.steps, tape.nit, tape.halves, tape.capF, tape.capB);
+        try {
+            tapeAllocInto(sp, tape.steps, tape.nit, tape.halves, tape.capF, tape.capB);
+        } catch (const SphError&) {   // an allocation that failed after all: one stream
+            sp = Tape{};
+            (void)hipGetLastError();
+            return false;
+        }
         return true;
     }
     void adjointRun() {
@@ -2413,6 +2419,10 @@ class Engine {
             adjS = stream;
             adjTape = &tape;
             rec.tapeSeg = -1;
+            try {   // back to where the recorded rollout ended, as after a sweep
+                restoreCheckpoint(rec.last);
+            } catch (...) {
+            }
             throw;
         }
         HIPCHK(hipEventRecord(evStop, stream));
