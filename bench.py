@@ -532,30 +532,81 @@ def b64_timed(steps):
     print(json.dumps({"steps": steps, "wall_s": t, "ms_per_step": 1000.0 * t / steps}), flush=True)
 
 
-def kernel_busy(trace_dir, steps):
-    """From a rocprofv3 --kernel-trace CSV of `--b64-steps`: the dispatches
-    after the marker pause, their span and the union of their intervals."""
-    import csv
-    import glob
-    from collections import Counter
+def a100k_sweep(steps):
+    """A100k DFSPH (8 + 8 iterations, checkpoint every 20): the recording
+    forward of `steps` steps, a 0.5 s pause (a marker in a kernel trace), then
+    the reverse sweep. Prints the wall times."""
+    from sph2o_amd import scenes as SC
+    from sph2o_amd.simulation import Context
+    from sph2o_amd._lib import SPH_E_NONFINITE, SPHError
 
-    rows = list(csv.DictReader(open(glob.glob(os.path.join(trace_dir, "*kernel_trace.csv"))[0])))
-    iv = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]) for r in rows)
-    gap, k = max((iv[q + 1][0] - iv[q][1], q) for q in range(len(iv) - 1))
-    seg = iv[k + 1:]
-    busy, (cs, ce) = 0, seg[0][:2]
-    for s_, e_, _ in seg[1:]:
+    sc, _, _ = SC.a100k(2, dt=5e-4)
+    td = tempfile.mkdtemp()
+    path = os.path.join(td, "a100k.json")
+    SC.write_json(sc, path)
+    ctx = Context(0)
+    sid = ctx.load_scene_json(path)
+    ctx.init()
+    x0 = ctx.fluid_state(sid)["x"]
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.forward_record(steps, 20)
+    ctx.synchronize()
+    t_rec = time.perf_counter() - t0
+    xT = ctx.fluid_state(sid)["x"].astype(np.float64)
+    seed = (2.0 * (xT - SC.adjoint_target(x0)) / len(x0)).astype(np.float32)
+    time.sleep(0.5)
+    t0 = time.perf_counter()
+    try:
+        ctx.adjoint({sid: (seed, None)})
+    except SPHError as e:
+        if e.code != SPH_E_NONFINITE:
+            raise
+    t_adj = time.perf_counter() - t0
+    ctx.close()
+    print(json.dumps({"steps": steps, "record_s": t_rec, "reverse_s": t_adj}), flush=True)
+
+
+def _union(iv):
+    """Total length of the union of [start, end) intervals (sorted by start)."""
+    if not iv:
+        return 0
+    busy, (cs, ce) = 0, iv[0][:2]
+    for s_, e_ in (q[:2] for q in iv[1:]):
         if s_ > ce:
             busy += ce - cs
             cs, ce = s_, e_
         else:
             ce = max(ce, e_)
-    busy += ce - cs
+    return busy + ce - cs
+
+
+def kernel_busy(trace_dir, steps):
+    """From a rocprofv3 --kernel-trace CSV of `--b64-steps` or `--a100k-sweep`:
+    the dispatches after the marker pause, their span, the union of their
+    intervals, and per stream the union of its own (a sum of those above the
+    overall union is time with kernels of two streams running at once)."""
+    import csv
+    import glob
+    from collections import Counter, defaultdict
+
+    rows = list(csv.DictReader(open(glob.glob(os.path.join(trace_dir, "*kernel_trace.csv"))[0])))
+    iv = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"], r.get("Stream_Id", "0"))
+                for r in rows)
+    gap, k = max((iv[q + 1][0] - iv[q][1], q) for q in range(len(iv) - 1))
+    seg = iv[k + 1:]
+    busy = _union(seg)
     span = seg[-1][1] - seg[0][0]
+    streams = defaultdict(list)
+    for q in seg:
+        streams[q[3]].append(q)
+    per = {sid: _union(v) / 1e6 for sid, v in streams.items()}
     print(json.dumps({"marker_gap_ms": gap / 1e6, "dispatches": len(seg), "span_ms": span / 1e6,
                       "kernel_busy_ms": busy / 1e6, "busy_fraction": busy / span,
                       "dispatches_per_step": len(seg) / float(steps),
-                      "top": Counter(n.split("(")[0] for _, _, n in seg).most_common(8)}))
+                      "streams": {sid: {"dispatches": len(streams[sid]), "kernel_busy_ms": per[sid]} for sid in per},
+                      "concurrent_ms": sum(per.values()) - busy / 1e6,
+                      "top": Counter(n.split("(")[0] for _, _, n, _ in seg).most_common(8)}))
 
 
 def main():
@@ -575,11 +626,16 @@ def main():
                     help="only time this many steps of the B64 context after a marker pause (for a kernel trace)")
     ap.add_argument("--configs", action="store_true",
                     help="print the W8k, P4M and flush-D1M GPU / CPU figures instead of the bench line")
+    ap.add_argument("--a100k-sweep", type=int, default=0,
+                    help="only record this many A100k DFSPH steps, pause, and run the reverse sweep (for a kernel trace)")
     ap.add_argument("--kernel-busy", metavar="TRACE_DIR",
-                    help="summarise a rocprofv3 kernel trace of --b64-steps (no GPU use)")
+                    help="summarise a rocprofv3 kernel trace of --b64-steps or --a100k-sweep (no GPU use)")
     args = ap.parse_args()
     if args.kernel_busy:
-        kernel_busy(args.kernel_busy, args.b64_steps or 300)
+        kernel_busy(args.kernel_busy, args.b64_steps or args.a100k_sweep or 300)
+        return
+    if args.a100k_sweep > 0:
+        a100k_sweep(args.a100k_sweep)
         return
     if args.b64_steps > 0:
         b64_timed(args.b64_steps)
