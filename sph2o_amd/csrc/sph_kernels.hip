@@ -255,14 +255,34 @@ __device__ __forceinline__ int globalCell(const SceneConst& S, float4 p, int& gx
 }
 // Overflow bucket of a cell outside the dense box (the reference's unbounded
 // hash grid, HashTable.h:13-16, has no box; these buckets give the same sets).
+// Cells go in 4 x 4 x 4 blocks: the block is hashed and its 64 cells take 64
+// consecutive buckets, so the 27 cells around a particle lie in a few runs of
+// the bucket table (their ranges share cache lines, as in the dense box) and
+// the particles of neighbouring cells sit in neighbouring slots.
+#ifndef SPH_BUCKET_BLOCKS
+#define SPH_BUCKET_BLOCKS 1
+#endif
 __device__ __forceinline__ uint32_t bucketOf(const SceneConst& S, int gx, int gy, int gz) {
-    uint32_t h = static_cast<uint32_t>(gx) * 0x9E3779B1u;
-    h ^= static_cast<uint32_t>(gy) * 0x85EBCA77u + (h << 6) + (h >> 2);
-    h ^= static_cast<uint32_t>(gz) * 0xC2B2AE3Du + (h << 6) + (h >> 2);
+#if SPH_BUCKET_BLOCKS
+    // >> on a negative int floors (arithmetic shift) and & 3 is then the
+    // matching non-negative remainder
+    const int bx = gx >> 2, by = gy >> 2, bz = gz >> 2;
+    const uint32_t local = static_cast<uint32_t>((gx & 3) | ((gy & 3) << 2) | ((gz & 3) << 4));
+#else
+    const int bx = gx, by = gy, bz = gz;
+    const uint32_t local = 0u;
+#endif
+    uint32_t h = static_cast<uint32_t>(bx) * 0x9E3779B1u;
+    h ^= static_cast<uint32_t>(by) * 0x85EBCA77u + (h << 6) + (h >> 2);
+    h ^= static_cast<uint32_t>(bz) * 0xC2B2AE3Du + (h << 6) + (h >> 2);
     h ^= h >> 15;
     h *= 0x2C1B3C6Du;
     h ^= h >> 12;
+#if SPH_BUCKET_BLOCKS
+    return ((h << 6) | local) & S.ovMask;
+#else
     return h & S.ovMask;
+#endif
 }
 // Where the particles of global cell g lie: a dense cell (bucket = false) or
 // the overflow bucket it hashes to, which other cells may share (scans then
