@@ -269,7 +269,9 @@ int sph_reset_kernel_stats(sph_ctx* ctx);
  * lists for every value, the lists HashTable::neighborhoodSearch gives,
  * HashTable.cpp:23-86). lanes 1, 2, 4 or 8 pins the count; 0 (the default)
  * lets the context time each count on its first 12 searches (three rounds of
- * 1, 2, 4, 8) and keep the fastest. */
+ * 1, 2, 4, 8) and keep the fastest, and time them again whenever the active
+ * particles of its scenes (emitters add them) have more than doubled or
+ * halved since. */
 int sph_set_search_lanes(sph_ctx* ctx, int lanes);
 /* The count the search uses; tuned = 1 once the timing rounds have picked it
  * (0 while they run or when the count is pinned); trial_ms (4 floats, may be

@@ -268,6 +268,7 @@ class Engine {
         static constexpr int kLanes[kCands] = {1, 2, 4, 8};
         int trial = 0, pendingIdx = 0;
         bool done = false, pending = false;
+        uint64_t activeAtPick = 0;   // active particles of all scenes when the count was picked
         float best[kCands] = {std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity(),
                               std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity()};
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1206,6 +1207,12 @@ class Engine {
     // count builds the same lists, so the tuning leaves the trajectory alone.
     void buildLists(const DevView& d) {
         tuneCollect();
+        if (!searchLanesFixed && tuner.done) {
+            // emitters grow a scene: tune again once the active particles have
+            // more than doubled (or halved) since the count was picked
+            const uint64_t na = activeParticles();
+            if (na > 2 * tuner.activeAtPick || 2 * na < tuner.activeAtPick) resetTuner();
+        }
         if (searchLanesFixed || tuner.done) {
             const int lanes = searchLanesFixed ? searchLanesFixed : searchLanes;
             timed(K_BUILD, [&] { return launchBuildNeighbors(d, lanes, stream); });
@@ -1246,18 +1253,25 @@ class Engine {
             if (tuner.best[c] < tuner.best[best]) best = c;
         searchLanes = SearchTuner::kLanes[best];
         tuner.done = true;
+        tuner.activeAtPick = activeParticles();
+    }
+    uint64_t activeParticles() const {
+        uint64_t n = 0;
+        for (const auto& sc : scenes) n += sc.na;
+        return n;
+    }
+    void resetTuner() {
+        tuner.trial = 0;
+        tuner.done = false;
+        tuner.pending = false;
+        std::fill(tuner.best, tuner.best + SearchTuner::kCands, std::numeric_limits<float>::infinity());
     }
     void setSearchLanes(int lanes) {
         if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8)
             throw SphError{SPH_E_ARG, "search lanes must be 0 (tune), 1, 2, 4 or 8"};
         if (stream) HIPCHK(hipStreamSynchronize(stream));
         searchLanesFixed = lanes;
-        if (lanes == 0) {   // tune afresh
-            tuner.trial = 0;
-            tuner.done = false;
-            tuner.pending = false;
-            std::fill(tuner.best, tuner.best + SearchTuner::kCands, std::numeric_limits<float>::infinity());
-        }
+        if (lanes == 0) resetTuner();   // tune afresh
     }
     // Interpret the flags of the last search (hFlags must be current). Returns
     // true if the lists are valid; otherwise grows what overflowed and returns

@@ -81,3 +81,56 @@ def test_search_lanes_rejects_other_counts():
         assert e.value.code == _lib.SPH_E_ARG
     finally:
         ctx.close()
+
+
+def _growing_scene():
+    """WCSPH: a 27-particle block and an emitter of 10 x 10 sheets (every 50
+    steps at 4 m/s), so the active particles more than double while it runs."""
+    return {
+        "Configuration": {"startTime": 0.0, "endTime": 1e6, "timeStep": 1e-4, "fps": 60.0, "minTimeStep": 0.0,
+                          "maxTimeStep": 0.005, "gravity": [0.0, 0.0, -9.81], "particleRadius": 0.01,
+                          "simulationMethod": 0, "boundaryMethod": 2, "cflFactor": 1.0, "stiffness": 1000.0,
+                          "gamma": 7.0},
+        "Fluid": {"viscosityMethod": 2, "surfaceTensionMethod": -1, "adhesionMethod": -1,
+                  "FluidModel": [{"density0": 1000.0,
+                                  "fluidBlocks": [{"min": [0.02, 0.02, 0.02], "max": [0.08, 0.08, 0.08]}],
+                                  "viscosity": 0.01, "boundaryViscosity": 0.0,
+                                  "emitters": [{"position": [0.3, 0.15, 0.25],
+                                                "rotation": [0.70710678, -0.70710678, 0.0, 0.0], "velocity": 4.0,
+                                                "numParticles": 600, "startTime": 0.0002, "type": 0, "width": 0.2,
+                                                "height": 0.2, "spacing": 1.0}]}]},
+        "Boundary": [{"box": [{"min": [0, 0, 0], "max": [0.6, 0.3, 0.4]}], "sphere": [], "geometry": []}],
+    }
+
+
+def test_search_lanes_retuned_as_emitters_grow(tmp_path):
+    """The tuned count is timed again once the active particles have more
+    than doubled since it was picked; the trajectory stays the pinned one's."""
+    path = os.path.join(str(tmp_path), "grow.json")
+    SC.write_json(_growing_scene(), path)
+    finals = []
+    for lanes in (0, 1):
+        ctx = Context(0)
+        try:
+            sid = ctx.load_scene_json(path)
+            if lanes:
+                ctx.set_search_lanes(lanes)
+            ctx.init()
+            tuned, active = [], []
+            for _ in range(160):
+                ctx.step(1)
+                tuned.append(ctx.search_lanes()[1])
+                active.append(ctx.num_active(sid))
+            finals.append(ctx.fluid_state(sid))
+        finally:
+            ctx.close()
+        if lanes:
+            assert not any(tuned)
+            continue
+        first = tuned.index(True)                  # the first pick
+        assert active[-1] > 2 * active[first], active
+        again = tuned.index(False, first)          # timed again after the growth
+        assert active[again - 1] > 2 * active[first], (again, active[first], active[again - 1])
+        assert tuned[-1], "the second tuning finishes"
+    for k in ("x", "v", "rho"):
+        assert np.array_equal(finals[0][k], finals[1][k]), k
