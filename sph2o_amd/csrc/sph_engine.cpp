@@ -492,7 +492,7 @@ class Engine {
                     rotate(e.P.rotation, pos);
                     rotate(e.P.rotation, vel);
                     for (int a = 0; a < 3; ++a) pos[a] += e.P.position[a];
-                    nx.push_back(make_float4(pos[0], pos[1], pos[2], dfsph ? 1.0f : 0.0f));
+                    nx.push_back(make_float4(pos[0], pos[1], pos[2], dfsph ? -1.0f : 0.0f));   // w < 0: emitted this step
                     nv.push_back(make_float4(vel[0], vel[1], vel[2], 0.0f));
                 }
                 e.remaining -= g;

@@ -203,7 +203,7 @@ __device__ __forceinline__ float dot3f(float ax, float ay, float az, float bx, f
     return __fmaf_rn(az, bz, __fmaf_rn(ay, by, __fmul_rn(ax, bx)));
 }
 struct XRP { float4 x; float rho, p; };
-struct XVR { float4 x, v; float rho; };
+struct XP { float4 x; float p; };
 __device__ __forceinline__ int floorCell(float x, float invh) { return static_cast<int>(floorf(__fmul_rn(x, invh))); }
 
 __device__ __forceinline__ uint32_t spread3(uint32_t v) {  // 10 bits -> every third bit
@@ -459,9 +459,10 @@ __global__ void __launch_bounds__(kBlock) kKeys(DevView d, int dfsphAdvance, int
     if (live) {
         float4 p = d.x[i];
         if (dfsphAdvance) {
-            // w != 0 marks a particle emitted in this step: the reference emits
+            // w < 0 marks a particle emitted in this step (w is otherwise the
+            // density of the last density pass, or 0): the reference emits
             // after updatePositions (DFSPHSolver.cpp:72), so it does not move yet
-            if (p.w == 0.0f) {
+            if (!(p.w < 0.0f)) {
                 const float dt = T.dt;
                 const float4 vi = VEL(d, i);
                 p.x += vi.x * dt;
@@ -1043,6 +1044,13 @@ __device__ __forceinline__ void pciCorrectPredicted(const DevView& d, const Scen
 // ---------------------------------------------------------------- density
 // SPHSolver::computeFluidDensities (SPHSolver.cpp:42-89) fused with
 // DFSPHSolver::computeAlpha (DFSPHSolver.cpp:109-178).
+// The density pass also files rho_i in the w of particle i's position record,
+// which the force passes that follow gather with the position (one record per
+// neighbour instead of two). The passes that run on predicted densities
+// (PCISPH) read d.rho instead.
+__device__ __forceinline__ void storeRhoInX(const DevView& d, uint32_t i, float rho) {
+    reinterpret_cast<float*>(d.x + i)[3] = rho;
+}
 __global__ void __launch_bounds__(kBlock) kDensityAlpha(DevView d, int withAlpha) {
     SCENE_PROLOGUE_LANES(2, kLppOnce)
     (void)dt;
@@ -1079,6 +1087,7 @@ __global__ void __launch_bounds__(kBlock) kDensityAlpha(DevView d, int withAlpha
     ssq = lppSum<kLppOnce>(ssq);
     if (!valid || sub != 0) return;
     d.rho[i] = rho;
+    storeRhoInX(d, i, rho);
     if (withAlpha) {
         const float den = sgx * sgx + sgy * sgy + sgz * sgz + ssq;
         d.alpha[i] = den > S.eps ? rho / den : 0.0f;
@@ -1104,6 +1113,7 @@ __global__ void __launch_bounds__(kBlock) kDensityPressure(DevView d) {
         rho += xb.w * cubicW(S, fastSqrt(rx * rx + ry * ry + rz * rz));
     }
     d.rho[i] = rho;
+    storeRhoInX(d, i, rho);
     d.p[i] = fmaxf(0.0f, S.wcB * (powf(rho / S.rho0, S.gamma) - 1.0f));
 }
 
@@ -1132,10 +1142,10 @@ __global__ void __launch_bounds__(kBlock) kForces(DevView d, int pressure) {
         const float dpi = pi_ / (rhoi * rhoi);
         const float dpb = pi_ / (S.rho0 * S.rho0);
         walkList<kLppOnce, kBatch>(
-            d.nbrF, NP, i, sub, nf, [&](uint32_t j) { return XRP{POSV(d, j), d.rho[j], d.p[j]}; },
-            [&](uint32_t, const XRP& q) {
+            d.nbrF, NP, i, sub, nf, [&](uint32_t j) { return XP{POSV(d, j), d.p[j]}; },
+            [&](uint32_t, const XP& q) {
                 const float4 xj = q.x;
-                const float rhoj = q.rho;
+                const float rhoj = xj.w;   // the density pass's rho_j
                 const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
                 const float g = S.m * (dpi + q.p / (rhoj * rhoj)) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
                 ax -= g * rx; ay -= g * ry; az -= g * rz;
@@ -1164,12 +1174,13 @@ __global__ void __launch_bounds__(kBlock) kForces(DevView d, int pressure) {
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;   // viscosity sum
         float nx = 0.0f, ny = 0.0f, nz = 0.0f;   // normal sum
         const float invdt = 1.0f / dt;
-        auto gatherXVR = [&](uint32_t j) {
-            return XVR{POSV(d, j), vm >= 0 ? VEL(d, j) : make_float4(0.0f, 0.0f, 0.0f, 0.0f), d.rho[j]};
+        // w of the position record is the density pass's rho_j
+        auto gatherXV = [&](uint32_t j) {
+            return XV{POSV(d, j), vm >= 0 ? VEL(d, j) : make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
         };
-        walkList<kLppOnce, kBatch, true>(d.nbrF, NP, i, sub, nf, gatherXVR, [&](uint32_t, const XVR& q) {
+        walkList<kLppOnce, kBatch, true>(d.nbrF, NP, i, sub, nf, gatherXV, [&](uint32_t, const XV& q) {
             const float4 xj = q.x;
-            const float rhoj = q.rho;
+            const float rhoj = xj.w;
             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
             const float r2 = rx * rx + ry * ry + rz * rz;
             const float rl = fastSqrt(r2);
