@@ -1094,24 +1094,31 @@ __global__ void __launch_bounds__(kBlock) kDensityAlpha(DevView d, int withAlpha
     }
 }
 
-// WCSPH density + Tait pressure (WCSPHSolver.cpp:80-95).
+// WCSPH density + Tait pressure (WCSPHSolver.cpp:80-95), with the lanes per
+// particle of the other once-per-step passes (kLppOnce).
 __global__ void __launch_bounds__(kBlock) kDensityPressure(DevView d) {
-    SCENE_PROLOGUE(2)
-    if (!valid) return;
+    SCENE_PROLOGUE_LANES(2, kLppOnce)
+    (void)dt;
+    (void)dt2;
     const size_t NP = d.NP;
-    const float4 xi = d.x[i];
     float rho = 0.0f;
-    const uint32_t nf = min(d.cntF[i], d.capF);
-    walkList<1, kBatch>(d.nbrF, NP, i, 0u, nf, [&](uint32_t j) { return d.x[j]; }, [&](uint32_t, const float4& xj) {
-        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        rho += S.m * cubicW(S, fastSqrt(rx * rx + ry * ry + rz * rz));
-    });
-    const uint32_t nb = S.akinci ? min(d.cntB[i], d.capB) : 0u;
-    for (uint32_t s = 0; s < nb; ++s) {
-        const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
-        const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
-        rho += xb.w * cubicW(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+    if (valid) {
+        const float4 xi = d.x[i];
+        const uint32_t nf = min(d.cntF[i], d.capF);
+        walkList<kLppOnce, kBatch, true>(d.nbrF, NP, i, sub, nf, [&](uint32_t j) { return d.x[j]; },
+                                         [&](uint32_t, const float4& xj) {
+                                             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+                                             rho += S.m * cubicW(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+                                         });
+        const uint32_t nb = S.akinci ? min(d.cntB[i], d.capB) : 0u;
+        for (uint32_t s = sub; s < nb; s += kLppOnce) {
+            const float4 xb = d.xb[ldList(d.nbrB + s * NP + i)];
+            const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
+            rho += xb.w * cubicW(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+        }
     }
+    rho = lppSum<kLppOnce>(rho);
+    if (!valid || sub != 0) return;
     d.rho[i] = rho;
     storeRhoInX(d, i, rho);
     d.p[i] = fmaxf(0.0f, S.wcB * (powf(rho / S.rho0, S.gamma) - 1.0f));
@@ -2011,7 +2018,9 @@ hipError_t launchBoundaryDensity(const DevView& d, int scene, uint32_t b0, uint3
     LAUNCH(kBoundaryDensity, g, d, scene, b0, b1, predicted, slot);
 }
 hipError_t launchDensityAlpha(const DevView& d, int wa, hipStream_t st) { LAUNCH(kDensityAlpha, d.numBlocks * kLppOnce, d, wa); }
-hipError_t launchDensityPressure(const DevView& d, hipStream_t st) { LAUNCH(kDensityPressure, d.numBlocks, d); }
+hipError_t launchDensityPressure(const DevView& d, hipStream_t st) {
+    LAUNCH(kDensityPressure, d.numBlocks * kLppOnce, d);
+}
 hipError_t launchForces(const DevView& d, int pressure, hipStream_t st) { LAUNCH(kForces, d.numBlocks * kLppOnce, d, pressure); }
 hipError_t launchSurfaceTension(const DevView& d, int vmax, hipStream_t st) {
     LAUNCH(kSurfaceTension, d.numBlocks * kLppOnce, d, vmax);
