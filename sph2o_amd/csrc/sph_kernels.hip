@@ -203,7 +203,7 @@ __device__ __forceinline__ float dot3f(float ax, float ay, float az, float bx, f
     return __fmaf_rn(az, bz, __fmaf_rn(ay, by, __fmul_rn(ax, bx)));
 }
 struct XRP { float4 x; float rho, p; };
-struct XP { float4 x; float p; };
+struct XVP { float4 x, v; float p; };
 __device__ __forceinline__ int floorCell(float x, float invh) { return static_cast<int>(floorf(__fmul_rn(x, invh))); }
 
 __device__ __forceinline__ uint32_t spread3(uint32_t v) {  // 10 bits -> every third bit
@@ -1143,52 +1143,32 @@ __global__ void __launch_bounds__(kBlock) kForces(DevView d, int pressure) {
     const float rhoi = d.rho[ii];
     const uint32_t nf = valid ? min(d.cntF[i], d.capF) : 0u;
     const uint32_t nb = (valid && S.akinci) ? min(d.cntB[i], d.capB) : 0u;
-    float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    if (pressure) {
-        const float pi_ = d.p[i];
-        const float dpi = pi_ / (rhoi * rhoi);
-        const float dpb = pi_ / (S.rho0 * S.rho0);
-        walkList<kLppOnce, kBatch>(
-            d.nbrF, NP, i, sub, nf, [&](uint32_t j) { return XP{POSV(d, j), d.p[j]}; },
-            [&](uint32_t, const XP& q) {
-                const float4 xj = q.x;
-                const float rhoj = xj.w;   // the density pass's rho_j
-                const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                const float g = S.m * (dpi + q.p / (rhoj * rhoj)) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
-                ax -= g * rx; ay -= g * ry; az -= g * rz;
-            });
-        for (uint32_t s = sub; s < nb; s += kLppOnce) {
-            const uint32_t bi = ldList(d.nbrB + s * NP + i);
-            const float4 xb = d.xb[bi];
-            const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
-            // Akinci: rho0 V_b p_i / rho0^2 (WCSPHSolver.cpp:176-186); a PCISPH
-            // boundary particle: m_b (p_i / rho_i^2 + p_b / rho_b^2) (:160-172)
-            float coef = dpi + dpb;
-            if (S.pciB) {
-                const float rb = d.bRho[bi];
-                coef = dpi + d.bP[bi] / (rb * rb);
-            }
-            const float g = xb.w * coef * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
-            ax -= g * rx; ay -= g * ry; az -= g * rz;
-        }
-        ax = lppSum<kLppOnce>(ax);
-        ay = lppSum<kLppOnce>(ay);
-        az = lppSum<kLppOnce>(az);
-    }
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;   // pressure sum
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;   // viscosity sum
+    float nx = 0.0f, ny = 0.0f, nz = 0.0f;   // normal sum
     const int vm = S.viscMethod;
     const bool normals = S.stOn;
-    if (vm >= 0 || normals) {
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f;   // viscosity sum
-        float nx = 0.0f, ny = 0.0f, nz = 0.0f;   // normal sum
-        const float invdt = 1.0f / dt;
-        // w of the position record is the density pass's rho_j
-        auto gatherXV = [&](uint32_t j) {
-            return XV{POSV(d, j), vm >= 0 ? VEL(d, j) : make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
+    const bool other = vm >= 0 || normals;
+    const float invdt = 1.0f / dt;
+    const float pi_ = pressure && valid ? d.p[i] : 0.0f;
+    const float dpi = pi_ / (rhoi * rhoi);
+    const float dpb = pi_ / (S.rho0 * S.rho0);
+    // one walk for the pressure, viscosity and normal sums, each in list order;
+    // w of the position record is the density pass's rho_j
+    if (pressure || other) {
+        auto gather = [&](uint32_t j) {
+            return XVP{POSV(d, j), vm >= 0 ? VEL(d, j) : make_float4(0.0f, 0.0f, 0.0f, 0.0f),
+                       pressure ? d.p[j] : 0.0f};
         };
-        walkList<kLppOnce, kBatch, true>(d.nbrF, NP, i, sub, nf, gatherXV, [&](uint32_t, const XV& q) {
+        walkList<kLppOnce, kBatch, true>(d.nbrF, NP, i, sub, nf, gather, [&](uint32_t, const XVP& q) {
             const float4 xj = q.x;
             const float rhoj = xj.w;
             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            if (pressure) {
+                const float g = S.m * (dpi + q.p / (rhoj * rhoj)) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+                ax -= g * rx; ay -= g * ry; az -= g * rz;
+            }
+            if (!other) return;
             const float r2 = rx * rx + ry * ry + rz * rz;
             const float rl = fastSqrt(r2);
             if (vm >= 0) {
@@ -1210,6 +1190,27 @@ __global__ void __launch_bounds__(kBlock) kForces(DevView d, int pressure) {
                 nx += c * rx; ny += c * ry; nz += c * rz;
             }
         });
+    }
+    if (pressure) {
+        for (uint32_t s = sub; s < nb; s += kLppOnce) {
+            const uint32_t bi = ldList(d.nbrB + s * NP + i);
+            const float4 xb = d.xb[bi];
+            const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
+            // Akinci: rho0 V_b p_i / rho0^2 (WCSPHSolver.cpp:176-186); a PCISPH
+            // boundary particle: m_b (p_i / rho_i^2 + p_b / rho_b^2) (:160-172)
+            float coef = dpi + dpb;
+            if (S.pciB) {
+                const float rb = d.bRho[bi];
+                coef = dpi + d.bP[bi] / (rb * rb);
+            }
+            const float g = xb.w * coef * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+            ax -= g * rx; ay -= g * ry; az -= g * rz;
+        }
+        ax = lppSum<kLppOnce>(ax);
+        ay = lppSum<kLppOnce>(ay);
+        az = lppSum<kLppOnce>(az);
+    }
+    if (other) {
         fx = lppSum<kLppOnce>(fx);
         fy = lppSum<kLppOnce>(fy);
         fz = lppSum<kLppOnce>(fz);
