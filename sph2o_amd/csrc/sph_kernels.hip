@@ -202,7 +202,6 @@ __device__ __forceinline__ float pairFma(float a, float b, float c) { return __f
 __device__ __forceinline__ float dot3f(float ax, float ay, float az, float bx, float by, float bz) {
     return __fmaf_rn(az, bz, __fmaf_rn(ay, by, __fmul_rn(ax, bx)));
 }
-struct XRP { float4 x; float rho, p; };
 struct XVP { float4 x, v; float p; };
 __device__ __forceinline__ int floorCell(float x, float invh) { return static_cast<int>(floorf(__fmul_rn(x, invh))); }
 
@@ -1044,13 +1043,16 @@ __device__ __forceinline__ void pciCorrectPredicted(const DevView& d, const Scen
 // ---------------------------------------------------------------- density
 // SPHSolver::computeFluidDensities (SPHSolver.cpp:42-89) fused with
 // DFSPHSolver::computeAlpha (DFSPHSolver.cpp:109-178).
-// The density pass also files rho_i in the w of particle i's position record,
-// which the force passes that follow gather with the position (one record per
-// neighbour instead of two). The passes that run on predicted densities
-// (PCISPH) read d.rho instead.
-__device__ __forceinline__ void storeRhoInX(const DevView& d, uint32_t i, float rho) {
-    reinterpret_cast<float*>(d.x + i)[3] = rho;
+// The w of particle i's position record carries a per-particle scalar that
+// the next neighbour pass gathers with the position (one record per neighbour
+// instead of two or three): the density passes file rho_i there for the force
+// pass, and within the PCISPH loop the predicted-density pass files
+// p_i / rho_i^2 for the pressure-acceleration pass (the force pass has read
+// rho by then, and the next step's density pass files it again).
+__device__ __forceinline__ void storeXW(const DevView& d, uint32_t i, float w) {
+    reinterpret_cast<float*>(d.x + i)[3] = w;
 }
+__device__ __forceinline__ void storeRhoInX(const DevView& d, uint32_t i, float rho) { storeXW(d, i, rho); }
 __global__ void __launch_bounds__(kBlock) kDensityAlpha(DevView d, int withAlpha) {
     SCENE_PROLOGUE_LANES(2, kLppOnce)
     (void)dt;
@@ -1808,7 +1810,9 @@ __global__ void __launch_bounds__(kBlock) kPciDensity(DevView d, int slot) {
         }
         d.rho[i] = rho;
         const float e = fmaxf(0.0f, rho - S.rho0);
-        d.p[i] += S.pciDelta * e;
+        const float pi_ = d.p[i] + S.pciDelta * e;
+        d.p[i] = pi_;
+        storeXW(d, i, pi_ / (rho * rho));
         err = static_cast<double>(e / S.rho0);
     }
     storePartial<false>(d, err);
@@ -1823,13 +1827,11 @@ __device__ __forceinline__ void pciPressureAcc(const DevView& d, const SceneCons
     const float dpi = d.p[i] / (rhoi * rhoi);
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     const uint32_t nf = min(d.cntF[i], d.capF);
-    walkList<1, kBatch>(d.nbrF, NP, i, 0u, nf, [&](uint32_t j) { return XRP{d.x[j], d.rho[j], d.p[j]}; },
-                        [&](uint32_t, const XRP& q) {
-                            const float4 xj = q.x;
-                            const float rhoj = q.rho;
+    // w of the gathered position is p_j / rho_j^2, filed by kPciDensity
+    walkList<1, kBatch>(d.nbrF, NP, i, 0u, nf, [&](uint32_t j) { return d.x[j]; },
+                        [&](uint32_t, const float4& xj) {
                             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                            const float g = S.m * (dpi + q.p / (rhoj * rhoj)) *
-                                            gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
+                            const float g = S.m * (dpi + xj.w) * gradFactor(S, fastSqrt(rx * rx + ry * ry + rz * rz));
                             ax -= g * rx; ay -= g * ry; az -= g * rz;
                         });
     const uint32_t nb = S.akinci ? min(d.cntB[i], d.capB) : 0u;
