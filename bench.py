@@ -120,15 +120,11 @@ def measured_valu(kernel):
     return None
 
 
-# Engine clock of the busy-cycle counters (MI355X peak engine clock)
-ENGINE_HZ = 2.4e9
-
-
 def measured_ta_busy(kernel):
-    """Busy cycles per launch of the texture-address unit (one per CU, every
-    vector memory instruction passes it) for `kernel` on this workload, from
-    the committed rocprofv3 TA_BUSY_avr summary
-    (profiles/d1m_vmem_pipeline_counters.md, lines `ta_busy <kernel> <cycles>`),
+    """Busy fraction of the texture-address unit (one per CU, every vector
+    memory instruction passes it) over `kernel`'s own dispatches in the
+    committed rocprofv3 TA_BUSY_avr run of this workload
+    (profiles/d1m_vmem_pipeline_counters.md, lines `ta_busy_frac <kernel> <f>`),
     or None."""
     path = os.path.join(ROOT, "profiles", "d1m_vmem_pipeline_counters.md")
     if not os.path.exists(path):
@@ -136,7 +132,7 @@ def measured_ta_busy(kernel):
     with open(path) as f:
         for line in f:
             parts = line.strip().strip("`").split()
-            if len(parts) == 3 and parts[0] == "ta_busy" and parts[1] == kernel:
+            if len(parts) == 3 and parts[0] == "ta_busy_frac" and parts[1] == kernel:
                 return float(parts[2])
     return None
 
@@ -703,8 +699,7 @@ def main():
     traffic = measured_traffic(dominant)
     valu = measured_valu(dominant)
     valu_frac = valu / (avg_ms * 1e-3) / VALU_PEAK_WAVE_INSTS if (valu and klaunch) else None
-    ta = measured_ta_busy(dominant)
-    ta_frac = ta / (avg_ms * 1e-3 * ENGINE_HZ) if (ta and klaunch) else None
+    ta_frac = measured_ta_busy(dominant)
     search = search_info(ctx)
 
     ctx.close()
@@ -756,10 +751,10 @@ def main():
                      # against the VALU issue peak
                      "valu_issue": {"wave_insts_per_launch": valu, "peak_wave_insts_per_s": VALU_PEAK_WAVE_INSTS,
                                     "frac": valu_frac},
-                     # TA_BUSY_avr per launch (counter run of this workload) over this run's launch time at
-                     # the engine clock: the vector-memory address unit, the busiest unit of the neighbour
-                     # passes (scattered 64-lane gathers, ~37 L1 lines each)
-                     "vmem": {"ta_busy_cycles_per_launch": ta, "engine_hz": ENGINE_HZ, "ta_busy_frac": ta_frac}},
+                     # TA_BUSY_avr over the kernel's dispatch time in the counter run of this workload: the
+                     # vector-memory address unit, the busiest unit of the neighbour passes (scattered 64-lane
+                     # gathers, ~37 L1 lines each)
+                     "vmem": {"ta_busy_frac": ta_frac, "source": "profiles/d1m_vmem_pipeline_counters.md"}},
         "batched_gain": batched["gain"] if batched else None,
         "batched": batched,
         # (recording forward + reverse sweep) / plain forward, the larger of the two methods
