@@ -40,6 +40,9 @@ __device__ __forceinline__ void axpy(F3& y, float s, const F3& x) {
     y.z += s * x.z;
 }
 __device__ __forceinline__ float4 f4(const F3& a) { return make_float4(a.x, a.y, a.z, 0.0f); }
+struct XV {
+    float4 x, v;
+};
 
 // G2 = G'(|r|) / |r| for the cubic spline's gradW = G r (CubicSpline.h:52-74):
 // G = l (3q - 2) / h^2 for q <= 1/2, -l (1 - q)^2 / (h^2 q) for 1/2 < q <= 1.
@@ -359,11 +362,14 @@ __global__ void __launch_bounds__(kBlock) kAdjHalfStart(AdjView a, int mode) {
     a.xBar[i] = make_float4(gx.x, gx.y, gx.z, 0.0f);
     if (mode == 0) {
         const float4 gv = a.adjV[o];
-        a.vBar[i] = make_float4(gv.x + dt * gx.x, gv.y + dt * gx.y, gv.z + dt * gx.z, 0.0f);
+        a.vBar[i] = make_float4(gv.x + dt * gx.x, gv.y + dt * gx.y, gv.z + dt * gx.z, a.kkNext ? a.kkNext[i] : 0.0f);
         a.rhoBar[i] = 0.0f;
         a.alphaBar[i] = 0.0f;
     } else {
-        if (mode == 1) a.vBar[i] = a.adjV[o];
+        if (mode == 1) {
+            const float4 gv = a.adjV[o];
+            a.vBar[i] = make_float4(gv.x, gv.y, gv.z, a.kkNext ? a.kkNext[i] : 0.0f);
+        }
         a.rhoBar[i] = a.adjRho[o];
         a.alphaBar[i] = a.adjAlpha[o];
     }
@@ -382,6 +388,14 @@ __global__ void __launch_bounds__(kBlock) kAdjHalfEnd(AdjView a, int mode) {
     }
 }
 
+// The (u_i, s-bar_i) record of a Jacobi iteration's source pass: the velocity
+// its forward source pass read and the source adjoint the velocity pass just
+// formed, gathered together by kAdjSource.
+__device__ __forceinline__ void storeSU(const AdjView& a, uint32_t i) {
+    const float4 u = a.u[i];
+    a.su[i] = make_float4(u.x, u.y, u.z, a.sBar[i]);
+}
+
 // Adjoint of a velocity pass given u'-bar (vBar, left unchanged: u' = u - ...):
 // k-bar, x-bar from the gradient's Jacobian, then the pass kind's chain:
 // loop passes add the k_ws / kv_ws accumulation and go through k(rho*, alpha,
@@ -398,24 +412,22 @@ __global__ void __launch_bounds__(kBlock) kAdjVelocity(AdjView a, int mode) {
     const uint32_t nf = valid ? min(a.cntF[i], a.capF) : 0u;
     // whole rounds of the self-padded list (a self pair adds exactly zero):
     // per slot k_j, x_j and v-bar_j at once, then the sums in slot order
-    struct KXV { float k; float4 x, v; };
+    // v-bar_j carries this pass's k_j in w (filed by the pass before), so a
+    // neighbour costs two gathers
     const uint32_t bytes = static_cast<uint32_t>(NP * sizeof(float4));
     const __amdgpu_buffer_rsrc_t rX = __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(a.x), 0, bytes, 0x00020000);
     const __amdgpu_buffer_rsrc_t rV = __builtin_amdgcn_make_buffer_rsrc(a.vBar, 0, bytes, 0x00020000);
-    const __amdgpu_buffer_rsrc_t rK =
-        __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(a.kk), 0, bytes / 4u, 0x00020000);
     walkRows<kAdjLpp, kPadRows / kAdjLpp, true>(
         a.nbrF, a.NP, ii, sub, nf,
         [&](uint32_t j) {
             const uint32_t off = j * static_cast<uint32_t>(sizeof(float4));
-            return KXV{__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rK, j * 4u, 0, 0)),
-                       __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rX, off, 0, 0)),
-                       __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rV, off, 0, 0))};
+            return XV{__builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rX, off, 0, 0)),
+                      __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rV, off, 0, 0))};
         },
-        [&](const KXV& q) {
+        [&](const XV& q) {
             // a closed gate (|k_i + k_j| <= eps) adds exact zeros: the same sums
             // as skipping the pair, without a branch per pair
-            const float ks = ki + q.k;
+            const float ks = ki + q.v.w;
             const bool on = fabsf(ks) > S.eps;
             const F3 r = xi - f3(q.x);
             float G, G2;
@@ -453,11 +465,13 @@ __global__ void __launch_bounds__(kBlock) kAdjVelocity(AdjView a, int mode) {
         kb += dt2 * a.adjKws[o];
         const float ex = src - S.rho0;
         a.sBar[i] = src > S.rho0 ? kb * al / (dt2 * rhoi) : 0.0f;
+        storeSU(a, i);
         a.alphaBar[i] += kb * ex / (dt2 * rhoi);
         a.rhoBar[i] += -kb * ex * al / (dt2 * rhoi * rhoi) + a.sBar[i];
     } else if (mode == kAdjDivergenceLoop) {
         kb += dt * a.adjKvws[o];
         a.sBar[i] = src > 0.0f ? kb * al / (dt * rhoi) : 0.0f;
+        storeSU(a, i);
         a.alphaBar[i] += kb * src / (dt * rhoi);
         a.rhoBar[i] += -kb * src * al / (dt * rhoi * rhoi);
     } else if (mode == kAdjDensityWarm) {
@@ -478,26 +492,24 @@ __global__ void __launch_bounds__(kBlock) kAdjSource(AdjView a, int mode) {
     const float si = a.sBar[ii];
     F3 gx{0.0f, 0.0f, 0.0f}, gu{0.0f, 0.0f, 0.0f};
     const uint32_t nf = valid ? min(a.cntF[i], a.capF) : 0u;
-    struct SXU { float s; float4 x, u; };   // as in kAdjVelocity
+    // the (u_j, s-bar_j) records the velocity pass filed: two gathers per neighbour
     const uint32_t bytes = static_cast<uint32_t>(NP * sizeof(float4));
     const __amdgpu_buffer_rsrc_t rX = __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(a.x), 0, bytes, 0x00020000);
-    const __amdgpu_buffer_rsrc_t rU = __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(a.u), 0, bytes, 0x00020000);
-    const __amdgpu_buffer_rsrc_t rS = __builtin_amdgcn_make_buffer_rsrc(a.sBar, 0, bytes / 4u, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rU = __builtin_amdgcn_make_buffer_rsrc(a.su, 0, bytes, 0x00020000);
     walkRows<kAdjLpp, kPadRows / kAdjLpp, true>(
         a.nbrF, a.NP, ii, sub, nf,
         [&](uint32_t j) {
             const uint32_t off = j * static_cast<uint32_t>(sizeof(float4));
-            return SXU{__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rS, j * 4u, 0, 0)),
-                       __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rX, off, 0, 0)),
-                       __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rU, off, 0, 0))};
+            return XV{__builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rX, off, 0, 0)),
+                      __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rU, off, 0, 0))};
         },
-        [&](const SXU& q) {
+        [&](const XV& q) {
             // s_i + s_j = 0 gives w = 0 and exact zeros: no branch per pair
-            const float ss = si + q.s;
+            const float ss = si + q.v.w;
             const F3 r = xi - f3(q.x);
             float G, G2;
             gradFactors(S, dRef(r.x, r.y, r.z), G, G2);
-            const F3 du = ui - f3(q.u);
+            const F3 du = ui - f3(q.v);
             const float w = c * S.m * ss;
             axpy(gu, w * G, r);
             axpy(gx, w * G, du);
@@ -521,7 +533,7 @@ __global__ void __launch_bounds__(kBlock) kAdjSource(AdjView a, int mode) {
     if (!valid || sub != 0) return;
     const float4 x0 = a.xBar[i], v0 = a.vBar[i];
     a.xBar[i] = make_float4(x0.x + gx.x, x0.y + gx.y, x0.z + gx.z, 0.0f);
-    a.vBar[i] = make_float4(v0.x + gu.x, v0.y + gu.y, v0.z + gu.z, 0.0f);
+    a.vBar[i] = make_float4(v0.x + gu.x, v0.y + gu.y, v0.z + gu.z, a.kkNext ? a.kkNext[i] : 0.0f);
 }
 
 // predictVelocities (DFSPHSolver.cpp:180-202): v* = v + (a + g) dt, so

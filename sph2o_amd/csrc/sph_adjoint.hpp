@@ -46,6 +46,9 @@ struct AdjView {
     // DFSPH: one Jacobi pass of the recorded step
     const float4* u;     // velocity the pass read
     const float* kk;     // its k (density: 1/dt^2 units, divergence: 1/dt)
+    const float* kkNext; // k of the velocity pass that runs next (null: none); the
+                         // pass before it files it in the w of v-bar
+    float4* su;          // (u, s-bar) of the current iteration, filed by kAdjVelocity
     const float* aux;    // its rho* (density) or clamped divergence
     const float* alpha;  // alpha at the step's positions
     float* kBar;         // adjoint of k of the pass

@@ -1665,7 +1665,7 @@ class Engine {
     DevBuf<SceneConst> dScenesAdj;
     DevBuf<SceneState> dStateAdj;
     uint64_t stateEpoch = 0;  // bumped by every call that changes the state outside a recording
-    DevBuf<float4> adjX, adjV, aBar, xBar, vBar, nBar, sgBar;
+    DevBuf<float4> adjX, adjV, aBar, xBar, vBar, nBar, sgBar, suBar;
     DevBuf<float> rhoBar, alphaBar, kBar, sBar, adjKws, adjKvws, adjRho, adjAlpha;
     DevBuf<double> adjPartials, adjAcc;
     std::vector<std::vector<float>> seedX, seedV;
@@ -2307,6 +2307,7 @@ class Engine {
         a.kBar = kBar.p;
         a.sBar = sBar.p;
         a.sgBar = sgBar.p;
+        a.su = suBar.p;
         a.adjKws = adjKws.p;
         a.adjKvws = adjKvws.p;
         a.adjRho = adjRho.p;
@@ -2315,7 +2316,15 @@ class Engine {
         a.acc = adjAcc.p;
         return a;
     }
-    // point a DFSPH view at iteration `it` of its half (warm start: it = nit)
+    // k of pass `it` (warm start: it = nit) of a half of step `slot` on the tape
+    const float* tapeK(int slot, int half, int it) const {
+        const Tape& tp = *adjTape;
+        return tp.h[half].k.p + (static_cast<size_t>(slot) * (tp.nit + 1) + it) * NP;
+    }
+    // point a DFSPH view at iteration `it` of its half (warm start: it = nit);
+    // the passes run velocity then source from it = nit - 1 down to 0, then the
+    // warm start's velocity pass, so the next velocity pass after the source
+    // pass of `it` is it - 1, or the warm start's after it = 0
     void adjPass(AdjView& a, int slot, int half, int it) {
         const Tape& tp = *adjTape;
         const HalfTape& t = tp.h[half];
@@ -2323,6 +2332,7 @@ class Engine {
         a.kk = t.k.p + ko;
         a.aux = t.aux.p + ko;
         a.u = it < tp.nit ? t.u.p + (static_cast<size_t>(slot) * tp.nit + it) * NP : nullptr;
+        a.kkNext = it < tp.nit ? tapeK(slot, half, it > 0 ? it - 1 : tp.nit) : nullptr;
     }
     void adjointStep(int slot) {
         if (method == SPH_DFSPH) return adjointStepDFSPH(slot);
@@ -2339,6 +2349,7 @@ class Engine {
     void adjointStepDFSPH(int slot) {
         const int nit = adjTape->nit;
         AdjView a = adjView(slot, 1);
+        a.kkNext = tapeK(slot, 1, nit - 1);   // the half's first velocity pass
         timed(K_ADJ_SEED, [&] { return launchAdjHalfStart(a, 1, adjS); });
         for (int it = nit - 1; it >= 0; --it) {
             adjPass(a, slot, 1, it);
@@ -2351,6 +2362,7 @@ class Engine {
         timed(K_ADJ_DENSITY, [&] { return launchAdjAlphaB(a, adjS); });
         timed(K_ADJ_SEED, [&] { return launchAdjHalfEnd(a, 1, adjS); });
         a = adjView(slot, 0);
+        a.kkNext = tapeK(slot, 0, nit - 1);
         timed(K_ADJ_SEED, [&] { return launchAdjHalfStart(a, 0, adjS); });
         for (int it = nit - 1; it >= 0; --it) {
             adjPass(a, slot, 0, it);
@@ -2397,7 +2409,7 @@ class Engine {
         if (!rec.valid || rec.epoch != stateEpoch) throw SphError{SPH_E_STATE, "no recorded rollout for this state"};
         checkDifferentiable();
         HIPCHK(hipSetDevice(device));
-        for (auto* b : {&adjX, &adjV, &aBar, &xBar, &vBar, &nBar, &sgBar})
+        for (auto* b : {&adjX, &adjV, &aBar, &xBar, &vBar, &nBar, &sgBar, &suBar})
             if (b->n != NP) b->alloc(NP);
         for (auto* b : {&rhoBar, &alphaBar, &kBar, &sBar, &adjKws, &adjKvws, &adjRho, &adjAlpha})
             if (b->n != NP) b->alloc(NP);
