@@ -115,6 +115,26 @@ __device__ __forceinline__ void lanesSum3(F3& v) {
     v.y = lppSum<kAdjLpp>(v.y);
     v.z = lppSum<kAdjLpp>(v.z);
 }
+// Lane `sub` of kAdjLpp visits list slots sub, sub + kAdjLpp, ... < n of
+// particle i (entries s NP + i): per round it loads B list entries, then
+// their gathers, then consumes them in slot order. Slots past n load slot
+// n - 1 (a valid entry) and are not consumed, so every pair of the list is
+// consumed once, the self pair included, and no padding row is.
+template <uint32_t B, class G, class C>
+__device__ __forceinline__ void laneWalk(const uint32_t* list, size_t NP, uint32_t i, uint32_t sub, uint32_t n,
+                                         G&& gather, C&& consume) {
+    for (uint32_t s0 = sub; s0 < n; s0 += B * kAdjLpp) {
+        uint32_t j[B];
+#pragma unroll
+        for (uint32_t k = 0; k < B; ++k) j[k] = list[static_cast<size_t>(min(s0 + k * kAdjLpp, n - 1u)) * NP + i];
+        decltype(gather(0u)) g[B];
+#pragma unroll
+        for (uint32_t k = 0; k < B; ++k) g[k] = gather(j[k]);
+#pragma unroll
+        for (uint32_t k = 0; k < B; ++k)
+            if (s0 + k * kAdjLpp < n) consume(g[k]);
+    }
+}
 
 // Adjoint of the symplectic Euler update: v-tilde = v'-bar + dt x'-bar is the
 // adjoint of v'; a-bar = dt v-tilde; x-bar and v-bar start from x'-bar and
@@ -136,39 +156,44 @@ __global__ void __launch_bounds__(kBlock) kAdjSeed(AdjView a) {
 // rho-bar (partial, incl. the pressure chain p(rho) and A = p / rho^2) and the
 // per-block sums of a-bar . da/dnu and a-bar . da/dgam.
 __global__ void __launch_bounds__(kBlock) kAdjForces(AdjView a, int pressure) {
-    ADJ_PROLOGUE
-    double nuTerm = 0.0, stTerm = 0.0;
-    if (valid) {
-        const size_t NP = a.NP;
-        const F3 xi = f3(a.x[i]), vi = f3(a.v[i]), ai = f3(a.aBar[i]), ni = f3(a.n[i]);
-        const float rhoi = a.rho[i], pi_ = pressure ? a.p[i] : 0.0f;
-        const float Ai = pi_ / (rhoi * rhoi);
-        const bool xsph = S.viscMethod == 2, st = S.stOn != 0;
-        const float invdt = 1.0f / dt;
-        const float cnu = S.visc * invdt;
-        const float mi = S.m / rhoi;
-        F3 gx{0.0f, 0.0f, 0.0f}, gv{0.0f, 0.0f, 0.0f}, gn{0.0f, 0.0f, 0.0f};
-        float gr = 0.0f, gA = 0.0f, nuSum = 0.0f, stSum = 0.0f;
-        const uint32_t nf = min(a.cntF[i], a.capF);
-        for (uint32_t s = 0; s < nf; ++s) {
-            const uint32_t j = a.nbrF[s * NP + i];
-            const F3 r = xi - f3(a.x[j]);
+    ADJ_PROLOGUE_LANES(kAdjLpp)
+    const size_t NP = a.NP;
+    const uint32_t ii = valid ? i : 0u;   // invalid lanes run empty loops up to the lane sums
+    const F3 xi = f3(a.x[ii]), vi = f3(a.v[ii]), ai = f3(a.aBar[ii]), ni = f3(a.n[ii]);
+    const float rhoi = a.rho[ii], pi_ = pressure ? a.p[ii] : 0.0f;
+    const float Ai = pi_ / (rhoi * rhoi);
+    const bool xsph = S.viscMethod == 2, st = S.stOn != 0;
+    const float invdt = 1.0f / dt;
+    const float cnu = S.visc * invdt;
+    const float mi = S.m / rhoi;
+    F3 gx{0.0f, 0.0f, 0.0f}, gv{0.0f, 0.0f, 0.0f}, gn{0.0f, 0.0f, 0.0f};
+    float gr = 0.0f, gA = 0.0f, nuSum = 0.0f, stSum = 0.0f;
+    const uint32_t nf = valid ? min(a.cntF[i], a.capF) : 0u;
+    struct Nb { float4 x, a, v, n; float rho, p; };
+    const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    laneWalk<2>(
+        a.nbrF, NP, ii, sub, nf,
+        [&](uint32_t j) {
+            return Nb{a.x[j], a.aBar[j], xsph ? a.v[j] : z4, st ? a.n[j] : z4, a.rho[j], pressure ? a.p[j] : 0.0f};
+        },
+        [&](const Nb& q) {
+            const F3 r = xi - f3(q.x);
             const float rl = fastSqrt(dot3(r, r));
             const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
-            const F3 aj = f3(a.aBar[j]);
+            const F3 aj = f3(q.a);
             const F3 da = ai - aj;
             const float rda = dot3(r, da);
-            const float rhoj = a.rho[j];
+            const float rhoj = q.rho;
             if (pressure) {
                 // A-bar_k = -m sum (a_k - a_j) . gradW; x-bar_k -= m (A_k + A_j) H (a_k - a_j)
-                const float Aj = a.p[j] / (rhoj * rhoj);
+                const float Aj = q.p / (rhoj * rhoj);
                 gA -= S.m * G * rda;
                 const float cP = -S.m * (Ai + Aj);
                 axpy(gx, cP * G, da);
                 axpy(gx, cP * G2 * rda, r);
             }
             if (xsph) {
-                const F3 dv = f3(a.v[j]) - vi;
+                const F3 dv = f3(q.v) - vi;
                 const float W = cubicW(S, rl);
                 const float mj = S.m / rhoj;
                 axpy(gv, cnu * W * mi, aj);
@@ -185,7 +210,7 @@ __global__ void __launch_bounds__(kBlock) kAdjForces(AdjView a, int pressure) {
                     const float kij = 2.0f * S.rho0 / (rhoi + rhoj);
                     const float D = C / rs;
                     const float Dp = (dC * rs - C) / (rs * rs * rs);   // D'(r) / r
-                    const F3 dn = ni - f3(a.n[j]);
+                    const F3 dn = ni - f3(q.n);
                     F3 sv{0.0f, 0.0f, 0.0f};   // S_kj = -gam m D r - gam (n_k - n_j)
                     axpy(sv, -S.st * S.m * D, r);
                     axpy(sv, -S.st, dn);
@@ -197,36 +222,46 @@ __global__ void __launch_bounds__(kBlock) kAdjForces(AdjView a, int pressure) {
                     stSum += kij * (-S.m * D * dot3(ai, r) - dot3(ai, dn));
                 }
             }
-        }
-        float gpMirror = 0.0f;
-        if (S.akinci) {
-            const uint32_t nb = min(a.cntB[i], a.capB);
-            const float pm = pi_ / (S.rho0 * S.rho0);
-            const float cb = (xsph && S.bviscOn) ? S.bvisc * invdt : 0.0f;
-            const float vai = dot3(ai, vi);
-            for (uint32_t s = 0; s < nb; ++s) {
-                const float4 q = a.xb[a.nbrB[s * NP + i]];
-                const F3 r = xi - f3(q);
-                const float rl = fastSqrt(dot3(r, r));
-                const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
-                const float psi = q.w;
-                const float rai = dot3(r, ai);
-                if (pressure) {
-                    gA -= psi * G * rai;
-                    gpMirror -= psi * G * rai;
-                    const float cPb = -psi * (Ai + pm);
-                    axpy(gx, cPb * G, ai);
-                    axpy(gx, cPb * G2 * rai, r);
-                }
-                if (cb != 0.0f) {
-                    const float W = cubicW(S, rl);
-                    axpy(gv, -cb * (psi / rhoi) * W, ai);
-                    gr += cb * (psi / (rhoi * rhoi)) * W * vai;
-                    axpy(gx, -cb * (psi / rhoi) * vai * G, r);
-                }
+        });
+    float gpMirror = 0.0f;
+    const float pm = pi_ / (S.rho0 * S.rho0);
+    if (valid && S.akinci) {
+        const uint32_t nb = min(a.cntB[i], a.capB);
+        const float cb = (xsph && S.bviscOn) ? S.bvisc * invdt : 0.0f;
+        const float vai = dot3(ai, vi);
+        for (uint32_t s = sub; s < nb; s += kAdjLpp) {
+            const float4 q = a.xb[a.nbrB[s * NP + i]];
+            const F3 r = xi - f3(q);
+            const float rl = fastSqrt(dot3(r, r));
+            const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
+            const float psi = q.w;
+            const float rai = dot3(r, ai);
+            if (pressure) {
+                gA -= psi * G * rai;
+                gpMirror -= psi * G * rai;
+                const float cPb = -psi * (Ai + pm);
+                axpy(gx, cPb * G, ai);
+                axpy(gx, cPb * G2 * rai, r);
             }
-            gpMirror /= S.rho0 * S.rho0;
+            if (cb != 0.0f) {
+                const float W = cubicW(S, rl);
+                axpy(gv, -cb * (psi / rhoi) * W, ai);
+                gr += cb * (psi / (rhoi * rhoi)) * W * vai;
+                axpy(gx, -cb * (psi / rhoi) * vai * G, r);
+            }
         }
+    }
+    lanesSum3(gx);
+    lanesSum3(gv);
+    lanesSum3(gn);
+    gr = lppSum<kAdjLpp>(gr);
+    gA = lppSum<kAdjLpp>(gA);
+    nuSum = lppSum<kAdjLpp>(nuSum);
+    stSum = lppSum<kAdjLpp>(stSum);
+    gpMirror = lppSum<kAdjLpp>(gpMirror);
+    double nuTerm = 0.0, stTerm = 0.0;
+    if (valid && sub == 0) {
+        gpMirror /= S.rho0 * S.rho0;
         // A = p / rho^2 and p(rho)
         const float gp = gA / (rhoi * rhoi) + gpMirror;
         gr -= 2.0f * gA * pi_ / (rhoi * rhoi * rhoi);
@@ -239,39 +274,46 @@ __global__ void __launch_bounds__(kBlock) kAdjForces(AdjView a, int pressure) {
         nuTerm = static_cast<double>(nuSum) * static_cast<double>(invdt);
         stTerm = static_cast<double>(stSum);
     }
+    // one partial per launch block (kAdjLpp per particle block): kAdjFinish
     const double t0 = blockReduce<false>(nuTerm);
     const double t1 = blockReduce<false>(stTerm);
     if (threadIdx.x == 0) {
         a.partials[lb] = t0;
-        a.partials[a.numBlocks + lb] = t1;
+        a.partials[a.numBlocks * kAdjLpp + lb] = t1;
     }
 }
 
 // Pass 2 (surface tension only): normal-bar through n_i = h sum (m / rho_j) gradW_ij.
 __global__ void __launch_bounds__(kBlock) kAdjNormals(AdjView a) {
-    ADJ_PROLOGUE
-    if (!valid || !S.stOn) return;
+    ADJ_PROLOGUE_LANES(kAdjLpp)
+    if (!S.stOn) return;   // uniform over the launch block (one scene)
     const size_t NP = a.NP;
-    const F3 xi = f3(a.x[i]), nbi = f3(a.nBar[i]);
-    const float rhoi = a.rho[i];
+    const uint32_t ii = valid ? i : 0u;
+    const F3 xi = f3(a.x[ii]), nbi = f3(a.nBar[ii]);
+    const float rhoi = a.rho[ii];
     const float hm = S.h * S.m;
     F3 gx{0.0f, 0.0f, 0.0f};
     float gr = 0.0f;
-    const uint32_t nf = min(a.cntF[i], a.capF);
-    for (uint32_t s = 0; s < nf; ++s) {
-        const uint32_t j = a.nbrF[s * NP + i];
-        const F3 r = xi - f3(a.x[j]);
-        const float rl = fastSqrt(dot3(r, r));
-        const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
-        const F3 nbj = f3(a.nBar[j]);
-        const float rhoj = a.rho[j];
-        gr += hm / (rhoi * rhoi) * G * dot3(nbj, r);
-        F3 u{0.0f, 0.0f, 0.0f};   // n-bar_k / rho_j - n-bar_j / rho_k
-        axpy(u, 1.0f / rhoj, nbi);
-        axpy(u, -1.0f / rhoi, nbj);
-        axpy(gx, hm * G, u);
-        axpy(gx, hm * G2 * dot3(r, u), r);
-    }
+    const uint32_t nf = valid ? min(a.cntF[i], a.capF) : 0u;
+    struct Nb { float4 x, n; float rho; };
+    laneWalk<2>(
+        a.nbrF, NP, ii, sub, nf, [&](uint32_t j) { return Nb{a.x[j], a.nBar[j], a.rho[j]}; },
+        [&](const Nb& q) {
+            const F3 r = xi - f3(q.x);
+            const float rl = fastSqrt(dot3(r, r));
+            const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
+            const F3 nbj = f3(q.n);
+            const float rhoj = q.rho;
+            gr += hm / (rhoi * rhoi) * G * dot3(nbj, r);
+            F3 u{0.0f, 0.0f, 0.0f};   // n-bar_k / rho_j - n-bar_j / rho_k
+            axpy(u, 1.0f / rhoj, nbi);
+            axpy(u, -1.0f / rhoi, nbj);
+            axpy(gx, hm * G, u);
+            axpy(gx, hm * G2 * dot3(r, u), r);
+        });
+    lanesSum3(gx);
+    gr = lppSum<kAdjLpp>(gr);
+    if (!valid || sub != 0) return;
     a.rhoBar[i] += gr;
     const float4 x0 = a.xBar[i];
     a.xBar[i] = make_float4(x0.x + gx.x, x0.y + gx.y, x0.z + gx.z, 0.0f);
@@ -280,30 +322,34 @@ __global__ void __launch_bounds__(kBlock) kAdjNormals(AdjView a) {
 // Pass 3: rho-bar through the density sum; writes the adjoints of the state
 // before the step back in original order.
 __global__ void __launch_bounds__(kBlock) kAdjDensity(AdjView a) {
-    ADJ_PROLOGUE
-    if (!valid) return;
+    ADJ_PROLOGUE_LANES(kAdjLpp)
     const size_t NP = a.NP;
-    const F3 xi = f3(a.x[i]);
-    const float rk = a.rhoBar[i];
-    F3 gx = f3(a.xBar[i]);
-    const uint32_t nf = min(a.cntF[i], a.capF);
-    for (uint32_t s = 0; s < nf; ++s) {
-        const uint32_t j = a.nbrF[s * NP + i];
-        const F3 r = xi - f3(a.x[j]);
-        const float G = gradFactor(S, fastSqrt(dot3(r, r)));
-        axpy(gx, S.m * (rk + a.rhoBar[j]) * G, r);
-    }
-    if (S.akinci) {
+    const uint32_t ii = valid ? i : 0u;
+    const F3 xi = f3(a.x[ii]);
+    const float rk = a.rhoBar[ii];
+    F3 gx{0.0f, 0.0f, 0.0f};
+    const uint32_t nf = valid ? min(a.cntF[i], a.capF) : 0u;
+    struct Nb { float4 x; float rb; };
+    laneWalk<4>(a.nbrF, NP, ii, sub, nf, [&](uint32_t j) { return Nb{a.x[j], a.rhoBar[j]}; },
+                [&](const Nb& q) {
+                    const F3 r = xi - f3(q.x);
+                    const float G = gradFactor(S, fastSqrt(dot3(r, r)));
+                    axpy(gx, S.m * (rk + q.rb) * G, r);
+                });
+    if (valid && S.akinci) {
         const uint32_t nb = min(a.cntB[i], a.capB);
-        for (uint32_t s = 0; s < nb; ++s) {
+        for (uint32_t s = sub; s < nb; s += kAdjLpp) {
             const float4 q = a.xb[a.nbrB[s * NP + i]];
             const F3 r = xi - f3(q);
             const float G = gradFactor(S, fastSqrt(dot3(r, r)));
             axpy(gx, q.w * rk * G, r);
         }
     }
+    lanesSum3(gx);
+    if (!valid || sub != 0) return;
+    const float4 x0 = a.xBar[i];
     const uint32_t o = S.fBegin + a.id[i];
-    a.adjX[o] = f4(gx);
+    a.adjX[o] = make_float4(x0.x + gx.x, x0.y + gx.y, x0.z + gx.z, 0.0f);
     a.adjV[o] = a.vBar[i];
 }
 
@@ -313,9 +359,10 @@ __global__ void __launch_bounds__(kBlock) kAdjFinish(AdjView a) {
     const int s = blockIdx.x;
     const SceneConst& C = a.scenes[s];
     double v0 = 0.0, v1 = 0.0;
-    for (uint32_t b = C.blockBegin + threadIdx.x; b < C.blockEnd; b += kBlock) {
+    const uint32_t n = a.numBlocks * kAdjLpp;   // kAdjForces' launch blocks
+    for (uint32_t b = C.blockBegin * kAdjLpp + threadIdx.x; b < C.blockEnd * kAdjLpp; b += kBlock) {
         v0 += a.partials[b];
-        v1 += a.partials[a.numBlocks + b];
+        v1 += a.partials[n + b];
     }
     v0 = blockReduce<false>(v0);
     v1 = blockReduce<false>(v1);
@@ -343,11 +390,6 @@ __global__ void __launch_bounds__(kBlock) kAdjFinish(AdjView a) {
 //   u'_i   = u_i - dt sum_j m (k_i + k_j) gradW_ij [|k_i+k_j| > eps] - dt sum_b psi_b k_i gradW_ib [|k_i| > eps]
 // Warm starts take k = k_ws / dt^2 where rho* > rho0 (:873-884), k = kv_ws / dt
 // where d > 0 (:977-989). Gates and clamps are piecewise constant.
-
-#define ADJ_GRAD_PAIR(xi, j)                               \
-    const F3 r = xi - f3(a.x[j]);                          \
-    const float rl = fastSqrt(dot3(r, r));                 \
-    const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
 
 // Gather the carried adjoints into the slot order of one half.
 //   second half: x-bar, u-bar (= v-bar of the step's result), rho-bar and
@@ -550,28 +592,30 @@ __global__ void __launch_bounds__(kBlock) kAdjPredict(AdjView a) {
 // S = sum_j m gradW_ij + sum_b psi_b gradW_ib. rho-bar += alpha-bar / den;
 // den-bar = -alpha-bar rho / den^2; sgBar = (2 den-bar S, den-bar).
 __global__ void __launch_bounds__(kBlock) kAdjAlphaA(AdjView a) {
-    ADJ_PROLOGUE
-    if (!valid) return;
+    ADJ_PROLOGUE_LANES(kAdjLpp)
     const size_t NP = a.NP;
-    const F3 xi = f3(a.x[i]);
+    const uint32_t ii = valid ? i : 0u;
+    const F3 xi = f3(a.x[ii]);
     F3 sg{0.0f, 0.0f, 0.0f};
     float ssq = 0.0f;
-    const uint32_t nf = min(a.cntF[i], a.capF);
-    for (uint32_t s = 0; s < nf; ++s) {
-        const uint32_t j = a.nbrF[s * NP + i];
-        const F3 r = xi - f3(a.x[j]);
+    const uint32_t nf = valid ? min(a.cntF[i], a.capF) : 0u;
+    laneWalk<4>(a.nbrF, NP, ii, sub, nf, [&](uint32_t j) { return a.x[j]; }, [&](const float4& xj) {
+        const F3 r = xi - f3(xj);
         const float g = S.m * gradFactor(S, fastSqrt(dot3(r, r)));
         axpy(sg, g, r);
         ssq += g * g * dot3(r, r);
-    }
-    if (S.akinci) {
+    });
+    if (valid && S.akinci) {
         const uint32_t nb = min(a.cntB[i], a.capB);
-        for (uint32_t s = 0; s < nb; ++s) {
+        for (uint32_t s = sub; s < nb; s += kAdjLpp) {
             const float4 q = a.xb[a.nbrB[s * NP + i]];
             const F3 r = xi - f3(q);
             axpy(sg, q.w * gradFactor(S, fastSqrt(dot3(r, r))), r);
         }
     }
+    lanesSum3(sg);
+    ssq = lppSum<kAdjLpp>(ssq);
+    if (!valid || sub != 0) return;
     const float den = dot3(sg, sg) + ssq;
     float dbar = 0.0f;
     if (den > S.eps) {
@@ -584,28 +628,31 @@ __global__ void __launch_bounds__(kBlock) kAdjAlphaA(AdjView a) {
 
 // Part 2: x-bar through S, through sum |m gradW|^2 and through the density sum.
 __global__ void __launch_bounds__(kBlock) kAdjAlphaB(AdjView a) {
-    ADJ_PROLOGUE
-    if (!valid) return;
+    ADJ_PROLOGUE_LANES(kAdjLpp)
     const size_t NP = a.NP;
-    const F3 xi = f3(a.x[i]);
-    const float4 sgi = a.sgBar[i];
+    const uint32_t ii = valid ? i : 0u;
+    const F3 xi = f3(a.x[ii]);
+    const float4 sgi = a.sgBar[ii];
     const F3 si = f3(sgi);
-    const float rk = a.rhoBar[i];
-    F3 gx = f3(a.xBar[i]);
-    const uint32_t nf = min(a.cntF[i], a.capF);
-    for (uint32_t s = 0; s < nf; ++s) {
-        const uint32_t j = a.nbrF[s * NP + i];
-        ADJ_GRAD_PAIR(xi, j)
-        const float4 sgj = a.sgBar[j];
-        const F3 w = si - f3(sgj);
-        axpy(gx, S.m * G, w);
-        axpy(gx, S.m * G2 * dot3(r, w), r);
-        const float cq = 2.0f * S.m * S.m * (sgi.w + sgj.w) * G * (G + G2 * dot3(r, r));
-        axpy(gx, cq + S.m * (rk + a.rhoBar[j]) * G, r);
-    }
-    if (S.akinci) {
+    const float rk = a.rhoBar[ii];
+    F3 gx{0.0f, 0.0f, 0.0f};
+    const uint32_t nf = valid ? min(a.cntF[i], a.capF) : 0u;
+    struct Nb { float4 x, sg; float rb; };
+    laneWalk<2>(
+        a.nbrF, NP, ii, sub, nf, [&](uint32_t j) { return Nb{a.x[j], a.sgBar[j], a.rhoBar[j]}; },
+        [&](const Nb& q) {
+            const F3 r = xi - f3(q.x);
+            const float rl = fastSqrt(dot3(r, r));
+            const float G = gradFactor(S, rl), G2 = gradFactor2(S, rl);
+            const F3 w = si - f3(q.sg);
+            axpy(gx, S.m * G, w);
+            axpy(gx, S.m * G2 * dot3(r, w), r);
+            const float cq = 2.0f * S.m * S.m * (sgi.w + q.sg.w) * G * (G + G2 * dot3(r, r));
+            axpy(gx, cq + S.m * (rk + q.rb) * G, r);
+        });
+    if (valid && S.akinci) {
         const uint32_t nb = min(a.cntB[i], a.capB);
-        for (uint32_t s = 0; s < nb; ++s) {
+        for (uint32_t s = sub; s < nb; s += kAdjLpp) {
             const float4 q = a.xb[a.nbrB[s * NP + i]];
             const F3 r = xi - f3(q);
             const float rl = fastSqrt(dot3(r, r));
@@ -614,7 +661,10 @@ __global__ void __launch_bounds__(kBlock) kAdjAlphaB(AdjView a) {
             axpy(gx, q.w * G2 * dot3(r, si) + q.w * rk * G, r);
         }
     }
-    a.xBar[i] = f4(gx);
+    lanesSum3(gx);
+    if (!valid || sub != 0) return;
+    const float4 x0 = a.xBar[i];
+    a.xBar[i] = make_float4(x0.x + gx.x, x0.y + gx.y, x0.z + gx.z, 0.0f);
 }
 
 #define ADJ_LAUNCH(kernel, grid)                                             \
@@ -624,15 +674,16 @@ __global__ void __launch_bounds__(kBlock) kAdjAlphaB(AdjView a) {
         return hipGetLastError();                                            \
     } while (0)
 
+int adjLanesPerParticle() { return static_cast<int>(kAdjLpp); }
 hipError_t launchAdjSeed(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjSeed, a.numBlocks); }
 hipError_t launchAdjForces(const AdjView& a, hipStream_t st) { return launchAdjForces2(a, 1, st); }
 hipError_t launchAdjForces2(const AdjView& a, int pressure, hipStream_t st) {
     if (a.numBlocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(kAdjForces, dim3(a.numBlocks), dim3(kBlock), 0, st, a, pressure);
+    hipLaunchKernelGGL(kAdjForces, dim3(a.numBlocks * kAdjLpp), dim3(kBlock), 0, st, a, pressure);
     return hipGetLastError();
 }
-hipError_t launchAdjNormals(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjNormals, a.numBlocks); }
-hipError_t launchAdjDensity(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjDensity, a.numBlocks); }
+hipError_t launchAdjNormals(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjNormals, a.numBlocks * kAdjLpp); }
+hipError_t launchAdjDensity(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjDensity, a.numBlocks * kAdjLpp); }
 hipError_t launchAdjFinish(const AdjView& a, hipStream_t st) {
     ADJ_LAUNCH(kAdjFinish, static_cast<uint32_t>(a.numScenes));
 }
@@ -657,7 +708,7 @@ hipError_t launchAdjSource(const AdjView& a, int mode, hipStream_t st) {
     return hipGetLastError();
 }
 hipError_t launchAdjPredict(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjPredict, a.numBlocks); }
-hipError_t launchAdjAlphaA(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjAlphaA, a.numBlocks); }
-hipError_t launchAdjAlphaB(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjAlphaB, a.numBlocks); }
+hipError_t launchAdjAlphaA(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjAlphaA, a.numBlocks * kAdjLpp); }
+hipError_t launchAdjAlphaB(const AdjView& a, hipStream_t st) { ADJ_LAUNCH(kAdjAlphaB, a.numBlocks * kAdjLpp); }
 
 }  // namespace sph

@@ -41,7 +41,7 @@ struct AdjView {
     float4* vBar;
     float4* nBar;
     float* rhoBar;
-    double* partials;    // 2 * numBlocks: viscosity and surface-tension terms
+    double* partials;    // 2 * numBlocks * adjLanesPerParticle(): viscosity and surface-tension terms
     double* acc;         // 2 per scene: dL/d(viscosity), dL/d(surface tension)
     // DFSPH: one Jacobi pass of the recorded step
     const float4* u;     // velocity the pass read
@@ -61,6 +61,10 @@ struct AdjView {
     float* adjRho;       // of rho / alpha at the start-of-step positions
     float* adjAlpha;
 };
+
+// Launch blocks per particle block of the adjoint neighbour passes (the
+// partials of kAdjForces are per launch block)
+int adjLanesPerParticle();
 
 // DFSPH pass kinds of the adjoint kernels
 enum AdjMode { kAdjDensityLoop = 0, kAdjDivergenceLoop = 1, kAdjDensityWarm = 2, kAdjDivergenceWarm = 3 };

@@ -2413,7 +2413,7 @@ class Engine {
             if (b->n != NP) b->alloc(NP);
         for (auto* b : {&rhoBar, &alphaBar, &kBar, &sBar, &adjKws, &adjKvws, &adjRho, &adjAlpha})
             if (b->n != NP) b->alloc(NP);
-        adjPartials.alloc(2 * static_cast<size_t>(numBlocks ? numBlocks : 1));
+        adjPartials.alloc(2 * static_cast<size_t>(numBlocks ? numBlocks : 1) * static_cast<size_t>(adjLanesPerParticle()));
         adjAcc.alloc(2 * scenes.size());
         std::vector<float4> hx(NP, make_float4(0, 0, 0, 0)), hv(NP, make_float4(0, 0, 0, 0));
         for (size_t si = 0; si < scenes.size(); ++si) {
