@@ -306,6 +306,9 @@ class Engine {
         if (hState) (void)hipHostFree(hState);
         hState = nullptr;
         hFlags = nullptr;
+        if (hReplayFlags) (void)hipHostFree(hReplayFlags);
+        hReplayFlags = nullptr;
+        replayFlagCap = 0;
     }
 
     // ------------------------------------------------------------ scenes
@@ -1313,13 +1316,14 @@ class Engine {
         s.keyRange += b - s.ovBuckets;
         s.ovBuckets = b;
     }
-    // Search and check at once (one host sync).
-    void neighborSearch(bool dfsphAdvance) {
-        if (!NP) return;
+    // Search and check at once (one host sync per attempt). Returns the
+    // number of attempts the search took.
+    int neighborSearch(bool dfsphAdvance) {
+        if (!NP) return 0;
         for (int attempt = 0; attempt < 6; ++attempt) {
             searchOnce(dfsphAdvance && attempt == 0);
             readState();
-            if (searchValid()) return;
+            if (searchValid()) return attempt + 1;
         }
         throw SphError{SPH_E_OUT_OF_GRID, "could not place particles into a grid"};
     }
@@ -1647,9 +1651,25 @@ class Engine {
         uint64_t epoch = 0;
         std::vector<Checkpoint> ckpt;
         Checkpoint last;
+        // per recorded step (the one search of a WCSPH step, the second
+        // half's of a DFSPH step): the flags of its accepted search
+        // (kFlagSlots per scene), the list rows it put on the tape, and
+        // whether it took more than one attempt
+        std::vector<int32_t> flags;
+        std::vector<uint32_t> rowsF, rowsB;
+        std::vector<uint8_t> retried;
     };
     Recording rec;
     Tape tape;
+    // Recorded steps: recBase is the global index of slot 0 of the segment
+    // being recorded or replayed; while `replaying`, searches that took one
+    // attempt in the recording are not waited for (recordedSearch), and
+    // their flags land in hReplayFlags (pinned, replayFlagCap entries) for
+    // the first replayChecks slots of the segment starting at replayBase.
+    bool replaying = false;
+    int recBase = 0, replayBase = 0, replayChecks = 0;
+    int32_t* hReplayFlags = nullptr;
+    size_t replayFlagCap = 0;
     // The reverse sweep runs two segments at once: the adjoint passes of one
     // segment read its tape (tapeRd) on adjStream while the forward recomputes
     // the segment before it into `tape` on the engine's stream; the tapes swap
@@ -2050,19 +2070,95 @@ class Engine {
         tape.capF = std::max(tape.capF, capF);
         tape.capB = std::max(tape.capB, capB);
     }
-    // x, ids and lists of the current search into a half of step `slot`; of
-    // the lists only the rows some particle uses (the longest list of the
-    // search, from the flags the host read after it, rounded up to whole
-    // kPadRows rounds: the self-padding rows the adjoint Jacobi passes walk)
-    void recordSearch(HalfTape& t, int slot) {
-        const size_t np = NP, o = static_cast<size_t>(slot) * np;
-        uint32_t rowsF = 0, rowsB = 0;
+    // The list rows some particle of the last search uses: its longest list,
+    // from the flags the host read after it, rounded up to whole kPadRows
+    // rounds (the self-padding rows the adjoint Jacobi passes walk).
+    void listRows(uint32_t& rowsF, uint32_t& rowsB) const {
+        rowsF = 0;
+        rowsB = 0;
         for (size_t si = 0; si < scenes.size(); ++si) {
             rowsF = std::max(rowsF, static_cast<uint32_t>(std::max(0, hFlags[si * kFlagSlots + kFlagMaxFluidNbr])));
             rowsB = std::max(rowsB, static_cast<uint32_t>(std::max(0, hFlags[si * kFlagSlots + kFlagMaxBoundaryNbr])));
         }
         rowsF = std::min((rowsF + kPadRows - 1u) / kPadRows * kPadRows, capF);   // with the self-padding rows
         rowsB = std::min(rowsB, capB);
+    }
+    // The search of step `slot` of a recorded segment (global step recBase +
+    // slot). The recording checks it on the host (neighborSearch) and files
+    // its flags, list rows and attempt count. A replay (the reverse sweep's
+    // recompute) runs from the segment's checkpoint with the recording's grid
+    // and at least its list capacities, so a step whose recorded search took
+    // one attempt searches the same positions the same way: it is queued
+    // without a host wait, and its flags are copied to hReplayFlags, which
+    // replayCheck compares with the recorded ones once the segment has run.
+    // A step whose search took more attempts is checked on the host again.
+    void recordedSearch(int slot, bool dfsphAdvance, uint32_t& rowsF, uint32_t& rowsB) {
+        const size_t g = static_cast<size_t>(recBase + slot);
+        const size_t nf = scenes.size() * kFlagSlots;
+        if (replaying && !rec.retried[g]) {
+            searchOnce(dfsphAdvance);
+            HIPCHK(hipMemcpyAsync(hReplayFlags + static_cast<size_t>(slot) * nf, dFlags, nf * sizeof(int32_t),
+                                  hipMemcpyDeviceToHost, stream));
+            replayChecks = std::max(replayChecks, slot + 1);
+            rowsF = rec.rowsF[g];
+            rowsB = rec.rowsB[g];
+            return;
+        }
+        const int attempts = neighborSearch(dfsphAdvance);
+        listRows(rowsF, rowsB);
+        if (replaying) {
+            if (!std::equal(hFlags, hFlags + nf, rec.flags.begin() + static_cast<std::ptrdiff_t>(g * nf)))
+                throw SphError{SPH_E_STATE, "a recomputed step's search differs from the recorded one"};
+            std::copy(hFlags, hFlags + nf, hReplayFlags + static_cast<size_t>(slot) * nf);
+            replayChecks = std::max(replayChecks, slot + 1);
+            return;
+        }
+        std::copy(hFlags, hFlags + nf, rec.flags.begin() + static_cast<std::ptrdiff_t>(g * nf));
+        rec.rowsF[g] = rowsF;
+        rec.rowsB[g] = rowsB;
+        rec.retried[g] = attempts > 1;
+    }
+    // Start replaying the segment whose first step is global step `base`
+    // (checking the flags of the segment replayed before it first).
+    void beginReplay(int base) {
+        replayCheck();
+        const size_t nf = scenes.size() * kFlagSlots;
+        const size_t need = static_cast<size_t>(rec.every) * nf;
+        if (replayFlagCap < need) {
+            if (hReplayFlags) HIPCHK(hipHostFree(hReplayFlags));
+            hReplayFlags = nullptr;
+            replayFlagCap = 0;
+            HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&hReplayFlags), need * sizeof(int32_t)));
+            replayFlagCap = need;
+        }
+        replaying = true;
+        recBase = base;
+        replayBase = base;
+    }
+    // Wait for the replayed segment and compare the flags of its searches
+    // with the recording's: equal flags (integer counts of the search) say
+    // that the recompute took the recorded path.
+    void replayCheck() {
+        replaying = false;
+        if (!replayChecks) return;
+        HIPCHK(hipStreamSynchronize(stream));
+        const size_t nf = scenes.size() * kFlagSlots;
+        const int n = replayChecks;
+        replayChecks = 0;
+        for (int s = 0; s < n; ++s) {
+            const size_t g = static_cast<size_t>(replayBase + s);
+            const int32_t* got = hReplayFlags + static_cast<size_t>(s) * nf;
+            for (size_t si = 0; si < scenes.size(); ++si)
+                if (got[si * kFlagSlots + kFlagNonFinite])
+                    throw SphError{SPH_E_NONFINITE, "non-finite particle position"};
+            if (!std::equal(got, got + nf, rec.flags.begin() + static_cast<std::ptrdiff_t>(g * nf)))
+                throw SphError{SPH_E_STATE, "a recomputed step's search differs from the recorded one"};
+        }
+    }
+    // x, ids and lists of the current search into a half of step `slot`;
+    // of the lists only `rowsF` / `rowsB` rows (listRows)
+    void recordSearch(HalfTape& t, int slot, uint32_t rowsF, uint32_t rowsB) {
+        const size_t np = NP, o = static_cast<size_t>(slot) * np;
         tapeCopy(t.x.p + o, x.p, np * sizeof(float4));
         tapeCopy(t.id.p + o, id.p, np * sizeof(uint32_t));
         tapeCopy(t.cntF.p + o, cntF.p, np * sizeof(uint32_t));
@@ -2100,12 +2196,13 @@ class Engine {
     // stepWCSPH with the tape of step `slot` of the current segment.
     void stepWCSPHRecord(int slot) {
         beginStats();
-        neighborSearch(false);
+        uint32_t rowsF = 0, rowsB = 0;
+        recordedSearch(slot, false, rowsF, rowsB);
         if (capF > tape.capF || capB > tape.capB) tapeGrow(slot);
         DevView d = view();
         HalfTape& t = tape.h[0];
         const size_t np = NP, o = static_cast<size_t>(slot) * np;
-        recordSearch(t, slot);
+        recordSearch(t, slot, rowsF, rowsB);
         tapeCopy(t.v.p + o, v.p, np * sizeof(float4));
         timed(K_DENSITY, [&] { return launchDensityPressure(d, stream); });
         timed(K_FORCES, [&] { return launchForces(d, 1, stream); });
@@ -2115,6 +2212,7 @@ class Engine {
         if (anyST()) tapeCopy(t.n.p + o, n.p, np * sizeof(float4));
         flushCopies();   // before the integration moves x and v
         timed(K_INTEGRATE, [&] { return launchIntegrate(d, stream); });
+        if (replaying) return;   // no statistics: the recording filed them
         for (size_t s = 0; s < scenes.size(); ++s) hState[s].t = hState[s].t + hState[s].dt;
         fileStats(false);
     }
@@ -2133,7 +2231,9 @@ class Engine {
         // of slot - 1 (searchHalf), and its start velocity was stored by the
         // last velocity pass of slot - 1; only a segment's first step copies
         if (slot == 0) {
-            recordSearch(A, slot);
+            uint32_t rowsF = 0, rowsB = 0;
+            listRows(rowsF, rowsB);   // the flags of the search before this step (checked on the host)
+            recordSearch(A, slot, rowsF, rowsB);
             tapeCopy(A.v.p + o, v.p, np * sizeof(float4));
             tapeCopy(A.rho.p + o, rho.p, np * sizeof(float));
             tapeCopy(A.alpha.p + o, alpha.p, np * sizeof(float));
@@ -2152,17 +2252,15 @@ class Engine {
             if (!fuseFinish) finish(d, kFinishDensity, it & 1);
             velocityPass(velView(d, it + 1 < nit ? uSlot(A, slot, it + 1) : nullptr), it & 1, kFinishDensity);
         }
-        readState();
-        for (size_t s = 0; s < scenes.size(); ++s) scenes[s].its = hState[s].it;
-        for (int attempt = 0;; ++attempt) {
-            searchOnce(attempt == 0);
+        if (!replaying) {
             readState();
-            if (searchValid()) break;
-            if (attempt >= 5) throw SphError{SPH_E_OUT_OF_GRID, "could not place particles into a grid"};
+            for (size_t s = 0; s < scenes.size(); ++s) scenes[s].its = hState[s].it;
         }
+        uint32_t rowsF = 0, rowsB = 0;
+        recordedSearch(slot, true, rowsF, rowsB);
         if (capF > tape.capF || capB > tape.capB) tapeGrow(slot + 1);
         d = view();
-        recordSearch(B, slot);
+        recordSearch(B, slot, rowsF, rowsB);
         timed(K_DENSITY, [&] { return launchDensityAlpha(d, 1, stream); });
         tapeCopy(B.rho.p + o, rho.p, np * sizeof(float));
         tapeCopy(B.alpha.p + o, alpha.p, np * sizeof(float));
@@ -2178,6 +2276,10 @@ class Engine {
         // aux of the context is the last pass's (sph_get_field reads it)
         tapeCopy(aux.p, B.aux.p + (o * (nit + 1) + static_cast<size_t>(nit - 1) * np), np * sizeof(float));
         flushCopies();   // x, lists, rho and alpha stay until the next step's keys pass
+        if (replaying) {   // no statistics: the recording filed them
+            timed(K_SCALE, [&] { return launchScaleDivergence(view(), stream); });
+            return;
+        }
         readState();
         for (size_t s = 0; s < scenes.size(); ++s) scenes[s].itsV = hState[s].it;
         timed(K_SCALE, [&] { return launchScaleDivergence(view(), stream); });
@@ -2240,9 +2342,16 @@ class Engine {
             tapeAlloc(every, nit, halves);
         launchesInStep = 0;
         syncsInStep = 0;
+        replaying = false;
+        replayChecks = 0;
+        rec.flags.assign(static_cast<size_t>(nsteps) * scenes.size() * kFlagSlots, 0);
+        rec.rowsF.assign(static_cast<size_t>(nsteps), 0);
+        rec.rowsB.assign(static_cast<size_t>(nsteps), 0);
+        rec.retried.assign(static_cast<size_t>(nsteps), 1);
         HIPCHK(hipEventRecord(evStart, stream));
         for (int st = 0; st < nsteps; ++st) {
             if (st % every == 0) saveCheckpoint(rec.ckpt[static_cast<size_t>(st / every)]);
+            recBase = st - st % every;
             stepRecord(st % every);
             if (stats)
                 for (size_t s = 0; s < scenes.size(); ++s) stats[st * scenes.size() + s] = lastStats[s];
@@ -2445,6 +2554,8 @@ class Engine {
             adjS = stream;
             adjTape = &tape;
             rec.tapeSeg = -1;
+            replaying = false;
+            replayChecks = 0;
             try {   // back to where the recorded rollout ended, as after a sweep
                 restoreCheckpoint(rec.last);
             } catch (...) {
@@ -2469,7 +2580,9 @@ class Engine {
                 const int first = seg * rec.every, count = std::min(rec.every, rec.nsteps - first);
                 if (rec.tapeSeg != seg) {
                     restoreCheckpoint(rec.ckpt[static_cast<size_t>(seg)]);
+                    beginReplay(first);
                     for (int j = 0; j < count; ++j) stepRecord(j);
+                    replayCheck();
                     rec.tapeSeg = seg;
                 }
                 for (int j = count - 1; j >= 0; --j) adjointStep(j);
@@ -2484,8 +2597,10 @@ class Engine {
             const int last = rec.segments - 1;
             if (rec.tapeSeg != last) {   // a second sweep over the same recording
                 restoreCheckpoint(rec.ckpt[static_cast<size_t>(last)]);
+                beginReplay(last * rec.every);
                 for (int j = 0; j < rec.nsteps - last * rec.every; ++j) stepRecord(j);
                 flushCopies();
+                replayCheck();
                 rec.tapeSeg = last;
             }
             adjS = adjStream;
@@ -2504,6 +2619,7 @@ class Engine {
                 if (countPrev) {
                     HIPCHK(hipStreamWaitEvent(stream, evAdjDone, 0));
                     restoreCheckpoint(rec.ckpt[static_cast<size_t>(seg - 1)]);
+                    beginReplay((seg - 1) * rec.every);   // checks the segment replayed before
                 }
                 while (j >= 0 || k < countPrev) {
                     if (j >= 0) adjointStep(j--);
@@ -2518,6 +2634,7 @@ class Engine {
                     rec.tapeSeg = seg;
                 }
             }
+            replayCheck();
         }
         if (method == SPH_DFSPH) {
             // rho and alpha of the first step came from the initial positions
