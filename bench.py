@@ -250,7 +250,11 @@ def adjoint_cost(steps, every, method, warmup=5):
     largest of 5e-4 / 2.5e-4 / 2e-4 that stays stable at r = 0.005. DFSPH with
     8 density and 8 divergence iterations at a fixed dt = 5e-4. Times the plain
     forward, the recording forward (checkpoint every `every` steps) and the
-    reverse sweep (segment recompute + adjoint passes) from the same state."""
+    reverse sweep (segment recompute + adjoint passes) from the same state.
+    The gradient is computed twice: the first call allocates the tapes (two
+    segments' worth) and is reported as `first_call`; the second, like the
+    forward after its warmup steps, is the measurement, and must give the
+    same gradient bit for bit."""
     from sph2o_amd import scenes as SC
     from sph2o_amd.simulation import Context
 
@@ -272,26 +276,32 @@ def adjoint_cost(steps, every, method, warmup=5):
     ctx.step(steps)
     ctx.synchronize()
     t_fwd = time.perf_counter() - t0
-    ctx.set_fluid_state(sid, x0, v0)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    ctx.forward_record(steps, every)
-    ctx.synchronize()
-    t_rec = time.perf_counter() - t0
-    xT = ctx.fluid_state(sid)["x"].astype(np.float64)
     y = SC.adjoint_target(x0)
-    seed = (2.0 * (xT - y) / n).astype(np.float32)
     from sph2o_amd._lib import SPH_E_NONFINITE, SPHError
 
-    t0 = time.perf_counter()
-    try:
-        g = ctx.adjoint({sid: (seed, None)})[sid]
-    except SPHError as e:   # the sweep completed; its gradient is not finite
-        if e.code != SPH_E_NONFINITE:
-            raise
-        g = None
-    t_adj = time.perf_counter() - t0
+    def gradient():
+        ctx.set_fluid_state(sid, x0, v0)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.forward_record(steps, every)
+        ctx.synchronize()
+        t_rec = time.perf_counter() - t0
+        xT = ctx.fluid_state(sid)["x"].astype(np.float64)
+        seed = (2.0 * (xT - y) / n).astype(np.float32)
+        t0 = time.perf_counter()
+        try:
+            g = ctx.adjoint({sid: (seed, None)})[sid]
+        except SPHError as e:   # the sweep completed; its gradient is not finite
+            if e.code != SPH_E_NONFINITE:
+                raise
+            g = None
+        return t_rec, time.perf_counter() - t0, xT, g
+
+    rec1, adj1, _, g1 = gradient()
+    t_rec, t_adj, xT, g = gradient()
     ctx.close()
+    same = (g is None) == (g1 is None) and (g is None or all(
+        np.array_equal(g[k], g1[k]) for k in ("x0", "v0", "viscosity", "surface_tension")))
     name = "WCSPH" if method == 0 else "DFSPH 8+8 iterations"
     finite = lambda q: float(q) if q is not None and np.isfinite(q) else None   # strict JSON has no inf / NaN
     gnorm = float(np.linalg.norm(g["v0"].astype(np.float64))) if g is not None else None
@@ -301,6 +311,8 @@ def adjoint_cost(steps, every, method, warmup=5):
             "adjoint_over_forward": t_adj / t_fwd,
             # the whole gradient: the recording forward plus the reverse sweep
             "gradient_over_forward": (t_rec + t_adj) / t_fwd,
+            "first_call": {"record_s": rec1, "adjoint_s": adj1, "gradient_over_forward": (rec1 + adj1) / t_fwd},
+            "repeat_bit_identical": bool(same),
             "loss": float(np.mean(np.sum((xT - y) ** 2, axis=1))),
             "gradient_finite": g is not None and bool(np.isfinite(gnorm)),
             "dL_dviscosity": finite(g["viscosity"] if g else None),
