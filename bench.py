@@ -541,9 +541,10 @@ def b64_timed(steps):
 
 
 def a100k_sweep(steps):
-    """A100k DFSPH (8 + 8 iterations, checkpoint every 20): the recording
-    forward of `steps` steps, a 0.5 s pause (a marker in a kernel trace), then
-    the reverse sweep. Prints the wall times."""
+    """A100k DFSPH (8 + 8 iterations, checkpoint every 20): an untimed
+    40-step gradient (the tapes' allocation), the recording forward of
+    `steps` steps, a 0.5 s pause (a marker in a kernel trace), then the
+    reverse sweep. Prints the wall times."""
     from sph2o_amd import scenes as SC
     from sph2o_amd.simulation import Context
     from sph2o_amd._lib import SPH_E_NONFINITE, SPHError
@@ -556,6 +557,9 @@ def a100k_sweep(steps):
     sid = ctx.load_scene_json(path)
     ctx.init()
     x0 = ctx.fluid_state(sid)["x"]
+    ctx.forward_record(40, 20)
+    ctx.adjoint({sid: (np.zeros_like(x0), None)})
+    ctx.set_fluid_state(sid, x0, np.zeros_like(x0))
     ctx.synchronize()
     t0 = time.perf_counter()
     ctx.forward_record(steps, 20)
