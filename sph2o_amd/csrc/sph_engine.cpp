@@ -1658,6 +1658,9 @@ class Engine {
         std::vector<int32_t> flags;
         std::vector<uint32_t> rowsF, rowsB;
         std::vector<uint8_t> retried;
+        // the segment the second tape (tapeRd) holds from the recording (the
+        // one before the last), or -1
+        int tapeSegRd = -1;
     };
     Recording rec;
     Tape tape;
@@ -2050,9 +2053,10 @@ class Engine {
     // The list capacity grew while a segment was recorded: widen the tape's
     // list blocks, keeping the steps already recorded (entry s of particle i
     // sits at s * NP + i within its step's block for any capacity).
-    void tapeGrow(int recorded) {
+    void tapeGrow(int recorded) { tapeWiden(tape, recorded); }
+    void tapeWiden(Tape& tp, int recorded) {
         flushCopies();   // no batched copy may target the blocks replaced here
-        const size_t k = static_cast<size_t>(tape.steps), np = tape.np;
+        const size_t k = static_cast<size_t>(tp.steps), np = tp.np;
         auto widen = [&](DevBuf<uint32_t>& buf, uint32_t oldCap, uint32_t newCap) {
             if (newCap <= oldCap) return;
             DevBuf<uint32_t> w;
@@ -2062,13 +2066,13 @@ class Engine {
             buf = std::move(w);
         };
         HIPCHK(hipStreamSynchronize(stream));
-        for (int hf = 0; hf < tape.halves; ++hf) {
-            widen(tape.h[hf].nbrF, tape.capF, capF);
-            widen(tape.h[hf].nbrB, tape.capB, capB);
+        for (int hf = 0; hf < tp.halves; ++hf) {
+            widen(tp.h[hf].nbrF, tp.capF, capF);
+            widen(tp.h[hf].nbrB, tp.capB, capB);
         }
         HIPCHK(hipStreamSynchronize(stream));
-        tape.capF = std::max(tape.capF, capF);
-        tape.capB = std::max(tape.capB, capB);
+        tp.capF = std::max(tp.capF, capF);
+        tp.capB = std::max(tp.capB, capB);
     }
     // The list rows some particle of the last search uses: its longest list,
     // from the flags the host read after it, rounded up to whole kPadRows
@@ -2340,6 +2344,11 @@ class Engine {
         if (tape.steps != every || tape.np != NP || tape.nit != nit || tape.halves != halves || tape.capF < capF ||
             tape.capB < capB)
             tapeAlloc(every, nit, halves);
+        // With room for the reverse sweep's second tape, the segments go to
+        // the two tapes in turn, so that the recording leaves the last two
+        // segments taped and the sweep recomputes one segment fewer.
+        rec.tapeSegRd = -1;
+        const bool two = rec.segments > 1 && prepareOverlap();
         launchesInStep = 0;
         syncsInStep = 0;
         replaying = false;
@@ -2350,13 +2359,25 @@ class Engine {
         rec.retried.assign(static_cast<size_t>(nsteps), 1);
         HIPCHK(hipEventRecord(evStart, stream));
         for (int st = 0; st < nsteps; ++st) {
-            if (st % every == 0) saveCheckpoint(rec.ckpt[static_cast<size_t>(st / every)]);
+            if (st % every == 0) {
+                saveCheckpoint(rec.ckpt[static_cast<size_t>(st / every)]);
+                if (two && st > 0) {
+                    flushCopies();
+                    std::swap(tape, tapeRd);
+                }
+            }
             recBase = st - st % every;
             stepRecord(st % every);
             if (stats)
                 for (size_t s = 0; s < scenes.size(); ++s) stats[st * scenes.size() + s] = lastStats[s];
         }
         saveCheckpoint(rec.last);
+        if (two) {
+            // the list capacity may have grown after the segment before the
+            // last was taped: widen its tape to the last one's dimensions
+            if (tapeRd.capF < tape.capF || tapeRd.capB < tape.capB) tapeWiden(tapeRd, every);
+            rec.tapeSegRd = rec.segments - 2;
+        }
         HIPCHK(hipEventRecord(evStop, stream));
         HIPCHK(hipEventSynchronize(evStop));
         float ms = 0;
@@ -2501,6 +2522,7 @@ class Engine {
             return true;
         HIPCHK(hipStreamSynchronize(stream));
         sp = Tape{};
+        rec.tapeSegRd = -1;   // whatever the second tape held is gone
         size_t freeB = 0, totalB = 0;
         HIPCHK(hipMemGetInfo(&freeB, &totalB));
         const size_t need = tapeBytes(tape);
@@ -2554,6 +2576,7 @@ class Engine {
             adjS = stream;
             adjTape = &tape;
             rec.tapeSeg = -1;
+            rec.tapeSegRd = -1;
             replaying = false;
             replayChecks = 0;
             try {   // back to where the recorded rollout ended, as after a sweep
@@ -2595,6 +2618,9 @@ class Engine {
             // A few adjoint steps are queued before the recompute's restore,
             // whose host waits would otherwise leave adjStream idle.
             const int last = rec.segments - 1;
+            // the recording may have left the segment before the last on tapeRd
+            const bool keptRd = rec.tapeSeg == last && rec.tapeSegRd == last - 1;
+            rec.tapeSegRd = -1;
             if (rec.tapeSeg != last) {   // a second sweep over the same recording
                 restoreCheckpoint(rec.ckpt[static_cast<size_t>(last)]);
                 beginReplay(last * rec.every);
@@ -2613,7 +2639,10 @@ class Engine {
             constexpr int kLead = 2;
             for (int seg = rec.segments - 1; seg >= 0; --seg) {
                 const int count = std::min(rec.every, rec.nsteps - seg * rec.every);
-                const int countPrev = seg > 0 ? rec.every : 0;   // segments before the last are whole
+                // segments before the last are whole; the one before the last
+                // needs no recompute if the recording kept it (it is in `tape`
+                // after the swap above)
+                const int countPrev = seg > 0 && !(keptRd && seg == last) ? rec.every : 0;
                 int j = count - 1, k = 0;
                 for (int l = 0; l < kLead && j >= 0; ++l) adjointStep(j--);
                 if (countPrev) {
@@ -2626,8 +2655,8 @@ class Engine {
                     if (k < countPrev) stepRecord(k++);
                 }
                 HIPCHK(hipEventRecord(evAdjDone, adjStream));
-                if (countPrev) {
-                    flushCopies();
+                if (seg > 0) {
+                    if (countPrev) flushCopies();
                     HIPCHK(hipEventRecord(evRecDone, stream));
                     HIPCHK(hipStreamWaitEvent(adjStream, evRecDone, 0));
                     std::swap(tape, tapeRd);   // tapeRd: segment seg - 1; tape: segment seg
