@@ -309,6 +309,9 @@ class Engine {
         if (hReplayFlags) (void)hipHostFree(hReplayFlags);
         hReplayFlags = nullptr;
         replayFlagCap = 0;
+        if (hQueueFlags) (void)hipHostFree(hQueueFlags);
+        hQueueFlags = nullptr;
+        queueFlagCap = 0;
     }
 
     // ------------------------------------------------------------ scenes
@@ -1155,8 +1158,9 @@ class Engine {
     // ------------------------------------------------------------ neighbour search
     // keys -> sort -> reorder (+ cell ranges) -> within-support lists.
     // dfsphAdvance fuses the DFSPH position update into the keys kernel.
-    void searchOnce(bool dfsphAdvance) {
-        HIPCHK(hipMemsetAsync(dFlags, 0, scenes.size() * kFlagSlots * sizeof(int32_t), stream));
+    // clearFlags false: the flags were zeroed after the last search (kFlagLog)
+    void searchOnce(bool dfsphAdvance, bool clearFlags = true) {
+        if (clearFlags) HIPCHK(hipMemsetAsync(dFlags, 0, scenes.size() * kFlagSlots * sizeof(int32_t), stream));
         DevView d = view();
         if (!radixOnce) {
             // counting sort: keys = order indices counted per index, then an
@@ -1360,12 +1364,28 @@ class Engine {
         launchesInStep = 0;
         syncsInStep = 0;
         HIPCHK(hipEventRecord(evStart, stream));
-        for (int st = 0; st < nsteps; ++st) {
+        for (int st = 0; st < nsteps;) {
+            if (method == SPH_WCSPH && canQueueWCSPH()) {
+                const int k = std::min(kQueuedSteps, nsteps - st);
+                if (queueWCSPH(k, stats ? stats + static_cast<size_t>(st) * scenes.size() : nullptr)) {
+                    st += k;
+                    continue;
+                }
+                // a search of the chunk failed its check: the chunk was undone;
+                // take its steps with a host check each
+                for (int j = 0; j < k; ++j, ++st) {
+                    stepWCSPH();
+                    if (stats)
+                        for (size_t s = 0; s < scenes.size(); ++s) stats[st * scenes.size() + s] = lastStats[s];
+                }
+                continue;
+            }
             if (method == SPH_DFSPH) stepDFSPH();
             else if (method == SPH_PCISPH) stepPCISPH();
             else stepWCSPH();
             if (stats)
                 for (size_t s = 0; s < scenes.size(); ++s) stats[st * scenes.size() + s] = lastStats[s];
+            ++st;
         }
         HIPCHK(hipEventRecord(evStop, stream));
         HIPCHK(hipEventSynchronize(evStop));
@@ -1375,6 +1395,127 @@ class Engine {
         collectProfile();
     }
     std::vector<sph_step_stats> lastStats;
+
+    // WCSPH steps queued without a host wait. A WCSPH step's only host
+    // decision is the check of its search (lists within capacity, particles
+    // within the grid's reach, cells the counting sort can order); where no
+    // emitter (host-timed) or PCISPH boundary object (a hash-table history
+    // that a redo would not restore) takes part and the search lanes are
+    // settled, a chunk of steps is queued at once: a one-block kernel moves
+    // each search's flags into a device log (and clears them for the next
+    // search), and after the chunk one copy and one wait check them all.
+    // If any failed, the chunk is undone (positions, velocities, ids and
+    // time from a snapshot taken before it, the same grid) and retaken step
+    // by step with the host check, which grows what overflowed; every step
+    // that passed its check is the step the checked path takes, bit for bit.
+    static constexpr int kQueuedSteps = 32;
+    int32_t* hQueueFlags = nullptr;
+    size_t queueFlagCap = 0;
+    DevBuf<int32_t> queueLog;   // the chunk's search flags, step by step
+    bool canQueueWCSPH() const {
+        if (!NP || replaying) return false;
+        if (!searchLanesFixed && !tuner.done) return false;
+        for (const auto& sc : scenes)
+            if (!sc.emitters.empty() || sc.pciB) return false;
+        return true;
+    }
+    // The checks of searchValid without its side effects.
+    bool flagsPass(const int32_t* flags) const {
+        for (size_t si = 0; si < scenes.size(); ++si) {
+            const int32_t* f = flags + si * kFlagSlots;
+            if (f[kFlagNonFinite] || f[kFlagBigCell] || f[kFlagFar]) return false;
+            if (static_cast<uint32_t>(std::max(0, f[kFlagOutOfGrid])) > scenes[si].ovBuckets / 2) return false;
+            if (static_cast<uint32_t>(std::max(0, f[kFlagMaxFluidNbr])) > capF) return false;
+            if (static_cast<uint32_t>(std::max(0, f[kFlagMaxBoundaryNbr])) > capB) return false;
+        }
+        return !radixOnce;
+    }
+    // With slot0 >= 0 the chunk is steps slot0 .. slot0 + steps - 1 of the
+    // segment being recorded (stepWCSPHRecord's tape writes; the lists are
+    // taped at capacity, since the longest list is not known on the host).
+    bool queueWCSPH(int steps, sph_step_stats* stats, int slot0 = -1) {
+        const size_t nf = scenes.size() * kFlagSlots;
+        if (queueFlagCap < static_cast<size_t>(steps) * nf) {
+            if (hQueueFlags) HIPCHK(hipHostFree(hQueueFlags));
+            hQueueFlags = nullptr;
+            queueFlagCap = 0;
+            HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&hQueueFlags), static_cast<size_t>(kQueuedSteps) * nf * sizeof(int32_t)));
+            queueFlagCap = static_cast<size_t>(kQueuedSteps) * nf;
+        }
+        if (queueLog.n < static_cast<size_t>(kQueuedSteps) * nf) queueLog.alloc(static_cast<size_t>(kQueuedSteps) * nf);
+        saveCheckpoint(queueSnap);
+        const uint32_t stepNo0 = stepNo, markNo0 = markNo;
+        HIPCHK(hipMemsetAsync(dFlags, 0, nf * sizeof(int32_t), stream));
+        const bool record = slot0 >= 0;
+        for (int k = 0; k < steps; ++k) {
+            if (!record) markNo = stepNo;
+            searchOnce(false, false);
+            HIPCHK(launchFlagLog(dFlags, queueLog.p + static_cast<size_t>(k) * nf, static_cast<uint32_t>(nf), stream));
+            const DevView d = view();
+            const size_t o = static_cast<size_t>(slot0 + k) * NP;
+            HalfTape& t = tape.h[0];
+            if (record) {
+                recordSearch(t, slot0 + k, capF, capB);
+                tapeCopy(t.v.p + o, v.p, NP * sizeof(float4));
+            }
+            timed(K_DENSITY, [&] { return launchDensityPressure(d, stream); });
+            timed(K_FORCES, [&] { return launchForces(d, 1, stream); });
+            if (anyST() || anyAdh()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
+            if (record) {
+                tapeCopy(t.rho.p + o, rho.p, NP * sizeof(float));
+                tapeCopy(t.p.p + o, p.p, NP * sizeof(float));
+                if (anyST()) tapeCopy(t.n.p + o, n.p, NP * sizeof(float4));
+                flushCopies();   // before the integration moves x and v
+            }
+            timed(K_INTEGRATE, [&] { return launchIntegrate(d, stream); });
+            if (anyCube()) timed(K_CUBE, [&] { return launchCubeBoundary(d, stream); });
+            if (!record) ++stepNo;
+        }
+        HIPCHK(hipMemcpyAsync(hQueueFlags, queueLog.p, static_cast<size_t>(steps) * nf * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, stream));
+        readState();   // waits for the chunk; the state block as the checked path leaves it
+        bool ok = !radixOnce;
+        for (int k = 0; k < steps && ok; ++k) ok = flagsPass(hQueueFlags + static_cast<size_t>(k) * nf);
+        if (!ok) {
+            stepNo = stepNo0;
+            markNo = markNo0;
+            restoreCheckpoint(queueSnap);
+            return false;
+        }
+        // the statistics of each step: t advances by dt per step (the float
+        // sums kIntegrate forms); the rest is the state block's
+        std::vector<float> t0(scenes.size());
+        for (size_t s = 0; s < scenes.size(); ++s) t0[s] = queueSnap.t[s];
+        for (int k = 0; k < steps; ++k) {
+            beginStats();
+            for (size_t s = 0; s < scenes.size(); ++s) {
+                t0[s] = t0[s] + hState[s].dt;
+                sph_step_stats& o = lastStats[s];
+                o.dt = hState[s].dt;
+                o.t = t0[s];
+                o.max_velocity = hState[s].vmax;
+                o.iterations = hState[s].its;
+                o.avg_density_error = hState[s].errD;
+                o.capped_density = hState[s].capped & 1;
+            }
+            if (stats)
+                for (size_t s = 0; s < scenes.size(); ++s) stats[static_cast<size_t>(k) * scenes.size() + s] = lastStats[s];
+        }
+        for (size_t s = 0; s < scenes.size(); ++s) {
+            scenes[s].dt = hState[s].dt;
+            scenes[s].t = hState[s].t;
+        }
+        if (record)   // what the replay of these steps checks (recordedSearch)
+            for (int k = 0; k < steps; ++k) {
+                const size_t g = static_cast<size_t>(recBase + slot0 + k);
+                std::copy(hQueueFlags + static_cast<size_t>(k) * nf, hQueueFlags + static_cast<size_t>(k + 1) * nf,
+                          rec.flags.begin() + static_cast<std::ptrdiff_t>(g * nf));
+                rec.rowsF[g] = capF;
+                rec.rowsB[g] = capB;
+                rec.retried[g] = 0;
+            }
+        return true;
+    }
 
     bool anyVisc() const {
         for (auto& s : scenes) if (s.viscOn) return true;
@@ -1663,6 +1804,7 @@ class Engine {
         int tapeSegRd = -1;
     };
     Recording rec;
+    Checkpoint queueSnap;   // the state before a chunk of queued WCSPH steps
     Tape tape;
     // Recorded steps: recBase is the global index of slot 0 of the segment
     // being recorded or replayed; while `replaying`, searches that took one
@@ -2358,7 +2500,7 @@ class Engine {
         rec.rowsB.assign(static_cast<size_t>(nsteps), 0);
         rec.retried.assign(static_cast<size_t>(nsteps), 1);
         HIPCHK(hipEventRecord(evStart, stream));
-        for (int st = 0; st < nsteps; ++st) {
+        for (int st = 0; st < nsteps;) {
             if (st % every == 0) {
                 saveCheckpoint(rec.ckpt[static_cast<size_t>(st / every)]);
                 if (two && st > 0) {
@@ -2367,9 +2509,20 @@ class Engine {
                 }
             }
             recBase = st - st % every;
-            stepRecord(st % every);
-            if (stats)
-                for (size_t s = 0; s < scenes.size(); ++s) stats[st * scenes.size() + s] = lastStats[s];
+            // WCSPH: chunks of steps queued without host waits, within the segment
+            int n = 1;
+            if (method == SPH_WCSPH && canQueueWCSPH() && capF <= tape.capF && capB <= tape.capB) {
+                n = std::min(kQueuedSteps, std::min(every - st % every, nsteps - st));
+                if (queueWCSPH(n, stats ? stats + static_cast<size_t>(st) * scenes.size() : nullptr, st % every)) {
+                    st += n;
+                    continue;
+                }
+            }
+            for (int j = 0; j < n; ++j, ++st) {
+                stepRecord(st % every);
+                if (stats)
+                    for (size_t s = 0; s < scenes.size(); ++s) stats[st * scenes.size() + s] = lastStats[s];
+            }
         }
         saveCheckpoint(rec.last);
         if (two) {

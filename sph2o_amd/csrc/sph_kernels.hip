@@ -2041,6 +2041,17 @@ hipError_t launchDivergence(const DevView& d, int warm, int slot, hipStream_t st
     LAUNCH_JACOBI(kDivergence, d.numBlocks * kLpp, d, warm, slot);
 }
 hipError_t launchScaleDivergence(const DevView& d, hipStream_t st) { LAUNCH(kScaleDivergence, d.numBlocks, d); }
+__global__ void kFlagLog(int32_t* flags, int32_t* log, uint32_t n) {
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        log[k] = flags[k];
+        flags[k] = 0;
+    }
+}
+hipError_t launchFlagLog(int32_t* flags, int32_t* log, uint32_t n, hipStream_t st) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(kFlagLog, dim3(1), dim3(256), 0, st, flags, log, n);
+    return hipGetLastError();
+}
 hipError_t launchIntegrate(const DevView& d, hipStream_t st) { LAUNCH(kIntegrate, d.numBlocks, d); }
 hipError_t launchPciInit(const DevView& d, hipStream_t st) { LAUNCH(kPciInit, d.numBlocks, d); }
 hipError_t launchPciDensity(const DevView& d, int slot, hipStream_t st) { LAUNCH(kPciDensity, d.numBlocks, d, slot); }

@@ -143,6 +143,8 @@ hipError_t launchRhoStar(const DevView& d, int warm, int slot, hipStream_t st);
 hipError_t launchVelocity(const DevView& d, int slot, int fin, hipStream_t st);
 hipError_t launchDivergence(const DevView& d, int warm, int slot, hipStream_t st);
 hipError_t launchScaleDivergence(const DevView& d, hipStream_t st);
+// Copy n search flags to `log` and zero them for the next search.
+hipError_t launchFlagLog(int32_t* flags, int32_t* log, uint32_t n, hipStream_t st);
 hipError_t launchIntegrate(const DevView& d, hipStream_t st);
 
 hipError_t launchPciInit(const DevView& d, hipStream_t st);

@@ -328,7 +328,9 @@ def test_replayed_segments_follow_a_growing_grid(tmp_path, method):
     and the recording's search grows the grid and runs again. A recomputed
     segment repeats such a step's search with a host check and queues the
     others without one; the gradients and the final state equal those of
-    one segment (nothing recomputed) bit for bit."""
+    one segment (nothing recomputed) bit for bit. Plain stepping from the
+    same state ends there too (WCSPH queues its 16 steps as one chunk, whose
+    failed search check undoes it and retakes the steps with host checks)."""
     sc = SC.scene_dict(method, R, [((SP, SP, SP), (12 * SP, 12 * SP, 12 * SP))], [((-0.1, -0.1, -0.1), (-0.08, -0.08, -0.08))],
                        dt=5e-4, max_dt=5e-4, min_dt=5e-4, min_it=NIT, max_it=NIT, min_it_v=NIT, max_it_v=NIT,
                        viscosity_method=2, viscosity=0.01, surface_tension_method=3, surface_tension=0.1)
@@ -348,6 +350,11 @@ def test_replayed_segments_follow_a_growing_grid(tmp_path, method):
                 assert np.array_equal(res[k], ref[1][k]), (every, k)
             assert res["viscosity"] == ref[1]["viscosity"] and res["surface_tension"] == ref[1]["surface_tension"]
         assert np.isfinite(ref[1]["v0"]).all() and np.abs(ref[1]["v0"]).max() > 0
+        case.ctx.set_fluid_state(case.sid, case.x0, case.v0)
+        stats = case.ctx.step(16)
+        assert np.array_equal(case.ctx.fluid_state(case.sid)["x"].astype(np.float64), ref[0])
+        ts = np.array([row[case.sid].t for row in stats])
+        assert np.allclose(np.diff(ts), 5e-4, rtol=1e-2) and all(row[case.sid].dt == np.float32(5e-4) for row in stats)
     finally:
         case.ctx.close()
 
