@@ -297,6 +297,7 @@ class Engine {
         if (evSync) (void)hipEventDestroy(evSync);
         if (tuner.e0) (void)hipEventDestroy(tuner.e0);
         if (tuner.e1) (void)hipEventDestroy(tuner.e1);
+        if (pairExec) (void)hipGraphExecDestroy(pairExec);
         if (evAdjDone) (void)hipEventDestroy(evAdjDone);
         if (evRecDone) (void)hipEventDestroy(evRecDone);
         if (adjStream) (void)hipStreamDestroy(adjStream);
@@ -1430,6 +1431,101 @@ class Engine {
         }
         return !radixOnce;
     }
+    // Two queued WCSPH steps as one HIP graph, replayed for the step pairs of
+    // an even chunk: after two searches every swapped slot buffer is back in
+    // place, so the same launches serve every pair. The pair's searches file
+    // their flags into one max-accumulating array (kFlagMax), all the chunk's
+    // check needs. The graph is captured again whenever anything its launches
+    // depend on (the device view, the search's buffers and path, the force
+    // options) differs from what it was captured with; a capture that fails
+    // leaves the host state as it was and switches graphs off for the context.
+    struct PairKey {
+        DevView d;
+        const void* ptr[16];
+        uint64_t val[12];
+        bool operator==(const PairKey& o) const { return std::memcmp(this, &o, sizeof(PairKey)) == 0; }
+    };
+    hipGraphExec_t pairExec = nullptr;
+    PairKey pairKey{};
+    int64_t pairLaunches = 0;
+    bool graphsOff = false;
+    DevBuf<int32_t> flagAcc;
+    PairKey currentPairKey() {
+        PairKey k;
+        std::memset(static_cast<void*>(&k), 0, sizeof(k));
+        k.d = view();
+        k.d.stepNo = 0;
+        k.d.markNo = 0;
+        const void* ps[16] = {xAlt.p, vAlt.p, kwsAlt.p, kvwsAlt.p, idAlt.p, keys.p, vals.p, keysAlt.p,
+                              valsAlt.p, scanTemp.p, sortTemp.p, occ.p, ncAlt.p, flagAcc.p, dFlags, nullptr};
+        std::copy(ps, ps + 16, k.ptr);
+        const uint64_t vs[12] = {scanTempBytes, sortTempBytes, ordTotal, static_cast<uint64_t>(keyBits),
+                                 radixOnce, rangesDirty, rangesFull, static_cast<uint64_t>(anyST() || anyAdh()),
+                                 static_cast<uint64_t>(anyCube()),
+                                 static_cast<uint64_t>(searchLanesFixed ? searchLanesFixed : searchLanes), tiles,
+                                 totalCells};
+        std::copy(vs, vs + 12, k.val);
+        return k;
+    }
+    // The launches of one queued step; flags go to the log slot or, with
+    // acc, into the accumulating array.
+    void queuedStep(int32_t* logSlot, bool acc) {
+        searchOnce(false, false);
+        const uint32_t nf = static_cast<uint32_t>(scenes.size() * kFlagSlots);
+        if (acc) HIPCHK(launchFlagMax(dFlags, flagAcc.p, nf, 0, stream));
+        else HIPCHK(launchFlagLog(dFlags, logSlot, nf, stream));
+        const DevView d = view();
+        timed(K_DENSITY, [&] { return launchDensityPressure(d, stream); });
+        timed(K_FORCES, [&] { return launchForces(d, 1, stream); });
+        if (anyST() || anyAdh()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
+        timed(K_INTEGRATE, [&] { return launchIntegrate(d, stream); });
+        if (anyCube()) timed(K_CUBE, [&] { return launchCubeBoundary(d, stream); });
+    }
+    void capturePair() {
+        if (pairExec) (void)hipGraphExecDestroy(pairExec);
+        pairExec = nullptr;
+        const PairKey key = currentPairKey();
+        // host state the two searches change, restored if the capture fails
+        float4* sx[4] = {x.p, xAlt.p, v.p, vAlt.p};
+        float* sf[4] = {kws.p, kwsAlt.p, kvws.p, kvwsAlt.p};
+        uint32_t* si[2] = {id.p, idAlt.p};
+        float4* sn[2] = {nc.p, ncAlt.p};
+        const bool sr[3] = {radixOnce, rangesDirty, rangesFull};
+        const int64_t l0 = launchesInStep;
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+        bool ok = e == hipSuccess;
+        if (ok) {
+            try {
+                queuedStep(nullptr, true);
+                queuedStep(nullptr, true);
+            } catch (const SphError&) {
+                ok = false;
+            }
+            e = hipStreamEndCapture(stream, &g);
+            ok = ok && e == hipSuccess && g;
+        }
+        if (ok) {
+            e = hipGraphInstantiate(&pairExec, g, nullptr, nullptr, 0);
+            ok = e == hipSuccess;
+        }
+        if (g) (void)hipGraphDestroy(g);
+        if (!ok) {
+            pairExec = nullptr;
+            graphsOff = true;
+            (void)hipGetLastError();
+            x.p = sx[0]; xAlt.p = sx[1]; v.p = sx[2]; vAlt.p = sx[3];
+            kws.p = sf[0]; kwsAlt.p = sf[1]; kvws.p = sf[2]; kvwsAlt.p = sf[3];
+            id.p = si[0]; idAlt.p = si[1];
+            nc.p = sn[0]; ncAlt.p = sn[1];
+            radixOnce = sr[0]; rangesDirty = sr[1]; rangesFull = sr[2];
+            launchesInStep = l0;
+            return;
+        }
+        pairKey = key;
+        pairLaunches = launchesInStep - l0;
+        launchesInStep = l0;
+    }
     // With slot0 >= 0 the chunk is steps slot0 .. slot0 + steps - 1 of the
     // segment being recorded (stepWCSPHRecord's tape writes; the lists are
     // taped at capacity, since the longest list is not known on the host).
@@ -1447,7 +1543,21 @@ class Engine {
         const uint32_t stepNo0 = stepNo, markNo0 = markNo;
         HIPCHK(hipMemsetAsync(dFlags, 0, nf * sizeof(int32_t), stream));
         const bool record = slot0 >= 0;
-        for (int k = 0; k < steps; ++k) {
+        bool graphed = false;
+        if (!record && steps % 2 == 0 && !graphsOff && !profiling) {
+            if (flagAcc.n < nf) flagAcc.alloc(nf);
+            HIPCHK(launchFlagMax(dFlags, flagAcc.p, static_cast<uint32_t>(nf), 1, stream));
+            if (!pairExec || !(currentPairKey() == pairKey)) capturePair();
+            if (pairExec) {
+                for (int q = 0; q < steps / 2; ++q) HIPCHK(hipGraphLaunch(pairExec, stream));
+                launchesInStep += pairLaunches * (steps / 2);
+                stepNo += static_cast<uint32_t>(steps);
+                markNo = stepNo - 1;
+                HIPCHK(hipMemcpyAsync(hQueueFlags, flagAcc.p, nf * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+                graphed = true;
+            }
+        }
+        for (int k = 0; k < steps && !graphed; ++k) {
             if (!record) markNo = stepNo;
             searchOnce(false, false);
             HIPCHK(launchFlagLog(dFlags, queueLog.p + static_cast<size_t>(k) * nf, static_cast<uint32_t>(nf), stream));
@@ -1471,11 +1581,12 @@ class Engine {
             if (anyCube()) timed(K_CUBE, [&] { return launchCubeBoundary(d, stream); });
             if (!record) ++stepNo;
         }
-        HIPCHK(hipMemcpyAsync(hQueueFlags, queueLog.p, static_cast<size_t>(steps) * nf * sizeof(int32_t),
-                              hipMemcpyDeviceToHost, stream));
+        if (!graphed)
+            HIPCHK(hipMemcpyAsync(hQueueFlags, queueLog.p, static_cast<size_t>(steps) * nf * sizeof(int32_t),
+                                  hipMemcpyDeviceToHost, stream));
         readState();   // waits for the chunk; the state block as the checked path leaves it
         bool ok = !radixOnce;
-        for (int k = 0; k < steps && ok; ++k) ok = flagsPass(hQueueFlags + static_cast<size_t>(k) * nf);
+        for (int k = 0; k < (graphed ? 1 : steps) && ok; ++k) ok = flagsPass(hQueueFlags + static_cast<size_t>(k) * nf);
         if (!ok) {
             stepNo = stepNo0;
             markNo = markNo0;

@@ -2052,6 +2052,17 @@ hipError_t launchFlagLog(int32_t* flags, int32_t* log, uint32_t n, hipStream_t s
     hipLaunchKernelGGL(kFlagLog, dim3(1), dim3(256), 0, st, flags, log, n);
     return hipGetLastError();
 }
+__global__ void kFlagMax(int32_t* flags, int32_t* acc, uint32_t n, int clear) {
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        acc[k] = clear ? 0 : max(acc[k], flags[k]);
+        flags[k] = 0;
+    }
+}
+hipError_t launchFlagMax(int32_t* flags, int32_t* acc, uint32_t n, int clear, hipStream_t st) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(kFlagMax, dim3(1), dim3(256), 0, st, flags, acc, n, clear);
+    return hipGetLastError();
+}
 hipError_t launchIntegrate(const DevView& d, hipStream_t st) { LAUNCH(kIntegrate, d.numBlocks, d); }
 hipError_t launchPciInit(const DevView& d, hipStream_t st) { LAUNCH(kPciInit, d.numBlocks, d); }
 hipError_t launchPciDensity(const DevView& d, int slot, hipStream_t st) { LAUNCH(kPciDensity, d.numBlocks, d, slot); }

@@ -145,6 +145,8 @@ hipError_t launchDivergence(const DevView& d, int warm, int slot, hipStream_t st
 hipError_t launchScaleDivergence(const DevView& d, hipStream_t st);
 // Copy n search flags to `log` and zero them for the next search.
 hipError_t launchFlagLog(int32_t* flags, int32_t* log, uint32_t n, hipStream_t st);
+// acc[k] = max(acc[k], flags[k]) and flags[k] = 0 (clear: both zeroed).
+hipError_t launchFlagMax(int32_t* flags, int32_t* acc, uint32_t n, int clear, hipStream_t st);
 hipError_t launchIntegrate(const DevView& d, hipStream_t st);
 
 hipError_t launchPciInit(const DevView& d, hipStream_t st);
