@@ -1558,28 +1558,29 @@ class Engine {
             }
         }
         for (int k = 0; k < steps && !graphed; ++k) {
-            if (!record) markNo = stepNo;
+            int32_t* logSlot = queueLog.p + static_cast<size_t>(k) * nf;
+            if (!record) {
+                markNo = stepNo;
+                queuedStep(logSlot, false);
+                ++stepNo;
+                continue;
+            }
+            // stepWCSPHRecord's launches and tape writes
             searchOnce(false, false);
-            HIPCHK(launchFlagLog(dFlags, queueLog.p + static_cast<size_t>(k) * nf, static_cast<uint32_t>(nf), stream));
+            HIPCHK(launchFlagLog(dFlags, logSlot, static_cast<uint32_t>(nf), stream));
             const DevView d = view();
             const size_t o = static_cast<size_t>(slot0 + k) * NP;
             HalfTape& t = tape.h[0];
-            if (record) {
-                recordSearch(t, slot0 + k, capF, capB);
-                tapeCopy(t.v.p + o, v.p, NP * sizeof(float4));
-            }
+            recordSearch(t, slot0 + k, capF, capB);
+            tapeCopy(t.v.p + o, v.p, NP * sizeof(float4));
             timed(K_DENSITY, [&] { return launchDensityPressure(d, stream); });
             timed(K_FORCES, [&] { return launchForces(d, 1, stream); });
-            if (anyST() || anyAdh()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
-            if (record) {
-                tapeCopy(t.rho.p + o, rho.p, NP * sizeof(float));
-                tapeCopy(t.p.p + o, p.p, NP * sizeof(float));
-                if (anyST()) tapeCopy(t.n.p + o, n.p, NP * sizeof(float4));
-                flushCopies();   // before the integration moves x and v
-            }
+            if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
+            tapeCopy(t.rho.p + o, rho.p, NP * sizeof(float));
+            tapeCopy(t.p.p + o, p.p, NP * sizeof(float));
+            if (anyST()) tapeCopy(t.n.p + o, n.p, NP * sizeof(float4));
+            flushCopies();   // before the integration moves x and v
             timed(K_INTEGRATE, [&] { return launchIntegrate(d, stream); });
-            if (anyCube()) timed(K_CUBE, [&] { return launchCubeBoundary(d, stream); });
-            if (!record) ++stepNo;
         }
         if (!graphed)
             HIPCHK(hipMemcpyAsync(hQueueFlags, queueLog.p, static_cast<size_t>(steps) * nf * sizeof(int32_t),
