@@ -1544,7 +1544,7 @@ class Engine {
         HIPCHK(hipMemsetAsync(dFlags, 0, nf * sizeof(int32_t), stream));
         const bool record = slot0 >= 0;
         bool graphed = false;
-        if (!record && steps % 2 == 0 && !graphsOff && !profiling) {
+        if (!record && steps % 2 == 0 && !graphsOff && !profiling && !radixOnce) {   // counting-sort searches only
             if (flagAcc.n < nf) flagAcc.alloc(nf);
             HIPCHK(launchFlagMax(dFlags, flagAcc.p, static_cast<uint32_t>(nf), 1, stream));
             if (!pairExec || !(currentPairKey() == pairKey)) capturePair();
