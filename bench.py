@@ -384,7 +384,7 @@ def gpu_leg(sc, x, v, windows):
 
 
 def d1m_flush_leg(threads, steps=20, ref_steps=3):
-    """D1M as the survey places it, flush against the walls: the per-step
+    """D1M with the fluid block flush against the walls (no gap): the per-step
     statistics of the first `ref_steps` steps on the GPU and in the compiled
     reference, and the GPU's time and mean density error over `steps`."""
     from oracle import sphref as S
