@@ -1648,8 +1648,10 @@ class Engine {
     // only checks after each chunk whether any scene still wants one.
     // afterFirstSync runs once, right after the first chunk's sync; if it
     // returns false the loop is abandoned (the caller redoes the phase).
+    // With sync false and a first chunk that covers the bound (a loop whose
+    // iteration count is fixed), the loop ends without a host wait.
     template <class A, class B, class C>
-    bool runLoop(int firstChunk, int bound, A&& passA, B&& passB, C&& afterFirstSync) {
+    bool runLoop(int firstChunk, int bound, A&& passA, B&& passB, C&& afterFirstSync, bool sync = true) {
         int issued = 0;
         int chunk = std::max(1, std::min(firstChunk, bound));
         bool firstSync = true;
@@ -1658,6 +1660,7 @@ class Engine {
                 passA(issued & 1);
                 passB(issued & 1);
             }
+            if (!sync && issued >= bound) return true;
             readState();
             if (firstSync) {
                 firstSync = false;
@@ -1731,15 +1734,22 @@ class Engine {
         timed(K_RHO_STAR, [&] { return launchRhoStar(d, 1, 2, stream); });
         timed(K_VELOCITY, [&] { return launchVelocity(d, 2, -1, stream); });
         int first = 1;
-        for (auto& sc : scenes) first = std::max(first, std::max(static_cast<int>(sc.P.min_iterations), sc.its));
-        runLoop(first, maxOver(&sph_scene_params::max_iterations),
+        bool fixedIts = true;   // every scene's density loop runs exactly max_iterations
+        for (auto& sc : scenes) {
+            first = std::max(first, std::max(static_cast<int>(sc.P.min_iterations), sc.its));
+            fixedIts = fixedIts && sc.P.min_iterations >= sc.P.max_iterations;
+        }
+        const int boundD = maxOver(&sph_scene_params::max_iterations);
+        runLoop(first, boundD,
                 [&](int slot) {
                     timed(K_RHO_STAR, [&] { return launchRhoStar(d, 0, slot, stream); });
                     if (!fuseFinish) finish(d, kFinishDensity, slot);
                 },
                 [&](int slot) { velocityPass(d, slot, kFinishDensity); },
-                [] { return true; });
-        for (size_t s = 0; s < scenes.size(); ++s) scenes[s].its = hState[s].it;
+                [] { return true; }, !(fixedIts && first >= boundD));
+        // (the device files the loop's count and error itself, in the keys pass)
+        for (size_t s = 0; s < scenes.size(); ++s)
+            scenes[s].its = fixedIts ? static_cast<int>(scenes[s].P.max_iterations) : hState[s].it;
         // 5-7. updatePositions fused with the keys (emitted particles are
         // placed after it, DFSPHSolver.cpp:72), then the neighbour search; its
         // validity is checked at the divergence loop's first sync
@@ -2510,10 +2520,7 @@ class Engine {
             if (!fuseFinish) finish(d, kFinishDensity, it & 1);
             velocityPass(velView(d, it + 1 < nit ? uSlot(A, slot, it + 1) : nullptr), it & 1, kFinishDensity);
         }
-        if (!replaying) {
-            readState();
-            for (size_t s = 0; s < scenes.size(); ++s) scenes[s].its = hState[s].it;
-        }
+        for (size_t s = 0; s < scenes.size(); ++s) scenes[s].its = nit;   // the loop ran exactly nit iterations
         uint32_t rowsF = 0, rowsB = 0;
         recordedSearch(slot, true, rowsF, rowsB);
         if (capF > tape.capF || capB > tape.capB) tapeGrow(slot + 1);
