@@ -1159,7 +1159,8 @@ class Engine {
     // ------------------------------------------------------------ neighbour search
     // keys -> sort -> reorder (+ cell ranges) -> within-support lists.
     // dfsphAdvance fuses the DFSPH position update into the keys kernel.
-    // clearFlags false: the flags were zeroed after the last search (kFlagLog)
+    // clearFlags false: the flags were zeroed after the last search (by a
+    // queued step's density pass)
     void searchOnce(bool dfsphAdvance, bool clearFlags = true) {
         if (clearFlags) HIPCHK(hipMemsetAsync(dFlags, 0, scenes.size() * kFlagSlots * sizeof(int32_t), stream));
         DevView d = view();
@@ -1402,9 +1403,10 @@ class Engine {
     // within the grid's reach, cells the counting sort can order); where no
     // emitter (host-timed) or PCISPH boundary object (a hash-table history
     // that a redo would not restore) takes part and the search lanes are
-    // settled, a chunk of steps is queued at once: a one-block kernel moves
-    // each search's flags into a device log (and clears them for the next
-    // search), and after the chunk one copy and one wait check them all.
+    // settled, a chunk of steps is queued at once: the first workgroup of
+    // each step's density pass moves the search's flags into a device log
+    // (and clears them for the next search), and after the chunk one copy
+    // and one wait check them all.
     // If any failed, the chunk is undone (positions, velocities, ids and
     // time from a snapshot taken before it, the same grid) and retaken step
     // by step with the host check, which grows what overflowed; every step
@@ -1434,7 +1436,7 @@ class Engine {
     // Two queued WCSPH steps as one HIP graph, replayed for the step pairs of
     // an even chunk: after two searches every swapped slot buffer is back in
     // place, so the same launches serve every pair. The pair's searches file
-    // their flags into one max-accumulating array (kFlagMax), all the chunk's
+    // their flags into one max-accumulating array, all the chunk's
     // check needs. The graph is captured again whenever anything its launches
     // depend on (the device view, the search's buffers and path, the force
     // options) differs from what it was captured with; a capture that fails
@@ -1467,15 +1469,22 @@ class Engine {
         std::copy(vs, vs + 12, k.val);
         return k;
     }
+    // A view whose density pass moves the search's flags to `log` (mode 1:
+    // copy, 2: maximum) and clears them.
+    DevView flagView(const DevView& d, int32_t* log, int mode) const {
+        DevView q = d;
+        q.flagLog = log;
+        q.flagCount = static_cast<uint32_t>(scenes.size() * kFlagSlots);
+        q.flagMode = mode;
+        return q;
+    }
     // The launches of one queued step; flags go to the log slot or, with
     // acc, into the accumulating array.
     void queuedStep(int32_t* logSlot, bool acc) {
         searchOnce(false, false);
-        const uint32_t nf = static_cast<uint32_t>(scenes.size() * kFlagSlots);
-        if (acc) HIPCHK(launchFlagMax(dFlags, flagAcc.p, nf, 0, stream));
-        else HIPCHK(launchFlagLog(dFlags, logSlot, nf, stream));
         const DevView d = view();
-        timed(K_DENSITY, [&] { return launchDensityPressure(d, stream); });
+        const DevView dl = flagView(d, acc ? flagAcc.p : logSlot, acc ? 2 : 1);
+        timed(K_DENSITY, [&] { return launchDensityPressure(dl, stream); });
         timed(K_FORCES, [&] { return launchForces(d, 1, stream); });
         if (anyST() || anyAdh()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
         timed(K_INTEGRATE, [&] { return launchIntegrate(d, stream); });
@@ -1546,7 +1555,7 @@ class Engine {
         bool graphed = false;
         if (!record && steps % 2 == 0 && !graphsOff && !profiling && !radixOnce) {   // counting-sort searches only
             if (flagAcc.n < nf) flagAcc.alloc(nf);
-            HIPCHK(launchFlagMax(dFlags, flagAcc.p, static_cast<uint32_t>(nf), 1, stream));
+            HIPCHK(launchFlagClear(dFlags, flagAcc.p, static_cast<uint32_t>(nf), stream));
             if (!pairExec || !(currentPairKey() == pairKey)) capturePair();
             if (pairExec) {
                 for (int q = 0; q < steps / 2; ++q) HIPCHK(hipGraphLaunch(pairExec, stream));
@@ -1567,13 +1576,13 @@ class Engine {
             }
             // stepWCSPHRecord's launches and tape writes
             searchOnce(false, false);
-            HIPCHK(launchFlagLog(dFlags, logSlot, static_cast<uint32_t>(nf), stream));
             const DevView d = view();
+            const DevView dl = flagView(d, logSlot, 1);
             const size_t o = static_cast<size_t>(slot0 + k) * NP;
             HalfTape& t = tape.h[0];
             recordSearch(t, slot0 + k, capF, capB);
             tapeCopy(t.v.p + o, v.p, NP * sizeof(float4));
-            timed(K_DENSITY, [&] { return launchDensityPressure(d, stream); });
+            timed(K_DENSITY, [&] { return launchDensityPressure(dl, stream); });
             timed(K_FORCES, [&] { return launchForces(d, 1, stream); });
             if (anyST()) timed(K_SURFACE_TENSION, [&] { return launchSurfaceTension(d, 0, stream); });
             tapeCopy(t.rho.p + o, rho.p, NP * sizeof(float));

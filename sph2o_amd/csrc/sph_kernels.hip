@@ -1099,6 +1099,12 @@ __global__ void __launch_bounds__(kBlock) kDensityAlpha(DevView d, int withAlpha
 // WCSPH density + Tait pressure (WCSPHSolver.cpp:80-95), with the lanes per
 // particle of the other once-per-step passes (kLppOnce).
 __global__ void __launch_bounds__(kBlock) kDensityPressure(DevView d) {
+    if (d.flagMode && blockIdx.x == 0)
+        for (uint32_t k = threadIdx.x; k < d.flagCount; k += kBlock) {
+            const int32_t f = d.flags[k];
+            d.flagLog[k] = d.flagMode == 2 ? max(d.flagLog[k], f) : f;
+            d.flags[k] = 0;
+        }
     SCENE_PROLOGUE_LANES(2, kLppOnce)
     (void)dt;
     (void)dt2;
@@ -2041,26 +2047,15 @@ hipError_t launchDivergence(const DevView& d, int warm, int slot, hipStream_t st
     LAUNCH_JACOBI(kDivergence, d.numBlocks * kLpp, d, warm, slot);
 }
 hipError_t launchScaleDivergence(const DevView& d, hipStream_t st) { LAUNCH(kScaleDivergence, d.numBlocks, d); }
-__global__ void kFlagLog(int32_t* flags, int32_t* log, uint32_t n) {
+__global__ void kFlagClear(int32_t* flags, int32_t* acc, uint32_t n) {
     for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
-        log[k] = flags[k];
+        acc[k] = 0;
         flags[k] = 0;
     }
 }
-hipError_t launchFlagLog(int32_t* flags, int32_t* log, uint32_t n, hipStream_t st) {
+hipError_t launchFlagClear(int32_t* flags, int32_t* acc, uint32_t n, hipStream_t st) {
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(kFlagLog, dim3(1), dim3(256), 0, st, flags, log, n);
-    return hipGetLastError();
-}
-__global__ void kFlagMax(int32_t* flags, int32_t* acc, uint32_t n, int clear) {
-    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
-        acc[k] = clear ? 0 : max(acc[k], flags[k]);
-        flags[k] = 0;
-    }
-}
-hipError_t launchFlagMax(int32_t* flags, int32_t* acc, uint32_t n, int clear, hipStream_t st) {
-    if (!n) return hipSuccess;
-    hipLaunchKernelGGL(kFlagMax, dim3(1), dim3(256), 0, st, flags, acc, n, clear);
+    hipLaunchKernelGGL(kFlagClear, dim3(1), dim3(256), 0, st, flags, acc, n);
     return hipGetLastError();
 }
 hipError_t launchIntegrate(const DevView& d, hipStream_t st) { LAUNCH(kIntegrate, d.numBlocks, d); }

@@ -73,6 +73,12 @@ struct DevView {
     // recording (null otherwise): the velocity pass also stores its result here
     float4* vRec;
     float* snapKvws;
+    // queued WCSPH steps: kDensityPressure's first workgroup moves the last
+    // search's flags (flagCount ints at `flags`) to flagLog (flagMode 1: copy,
+    // 2: maximum) and clears them for the next search; flagMode 0: untouched
+    int32_t* flagLog;
+    uint32_t flagCount;
+    int32_t flagMode;
     // neighbour lists, column-major: list[slot * NP + i]
     uint32_t* nbrF;
     uint32_t* cntF;
@@ -143,10 +149,9 @@ hipError_t launchRhoStar(const DevView& d, int warm, int slot, hipStream_t st);
 hipError_t launchVelocity(const DevView& d, int slot, int fin, hipStream_t st);
 hipError_t launchDivergence(const DevView& d, int warm, int slot, hipStream_t st);
 hipError_t launchScaleDivergence(const DevView& d, hipStream_t st);
-// Copy n search flags to `log` and zero them for the next search.
-hipError_t launchFlagLog(int32_t* flags, int32_t* log, uint32_t n, hipStream_t st);
-// acc[k] = max(acc[k], flags[k]) and flags[k] = 0 (clear: both zeroed).
-hipError_t launchFlagMax(int32_t* flags, int32_t* acc, uint32_t n, int clear, hipStream_t st);
+// Zero n search flags and their accumulator (kernel nodes only: no memset
+// in a captured graph).
+hipError_t launchFlagClear(int32_t* flags, int32_t* acc, uint32_t n, hipStream_t st);
 hipError_t launchIntegrate(const DevView& d, hipStream_t st);
 
 hipError_t launchPciInit(const DevView& d, hipStream_t st);
